@@ -1,0 +1,10 @@
+"""crowdphot_amd -- MI355X (gfx950) build of CrowdPhot.jl's PSF-fitting hot path.
+
+Host-side mirror of the reference Julia API over libcrowdphot_hip.so:
+  fit_all_stars_simultaneous  (src/photometry/psf_photometry_simultaneous.jl:953)
+  fit_star / fit_star_batch    (src/psf/psf_fitting.jl:229, lm_irls src/levenberg_marquardt.jl:431)
+  CircularGaussianPSF, CircularMoffatPSF, ImagePSF, GriddedPSFModel
+"""
+from . import _abi  # noqa: F401
+from .psf import (CircularGaussianPSF, CircularMoffatPSF, GriddedPSFModel,  # noqa: F401
+                  ImagePSF, moffat_epsf, pixel_response_kernel)

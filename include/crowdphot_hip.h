@@ -1,0 +1,250 @@
+/*
+ * crowdphot_hip.h -- C ABI of libcrowdphot_hip.so, the MI355X (gfx950) build of
+ * CrowdPhot.jl's PSF-fitting hot path.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative CP_E* code on failure; never throw.
+ *     The message of the last failure is kept in cp_last_error(ctx).
+ *   - images are column-major (y is the fastest index), 1-based coordinates:
+ *     pixel (gy, gx) has flat index (gy-1) + (gx-1)*ny and its centre sits at
+ *     (y, x) = (gy, gx), as in the reference (docs/src/index.md).
+ *   - star parameters are float64; pixels, weights, PSF grids are float32.
+ *   - a cp_ctx is not thread-safe; use one per host thread.  Every context
+ *     owns one HIP stream (default priority) on its device.
+ *
+ * Reference interfaces each declaration replaces are cited as file:line of
+ * cgarling/CrowdPhot.jl.
+ */
+#ifndef CROWDPHOT_HIP_H
+#define CROWDPHOT_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define CP_OK 0
+#define CP_EARG (-1)     /* ArgumentError in the reference */
+#define CP_EHIP (-2)     /* HIP runtime failure */
+#define CP_ENOMEM (-3)   /* HBM budget or allocation failure */
+#define CP_ESTATE (-4)   /* bad handle / ticket */
+
+/* ---- PSF models --------------------------------------------------------
+ * CP_PSF_CIRCULAR_GAUSSIAN  src/psf/parametric_models.jl:62-113
+ *     parameters (y, x, fwhm, flux, bkg)
+ * CP_PSF_CIRCULAR_MOFFAT    src/psf/parametric_models.jl:672-724
+ *     parameters (y, x, alpha, beta, flux, bkg)
+ * CP_PSF_GRIDDED_IMAGE      GriddedPSFModel{T,<:ImagePSF{T}}
+ *     src/psf/gridded_psf.jl:51-271 over ImagePSF src/psf/empirical_models.jl:76-318;
+ *     a plain ImagePSF is the one-node case.  parameters (y, x, flux, bkg)
+ */
+#define CP_PSF_CIRCULAR_GAUSSIAN 0
+#define CP_PSF_CIRCULAR_MOFFAT 1
+#define CP_PSF_GRIDDED_IMAGE 2
+
+/* Parameter slots in getproperties order, used by free masks. */
+#define CP_PARAM_Y 0
+#define CP_PARAM_X 1
+
+typedef struct cp_psf_desc {
+    int32_t kind;
+    double fwhm;               /* CircularGaussianPSF.fwhm */
+    double alpha, beta;        /* CircularMoffatPSF.alpha, .beta */
+    /* ePSF nodes (kind == CP_PSF_GRIDDED_IMAGE); every node shares one geometry */
+    int32_t n_nodes;
+    int32_t gny, gnx;          /* rows (y) and columns (x) of each node grid, >= 4 */
+    const float* grids;        /* n_nodes * gny * gnx, each node column-major */
+    int32_t os_x, os_y;        /* ImagePSF.oversampling = (x factor, y factor) */
+    double origin_y, origin_x; /* ImagePSF.origin (1-based grid coordinates) */
+    double fill_value;         /* ImagePSF.fill_value */
+    const double* node_y;      /* GriddedPSFModel.grid_y (n_nodes) */
+    const double* node_x;      /* GriddedPSFModel.grid_x (n_nodes) */
+} cp_psf_desc;
+
+/* ---- simultaneous fit -----------------------------------------------------
+ * Keyword arguments of fit_all_stars_simultaneous,
+ * src/photometry/psf_photometry_simultaneous.jl:953-981, plus the build's
+ * own solver / precision choices.  cp_sim_opts_default fills the reference
+ * defaults. */
+#define CP_SOLVER_LSQR 0       /* reference default (Krylov.jl :lsqr) */
+#define CP_SOLVER_CGLS 1       /* CG on the damped normal equations */
+
+#define CP_PREC_MIXED 0        /* FP32 pixels/Jacobian/residuals, FP64 reductions + star vectors */
+#define CP_PREC_FP64 1         /* everything FP64 (parity runs) */
+
+#define CP_COV_DEFAULT 0       /* known weights if inv_var given, else reweighted */
+#define CP_COV_KNOWN 1
+#define CP_COV_REWEIGHTED 2
+#define CP_COV_NONE (-1)
+
+#define CP_FREE_Y 1
+#define CP_FREE_X 2
+#define CP_FREE_FLUX 4
+
+typedef struct cp_sim_opts {
+    int32_t solver;            /* CP_SOLVER_* */
+    int32_t inner_iterations;  /* 10 */
+    double linear_tol;         /* 1e-6, atol = btol of the inner solve */
+    double model_rad;          /* > 0: scalar half-width; <= 0: :auto */
+    double model_rad_max;      /* <= 0: 10 * fit_rad */
+    double model_rad_nsigma;   /* 1.0 */
+    double max_step;           /* 0.25 px */
+    int32_t max_trials;        /* 8 */
+    int32_t max_iter;          /* 40 */
+    double x_tol, f_tol, g_tol;            /* 1e-5, 1e-4, 1e-4 */
+    double lambda_init, lambda_up, lambda_down, lambda_min, lambda_max;
+    int32_t covariance;        /* CP_COV_* */
+    int32_t free_mask;         /* CP_FREE_* bits; fixed members take fixed_* values */
+    double fixed_y, fixed_x, fixed_flux, fixed_bkg;  /* fixed values (bkg is always fixed) */
+    int32_t precision;         /* CP_PREC_* */
+    int32_t tile;              /* image tile side in px for the star bins (0 = autotuned) */
+} cp_sim_opts;
+
+/* Caller-allocated outputs, catalogue order (length n_stars unless noted).
+ * Mirrors MultiPassPhotResult, src/photometry/psf_photometry_single.jl:99-122. */
+typedef struct cp_sim_result {
+    double *y, *x, *flux, *bkg;
+    double *y_err, *x_err, *flux_err, *bkg_err;
+    double *chisq;             /* may be NULL */
+    uint8_t *converged, *valid;
+    int32_t *n_iter;
+    float *residual;           /* ny*nx column-major, may be NULL */
+    int32_t n_passes, n_failed;
+    /* run statistics */
+    int32_t n_active;
+    int64_t n_union;           /* |union_pix| */
+    double cost_final;
+    int32_t n_trials_total, n_inner_total;
+    double *trace;             /* optional: 6 doubles per trial (iter, trial, cost, cost_cand, lambda, n_live) */
+    int32_t trace_cap, trace_len;
+} cp_sim_result;
+
+/* ---- per-star Levenberg-Marquardt -----------------------------------------
+ * lm_irls keyword arguments, src/levenberg_marquardt.jl:431-448 (no IRLS). */
+#define CP_DAMP_MARQUARDT 0
+#define CP_DAMP_LEVENBERG 1
+#define CP_DAMP_NONE 2
+
+typedef struct cp_lm_opts {
+    double lambda_init, lambda_up, lambda_down, lambda_min, lambda_max;
+    int32_t max_iter;          /* 200 */
+    double x_tol, f_tol, g_tol;            /* 1e-8, 1e-4, 1e-4 */
+    int32_t damping;           /* CP_DAMP_* */
+    double min_diagonal;       /* MarquardtDamping.min_diagonal = 1e-6 */
+    int32_t covariance;        /* CP_COV_* */
+} cp_lm_opts;
+
+/* Per-cutout outcome, the scalar fields of LMResult (src/levenberg_marquardt.jl:28-41). */
+typedef struct cp_lm_stat {
+    double minimum, cost_init, lambda_final, chisq;
+    int32_t iterations;
+    uint8_t converged, x_converged, f_converged, g_converged;
+    int32_t status;            /* 0 ok, CP_EARG for a cutout the reference would reject */
+} cp_lm_stat;
+
+typedef struct cp_ctx cp_ctx;
+
+/* ---- context -------------------------------------------------------------- */
+/* New context on HIP device `device`; hbm_budget_bytes = 0 means no cap. */
+int cp_create(cp_ctx** out, int device, uint64_t hbm_budget_bytes);
+void cp_destroy(cp_ctx* ctx);
+const char* cp_last_error(const cp_ctx* ctx);
+/* The context's HIP stream (hipStream_t), for callers that order their own work on it. */
+void* cp_stream(cp_ctx* ctx);
+int cp_synchronize(cp_ctx* ctx);
+
+void cp_sim_opts_default(cp_sim_opts* o);
+void cp_lm_opts_default(cp_lm_opts* o);
+
+/* ---- PSF model evaluation (src/psf/PSF.jl:34-40,207-226) ------------------
+ * Evaluates f and the full gradient G (getproperties order) of a model with
+ * parameters prm (getproperties order) at n points (py[i], px[i]) on the
+ * device.  G has nparams(kind) entries per point.  Host arrays. */
+int cp_psf_nparams(int32_t kind);
+int cp_evaluate_fg(cp_ctx* ctx, const cp_psf_desc* psf, const double* prm,
+                   const double* py, const double* px, int64_t n,
+                   double* f, double* G);
+
+/* ---- simultaneous fit: fit_all_stars_simultaneous -------------------------
+ * src/photometry/psf_photometry_simultaneous.jl:953-1431.
+ * image / inv_var: ny*nx float32, column-major; inv_var may be NULL.
+ * y0/x0/flux0: catalogue (n_stars).  Host pointers. */
+int cp_fit_simultaneous(cp_ctx* ctx, const cp_psf_desc* psf,
+                        const float* image, const float* inv_var, int32_t ny, int32_t nx,
+                        const double* y0, const double* x0, const double* flux0, int32_t n_stars,
+                        double fit_rad, const cp_sim_opts* opts, cp_sim_result* out);
+
+/* Same, with image / inv_var / y0 / x0 / flux0 already resident in device
+ * memory (device pointers); outputs in cp_sim_result are host pointers. */
+int cp_fit_simultaneous_device(cp_ctx* ctx, const cp_psf_desc* psf,
+                               const float* d_image, const float* d_inv_var, int32_t ny, int32_t nx,
+                               const double* d_y0, const double* d_x0, const double* d_flux0,
+                               int32_t n_stars, double fit_rad, const cp_sim_opts* opts,
+                               cp_sim_result* out);
+
+/* ---- star binning (integer path, exposed for bit-exact checks) ------------
+ * _build_stamps! + _touched_pixels, psf_photometry_simultaneous.jl:245-335.
+ * Host inputs (catalogue, weights ny*nx float32 or NULL = all ones).
+ * Outputs (host, caller-allocated):
+ *   anchor_y/anchor_x [n_stars] (round-half-even of y0/x0; INT32_MIN when not finite)
+ *   active [n_stars]  catalogue ids (0-based) of active stars in Morton order, *n_active set
+ *   pixels [S2 * n_stars] 1-based flat indices per active-order star, 0 = masked
+ *   union_mask [ny*nx] 1 where the pixel is in union_pix
+ * Stars whose anchor makes the reference's Morton key throw (negative
+ * coordinates) or whose y/x are non-finite are excluded, as failures. */
+int cp_build_stamps(cp_ctx* ctx, const float* w, int32_t ny, int32_t nx,
+                    const double* y0, const double* x0, const double* flux0, int32_t n_stars,
+                    double fit_rad, int32_t* anchor_y, int32_t* anchor_x,
+                    int32_t* active, int32_t* n_active, int32_t* pixels, uint8_t* union_mask);
+
+/* ---- batched per-star LM: fit_star over many cutouts ----------------------
+ * fit_star src/psf/psf_fitting.jl:229-308 (+ the CircularGaussian / CircularMoffat
+ * specialisations :314-451, :772-862) driving lm_irls src/levenberg_marquardt.jl:431-680.
+ * Cutout c occupies pix[offsets[c] .. offsets[c] + ny[c]*nx[c]) column-major,
+ * fit over all its pixels (inds = axes(cutout)), coordinates 1-based within
+ * the cutout.  inv_var: NULL or packed like pix.  theta0 / theta_out:
+ * n_cutouts * nparams(kind), getproperties order, fixed slots taken from
+ * theta0.  cov_out: NULL or n_cutouts * p * p (p = popcount(free_mask)).
+ * Host pointers; the _device variant takes pix / inv_var / offsets / ny / nx /
+ * theta0 as device pointers and writes theta_out / cov_out / stats to device memory. */
+int cp_fit_star_batch(cp_ctx* ctx, const cp_psf_desc* psf,
+                      const float* pix, const int64_t* offsets, const int32_t* ny, const int32_t* nx,
+                      const float* inv_var, const double* theta0, uint32_t free_mask,
+                      int32_t n_cutouts, const cp_lm_opts* opts,
+                      double* theta_out, double* cov_out, cp_lm_stat* stats);
+int cp_fit_star_batch_device(cp_ctx* ctx, const cp_psf_desc* psf,
+                             const float* d_pix, const int64_t* d_offsets,
+                             const int32_t* d_ny, const int32_t* d_nx,
+                             const float* d_inv_var, const double* d_theta0, uint32_t free_mask,
+                             int32_t n_cutouts, const cp_lm_opts* opts,
+                             double* d_theta_out, double* d_cov_out, cp_lm_stat* d_stats);
+
+/* ---- synthetic inputs: simulate_image (src/simulation.jl:211-498) ----------
+ * Seeded xoshiro256** streams; host arrays.
+ * cp_sim_sources: n positions uniform in [lo, hi] per axis and fluxes from
+ *   flux_dist 0 = uniform, 1 = loguniform, 2 = powerlaw dN/dF ~ F^-flux_power
+ *   (_sample_flux, src/simulation.jl:1-28).
+ * cp_sim_sources_plummer: Plummer surface density of scale radius a about
+ *   (cy, cx), rejecting points outside [lo, hi] per axis.
+ * cp_render_sources: image[ny*nx] += each source over floor(y-r):ceil(y+r)
+ *   (render_sources!, src/simulation.jl:321-347), float64 image.
+ * cp_add_noise: Poisson(gain*v)/gain (Knuth below 30, rounded normal
+ *   approximation above) then + N(0, read_noise^2) (add_noise!, :409-435). */
+int cp_sim_sources(uint64_t seed, int32_t n, double ylo, double yhi, double xlo, double xhi,
+                   int32_t flux_dist, double flo, double fhi, double flux_power,
+                   double* y, double* x, double* flux);
+int cp_sim_sources_plummer(uint64_t seed, int32_t n, double a, double cy, double cx,
+                           double lo, double hi, int32_t flux_dist, double flo, double fhi,
+                           double flux_power, double* y, double* x, double* flux);
+int cp_render_sources(const cp_psf_desc* psf, int32_t ny, int32_t nx, const double* y,
+                      const double* x, const double* flux, int32_t n, double model_radius,
+                      double* image);
+int cp_add_noise(uint64_t seed, double* image, int64_t npix, double gain, double read_noise);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CROWDPHOT_HIP_H */

@@ -1,0 +1,59 @@
+"""Python wrappers around the oracle's simultaneous fit (checker only)."""
+import ctypes as C
+
+import numpy as np
+
+from _support import f64, load_pkg, oracle, ptr
+
+abi = load_pkg()._abi
+
+
+class Result:
+    pass
+
+
+def opts(**kw):
+    o = abi.sim_opts_default()
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def fit(psf_model, image, ys, xs, fs, fit_rad, inv_var=None, trace=0, **kw):
+    """image: numpy [ny, nx] (row = y).  Returns a Result with numpy fields."""
+    lib = oracle()
+    ny, nx = image.shape
+    n = len(ys)
+    img = f64(image.ravel(order="F"))
+    iv = None if inv_var is None else f64(np.asarray(inv_var).ravel(order="F"))
+    r = Result()
+    for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq"):
+        setattr(r, k, np.zeros(n))
+    r.converged = np.zeros(n, np.uint8)
+    r.valid = np.zeros(n, np.uint8)
+    r.n_iter = np.zeros(n, np.int32)
+    r.residual = np.zeros(ny * nx, np.float32)
+    r.trace = np.zeros(max(trace, 1) * 6)
+    s = abi.SimResult()
+    for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq"):
+        setattr(s, k, ptr(getattr(r, k), abi.f64p))
+    s.converged = ptr(r.converged, abi.u8p)
+    s.valid = ptr(r.valid, abi.u8p)
+    s.n_iter = ptr(r.n_iter, abi.i32p)
+    s.residual = ptr(r.residual, abi.f32p)
+    s.trace = ptr(r.trace, abi.f64p)
+    s.trace_cap = trace
+    o = opts(**kw)
+    d = psf_model.desc()
+    rc = lib.or_fit_simultaneous(d, ptr(img, abi.f64p), ptr(iv, abi.f64p), ny, nx, ptr(f64(ys), abi.f64p),
+                                 ptr(f64(xs), abi.f64p), ptr(f64(fs), abi.f64p), n, float(fit_rad),
+                                 C.byref(o), C.byref(s))
+    if rc != 0:
+        raise ValueError(lib.or_last_error().decode())
+    r.n_passes, r.n_failed, r.n_active, r.n_union = s.n_passes, s.n_failed, s.n_active, s.n_union
+    r.cost_final = s.cost_final
+    r.trace = r.trace[: 6 * s.trace_len].reshape(-1, 6)
+    r.residual = r.residual.reshape(nx, ny).T
+    r.valid = r.valid.astype(bool)
+    r.converged = r.converged.astype(bool)
+    return r
