@@ -6,5 +6,9 @@ Host-side mirror of the reference Julia API over libcrowdphot_hip.so:
   CircularGaussianPSF, CircularMoffatPSF, ImagePSF, GriddedPSFModel
 """
 from . import _abi  # noqa: F401
+from .photometry import (ArgumentError, Context, LMResult, MultiPassPhotResult,  # noqa: F401
+                         build_stamps, default_context, evaluate_fg, fit_all_stars_simultaneous,
+                         fit_star, fit_star_batch)
 from .psf import (CircularGaussianPSF, CircularMoffatPSF, GriddedPSFModel,  # noqa: F401
                   ImagePSF, moffat_epsf, pixel_response_kernel)
+from . import simulation  # noqa: F401

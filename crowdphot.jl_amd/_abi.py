@@ -106,6 +106,7 @@ class SimResult(C.Structure):
         ("chisq", f64p),
         ("converged", u8p), ("valid", u8p),
         ("n_iter", i32p),
+        ("active", u8p),
         ("residual", f32p),
         ("n_passes", C.c_int32), ("n_failed", C.c_int32),
         ("n_active", C.c_int32),
@@ -203,6 +204,8 @@ SIGNATURES = {
     "cp_last_error": (C.c_char_p, [C.c_void_p]),
     "cp_stream": (C.c_void_p, [C.c_void_p]),
     "cp_synchronize": (C.c_int, [C.c_void_p]),
+    "cp_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "cp_profile_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "cp_sim_opts_default": (None, [C.POINTER(SimOpts)]),
     "cp_lm_opts_default": (None, [C.POINTER(LMOpts)]),
     "cp_psf_nparams": (C.c_int, [C.c_int32]),

@@ -1,0 +1,181 @@
+// context.hip -- context lifetime, defaults and the model-evaluation entry point.
+#include <cstring>
+
+#include "sim_kernels.h"
+#include "psf_host.h"
+
+namespace cp {
+
+// f and the full getproperties-ordered gradient of one model at n points
+// (evaluate_fg, src/psf/PSF.jl:207-226), float64, through the same device
+// formulas the fit kernels use.
+template <int KIND>
+__global__ void k_evaluate(ModelConst<double> M, NodeGrid G, double Y, double X, double F, double fwhm,
+                           double alpha, double beta, const double* __restrict__ py,
+                           const double* __restrict__ px, int64_t n, double* __restrict__ f,
+                           double* __restrict__ Gout) {
+    int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i >= n) return;
+    StarRec<double> r{};
+    r.flux = F;
+    if (KIND == KIND_GAUSS) r.amp = F * M.inv_norm_g;
+    else if (KIND == KIND_MOFFAT) r.amp = F * (beta - 1) / (M_PI * alpha * alpha);
+    else r.amp = F;
+    CornerRec<double> cr;
+    if (KIND == KIND_EPSF) grid_corners<double>(G, M.n_nodes, Y, X, cr);
+    const double ddy = py[i] - Y, ddx = px[i] - X;
+    double gy, gx, gf;
+    f[i] = star_value<double, KIND>(M, r, ddy, ddx, M.grids, &cr);
+    star_grad<double, KIND>(M, r, ddy, ddx, M.grids, &cr, gy, gx, gf);
+    if (KIND == KIND_GAUSS) {
+        double sq = (ddx * ddx + ddy * ddy) / M.fwhm2;
+        double Ag = r.amp * exp(CP_GAUSS_PRE * sq);
+        double* g = Gout + i * 5;
+        g[0] = gy; g[1] = gx; g[2] = -2 * Ag * (1 + CP_GAUSS_PRE * sq) / fwhm; g[3] = gf; g[4] = 1.0;
+    } else if (KIND == KIND_MOFFAT) {
+        double r2 = ddx * ddx + ddy * ddy, u = 1 + r2 / M.alpha2;
+        double Ag = (F / M.norm_m) * pow(u, -beta);
+        double* g = Gout + i * 6;
+        g[0] = gy; g[1] = gx;
+        g[2] = Ag * (-2 / alpha + 2 * beta * r2 / (alpha * M.alpha2 * u));
+        g[3] = Ag * (1 / (beta - 1) - log(u));
+        g[4] = gf; g[5] = 1.0;
+    } else {
+        double* g = Gout + i * 4;
+        g[0] = gy; g[1] = gx; g[2] = gf; g[3] = 1.0;
+    }
+}
+
+thread_local cp_ctx* tl_ctx = nullptr;
+
+}  // namespace cp
+
+using namespace cp;
+
+extern "C" {
+
+int cp_create(cp_ctx** out, int device, uint64_t hbm_budget_bytes) {
+    if (!out) return CP_EARG;
+    *out = nullptr;
+    cp_ctx* c = new (std::nothrow) cp_ctx();
+    if (!c) return CP_ENOMEM;
+    int rc = guarded(c, [&] {
+        int ndev = 0;
+        CP_HIP(hipGetDeviceCount(&ndev));
+        CP_ARG(device >= 0 && device < ndev, "no such HIP device");
+        CP_HIP(hipSetDevice(device));
+        c->device = device;
+        c->budget = hbm_budget_bytes;
+        CP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        CP_HIP(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    });
+    if (rc != CP_OK) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return CP_OK;
+}
+
+void cp_destroy(cp_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    for (auto& p : c->prof_pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    delete c;
+}
+
+const char* cp_last_error(const cp_ctx* c) { return c ? c->err.c_str() : "NULL context"; }
+
+void* cp_stream(cp_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int cp_synchronize(cp_ctx* c) {
+    if (!c) return CP_ESTATE;
+    return guarded(c, [&] { CP_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+int cp_profile(cp_ctx* c, int enable) {
+    if (!c) return CP_ESTATE;
+    return guarded(c, [&] {
+        CP_HIP(hipStreamSynchronize(c->stream));
+        prof_flush(c);
+        c->prof = enable != 0;
+        if (enable) c->prof_acc.clear();
+    });
+}
+
+int cp_profile_report(cp_ctx* c, char* buf, int32_t cap) {
+    if (!c || !buf || cap <= 0) return CP_ESTATE;
+    return guarded(c, [&] {
+        prof_flush(c);
+        std::string s;
+        char line[256];
+        for (auto& kv : c->prof_acc) {
+            snprintf(line, sizeof line, "%s %ld %.6f\n", kv.first.c_str(), kv.second.second, kv.second.first);
+            s += line;
+        }
+        snprintf(buf, cap, "%s", s.c_str());
+    });
+}
+
+int cp_psf_nparams(int32_t kind) {
+    switch (kind) {
+        case CP_PSF_CIRCULAR_GAUSSIAN: return 5;
+        case CP_PSF_CIRCULAR_MOFFAT: return 6;
+        case CP_PSF_GRIDDED_IMAGE: return 4;
+    }
+    return CP_EARG;
+}
+
+int cp_evaluate_fg(cp_ctx* ctx, const cp_psf_desc* psf, const double* prm, const double* py, const double* px,
+                   int64_t n, double* f, double* G) {
+    if (!ctx) return CP_ESTATE;
+    return guarded(ctx, [&] {
+        CP_ARG(prm && py && px && f && G && n >= 0, "NULL argument");
+        HostPsf P = make_host_psf(psf);
+        CP_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        DevPsf DP;
+        upload_psf(ctx, psf, P, DP);
+        const int np = cp_psf_nparams(P.kind);
+        if (n == 0) return;
+        DevBuf dy(ctx, n * 8), dx(ctx, n * 8), df(ctx, n * 8), dG(ctx, n * np * 8);
+        CP_HIP(hipMemcpyAsync(dy.get(), py, n * 8, hipMemcpyHostToDevice, st));
+        CP_HIP(hipMemcpyAsync(dx.get(), px, n * 8, hipMemcpyHostToDevice, st));
+        double bkg = prm[np - 1], F = prm[np - 2];
+        ModelConst<double> M = model_const<double>(P, DP, bkg);
+        dim3 g((unsigned)((n + BLOCK - 1) / BLOCK));
+        if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN) {
+            CP_ARG(prm[2] > 0, "fwhm must be positive");
+            double f2 = prm[2] * prm[2];
+            M.fwhm2 = f2;
+            M.inv_norm_g = 1.0 / (-(M_PI * f2 / CP_GAUSS_PRE));
+            hipLaunchKernelGGL(k_evaluate<KIND_GAUSS>, g, dim3(BLOCK), 0, st, M, DP.G, prm[0], prm[1], F, prm[2],
+                               0.0, 0.0, dy.as<double>(), dx.as<double>(), n, df.as<double>(), dG.as<double>());
+        } else if (P.kind == CP_PSF_CIRCULAR_MOFFAT) {
+            double a = prm[2], b = prm[3];
+            M.alpha2 = a * a;
+            M.beta = b;
+            M.norm_m = M_PI * a * a / (b - 1);
+            hipLaunchKernelGGL(k_evaluate<KIND_MOFFAT>, g, dim3(BLOCK), 0, st, M, DP.G, prm[0], prm[1], F, 0.0, a,
+                               b, dy.as<double>(), dx.as<double>(), n, df.as<double>(), dG.as<double>());
+        } else {
+            hipLaunchKernelGGL(k_evaluate<KIND_EPSF>, g, dim3(BLOCK), 0, st, M, DP.G, prm[0], prm[1], F, 0.0, 0.0,
+                               0.0, dy.as<double>(), dx.as<double>(), n, df.as<double>(), dG.as<double>());
+        }
+        CP_HIP(hipGetLastError());
+        CP_HIP(hipMemcpyAsync(f, df.get(), n * 8, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipMemcpyAsync(G, dG.get(), n * np * 8, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,1121 @@
+// sim_kernels.h -- device kernels of the simultaneous Gauss-Newton fit
+// (fit_all_stars_simultaneous, src/photometry/psf_photometry_simultaneous.jl).
+//
+// Design (MI355X, wave64):
+//  * image tiles of TY x TX = 1024 px each own two star lists (stars whose
+//    fit-box or model-box overlaps the tile), built once by a stable radix
+//    sort so each list is in the reference's Morton star order.  Every
+//    image-space product (model render, residual, cost, J*v, freezing
+//    subtraction) is a per-pixel gather over the tile's list: no float
+//    atomics, and each pixel sums its stars in the same order as the
+//    reference's scatter-add.
+//  * per-star products (fill + J^T r, J^T u, steps, freeze tests) run one
+//    wavefront per star with wave-shuffle reductions in float64.
+//  * grid-wide sums use per-block partials and a last-block-done counter,
+//    summed in block order: deterministic run to run.
+//  * persistent blocks (a fixed multiple of the CU count) stage the ePSF node
+//    grid in LDS once and walk contiguous tile / star ranges.
+#pragma once
+
+#include "cp_internal.h"
+#include "psf_device.h"
+
+namespace cp {
+
+constexpr int BLOCK = 256;
+constexpr int NWAVE = BLOCK / 64;
+constexpr int TILE_PX = 1024;           // pixels per tile
+constexpr int PPT = TILE_PX / BLOCK;    // pixels per thread
+constexpr int CHUNK = 128;              // star records staged per LDS chunk
+constexpr int MAX_EPL = 16;             // stamp entries per lane (S^2 <= 1024)
+constexpr double DEPS = 2.220446049250313e-16;
+
+enum : int { MODE_RESID = 0, MODE_COST = 1, MODE_FREEZE = 2, MODE_FINAL = 3 };
+
+// Device-resident scalars of one fit.
+struct Scal {
+    double cost, cost_cand, dof, bnorm2;
+    unsigned long long gmax_bits;
+    int n_live, n_nonfinite, n_frozen_now, pad0;
+    unsigned long long n_union;
+    // inner solver state
+    double lam, beta1, alpha, beta_cur, beta_next, rhobar, phibar;
+    double anorm2, dnorm2, xnorm2, c2, s2, z, res2, xenorm2, errv[5];
+    double sig, tau;        // pending x/w update of the last finished iteration
+    double atol, btol;
+    int iter, done, itmax, solver;
+    // CGLS
+    double gamma, cg_alpha, cg_beta, pnorm2, qnorm2, s0norm;
+    unsigned int counters[8];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block sum in fixed order; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NWAVE; ++i) r += sh[i];
+    return r;
+}
+
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publish this block's partial (thread 0 holds v) and report whether this
+// block is the last of nblocks to finish.  The counter is left at 0.
+__device__ __forceinline__ bool publish_last(double v, double* partials, unsigned int* counter,
+                                             unsigned int nblocks) {
+    __shared__ int am_last;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(partials + blockIdx.x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        unsigned int prev = atomicAdd(counter, 1u);
+        am_last = (prev == nblocks - 1);
+        if (am_last) {
+            __threadfence();
+            *counter = 0;
+        }
+    }
+    __syncthreads();
+    return am_last != 0;
+}
+
+// Sum partials[0..n) in fixed order inside the last block; result in thread 0.
+__device__ __forceinline__ double sum_partials(const double* partials, int n, double* sh) {
+    double acc = 0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) acc += ld_agent(partials + i);
+    return block_sum(acc, sh);
+}
+
+// Symmetric Givens rotation: [c s; s -c] [a; b] = [rho; 0].
+__device__ __forceinline__ void sym_givens(double a, double b, double& c, double& s, double& rho) {
+    auto sgn = [](double v) { return v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0); };
+    if (b == 0) {
+        c = (a == 0) ? 1.0 : sgn(a);
+        s = 0;
+        rho = fabs(a);
+    } else if (a == 0) {
+        c = 0;
+        s = sgn(b);
+        rho = fabs(b);
+    } else if (fabs(b) > fabs(a)) {
+        double t = a / b;
+        double tau = sgn(b) * sqrt(1 + t * t);
+        s = 1 / tau;
+        c = s * t;
+        rho = b * tau;
+    } else {
+        double t = b / a;
+        double tau = sgn(a) * sqrt(1 + t * t);
+        c = 1 / tau;
+        s = c * t;
+        rho = a * tau;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// setup kernels
+// ---------------------------------------------------------------------------
+
+// Data and weights (psf_photometry_simultaneous.jl:1056-1084).
+template <typename T>
+static __global__ void k_prep_pixels(const float* __restrict__ img, const float* __restrict__ iv, int64_t P,
+                              T* __restrict__ data, T* __restrict__ dw, T* __restrict__ w, Scal* S) {
+    int bad = 0;
+    for (int64_t q = blockIdx.x * (int64_t)BLOCK + threadIdx.x; q < P; q += (int64_t)gridDim.x * BLOCK) {
+        float d = img[q];
+        float wv = 1.0f;
+        if (iv) {
+            float t = iv[q];
+            wv = (isfinite(t) && t > 0.0f) ? t : 0.0f;
+        }
+        if (!isfinite(d)) {
+            bad += (iv == nullptr);
+            d = 0.0f;
+            wv = 0.0f;
+        }
+        data[q] = (T)d;
+        dw[q] = (T)d;
+        w[q] = (T)wv;
+    }
+    if (bad) atomicAdd(&S->n_nonfinite, bad);
+}
+
+__device__ __forceinline__ uint64_t spread32(uint64_t v) {
+    v &= 0x00000000ffffffffULL;
+    v = (v | (v << 16)) & 0x0000ffff0000ffffULL;
+    v = (v | (v << 8)) & 0x00ff00ff00ff00ffULL;
+    v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0fULL;
+    v = (v | (v << 2)) & 0x3333333333333333ULL;
+    v = (v | (v << 1)) & 0x5555555555555555ULL;
+    return v;
+}
+
+// Anchors (round half to even), activity and Morton key per catalogue star
+// (_build_stamps!, :245-306; _morton2d :163-174).
+template <typename T>
+static __global__ void k_anchor(const double* __restrict__ y0, const double* __restrict__ x0,
+                         const double* __restrict__ f0, int n, const T* __restrict__ w, int ny, int nx,
+                         int R, int* __restrict__ ay, int* __restrict__ ax, uint8_t* __restrict__ ok) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double y = y0[i], x = x0[i];
+    bool fy = isfinite(y) && fabs(rint(y)) <= 1073741824.0;
+    bool fx = isfinite(x) && fabs(rint(x)) <= 1073741824.0;
+    int a = fy ? (int)rint(y) : INT_MIN;
+    int b = fx ? (int)rint(x) : INT_MIN;
+    ay[i] = a;
+    ax[i] = b;
+    bool any = false;
+    if (fy && fx) {
+        for (int dx = -R; dx <= R && !any; ++dx)
+            for (int dy = -R; dy <= R; ++dy) {
+                int gy = a + dy, gx = b + dx;
+                if (gy >= 1 && gy <= ny && gx >= 1 && gx <= nx &&
+                    w[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] > T(0)) {
+                    any = true;
+                    break;
+                }
+            }
+    }
+    ok[i] = any && isfinite(f0[i]) && a >= 0 && b >= 0;
+}
+
+static __global__ void k_morton_keys(const int* __restrict__ sel, int na, const int* __restrict__ ay,
+                              const int* __restrict__ ax, unsigned long long* __restrict__ keys) {
+    int j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= na) return;
+    int i = sel[j];
+    keys[j] = spread32((uint64_t)ay[i]) | (spread32((uint64_t)ax[i]) << 1);
+}
+
+// Per active star (Morton order): anchors, theta, model half-width, state.
+static __global__ void k_star_setup(const int* __restrict__ act, int na, const int* __restrict__ ay_cat,
+                             const int* __restrict__ ax_cat, const double* __restrict__ y0,
+                             const double* __restrict__ x0, const double* __restrict__ f0, int p, int gc0,
+                             int gc1, int gc2, const int* __restrict__ mR_src, int mR_scalar,
+                             int* __restrict__ ay, int* __restrict__ ax, int* __restrict__ mR,
+                             double* __restrict__ theta, uint8_t* __restrict__ live,
+                             int* __restrict__ frozen_at, uint8_t* __restrict__ conv) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    int i = act[a];
+    ay[a] = ay_cat[i];
+    ax[a] = ax_cat[i];
+    mR[a] = mR_src ? mR_src[a] : mR_scalar;
+    const int gc[3] = {gc0, gc1, gc2};
+    for (int k = 0; k < p; ++k) {
+        double v = gc[k] == 1 ? y0[i] : (gc[k] == 2 ? x0[i] : f0[i]);
+        theta[(size_t)a * p + k] = v;
+    }
+    live[a] = 1;
+    frozen_at[a] = 0;
+    conv[a] = 0;
+}
+
+// union_pix as a byte mask (_touched_pixels, :329-335).
+template <typename T>
+static __global__ void k_union(int na, const int* __restrict__ ay, const int* __restrict__ ax, int R, int S,
+                        const T* __restrict__ w, int ny, int nx, uint8_t* __restrict__ umask) {
+    const int S2 = S * S;
+    int64_t e = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (e >= (int64_t)na * S2) return;
+    int a = (int)(e / S2), m = (int)(e % S2);
+    int gy = ay[a] + (m % S) - R, gx = ax[a] + (m / S) - R;
+    if (gy < 1 || gy > ny || gx < 1 || gx > nx) return;
+    size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * ny;
+    if (w[q] > T(0)) umask[q] = 1;
+}
+
+static __global__ void k_count_mask(const uint8_t* __restrict__ m, int64_t P, unsigned long long* out) {
+    unsigned long long c = 0;
+    for (int64_t q = blockIdx.x * (int64_t)BLOCK + threadIdx.x; q < P; q += (int64_t)gridDim.x * BLOCK)
+        c += m[q];
+    c = (unsigned long long)wave_sum_int((int)c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+// Stamp pixel indices of active stars (1-based, 0 = masked), for export.
+template <typename T>
+static __global__ void k_pixels(int na, const int* __restrict__ ay, const int* __restrict__ ax, int R, int S,
+                         const T* __restrict__ w, int ny, int nx, int* __restrict__ pix) {
+    const int S2 = S * S;
+    int64_t e = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (e >= (int64_t)na * S2) return;
+    int a = (int)(e / S2), m = (int)(e % S2);
+    int gy = ay[a] + (m % S) - R, gx = ax[a] + (m / S) - R;
+    int v = 0;
+    if (gy >= 1 && gy <= ny && gx >= 1 && gx <= nx) {
+        size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * ny;
+        if (w[q] > T(0)) v = (int)(q + 1);
+    }
+    pix[e] = v;
+}
+
+// ---------------------------------------------------------------------------
+// tile binning
+// ---------------------------------------------------------------------------
+struct TileGeo {
+    int TY, TX, nty, ntx, ny, nx;
+};
+
+__device__ __forceinline__ bool tile_range(const TileGeo& g, int ay, int ax, int h, int& ty0, int& ty1,
+                                           int& tx0, int& tx1) {
+    int y0 = max(ay - h, 1), y1 = min(ay + h, g.ny);
+    int x0 = max(ax - h, 1), x1 = min(ax + h, g.nx);
+    if (y0 > y1 || x0 > x1) return false;
+    ty0 = (y0 - 1) / g.TY;
+    ty1 = (y1 - 1) / g.TY;
+    tx0 = (x0 - 1) / g.TX;
+    tx1 = (x1 - 1) / g.TX;
+    return true;
+}
+
+static __global__ void k_tile_count(int na, const int* __restrict__ ay, const int* __restrict__ ax,
+                             const int* __restrict__ hw, int h_scalar, TileGeo g, int* __restrict__ cnt) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    int h = hw ? hw[a] : h_scalar, ty0, ty1, tx0, tx1;
+    cnt[a] = tile_range(g, ay[a], ax[a], h, ty0, ty1, tx0, tx1) ? (ty1 - ty0 + 1) * (tx1 - tx0 + 1) : 0;
+}
+
+static __global__ void k_tile_emit(int na, const int* __restrict__ ay, const int* __restrict__ ax,
+                            const int* __restrict__ hw, int h_scalar, TileGeo g, const int* __restrict__ off,
+                            unsigned int* __restrict__ keys, int* __restrict__ vals) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    int h = hw ? hw[a] : h_scalar, ty0, ty1, tx0, tx1;
+    if (!tile_range(g, ay[a], ax[a], h, ty0, ty1, tx0, tx1)) return;
+    int o = off[a];
+    for (int tx = tx0; tx <= tx1; ++tx)
+        for (int ty = ty0; ty <= ty1; ++ty) {
+            keys[o] = (unsigned int)(ty + tx * g.nty);
+            vals[o] = a;
+            ++o;
+        }
+}
+
+static __global__ void k_tile_ranges(const unsigned int* __restrict__ keys, int npairs, int* __restrict__ tstart,
+                              int* __restrict__ tend) {
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= npairs) return;
+    unsigned int k = keys[i];
+    if (i == 0 || keys[i - 1] != k) tstart[k] = i;
+    if (i == npairs - 1 || keys[i + 1] != k) tend[k] = i + 1;
+}
+
+// ---------------------------------------------------------------------------
+// star records
+// ---------------------------------------------------------------------------
+struct FreeMap {
+    int p, gc0, gc1, gc2;   // gradient column (1 = y, 2 = x, 3 = flux) of free slot k
+    int y_fixed, x_fixed, f_fixed;
+    double fixed_y, fixed_x, fixed_f;
+};
+
+__host__ __device__ __forceinline__ int gc_of(const FreeMap& fm, int k) {
+    return k == 0 ? fm.gc0 : (k == 1 ? fm.gc1 : fm.gc2);
+}
+
+__device__ __forceinline__ void star_yxf(const FreeMap& fm, const double* th, double& Y, double& X,
+                                         double& F) {
+    Y = fm.fixed_y;
+    X = fm.fixed_x;
+    F = fm.fixed_f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k < fm.p) {
+            const int c = gc_of(fm, k);
+            const double v = th[k];
+            if (c == 1) Y = v;
+            if (c == 2) X = v;
+            if (c == 3) F = v;
+        }
+    }
+}
+
+struct NodeGrid {
+    int ngy, ngx;
+    const double* ygrid;
+    const double* xgrid;
+    const int* index_grid;   // ngy x ngx column-major, 0-based node ids
+};
+
+// _grid_corners_dw (gridded_psf.jl:176-208).
+template <typename T>
+__device__ void grid_corners(const NodeGrid& G, int n_nodes, double Y, double X, CornerRec<T>& c) {
+    if (n_nodes == 1) {
+        c.idx[0] = c.idx[1] = c.idx[2] = c.idx[3] = 0;
+        c.w[0] = T(1);
+        c.w[1] = c.w[2] = c.w[3] = T(0);
+        for (int k = 0; k < 4; ++k) c.dwdy[k] = c.dwdx[k] = T(0);
+        return;
+    }
+    auto last_le = [](const double* g, int n, double v) {
+        int lo = 0, hi = n;  // count of elements <= v
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (g[mid] <= v) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    int iy = min(max(last_le(G.ygrid, G.ngy, Y), 1), G.ngy - 1);
+    int ix = min(max(last_le(G.xgrid, G.ngx, X), 1), G.ngx - 1);
+    double y0 = G.ygrid[iy - 1], y1 = G.ygrid[iy], x0 = G.xgrid[ix - 1], x1 = G.xgrid[ix];
+    double Yc = fmin(fmax(Y, y0), y1), Xc = fmin(fmax(X, x0), x1);
+    double cy = (Y > y0 && Y < y1) ? 1.0 : 0.0, cx = (X > x0 && X < x1) ? 1.0 : 0.0;
+    double inv = 1.0 / ((x1 - x0) * (y1 - y0));
+    double dy1 = y1 - Yc, dy0 = Yc - y0, dx1 = x1 - Xc, dx0 = Xc - x0;
+    auto ig = [&](int a, int b) { return G.index_grid[(a - 1) + (b - 1) * G.ngy]; };
+    c.idx[0] = ig(iy, ix);
+    c.idx[1] = ig(iy, ix + 1);
+    c.idx[2] = ig(iy + 1, ix);
+    c.idx[3] = ig(iy + 1, ix + 1);
+    double w[4] = {dx1 * dy1 * inv, dx0 * dy1 * inv, dx1 * dy0 * inv, dx0 * dy0 * inv};
+    double wy[4] = {-dx1 * cy * inv, -dx0 * cy * inv, dx1 * cy * inv, dx0 * cy * inv};
+    double wx[4] = {-dy1 * cx * inv, dy1 * cx * inv, -dy0 * cx * inv, dy0 * cx * inv};
+    for (int k = 0; k < 4; ++k) {
+        c.w[k] = (T)w[k];
+        c.dwdy[k] = (T)wy[k];
+        c.dwdx[k] = (T)wx[k];
+    }
+}
+
+// StarRec from theta.  frozen_iter >= 1 marks stars frozen at that iteration.
+template <typename T, int KIND>
+static __global__ void k_star_rec(int na, const double* __restrict__ theta, FreeMap fm, ModelConst<T> M,
+                           double inv_norm_g64, double moffat_amp64, const int* __restrict__ ay,
+                           const int* __restrict__ ax, const int* __restrict__ mR,
+                           const uint8_t* __restrict__ live, const int* __restrict__ frozen_at,
+                           int frozen_iter, int all_live, StarRec<T>* __restrict__ recs, NodeGrid G,
+                           CornerRec<T>* __restrict__ crs) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    double Y, X, F;
+    star_yxf(fm, theta + (size_t)a * fm.p, Y, X, F);
+    StarRec<T> r;
+    r.ay = ay[a];
+    r.ax = ax[a];
+    r.fy = (T)(Y - r.ay);
+    r.fx = (T)(X - r.ax);
+    r.flux = (T)F;
+    if (KIND == KIND_GAUSS) r.amp = (T)(F * inv_norm_g64);
+    else if (KIND == KIND_MOFFAT) r.amp = (T)(F * moffat_amp64);
+    else r.amp = (T)F;
+    r.mR = mR[a];
+    int fl = (all_live || live[a]) ? 1 : 0;
+    if (frozen_iter > 0 && frozen_at[a] == frozen_iter && !live[a]) fl |= 2;
+    r.flags = fl;
+    recs[a] = r;
+    if (KIND == KIND_EPSF && M.n_nodes > 1) grid_corners<T>(G, M.n_nodes, Y, X, crs[a]);
+}
+
+// ---------------------------------------------------------------------------
+// tile-gather render: residual / candidate cost / freeze subtraction / final
+// (_render_model! :545-673, _residual_cost! :675-684, _cost! :686-693,
+//  _freeze_star! :739-754, final render :1344-1356)
+// ---------------------------------------------------------------------------
+template <typename T, int KIND, int MODE, bool LDSG>
+__global__ __launch_bounds__(BLOCK) void k_render(
+    ModelConst<T> M, const StarRec<T>* __restrict__ recs, const CornerRec<T>* __restrict__ crs,
+    const int* __restrict__ tstart, const int* __restrict__ tend, const int* __restrict__ list, TileGeo g,
+    int ntiles, int tiles_per_block, const uint8_t* __restrict__ umask, T* __restrict__ dw,
+    const T* __restrict__ w, const T* __restrict__ data, T* __restrict__ wt_resid, T* __restrict__ resid,
+    double* __restrict__ partials, Scal* S) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double sred[NWAVE];
+    const int ngrid = LDSG ? M.gny * M.gnx : 0;
+    float* sgrid = reinterpret_cast<float*>(smem);
+    StarRec<T>* srec = reinterpret_cast<StarRec<T>*>(smem + ((ngrid * 4 + 15) & ~15));
+    int* sid = reinterpret_cast<int*>(srec + CHUNK);
+    if (LDSG) {
+        for (int i = threadIdx.x; i < ngrid; i += BLOCK) sgrid[i] = M.grids[i];
+    }
+    const float* grid = LDSG ? sgrid : M.grids;
+    double acc = 0;
+    const int t0 = blockIdx.x * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int ty = tile % g.nty, tx = tile / g.nty;
+        const int gy0 = ty * g.TY + 1, gx0 = tx * g.TX + 1;
+        T val[PPT];
+        int gyv[PPT], gxv[PPT];
+        bool act[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            int lin = threadIdx.x + k * BLOCK;
+            gyv[k] = gy0 + lin % g.TY;
+            gxv[k] = gx0 + lin / g.TY;
+            act[k] = gyv[k] <= g.ny && gxv[k] <= g.nx;
+            size_t q = (size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny;
+            if ((MODE == MODE_RESID || MODE == MODE_COST) && act[k]) act[k] = umask[q] != 0;
+            val[k] = T(0);
+            if (MODE == MODE_FREEZE && act[k]) val[k] = dw[q];
+        }
+        const int s0 = tstart[tile], s1 = tend[tile];
+        bool touched = false;
+        for (int c0 = s0; c0 < s1; c0 += CHUNK) {
+            const int cn = min(CHUNK, s1 - c0);
+            __syncthreads();
+            for (int i = threadIdx.x; i < cn; i += BLOCK) {
+                int a = list[c0 + i];
+                srec[i] = recs[a];
+                sid[i] = a;
+            }
+            __syncthreads();
+            for (int j = 0; j < cn; ++j) {
+                const StarRec<T> r = srec[j];
+                const bool use = MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (r.flags & 2) : (r.flags & 1));
+                if (!use) continue;
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    int dy = gyv[k] - r.ay, dx = gxv[k] - r.ax;
+                    if (act[k] && abs(dy) <= r.mR && abs(dx) <= r.mR) {
+                        T v = star_value<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + sid[j]);
+                        if (MODE == MODE_FREEZE) {
+                            val[k] -= v;
+                            touched = true;
+                        } else {
+                            val[k] += v;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            if (!act[k]) continue;
+            size_t q = (size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny;
+            if (MODE == MODE_RESID) {
+                T r = val[k] - dw[q];
+                T wr = sqrt(w[q]) * r;
+                wt_resid[q] = wr;
+                acc += (double)wr * (double)wr;
+            } else if (MODE == MODE_COST) {
+                T r = val[k] - dw[q];
+                acc += (double)w[q] * (double)r * (double)r;
+            } else if (MODE == MODE_FREEZE) {
+                if (touched) dw[q] = val[k];
+            } else {
+                resid[q] = data[q] - val[k];
+            }
+        }
+    }
+    if (MODE == MODE_RESID || MODE == MODE_COST) {
+        double b = block_sum(acc, sred);
+        if (publish_last(b, partials, &S->counters[0], gridDim.x)) {
+            double tot = sum_partials(partials, gridDim.x, sred);
+            if (threadIdx.x == 0) {
+                if (MODE == MODE_RESID) S->cost = tot;
+                else S->cost_cand = tot;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fill + J^T r (one wave per star; _fill_stamps! :342-543, apply_JT! :55-81,
+// g-cosine :1195-1206 and _g_tol_local :789-801)
+// ---------------------------------------------------------------------------
+template <typename T, int KIND, bool FINAL, bool LDSG, int EPL>
+__global__ __launch_bounds__(BLOCK) void k_fill(
+    ModelConst<T> M, const StarRec<T>* __restrict__ recs, const CornerRec<T>* __restrict__ crs, int na,
+    int R, int S, FreeMap fm, const T* __restrict__ w, int ny, int nx, const T* __restrict__ wt_resid,
+    T* __restrict__ V, double* __restrict__ colnorm, double* __restrict__ b_scaled,
+    uint8_t* __restrict__ gloc, double g_tol, double* __restrict__ partials, Scal* Sc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double sred[NWAVE];
+    const int ngrid = LDSG ? M.gny * M.gnx : 0;
+    float* sgrid = reinterpret_cast<float*>(smem);
+    if (LDSG) {
+        for (int i = threadIdx.x; i < ngrid; i += BLOCK) sgrid[i] = M.grids[i];
+        __syncthreads();
+    }
+    const float* grid = LDSG ? sgrid : M.grids;
+    const int lane = threadIdx.x & 63;
+    const int S2 = S * S;
+    const int p = fm.p;
+    const double C_r = FINAL ? 0.0 : Sc->cost / Sc->dof;
+    double bsq = 0, gmax = 0;
+    for (int a = blockIdx.x * NWAVE + (threadIdx.x >> 6); a < na; a += gridDim.x * NWAVE) {
+        const StarRec<T> r = recs[a];
+        if (!FINAL && !(r.flags & 1)) continue;
+        T raw[3][EPL];
+        T wres[EPL];
+        double cn2[3] = {0, 0, 0};
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) {
+            const int m = lane + 64 * e;
+            raw[0][e] = raw[1][e] = raw[2][e] = T(0);
+            wres[e] = T(0);
+            const int dy = (m % S) - R, dx = (m / S) - R;
+            const int gy = r.ay + dy, gx = r.ax + dx;
+            const bool on = m < S2 && gy >= 1 && gy <= ny && gx >= 1 && gx <= nx;
+            const size_t q = on ? (size_t)(gy - 1) + (size_t)(gx - 1) * ny : 0;
+            const T wv = on ? w[q] : T(0);
+            if (!(wv > T(0))) continue;
+            T g3[3];
+            star_grad<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a, g3[0], g3[1], g3[2]);
+            T sw = sqrt(wv);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k < p) {
+                    int c = gc_of(fm, k);
+                    T gv = c == 1 ? g3[0] : (c == 2 ? g3[1] : g3[2]);
+                    T v = sw * gv;
+                    raw[k][e] = v;
+                    cn2[k] += (double)v * (double)v;
+                }
+            }
+            if (!FINAL) wres[e] = wt_resid[q];
+        }
+        double cn[3] = {1, 1, 1}, bk[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k < p) {
+                double c = sqrt(wave_sum(cn2[k]));
+                cn[k] = c < DEPS ? DEPS : c;
+                double inv = 1.0 / cn[k];
+                double b = 0;
+#pragma unroll
+                for (int e = 0; e < EPL; ++e) {
+                    const int m = lane + 64 * e;
+                    if (m < S2) {
+                        T v = (T)((double)raw[k][e] * inv);
+                        V[((size_t)a * S2 + m) * p + k] = v;
+                        b += (double)v * (double)wres[e];
+                    }
+                }
+                bk[k] = FINAL ? 0.0 : wave_sum(b);
+            }
+        }
+        if (lane == 0) {
+            double gn = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k < p) {
+                    colnorm[(size_t)a * p + k] = cn[k];
+                    if (!FINAL) {
+                        b_scaled[(size_t)a * p + k] = bk[k];
+                        bsq += bk[k] * bk[k];
+                        double bt = cn[k] * bk[k];
+                        gn = fmax(gn, fabs(bt) / (sqrt(cn[k] * cn[k] * C_r) + DEPS));
+                    }
+                }
+            }
+            if (!FINAL) {
+                gloc[a] = gn <= g_tol;
+                gmax = fmax(gmax, gn);
+            }
+        }
+    }
+    if (!FINAL) {
+        if (lane == 0 && gmax > 0) atomicMax(&Sc->gmax_bits, (unsigned long long)__double_as_longlong(gmax));
+        double b = block_sum(bsq, sred);
+        if (publish_last(b, partials, &Sc->counters[1], gridDim.x)) {
+            double tot = sum_partials(partials, gridDim.x, sred);
+            if (threadIdx.x == 0) Sc->bnorm2 = tot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inner solver: LSQR (Paige & Saunders 1982) on the stamp operator
+// ---------------------------------------------------------------------------
+static __global__ void k_lsqr_init(Scal* S, double lam, int itmax, double atol, double btol) {
+    if (threadIdx.x != 0) return;
+    S->lam = lam;
+    S->itmax = itmax;
+    S->atol = atol;
+    S->btol = btol;
+    double beta1 = sqrt(S->cost);
+    double alpha = beta1 > 0 ? sqrt(S->bnorm2) / beta1 : 0.0;
+    S->beta1 = beta1;
+    S->alpha = alpha;
+    S->beta_cur = beta1;
+    S->beta_next = 0;
+    S->rhobar = alpha;
+    S->phibar = beta1;
+    S->anorm2 = alpha * alpha;
+    S->dnorm2 = S->xnorm2 = S->z = S->res2 = S->xenorm2 = 0;
+    for (int i = 0; i < 5; ++i) S->errv[i] = 0;
+    S->c2 = -1;
+    S->s2 = 0;
+    S->sig = S->tau = 0;
+    S->iter = 0;
+    S->done = (beta1 == 0 || alpha == 0);
+}
+
+// u <- J v - alpha u over the tile's fit list (apply_J! :95-133 as a gather).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_Jv(
+    const int* __restrict__ tstart, const int* __restrict__ tend, const int* __restrict__ list, TileGeo g,
+    int ntiles, int tiles_per_block, const int* __restrict__ ay, const int* __restrict__ ax,
+    const uint8_t* __restrict__ live, int R, int S, int p, const T* __restrict__ V,
+    const double* __restrict__ tsrc, const T* __restrict__ usrc, T* __restrict__ ubuf,
+    const uint8_t* __restrict__ umask, int first, double* __restrict__ partials, Scal* Sc) {
+    if (Sc->done) return;
+    __shared__ double sred[NWAVE];
+    __shared__ int sa[CHUNK], say[CHUNK], sax[CHUNK];
+    __shared__ double sv[CHUNK][3];
+    const double alpha = Sc->alpha;
+    const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
+    double tfac, ufac;
+    if (first) {
+        tfac = -ia / Sc->beta1;
+        ufac = -1.0 / Sc->beta1;
+    } else {
+        tfac = ia;
+        ufac = Sc->beta_cur != 0 ? 1.0 / Sc->beta_cur : 1.0;
+    }
+    const int S2 = S * S;
+    double acc2 = 0;
+    const int t0 = blockIdx.x * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int ty = tile % g.nty, tx = tile / g.nty;
+        const int gy0 = ty * g.TY + 1, gx0 = tx * g.TX + 1;
+        int gyv[PPT], gxv[PPT];
+        bool act[PPT];
+        double yv[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            int lin = threadIdx.x + k * BLOCK;
+            gyv[k] = gy0 + lin % g.TY;
+            gxv[k] = gx0 + lin / g.TY;
+            act[k] = gyv[k] <= g.ny && gxv[k] <= g.nx;
+            if (act[k]) act[k] = umask[(size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny] != 0;
+            yv[k] = 0;
+        }
+        const int s0 = tstart[tile], s1 = tend[tile];
+        for (int c0 = s0; c0 < s1; c0 += CHUNK) {
+            const int cn = min(CHUNK, s1 - c0);
+            __syncthreads();
+            for (int i = threadIdx.x; i < cn; i += BLOCK) {
+                int a = list[c0 + i];
+                bool lv = live[a];
+                sa[i] = lv ? a : -1;
+                say[i] = ay[a];
+                sax[i] = ax[a];
+                for (int k = 0; k < 3; ++k) sv[i][k] = (lv && k < p) ? tsrc[(size_t)a * p + k] * tfac : 0.0;
+            }
+            __syncthreads();
+            for (int j = 0; j < cn; ++j) {
+                const int a = sa[j];
+                if (a < 0) continue;
+                const int sy = say[j], sx = sax[j];
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    int dy = gyv[k] - sy, dx = gxv[k] - sx;
+                    if (act[k] && abs(dy) <= R && abs(dx) <= R) {
+                        const T* Vm = V + ((size_t)a * S2 + (size_t)(dx + R) * S + (dy + R)) * p;
+                        double s = 0;
+                        for (int kk = 0; kk < p; ++kk) s += (double)Vm[kk] * sv[j][kk];
+                        yv[k] += s;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            if (!act[k]) continue;
+            size_t q = (size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny;
+            T u = (T)(yv[k] - alpha * ((double)usrc[q] * ufac));
+            ubuf[q] = u;
+            acc2 += (double)u * (double)u;
+        }
+    }
+    double b = block_sum(acc2, sred);
+    if (publish_last(b, partials, &Sc->counters[2], gridDim.x)) {
+        double tot = sum_partials(partials, gridDim.x, sred);
+        if (threadIdx.x == 0) Sc->beta_next = sqrt(tot);
+    }
+}
+
+// v <- J^T u - beta v per star, with the pending x/w update of the previous
+// iteration folded in; the last block advances the LSQR recurrences.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_JTu(
+    int na, const int* __restrict__ ay, const int* __restrict__ ax, const uint8_t* __restrict__ live,
+    int R, int S, int p, int ny, int nx, const T* __restrict__ V, const T* __restrict__ ubuf,
+    const double* __restrict__ b_scaled, double* __restrict__ tbuf, double* __restrict__ x,
+    double* __restrict__ wv, int first, int it, double* __restrict__ partials, double* __restrict__ partials2,
+    Scal* Sc) {
+    if (Sc->done) return;
+    __shared__ double sred[NWAVE];
+    const double alpha = Sc->alpha, beta = Sc->beta_next;
+    const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
+    const double ib = beta != 0 ? 1.0 / beta : 1.0;
+    const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau;
+    const double ib1 = first ? 1.0 / Sc->beta1 : 0.0;
+    const int lane = threadIdx.x & 63;
+    const int S2 = S * S;
+    double tsq = 0, wsq = 0;
+    for (int a = blockIdx.x * NWAVE + (threadIdx.x >> 6); a < na; a += gridDim.x * NWAVE) {
+        const size_t base = (size_t)a * p;
+        if (!live[a]) {
+            if (first && lane < p) {
+                x[base + lane] = 0;
+                wv[base + lane] = 0;
+                tbuf[base + lane] = 0;
+            }
+            continue;
+        }
+        const int sy = ay[a], sx = ax[a];
+        double acc[3] = {0, 0, 0};
+        for (int m = lane; m < S2; m += 64) {
+            int gy = sy + (m % S) - R, gx = sx + (m / S) - R;
+            if (gy < 1 || gy > ny || gx < 1 || gx > nx) continue;
+            double u = (double)ubuf[(size_t)(gy - 1) + (size_t)(gx - 1) * ny];
+            const T* Vm = V + ((size_t)a * S2 + m) * p;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (k < p) acc[k] += (double)Vm[k] * u;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] = wave_sum(acc[k]);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k >= p) continue;
+                double told = first ? -b_scaled[base + k] * ib1 : tbuf[base + k];
+                double v = told * ia;
+                double xo = first ? 0.0 : x[base + k];
+                double wo = first ? 0.0 : wv[base + k];
+                x[base + k] = xo + sig * wo;
+                double wn = v - tau * wo;
+                wv[base + k] = wn;
+                wsq += wn * wn;
+                double tn = beta != 0 ? acc[k] * ib - beta * v : told;
+                tbuf[base + k] = tn;
+                tsq += tn * tn;
+            }
+        }
+    }
+    double bt = block_sum(tsq, sred);
+    double bw = block_sum(wsq, sred);
+    __shared__ int am_last;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(partials + blockIdx.x, bt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(partials2 + blockIdx.x, bw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        unsigned int prev = atomicAdd(&Sc->counters[3], 1u);
+        am_last = (prev == gridDim.x - 1);
+        if (am_last) {
+            __threadfence();
+            Sc->counters[3] = 0;
+        }
+    }
+    __syncthreads();
+    if (!am_last) return;
+    double tt = sum_partials(partials, gridDim.x, sred);
+    double ww = sum_partials(partials2, gridDim.x, sred);
+    if (threadIdx.x != 0) return;
+    // ---- LSQR scalar recurrences of iteration `it` (one thread) ----
+    const double lam = Sc->lam;
+    double alpha_new = alpha;
+    if (beta != 0) {
+        Sc->anorm2 += alpha * alpha + beta * beta + (lam > 0 ? lam * lam : 0.0);
+        alpha_new = sqrt(tt);
+    }
+    double c1, s1, rhobar1;
+    sym_givens(Sc->rhobar, lam, c1, s1, rhobar1);
+    double psi = s1 * Sc->phibar;
+    double phibar = c1 * Sc->phibar;
+    double c, s, rho;
+    sym_givens(rhobar1, beta, c, s, rho);
+    double theta = s * alpha_new;
+    Sc->rhobar = -c * alpha_new;
+    double phi = c * phibar;
+    phibar = s * phibar;
+    Sc->phibar = phibar;
+    Sc->xenorm2 += phi * phi;
+    Sc->errv[it % 5] = phi;
+    double err_lbnd = 0;
+    if (it >= 5) {
+        double e2 = 0;
+        for (int i = 0; i < 5; ++i) e2 += Sc->errv[i] * Sc->errv[i];
+        err_lbnd = sqrt(e2);
+    }
+    double tau_r = s * phi;
+    Sc->dnorm2 += ww / (rho * rho);
+    Sc->sig = phi / rho;
+    Sc->tau = theta / rho;
+    double delta = Sc->s2 * rho, gambar = -Sc->c2 * rho, rhs = phi - delta * Sc->z;
+    double zbar = rhs / gambar;
+    double xnorm = sqrt(Sc->xnorm2 + zbar * zbar);
+    double c2, s2, gamma;
+    sym_givens(gambar, theta, c2, s2, gamma);
+    Sc->c2 = c2;
+    Sc->s2 = s2;
+    Sc->z = rhs / gamma;
+    Sc->xnorm2 += Sc->z * Sc->z;
+    double anorm = sqrt(Sc->anorm2);
+    double acond = anorm * sqrt(Sc->dnorm2);
+    Sc->res2 += psi * psi;
+    double rnorm = sqrt(phibar * phibar + Sc->res2);
+    double arnorm = alpha_new * fabs(tau_r);
+    double beta1 = Sc->beta1;
+    double test1 = rnorm / beta1, test2 = arnorm / (anorm * rnorm), test3 = 1 / acond;
+    double t1 = test1 / (1 + anorm * xnorm / beta1);
+    const double axtol = 1.4901161193847656e-08, etol = axtol, ctol = axtol;
+    double rtol = Sc->btol + axtol * anorm * xnorm / beta1;
+    bool ill = (1 + test3 <= 1) || (test3 <= ctol);
+    bool solved = (1 + test2 <= 1) || (test2 <= axtol) || (arnorm <= Sc->atol) || (1 + t1 <= 1) ||
+                  (test1 <= rtol) || (it >= 5 && err_lbnd <= etol * sqrt(Sc->xenorm2));
+    Sc->alpha = alpha_new;
+    Sc->beta_cur = beta;
+    Sc->iter = it;
+    Sc->done = solved || ill || it >= Sc->itmax;
+}
+
+// Final x update, step capping and candidate theta (:1224-1230, _cap_position_step! :700-716).
+static __global__ void k_step(int na, int p, int k_y, int k_x, double max_step, const double* __restrict__ wv,
+                       double* __restrict__ x, const double* __restrict__ colnorm,
+                       const double* __restrict__ theta, double* __restrict__ dsc, double* __restrict__ theta_c,
+                       const uint8_t* __restrict__ live, const Scal* Sc) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    const size_t base = (size_t)a * p;
+    const bool any_iter = Sc->iter > 0;
+    const double sig = Sc->sig;
+    double d[3];
+    for (int k = 0; k < p; ++k) {
+        double xv = (any_iter && live[a]) ? x[base + k] + sig * wv[base + k] : 0.0;
+        x[base + k] = xv;
+        d[k] = xv / colnorm[base + k];
+    }
+    if (k_y >= 0 || k_x >= 0) {
+        double dy = k_y >= 0 ? d[k_y] : 0.0, dx = k_x >= 0 ? d[k_x] : 0.0;
+        double len = sqrt(dy * dy + dx * dx);
+        if (len > max_step) {
+            double sc = max_step / len;
+            if (k_y >= 0) d[k_y] *= sc;
+            if (k_x >= 0) d[k_x] *= sc;
+        }
+    }
+    for (int k = 0; k < p; ++k) {
+        dsc[base + k] = colnorm[base + k] * d[k];
+        theta_c[base + k] = theta[base + k] + d[k];
+    }
+}
+
+// Accepted step: theta, D, per-star freeze tests (:1246-1283, _x_tol_local :810-823).
+static __global__ void k_accept(int na, int p, int iter, double x_tol, double* __restrict__ theta,
+                         const double* __restrict__ theta_c, double* __restrict__ D,
+                         const double* __restrict__ colnorm, const double* __restrict__ dsc,
+                         const uint8_t* __restrict__ gloc, uint8_t* __restrict__ live,
+                         int* __restrict__ frozen_at, uint8_t* __restrict__ conv, Scal* Sc) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    int nl = 0, nf = 0;
+    if (a < na) {
+        const size_t base = (size_t)a * p;
+        for (int k = 0; k < p; ++k) {
+            theta[base + k] = theta_c[base + k];
+            D[base + k] = fmax(D[base + k], colnorm[base + k]);
+        }
+        if (live[a]) {
+            double num = 0, xn = 0;
+            for (int k = 0; k < p; ++k) {
+                double v = (D[base + k] / colnorm[base + k]) * dsc[base + k];
+                num += v * v;
+                double xv = D[base + k] * theta[base + k];
+                xn += xv * xv;
+            }
+            bool xt = sqrt(num) <= x_tol * (sqrt(xn) + x_tol);
+            if (xt || gloc[a]) {
+                live[a] = 0;
+                frozen_at[a] = iter;
+                conv[a] = 1;
+                nf = 1;
+            } else {
+                nl = 1;
+            }
+        }
+    }
+    nl = wave_sum_int(nl);
+    nf = wave_sum_int(nf);
+    if ((threadIdx.x & 63) == 0) {
+        if (nl) atomicAdd(&Sc->n_live, nl);
+        if (nf) atomicAdd(&Sc->n_frozen_now, nf);
+    }
+}
+
+// _freeze_remaining! (:767-776).
+static __global__ void k_freeze_all(int na, int iter, uint8_t* __restrict__ live, int* __restrict__ frozen_at,
+                             uint8_t* __restrict__ conv) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na || !live[a]) return;
+    live[a] = 0;
+    frozen_at[a] = iter;
+    conv[a] = 1;
+}
+
+// Per-star error block, covariance and errors (:1305-1329, covariance!
+// levenberg_marquardt.jl:366-388).  cov_mode 1 known, 2 reweighted, -1 none.
+template <typename T>
+static __global__ void k_errors(int na, int p, int R, int S, const int* __restrict__ ay, const int* __restrict__ ax,
+                         const T* __restrict__ w, int ny, int nx, const T* __restrict__ V,
+                         const double* __restrict__ colnorm, int cov_mode, double fac_rw,
+                         double* __restrict__ errs) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    const int S2 = S * S;
+    double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < S2; ++m) {
+        int gy = ay[a] + (m % S) - R, gx = ax[a] + (m / S) - R;
+        if (gy < 1 || gy > ny || gx < 1 || gx > nx) continue;
+        if (!(w[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] > T(0))) continue;
+        const T* Vm = V + ((size_t)a * S2 + m) * p;
+        for (int k = 0; k < p; ++k)
+            for (int l = 0; l < p; ++l) blk[k * 3 + l] += (double)Vm[k] * (double)Vm[l];
+    }
+    const double* cn = colnorm + (size_t)a * p;
+    double tr = 0;
+    for (int k = 0; k < p; ++k)
+        for (int l = 0; l < p; ++l) blk[k * 3 + l] *= cn[k] * cn[l];
+    for (int k = 0; k < p; ++k) tr += blk[k * 3 + k];
+    for (int k = 0; k < p; ++k) blk[k * 3 + k] += 1e-12 * tr;
+    double cov[9];
+    // Cholesky inverse, else eigen pseudo-inverse (pinv)
+    double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool pd = true;
+    for (int j = 0; j < p && pd; ++j) {
+        double s = blk[j * 3 + j];
+        for (int k = 0; k < j; ++k) s -= L[j * 3 + k] * L[j * 3 + k];
+        if (!(s > 0)) { pd = false; break; }
+        double d = sqrt(s);
+        L[j * 3 + j] = d;
+        for (int i = j + 1; i < p; ++i) {
+            double t = blk[j * 3 + i];
+            for (int k = 0; k < j; ++k) t -= L[i * 3 + k] * L[j * 3 + k];
+            L[i * 3 + j] = t / d;
+        }
+    }
+    if (pd) {
+        for (int j = 0; j < p; ++j) {
+            double y[3], xs[3];
+            for (int i = 0; i < p; ++i) {
+                double s = (i == j);
+                for (int k = 0; k < i; ++k) s -= L[i * 3 + k] * y[k];
+                y[i] = s / L[i * 3 + i];
+            }
+            for (int i = p - 1; i >= 0; --i) {
+                double s = y[i];
+                for (int k = i + 1; k < p; ++k) s -= L[k * 3 + i] * xs[k];
+                xs[i] = s / L[i * 3 + i];
+            }
+            for (int i = 0; i < p; ++i) cov[i * 3 + j] = xs[i];
+        }
+    } else {
+        double A[9], E[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < 9; ++i) A[i] = blk[i];
+        for (int i = 0; i < p; ++i) E[i * 3 + i] = 1;
+        for (int sweep = 0; sweep < 60; ++sweep) {
+            double off = 0;
+            for (int i = 0; i < p; ++i)
+                for (int j = i + 1; j < p; ++j) off += A[i * 3 + j] * A[i * 3 + j];
+            if (off < 1e-300) break;
+            for (int i = 0; i < p; ++i)
+                for (int j = i + 1; j < p; ++j) {
+                    double aij = A[i * 3 + j];
+                    if (aij == 0) continue;
+                    double th = (A[j * 3 + j] - A[i * 3 + i]) / (2 * aij);
+                    double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                    double cc = 1 / sqrt(t * t + 1), ss = t * cc;
+                    for (int k = 0; k < p; ++k) {
+                        double aki = A[k * 3 + i], akj = A[k * 3 + j];
+                        A[k * 3 + i] = cc * aki - ss * akj;
+                        A[k * 3 + j] = ss * aki + cc * akj;
+                    }
+                    for (int k = 0; k < p; ++k) {
+                        double aik = A[i * 3 + k], ajk = A[j * 3 + k];
+                        A[i * 3 + k] = cc * aik - ss * ajk;
+                        A[j * 3 + k] = ss * aik + cc * ajk;
+                    }
+                    for (int k = 0; k < p; ++k) {
+                        double vki = E[k * 3 + i], vkj = E[k * 3 + j];
+                        E[k * 3 + i] = cc * vki - ss * vkj;
+                        E[k * 3 + j] = ss * vki + cc * vkj;
+                    }
+                }
+        }
+        double smax = 0;
+        for (int i = 0; i < p; ++i) smax = fmax(smax, fabs(A[i * 3 + i]));
+        double tol = DEPS * p * smax;
+        for (int i = 0; i < p; ++i)
+            for (int j = 0; j < p; ++j) {
+                double s = 0;
+                for (int k = 0; k < p; ++k)
+                    if (fabs(A[k * 3 + k]) > tol) s += E[i * 3 + k] * E[j * 3 + k] / A[k * 3 + k];
+                cov[i * 3 + j] = s;
+            }
+    }
+    double fac = cov_mode == 2 ? fac_rw : 1.0;
+    for (int k = 0; k < p; ++k) errs[(size_t)a * p + k] = sqrt(fmax(0.0, fac * cov[k * 3 + k]));
+}
+
+// Per-star reduced chi^2 over the +-fit_rad box at the fitted position (:1368-1397).
+template <typename T>
+static __global__ void k_chisq(int na, int p, const double* __restrict__ theta, FreeMap fm, double fit_rad,
+                        const T* __restrict__ resid, const float* __restrict__ iv, int ny, int nx,
+                        double* __restrict__ chisq) {
+    int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    double Y, X, F;
+    star_yxf(fm, theta + (size_t)a * p, Y, X, F);
+    chisq[a] = 0;
+    if (!(isfinite(Y) && isfinite(X) && isfinite(F) && F > 0)) return;
+    int ylo = max((int)floor(Y - fit_rad), 1), yhi = min((int)ceil(Y + fit_rad), ny);
+    int xlo = max((int)floor(X - fit_rad), 1), xhi = min((int)ceil(X + fit_rad), nx);
+    if (ylo > yhi || xlo > xhi) return;
+    double num = 0;
+    int good = 0;
+    for (int gx = xlo; gx <= xhi; ++gx)
+        for (int gy = ylo; gy <= yhi; ++gy) {
+            size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * ny;
+            double ivv = iv ? (double)iv[q] : 1.0;
+            if (isfinite(ivv) && ivv > 0) {
+                double r = (double)resid[q];
+                num += ivv * r * r;
+                ++good;
+            }
+        }
+    int den = good - p;
+    if (den > 0) chisq[a] = num / den;
+}
+
+template <typename T>
+static __global__ void k_to_float(const T* __restrict__ src, float* __restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+        dst[i] = (float)src[i];
+}
+
+static __global__ void k_w_sample(const float* __restrict__ src, int64_t P, int64_t step, float* __restrict__ dst,
+                           int64_t nout) {
+    int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (i < nout) dst[i] = src[i * step];
+}
+
+}  // namespace cp

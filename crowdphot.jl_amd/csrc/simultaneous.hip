@@ -1,0 +1,818 @@
+// simultaneous.hip -- host orchestration of fit_all_stars_simultaneous on
+// gfx950 (src/photometry/psf_photometry_simultaneous.jl:953-1431).
+//
+// Control flow mirrors the reference outer loop: per outer iteration one
+// fused render+residual pass, one fill+J^T r pass, then damping trials of
+// (inner LSQR solve, capped step, candidate cost) until a trial lowers the
+// cost; accepted steps run the per-star freeze tests and subtract newly
+// frozen stars from the working image.  The host reads back a few scalars per
+// trial (cost, cost_cand, counts) to take the same accept / converge
+// decisions; everything else stays on the device.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sim_kernels.h"
+#include "psf_host.h"
+
+namespace cp {
+
+// ---------------------------------------------------------------------------
+// hipcub helpers
+// ---------------------------------------------------------------------------
+struct Cub {
+    cp_ctx* ctx;
+    DevBuf tmp;
+    void ensure(size_t b) {
+        if (tmp.bytes() < b) tmp.alloc(ctx, std::max<size_t>(b, 1 << 20));
+    }
+};
+
+// Selected (ok) catalogue ids, stably sorted by Morton key: the active list.
+int build_active(cp_ctx* ctx, Cub& cub, const int* d_ay, const int* d_ax, const uint8_t* d_ok, int n,
+                 int coord_bits, DevBuf& act_out) {
+    hipStream_t st = ctx->stream;
+    DevBuf sel(ctx, (size_t)std::max(n, 1) * 4), nsel(ctx, 8);
+    hipcub::CountingInputIterator<int> it(0);
+    size_t b = 0;
+    CP_HIP(hipcub::DeviceSelect::Flagged(nullptr, b, it, d_ok, sel.as<int>(), nsel.as<int>(), n, st));
+    cub.ensure(b);
+    b = cub.tmp.bytes();
+    CP_HIP(hipcub::DeviceSelect::Flagged(cub.tmp.get(), b, it, d_ok, sel.as<int>(), nsel.as<int>(), n, st));
+    int na = 0;
+    CP_HIP(hipMemcpyAsync(&na, nsel.get(), 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipStreamSynchronize(st));
+    act_out.alloc(ctx, (size_t)std::max(na, 1) * 4);
+    if (na == 0) return 0;
+    DevBuf keys(ctx, (size_t)na * 8), keys2(ctx, (size_t)na * 8);
+    launch(k_morton_keys, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, (const int*)sel.as<int>(), na, d_ay, d_ax,
+           keys.as<unsigned long long>());
+    int end_bit = std::min(64, 2 * coord_bits);
+    b = 0;
+    CP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, keys.as<unsigned long long>(),
+                                              keys2.as<unsigned long long>(), sel.as<int>(), act_out.as<int>(),
+                                              na, 0, end_bit, st));
+    cub.ensure(b);
+    b = cub.tmp.bytes();
+    CP_HIP(hipcub::DeviceRadixSort::SortPairs(cub.tmp.get(), b, keys.as<unsigned long long>(),
+                                              keys2.as<unsigned long long>(), sel.as<int>(), act_out.as<int>(),
+                                              na, 0, end_bit, st));
+    return na;
+}
+
+struct TileLists {
+    DevBuf start, end, list;
+    int npairs = 0;
+};
+
+// Stars whose box (half-width hw[a] or h) overlaps each tile, in star order.
+void build_tiles(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax, const int* d_hw, int h,
+                 const TileGeo& g, TileLists& L) {
+    hipStream_t st = ctx->stream;
+    const int ntiles = g.nty * g.ntx;
+    DevBuf cnt(ctx, (size_t)na * 4), off(ctx, (size_t)na * 4), tot(ctx, 8);
+    launch(k_tile_count, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, d_ay, d_ax, d_hw, h, g, cnt.as<int>());
+    size_t b = 0;
+    CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, cnt.as<int>(), off.as<int>(), na, st));
+    cub.ensure(b);
+    b = cub.tmp.bytes();
+    CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, cnt.as<int>(), off.as<int>(), na, st));
+    int last_off = 0, last_cnt = 0;
+    CP_HIP(hipMemcpyAsync(&last_off, off.as<int>() + na - 1, 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipMemcpyAsync(&last_cnt, cnt.as<int>() + na - 1, 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipStreamSynchronize(st));
+    const int np = last_off + last_cnt;
+    L.npairs = np;
+    L.start.alloc(ctx, (size_t)ntiles * 4);
+    L.end.alloc(ctx, (size_t)ntiles * 4);
+    CP_HIP(hipMemsetAsync(L.start.get(), 0, (size_t)ntiles * 4, st));
+    CP_HIP(hipMemsetAsync(L.end.get(), 0, (size_t)ntiles * 4, st));
+    L.list.alloc(ctx, (size_t)std::max(np, 1) * 4);
+    if (np == 0) return;
+    DevBuf k1(ctx, (size_t)np * 4), k2(ctx, (size_t)np * 4), v1(ctx, (size_t)np * 4);
+    launch(k_tile_emit, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, d_ay, d_ax, d_hw, h, g,
+           (const int*)off.as<int>(), k1.as<unsigned int>(), v1.as<int>());
+    int end_bit = 1;
+    while ((1LL << end_bit) < ntiles) ++end_bit;
+    b = 0;
+    CP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, k1.as<unsigned int>(), k2.as<unsigned int>(),
+                                              v1.as<int>(), L.list.as<int>(), np, 0, end_bit, st));
+    cub.ensure(b);
+    b = cub.tmp.bytes();
+    CP_HIP(hipcub::DeviceRadixSort::SortPairs(cub.tmp.get(), b, k1.as<unsigned int>(), k2.as<unsigned int>(),
+                                              v1.as<int>(), L.list.as<int>(), np, 0, end_bit, st));
+    launch(k_tile_ranges, dim3(cdiv(np, BLOCK)), dim3(BLOCK), 0, st, (const unsigned int*)k2.as<unsigned int>(),
+           np, L.start.as<int>(), L.end.as<int>());
+}
+
+static int bit_length(int64_t v) {
+    int b = 1;
+    while ((1LL << b) <= v) ++b;
+    return b;
+}
+
+// Julia type-7 quantile.
+static double quantile7(std::vector<double> v, double p) {
+    std::sort(v.begin(), v.end());
+    size_t n = v.size();
+    double h = (n - 1) * p;
+    size_t lo = (size_t)std::floor(h);
+    if (lo + 1 >= n) return v[n - 1];
+    return v[lo] + (h - lo) * (v[lo + 1] - v[lo]);
+}
+
+// _model_radii :auto for the analytic-CoG models (:209-243, curve_of_growth.jl:180-211).
+static std::vector<int> auto_radii(const HostPsf& P, double nsigma, int R_fit, int R_cap,
+                                   const std::vector<double>& wsample, const std::vector<double>& flux) {
+    CP_ARG(P.kind != CP_PSF_GRIDDED_IMAGE,
+           "model_rad=:auto for ImagePSF/GriddedPSFModel needs the aperture curve of growth, not built; "
+           "pass a scalar model_rad");
+    std::vector<double> ws;
+    for (double v : wsample)
+        if (v > 0 && std::isfinite(v)) ws.push_back(v);
+    CP_ARG(!ws.empty(), "no positive finite weights for the :auto noise estimate");
+    double wv = quantile7(ws, 0.84);
+    double sigma_bg = wv > 0 ? std::sqrt(1 / wv) : 1.0;
+    auto cog = [&](double r) {
+        if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN) return 1 - std::exp(CP_GAUSS_PRE * r * r / (P.fwhm * P.fwhm));
+        return 1 - std::pow(1 + r * r / (P.alpha * P.alpha), 1 - P.beta);
+    };
+    std::vector<double> sb(R_cap);
+    double last = cog(R_cap), prev = 0;
+    for (int r = 1; r <= R_cap; ++r) {
+        double ee = cog(r) / last;
+        sb[r - 1] = std::max((ee - prev) / (M_PI * (double)(r * r - (r - 1) * (r - 1))), 0.0);
+        prev = ee;
+    }
+    double thr = nsigma * sigma_bg;
+    std::vector<int> out(flux.size());
+    for (size_t j = 0; j < flux.size(); ++j) {
+        double F = std::max(flux[j], 0.0);
+        int r = R_fit;
+        while (r < R_cap && F * sb[r - 1] > thr) ++r;
+        out[j] = r;
+    }
+    return out;
+}
+
+static int pick_epl(int S2) {
+    int need = (S2 + 63) / 64;
+    int e = 1;
+    while (e < need) e <<= 1;
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// the fit
+// ---------------------------------------------------------------------------
+template <typename T, int KIND>
+struct Sim {
+    cp_ctx* ctx;
+    const HostPsf& P;
+    DevPsf& DP;
+    hipStream_t st;
+
+    Sim(cp_ctx* c, const HostPsf& p, DevPsf& d) : ctx(c), P(p), DP(d), st(c->stream) {}
+
+    template <bool FINAL, bool LDSG>
+    void fill_launch(int epl, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
+                     const CornerRec<T>* crs, int na, int R, int S, const FreeMap& fm, const T* w, int ny, int nx,
+                     const T* wres, T* V, double* cn, double* b, uint8_t* gloc, double gtol, double* part,
+                     Scal* Sc) {
+#define CP_FILL(E)                                                                                           \
+    launch_n(FINAL ? "k_fill_final" : "k_fill", k_fill<T, KIND, FINAL, LDSG, E>, grid, dim3(BLOCK), sh, st, M, recs, crs, na, R, S, fm, w, ny, nx, \
+           wres, V, cn, b, gloc, gtol, part, Sc)
+        switch (epl) {
+            case 1: CP_FILL(1); break;
+            case 2: CP_FILL(2); break;
+            case 4: CP_FILL(4); break;
+            case 8: CP_FILL(8); break;
+            default: CP_FILL(16); break;
+        }
+#undef CP_FILL
+    }
+
+    template <int MODE>
+    void render_launch(bool ldsg, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
+                       const CornerRec<T>* crs, const TileLists& L, const TileGeo& g, int ntiles, int tpb,
+                       const uint8_t* umask, T* dw, const T* w, const T* data, T* wres, T* resid, double* part,
+                       Scal* Sc) {
+        static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
+        if (ldsg)
+            launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(BLOCK), sh, st, M, recs, crs,
+                   (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
+                   ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
+        else
+            launch_n(nm[MODE], k_render<T, KIND, MODE, false>, grid, dim3(BLOCK), sh, st, M, recs, crs,
+                   (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
+                   ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
+    }
+
+    void run(const float* d_img, const float* d_iv, int ny, int nx, const double* d_y0, const double* d_x0,
+             const double* d_f0, int n, double fit_rad, const cp_sim_opts& o, cp_sim_result* out,
+             const std::vector<double>& hy0, const std::vector<double>& hx0, const std::vector<double>& hf0) {
+        const int64_t Pn = (int64_t)ny * nx;
+        const int R = (int)std::ceil(fit_rad), S = 2 * R + 1, S2 = S * S;
+        CP_ARG(S2 <= 64 * MAX_EPL, "fit_rad too large for this build (ceil(fit_rad) <= 15)");
+        // free set (y, x, flux in getproperties order)
+        FreeMap fm{};
+        int gcs[3] = {0, 0, 0}, np_ = 0;
+        if (o.free_mask & CP_FREE_Y) gcs[np_++] = 1;
+        if (o.free_mask & CP_FREE_X) gcs[np_++] = 2;
+        if (o.free_mask & CP_FREE_FLUX) gcs[np_++] = 3;
+        fm.p = np_;
+        fm.gc0 = gcs[0];
+        fm.gc1 = gcs[1];
+        fm.gc2 = gcs[2];
+        const int p = fm.p;
+        fm.y_fixed = !(o.free_mask & CP_FREE_Y);
+        fm.x_fixed = !(o.free_mask & CP_FREE_X);
+        fm.f_fixed = !(o.free_mask & CP_FREE_FLUX);
+        fm.fixed_y = o.fixed_y;
+        fm.fixed_x = o.fixed_x;
+        fm.fixed_f = o.fixed_flux;
+        int k_y = -1, k_x = -1;
+        for (int k = 0; k < p; ++k) {
+            if (gcs[k] == 1) k_y = k;
+            if (gcs[k] == 2) k_x = k;
+        }
+        const double model_rad_max = o.model_rad_max > 0 ? o.model_rad_max : 10 * fit_rad;
+        const int n_cu = std::max(ctx->n_cu, 1);
+
+        // --- outputs for the empty / all-failed cases
+        auto write_initial = [&](const std::vector<uint8_t>& valid) {
+            for (int i = 0; i < n; ++i) {
+                out->y[i] = hy0[i];
+                out->x[i] = hx0[i];
+                out->flux[i] = hf0[i];
+                out->bkg[i] = o.fixed_bkg;
+                out->y_err[i] = fm.y_fixed ? 0.0 : NAN;
+                out->x_err[i] = fm.x_fixed ? 0.0 : NAN;
+                out->flux_err[i] = fm.f_fixed ? 0.0 : NAN;
+                out->bkg_err[i] = 0;
+                out->converged[i] = 0;
+                out->valid[i] = valid[i];
+                out->n_iter[i] = 0;
+                if (out->chisq) out->chisq[i] = 0;
+            }
+        };
+        out->trace_len = 0;
+        out->n_trials_total = 0;
+        out->n_inner_total = 0;
+        out->n_active = 0;
+        out->n_union = 0;
+        out->cost_final = 0;
+        if (n == 0) {
+            out->n_passes = 0;
+            out->n_failed = 0;
+            return;
+        }
+
+        // --- device scalars
+        DevBuf dS(ctx, sizeof(Scal));
+        Scal* Sc = dS.as<Scal>();
+        Scal* hS = static_cast<Scal*>(pinned(ctx, sizeof(Scal)));
+        CP_HIP(hipMemsetAsync(Sc, 0, sizeof(Scal), st));
+        auto read_S = [&]() {
+            CP_HIP(hipMemcpyAsync(hS, Sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
+            CP_HIP(hipStreamSynchronize(st));
+        };
+
+        // --- data and weights
+        DevBuf data(ctx, Pn * sizeof(T)), dw(ctx, Pn * sizeof(T)), w(ctx, Pn * sizeof(T));
+        launch(k_prep_pixels<T>, dim3(std::min<int64_t>(cdiv(Pn, BLOCK), n_cu * 16)), dim3(BLOCK), 0, st, d_img,
+               d_iv, Pn, data.as<T>(), dw.as<T>(), w.as<T>(), Sc);
+        read_S();
+        CP_ARG(hS->n_nonfinite == 0 || d_iv != nullptr,
+               "the image contains non-finite values but `inv_var` was not provided; pass `inv_var` to mask "
+               "them or clean the image first");
+
+        // --- stamps, active list (Morton order), union
+        Cub cub{ctx, DevBuf()};
+        DevBuf ay_cat(ctx, (size_t)n * 4), ax_cat(ctx, (size_t)n * 4), ok(ctx, (size_t)n);
+        launch(k_anchor<T>, dim3(cdiv(n, BLOCK)), dim3(BLOCK), 0, st, d_y0, d_x0, d_f0, n, (const T*)w.as<T>(), ny,
+               nx, R, ay_cat.as<int>(), ax_cat.as<int>(), ok.as<uint8_t>());
+        DevBuf act;
+        int coord_bits = bit_length((int64_t)std::max(ny, nx) + R + 1);
+        const int na = build_active(ctx, cub, ay_cat.as<int>(), ax_cat.as<int>(), ok.as<uint8_t>(), n, coord_bits, act);
+        std::vector<int> hact(std::max(na, 1));
+        if (na > 0) {
+            CP_HIP(hipMemcpyAsync(hact.data(), act.get(), (size_t)na * 4, hipMemcpyDeviceToHost, st));
+            CP_HIP(hipStreamSynchronize(st));
+        }
+        std::vector<uint8_t> valid(n, 0);
+        for (int j = 0; j < na; ++j) valid[hact[j]] = 1;
+        if (out->active)
+            for (int i = 0; i < n; ++i) out->active[i] = valid[i];
+        out->n_failed = n - na;
+        if (na == 0) {
+            write_initial(valid);
+            if (out->residual) std::memset(out->residual, 0, sizeof(float) * Pn);
+            out->n_passes = 0;
+            return;
+        }
+        DevBuf umask(ctx, Pn);
+        CP_HIP(hipMemsetAsync(umask.get(), 0, Pn, st));
+        DevBuf ay(ctx, (size_t)na * 4), ax(ctx, (size_t)na * 4), mR(ctx, (size_t)na * 4);
+        DevBuf theta(ctx, (size_t)na * p * 8), live(ctx, na), frozen_at(ctx, (size_t)na * 4), conv(ctx, na);
+
+        // --- model radii (:1113-1124)
+        CP_ARG(!(o.model_rad <= 0 && d_iv == nullptr),
+               "model_rad=:auto requires inv_var (it needs a noise scale); pass inv_var or an explicit scalar model_rad");
+        const int R_cap = (int)std::ceil(o.model_rad <= 0 ? model_rad_max : std::max(o.model_rad, fit_rad));
+        int mR_scalar = 0;
+        DevBuf mR_src;
+        if (o.model_rad > 0) {
+            mR_scalar = std::min(std::max((int)std::ceil(o.model_rad), R), R_cap);
+        } else {
+            int64_t step = std::max<int64_t>(1, Pn / 65536);
+            int64_t ns = (Pn + step - 1) / step;
+            std::vector<T> hw(ns);
+            // strided sample of w (w[1:step:end])
+            DevBuf samp(ctx, ns * sizeof(T));
+            CP_HIP(hipMemcpy2DAsync(samp.get(), sizeof(T), w.get(), step * sizeof(T), sizeof(T), ns,
+                                    hipMemcpyDeviceToDevice, st));
+            CP_HIP(hipMemcpyAsync(hw.data(), samp.get(), ns * sizeof(T), hipMemcpyDeviceToHost, st));
+            CP_HIP(hipStreamSynchronize(st));
+            std::vector<double> wsd(hw.begin(), hw.end()), fl(na);
+            for (int j = 0; j < na; ++j) fl[j] = hf0[hact[j]];
+            std::vector<int> rr = auto_radii(P, o.model_rad_nsigma, R, R_cap, wsd, fl);
+            mR_src.alloc(ctx, (size_t)na * 4);
+            CP_HIP(hipMemcpyAsync(mR_src.get(), rr.data(), (size_t)na * 4, hipMemcpyHostToDevice, st));
+            CP_HIP(hipStreamSynchronize(st));
+        }
+        launch(k_star_setup, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, (const int*)act.as<int>(), na,
+               (const int*)ay_cat.as<int>(), (const int*)ax_cat.as<int>(), d_y0, d_x0, d_f0, p, fm.gc0, fm.gc1,
+               fm.gc2, (const int*)(o.model_rad > 0 ? nullptr : mR_src.as<int>()), mR_scalar, ay.as<int>(),
+               ax.as<int>(), mR.as<int>(), theta.as<double>(), live.as<uint8_t>(), frozen_at.as<int>(),
+               conv.as<uint8_t>());
+        launch(k_union<T>, dim3(cdiv((int64_t)na * S2, BLOCK)), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
+               (const int*)ax.as<int>(), R, S, (const T*)w.as<T>(), ny, nx, umask.as<uint8_t>());
+        launch(k_count_mask, dim3(std::min<int64_t>(cdiv(Pn, BLOCK), n_cu * 16)), dim3(BLOCK), 0, st,
+               (const uint8_t*)umask.as<uint8_t>(), Pn, &Sc->n_union);
+        read_S();
+        const int64_t U = (int64_t)hS->n_union;
+        const int64_t nvar = (int64_t)p * na;
+        const double dof = (double)std::max<int64_t>(U - nvar, 1);
+        CP_HIP(hipMemcpyAsync(&Sc->dof, &dof, 8, hipMemcpyHostToDevice, st));
+
+        // --- tiles: fit list (stamp box) and model list (model box)
+        int TY = 32;
+        if (o.tile == 16 || o.tile == 32 || o.tile == 64 || o.tile == 128 || o.tile == 256) TY = o.tile;
+        TileGeo g{TY, TILE_PX / TY, (ny + TY - 1) / TY, (nx + TILE_PX / TY - 1) / (TILE_PX / TY), ny, nx};
+        const int ntiles = g.nty * g.ntx;
+        TileLists Lfit, Lmod;
+        build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), nullptr, R, g, Lfit);
+        build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), o.model_rad > 0 ? nullptr : mR.as<int>(),
+                    mR_scalar, g, Lmod);
+        cub.tmp.release();
+
+        // --- working buffers
+        DevBuf wres(ctx, Pn * sizeof(T)), ubuf(ctx, Pn * sizeof(T));
+        CP_HIP(hipMemsetAsync(wres.get(), 0, Pn * sizeof(T), st));
+        CP_HIP(hipMemsetAsync(ubuf.get(), 0, Pn * sizeof(T), st));
+        DevBuf V(ctx, (size_t)na * S2 * p * sizeof(T));
+        CP_HIP(hipMemsetAsync(V.get(), 0, V.bytes(), st));
+        const size_t nb8 = (size_t)na * p * 8;
+        DevBuf colnorm(ctx, nb8), D(ctx, nb8), bsc(ctx, nb8), xs(ctx, nb8), wl(ctx, nb8), tb(ctx, nb8), dsc(ctx, nb8),
+            theta_c(ctx, nb8);
+        CP_HIP(hipMemsetAsync(colnorm.get(), 0, nb8, st));
+        CP_HIP(hipMemsetAsync(xs.get(), 0, nb8, st));
+        CP_HIP(hipMemsetAsync(wl.get(), 0, nb8, st));
+        CP_HIP(hipMemsetAsync(tb.get(), 0, nb8, st));
+        DevBuf gloc(ctx, na), recs(ctx, (size_t)na * sizeof(StarRec<T>)), recs_c(ctx, (size_t)na * sizeof(StarRec<T>));
+        DevBuf crs, crs_c;
+        const bool multi = (KIND == KIND_EPSF && P.n_nodes > 1);
+        if (multi) {
+            crs.alloc(ctx, (size_t)na * sizeof(CornerRec<T>));
+            crs_c.alloc(ctx, (size_t)na * sizeof(CornerRec<T>));
+        } else {
+            crs.alloc(ctx, sizeof(CornerRec<T>));
+            crs_c.alloc(ctx, sizeof(CornerRec<T>));
+        }
+        const int maxpb = n_cu * 8;
+        DevBuf part(ctx, (size_t)maxpb * 8), part2(ctx, (size_t)maxpb * 8);
+
+        const ModelConst<T> M = model_const<T>(P, DP, o.fixed_bkg);
+        const double inv_norm_g64 = (KIND == KIND_GAUSS) ? 1.0 / (-(M_PI * P.fwhm * P.fwhm / CP_GAUSS_PRE)) : 0.0;
+        const double moffat_amp64 = (KIND == KIND_MOFFAT) ? (P.beta - 1) / (M_PI * P.alpha * P.alpha) : 0.0;
+        const bool ldsg = (KIND == KIND_EPSF) && P.n_nodes == 1 && (size_t)P.gny * P.gnx * 4 <= 56 * 1024;
+        const size_t grid_sh = ldsg ? (((size_t)P.gny * P.gnx * 4 + 15) & ~(size_t)15) : 0;
+        const size_t render_sh = grid_sh + CHUNK * (sizeof(StarRec<T>) + 4);
+        const size_t fill_sh = grid_sh;
+
+        const int nb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 4);
+        const int tpb_render = (int)cdiv(ntiles, nb_render);
+        const int gb_render = (int)cdiv(ntiles, tpb_render);
+        const int nb_jv = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 8);
+        const int tpb_jv = (int)cdiv(ntiles, nb_jv);
+        const int gb_jv = (int)cdiv(ntiles, tpb_jv);
+        const int gb_fill = (int)std::min<int64_t>(cdiv(na, NWAVE), (int64_t)n_cu * 4);
+        const int gb_jtu = (int)std::min<int64_t>(cdiv(na, NWAVE), (int64_t)n_cu * 8);
+        const int epl = pick_epl(S2);
+        const unsigned int gstar = cdiv(na, BLOCK);
+
+        auto star_rec = [&](const double* th, DevBuf& r, DevBuf& c, int frozen_iter, int all_live) {
+            launch(k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, th, fm, M, inv_norm_g64, moffat_amp64,
+                   (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const int*)mR.as<int>(),
+                   (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), frozen_iter, all_live,
+                   r.as<StarRec<T>>(), DP.G, c.as<CornerRec<T>>());
+        };
+        auto fill = [&](bool final_mode) {
+            if (ldsg) {
+                if (final_mode)
+                    fill_launch<true, true>(epl, dim3(gb_fill), fill_sh, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(), na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                else
+                    fill_launch<false, true>(epl, dim3(gb_fill), fill_sh, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(), na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+            } else {
+                if (final_mode)
+                    fill_launch<true, false>(epl, dim3(gb_fill), 0, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(), na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                else
+                    fill_launch<false, false>(epl, dim3(gb_fill), 0, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(), na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+            }
+        };
+        auto zero_field = [&](void* dev_field, size_t bytes) { CP_HIP(hipMemsetAsync(dev_field, 0, bytes, st)); };
+
+        // --- outer damped Gauss-Newton loop (:1186-1284)
+        double lam = o.lambda_init, cost = 0;
+        int n_run = 0, n_live_now = na;
+        bool D_init = false;
+        for (int iter = 1; iter <= o.max_iter; ++iter) {
+            star_rec(theta.as<double>(), recs, crs, 0, 0);
+            render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
+                                      crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render, umask.as<uint8_t>(),
+                                      dw.as<T>(), w.as<T>(), data.as<T>(), wres.as<T>(), nullptr, part.as<double>(), Sc);
+            zero_field(&Sc->gmax_bits, 8);
+            fill(false);
+            if (!D_init) {
+                CP_HIP(hipMemcpyAsync(D.get(), colnorm.get(), nb8, hipMemcpyDeviceToDevice, st));
+                D_init = true;
+            }
+            read_S();
+            cost = hS->cost;
+            double gnorm;
+            {
+                unsigned long long bits = hS->gmax_bits;
+                std::memcpy(&gnorm, &bits, 8);
+            }
+            if (gnorm <= o.g_tol) {
+                launch(k_freeze_all, dim3(gstar), dim3(BLOCK), 0, st, na, iter - 1, live.as<uint8_t>(),
+                       frozen_at.as<int>(), conv.as<uint8_t>());
+                n_run = iter - 1;
+                break;
+            }
+            bool accepted = false, f_conv = false;
+            for (int trial = 1; trial <= o.max_trials; ++trial) {
+                const double lam_solve = std::sqrt(lam);
+                launch(k_lsqr_init, dim3(1), dim3(64), 0, st, Sc, lam_solve, o.inner_iterations, o.linear_tol,
+                       o.linear_tol);
+                for (int it = 1; it <= o.inner_iterations; ++it) {
+                    const int first = it == 1;
+                    launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, (const int*)Lfit.start.as<int>(),
+                           (const int*)Lfit.end.as<int>(), (const int*)Lfit.list.as<int>(), g, ntiles, tpb_jv,
+                           (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S,
+                           p, (const T*)V.as<T>(), (const double*)(first ? bsc.as<double>() : tb.as<double>()),
+                           (const T*)(first ? wres.as<T>() : ubuf.as<T>()), ubuf.as<T>(),
+                           (const uint8_t*)umask.as<uint8_t>(), first, part.as<double>(), Sc);
+                    launch_n("k_JTu", k_JTu<T>, dim3(gb_jtu), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
+                           (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
+                           (const T*)V.as<T>(), (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(),
+                           tb.as<double>(), xs.as<double>(), wl.as<double>(), first, it, part.as<double>(),
+                           part2.as<double>(), Sc);
+                }
+                launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
+                       (const double*)wl.as<double>(), xs.as<double>(), (const double*)colnorm.as<double>(),
+                       (const double*)theta.as<double>(), dsc.as<double>(), theta_c.as<double>(),
+                       (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
+                star_rec(theta_c.as<double>(), recs_c, crs_c, 0, 0);
+                render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
+                                         crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
+                                         umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres.as<T>(),
+                                         nullptr, part.as<double>(), Sc);
+                read_S();
+                const double cost_c = hS->cost_cand;
+                out->n_trials_total += 1;
+                out->n_inner_total += hS->iter;
+                const double dcost = cost - cost_c;
+                f_conv = (cost_c < cost) && (dcost / cost <= o.f_tol);
+                if (out->trace && out->trace_len < out->trace_cap) {
+                    double* t = out->trace + 8 * (size_t)out->trace_len++;
+                    t[0] = iter;
+                    t[1] = trial;
+                    t[2] = cost;
+                    t[3] = cost_c;
+                    t[4] = lam;
+                    t[5] = n_live_now;
+                    t[6] = hS->iter;
+                    t[7] = 0;
+                }
+                if (cost_c < cost) {
+                    accepted = true;
+                    cost = cost_c;
+                    lam = std::max(lam / o.lambda_down, o.lambda_min);
+                    break;
+                }
+                lam = std::min(lam * o.lambda_up, o.lambda_max);
+            }
+            n_run = iter;
+            if (f_conv) {
+                // theta <- theta + delta for the accepted step, then freeze everyone
+                zero_field(&Sc->n_live, 4);
+                zero_field(&Sc->n_frozen_now, 4);
+                launch(k_accept, dim3(gstar), dim3(BLOCK), 0, st, na, p, iter, -1.0, theta.as<double>(),
+                       (const double*)theta_c.as<double>(), D.as<double>(), (const double*)colnorm.as<double>(),
+                       (const double*)dsc.as<double>(), (const uint8_t*)gloc.as<uint8_t>(), live.as<uint8_t>(),
+                       frozen_at.as<int>(), conv.as<uint8_t>(), Sc);
+                launch(k_freeze_all, dim3(gstar), dim3(BLOCK), 0, st, na, iter, live.as<uint8_t>(),
+                       frozen_at.as<int>(), conv.as<uint8_t>());
+                break;
+            }
+            if (accepted) {
+                zero_field(&Sc->n_live, 4);
+                zero_field(&Sc->n_frozen_now, 4);
+                launch(k_accept, dim3(gstar), dim3(BLOCK), 0, st, na, p, iter, o.x_tol, theta.as<double>(),
+                       (const double*)theta_c.as<double>(), D.as<double>(), (const double*)colnorm.as<double>(),
+                       (const double*)dsc.as<double>(), (const uint8_t*)gloc.as<uint8_t>(), live.as<uint8_t>(),
+                       frozen_at.as<int>(), conv.as<uint8_t>(), Sc);
+                read_S();
+                n_live_now = hS->n_live;
+                if (hS->n_frozen_now > 0) {
+                    star_rec(theta.as<double>(), recs, crs, iter, 0);
+                    render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
+                                               crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
+                                               umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(),
+                                               wres.as<T>(), nullptr, part.as<double>(), Sc);
+                }
+                if (hS->n_live == 0) break;
+            }
+        }
+
+        // --- epilogue: errors (:1302-1329), final render and chisq (:1344-1397)
+        const int cov_mode = o.covariance == CP_COV_DEFAULT ? (d_iv ? CP_COV_KNOWN : CP_COV_REWEIGHTED) : o.covariance;
+        star_rec(theta.as<double>(), recs, crs, 0, 1);
+        fill(true);
+        DevBuf errs(ctx, nb8);
+        CP_HIP(hipMemsetAsync(errs.get(), 0, nb8, st));
+        if (cov_mode != CP_COV_NONE)
+            launch(k_errors<T>, dim3(gstar), dim3(BLOCK), 0, st, na, p, R, S, (const int*)ay.as<int>(),
+                   (const int*)ax.as<int>(), (const T*)w.as<T>(), ny, nx, (const T*)V.as<T>(),
+                   (const double*)colnorm.as<double>(), cov_mode, cost / dof, errs.as<double>());
+        T* resid = ubuf.as<T>();
+        render_launch<MODE_FINAL>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(),
+                                  Lmod, g, ntiles, tpb_render, umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
+                                  data.as<T>(), wres.as<T>(), resid, part.as<double>(), Sc);
+        DevBuf chis(ctx, (size_t)na * 8);
+        if (out->chisq)
+            launch(k_chisq<T>, dim3(gstar), dim3(BLOCK), 0, st, na, p, (const double*)theta.as<double>(), fm, fit_rad,
+                   (const T*)resid, d_iv, ny, nx, chis.as<double>());
+        std::vector<double> hth((size_t)na * p), herr((size_t)na * p), hchi(na, 0.0);
+        std::vector<uint8_t> hlive(na), hconv(na);
+        std::vector<int> hfz(na);
+        CP_HIP(hipMemcpyAsync(hth.data(), theta.get(), nb8, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipMemcpyAsync(herr.data(), errs.get(), nb8, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipMemcpyAsync(hlive.data(), live.get(), na, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipMemcpyAsync(hconv.data(), conv.get(), na, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipMemcpyAsync(hfz.data(), frozen_at.get(), (size_t)na * 4, hipMemcpyDeviceToHost, st));
+        if (out->chisq) CP_HIP(hipMemcpyAsync(hchi.data(), chis.get(), (size_t)na * 8, hipMemcpyDeviceToHost, st));
+        if (out->residual) {
+            if (sizeof(T) == 4) {
+                CP_HIP(hipMemcpyAsync(out->residual, resid, Pn * 4, hipMemcpyDeviceToHost, st));
+            } else {
+                DevBuf rf(ctx, Pn * 4);
+                launch(k_to_float<T>, dim3(std::min<int64_t>(cdiv(Pn, BLOCK), n_cu * 16)), dim3(BLOCK), 0, st,
+                       (const T*)resid, rf.as<float>(), Pn);
+                CP_HIP(hipMemcpyAsync(out->residual, rf.get(), Pn * 4, hipMemcpyDeviceToHost, st));
+                CP_HIP(hipStreamSynchronize(st));
+            }
+        }
+        CP_HIP(hipStreamSynchronize(st));
+
+        // --- assemble (:1289-1300, :1331-1337, :1414-1430)
+        write_initial(valid);
+        for (int j = 0; j < na; ++j) {
+            const int i = hact[j];
+            double Y = hy0[i], X = hx0[i], F = hf0[i];
+            for (int k = 0; k < p; ++k) {
+                double v = hth[(size_t)j * p + k], e = herr[(size_t)j * p + k];
+                if (gcs[k] == 1) { Y = v; if (cov_mode != CP_COV_NONE) out->y_err[i] = e; }
+                if (gcs[k] == 2) { X = v; if (cov_mode != CP_COV_NONE) out->x_err[i] = e; }
+                if (gcs[k] == 3) { F = v; if (cov_mode != CP_COV_NONE) out->flux_err[i] = e; }
+            }
+            out->y[i] = Y;
+            out->x[i] = X;
+            out->flux[i] = F;
+            out->converged[i] = hconv[j];
+            out->n_iter[i] = hlive[j] ? n_run : hfz[j];
+            if (out->chisq) out->chisq[i] = hchi[j];
+        }
+        for (int i = 0; i < n; ++i) {
+            if (fm.y_fixed) out->y[i] = o.fixed_y;
+            if (fm.x_fixed) out->x[i] = o.fixed_x;
+            if (fm.f_fixed) out->flux[i] = o.fixed_flux;
+            if (out->valid[i] && !(std::isfinite(out->y[i]) && std::isfinite(out->x[i]) &&
+                                   std::isfinite(out->flux[i]) && out->flux[i] > 0))
+                out->valid[i] = 0;
+            if (!out->valid[i] && out->chisq) out->chisq[i] = 0;
+        }
+        out->n_passes = n_run;
+        out->n_active = na;
+        out->n_union = U;
+        out->cost_final = cost;
+    }
+};
+
+void validate_sim(const cp_sim_opts& o, double fit_rad) {
+    CP_ARG(o.max_iter > 0, "max_iter must be positive");
+    CP_ARG(o.inner_iterations > 0, "inner_iterations must be positive");
+    CP_ARG(o.linear_tol > 0, "linear_tol must be positive");
+    CP_ARG(o.max_trials > 0, "max_trials must be positive");
+    double mrm = o.model_rad_max > 0 ? o.model_rad_max : 10 * fit_rad;
+    CP_ARG(mrm >= fit_rad, "model_rad_max must be >= fit_rad");
+    CP_ARG(o.model_rad_nsigma > 0, "model_rad_nsigma must be positive");
+    CP_ARG(o.solver == CP_SOLVER_LSQR, "solver must be LSQR in this build (CGLS runs in the oracle only)");
+    CP_ARG(fit_rad > 0 && std::isfinite(fit_rad), "fit_rad must be positive");
+    CP_ARG((o.free_mask & 7) != 0, "all model parameters are fixed; nothing to fit");
+    CP_ARG(o.precision == CP_PREC_MIXED || o.precision == CP_PREC_FP64, "unknown precision mode");
+}
+
+void fit_sim_device(cp_ctx* ctx, const cp_psf_desc* psfd, const float* d_img, const float* d_iv, int ny, int nx,
+                    const double* d_y0, const double* d_x0, const double* d_f0, int n, double fit_rad,
+                    const cp_sim_opts* po, cp_sim_result* out) {
+    CP_ARG(po && out, "NULL options or result");
+    CP_ARG(ny > 0 && nx > 0, "image must be non-empty");
+    CP_ARG(n >= 0, "negative star count");
+    const cp_sim_opts& o = *po;
+    validate_sim(o, fit_rad);
+    HostPsf P = make_host_psf(psfd);
+    CP_HIP(hipSetDevice(ctx->device));
+    DevPsf DP;
+    upload_psf(ctx, psfd, P, DP);
+    std::vector<double> hy(n), hx(n), hf(n);
+    if (n > 0) {
+        CP_HIP(hipMemcpyAsync(hy.data(), d_y0, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        CP_HIP(hipMemcpyAsync(hx.data(), d_x0, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        CP_HIP(hipMemcpyAsync(hf.data(), d_f0, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        CP_HIP(hipStreamSynchronize(ctx->stream));
+    }
+#define CP_RUN(TT, KK) Sim<TT, KK>(ctx, P, DP).run(d_img, d_iv, ny, nx, d_y0, d_x0, d_f0, n, fit_rad, o, out, hy, hx, hf)
+    const bool f64 = o.precision == CP_PREC_FP64;
+    switch (P.kind) {
+        case CP_PSF_CIRCULAR_GAUSSIAN:
+            if (f64) CP_RUN(double, KIND_GAUSS); else CP_RUN(float, KIND_GAUSS);
+            break;
+        case CP_PSF_CIRCULAR_MOFFAT:
+            if (f64) CP_RUN(double, KIND_MOFFAT); else CP_RUN(float, KIND_MOFFAT);
+            break;
+        default:
+            if (f64) CP_RUN(double, KIND_EPSF); else CP_RUN(float, KIND_EPSF);
+            break;
+    }
+#undef CP_RUN
+    CP_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+}  // namespace cp
+
+using namespace cp;
+
+extern "C" {
+
+void cp_sim_opts_default(cp_sim_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->solver = CP_SOLVER_LSQR;
+    o->inner_iterations = 10;
+    o->linear_tol = 1e-6;
+    o->model_rad = 0;
+    o->model_rad_max = 0;
+    o->model_rad_nsigma = 1.0;
+    o->max_step = 0.25;
+    o->max_trials = 8;
+    o->max_iter = 40;
+    o->x_tol = 1e-5;
+    o->f_tol = 1e-4;
+    o->g_tol = 1e-4;
+    o->lambda_init = 1e-4;
+    o->lambda_up = 10.0;
+    o->lambda_down = 10.0;
+    o->lambda_min = 1e-12;
+    o->lambda_max = 1e12;
+    o->covariance = CP_COV_DEFAULT;
+    o->free_mask = CP_FREE_Y | CP_FREE_X | CP_FREE_FLUX;
+    o->precision = CP_PREC_MIXED;
+    o->tile = 0;
+}
+
+int cp_fit_simultaneous_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_image, const float* d_inv_var,
+                               int32_t ny, int32_t nx, const double* d_y0, const double* d_x0,
+                               const double* d_flux0, int32_t n_stars, double fit_rad, const cp_sim_opts* opts,
+                               cp_sim_result* out) {
+    if (!ctx) return CP_ESTATE;
+    return guarded(ctx, [&] {
+        fit_sim_device(ctx, psf, d_image, d_inv_var, ny, nx, d_y0, d_x0, d_flux0, n_stars, fit_rad, opts, out);
+    });
+}
+
+int cp_fit_simultaneous(cp_ctx* ctx, const cp_psf_desc* psf, const float* image, const float* inv_var, int32_t ny,
+                        int32_t nx, const double* y0, const double* x0, const double* flux0, int32_t n_stars,
+                        double fit_rad, const cp_sim_opts* opts, cp_sim_result* out) {
+    if (!ctx) return CP_ESTATE;
+    return guarded(ctx, [&] {
+        CP_ARG(ny > 0 && nx > 0, "image must be non-empty");
+        CP_ARG(image != nullptr, "image is NULL");
+        CP_HIP(hipSetDevice(ctx->device));
+        const size_t P = (size_t)ny * nx;
+        DevBuf di(ctx, P * 4), dv, dy(ctx, (size_t)std::max(n_stars, 1) * 8), dx(ctx, (size_t)std::max(n_stars, 1) * 8),
+            df(ctx, (size_t)std::max(n_stars, 1) * 8);
+        CP_HIP(hipMemcpyAsync(di.get(), image, P * 4, hipMemcpyHostToDevice, ctx->stream));
+        if (inv_var) {
+            dv.alloc(ctx, P * 4);
+            CP_HIP(hipMemcpyAsync(dv.get(), inv_var, P * 4, hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (n_stars > 0) {
+            CP_HIP(hipMemcpyAsync(dy.get(), y0, (size_t)n_stars * 8, hipMemcpyHostToDevice, ctx->stream));
+            CP_HIP(hipMemcpyAsync(dx.get(), x0, (size_t)n_stars * 8, hipMemcpyHostToDevice, ctx->stream));
+            CP_HIP(hipMemcpyAsync(df.get(), flux0, (size_t)n_stars * 8, hipMemcpyHostToDevice, ctx->stream));
+        }
+        fit_sim_device(ctx, psf, di.as<float>(), inv_var ? dv.as<float>() : nullptr, ny, nx, dy.as<double>(),
+                       dx.as<double>(), df.as<double>(), n_stars, fit_rad, opts, out);
+    });
+}
+
+int cp_build_stamps(cp_ctx* ctx, const float* w_in, int32_t ny, int32_t nx, const double* y0, const double* x0,
+                    const double* flux0, int32_t n_stars, double fit_rad, int32_t* anchor_y, int32_t* anchor_x,
+                    int32_t* active, int32_t* n_active, int32_t* pixels, uint8_t* union_mask) {
+    if (!ctx) return CP_ESTATE;
+    return guarded(ctx, [&] {
+        CP_ARG(ny > 0 && nx > 0 && n_stars >= 0, "bad sizes");
+        CP_ARG(fit_rad > 0, "fit_rad must be positive");
+        CP_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const int64_t P = (int64_t)ny * nx;
+        const int R = (int)std::ceil(fit_rad), S = 2 * R + 1, S2 = S * S;
+        DevBuf w(ctx, P * 4);
+        if (w_in) {
+            CP_HIP(hipMemcpyAsync(w.get(), w_in, P * 4, hipMemcpyHostToDevice, st));
+        } else {
+            std::vector<float> ones(P, 1.0f);
+            CP_HIP(hipMemcpyAsync(w.get(), ones.data(), P * 4, hipMemcpyHostToDevice, st));
+            CP_HIP(hipStreamSynchronize(st));
+        }
+        const int n = std::max(n_stars, 1);
+        DevBuf dy(ctx, (size_t)n * 8), dx(ctx, (size_t)n * 8), df(ctx, (size_t)n * 8);
+        if (n_stars > 0) {
+            CP_HIP(hipMemcpyAsync(dy.get(), y0, (size_t)n_stars * 8, hipMemcpyHostToDevice, st));
+            CP_HIP(hipMemcpyAsync(dx.get(), x0, (size_t)n_stars * 8, hipMemcpyHostToDevice, st));
+            CP_HIP(hipMemcpyAsync(df.get(), flux0, (size_t)n_stars * 8, hipMemcpyHostToDevice, st));
+        }
+        DevBuf ay(ctx, (size_t)n * 4), ax(ctx, (size_t)n * 4), ok(ctx, n);
+        *n_active = 0;
+        if (n_stars > 0) {
+            launch(k_anchor<float>, dim3(cdiv(n_stars, BLOCK)), dim3(BLOCK), 0, st, dy.as<double>(), dx.as<double>(),
+                   df.as<double>(), n_stars, (const float*)w.as<float>(), ny, nx, R, ay.as<int>(), ax.as<int>(),
+                   ok.as<uint8_t>());
+            Cub cub{ctx, DevBuf()};
+            DevBuf act;
+            int coord_bits = bit_length((int64_t)std::max(ny, nx) + R + 1);
+            int na = build_active(ctx, cub, ay.as<int>(), ax.as<int>(), ok.as<uint8_t>(), n_stars, coord_bits, act);
+            *n_active = na;
+            CP_HIP(hipMemcpyAsync(anchor_y, ay.get(), (size_t)n_stars * 4, hipMemcpyDeviceToHost, st));
+            CP_HIP(hipMemcpyAsync(anchor_x, ax.get(), (size_t)n_stars * 4, hipMemcpyDeviceToHost, st));
+            if (na > 0) {
+                CP_HIP(hipMemcpyAsync(active, act.get(), (size_t)na * 4, hipMemcpyDeviceToHost, st));
+                DevBuf aay(ctx, (size_t)na * 4), aax(ctx, (size_t)na * 4), amr(ctx, (size_t)na * 4),
+                    th(ctx, (size_t)na * 8), lv(ctx, na), fz(ctx, (size_t)na * 4), cv(ctx, na);
+                launch(k_star_setup, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, (const int*)act.as<int>(), na,
+                       (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const double*)dy.as<double>(),
+                       (const double*)dx.as<double>(), (const double*)df.as<double>(), 1, 3, 0, 0, (const int*)nullptr,
+                       R, aay.as<int>(), aax.as<int>(), amr.as<int>(), th.as<double>(), lv.as<uint8_t>(),
+                       fz.as<int>(), cv.as<uint8_t>());
+                if (pixels) {
+                    DevBuf pix(ctx, (size_t)na * S2 * 4);
+                    launch(k_pixels<float>, dim3(cdiv((int64_t)na * S2, BLOCK)), dim3(BLOCK), 0, st, na,
+                           (const int*)aay.as<int>(), (const int*)aax.as<int>(), R, S, (const float*)w.as<float>(),
+                           ny, nx, pix.as<int>());
+                    CP_HIP(hipMemcpyAsync(pixels, pix.get(), (size_t)na * S2 * 4, hipMemcpyDeviceToHost, st));
+                    CP_HIP(hipStreamSynchronize(st));
+                }
+                if (union_mask) {
+                    DevBuf um(ctx, P);
+                    CP_HIP(hipMemsetAsync(um.get(), 0, P, st));
+                    launch(k_union<float>, dim3(cdiv((int64_t)na * S2, BLOCK)), dim3(BLOCK), 0, st, na,
+                           (const int*)aay.as<int>(), (const int*)aax.as<int>(), R, S, (const float*)w.as<float>(), ny,
+                           nx, um.as<uint8_t>());
+                    CP_HIP(hipMemcpyAsync(union_mask, um.get(), P, hipMemcpyDeviceToHost, st));
+                    CP_HIP(hipStreamSynchronize(st));
+                }
+            }
+            CP_HIP(hipStreamSynchronize(st));
+        }
+        if (union_mask && *n_active == 0) std::memset(union_mask, 0, P);
+    });
+}
+
+}  // extern "C"

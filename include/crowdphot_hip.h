@@ -111,6 +111,7 @@ typedef struct cp_sim_result {
     double *chisq;             /* may be NULL */
     uint8_t *converged, *valid;
     int32_t *n_iter;
+    uint8_t *active;           /* may be NULL: 1 where the star entered the solve */
     float *residual;           /* ny*nx column-major, may be NULL */
     int32_t n_passes, n_failed;
     /* run statistics */
@@ -118,7 +119,8 @@ typedef struct cp_sim_result {
     int64_t n_union;           /* |union_pix| */
     double cost_final;
     int32_t n_trials_total, n_inner_total;
-    double *trace;             /* optional: 6 doubles per trial (iter, trial, cost, cost_cand, lambda, n_live) */
+    double *trace;             /* optional: 8 doubles per trial (iter, trial, cost, cost_cand, lambda,
+                                  n_live, inner iterations, 0) */
     int32_t trace_cap, trace_len;
 } cp_sim_result;
 
@@ -155,6 +157,12 @@ const char* cp_last_error(const cp_ctx* ctx);
 /* The context's HIP stream (hipStream_t), for callers that order their own work on it. */
 void* cp_stream(cp_ctx* ctx);
 int cp_synchronize(cp_ctx* ctx);
+
+/* Instrumentation, no reference counterpart: per-kernel HIP-event timing on
+ * the context's stream.  cp_profile(ctx, 1) clears and starts it; the report
+ * is one "name launches total_ms" line per kernel. */
+int cp_profile(cp_ctx* ctx, int enable);
+int cp_profile_report(cp_ctx* ctx, char* buf, int32_t cap);
 
 void cp_sim_opts_default(cp_sim_opts* o);
 void cp_lm_opts_default(cp_lm_opts* o);

@@ -1023,6 +1023,8 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
                     active.data(), &na, pixels.data(), umask.data());
     std::vector<uint8_t> valid(n_stars, 0);
     for (int j = 0; j < na; ++j) valid[active[j]] = 1;
+    if (out->active)
+        for (int i = 0; i < n_stars; ++i) out->active[i] = valid[i];
     const int n_failed = n_stars - na;
     if (na == 0) {
         // _empty_simultaneous_result (:1433-1445)
@@ -1233,8 +1235,8 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
             if (out->trace && out->trace_len < out->trace_cap) {
                 int nl = 0;
                 for (int j = 0; j < na; ++j) nl += live[j];
-                double* t = out->trace + 6 * (size_t)out->trace_len++;
-                t[0] = iter; t[1] = trial; t[2] = cost; t[3] = cost_c; t[4] = lam; t[5] = nl;
+                double* t = out->trace + 8 * (size_t)out->trace_len++;
+                t[0] = iter; t[1] = trial; t[2] = cost; t[3] = cost_c; t[4] = lam; t[5] = nl; t[6] = its; t[7] = 0;
             }
             if (cost_c < cost) {
                 accepted = true;

@@ -32,14 +32,16 @@ def fit(psf_model, image, ys, xs, fs, fit_rad, inv_var=None, trace=0, **kw):
     r.converged = np.zeros(n, np.uint8)
     r.valid = np.zeros(n, np.uint8)
     r.n_iter = np.zeros(n, np.int32)
+    r.active = np.zeros(n, np.uint8)
     r.residual = np.zeros(ny * nx, np.float32)
-    r.trace = np.zeros(max(trace, 1) * 6)
+    r.trace = np.zeros(max(trace, 1) * 8)
     s = abi.SimResult()
     for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq"):
         setattr(s, k, ptr(getattr(r, k), abi.f64p))
     s.converged = ptr(r.converged, abi.u8p)
     s.valid = ptr(r.valid, abi.u8p)
     s.n_iter = ptr(r.n_iter, abi.i32p)
+    s.active = ptr(r.active, abi.u8p)
     s.residual = ptr(r.residual, abi.f32p)
     s.trace = ptr(r.trace, abi.f64p)
     s.trace_cap = trace
@@ -52,7 +54,7 @@ def fit(psf_model, image, ys, xs, fs, fit_rad, inv_var=None, trace=0, **kw):
         raise ValueError(lib.or_last_error().decode())
     r.n_passes, r.n_failed, r.n_active, r.n_union = s.n_passes, s.n_failed, s.n_active, s.n_union
     r.cost_final = s.cost_final
-    r.trace = r.trace[: 6 * s.trace_len].reshape(-1, 6)
+    r.trace = r.trace[: 8 * s.trace_len].reshape(-1, 8)
     r.residual = r.residual.reshape(nx, ny).T
     r.valid = r.valid.astype(bool)
     r.converged = r.converged.astype(bool)
