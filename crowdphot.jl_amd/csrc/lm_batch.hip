@@ -1,11 +1,11 @@
 // lm_batch.hip -- batched per-star Levenberg-Marquardt (fit_star over many
 // uneven cutouts) on gfx950.
 //
-// One wavefront owns one cutout for its whole fit: the cutout's pixels sit in
-// registers (<= 16 per lane, sides up to 31 px), each accum! pass evaluates the
-// model and its gradient per pixel in the storage precision T (float32 in the
-// mixed mode) and accumulates the normal equations A = J'WJ, b = J'Wr and the
-// cost in float64 with wave-shuffle reductions.  Every lane then runs the same
+// One wavefront owns one cutout for its whole fit: each accum! pass strides the
+// wave's lanes over the cutout (read once from HBM, then L1/L2-resident),
+// evaluates the model and its gradient per pixel in the storage precision T
+// (float32 in the mixed mode) and accumulates the normal equations A = J'WJ,
+// b = J'Wr and the cost in float64 with wave-shuffle reductions.  Every lane then runs the same
 // lm_irls step (float64 NP x NP Cholesky) so control flow stays uniform.
 // Waves pull cutouts from an integer work counter until it passes the end.
 //
@@ -18,8 +18,6 @@
 #include "psf_host.h"
 
 namespace cp {
-
-constexpr int LM_EPL = 16;  // pixels per lane (cutouts up to 1024 px)
 
 struct LMConst {
     int kind, npf;          // model kind, full parameter count
@@ -86,10 +84,11 @@ __device__ __forceinline__ double pick6(const double* v, int k) {
     return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : k == 3 ? v[3] : k == 4 ? v[4] : v[5];
 }
 
-// accum!: cost, b = J'W r, A = J'W J over the cutout held in registers.
+// accum!: cost, b = J'W r, A = J'W J over the cutout (lanes stride over its
+// pixels; the cutout stays L1/L2-resident across the passes of one fit).
 template <typename T, int KIND, int NP>
-__device__ void accum(const LMConst& C, const double* full, const T* d, const T* wt, const T* pi, const T* pj,
-                      int ne, int npx, NE<NP>& out) {
+__device__ __forceinline__ void accum(const LMConst& C, const double* full, const float* __restrict__ pix,
+                                      const float* __restrict__ iv, int ny, int npx, NE<NP>& out) {
     const int lane = threadIdx.x & 63;
     T prm[6];
 #pragma unroll
@@ -100,27 +99,28 @@ __device__ void accum(const LMConst& C, const double* full, const T* d, const T*
 #pragma unroll
     for (int i = 0; i < NP; ++i) acc.b[i] = 0;
     acc.cost = 0;
+    for (int m = lane; m < npx; m += 64) {
+        const int col = m / ny;
+        const T pi = T(m - col * ny + 1), pj = T(col + 1);
+        double w = 1.0;
+        if (C.have_iv) {
+            float wv = iv[m];
+            w = (isfinite(wv) && wv > 0.0f) ? (double)wv : 0.0;
+        }
+        T f, G[6];
+        model_fg<T, KIND>(prm, pi - prm[0], pj - prm[1], f, G);
+        const T r = f - (T)pix[m];
+        const double wr = w * (double)r;
+        acc.cost = fma(wr, (double)r, acc.cost);
+        const double G6[6] = {(double)G[0], (double)G[1], (double)G[2], (double)G[3], (double)G[4], (double)G[5]};
+        double g[NP];
 #pragma unroll
-    for (int e = 0; e < LM_EPL; ++e) {
-        if (e < ne && lane + 64 * e < npx) {
-            T f, G[6];
-            model_fg<T, KIND>(prm, pi[e] - prm[0], pj[e] - prm[1], f, G);
-            T r = f - d[e];
-            double w = C.have_iv ? (double)wt[e] : 1.0;
-            double wr = w * (double)r;
-            acc.cost = fma(wr, (double)r, acc.cost);
-            double g[NP];
+        for (int j = 0; j < NP; ++j) g[j] = pick6(G6, C.fidx[j]);
 #pragma unroll
-            for (int j = 0; j < NP; ++j) {
-                double G6[6] = {(double)G[0], (double)G[1], (double)G[2], (double)G[3], (double)G[4], (double)G[5]};
-                g[j] = pick6(G6, C.fidx[j]);
-            }
+        for (int j = 0; j < NP; ++j) {
+            acc.b[j] = fma(wr, g[j], acc.b[j]);
 #pragma unroll
-            for (int j = 0; j < NP; ++j) {
-                acc.b[j] = fma(wr, g[j], acc.b[j]);
-#pragma unroll
-                for (int i = 0; i <= j; ++i) acc.A[tri(i, j)] = fma(w * g[i], g[j], acc.A[tri(i, j)]);
-            }
+            for (int i = 0; i <= j; ++i) acc.A[tri(i, j)] = fma(w * g[i], g[j], acc.A[tri(i, j)]);
         }
     }
 #pragma unroll
@@ -271,35 +271,24 @@ __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, const float* __re
         if (c >= n) return;
         const int ny = cny[c], nx = cnx[c];
         const int npx = ny * nx;
-        const int ne = (npx + 63) / 64;
         const int64_t off = offsets[c];
         double full[6] = {0, 0, 0, 0, 0, 0};
-        for (int k = 0; k < C.npf; ++k) full[k] = theta0[(size_t)c * C.npf + k];
-        cp_lm_stat st{};
-        // input contract (_prepare_fit_star_inputs, psf_fitting.jl:36-101; lm_irls :451-472)
-        bool bad = npx > 64 * LM_EPL || npx - NP <= 0;
-        T d[LM_EPL], wt[LM_EPL], pi[LM_EPL], pj[LM_EPL];
-        int nvalid = 0;
 #pragma unroll
-        for (int e = 0; e < LM_EPL; ++e) {
-            const int m = lane + 64 * e;
-            d[e] = T(0);
-            wt[e] = T(0);
-            pi[e] = T(0);
-            pj[e] = T(0);
-            if (!bad && m < npx) {
-                d[e] = (T)pix[off + m];
-                pi[e] = T(m % ny + 1);
-                pj[e] = T(m / ny + 1);
-                if (C.have_iv) {
-                    float wv = iv[off + m];
-                    bool ok = isfinite(wv) && wv > 0.0f;
-                    wt[e] = ok ? (T)wv : T(0);
-                    nvalid += ok;
-                }
+        for (int k = 0; k < 6; ++k)
+            if (k < C.npf) full[k] = theta0[(size_t)c * C.npf + k];
+        cp_lm_stat st{};
+        const float* cpix = pix + off;
+        const float* civ = C.have_iv ? iv + off : nullptr;
+        // input contract (_prepare_fit_star_inputs, psf_fitting.jl:36-101; lm_irls :451-472)
+        bool bad = npx - NP <= 0;
+        if (C.have_iv && !bad) {
+            int nvalid = 0;
+            for (int m = lane; m < npx; m += 64) {
+                float wv = civ[m];
+                nvalid += (isfinite(wv) && wv > 0.0f) ? 1 : 0;
             }
+            bad = !(wave_sum_int(nvalid) > NP);
         }
-        if (C.have_iv) bad = bad || !(wave_sum_int(nvalid) > NP);
         if (bad) {
             if (lane == 0) {
                 st.status = CP_EARG;
@@ -327,7 +316,7 @@ __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, const float* __re
         NE<NP> cur, cand;
         double fx[6];
         to_full(x, fx);
-        accum<T, KIND, NP>(C, fx, d, wt, pi, pj, ne, npx, cur);
+        accum<T, KIND, NP>(C, fx, cpix, civ, ny, npx, cur);
         const double cost_init = cur.cost;
         double D[NP];
 #pragma unroll
@@ -374,7 +363,7 @@ __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, const float* __re
 #pragma unroll
             for (int i = 0; i < NP; ++i) xc[i] = x[i] + dlt[i];
             to_full(xc, fx);
-            accum<T, KIND, NP>(C, fx, d, wt, pi, pj, ne, npx, cand);
+            accum<T, KIND, NP>(C, fx, cpix, civ, ny, npx, cand);
             const bool accepted = cand.cost < cur.cost;
             const double dcost = cur.cost - cand.cost;
             const double ratio = prered > 0 ? dcost / prered : 0.0;

@@ -1,0 +1,206 @@
+"""
+    CrowdPhotAMD
+
+Julia front-end of libcrowdphot_hip.so (MI355X / gfx950).  It keeps the names,
+keyword arguments and result types of CrowdPhot.jl's fitting entry points and
+forwards the work to the HIP library through `ccall`:
+
+- `fit_all_stars_simultaneous`  (src/photometry/psf_photometry_simultaneous.jl:953)
+- `fit_star_batch`              (batched `fit_star`, src/psf/psf_fitting.jl:229)
+
+Depends on the standard library only (`Libdl`), so it loads without `Pkg`
+network access.  The PSF model types are CrowdPhot.jl's own
+(`CircularGaussianPSF`, `CircularMoffatPSF`, `GriddedPSFModel{T,<:ImagePSF}`);
+this module only reads their fields.
+"""
+module CrowdPhotAMD
+
+using Libdl
+
+const LIB = Ref{Ptr{Cvoid}}(C_NULL)
+libpath() = get(ENV, "CROWDPHOT_HIP_LIB", joinpath(@__DIR__, "..", "libcrowdphot_hip.so"))
+
+function __init__()
+    LIB[] = Libdl.dlopen(libpath())
+end
+
+sym(name) = Libdl.dlsym(LIB[], name)
+
+# ---- C structures (field for field with include/crowdphot_hip.h) ----------
+struct CpPsfDesc
+    kind::Int32
+    fwhm::Float64
+    alpha::Float64
+    beta::Float64
+    n_nodes::Int32
+    gny::Int32
+    gnx::Int32
+    grids::Ptr{Float32}
+    os_x::Int32
+    os_y::Int32
+    origin_y::Float64
+    origin_x::Float64
+    fill_value::Float64
+    node_y::Ptr{Float64}
+    node_x::Ptr{Float64}
+end
+
+mutable struct CpSimOpts
+    solver::Int32
+    inner_iterations::Int32
+    linear_tol::Float64
+    model_rad::Float64
+    model_rad_max::Float64
+    model_rad_nsigma::Float64
+    max_step::Float64
+    max_trials::Int32
+    max_iter::Int32
+    x_tol::Float64
+    f_tol::Float64
+    g_tol::Float64
+    lambda_init::Float64
+    lambda_up::Float64
+    lambda_down::Float64
+    lambda_min::Float64
+    lambda_max::Float64
+    covariance::Int32
+    free_mask::Int32
+    fixed_y::Float64
+    fixed_x::Float64
+    fixed_flux::Float64
+    fixed_bkg::Float64
+    precision::Int32
+    tile::Int32
+end
+
+mutable struct CpSimResult
+    y::Ptr{Float64}; x::Ptr{Float64}; flux::Ptr{Float64}; bkg::Ptr{Float64}
+    y_err::Ptr{Float64}; x_err::Ptr{Float64}; flux_err::Ptr{Float64}; bkg_err::Ptr{Float64}
+    chisq::Ptr{Float64}
+    converged::Ptr{UInt8}; valid::Ptr{UInt8}
+    n_iter::Ptr{Int32}
+    active::Ptr{UInt8}
+    residual::Ptr{Float32}
+    n_passes::Int32; n_failed::Int32
+    n_active::Int32
+    n_union::Int64
+    cost_final::Float64
+    n_trials_total::Int32; n_inner_total::Int32
+    trace::Ptr{Float64}
+    trace_cap::Int32; trace_len::Int32
+end
+
+mutable struct Context
+    handle::Ptr{Cvoid}
+end
+
+function Context(device::Integer = 0; hbm_budget::Integer = 0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall(sym(:cp_create), Cint, (Ref{Ptr{Cvoid}}, Cint, UInt64), h, device, hbm_budget)
+    rc == 0 || error("cp_create failed ($rc)")
+    ctx = Context(h[])
+    finalizer(c -> ccall(sym(:cp_destroy), Cvoid, (Ptr{Cvoid},), c.handle), ctx)
+    return ctx
+end
+
+function check(ctx::Context, rc)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall(sym(:cp_last_error), Cstring, (Ptr{Cvoid},), ctx.handle))
+    rc == -1 && throw(ArgumentError(msg))
+    error("libcrowdphot_hip error $rc: $msg")
+end
+
+# PSF descriptor of a CrowdPhot model (fields read by name; `fixed` shape
+# values override the model's own, as model_from_vector does).
+function psf_desc(psf, fixed, keep)
+    T = typeof(psf)
+    name = nameof(T)
+    g(k) = haskey(fixed, k) ? Float64(getfield(fixed, k)) : Float64(getproperty(psf, k))
+    if name === :CircularGaussianPSF
+        return CpPsfDesc(0, g(:fwhm), 0, 0, 0, 0, 0, C_NULL, 1, 1, 0, 0, 0, C_NULL, C_NULL)
+    elseif name === :CircularMoffatPSF
+        return CpPsfDesc(1, 0, g(:α), g(:β), 0, 0, 0, C_NULL, 1, 1, 0, 0, 0, C_NULL, C_NULL)
+    elseif name === :GriddedPSFModel || name === :ImagePSF
+        nodes = name === :ImagePSF ? [psf] : psf.psfs
+        gy = name === :ImagePSF ? [0.0] : Float64.(psf.grid_y)
+        gx = name === :ImagePSF ? [0.0] : Float64.(psf.grid_x)
+        n0 = nodes[1]
+        grids = reduce(vcat, [vec(Float32.(n.data)) for n in nodes])   # column-major per node
+        push!(keep, grids, gy, gx)
+        return CpPsfDesc(2, 0, 0, 0, length(nodes), size(n0.data, 1), size(n0.data, 2), pointer(grids),
+                         n0.oversampling[1], n0.oversampling[2], n0.origin.y, n0.origin.x, n0.fill_value,
+                         pointer(gy), pointer(gx))
+    end
+    throw(ArgumentError("model type $name is not built on the GPU"))
+end
+
+"""
+    fit_all_stars_simultaneous(image, psf, sources, fit_rad; kws...)
+
+Same contract, keywords and `MultiPassPhotResult` as CrowdPhot.jl
+(psf_photometry_simultaneous.jl:953-1431), computed on the GPU.  Extra keywords:
+`precision = :mixed | :fp64`, `ctx::Context`.  Returns a NamedTuple with the
+MultiPassPhotResult fields (construct the CrowdPhot type from it if needed).
+"""
+function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad::Real;
+        fixed::NamedTuple = (;), inv_var = nothing, solver::Symbol = :lsqr, inner_iterations::Int = 10,
+        linear_tol::Real = 1.0e-6, model_rad::Union{Symbol, Real} = :auto, model_rad_max::Real = 10 * fit_rad,
+        model_rad_nsigma::Real = 1.0, max_step::Real = 0.25, max_trials::Int = 8, max_iter::Integer = 40,
+        x_tol::Real = 1.0e-5, f_tol::Real = 1.0e-4, g_tol::Real = 1.0e-4, λ_init::Real = 1.0e-4,
+        λ_up::Real = 10.0, λ_down::Real = 10.0, λ_min::Real = 1.0e-12, λ_max::Real = 1.0e12,
+        precision::Symbol = :mixed, ctx::Context = Context(0), kws...)
+    solver in (:lsqr,) || throw(ArgumentError("solver must be :lsqr on the GPU, got $(repr(solver))"))
+    (model_rad === :auto || (model_rad isa Real && model_rad > 0)) ||
+        throw(ArgumentError("model_rad must be :auto or a positive real, got $(repr(model_rad))"))
+    props = propertynames_fit(psf)
+    free = Tuple(k for k in props if !haskey(fixed, k))
+    isempty(free) && throw(ArgumentError("all model parameters are fixed; nothing to fit"))
+    off = setdiff(free, (:y, :x, :flux))
+    isempty(off) || throw(ArgumentError("fit_all_stars_simultaneous fits only (y, x, flux) per star; got free parameters $(off)"))
+    mask = Int32((:y in free ? 1 : 0) | (:x in free ? 2 : 0) | (:flux in free ? 4 : 0))
+    y0 = Float64.(sources.y); x0 = Float64.(sources.x)
+    f0 = haskey(sources, :flux) ? Float64.(sources.flux) : ones(length(y0))
+    n = length(y0)
+    ny, nx = size(image)
+    img = Matrix{Float32}(image)                       # column-major already
+    iv = inv_var === nothing ? nothing : Matrix{Float32}(inv_var)
+    keep = Any[]
+    desc = psf_desc(psf, fixed, keep)
+    o = CpSimOpts(0, inner_iterations, linear_tol, model_rad === :auto ? 0.0 : Float64(model_rad),
+                  model_rad_max, model_rad_nsigma, max_step, max_trials, max_iter, x_tol, f_tol, g_tol,
+                  λ_init, λ_up, λ_down, λ_min, λ_max, 0, mask,
+                  Float64(get(fixed, :y, 0)), Float64(get(fixed, :x, 0)), Float64(get(fixed, :flux, 0)),
+                  Float64(get(fixed, :bkg, 0)), precision === :fp64 ? 1 : 0, 0)
+    out = (y = zeros(n), x = zeros(n), flux = zeros(n), bkg = zeros(n), y_err = zeros(n), x_err = zeros(n),
+           flux_err = zeros(n), bkg_err = zeros(n), chisq = zeros(n), converged = zeros(UInt8, n),
+           valid = zeros(UInt8, n), n_iter = zeros(Int32, n), active = zeros(UInt8, n),
+           residual = zeros(Float32, ny, nx))
+    r = CpSimResult(pointer(out.y), pointer(out.x), pointer(out.flux), pointer(out.bkg), pointer(out.y_err),
+                    pointer(out.x_err), pointer(out.flux_err), pointer(out.bkg_err), pointer(out.chisq),
+                    pointer(out.converged), pointer(out.valid), pointer(out.n_iter), pointer(out.active),
+                    pointer(out.residual), 0, 0, 0, 0, 0.0, 0, 0, C_NULL, 0, 0)
+    GC.@preserve keep out img iv begin
+        rc = ccall(sym(:cp_fit_simultaneous), Cint,
+                   (Ptr{Cvoid}, Ref{CpPsfDesc}, Ptr{Float32}, Ptr{Float32}, Int32, Int32, Ptr{Float64}, Ptr{Float64},
+                    Ptr{Float64}, Int32, Float64, Ref{CpSimOpts}, Ref{CpSimResult}),
+                   ctx.handle, desc, img, iv === nothing ? C_NULL : iv, ny, nx, y0, x0, f0, n, fit_rad, o, r)
+        check(ctx, rc)
+    end
+    nanv = fill(NaN, n)
+    msgs = ["star $i: excluded before the solve (no usable pixels or non-finite initial parameters)"
+            for i in 1:n if out.active[i] == 0]
+    return (y = out.y, x = out.x, y_err = out.y_err, x_err = out.x_err, flux = out.flux, flux_err = out.flux_err,
+            bkg = out.bkg, bkg_err = out.bkg_err, converged = BitVector(out.converged .!= 0),
+            valid = BitVector(out.valid .!= 0), chisq = out.chisq, qfit = nanv, qfit_expected = copy(nanv),
+            qfit_z = copy(nanv), crowding = copy(nanv), spread_model = copy(nanv), spread_model_err = copy(nanv),
+            n_iter = Int.(out.n_iter), n_passes = Int(r.n_passes), n_failed = Int(r.n_failed),
+            failure_msgs = msgs, residual = out.residual)
+end
+
+propertynames_fit(psf) = begin
+    name = nameof(typeof(psf))
+    name === :CircularGaussianPSF ? (:y, :x, :fwhm, :flux, :bkg) :
+    name === :CircularMoffatPSF ? (:y, :x, :α, :β, :flux, :bkg) : (:y, :x, :flux, :bkg)
+end
+
+end # module

@@ -105,18 +105,23 @@ def _compare(r_gpu, r_or, flux_rtol, pos_atol, frac=1.0):
 
 
 def test_isolated_noiseless_recovery_both_precisions(ctx):
+    """Reference fixture :219-253.  Pixels enter the library as float32, so the
+    float64 device run is compared with the oracle on the same float32-rounded
+    image (tight), and both with the truth at the float32 input limit."""
     rng = np.random.default_rng(42)
     shape = (128, 128)
     ys, xs = fixtures.separated_positions(rng, 5, shape, 10, 15)
     fs = rng.uniform(600, 900, 5)
-    img = fixtures.render_gauss(shape, ys, xs, fs, 2.0, 5)
+    img = fixtures.render_gauss(shape, ys, xs, fs, 2.0, 5).astype(np.float32).astype(np.float64)
     psf = pkg.CircularGaussianPSF(fwhm=2.0)
     cat = {"y": ys, "x": xs, "flux": np.full(5, 400.0)}
     kw = dict(fixed={"fwhm": 2.0, "bkg": 0.0}, max_iter=40, inner_iterations=10, x_tol=1e-8, model_rad=5)
+    ro = oracle_fit.fit(psf, img, ys, xs, cat["flux"], 5, model_rad=5.0, max_iter=40, x_tol=1e-8)
     r64 = pkg.fit_all_stars_simultaneous(img, psf, cat, 5, precision="fp64", ctx=ctx, **kw)
     assert r64.valid.all()
-    np.testing.assert_allclose(r64.flux, fs, rtol=1e-10)
-    np.testing.assert_allclose(r64.y, ys, atol=1e-10)
+    np.testing.assert_allclose(r64.flux, ro.flux, rtol=1e-10)
+    np.testing.assert_allclose(r64.y, ro.y, atol=1e-10)
+    np.testing.assert_allclose(r64.flux, fs, rtol=1e-6)
     r32 = pkg.fit_all_stars_simultaneous(img, psf, cat, 5, precision="mixed", ctx=ctx, **kw)
     assert r32.valid.all()
     np.testing.assert_allclose(r32.flux, fs, rtol=1e-5)
@@ -129,11 +134,13 @@ def test_isolated_noiseless_recovery_epsf(ctx):
     ys, xs = fixtures.separated_positions(rng, 5, shape, 10, 15)
     fs = rng.uniform(600, 900, 5)
     gpsf = pkg.GriddedPSFModel([pkg.moffat_epsf(fwhm=2.2, beta=2.5, size=41, oversampling=4)], [0.0], [0.0])
-    img = sim.render_sources(gpsf, shape, ys, xs, fs, 5.0)
+    img = sim.render_sources(gpsf, shape, ys, xs, fs, 5.0).astype(np.float32).astype(np.float64)
     cat = {"y": ys, "x": xs, "flux": np.full(5, 400.0)}
     kw = dict(fixed={"bkg": 0.0}, max_iter=40, x_tol=1e-8, model_rad=5)
+    ro = oracle_fit.fit(gpsf, img, ys, xs, cat["flux"], 5, model_rad=5.0, max_iter=40, x_tol=1e-8)
     r64 = pkg.fit_all_stars_simultaneous(img, gpsf, cat, 5, precision="fp64", ctx=ctx, **kw)
-    np.testing.assert_allclose(r64.flux, fs, rtol=1e-9)
+    np.testing.assert_allclose(r64.flux, ro.flux, rtol=1e-10)
+    np.testing.assert_allclose(r64.flux, fs, rtol=1e-6)
     r32 = pkg.fit_all_stars_simultaneous(img, gpsf, cat, 5, precision="mixed", ctx=ctx, **kw)
     np.testing.assert_allclose(r32.flux, fs, rtol=1e-5)
 
@@ -249,6 +256,12 @@ def test_fit_star_batch_moffat_parity(ctx):
     tho, sto = _oracle_batch(model, c, c["free_mask"])
     ok = np.array([s.status == 0 for s in sto])
     assert ok.all() and all(s.status == 0 for s in st)
+    its_g = np.array([s.iterations for s in st])
+    its_o = np.array([s.iterations for s in sto])
+    same0 = np.mean(np.all(th == c["theta0"], axis=1))
+    print(f"gpu iterations median {np.median(its_g)} max {its_g.max()}; oracle median {np.median(its_o)}; "
+          f"fraction unchanged from theta0 {same0:.3f}; converged gpu {np.mean([s.converged for s in st]):.3f}; "
+          f"first rows gpu {th[:2]} oracle {tho[:2]} theta0 {c['theta0'][:2]}")
     rel = np.abs(th[:, 4] - tho[:, 4]) / np.abs(tho[:, 4])
     dp = np.maximum(np.abs(th[:, 0] - tho[:, 0]), np.abs(th[:, 1] - tho[:, 1]))
     assert np.mean((rel <= FLUX_RTOL) & (dp <= POS_ATOL)) >= 0.99, (rel.max(), dp.max())
