@@ -33,6 +33,22 @@ struct ModelConst {
     T osx, osy, ox, oy, fill;
     const float* grids;    // device, n_nodes * gny * gnx, node-major, each column-major
     T bkg;                 // fixed background added by every star's model
+    int ios_y, ios_x;      // integer oversampling factors
+    int Qy, Qx;            // polyphase plane sides: ceil(gny / os_y), ceil(gnx / os_x)
+};
+
+// Per-star constants of the single-node ePSF fast path.  With integer
+// oversampling every pixel of a star shares the fractional grid offset, so the
+// Keys cubic-convolution weights (the expansion of cubic4,
+// empirical_models.jl:185-196) are fixed per star: value = sum_ab wy[a] wx[b] g,
+// d/dv uses dwy, d/du uses dwx.  ra/ca are the polyphase LDS addresses of the
+// four stencil rows / columns at the anchor pixel; a pixel at offset (dy, dx)
+// adds dy + dx * Qy.
+template <typename T>
+struct EpsfRec {
+    T wy[4], wx[4], dwy[4], dwx[4];
+    int ly, lx;      // floor(v), floor(u) at the anchor pixel (1-based grid coordinates)
+    int ra[4], ca[4];
 };
 
 // Per-star record used by the gather kernels (rebuilt whenever theta changes).
@@ -52,6 +68,114 @@ struct CornerRec {
     int idx[4];   // 0-based node ids; weight 0 slots point at node 0
     T w[4], dwdy[4], dwdx[4];
 };
+
+// Keys a = -1/2 weights of cubic4 (empirical_models.jl:185-196) and their t-derivatives.
+template <typename T>
+__host__ __device__ __forceinline__ void keys_weights(T t, T w[4], T dw[4]) {
+    const T t2 = t * t, t3 = t2 * t;
+    w[0] = T(-0.5) * t3 + t2 - T(0.5) * t;
+    w[1] = T(1.5) * t3 - T(2.5) * t2 + T(1);
+    w[2] = T(-1.5) * t3 + T(2) * t2 + T(0.5) * t;
+    w[3] = T(0.5) * t3 - T(0.5) * t2;
+    dw[0] = T(-1.5) * t2 + T(2) * t - T(0.5);
+    dw[1] = T(4.5) * t2 - T(5) * t;
+    dw[2] = T(-4.5) * t2 + T(4) * t + T(0.5);
+    dw[3] = T(1.5) * t2 - t;
+}
+
+// Polyphase address of 0-based grid sample (r, c): plane (r mod osy, c mod osx),
+// column-major inside the plane.  Consecutive pixels of one star then read
+// consecutive LDS words.
+__host__ __device__ __forceinline__ int poly_addr(int r, int c, int osy, int osx, int Qy, int Qx) {
+    return ((r % osy) * osx + (c % osx)) * (Qy * Qx) + (c / osx) * Qy + (r / osy);
+}
+
+// Stage one node grid (global, column-major) into LDS in the polyphase layout.
+__device__ __forceinline__ void stage_poly(const float* __restrict__ g, int gny, int gnx, int osy, int osx,
+                                           int Qy, int Qx, float* __restrict__ sg, int tid, int nthr) {
+    for (int i = tid; i < gny * gnx; i += nthr) {
+        const int c = i / gny, r = i - c * gny;
+        sg[poly_addr(r, c, osy, osx, Qy, Qx)] = g[i];
+    }
+}
+
+// Build the per-star fast-path constants from its fractional position.
+template <typename T>
+__device__ __forceinline__ void epsf_rec(const ModelConst<T>& M, double fy, double fx, EpsfRec<T>& e) {
+    double v0 = (double)M.ios_y * (0.0 - fy) + (double)M.oy;   // v at the anchor pixel
+    double u0 = (double)M.ios_x * (0.0 - fx) + (double)M.ox;
+    if (!(fabs(v0) < 1e8 && fabs(u0) < 1e8)) {  // non-finite or absurd position: every pixel off-grid
+        v0 = -1e8;
+        u0 = -1e8;
+    }
+    const double fly = floor(v0), flx = floor(u0);
+    e.ly = (int)fly;
+    e.lx = (int)flx;
+    T wd[4];
+    keys_weights<T>((T)(v0 - fly), e.wy, wd);
+    for (int a = 0; a < 4; ++a) e.dwy[a] = wd[a];
+    keys_weights<T>((T)(u0 - flx), e.wx, wd);
+    for (int a = 0; a < 4; ++a) e.dwx[a] = wd[a];
+    const int QQ = M.Qy * M.Qx;
+    for (int a = 0; a < 4; ++a) {
+        // 0-based row of stencil entry a is ly - 2 + a; floor division keeps
+        // the plane arithmetic exact for negative rows (used only off the fast path)
+        int r = e.ly - 2 + a, c = e.lx - 2 + a;
+        int pr = ((r % M.ios_y) + M.ios_y) % M.ios_y, qr = (r - pr) / M.ios_y;
+        int pc = ((c % M.ios_x) + M.ios_x) % M.ios_x, qc = (c - pc) / M.ios_x;
+        e.ra[a] = pr * M.ios_x * QQ + qr;
+        e.ca[a] = pc * QQ + qc * M.Qy;
+    }
+}
+
+// Grid sample with the reference's stencil clamping, polyphase LDS layout.
+__device__ __forceinline__ float poly_sample(const float* __restrict__ sg, int r, int c, int gny, int gnx, int osy,
+                                             int osx, int Qy, int Qx) {
+    r = min(max(r, 0), gny - 1);
+    c = min(max(c, 0), gnx - 1);
+    return sg[poly_addr(r, c, osy, osx, Qy, Qx)];
+}
+
+// ePSF value (no flux/bkg) of a star at pixel offset (dy, dx), fast path.
+template <typename T, bool DERIV>
+__device__ __forceinline__ void epsf_eval(const ModelConst<T>& M, const EpsfRec<T>& e, const float* __restrict__ sg,
+                                          int dy, int dx, T& val, T& dpdv, T& dpdu) {
+    const int ly = e.ly + M.ios_y * dy, lx = e.lx + M.ios_x * dx;
+    // closed-bounds test of bicubic_interpolate: floor(v) in [1, gny-1], or v == gny exactly
+    const bool inb = ly >= 1 && lx >= 1 && (ly < M.gny || (ly == M.gny && e.wy[1] == T(1))) &&
+                     (lx < M.gnx || (lx == M.gnx && e.wx[1] == T(1)));
+    if (!inb) {
+        val = M.fill;
+        dpdv = dpdu = T(0);
+        return;
+    }
+    T row[4], drow[4];
+    if (ly >= 2 && ly <= M.gny - 2 && lx >= 2 && lx <= M.gnx - 2) {
+        const int off = dy + dx * M.Qy;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const float* rp = sg + e.ra[a] + off;
+            const T g0 = (T)rp[e.ca[0]], g1 = (T)rp[e.ca[1]], g2 = (T)rp[e.ca[2]], g3 = (T)rp[e.ca[3]];
+            row[a] = e.wx[0] * g0 + e.wx[1] * g1 + e.wx[2] * g2 + e.wx[3] * g3;
+            if (DERIV) drow[a] = e.dwx[0] * g0 + e.dwx[1] * g1 + e.dwx[2] * g2 + e.dwx[3] * g3;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            T g[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                g[b] = (T)poly_sample(sg, ly - 2 + a, lx - 2 + b, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx);
+            row[a] = e.wx[0] * g[0] + e.wx[1] * g[1] + e.wx[2] * g[2] + e.wx[3] * g[3];
+            if (DERIV) drow[a] = e.dwx[0] * g[0] + e.dwx[1] * g[1] + e.dwx[2] * g[2] + e.dwx[3] * g[3];
+        }
+    }
+    val = e.wy[0] * row[0] + e.wy[1] * row[1] + e.wy[2] * row[2] + e.wy[3] * row[3];
+    if (DERIV) {
+        dpdv = e.dwy[0] * row[0] + e.dwy[1] * row[1] + e.dwy[2] * row[2] + e.dwy[3] * row[3];
+        dpdu = e.wy[0] * drow[0] + e.wy[1] * drow[1] + e.wy[2] * drow[2] + e.wy[3] * drow[3];
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ void cubic4(T f1, T f2, T f3, T f4, T t, T& val, T& der) {

@@ -76,6 +76,7 @@ inline HostPsf make_host_psf(const cp_psf_desc* d) {
     CP_ARG(d->n_nodes >= 1, "`psfs` must be non-empty");
     CP_ARG(d->gny >= 4 && d->gnx >= 4, "each axis of `data` must have at least 4 samples");
     CP_ARG(d->grids != nullptr, "ePSF grids pointer is NULL");
+    CP_ARG(d->os_x >= 1 && d->os_y >= 1, "oversampling must be a positive integer");
     P.n_nodes = d->n_nodes;
     P.gny = d->gny;
     P.gnx = d->gnx;
@@ -159,6 +160,10 @@ ModelConst<T> model_const(const HostPsf& P, const DevPsf& D, double bkg) {
         M.oy = (T)P.oy;
         M.fill = (T)P.fill;
         M.grids = D.grids.as<float>();
+        M.ios_y = (int)P.osy;
+        M.ios_x = (int)P.osx;
+        M.Qy = (P.gny + M.ios_y - 1) / M.ios_y;
+        M.Qx = (P.gnx + M.ios_x - 1) / M.ios_x;
     }
     M.bkg = (T)bkg;
     return M;

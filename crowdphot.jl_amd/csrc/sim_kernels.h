@@ -404,14 +404,15 @@ __device__ void grid_corners(const NodeGrid& G, int n_nodes, double Y, double X,
     }
 }
 
-// StarRec from theta.  frozen_iter >= 1 marks stars frozen at that iteration.
+// StarRec (and the single-node ePSF constants) from theta.  frozen_iter >= 1
+// marks stars frozen at that iteration.
 template <typename T, int KIND>
 static __global__ void k_star_rec(int na, const double* __restrict__ theta, FreeMap fm, ModelConst<T> M,
-                           double inv_norm_g64, double moffat_amp64, const int* __restrict__ ay,
-                           const int* __restrict__ ax, const int* __restrict__ mR,
-                           const uint8_t* __restrict__ live, const int* __restrict__ frozen_at,
-                           int frozen_iter, int all_live, StarRec<T>* __restrict__ recs, NodeGrid G,
-                           CornerRec<T>* __restrict__ crs) {
+                                  double inv_norm_g64, double moffat_amp64, const int* __restrict__ ay,
+                                  const int* __restrict__ ax, const int* __restrict__ mR,
+                                  const uint8_t* __restrict__ live, const int* __restrict__ frozen_at,
+                                  int frozen_iter, int all_live, StarRec<T>* __restrict__ recs, NodeGrid G,
+                                  CornerRec<T>* __restrict__ crs, EpsfRec<T>* __restrict__ ers) {
     int a = blockIdx.x * BLOCK + threadIdx.x;
     if (a >= na) return;
     double Y, X, F;
@@ -419,8 +420,9 @@ static __global__ void k_star_rec(int na, const double* __restrict__ theta, Free
     StarRec<T> r;
     r.ay = ay[a];
     r.ax = ax[a];
-    r.fy = (T)(Y - r.ay);
-    r.fx = (T)(X - r.ax);
+    const double fy = Y - r.ay, fx = X - r.ax;
+    r.fy = (T)fy;
+    r.fx = (T)fx;
     r.flux = (T)F;
     if (KIND == KIND_GAUSS) r.amp = (T)(F * inv_norm_g64);
     else if (KIND == KIND_MOFFAT) r.amp = (T)(F * moffat_amp64);
@@ -430,7 +432,10 @@ static __global__ void k_star_rec(int na, const double* __restrict__ theta, Free
     if (frozen_iter > 0 && frozen_at[a] == frozen_iter && !live[a]) fl |= 2;
     r.flags = fl;
     recs[a] = r;
-    if (KIND == KIND_EPSF && M.n_nodes > 1) grid_corners<T>(G, M.n_nodes, Y, X, crs[a]);
+    if (KIND == KIND_EPSF) {
+        if (M.n_nodes > 1) grid_corners<T>(G, M.n_nodes, Y, X, crs[a]);
+        else if (ers) epsf_rec<T>(M, fy, fx, ers[a]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -438,22 +443,36 @@ static __global__ void k_star_rec(int na, const double* __restrict__ theta, Free
 // (_render_model! :545-673, _residual_cost! :675-684, _cost! :686-693,
 //  _freeze_star! :739-754, final render :1344-1356)
 // ---------------------------------------------------------------------------
+constexpr int RCHUNK = 64;  // star records staged per LDS chunk in the render
+
+// Model value of star j (staged in LDS) at pixel offset (dy, dx).
+template <typename T, int KIND, bool LDSG>
+__device__ __forceinline__ T staged_value(const ModelConst<T>& M, const StarRec<T>& r, const EpsfRec<T>* er,
+                                          int dy, int dx, const float* grid, const CornerRec<T>* cr) {
+    if constexpr (KIND == KIND_EPSF && LDSG) {
+        T p, d0, d1;
+        epsf_eval<T, false>(M, *er, grid, dy, dx, p, d0, d1);
+        return fma(r.flux, p, M.bkg);
+    } else {
+        return star_value<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, cr);
+    }
+}
+
 template <typename T, int KIND, int MODE, bool LDSG>
 __global__ __launch_bounds__(BLOCK) void k_render(
-    ModelConst<T> M, const StarRec<T>* __restrict__ recs, const CornerRec<T>* __restrict__ crs,
-    const int* __restrict__ tstart, const int* __restrict__ tend, const int* __restrict__ list, TileGeo g,
-    int ntiles, int tiles_per_block, const uint8_t* __restrict__ umask, T* __restrict__ dw,
-    const T* __restrict__ w, const T* __restrict__ data, T* __restrict__ wt_resid, T* __restrict__ resid,
-    double* __restrict__ partials, Scal* S) {
+    ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
+    const CornerRec<T>* __restrict__ crs, const int* __restrict__ tstart, const int* __restrict__ tend,
+    const int* __restrict__ list, TileGeo g, int ntiles, int tiles_per_block, const uint8_t* __restrict__ umask,
+    T* __restrict__ dw, const T* __restrict__ w, const T* __restrict__ data, T* __restrict__ wt_resid,
+    T* __restrict__ resid, double* __restrict__ partials, Scal* S) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[NWAVE];
-    const int ngrid = LDSG ? M.gny * M.gnx : 0;
+    const int ngrid = LDSG ? M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
     StarRec<T>* srec = reinterpret_cast<StarRec<T>*>(smem + ((ngrid * 4 + 15) & ~15));
-    int* sid = reinterpret_cast<int*>(srec + CHUNK);
-    if (LDSG) {
-        for (int i = threadIdx.x; i < ngrid; i += BLOCK) sgrid[i] = M.grids[i];
-    }
+    EpsfRec<T>* serec = reinterpret_cast<EpsfRec<T>*>(srec + RCHUNK);
+    int* sid = reinterpret_cast<int*>(serec + ((KIND == KIND_EPSF && LDSG) ? RCHUNK : 0));
+    if (LDSG) stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, BLOCK);
     const float* grid = LDSG ? sgrid : M.grids;
     double acc = 0;
     const int t0 = blockIdx.x * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
@@ -476,12 +495,13 @@ __global__ __launch_bounds__(BLOCK) void k_render(
         }
         const int s0 = tstart[tile], s1 = tend[tile];
         bool touched = false;
-        for (int c0 = s0; c0 < s1; c0 += CHUNK) {
-            const int cn = min(CHUNK, s1 - c0);
+        for (int c0 = s0; c0 < s1; c0 += RCHUNK) {
+            const int cn = min(RCHUNK, s1 - c0);
             __syncthreads();
             for (int i = threadIdx.x; i < cn; i += BLOCK) {
                 int a = list[c0 + i];
                 srec[i] = recs[a];
+                if (KIND == KIND_EPSF && LDSG) serec[i] = ers[a];
                 sid[i] = a;
             }
             __syncthreads();
@@ -493,7 +513,7 @@ __global__ __launch_bounds__(BLOCK) void k_render(
                 for (int k = 0; k < PPT; ++k) {
                     int dy = gyv[k] - r.ay, dx = gxv[k] - r.ax;
                     if (act[k] && abs(dy) <= r.mR && abs(dx) <= r.mR) {
-                        T v = star_value<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + sid[j]);
+                        T v = staged_value<T, KIND, LDSG>(M, r, serec + j, dy, dx, grid, crs + sid[j]);
                         if (MODE == MODE_FREEZE) {
                             val[k] -= v;
                             touched = true;
@@ -539,35 +559,46 @@ __global__ __launch_bounds__(BLOCK) void k_render(
 // fill + J^T r (one wave per star; _fill_stamps! :342-543, apply_JT! :55-81,
 // g-cosine :1195-1206 and _g_tol_local :789-801)
 // ---------------------------------------------------------------------------
-template <typename T, int KIND, bool FINAL, bool LDSG, int EPL>
+// GL lanes per star (16 or 64), EPL stamp entries per lane.
+template <int GL>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = GL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T, int KIND, bool FINAL, bool LDSG, int EPL, int GL>
 __global__ __launch_bounds__(BLOCK) void k_fill(
-    ModelConst<T> M, const StarRec<T>* __restrict__ recs, const CornerRec<T>* __restrict__ crs, int na,
-    int R, int S, FreeMap fm, const T* __restrict__ w, int ny, int nx, const T* __restrict__ wt_resid,
-    T* __restrict__ V, double* __restrict__ colnorm, double* __restrict__ b_scaled,
-    uint8_t* __restrict__ gloc, double g_tol, double* __restrict__ partials, Scal* Sc) {
+    ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
+    const CornerRec<T>* __restrict__ crs, int na, int R, int S, FreeMap fm, const T* __restrict__ w, int ny,
+    int nx, const T* __restrict__ wt_resid, T* __restrict__ V, double* __restrict__ colnorm,
+    double* __restrict__ b_scaled, uint8_t* __restrict__ gloc, double g_tol, double* __restrict__ partials,
+    Scal* Sc) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[NWAVE];
-    const int ngrid = LDSG ? M.gny * M.gnx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
     if (LDSG) {
-        for (int i = threadIdx.x; i < ngrid; i += BLOCK) sgrid[i] = M.grids[i];
+        stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, BLOCK);
         __syncthreads();
     }
     const float* grid = LDSG ? sgrid : M.grids;
-    const int lane = threadIdx.x & 63;
+    constexpr int GPB = BLOCK / GL;   // stars in flight per block
+    const int sub = threadIdx.x % GL;
     const int S2 = S * S;
     const int p = fm.p;
     const double C_r = FINAL ? 0.0 : Sc->cost / Sc->dof;
     double bsq = 0, gmax = 0;
-    for (int a = blockIdx.x * NWAVE + (threadIdx.x >> 6); a < na; a += gridDim.x * NWAVE) {
+    for (int a = blockIdx.x * GPB + threadIdx.x / GL; a < na; a += gridDim.x * GPB) {
         const StarRec<T> r = recs[a];
-        if (!FINAL && !(r.flags & 1)) continue;
+        if (!FINAL && !(r.flags & 1)) continue;   // uniform within the group
+        EpsfRec<T> er;
+        if (KIND == KIND_EPSF && LDSG) er = ers[a];
         T raw[3][EPL];
         T wres[EPL];
         double cn2[3] = {0, 0, 0};
 #pragma unroll
         for (int e = 0; e < EPL; ++e) {
-            const int m = lane + 64 * e;
+            const int m = sub + GL * e;
             raw[0][e] = raw[1][e] = raw[2][e] = T(0);
             wres[e] = T(0);
             const int dy = (m % S) - R, dx = (m / S) - R;
@@ -577,8 +608,16 @@ __global__ __launch_bounds__(BLOCK) void k_fill(
             const T wv = on ? w[q] : T(0);
             if (!(wv > T(0))) continue;
             T g3[3];
-            star_grad<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a, g3[0], g3[1], g3[2]);
-            T sw = sqrt(wv);
+            if constexpr (KIND == KIND_EPSF && LDSG) {
+                T pv, pdv, pdu;
+                epsf_eval<T, true>(M, er, grid, dy, dx, pv, pdv, pdu);
+                g3[0] = r.flux * (-(M.osy * pdv));
+                g3[1] = r.flux * (-(M.osx * pdu));
+                g3[2] = pv;
+            } else {
+                star_grad<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a, g3[0], g3[1], g3[2]);
+            }
+            const T sw = sqrt(wv);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 if (k < p) {
@@ -595,23 +634,23 @@ __global__ __launch_bounds__(BLOCK) void k_fill(
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (k < p) {
-                double c = sqrt(wave_sum(cn2[k]));
+                double c = sqrt(group_sum<GL>(cn2[k]));
                 cn[k] = c < DEPS ? DEPS : c;
                 double inv = 1.0 / cn[k];
                 double b = 0;
 #pragma unroll
                 for (int e = 0; e < EPL; ++e) {
-                    const int m = lane + 64 * e;
+                    const int m = sub + GL * e;
                     if (m < S2) {
                         T v = (T)((double)raw[k][e] * inv);
                         V[((size_t)a * S2 + m) * p + k] = v;
                         b += (double)v * (double)wres[e];
                     }
                 }
-                bk[k] = FINAL ? 0.0 : wave_sum(b);
+                bk[k] = FINAL ? 0.0 : group_sum<GL>(b);
             }
         }
-        if (lane == 0) {
+        if (sub == 0) {
             double gn = 0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -632,7 +671,9 @@ __global__ __launch_bounds__(BLOCK) void k_fill(
         }
     }
     if (!FINAL) {
-        if (lane == 0 && gmax > 0) atomicMax(&Sc->gmax_bits, (unsigned long long)__double_as_longlong(gmax));
+        gmax = wave_max(gmax);
+        if ((threadIdx.x & 63) == 0 && gmax > 0)
+            atomicMax(&Sc->gmax_bits, (unsigned long long)__double_as_longlong(gmax));
         double b = block_sum(bsq, sred);
         if (publish_last(b, partials, &Sc->counters[1], gridDim.x)) {
             double tot = sum_partials(partials, gridDim.x, sred);
@@ -669,17 +710,24 @@ static __global__ void k_lsqr_init(Scal* S, double lam, int itmax, double atol, 
 }
 
 // u <- J v - alpha u over the tile's fit list (apply_J! :95-133 as a gather).
+// The V blocks (p x S^2 values) of a chunk of CJ stars are copied into LDS
+// with coalesced loads, then every pixel sums its covering stars from LDS.
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_Jv(
     const int* __restrict__ tstart, const int* __restrict__ tend, const int* __restrict__ list, TileGeo g,
     int ntiles, int tiles_per_block, const int* __restrict__ ay, const int* __restrict__ ax,
-    const uint8_t* __restrict__ live, int R, int S, int p, const T* __restrict__ V,
+    const uint8_t* __restrict__ live, int R, int S, int p, int CJ, const T* __restrict__ V,
     const double* __restrict__ tsrc, const T* __restrict__ usrc, T* __restrict__ ubuf,
     const uint8_t* __restrict__ umask, int first, double* __restrict__ partials, Scal* Sc) {
     if (Sc->done) return;
+    extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[NWAVE];
-    __shared__ int sa[CHUNK], say[CHUNK], sax[CHUNK];
-    __shared__ double sv[CHUNK][3];
+    const int S2 = S * S, SP = S2 * p;
+    T* sV = reinterpret_cast<T*>(smem);
+    double* sv = reinterpret_cast<double*>(smem + (((size_t)CJ * SP * sizeof(T) + 15) & ~(size_t)15));
+    int* sa = reinterpret_cast<int*>(sv + 3 * CJ);
+    int* say = sa + CJ;
+    int* sax = say + CJ;
     const double alpha = Sc->alpha;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
     double tfac, ufac;
@@ -690,7 +738,6 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(
         tfac = ia;
         ufac = Sc->beta_cur != 0 ? 1.0 / Sc->beta_cur : 1.0;
     }
-    const int S2 = S * S;
     double acc2 = 0;
     const int t0 = blockIdx.x * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
     for (int tile = t0; tile < t1; ++tile) {
@@ -709,8 +756,8 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(
             yv[k] = 0;
         }
         const int s0 = tstart[tile], s1 = tend[tile];
-        for (int c0 = s0; c0 < s1; c0 += CHUNK) {
-            const int cn = min(CHUNK, s1 - c0);
+        for (int c0 = s0; c0 < s1; c0 += CJ) {
+            const int cn = min(CJ, s1 - c0);
             __syncthreads();
             for (int i = threadIdx.x; i < cn; i += BLOCK) {
                 int a = list[c0 + i];
@@ -718,20 +765,27 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(
                 sa[i] = lv ? a : -1;
                 say[i] = ay[a];
                 sax[i] = ax[a];
-                for (int k = 0; k < 3; ++k) sv[i][k] = (lv && k < p) ? tsrc[(size_t)a * p + k] * tfac : 0.0;
+                for (int k = 0; k < 3; ++k) sv[3 * i + k] = (lv && k < p) ? tsrc[(size_t)a * p + k] * tfac : 0.0;
+            }
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < cn * SP; idx += BLOCK) {
+                const int j = idx / SP, o = idx - j * SP;
+                const int a = sa[j];
+                if (a >= 0) sV[idx] = V[(size_t)a * SP + o];
             }
             __syncthreads();
             for (int j = 0; j < cn; ++j) {
-                const int a = sa[j];
-                if (a < 0) continue;
+                if (sa[j] < 0) continue;
                 const int sy = say[j], sx = sax[j];
+                const double v0 = sv[3 * j], v1 = sv[3 * j + 1], v2 = sv[3 * j + 2];
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
                     int dy = gyv[k] - sy, dx = gxv[k] - sx;
                     if (act[k] && abs(dy) <= R && abs(dx) <= R) {
-                        const T* Vm = V + ((size_t)a * S2 + (size_t)(dx + R) * S + (dy + R)) * p;
-                        double s = 0;
-                        for (int kk = 0; kk < p; ++kk) s += (double)Vm[kk] * sv[j][kk];
+                        const T* Vm = sV + j * SP + ((dx + R) * S + (dy + R)) * p;
+                        double s = (double)Vm[0] * v0;
+                        if (p > 1) s += (double)Vm[1] * v1;
+                        if (p > 2) s += (double)Vm[2] * v2;
                         yv[k] += s;
                     }
                 }
@@ -754,8 +808,9 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(
 }
 
 // v <- J^T u - beta v per star, with the pending x/w update of the previous
-// iteration folded in; the last block advances the LSQR recurrences.
-template <typename T>
+// iteration folded in; the last block advances the LSQR recurrences.  GL
+// lanes per star (GL = 16 for stamps up to 16 x 16, else 64).
+template <typename T, int GL>
 __global__ __launch_bounds__(BLOCK) void k_JTu(
     int na, const int* __restrict__ ay, const int* __restrict__ ax, const uint8_t* __restrict__ live,
     int R, int S, int p, int ny, int nx, const T* __restrict__ V, const T* __restrict__ ubuf,
@@ -769,48 +824,48 @@ __global__ __launch_bounds__(BLOCK) void k_JTu(
     const double ib = beta != 0 ? 1.0 / beta : 1.0;
     const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau;
     const double ib1 = first ? 1.0 / Sc->beta1 : 0.0;
-    const int lane = threadIdx.x & 63;
+    constexpr int GPB = BLOCK / GL;
+    const int sub = threadIdx.x % GL;
     const int S2 = S * S;
     double tsq = 0, wsq = 0;
-    for (int a = blockIdx.x * NWAVE + (threadIdx.x >> 6); a < na; a += gridDim.x * NWAVE) {
+    for (int a = blockIdx.x * GPB + threadIdx.x / GL; a < na; a += gridDim.x * GPB) {
         const size_t base = (size_t)a * p;
         if (!live[a]) {
-            if (first && lane < p) {
-                x[base + lane] = 0;
-                wv[base + lane] = 0;
-                tbuf[base + lane] = 0;
+            if (first && sub < p) {
+                x[base + sub] = 0;
+                wv[base + sub] = 0;
+                tbuf[base + sub] = 0;
             }
             continue;
         }
         const int sy = ay[a], sx = ax[a];
-        double acc[3] = {0, 0, 0};
-        for (int m = lane; m < S2; m += 64) {
+        double acc0 = 0, acc1 = 0, acc2 = 0;
+        for (int m = sub; m < S2; m += GL) {
             int gy = sy + (m % S) - R, gx = sx + (m / S) - R;
             if (gy < 1 || gy > ny || gx < 1 || gx > nx) continue;
-            double u = (double)ubuf[(size_t)(gy - 1) + (size_t)(gx - 1) * ny];
+            const double u = (double)ubuf[(size_t)(gy - 1) + (size_t)(gx - 1) * ny];
             const T* Vm = V + ((size_t)a * S2 + m) * p;
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                if (k < p) acc[k] += (double)Vm[k] * u;
+            acc0 += (double)Vm[0] * u;
+            if (p > 1) acc1 += (double)Vm[1] * u;
+            if (p > 2) acc2 += (double)Vm[2] * u;
         }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) acc[k] = wave_sum(acc[k]);
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (k >= p) continue;
-                double told = first ? -b_scaled[base + k] * ib1 : tbuf[base + k];
-                double v = told * ia;
-                double xo = first ? 0.0 : x[base + k];
-                double wo = first ? 0.0 : wv[base + k];
-                x[base + k] = xo + sig * wo;
-                double wn = v - tau * wo;
-                wv[base + k] = wn;
-                wsq += wn * wn;
-                double tn = beta != 0 ? acc[k] * ib - beta * v : told;
-                tbuf[base + k] = tn;
-                tsq += tn * tn;
-            }
+        acc0 = group_sum<GL>(acc0);
+        acc1 = group_sum<GL>(acc1);
+        acc2 = group_sum<GL>(acc2);
+        if (sub < p) {
+            const int k = sub;
+            const double acck = k == 0 ? acc0 : (k == 1 ? acc1 : acc2);
+            const double told = first ? -b_scaled[base + k] * ib1 : tbuf[base + k];
+            const double v = told * ia;
+            const double xo = first ? 0.0 : x[base + k];
+            const double wo = first ? 0.0 : wv[base + k];
+            x[base + k] = xo + sig * wo;
+            const double wn = v - tau * wo;
+            wv[base + k] = wn;
+            wsq += wn * wn;
+            const double tn = beta != 0 ? acck * ib - beta * v : told;
+            tbuf[base + k] = tn;
+            tsq += tn * tn;
         }
     }
     double bt = block_sum(tsq, sred);

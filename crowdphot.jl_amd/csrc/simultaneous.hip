@@ -159,12 +159,6 @@ static std::vector<int> auto_radii(const HostPsf& P, double nsigma, int R_fit, i
     return out;
 }
 
-static int pick_epl(int S2) {
-    int need = (S2 + 63) / 64;
-    int e = 1;
-    while (e < need) e <<= 1;
-    return e;
-}
 
 // ---------------------------------------------------------------------------
 // the fit
@@ -179,37 +173,38 @@ struct Sim {
     Sim(cp_ctx* c, const HostPsf& p, DevPsf& d) : ctx(c), P(p), DP(d), st(c->stream) {}
 
     template <bool FINAL, bool LDSG>
-    void fill_launch(int epl, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
-                     const CornerRec<T>* crs, int na, int R, int S, const FreeMap& fm, const T* w, int ny, int nx,
-                     const T* wres, T* V, double* cn, double* b, uint8_t* gloc, double gtol, double* part,
-                     Scal* Sc) {
-#define CP_FILL(E)                                                                                           \
-    launch_n(FINAL ? "k_fill_final" : "k_fill", k_fill<T, KIND, FINAL, LDSG, E>, grid, dim3(BLOCK), sh, st, M, recs, crs, na, R, S, fm, w, ny, nx, \
-           wres, V, cn, b, gloc, gtol, part, Sc)
-        switch (epl) {
-            case 1: CP_FILL(1); break;
-            case 2: CP_FILL(2); break;
-            case 4: CP_FILL(4); break;
-            case 8: CP_FILL(8); break;
-            default: CP_FILL(16); break;
-        }
+    void fill_launch(int S2, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
+                     const EpsfRec<T>* ers, const CornerRec<T>* crs, int na, int R, int S, const FreeMap& fm,
+                     const T* w, int ny, int nx, const T* wres, T* V, double* cn, double* b, uint8_t* gloc,
+                     double gtol, double* part, Scal* Sc) {
+        const char* nm = FINAL ? "k_fill_final" : "k_fill";
+#define CP_FILL(E, G)                                                                                        \
+    launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G>, grid, dim3(BLOCK), sh, st, M, recs, ers, crs, na, R, S, fm, \
+             w, ny, nx, wres, V, cn, b, gloc, gtol, part, Sc)
+        if (S2 <= 16) CP_FILL(1, 16);
+        else if (S2 <= 32) CP_FILL(2, 16);
+        else if (S2 <= 64) CP_FILL(4, 16);
+        else if (S2 <= 128) CP_FILL(8, 16);
+        else if (S2 <= 256) CP_FILL(16, 16);
+        else if (S2 <= 512) CP_FILL(8, 64);
+        else CP_FILL(16, 64);
 #undef CP_FILL
     }
 
     template <int MODE>
     void render_launch(bool ldsg, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
-                       const CornerRec<T>* crs, const TileLists& L, const TileGeo& g, int ntiles, int tpb,
-                       const uint8_t* umask, T* dw, const T* w, const T* data, T* wres, T* resid, double* part,
-                       Scal* Sc) {
+                       const EpsfRec<T>* ers, const CornerRec<T>* crs, const TileLists& L, const TileGeo& g,
+                       int ntiles, int tpb, const uint8_t* umask, T* dw, const T* w, const T* data, T* wres,
+                       T* resid, double* part, Scal* Sc) {
         static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
         if (ldsg)
-            launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(BLOCK), sh, st, M, recs, crs,
-                   (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                   ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
+            launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(BLOCK), sh, st, M, recs, ers, crs,
+                     (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
+                     ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
         else
-            launch_n(nm[MODE], k_render<T, KIND, MODE, false>, grid, dim3(BLOCK), sh, st, M, recs, crs,
-                   (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                   ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
+            launch_n(nm[MODE], k_render<T, KIND, MODE, false>, grid, dim3(BLOCK), sh, st, M, recs, ers, crs,
+                     (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
+                     ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
     }
 
     void run(const float* d_img, const float* d_iv, int ny, int nx, const double* d_y0, const double* d_x0,
@@ -361,12 +356,15 @@ struct Sim {
         CP_HIP(hipMemcpyAsync(&Sc->dof, &dof, 8, hipMemcpyHostToDevice, st));
 
         // --- tiles: fit list (stamp box) and model list (model box)
-        int TY = 32;
-        if (o.tile == 16 || o.tile == 32 || o.tile == 64 || o.tile == 128 || o.tile == 256) TY = o.tile;
-        TileGeo g{TY, TILE_PX / TY, (ny + TY - 1) / TY, (nx + TILE_PX / TY - 1) / (TILE_PX / TY), ny, nx};
-        const int ntiles = g.nty * g.ntx;
+        // model-box lists (render passes) use TYm-row tiles, stamp lists (J v) 32-row tiles
+        int TYm = 16;
+        if (o.tile == 8 || o.tile == 16 || o.tile == 32 || o.tile == 64 || o.tile == 128) TYm = o.tile;
+        const int TYf = 32;
+        TileGeo g{TYm, TILE_PX / TYm, (ny + TYm - 1) / TYm, (nx + TILE_PX / TYm - 1) / (TILE_PX / TYm), ny, nx};
+        TileGeo gf{TYf, TILE_PX / TYf, (ny + TYf - 1) / TYf, (nx + TILE_PX / TYf - 1) / (TILE_PX / TYf), ny, nx};
+        const int ntiles = g.nty * g.ntx, ntiles_f = gf.nty * gf.ntx;
         TileLists Lfit, Lmod;
-        build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), nullptr, R, g, Lfit);
+        build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), nullptr, R, gf, Lfit);
         build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), o.model_rad > 0 ? nullptr : mR.as<int>(),
                     mR_scalar, g, Lmod);
         cub.tmp.release();
@@ -385,8 +383,11 @@ struct Sim {
         CP_HIP(hipMemsetAsync(wl.get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(tb.get(), 0, nb8, st));
         DevBuf gloc(ctx, na), recs(ctx, (size_t)na * sizeof(StarRec<T>)), recs_c(ctx, (size_t)na * sizeof(StarRec<T>));
-        DevBuf crs, crs_c;
+        DevBuf crs, crs_c, ers, ers_c;
         const bool multi = (KIND == KIND_EPSF && P.n_nodes > 1);
+        const bool single_epsf = (KIND == KIND_EPSF && P.n_nodes == 1);
+        ers.alloc(ctx, (size_t)(single_epsf ? na : 1) * sizeof(EpsfRec<T>));
+        ers_c.alloc(ctx, (size_t)(single_epsf ? na : 1) * sizeof(EpsfRec<T>));
         if (multi) {
             crs.alloc(ctx, (size_t)na * sizeof(CornerRec<T>));
             crs_c.alloc(ctx, (size_t)na * sizeof(CornerRec<T>));
@@ -394,45 +395,54 @@ struct Sim {
             crs.alloc(ctx, sizeof(CornerRec<T>));
             crs_c.alloc(ctx, sizeof(CornerRec<T>));
         }
-        const int maxpb = n_cu * 8;
+        const int maxpb = n_cu * 16;
         DevBuf part(ctx, (size_t)maxpb * 8), part2(ctx, (size_t)maxpb * 8);
 
         const ModelConst<T> M = model_const<T>(P, DP, o.fixed_bkg);
         const double inv_norm_g64 = (KIND == KIND_GAUSS) ? 1.0 / (-(M_PI * P.fwhm * P.fwhm / CP_GAUSS_PRE)) : 0.0;
         const double moffat_amp64 = (KIND == KIND_MOFFAT) ? (P.beta - 1) / (M_PI * P.alpha * P.alpha) : 0.0;
-        const bool ldsg = (KIND == KIND_EPSF) && P.n_nodes == 1 && (size_t)P.gny * P.gnx * 4 <= 56 * 1024;
-        const size_t grid_sh = ldsg ? (((size_t)P.gny * P.gnx * 4 + 15) & ~(size_t)15) : 0;
-        const size_t render_sh = grid_sh + CHUNK * (sizeof(StarRec<T>) + 4);
+        const size_t poly_words = single_epsf ? (size_t)M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
+        const bool ldsg = single_epsf && poly_words * 4 <= 44 * 1024;
+        const size_t grid_sh = ldsg ? ((poly_words * 4 + 15) & ~(size_t)15) : 0;
+        const size_t render_sh =
+            grid_sh + RCHUNK * (sizeof(StarRec<T>) + (ldsg ? sizeof(EpsfRec<T>) : 0) + 4);
         const size_t fill_sh = grid_sh;
+        // J v: stamp values of CJ stars per LDS chunk
+        const size_t vbytes = (size_t)S2 * p * sizeof(T);
+        const int CJ = (int)std::max<size_t>(1, std::min<size_t>(64, (40 * 1024) / vbytes));
+        const size_t jv_sh = (((size_t)CJ * vbytes + 15) & ~(size_t)15) + (size_t)CJ * (3 * 8 + 3 * 4);
 
         const int nb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 4);
         const int tpb_render = (int)cdiv(ntiles, nb_render);
         const int gb_render = (int)cdiv(ntiles, tpb_render);
-        const int nb_jv = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 8);
-        const int tpb_jv = (int)cdiv(ntiles, nb_jv);
-        const int gb_jv = (int)cdiv(ntiles, tpb_jv);
-        const int gb_fill = (int)std::min<int64_t>(cdiv(na, NWAVE), (int64_t)n_cu * 4);
-        const int gb_jtu = (int)std::min<int64_t>(cdiv(na, NWAVE), (int64_t)n_cu * 8);
-        const int epl = pick_epl(S2);
+        const int nb_jv = (int)std::min<int64_t>(ntiles_f, (int64_t)n_cu * 8);
+        const int tpb_jv = (int)cdiv(ntiles_f, nb_jv);
+        const int gb_jv = (int)cdiv(ntiles_f, tpb_jv);
+        const int GLs = S2 <= 256 ? 16 : 64;
+        const int gb_fill = (int)std::min<int64_t>(cdiv(na, BLOCK / GLs), (int64_t)n_cu * 4);
+        const int gb_jtu = (int)std::min<int64_t>(cdiv(na, BLOCK / GLs), (int64_t)n_cu * 16);
         const unsigned int gstar = cdiv(na, BLOCK);
 
-        auto star_rec = [&](const double* th, DevBuf& r, DevBuf& c, int frozen_iter, int all_live) {
+        auto star_rec = [&](const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter, int all_live) {
             launch(k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, th, fm, M, inv_norm_g64, moffat_amp64,
                    (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const int*)mR.as<int>(),
                    (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), frozen_iter, all_live,
-                   r.as<StarRec<T>>(), DP.G, c.as<CornerRec<T>>());
+                   r.as<StarRec<T>>(), DP.G, c.as<CornerRec<T>>(), single_epsf ? e.as<EpsfRec<T>>() : nullptr);
         };
         auto fill = [&](bool final_mode) {
+            const StarRec<T>* rp = recs.as<StarRec<T>>();
+            const EpsfRec<T>* ep = ers.as<EpsfRec<T>>();
+            const CornerRec<T>* cp = crs.as<CornerRec<T>>();
             if (ldsg) {
                 if (final_mode)
-                    fill_launch<true, true>(epl, dim3(gb_fill), fill_sh, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(), na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<true, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
                 else
-                    fill_launch<false, true>(epl, dim3(gb_fill), fill_sh, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(), na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<false, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             } else {
                 if (final_mode)
-                    fill_launch<true, false>(epl, dim3(gb_fill), 0, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(), na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<true, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
                 else
-                    fill_launch<false, false>(epl, dim3(gb_fill), 0, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(), na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<false, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             }
         };
         auto zero_field = [&](void* dev_field, size_t bytes) { CP_HIP(hipMemsetAsync(dev_field, 0, bytes, st)); };
@@ -442,9 +452,9 @@ struct Sim {
         int n_run = 0, n_live_now = na;
         bool D_init = false;
         for (int iter = 1; iter <= o.max_iter; ++iter) {
-            star_rec(theta.as<double>(), recs, crs, 0, 0);
+            star_rec(theta.as<double>(), recs, crs, ers, 0, 0);
             render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
-                                      crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render, umask.as<uint8_t>(),
+                                      ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render, umask.as<uint8_t>(),
                                       dw.as<T>(), w.as<T>(), data.as<T>(), wres.as<T>(), nullptr, part.as<double>(), Sc);
             zero_field(&Sc->gmax_bits, 8);
             fill(false);
@@ -472,25 +482,33 @@ struct Sim {
                        o.linear_tol);
                 for (int it = 1; it <= o.inner_iterations; ++it) {
                     const int first = it == 1;
-                    launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, (const int*)Lfit.start.as<int>(),
-                           (const int*)Lfit.end.as<int>(), (const int*)Lfit.list.as<int>(), g, ntiles, tpb_jv,
-                           (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S,
-                           p, (const T*)V.as<T>(), (const double*)(first ? bsc.as<double>() : tb.as<double>()),
-                           (const T*)(first ? wres.as<T>() : ubuf.as<T>()), ubuf.as<T>(),
-                           (const uint8_t*)umask.as<uint8_t>(), first, part.as<double>(), Sc);
-                    launch_n("k_JTu", k_JTu<T>, dim3(gb_jtu), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
-                           (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
-                           (const T*)V.as<T>(), (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(),
-                           tb.as<double>(), xs.as<double>(), wl.as<double>(), first, it, part.as<double>(),
-                           part2.as<double>(), Sc);
+                    launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), jv_sh, st, (const int*)Lfit.start.as<int>(),
+                             (const int*)Lfit.end.as<int>(), (const int*)Lfit.list.as<int>(), gf, ntiles_f, tpb_jv,
+                             (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R,
+                             S, p, CJ, (const T*)V.as<T>(),
+                             (const double*)(first ? bsc.as<double>() : tb.as<double>()),
+                             (const T*)(first ? wres.as<T>() : ubuf.as<T>()), ubuf.as<T>(),
+                             (const uint8_t*)umask.as<uint8_t>(), first, part.as<double>(), Sc);
+                    if (GLs == 16)
+                        launch_n("k_JTu", k_JTu<T, 16>, dim3(gb_jtu), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
+                                 (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
+                                 (const T*)V.as<T>(), (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(),
+                                 tb.as<double>(), xs.as<double>(), wl.as<double>(), first, it, part.as<double>(),
+                                 part2.as<double>(), Sc);
+                    else
+                        launch_n("k_JTu", k_JTu<T, 64>, dim3(gb_jtu), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
+                                 (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
+                                 (const T*)V.as<T>(), (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(),
+                                 tb.as<double>(), xs.as<double>(), wl.as<double>(), first, it, part.as<double>(),
+                                 part2.as<double>(), Sc);
                 }
                 launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
                        (const double*)wl.as<double>(), xs.as<double>(), (const double*)colnorm.as<double>(),
                        (const double*)theta.as<double>(), dsc.as<double>(), theta_c.as<double>(),
                        (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
-                star_rec(theta_c.as<double>(), recs_c, crs_c, 0, 0);
+                star_rec(theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
                 render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
-                                         crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
+                                         ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
                                          umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres.as<T>(),
                                          nullptr, part.as<double>(), Sc);
                 read_S();
@@ -541,9 +559,9 @@ struct Sim {
                 read_S();
                 n_live_now = hS->n_live;
                 if (hS->n_frozen_now > 0) {
-                    star_rec(theta.as<double>(), recs, crs, iter, 0);
+                    star_rec(theta.as<double>(), recs, crs, ers, iter, 0);
                     render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
-                                               crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
+                                               ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
                                                umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(),
                                                wres.as<T>(), nullptr, part.as<double>(), Sc);
                 }
@@ -553,7 +571,7 @@ struct Sim {
 
         // --- epilogue: errors (:1302-1329), final render and chisq (:1344-1397)
         const int cov_mode = o.covariance == CP_COV_DEFAULT ? (d_iv ? CP_COV_KNOWN : CP_COV_REWEIGHTED) : o.covariance;
-        star_rec(theta.as<double>(), recs, crs, 0, 1);
+        star_rec(theta.as<double>(), recs, crs, ers, 0, 1);
         fill(true);
         DevBuf errs(ctx, nb8);
         CP_HIP(hipMemsetAsync(errs.get(), 0, nb8, st));
@@ -562,7 +580,8 @@ struct Sim {
                    (const int*)ax.as<int>(), (const T*)w.as<T>(), ny, nx, (const T*)V.as<T>(),
                    (const double*)colnorm.as<double>(), cov_mode, cost / dof, errs.as<double>());
         T* resid = ubuf.as<T>();
-        render_launch<MODE_FINAL>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(), crs.as<CornerRec<T>>(),
+        render_launch<MODE_FINAL>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(), ers.as<EpsfRec<T>>(),
+                                  crs.as<CornerRec<T>>(),
                                   Lmod, g, ntiles, tpb_render, umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
                                   data.as<T>(), wres.as<T>(), resid, part.as<double>(), Sc);
         DevBuf chis(ctx, (size_t)na * 8);
