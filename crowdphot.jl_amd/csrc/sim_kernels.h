@@ -11,10 +11,12 @@
 //    reference's scatter-add.
 //  * per-star products (fill + J^T r, J^T u, steps, freeze tests) run one
 //    wavefront per star with wave-shuffle reductions in float64.
-//  * grid-wide sums use per-block partials and a last-block-done counter,
-//    summed in block order: deterministic run to run.
-//  * persistent blocks (a fixed multiple of the CU count) stage the ePSF node
-//    grid in LDS once and walk contiguous tile / star ranges.
+//  * grid-wide sums: every block writes its partial with a plain store and a
+//    one-block k_finish launch sums them in block order (deterministic run to
+//    run).  No same-address atomics and no grid-wide fences: on gfx950 both
+//    serialise at the L2 (about 12 ns per block).
+//  * blocks stage the ePSF node grid in LDS (polyphase layout) once and walk
+//    contiguous tile / star ranges.
 #pragma once
 
 #include "cp_internal.h"
@@ -80,34 +82,23 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     return r;
 }
 
-__device__ __forceinline__ double ld_agent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Block-sum v and store it as this block's partial (plain store; the k_finish
+// launch that follows reads it after the kernel boundary).
+__device__ __forceinline__ void block_store(double v, double* partials, double* sh) {
+    const double b = block_sum(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = b;
 }
 
-// Publish this block's partial (thread 0 holds v) and report whether this
-// block is the last of nblocks to finish.  The counter is left at 0.
-__device__ __forceinline__ bool publish_last(double v, double* partials, unsigned int* counter,
-                                             unsigned int nblocks) {
-    __shared__ int am_last;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(partials + blockIdx.x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned int prev = atomicAdd(counter, 1u);
-        am_last = (prev == nblocks - 1);
-        if (am_last) {
-            __threadfence();
-            *counter = 0;
-        }
-    }
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __syncthreads();
-    return am_last != 0;
-}
-
-// Sum partials[0..n) in fixed order inside the last block; result in thread 0.
-__device__ __forceinline__ double sum_partials(const double* partials, int n, double* sh) {
-    double acc = 0;
-    for (int i = threadIdx.x; i < n; i += BLOCK) acc += ld_agent(partials + i);
-    return block_sum(acc, sh);
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NWAVE; ++i) r = fmax(r, sh[i]);
+    return r;
 }
 
 // Symmetric Givens rotation: [c s; s -c] [a; b] = [rho; 0].
@@ -250,12 +241,17 @@ static __global__ void k_union(int na, const int* __restrict__ ay, const int* __
     if (w[q] > T(0)) umask[q] = 1;
 }
 
-static __global__ void k_count_mask(const uint8_t* __restrict__ m, int64_t P, unsigned long long* out) {
-    unsigned long long c = 0;
-    for (int64_t q = blockIdx.x * (int64_t)BLOCK + threadIdx.x; q < P; q += (int64_t)gridDim.x * BLOCK)
-        c += m[q];
-    c = (unsigned long long)wave_sum_int((int)c);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+static __global__ void k_count_mask(const uint8_t* __restrict__ m, int64_t P, double* __restrict__ partials) {
+    __shared__ double sred[NWAVE];
+    int c = 0;
+    const int64_t P4 = P / 4;
+    const uint32_t* m4 = reinterpret_cast<const uint32_t*>(m);
+    for (int64_t q = blockIdx.x * (int64_t)BLOCK + threadIdx.x; q < P4; q += (int64_t)gridDim.x * BLOCK) {
+        const uint32_t v = m4[q];
+        c += (v & 0xff) + ((v >> 8) & 0xff) + ((v >> 16) & 0xff) + (v >> 24);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (int)(P - P4 * 4)) c += m[P4 * 4 + threadIdx.x];
+    block_store((double)c, partials, sred);
 }
 
 // Stamp pixel indices of active stars (1-based, 0 = masked), for export.
@@ -528,14 +524,11 @@ __global__ __launch_bounds__(BLOCK) void k_render(
         for (int k = 0; k < PPT; ++k) {
             if (!act[k]) continue;
             size_t q = (size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny;
-            if (MODE == MODE_RESID) {
+            if (MODE == MODE_RESID || MODE == MODE_COST) {
                 T r = val[k] - dw[q];
                 T wr = sqrt(w[q]) * r;
                 wt_resid[q] = wr;
                 acc += (double)wr * (double)wr;
-            } else if (MODE == MODE_COST) {
-                T r = val[k] - dw[q];
-                acc += (double)w[q] * (double)r * (double)r;
             } else if (MODE == MODE_FREEZE) {
                 if (touched) dw[q] = val[k];
             } else {
@@ -543,16 +536,7 @@ __global__ __launch_bounds__(BLOCK) void k_render(
             }
         }
     }
-    if (MODE == MODE_RESID || MODE == MODE_COST) {
-        double b = block_sum(acc, sred);
-        if (publish_last(b, partials, &S->counters[0], gridDim.x)) {
-            double tot = sum_partials(partials, gridDim.x, sred);
-            if (threadIdx.x == 0) {
-                if (MODE == MODE_RESID) S->cost = tot;
-                else S->cost_cand = tot;
-            }
-        }
-    }
+    if (MODE == MODE_RESID || MODE == MODE_COST) block_store(acc, partials, sred);
 }
 
 // ---------------------------------------------------------------------------
@@ -671,14 +655,10 @@ __global__ __launch_bounds__(BLOCK) void k_fill(
         }
     }
     if (!FINAL) {
-        gmax = wave_max(gmax);
-        if ((threadIdx.x & 63) == 0 && gmax > 0)
-            atomicMax(&Sc->gmax_bits, (unsigned long long)__double_as_longlong(gmax));
-        double b = block_sum(bsq, sred);
-        if (publish_last(b, partials, &Sc->counters[1], gridDim.x)) {
-            double tot = sum_partials(partials, gridDim.x, sred);
-            if (threadIdx.x == 0) Sc->bnorm2 = tot;
-        }
+        // partials[0, nb): |b|^2 per block, partials[nb, 2 nb): max g-cosine per block
+        block_store(bsq, partials, sred);
+        const double m = block_max(gmax, sred);
+        if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = m;
     }
 }
 
@@ -800,11 +780,7 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(
             acc2 += (double)u * (double)u;
         }
     }
-    double b = block_sum(acc2, sred);
-    if (publish_last(b, partials, &Sc->counters[2], gridDim.x)) {
-        double tot = sum_partials(partials, gridDim.x, sred);
-        if (threadIdx.x == 0) Sc->beta_next = sqrt(tot);
-    }
+    block_store(acc2, partials, sred);
 }
 
 // v <- J^T u - beta v per star, with the pending x/w update of the previous
@@ -868,26 +844,15 @@ __global__ __launch_bounds__(BLOCK) void k_JTu(
             tsq += tn * tn;
         }
     }
-    double bt = block_sum(tsq, sred);
-    double bw = block_sum(wsq, sred);
-    __shared__ int am_last;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(partials + blockIdx.x, bt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(partials2 + blockIdx.x, bw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned int prev = atomicAdd(&Sc->counters[3], 1u);
-        am_last = (prev == gridDim.x - 1);
-        if (am_last) {
-            __threadfence();
-            Sc->counters[3] = 0;
-        }
-    }
-    __syncthreads();
-    if (!am_last) return;
-    double tt = sum_partials(partials, gridDim.x, sred);
-    double ww = sum_partials(partials2, gridDim.x, sred);
-    if (threadIdx.x != 0) return;
-    // ---- LSQR scalar recurrences of iteration `it` (one thread) ----
+    block_store(tsq, partials, sred);
+    block_store(wsq, partials2, sred);
+}
+
+// LSQR scalar recurrences of iteration `it` (one thread), from tt = |J^T u -
+// beta v|^2 and ww = |w|^2 of that iteration (Paige & Saunders 1982, with the
+// Krylov.jl-style stopping tests).
+__device__ void lsqr_advance(Scal* Sc, double tt, double ww, int it) {
+    const double alpha = Sc->alpha, beta = Sc->beta_next;
     const double lam = Sc->lam;
     double alpha_new = alpha;
     if (beta != 0) {
@@ -945,6 +910,59 @@ __global__ __launch_bounds__(BLOCK) void k_JTu(
     Sc->done = solved || ill || it >= Sc->itmax;
 }
 
+// ---------------------------------------------------------------------------
+// k_finish: one block sums the per-block partials of the preceding launch in
+// block order and updates the device scalars
+// ---------------------------------------------------------------------------
+enum : int { FIN_COST = 0, FIN_COST_CAND = 1, FIN_FILL = 2, FIN_BETA = 3, FIN_LSQR = 4, FIN_ACCEPT = 5, FIN_COUNT = 6 };
+constexpr int FIN_BLOCK = 1024;
+
+template <int OP>
+static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const double* __restrict__ part,
+                                                             const double* __restrict__ part2, int nb, int it) {
+    if ((OP == FIN_BETA || OP == FIN_LSQR) && Sc->done) return;
+    __shared__ double sh[2][FIN_BLOCK / 64];
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nb; i += FIN_BLOCK) {
+        a += part[i];
+        if (OP == FIN_FILL) b = fmax(b, part[nb + i]);
+        if (OP == FIN_ACCEPT) b += part[nb + i];
+        if (OP == FIN_LSQR) b += part2[i];
+    }
+    a = wave_sum(a);
+    b = OP == FIN_FILL ? wave_max(b) : wave_sum(b);
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double ta = 0, tb = 0;
+    for (int i = 0; i < FIN_BLOCK / 64; ++i) {
+        ta += sh[0][i];
+        tb = OP == FIN_FILL ? fmax(tb, sh[1][i]) : tb + sh[1][i];
+    }
+    if (OP == FIN_COST) Sc->cost = ta;
+    if (OP == FIN_COST_CAND) Sc->cost_cand = ta;
+    if (OP == FIN_FILL) {
+        Sc->bnorm2 = ta;
+        Sc->gmax_bits = (unsigned long long)__double_as_longlong(tb);
+    }
+    if (OP == FIN_BETA) Sc->beta_next = sqrt(ta);
+    if (OP == FIN_LSQR) lsqr_advance(Sc, ta, tb, it);
+    if (OP == FIN_ACCEPT) {
+        Sc->n_live = (int)ta;
+        Sc->n_frozen_now = (int)tb;
+    }
+    if (OP == FIN_COUNT) Sc->n_union = (unsigned long long)ta;
+}
+
+// The accepted candidate's cost becomes the current cost (its weighted
+// residual is reused as the next iteration's residual).
+static __global__ void k_take_candidate(Scal* Sc) {
+    if (threadIdx.x == 0) Sc->cost = Sc->cost_cand;
+}
+
 // Final x update, step capping and candidate theta (:1224-1230, _cap_position_step! :700-716).
 static __global__ void k_step(int na, int p, int k_y, int k_x, double max_step, const double* __restrict__ wv,
                        double* __restrict__ x, const double* __restrict__ colnorm,
@@ -981,7 +999,7 @@ static __global__ void k_accept(int na, int p, int iter, double x_tol, double* _
                          const double* __restrict__ theta_c, double* __restrict__ D,
                          const double* __restrict__ colnorm, const double* __restrict__ dsc,
                          const uint8_t* __restrict__ gloc, uint8_t* __restrict__ live,
-                         int* __restrict__ frozen_at, uint8_t* __restrict__ conv, Scal* Sc) {
+                         int* __restrict__ frozen_at, uint8_t* __restrict__ conv, double* __restrict__ partials) {
     int a = blockIdx.x * BLOCK + threadIdx.x;
     int nl = 0, nf = 0;
     if (a < na) {
@@ -1009,12 +1027,11 @@ static __global__ void k_accept(int na, int p, int iter, double x_tol, double* _
             }
         }
     }
-    nl = wave_sum_int(nl);
-    nf = wave_sum_int(nf);
-    if ((threadIdx.x & 63) == 0) {
-        if (nl) atomicAdd(&Sc->n_live, nl);
-        if (nf) atomicAdd(&Sc->n_frozen_now, nf);
-    }
+    __shared__ double sred[NWAVE];
+    // partials[0, nb): live count, partials[nb, 2 nb): newly frozen count
+    block_store((double)nl, partials, sred);
+    const double f = block_sum((double)nf, sred);
+    if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = f;
 }
 
 // _freeze_remaining! (:767-776).
@@ -1029,13 +1046,14 @@ static __global__ void k_freeze_all(int na, int iter, uint8_t* __restrict__ live
 
 // Per-star error block, covariance and errors (:1305-1329, covariance!
 // levenberg_marquardt.jl:366-388).  cov_mode 1 known, 2 reweighted, -1 none.
-template <typename T>
-static __global__ void k_errors(int na, int p, int R, int S, const int* __restrict__ ay, const int* __restrict__ ax,
+template <typename T, int P>
+static __global__ void k_errors(int na, int R, int S, const int* __restrict__ ay, const int* __restrict__ ax,
                          const T* __restrict__ w, int ny, int nx, const T* __restrict__ V,
                          const double* __restrict__ colnorm, int cov_mode, double fac_rw,
                          double* __restrict__ errs) {
     int a = blockIdx.x * BLOCK + threadIdx.x;
     if (a >= na) return;
+    constexpr int p = P;
     const int S2 = S * S;
     double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int m = 0; m < S2; ++m) {
@@ -1043,72 +1061,96 @@ static __global__ void k_errors(int na, int p, int R, int S, const int* __restri
         if (gy < 1 || gy > ny || gx < 1 || gx > nx) continue;
         if (!(w[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] > T(0))) continue;
         const T* Vm = V + ((size_t)a * S2 + m) * p;
+#pragma unroll
         for (int k = 0; k < p; ++k)
+#pragma unroll
             for (int l = 0; l < p; ++l) blk[k * 3 + l] += (double)Vm[k] * (double)Vm[l];
     }
     const double* cn = colnorm + (size_t)a * p;
     double tr = 0;
+#pragma unroll
     for (int k = 0; k < p; ++k)
+#pragma unroll
         for (int l = 0; l < p; ++l) blk[k * 3 + l] *= cn[k] * cn[l];
+#pragma unroll
     for (int k = 0; k < p; ++k) tr += blk[k * 3 + k];
+#pragma unroll
     for (int k = 0; k < p; ++k) blk[k * 3 + k] += 1e-12 * tr;
     double cov[9];
     // Cholesky inverse, else eigen pseudo-inverse (pinv)
     double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool pd = true;
-    for (int j = 0; j < p && pd; ++j) {
+#pragma unroll
+    for (int j = 0; j < p; ++j) {
         double s = blk[j * 3 + j];
+#pragma unroll
         for (int k = 0; k < j; ++k) s -= L[j * 3 + k] * L[j * 3 + k];
-        if (!(s > 0)) { pd = false; break; }
-        double d = sqrt(s);
+        pd = pd && (s > 0);
+        const double d = pd ? sqrt(s) : 1.0;
         L[j * 3 + j] = d;
+#pragma unroll
         for (int i = j + 1; i < p; ++i) {
             double t = blk[j * 3 + i];
+#pragma unroll
             for (int k = 0; k < j; ++k) t -= L[i * 3 + k] * L[j * 3 + k];
             L[i * 3 + j] = t / d;
         }
     }
     if (pd) {
+#pragma unroll
         for (int j = 0; j < p; ++j) {
             double y[3], xs[3];
+#pragma unroll
             for (int i = 0; i < p; ++i) {
                 double s = (i == j);
                 for (int k = 0; k < i; ++k) s -= L[i * 3 + k] * y[k];
                 y[i] = s / L[i * 3 + i];
             }
+#pragma unroll
             for (int i = p - 1; i >= 0; --i) {
                 double s = y[i];
+#pragma unroll
                 for (int k = i + 1; k < p; ++k) s -= L[k * 3 + i] * xs[k];
                 xs[i] = s / L[i * 3 + i];
             }
+#pragma unroll
             for (int i = 0; i < p; ++i) cov[i * 3 + j] = xs[i];
         }
     } else {
         double A[9], E[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
         for (int i = 0; i < 9; ++i) A[i] = blk[i];
+#pragma unroll
         for (int i = 0; i < p; ++i) E[i * 3 + i] = 1;
         for (int sweep = 0; sweep < 60; ++sweep) {
             double off = 0;
+#pragma unroll
             for (int i = 0; i < p; ++i)
+#pragma unroll
                 for (int j = i + 1; j < p; ++j) off += A[i * 3 + j] * A[i * 3 + j];
             if (off < 1e-300) break;
+#pragma unroll
             for (int i = 0; i < p; ++i)
+#pragma unroll
                 for (int j = i + 1; j < p; ++j) {
                     double aij = A[i * 3 + j];
                     if (aij == 0) continue;
                     double th = (A[j * 3 + j] - A[i * 3 + i]) / (2 * aij);
                     double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
                     double cc = 1 / sqrt(t * t + 1), ss = t * cc;
+#pragma unroll
                     for (int k = 0; k < p; ++k) {
                         double aki = A[k * 3 + i], akj = A[k * 3 + j];
                         A[k * 3 + i] = cc * aki - ss * akj;
                         A[k * 3 + j] = ss * aki + cc * akj;
                     }
+#pragma unroll
                     for (int k = 0; k < p; ++k) {
                         double aik = A[i * 3 + k], ajk = A[j * 3 + k];
                         A[i * 3 + k] = cc * aik - ss * ajk;
                         A[j * 3 + k] = ss * aik + cc * ajk;
                     }
+#pragma unroll
                     for (int k = 0; k < p; ++k) {
                         double vki = E[k * 3 + i], vkj = E[k * 3 + j];
                         E[k * 3 + i] = cc * vki - ss * vkj;
@@ -1117,17 +1159,22 @@ static __global__ void k_errors(int na, int p, int R, int S, const int* __restri
                 }
         }
         double smax = 0;
+#pragma unroll
         for (int i = 0; i < p; ++i) smax = fmax(smax, fabs(A[i * 3 + i]));
         double tol = DEPS * p * smax;
+#pragma unroll
         for (int i = 0; i < p; ++i)
+#pragma unroll
             for (int j = 0; j < p; ++j) {
                 double s = 0;
+#pragma unroll
                 for (int k = 0; k < p; ++k)
                     if (fabs(A[k * 3 + k]) > tol) s += E[i * 3 + k] * E[j * 3 + k] / A[k * 3 + k];
                 cov[i * 3 + j] = s;
             }
     }
     double fac = cov_mode == 2 ? fac_rw : 1.0;
+#pragma unroll
     for (int k = 0; k < p; ++k) errs[(size_t)a * p + k] = sqrt(fmax(0.0, fac * cov[k * 3 + k]));
 }
 

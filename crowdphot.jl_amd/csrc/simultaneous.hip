@@ -347,9 +347,15 @@ struct Sim {
                conv.as<uint8_t>());
         launch(k_union<T>, dim3(cdiv((int64_t)na * S2, BLOCK)), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
                (const int*)ax.as<int>(), R, S, (const T*)w.as<T>(), ny, nx, umask.as<uint8_t>());
-        launch(k_count_mask, dim3(std::min<int64_t>(cdiv(Pn, BLOCK), n_cu * 16)), dim3(BLOCK), 0, st,
-               (const uint8_t*)umask.as<uint8_t>(), Pn, &Sc->n_union);
-        read_S();
+        {
+            const int gcm = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(Pn / 4, BLOCK), n_cu * 4));
+            DevBuf cm_part(ctx, (size_t)gcm * 8);
+            launch(k_count_mask, dim3(gcm), dim3(BLOCK), 0, st, (const uint8_t*)umask.as<uint8_t>(), Pn,
+                   cm_part.as<double>());
+            launch(k_finish<FIN_COUNT>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc, (const double*)cm_part.as<double>(),
+                   (const double*)nullptr, gcm, 0);
+            read_S();
+        }
         const int64_t U = (int64_t)hS->n_union;
         const int64_t nvar = (int64_t)p * na;
         const double dof = (double)std::max<int64_t>(U - nvar, 1);
@@ -370,8 +376,12 @@ struct Sim {
         cub.tmp.release();
 
         // --- working buffers
-        DevBuf wres(ctx, Pn * sizeof(T)), ubuf(ctx, Pn * sizeof(T));
-        CP_HIP(hipMemsetAsync(wres.get(), 0, Pn * sizeof(T), st));
+        // weighted residual of the current point and of the trial candidate
+        DevBuf wres_a(ctx, Pn * sizeof(T)), wres_b(ctx, Pn * sizeof(T)), ubuf(ctx, Pn * sizeof(T));
+        CP_HIP(hipMemsetAsync(wres_a.get(), 0, Pn * sizeof(T), st));
+        CP_HIP(hipMemsetAsync(wres_b.get(), 0, Pn * sizeof(T), st));
+        T* wres_cur = wres_a.as<T>();
+        T* wres_cand = wres_b.as<T>();
         CP_HIP(hipMemsetAsync(ubuf.get(), 0, Pn * sizeof(T), st));
         DevBuf V(ctx, (size_t)na * S2 * p * sizeof(T));
         CP_HIP(hipMemsetAsync(V.get(), 0, V.bytes(), st));
@@ -395,8 +405,8 @@ struct Sim {
             crs.alloc(ctx, sizeof(CornerRec<T>));
             crs_c.alloc(ctx, sizeof(CornerRec<T>));
         }
-        const int maxpb = n_cu * 16;
-        DevBuf part(ctx, (size_t)maxpb * 8), part2(ctx, (size_t)maxpb * 8);
+        const int maxpb = (int)std::max<int64_t>((int64_t)n_cu * 16, cdiv(na, BLOCK));
+        DevBuf part(ctx, (size_t)2 * maxpb * 8), part2(ctx, (size_t)maxpb * 8);
 
         const ModelConst<T> M = model_const<T>(P, DP, o.fixed_bkg);
         const double inv_norm_g64 = (KIND == KIND_GAUSS) ? 1.0 / (-(M_PI * P.fwhm * P.fwhm / CP_GAUSS_PRE)) : 0.0;
@@ -435,29 +445,41 @@ struct Sim {
             const CornerRec<T>* cp = crs.as<CornerRec<T>>();
             if (ldsg) {
                 if (final_mode)
-                    fill_launch<true, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<true, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
                 else
-                    fill_launch<false, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<false, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             } else {
                 if (final_mode)
-                    fill_launch<true, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<true, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
                 else
-                    fill_launch<false, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, wres.as<T>(), V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<false, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             }
         };
-        auto zero_field = [&](void* dev_field, size_t bytes) { CP_HIP(hipMemsetAsync(dev_field, 0, bytes, st)); };
+        auto finish = [&](auto kern, int nb, int it) {
+            launch(kern, dim3(1), dim3(FIN_BLOCK), 0, st, Sc, (const double*)part.as<double>(),
+                   (const double*)part2.as<double>(), nb, it);
+        };
 
         // --- outer damped Gauss-Newton loop (:1186-1284)
+        // An accepted candidate's weighted residual (written by its cost
+        // render) is the next iteration's residual: the live-star model minus
+        // the frozen-star-subtracted data is the same image, so only the first
+        // iteration renders it separately.
         double lam = o.lambda_init, cost = 0;
         int n_run = 0, n_live_now = na;
-        bool D_init = false;
+        bool D_init = false, have_resid = false;
         for (int iter = 1; iter <= o.max_iter; ++iter) {
             star_rec(theta.as<double>(), recs, crs, ers, 0, 0);
-            render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
-                                      ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render, umask.as<uint8_t>(),
-                                      dw.as<T>(), w.as<T>(), data.as<T>(), wres.as<T>(), nullptr, part.as<double>(), Sc);
-            zero_field(&Sc->gmax_bits, 8);
+            if (!have_resid) {
+                render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
+                                          ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
+                                          umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
+                                          part.as<double>(), Sc);
+                finish(k_finish<FIN_COST>, gb_render, 0);
+                have_resid = true;
+            }
             fill(false);
+            finish(k_finish<FIN_FILL>, gb_fill, 0);
             if (!D_init) {
                 CP_HIP(hipMemcpyAsync(D.get(), colnorm.get(), nb8, hipMemcpyDeviceToDevice, st));
                 D_init = true;
@@ -487,8 +509,9 @@ struct Sim {
                              (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R,
                              S, p, CJ, (const T*)V.as<T>(),
                              (const double*)(first ? bsc.as<double>() : tb.as<double>()),
-                             (const T*)(first ? wres.as<T>() : ubuf.as<T>()), ubuf.as<T>(),
+                             (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(),
                              (const uint8_t*)umask.as<uint8_t>(), first, part.as<double>(), Sc);
+                    finish(k_finish<FIN_BETA>, gb_jv, it);
                     if (GLs == 16)
                         launch_n("k_JTu", k_JTu<T, 16>, dim3(gb_jtu), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
                                  (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
@@ -501,6 +524,7 @@ struct Sim {
                                  (const T*)V.as<T>(), (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(),
                                  tb.as<double>(), xs.as<double>(), wl.as<double>(), first, it, part.as<double>(),
                                  part2.as<double>(), Sc);
+                    finish(k_finish<FIN_LSQR>, gb_jtu, it);
                 }
                 launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
                        (const double*)wl.as<double>(), xs.as<double>(), (const double*)colnorm.as<double>(),
@@ -509,8 +533,9 @@ struct Sim {
                 star_rec(theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
                 render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
                                          ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
-                                         umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres.as<T>(),
+                                         umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cand,
                                          nullptr, part.as<double>(), Sc);
+                finish(k_finish<FIN_COST_CAND>, gb_render, 0);
                 read_S();
                 const double cost_c = hS->cost_cand;
                 out->n_trials_total += 1;
@@ -539,31 +564,30 @@ struct Sim {
             n_run = iter;
             if (f_conv) {
                 // theta <- theta + delta for the accepted step, then freeze everyone
-                zero_field(&Sc->n_live, 4);
-                zero_field(&Sc->n_frozen_now, 4);
                 launch(k_accept, dim3(gstar), dim3(BLOCK), 0, st, na, p, iter, -1.0, theta.as<double>(),
                        (const double*)theta_c.as<double>(), D.as<double>(), (const double*)colnorm.as<double>(),
                        (const double*)dsc.as<double>(), (const uint8_t*)gloc.as<uint8_t>(), live.as<uint8_t>(),
-                       frozen_at.as<int>(), conv.as<uint8_t>(), Sc);
+                       frozen_at.as<int>(), conv.as<uint8_t>(), part.as<double>());
                 launch(k_freeze_all, dim3(gstar), dim3(BLOCK), 0, st, na, iter, live.as<uint8_t>(),
                        frozen_at.as<int>(), conv.as<uint8_t>());
                 break;
             }
             if (accepted) {
-                zero_field(&Sc->n_live, 4);
-                zero_field(&Sc->n_frozen_now, 4);
                 launch(k_accept, dim3(gstar), dim3(BLOCK), 0, st, na, p, iter, o.x_tol, theta.as<double>(),
                        (const double*)theta_c.as<double>(), D.as<double>(), (const double*)colnorm.as<double>(),
                        (const double*)dsc.as<double>(), (const uint8_t*)gloc.as<uint8_t>(), live.as<uint8_t>(),
-                       frozen_at.as<int>(), conv.as<uint8_t>(), Sc);
+                       frozen_at.as<int>(), conv.as<uint8_t>(), part.as<double>());
+                finish(k_finish<FIN_ACCEPT>, (int)gstar, 0);
+                launch(k_take_candidate, dim3(1), dim3(64), 0, st, Sc);
+                std::swap(wres_cur, wres_cand);
                 read_S();
                 n_live_now = hS->n_live;
                 if (hS->n_frozen_now > 0) {
                     star_rec(theta.as<double>(), recs, crs, ers, iter, 0);
                     render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
-                                               ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
-                                               umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(),
-                                               wres.as<T>(), nullptr, part.as<double>(), Sc);
+                                               ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles,
+                                               tpb_render, umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(),
+                                               wres_cur, nullptr, part.as<double>(), Sc);
                 }
                 if (hS->n_live == 0) break;
             }
@@ -575,15 +599,21 @@ struct Sim {
         fill(true);
         DevBuf errs(ctx, nb8);
         CP_HIP(hipMemsetAsync(errs.get(), 0, nb8, st));
-        if (cov_mode != CP_COV_NONE)
-            launch(k_errors<T>, dim3(gstar), dim3(BLOCK), 0, st, na, p, R, S, (const int*)ay.as<int>(),
-                   (const int*)ax.as<int>(), (const T*)w.as<T>(), ny, nx, (const T*)V.as<T>(),
-                   (const double*)colnorm.as<double>(), cov_mode, cost / dof, errs.as<double>());
+        if (cov_mode != CP_COV_NONE) {
+#define CP_ERR(PP)                                                                                             \
+    launch(k_errors<T, PP>, dim3(gstar), dim3(BLOCK), 0, st, na, R, S, (const int*)ay.as<int>(),               \
+           (const int*)ax.as<int>(), (const T*)w.as<T>(), ny, nx, (const T*)V.as<T>(),                          \
+           (const double*)colnorm.as<double>(), cov_mode, cost / dof, errs.as<double>())
+            if (p == 1) CP_ERR(1);
+            else if (p == 2) CP_ERR(2);
+            else CP_ERR(3);
+#undef CP_ERR
+        }
         T* resid = ubuf.as<T>();
         render_launch<MODE_FINAL>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(), ers.as<EpsfRec<T>>(),
                                   crs.as<CornerRec<T>>(),
                                   Lmod, g, ntiles, tpb_render, umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
-                                  data.as<T>(), wres.as<T>(), resid, part.as<double>(), Sc);
+                                  data.as<T>(), wres_cur, resid, part.as<double>(), Sc);
         DevBuf chis(ctx, (size_t)na * 8);
         if (out->chisq)
             launch(k_chisq<T>, dim3(gstar), dim3(BLOCK), 0, st, na, p, (const double*)theta.as<double>(), fm, fit_rad,
