@@ -138,6 +138,13 @@ def main():
     o32 = make_opts(abi.CP_PREC_MIXED)
     s32, r32 = make_result(512)
     if args.profile_only:
+        # calibration: one plain device copy of 256 MiB (known read and write
+        # bytes) for scaling the FETCH_SIZE / WRITE_SIZE counters
+        a_cal = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        b_cal = torch.empty_like(a_cal)
+        b_cal.copy_(a_cal)
+        torch.cuda.synchronize()
+        del a_cal, b_cal
         fit(o32, s32)
         torch.cuda.synchronize()
         log(f"[bench] profile-only fit done: passes {s32.n_passes}")

@@ -28,11 +28,35 @@ constexpr int BLOCK = 256;
 constexpr int NWAVE = BLOCK / 64;
 constexpr int TILE_PX = 1024;           // pixels per tile
 constexpr int PPT = TILE_PX / BLOCK;    // pixels per thread
-constexpr int CHUNK = 128;              // star records staged per LDS chunk
+constexpr int RBLOCK = 1024;            // render / fill blocks share one staged ePSF grid
+constexpr int RWAVE = RBLOCK / 64;
+constexpr int RPPT = TILE_PX / RBLOCK;  // pixels per thread in the render
+constexpr int FBLOCK = 1024;            // fill blocks: two per CU with the ePSF grid in LDS
+constexpr int FWAVE = FBLOCK / 64;
 constexpr int MAX_EPL = 16;             // stamp entries per lane (S^2 <= 1024)
 constexpr double DEPS = 2.220446049250313e-16;
 
 enum : int { MODE_RESID = 0, MODE_COST = 1, MODE_FREEZE = 2, MODE_FINAL = 3 };
+
+// Stamp values V: one row of VP = 4 values (the p <= 3 free columns, zero
+// padded) per stamp pixel, so a row is a single 16-byte (float) load.  The
+// LSQR vectors read by J v (b and t, float64) use the same padded stride and
+// are zero for stars that are not live, so J v needs no live test.
+constexpr int VP = 4;
+template <typename T> struct Vec4Of;
+template <> struct Vec4Of<float> { using type = float4; };
+template <> struct Vec4Of<double> { using type = double4; };
+template <typename T> using vec4_t = typename Vec4Of<T>::type;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Renumber blocks so every XCD receives one contiguous range of logical
+// blocks: neighbouring pixel / star ranges then share an L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    constexpr int NX = 8;
+    const int per = nb / NX, rem = nb % NX;
+    const int x = bid % NX, i = bid / NX;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+}
 
 // Device-resident scalars of one fit.
 struct Scal {
@@ -69,7 +93,8 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Block sum in fixed order; result valid in thread 0.
+// Block sum in fixed order over NW waves; result valid in thread 0.
+template <int NW = NWAVE>
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -78,17 +103,19 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     __syncthreads();
     double r = 0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < NWAVE; ++i) r += sh[i];
+        for (int i = 0; i < NW; ++i) r += sh[i];
     return r;
 }
 
 // Block-sum v and store it as this block's partial (plain store; the k_finish
 // launch that follows reads it after the kernel boundary).
+template <int NW = NWAVE>
 __device__ __forceinline__ void block_store(double v, double* partials, double* sh) {
-    const double b = block_sum(v, sh);
+    const double b = block_sum<NW>(v, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = b;
 }
 
+template <int NW = NWAVE>
 __device__ __forceinline__ double block_max(double v, double* sh) {
     v = wave_max(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -97,7 +124,7 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     __syncthreads();
     double r = 0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < NWAVE; ++i) r = fmax(r, sh[i]);
+        for (int i = 0; i < NW; ++i) r = fmax(r, sh[i]);
     return r;
 }
 
@@ -455,32 +482,34 @@ __device__ __forceinline__ T staged_value(const ModelConst<T>& M, const StarRec<
 }
 
 template <typename T, int KIND, int MODE, bool LDSG>
-__global__ __launch_bounds__(BLOCK) void k_render(
+__global__ __launch_bounds__(RBLOCK) void k_render(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
     const CornerRec<T>* __restrict__ crs, const int* __restrict__ tstart, const int* __restrict__ tend,
     const int* __restrict__ list, TileGeo g, int ntiles, int tiles_per_block, const uint8_t* __restrict__ umask,
     T* __restrict__ dw, const T* __restrict__ w, const T* __restrict__ data, T* __restrict__ wt_resid,
     T* __restrict__ resid, double* __restrict__ partials, Scal* S) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double sred[NWAVE];
+    __shared__ double sred[RWAVE];
+    __shared__ int s_nuse;
+    static_assert(RCHUNK == 64, "one wave stages a chunk");
     const int ngrid = LDSG ? M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
     StarRec<T>* srec = reinterpret_cast<StarRec<T>*>(smem + ((ngrid * 4 + 15) & ~15));
     EpsfRec<T>* serec = reinterpret_cast<EpsfRec<T>*>(srec + RCHUNK);
     int* sid = reinterpret_cast<int*>(serec + ((KIND == KIND_EPSF && LDSG) ? RCHUNK : 0));
-    if (LDSG) stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, BLOCK);
+    if (LDSG) stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, RBLOCK);
     const float* grid = LDSG ? sgrid : M.grids;
     double acc = 0;
-    const int t0 = blockIdx.x * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
+    const int t0 = xcd_remap(blockIdx.x, gridDim.x) * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
     for (int tile = t0; tile < t1; ++tile) {
         const int ty = tile % g.nty, tx = tile / g.nty;
         const int gy0 = ty * g.TY + 1, gx0 = tx * g.TX + 1;
-        T val[PPT];
-        int gyv[PPT], gxv[PPT];
-        bool act[PPT];
+        T val[RPPT];
+        int gyv[RPPT], gxv[RPPT];
+        bool act[RPPT];
 #pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            int lin = threadIdx.x + k * BLOCK;
+        for (int k = 0; k < RPPT; ++k) {
+            int lin = threadIdx.x + k * RBLOCK;
             gyv[k] = gy0 + lin % g.TY;
             gxv[k] = gx0 + lin / g.TY;
             act[k] = gyv[k] <= g.ny && gxv[k] <= g.nx;
@@ -492,21 +521,34 @@ __global__ __launch_bounds__(BLOCK) void k_render(
         const int s0 = tstart[tile], s1 = tend[tile];
         bool touched = false;
         for (int c0 = s0; c0 < s1; c0 += RCHUNK) {
-            const int cn = min(RCHUNK, s1 - c0);
+            // the first wave stages the chunk's stars that this mode renders
+            // (live / newly frozen / all), compacted in list order
             __syncthreads();
-            for (int i = threadIdx.x; i < cn; i += BLOCK) {
-                int a = list[c0 + i];
-                srec[i] = recs[a];
-                if (KIND == KIND_EPSF && LDSG) serec[i] = ers[a];
-                sid[i] = a;
+            if (threadIdx.x < RCHUNK) {
+                const int i = threadIdx.x;
+                bool use = false;
+                int a = 0;
+                StarRec<T> r;
+                if (c0 + i < s1) {
+                    a = list[c0 + i];
+                    r = recs[a];
+                    use = MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (r.flags & 2) != 0 : (r.flags & 1) != 0);
+                }
+                const unsigned long long m = __ballot(use);
+                const int slot = __popcll(m & ((1ull << i) - 1ull));
+                if (use) {
+                    srec[slot] = r;
+                    if (KIND == KIND_EPSF && LDSG) serec[slot] = ers[a];
+                    sid[slot] = a;
+                }
+                if (i == 0) s_nuse = __popcll(m);
             }
             __syncthreads();
+            const int cn = s_nuse;
             for (int j = 0; j < cn; ++j) {
                 const StarRec<T> r = srec[j];
-                const bool use = MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (r.flags & 2) : (r.flags & 1));
-                if (!use) continue;
 #pragma unroll
-                for (int k = 0; k < PPT; ++k) {
+                for (int k = 0; k < RPPT; ++k) {
                     int dy = gyv[k] - r.ay, dx = gxv[k] - r.ax;
                     if (act[k] && abs(dy) <= r.mR && abs(dx) <= r.mR) {
                         T v = staged_value<T, KIND, LDSG>(M, r, serec + j, dy, dx, grid, crs + sid[j]);
@@ -521,7 +563,7 @@ __global__ __launch_bounds__(BLOCK) void k_render(
             }
         }
 #pragma unroll
-        for (int k = 0; k < PPT; ++k) {
+        for (int k = 0; k < RPPT; ++k) {
             if (!act[k]) continue;
             size_t q = (size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny;
             if (MODE == MODE_RESID || MODE == MODE_COST) {
@@ -536,7 +578,7 @@ __global__ __launch_bounds__(BLOCK) void k_render(
             }
         }
     }
-    if (MODE == MODE_RESID || MODE == MODE_COST) block_store(acc, partials, sred);
+    if (MODE == MODE_RESID || MODE == MODE_COST) block_store<RWAVE>(acc, partials, sred);
 }
 
 // ---------------------------------------------------------------------------
@@ -551,40 +593,45 @@ __device__ __forceinline__ double group_sum(double v) {
     return v;
 }
 
+// one wave per star; float fills with one or two stamp entries per lane fit
+// 64 VGPRs (8 waves / SIMD)
 template <typename T, int KIND, bool FINAL, bool LDSG, int EPL, int GL>
-__global__ __launch_bounds__(BLOCK) void k_fill(
+__global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_fill(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
     const CornerRec<T>* __restrict__ crs, int na, int R, int S, FreeMap fm, const T* __restrict__ w, int ny,
     int nx, const T* __restrict__ wt_resid, T* __restrict__ V, double* __restrict__ colnorm,
     double* __restrict__ b_scaled, uint8_t* __restrict__ gloc, double g_tol, double* __restrict__ partials,
     Scal* Sc) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double sred[NWAVE];
+    __shared__ double sred[FWAVE];
     float* sgrid = reinterpret_cast<float*>(smem);
     if (LDSG) {
-        stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, BLOCK);
+        stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, FBLOCK);
         __syncthreads();
     }
     const float* grid = LDSG ? sgrid : M.grids;
-    constexpr int GPB = BLOCK / GL;   // stars in flight per block
+    constexpr int GPB = FBLOCK / GL;   // stars in flight per block
     const int sub = threadIdx.x % GL;
     const int S2 = S * S;
     const int p = fm.p;
     const double C_r = FINAL ? 0.0 : Sc->cost / Sc->dof;
     double bsq = 0, gmax = 0;
-    for (int a = blockIdx.x * GPB + threadIdx.x / GL; a < na; a += gridDim.x * GPB) {
+    for (int a = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL; a < na; a += gridDim.x * GPB) {
         const StarRec<T> r = recs[a];
-        if (!FINAL && !(r.flags & 1)) continue;   // uniform within the group
+        if (!FINAL && !(r.flags & 1)) {   // uniform within the group
+            if (sub < VP) b_scaled[(size_t)a * VP + sub] = 0.0;
+            continue;
+        }
         EpsfRec<T> er;
         if (KIND == KIND_EPSF && LDSG) er = ers[a];
+        // raw weighted gradient columns; b = J^T r is accumulated from the raw
+        // columns and scaled by 1 / |column| once the norms are known
         T raw[3][EPL];
-        T wres[EPL];
-        double cn2[3] = {0, 0, 0};
+        double cn2[3] = {0, 0, 0}, braw[3] = {0, 0, 0};
 #pragma unroll
         for (int e = 0; e < EPL; ++e) {
             const int m = sub + GL * e;
             raw[0][e] = raw[1][e] = raw[2][e] = T(0);
-            wres[e] = T(0);
             const int dy = (m % S) - R, dx = (m / S) - R;
             const int gy = r.ay + dy, gx = r.ax + dx;
             const bool on = m < S2 && gy >= 1 && gy <= ny && gx >= 1 && gx <= nx;
@@ -602,6 +649,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill(
                 star_grad<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a, g3[0], g3[1], g3[2]);
             }
             const T sw = sqrt(wv);
+            const double wr = FINAL ? 0.0 : (double)wt_resid[q];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 if (k < p) {
@@ -610,28 +658,31 @@ __global__ __launch_bounds__(BLOCK) void k_fill(
                     T v = sw * gv;
                     raw[k][e] = v;
                     cn2[k] += (double)v * (double)v;
+                    braw[k] += (double)v * wr;
                 }
             }
-            if (!FINAL) wres[e] = wt_resid[q];
         }
-        double cn[3] = {1, 1, 1}, bk[3] = {0, 0, 0};
+        double cn[3] = {1, 1, 1}, bk[3] = {0, 0, 0}, inv[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (k < p) {
                 double c = sqrt(group_sum<GL>(cn2[k]));
                 cn[k] = c < DEPS ? DEPS : c;
-                double inv = 1.0 / cn[k];
-                double b = 0;
+                inv[k] = 1.0 / cn[k];
+                bk[k] = FINAL ? 0.0 : group_sum<GL>(braw[k]) * inv[k];
+            }
+        }
+        vec4_t<T>* Vrow = reinterpret_cast<vec4_t<T>*>(V) + (size_t)a * S2;
 #pragma unroll
-                for (int e = 0; e < EPL; ++e) {
-                    const int m = sub + GL * e;
-                    if (m < S2) {
-                        T v = (T)((double)raw[k][e] * inv);
-                        V[((size_t)a * S2 + m) * p + k] = v;
-                        b += (double)v * (double)wres[e];
-                    }
-                }
-                bk[k] = FINAL ? 0.0 : group_sum<GL>(b);
+        for (int e = 0; e < EPL; ++e) {
+            const int m = sub + GL * e;
+            if (m < S2) {
+                vec4_t<T> v;
+                v.x = (T)((double)raw[0][e] * inv[0]);
+                v.y = (T)((double)raw[1][e] * inv[1]);
+                v.z = (T)((double)raw[2][e] * inv[2]);
+                v.w = T(0);
+                Vrow[m] = v;
             }
         }
         if (sub == 0) {
@@ -641,7 +692,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill(
                 if (k < p) {
                     colnorm[(size_t)a * p + k] = cn[k];
                     if (!FINAL) {
-                        b_scaled[(size_t)a * p + k] = bk[k];
+                        b_scaled[(size_t)a * VP + k] = bk[k];
                         bsq += bk[k] * bk[k];
                         double bt = cn[k] * bk[k];
                         gn = fmax(gn, fabs(bt) / (sqrt(cn[k] * cn[k] * C_r) + DEPS));
@@ -656,8 +707,8 @@ __global__ __launch_bounds__(BLOCK) void k_fill(
     }
     if (!FINAL) {
         // partials[0, nb): |b|^2 per block, partials[nb, 2 nb): max g-cosine per block
-        block_store(bsq, partials, sred);
-        const double m = block_max(gmax, sred);
+        block_store<FWAVE>(bsq, partials, sred);
+        const double m = block_max<FWAVE>(gmax, sred);
         if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = m;
     }
 }
@@ -689,25 +740,62 @@ static __global__ void k_lsqr_init(Scal* S, double lam, int itmax, double atol, 
     S->done = (beta1 == 0 || alpha == 0);
 }
 
-// u <- J v - alpha u over the tile's fit list (apply_J! :95-133 as a gather).
-// The V blocks (p x S^2 values) of a chunk of CJ stars are copied into LDS
-// with coalesced loads, then every pixel sums its covering stars from LDS.
+// ---------------------------------------------------------------------------
+// pixel-major stamp structure for J v.  The stamp anchors are fixed for the
+// whole fit, so every union pixel's covering (star, stamp entry) rows are
+// listed once, in active (Morton) star order: J v is then a streaming sparse
+// product where each pixel reads only its own rows of the star-major V.
+// ---------------------------------------------------------------------------
+constexpr unsigned int CSR_NONE = 0xffffffffu;
+
+// key = image index of stamp entry i = a S^2 + m (CSR_NONE off-image or w == 0).
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_Jv(
-    const int* __restrict__ tstart, const int* __restrict__ tend, const int* __restrict__ list, TileGeo g,
-    int ntiles, int tiles_per_block, const int* __restrict__ ay, const int* __restrict__ ax,
-    const uint8_t* __restrict__ live, int R, int S, int p, int CJ, const T* __restrict__ V,
-    const double* __restrict__ tsrc, const T* __restrict__ usrc, T* __restrict__ ubuf,
-    const uint8_t* __restrict__ umask, int first, double* __restrict__ partials, Scal* Sc) {
+static __global__ void k_csr_pairs(int na, const int* __restrict__ ay, const int* __restrict__ ax, int R, int S,
+                                   const T* __restrict__ w, int ny, int nx, unsigned int* __restrict__ key,
+                                   int* __restrict__ val) {
+    const int S2 = S * S;
+    const int64_t n = (int64_t)na * S2;
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const int a = (int)(i / S2), m = (int)(i - (int64_t)a * S2);
+        const int gy = ay[a] + (m % S) - R, gx = ax[a] + (m / S) - R;
+        unsigned int k = CSR_NONE;
+        if (gy >= 1 && gy <= ny && gx >= 1 && gx <= nx) {
+            const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * ny;
+            if (w[q] > T(0)) k = (unsigned int)q;
+        }
+        key[i] = k;
+        val[i] = (int)i;
+    }
+}
+
+// Run heads of the sorted keys, and E = number of entries before the first CSR_NONE.
+static __global__ void k_csr_heads(const unsigned int* __restrict__ key, int64_t n, uint8_t* __restrict__ head,
+                                   int* __restrict__ E) {
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const unsigned int k = key[i];
+        head[i] = k != CSR_NONE && (i == 0 || key[i - 1] != k);
+        if (k != CSR_NONE && (i + 1 == n || key[i + 1] == CSR_NONE)) *E = (int)(i + 1);
+    }
+}
+
+static __global__ void k_csr_pix(const unsigned int* __restrict__ key, int* __restrict__ ptr, int U,
+                                 const int* __restrict__ E, int* __restrict__ pix) {
+    const int u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u < U) pix[u] = (int)key[ptr[u]];
+    if (u == 0) ptr[U] = *E;
+}
+
+// u <- J v - alpha u over the union pixels (apply_J! :95-133 as a gather).
+// Row r = a S^2 + m of V belongs to star a = floor(r / S^2), computed as
+// floor(r * (1/S^2) + 0.5/S^2) in float64 (exact for r < 2^31).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_Jv(int U, const int* __restrict__ pix, const int* __restrict__ ptr,
+                                              const int* __restrict__ ent, int S2, const T* __restrict__ V,
+                                              const double* __restrict__ tsrc,
+                                              const T* __restrict__ usrc, T* __restrict__ ubuf, int first,
+                                              double* __restrict__ partials, Scal* Sc) {
     if (Sc->done) return;
-    extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[NWAVE];
-    const int S2 = S * S, SP = S2 * p;
-    T* sV = reinterpret_cast<T*>(smem);
-    double* sv = reinterpret_cast<double*>(smem + (((size_t)CJ * SP * sizeof(T) + 15) & ~(size_t)15));
-    int* sa = reinterpret_cast<int*>(sv + 3 * CJ);
-    int* say = sa + CJ;
-    int* sax = say + CJ;
     const double alpha = Sc->alpha;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
     double tfac, ufac;
@@ -718,67 +806,24 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(
         tfac = ia;
         ufac = Sc->beta_cur != 0 ? 1.0 / Sc->beta_cur : 1.0;
     }
+    const double inv_s2 = 1.0 / S2, half_s2 = 0.5 / S2;
+    const vec4_t<T>* V4 = reinterpret_cast<const vec4_t<T>*>(V);
+    const double4* t4 = reinterpret_cast<const double4*>(tsrc);
     double acc2 = 0;
-    const int t0 = blockIdx.x * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
-    for (int tile = t0; tile < t1; ++tile) {
-        const int ty = tile % g.nty, tx = tile / g.nty;
-        const int gy0 = ty * g.TY + 1, gx0 = tx * g.TX + 1;
-        int gyv[PPT], gxv[PPT];
-        bool act[PPT];
-        double yv[PPT];
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            int lin = threadIdx.x + k * BLOCK;
-            gyv[k] = gy0 + lin % g.TY;
-            gxv[k] = gx0 + lin / g.TY;
-            act[k] = gyv[k] <= g.ny && gxv[k] <= g.nx;
-            if (act[k]) act[k] = umask[(size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny] != 0;
-            yv[k] = 0;
+    for (int u = blockIdx.x * BLOCK + threadIdx.x; u < U; u += gridDim.x * BLOCK) {
+        const int q = pix[u];
+        const int e0 = ptr[u], e1 = ptr[u + 1];
+        double y = 0;
+        for (int e = e0; e < e1; ++e) {
+            const int r = ent[e];
+            const int a = (int)fma((double)r, inv_s2, half_s2);
+            const vec4_t<T> v = V4[r];
+            const double4 t = t4[a];
+            y += (double)v.x * t.x + (double)v.y * t.y + (double)v.z * t.z;
         }
-        const int s0 = tstart[tile], s1 = tend[tile];
-        for (int c0 = s0; c0 < s1; c0 += CJ) {
-            const int cn = min(CJ, s1 - c0);
-            __syncthreads();
-            for (int i = threadIdx.x; i < cn; i += BLOCK) {
-                int a = list[c0 + i];
-                bool lv = live[a];
-                sa[i] = lv ? a : -1;
-                say[i] = ay[a];
-                sax[i] = ax[a];
-                for (int k = 0; k < 3; ++k) sv[3 * i + k] = (lv && k < p) ? tsrc[(size_t)a * p + k] * tfac : 0.0;
-            }
-            __syncthreads();
-            for (int idx = threadIdx.x; idx < cn * SP; idx += BLOCK) {
-                const int j = idx / SP, o = idx - j * SP;
-                const int a = sa[j];
-                if (a >= 0) sV[idx] = V[(size_t)a * SP + o];
-            }
-            __syncthreads();
-            for (int j = 0; j < cn; ++j) {
-                if (sa[j] < 0) continue;
-                const int sy = say[j], sx = sax[j];
-                const double v0 = sv[3 * j], v1 = sv[3 * j + 1], v2 = sv[3 * j + 2];
-#pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    int dy = gyv[k] - sy, dx = gxv[k] - sx;
-                    if (act[k] && abs(dy) <= R && abs(dx) <= R) {
-                        const T* Vm = sV + j * SP + ((dx + R) * S + (dy + R)) * p;
-                        double s = (double)Vm[0] * v0;
-                        if (p > 1) s += (double)Vm[1] * v1;
-                        if (p > 2) s += (double)Vm[2] * v2;
-                        yv[k] += s;
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            if (!act[k]) continue;
-            size_t q = (size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny;
-            T u = (T)(yv[k] - alpha * ((double)usrc[q] * ufac));
-            ubuf[q] = u;
-            acc2 += (double)u * (double)u;
-        }
+        const T uu = (T)(y * tfac - alpha * ((double)usrc[q] * ufac));
+        ubuf[q] = uu;
+        acc2 += (double)uu * (double)uu;
     }
     block_store(acc2, partials, sred);
 }
@@ -786,7 +831,7 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(
 // v <- J^T u - beta v per star, with the pending x/w update of the previous
 // iteration folded in; the last block advances the LSQR recurrences.  GL
 // lanes per star (GL = 16 for stamps up to 16 x 16, else 64).
-template <typename T, int GL>
+template <typename T, int EPL, int GL>
 __global__ __launch_bounds__(BLOCK) void k_JTu(
     int na, const int* __restrict__ ay, const int* __restrict__ ax, const uint8_t* __restrict__ live,
     int R, int S, int p, int ny, int nx, const T* __restrict__ V, const T* __restrict__ ubuf,
@@ -804,44 +849,56 @@ __global__ __launch_bounds__(BLOCK) void k_JTu(
     const int sub = threadIdx.x % GL;
     const int S2 = S * S;
     double tsq = 0, wsq = 0;
-    for (int a = blockIdx.x * GPB + threadIdx.x / GL; a < na; a += gridDim.x * GPB) {
+    const int a = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL;
+    if (a < na) {
         const size_t base = (size_t)a * p;
         if (!live[a]) {
             if (first && sub < p) {
                 x[base + sub] = 0;
                 wv[base + sub] = 0;
-                tbuf[base + sub] = 0;
+                tbuf[(size_t)a * VP + sub] = 0;
             }
-            continue;
-        }
-        const int sy = ay[a], sx = ax[a];
-        double acc0 = 0, acc1 = 0, acc2 = 0;
-        for (int m = sub; m < S2; m += GL) {
-            int gy = sy + (m % S) - R, gx = sx + (m / S) - R;
-            if (gy < 1 || gy > ny || gx < 1 || gx > nx) continue;
-            const double u = (double)ubuf[(size_t)(gy - 1) + (size_t)(gx - 1) * ny];
-            const T* Vm = V + ((size_t)a * S2 + m) * p;
-            acc0 += (double)Vm[0] * u;
-            if (p > 1) acc1 += (double)Vm[1] * u;
-            if (p > 2) acc2 += (double)Vm[2] * u;
-        }
-        acc0 = group_sum<GL>(acc0);
-        acc1 = group_sum<GL>(acc1);
-        acc2 = group_sum<GL>(acc2);
-        if (sub < p) {
-            const int k = sub;
-            const double acck = k == 0 ? acc0 : (k == 1 ? acc1 : acc2);
-            const double told = first ? -b_scaled[base + k] * ib1 : tbuf[base + k];
-            const double v = told * ia;
-            const double xo = first ? 0.0 : x[base + k];
-            const double wo = first ? 0.0 : wv[base + k];
-            x[base + k] = xo + sig * wo;
-            const double wn = v - tau * wo;
-            wv[base + k] = wn;
-            wsq += wn * wn;
-            const double tn = beta != 0 ? acck * ib - beta * v : told;
-            tbuf[base + k] = tn;
-            tsq += tn * tn;
+        } else {
+            const int sy = ay[a], sx = ax[a];
+            double acc0 = 0, acc1 = 0, acc2 = 0;
+            T uv[EPL], v0[EPL], v1[EPL], v2[EPL];
+#pragma unroll
+            for (int e = 0; e < EPL; ++e) {
+                const int m = sub + GL * e;
+                const int gy = sy + (m % S) - R, gx = sx + (m / S) - R;
+                const bool on = m < S2 && gy >= 1 && gy <= ny && gx >= 1 && gx <= nx;
+                uv[e] = on ? ubuf[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] : T(0);
+                const vec4_t<T> vr = reinterpret_cast<const vec4_t<T>*>(V)[(size_t)a * S2 + (m < S2 ? m : 0)];
+                v0[e] = vr.x;
+                v1[e] = vr.y;
+                v2[e] = vr.z;
+            }
+#pragma unroll
+            for (int e = 0; e < EPL; ++e) {
+                const double u = (double)uv[e];
+                acc0 += (double)v0[e] * u;
+                acc1 += (double)v1[e] * u;
+                acc2 += (double)v2[e] * u;
+            }
+            acc0 = group_sum<GL>(acc0);
+            acc1 = group_sum<GL>(acc1);
+            acc2 = group_sum<GL>(acc2);
+            if (sub < p) {
+                const int k = sub;
+                const double acck = k == 0 ? acc0 : (k == 1 ? acc1 : acc2);
+                const size_t tk = (size_t)a * VP + k;
+                const double told = first ? -b_scaled[tk] * ib1 : tbuf[tk];
+                const double v = told * ia;
+                const double xo = first ? 0.0 : x[base + k];
+                const double wo = first ? 0.0 : wv[base + k];
+                x[base + k] = xo + sig * wo;
+                const double wn = v - tau * wo;
+                wv[base + k] = wn;
+                wsq += wn * wn;
+                const double tn = beta != 0 ? acck * ib - beta * v : told;
+                tbuf[tk] = tn;
+                tsq += tn * tn;
+            }
         }
     }
     block_store(tsq, partials, sred);
@@ -1060,7 +1117,7 @@ static __global__ void k_errors(int na, int R, int S, const int* __restrict__ ay
         int gy = ay[a] + (m % S) - R, gx = ax[a] + (m / S) - R;
         if (gy < 1 || gy > ny || gx < 1 || gx > nx) continue;
         if (!(w[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] > T(0))) continue;
-        const T* Vm = V + ((size_t)a * S2 + m) * p;
+        const T* Vm = V + ((size_t)a * S2 + m) * VP;
 #pragma unroll
         for (int k = 0; k < p; ++k)
 #pragma unroll

@@ -109,6 +109,63 @@ void build_tiles(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
            np, L.start.as<int>(), L.end.as<int>());
 }
 
+struct PixCSR {
+    DevBuf pix, ptr, ent;
+    int U = 0, E = 0;
+};
+
+static int bit_length(int64_t v);
+
+// Union pixels with their covering stamp rows (a S^2 + m), rows of each pixel
+// in active star order (stable key sort of rows generated in star order).
+template <typename T>
+void build_pixel_csr(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax, int R, int S, const T* d_w,
+                     int ny, int nx, PixCSR& C) {
+    hipStream_t st = ctx->stream;
+    const int64_t n = (int64_t)na * S * S;
+    CP_ARG(n < (int64_t)INT_MAX, "too many stamp pixels for one fit (n_stars * (2 ceil(fit_rad) + 1)^2 >= 2^31)");
+    CP_ARG((int64_t)ny * nx < (int64_t)INT_MAX, "image too large for one fit (>= 2^31 pixels)");
+    C.pix.alloc(ctx, (size_t)std::max<int64_t>(n, 1) * 4);
+    C.ptr.alloc(ctx, (size_t)(n + 1) * 4);
+    C.ent.alloc(ctx, (size_t)std::max<int64_t>(n, 1) * 4);
+    C.U = C.E = 0;
+    if (n == 0) return;
+    DevBuf k1(ctx, (size_t)n * 4), k2(ctx, (size_t)n * 4), v1(ctx, (size_t)n * 4), head(ctx, (size_t)n),
+        dE(ctx, 8), nsel(ctx, 8);
+    CP_HIP(hipMemsetAsync(dE.get(), 0, 8, st));
+    const unsigned int gpairs = (unsigned int)std::min<int64_t>(cdiv(n, BLOCK), 65536);
+    launch(k_csr_pairs<T>, dim3(gpairs), dim3(BLOCK), 0, st, na, d_ay, d_ax, R, S, d_w, ny, nx,
+           k1.as<unsigned int>(), v1.as<int>());
+    // q <= ny nx - 1 < 2^end_bit - 1, so CSR_NONE (all ones in the sorted bits) sorts last
+    const int end_bit = std::min(32, bit_length((int64_t)ny * nx));
+    size_t b = 0;
+    CP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, k1.as<unsigned int>(), k2.as<unsigned int>(),
+                                              v1.as<int>(), C.ent.as<int>(), (int)n, 0, end_bit, st));
+    cub.ensure(b);
+    b = cub.tmp.bytes();
+    CP_HIP(hipcub::DeviceRadixSort::SortPairs(cub.tmp.get(), b, k1.as<unsigned int>(), k2.as<unsigned int>(),
+                                              v1.as<int>(), C.ent.as<int>(), (int)n, 0, end_bit, st));
+    launch(k_csr_heads, dim3(gpairs), dim3(BLOCK), 0, st, (const unsigned int*)k2.as<unsigned int>(), n,
+           head.as<uint8_t>(), dE.as<int>());
+    hipcub::CountingInputIterator<int> it(0);
+    b = 0;
+    CP_HIP(hipcub::DeviceSelect::Flagged(nullptr, b, it, head.as<uint8_t>(), C.ptr.as<int>(), nsel.as<int>(),
+                                         (int)n, st));
+    cub.ensure(b);
+    b = cub.tmp.bytes();
+    CP_HIP(hipcub::DeviceSelect::Flagged(cub.tmp.get(), b, it, head.as<uint8_t>(), C.ptr.as<int>(),
+                                         nsel.as<int>(), (int)n, st));
+    int hv[2] = {0, 0};
+    CP_HIP(hipMemcpyAsync(&hv[0], nsel.get(), 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipMemcpyAsync(&hv[1], dE.get(), 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipStreamSynchronize(st));
+    C.U = hv[0];
+    C.E = hv[1];
+    launch(k_csr_pix, dim3(cdiv(std::max(C.U, 1), BLOCK)), dim3(BLOCK), 0, st,
+           (const unsigned int*)k2.as<unsigned int>(), C.ptr.as<int>(), C.U, (const int*)dE.as<int>(),
+           C.pix.as<int>());
+}
+
 static int bit_length(int64_t v) {
     int b = 1;
     while ((1LL << b) <= v) ++b;
@@ -179,16 +236,30 @@ struct Sim {
                      double gtol, double* part, Scal* Sc) {
         const char* nm = FINAL ? "k_fill_final" : "k_fill";
 #define CP_FILL(E, G)                                                                                        \
-    launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G>, grid, dim3(BLOCK), sh, st, M, recs, ers, crs, na, R, S, fm, \
+    launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G>, grid, dim3(FBLOCK), sh, st, M, recs, ers, crs, na, R, S, fm, \
              w, ny, nx, wres, V, cn, b, gloc, gtol, part, Sc)
-        if (S2 <= 16) CP_FILL(1, 16);
-        else if (S2 <= 32) CP_FILL(2, 16);
-        else if (S2 <= 64) CP_FILL(4, 16);
-        else if (S2 <= 128) CP_FILL(8, 16);
-        else if (S2 <= 256) CP_FILL(16, 16);
+        if (S2 <= 64) CP_FILL(1, 64);
+        else if (S2 <= 128) CP_FILL(2, 64);
+        else if (S2 <= 256) CP_FILL(4, 64);
         else if (S2 <= 512) CP_FILL(8, 64);
         else CP_FILL(16, 64);
 #undef CP_FILL
+    }
+
+    void jtu_launch(int S2, dim3 grid, int na, const int* ay, const int* ax, const uint8_t* live, int R, int S, int p,
+                    int ny, int nx, const T* V, const T* u, const double* b, double* t, double* x, double* wv,
+                    int first, int it, double* part, double* part2, Scal* Sc) {
+#define CP_JTU(E, G)                                                                                          \
+    launch_n("k_JTu", k_JTu<T, E, G>, grid, dim3(BLOCK), 0, st, na, ay, ax, live, R, S, p, ny, nx, V, u, b, t, x, wv, \
+             first, it, part, part2, Sc)
+        if (S2 <= 16) CP_JTU(1, 16);
+        else if (S2 <= 32) CP_JTU(2, 16);
+        else if (S2 <= 64) CP_JTU(4, 16);
+        else if (S2 <= 128) CP_JTU(8, 16);
+        else if (S2 <= 256) CP_JTU(16, 16);
+        else if (S2 <= 512) CP_JTU(8, 64);
+        else CP_JTU(16, 64);
+#undef CP_JTU
     }
 
     template <int MODE>
@@ -198,11 +269,11 @@ struct Sim {
                        T* resid, double* part, Scal* Sc) {
         static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
         if (ldsg)
-            launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(BLOCK), sh, st, M, recs, ers, crs,
+            launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
                      ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
         else
-            launch_n(nm[MODE], k_render<T, KIND, MODE, false>, grid, dim3(BLOCK), sh, st, M, recs, ers, crs,
+            launch_n(nm[MODE], k_render<T, KIND, MODE, false>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
                      ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
     }
@@ -362,15 +433,14 @@ struct Sim {
         CP_HIP(hipMemcpyAsync(&Sc->dof, &dof, 8, hipMemcpyHostToDevice, st));
 
         // --- tiles: fit list (stamp box) and model list (model box)
-        // model-box lists (render passes) use TYm-row tiles, stamp lists (J v) 32-row tiles
+        // model-box tile lists (render passes) and the pixel-major stamp structure (J v)
         int TYm = 16;
         if (o.tile == 8 || o.tile == 16 || o.tile == 32 || o.tile == 64 || o.tile == 128) TYm = o.tile;
-        const int TYf = 32;
         TileGeo g{TYm, TILE_PX / TYm, (ny + TYm - 1) / TYm, (nx + TILE_PX / TYm - 1) / (TILE_PX / TYm), ny, nx};
-        TileGeo gf{TYf, TILE_PX / TYf, (ny + TYf - 1) / TYf, (nx + TILE_PX / TYf - 1) / (TILE_PX / TYf), ny, nx};
-        const int ntiles = g.nty * g.ntx, ntiles_f = gf.nty * gf.ntx;
-        TileLists Lfit, Lmod;
-        build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), nullptr, R, gf, Lfit);
+        const int ntiles = g.nty * g.ntx;
+        TileLists Lmod;
+        PixCSR csr;
+        build_pixel_csr<T>(ctx, cub, na, ay.as<int>(), ax.as<int>(), R, S, (const T*)w.as<T>(), ny, nx, csr);
         build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), o.model_rad > 0 ? nullptr : mR.as<int>(),
                     mR_scalar, g, Lmod);
         cub.tmp.release();
@@ -383,15 +453,17 @@ struct Sim {
         T* wres_cur = wres_a.as<T>();
         T* wres_cand = wres_b.as<T>();
         CP_HIP(hipMemsetAsync(ubuf.get(), 0, Pn * sizeof(T), st));
-        DevBuf V(ctx, (size_t)na * S2 * p * sizeof(T));
+        DevBuf V(ctx, (size_t)na * S2 * VP * sizeof(T));
         CP_HIP(hipMemsetAsync(V.get(), 0, V.bytes(), st));
         const size_t nb8 = (size_t)na * p * 8;
-        DevBuf colnorm(ctx, nb8), D(ctx, nb8), bsc(ctx, nb8), xs(ctx, nb8), wl(ctx, nb8), tb(ctx, nb8), dsc(ctx, nb8),
+        const size_t nb32 = (size_t)na * VP * 8;   // padded float64 LSQR vectors read by J v
+        DevBuf colnorm(ctx, nb8), D(ctx, nb8), bsc(ctx, nb32), xs(ctx, nb8), wl(ctx, nb8), tb(ctx, nb32), dsc(ctx, nb8),
             theta_c(ctx, nb8);
         CP_HIP(hipMemsetAsync(colnorm.get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(xs.get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(wl.get(), 0, nb8, st));
-        CP_HIP(hipMemsetAsync(tb.get(), 0, nb8, st));
+        CP_HIP(hipMemsetAsync(tb.get(), 0, nb32, st));
+        CP_HIP(hipMemsetAsync(bsc.get(), 0, nb32, st));
         DevBuf gloc(ctx, na), recs(ctx, (size_t)na * sizeof(StarRec<T>)), recs_c(ctx, (size_t)na * sizeof(StarRec<T>));
         DevBuf crs, crs_c, ers, ers_c;
         const bool multi = (KIND == KIND_EPSF && P.n_nodes > 1);
@@ -405,8 +477,6 @@ struct Sim {
             crs.alloc(ctx, sizeof(CornerRec<T>));
             crs_c.alloc(ctx, sizeof(CornerRec<T>));
         }
-        const int maxpb = (int)std::max<int64_t>((int64_t)n_cu * 16, cdiv(na, BLOCK));
-        DevBuf part(ctx, (size_t)2 * maxpb * 8), part2(ctx, (size_t)maxpb * 8);
 
         const ModelConst<T> M = model_const<T>(P, DP, o.fixed_bkg);
         const double inv_norm_g64 = (KIND == KIND_GAUSS) ? 1.0 / (-(M_PI * P.fwhm * P.fwhm / CP_GAUSS_PRE)) : 0.0;
@@ -417,21 +487,20 @@ struct Sim {
         const size_t render_sh =
             grid_sh + RCHUNK * (sizeof(StarRec<T>) + (ldsg ? sizeof(EpsfRec<T>) : 0) + 4);
         const size_t fill_sh = grid_sh;
-        // J v: stamp values of CJ stars per LDS chunk
-        const size_t vbytes = (size_t)S2 * p * sizeof(T);
-        const int CJ = (int)std::max<size_t>(1, std::min<size_t>(64, (40 * 1024) / vbytes));
-        const size_t jv_sh = (((size_t)CJ * vbytes + 15) & ~(size_t)15) + (size_t)CJ * (3 * 8 + 3 * 4);
 
-        const int nb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 4);
+        // two 1024-thread render / fill blocks fit one CU (LDS and wave slots)
+        const int nb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 2);
         const int tpb_render = (int)cdiv(ntiles, nb_render);
         const int gb_render = (int)cdiv(ntiles, tpb_render);
-        const int nb_jv = (int)std::min<int64_t>(ntiles_f, (int64_t)n_cu * 8);
-        const int tpb_jv = (int)cdiv(ntiles_f, nb_jv);
-        const int gb_jv = (int)cdiv(ntiles_f, tpb_jv);
+        const int gb_jv = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * 8));
         const int GLs = S2 <= 256 ? 16 : 64;
-        const int gb_fill = (int)std::min<int64_t>(cdiv(na, BLOCK / GLs), (int64_t)n_cu * 4);
-        const int gb_jtu = (int)std::min<int64_t>(cdiv(na, BLOCK / GLs), (int64_t)n_cu * 16);
+        const int gb_fill = (int)std::min<int64_t>(cdiv(na, FBLOCK / 64), (int64_t)n_cu * 2);
+        const int gb_jtu = (int)cdiv(na, BLOCK / GLs);
         const unsigned int gstar = cdiv(na, BLOCK);
+        // per-block partials of the widest launch (two sets for fill / accept)
+        const int maxpb = (int)std::max<int64_t>(
+            {(int64_t)gb_render, (int64_t)gb_fill, (int64_t)gb_jv, (int64_t)gb_jtu, (int64_t)gstar});
+        DevBuf part(ctx, (size_t)2 * maxpb * 8), part2(ctx, (size_t)maxpb * 8);
 
         auto star_rec = [&](const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter, int all_live) {
             launch(k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, th, fm, M, inv_norm_g64, moffat_amp64,
@@ -504,26 +573,15 @@ struct Sim {
                        o.linear_tol);
                 for (int it = 1; it <= o.inner_iterations; ++it) {
                     const int first = it == 1;
-                    launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), jv_sh, st, (const int*)Lfit.start.as<int>(),
-                             (const int*)Lfit.end.as<int>(), (const int*)Lfit.list.as<int>(), gf, ntiles_f, tpb_jv,
-                             (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R,
-                             S, p, CJ, (const T*)V.as<T>(),
+                    launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, csr.U, (const int*)csr.pix.as<int>(),
+                             (const int*)csr.ptr.as<int>(), (const int*)csr.ent.as<int>(), S2, (const T*)V.as<T>(),
                              (const double*)(first ? bsc.as<double>() : tb.as<double>()),
-                             (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(),
-                             (const uint8_t*)umask.as<uint8_t>(), first, part.as<double>(), Sc);
+                             (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(), first, part.as<double>(), Sc);
                     finish(k_finish<FIN_BETA>, gb_jv, it);
-                    if (GLs == 16)
-                        launch_n("k_JTu", k_JTu<T, 16>, dim3(gb_jtu), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
-                                 (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
-                                 (const T*)V.as<T>(), (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(),
-                                 tb.as<double>(), xs.as<double>(), wl.as<double>(), first, it, part.as<double>(),
-                                 part2.as<double>(), Sc);
-                    else
-                        launch_n("k_JTu", k_JTu<T, 64>, dim3(gb_jtu), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
-                                 (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
-                                 (const T*)V.as<T>(), (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(),
-                                 tb.as<double>(), xs.as<double>(), wl.as<double>(), first, it, part.as<double>(),
-                                 part2.as<double>(), Sc);
+                    jtu_launch(S2, dim3(gb_jtu), na, (const int*)ay.as<int>(), (const int*)ax.as<int>(),
+                               (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx, (const T*)V.as<T>(),
+                               (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(), tb.as<double>(),
+                               xs.as<double>(), wl.as<double>(), first, it, part.as<double>(), part2.as<double>(), Sc);
                     finish(k_finish<FIN_LSQR>, gb_jtu, it);
                 }
                 launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
