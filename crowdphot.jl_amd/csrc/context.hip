@@ -85,6 +85,7 @@ void cp_destroy(cp_ctx* c) {
         (void)hipStreamDestroy(c->stream);
     }
     if (c->pinned) (void)hipHostFree(c->pinned);
+    cp::cache_trim(c, UINT64_MAX);
     for (auto& p : c->prof_pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);

@@ -42,6 +42,10 @@ struct cp_ctx {
     uint64_t budget = 0;   // 0 = no cap
     uint64_t in_use = 0;   // bytes currently allocated through DevBuf
     uint64_t peak = 0;
+    // released device blocks kept for reuse (size -> pointer); every use is
+    // ordered on `stream`, so a block can be handed out again immediately
+    std::multimap<size_t, void*> cache;
+    uint64_t cached = 0;
     std::string err;
     // pinned host staging for small scalar read-backs
     void* pinned = nullptr;
@@ -60,6 +64,39 @@ struct cp_ctx {
 
 namespace cp {
 
+// Allocation granule: small requests round to a power of two (>= 256 B),
+// larger ones to 2 MiB, so repeated calls with similar sizes hit the cache.
+inline size_t alloc_granule(size_t b) {
+    if (b <= (size_t(1) << 21)) {
+        size_t g = 256;
+        while (g < b) g <<= 1;
+        return g;
+    }
+    const size_t m = size_t(1) << 21;
+    return (b + m - 1) / m * m;
+}
+
+inline void cache_trim(cp_ctx* c, uint64_t need) {
+    // free cached blocks, largest first, until `need` more bytes fit the budget
+    while (!c->cache.empty() && (need == UINT64_MAX || c->in_use + c->cached + need > c->budget)) {
+        auto it = std::prev(c->cache.end());
+        (void)hipFree(it->second);
+        c->cached -= it->first;
+        c->cache.erase(it);
+    }
+}
+
+// A cached block of at least g and at most 1.25 g bytes, or null; its size in *got.
+inline void* cache_take(cp_ctx* c, size_t g, size_t* got) {
+    auto it = c->cache.lower_bound(g);
+    if (it == c->cache.end() || it->first > g + g / 4) return nullptr;
+    void* p = it->second;
+    *got = it->first;
+    c->cached -= it->first;
+    c->cache.erase(it);
+    return p;
+}
+
 // Device allocation charged against the context's HBM budget.
 class DevBuf {
    public:
@@ -75,8 +112,10 @@ class DevBuf {
             ctx_ = o.ctx_;
             ptr_ = o.ptr_;
             bytes_ = o.bytes_;
+            gbytes_ = o.gbytes_;
             o.ptr_ = nullptr;
             o.bytes_ = 0;
+            o.gbytes_ = 0;
         }
         return *this;
     }
@@ -84,26 +123,43 @@ class DevBuf {
         release();
         ctx_ = c;
         if (bytes == 0) return;
-        if (c->budget && c->in_use + bytes > c->budget)
-            throw Error(CP_ENOMEM, "HBM budget exceeded: need " + std::to_string(bytes) +
+        size_t g = alloc_granule(bytes);
+        if (c->budget && c->in_use + g > c->budget)
+            throw Error(CP_ENOMEM, "HBM budget exceeded: need " + std::to_string(g) +
                                        " more bytes, " + std::to_string(c->budget - c->in_use) +
                                        " left of " + std::to_string(c->budget));
-        hipError_t e = hipMalloc(&ptr_, bytes);
-        if (e != hipSuccess) {
-            ptr_ = nullptr;
-            throw Error(CP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        size_t got = 0;
+        void* p = cache_take(c, g, &got);
+        if (p) {
+            g = got;   // charge the block's real size
+        } else {
+            if (c->budget) cache_trim(c, g);
+            hipError_t e = hipMalloc(&p, g);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                cache_trim(c, UINT64_MAX);
+                e = hipMalloc(&p, g);
+            }
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(CP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            }
         }
+        ptr_ = p;
         bytes_ = bytes;
-        c->in_use += bytes;
+        gbytes_ = p ? g : 0;
+        c->in_use += g;
         if (c->in_use > c->peak) c->peak = c->in_use;
     }
     void release() {
         if (ptr_) {
-            (void)hipFree(ptr_);
-            ctx_->in_use -= bytes_;
+            ctx_->in_use -= gbytes_;
+            ctx_->cache.emplace(gbytes_, ptr_);
+            ctx_->cached += gbytes_;
         }
         ptr_ = nullptr;
         bytes_ = 0;
+        gbytes_ = 0;
     }
     template <typename T>
     T* as() const { return static_cast<T*>(ptr_); }
@@ -114,6 +170,7 @@ class DevBuf {
     cp_ctx* ctx_ = nullptr;
     void* ptr_ = nullptr;
     size_t bytes_ = 0;
+    size_t gbytes_ = 0;   // size of the underlying block (granule or cached block)
 };
 
 // Context of the C-ABI call running on this thread (for launch timing).
