@@ -49,6 +49,7 @@ struct EpsfRec {
     T wy[4], wx[4], dwy[4], dwx[4];
     int ly, lx;      // floor(v), floor(u) at the anchor pixel (1-based grid coordinates)
     int ra[4], ca[4];
+    int ih;          // every pixel with |dy|, |dx| <= ih evaluates on the unclamped interior stencil
 };
 
 // Per-star record used by the gather kernels (rebuilt whenever theta changes).
@@ -126,6 +127,10 @@ __device__ __forceinline__ void epsf_rec(const ModelConst<T>& M, double fy, doub
         e.ra[a] = pr * M.ios_x * QQ + qr;
         e.ca[a] = pc * QQ + qc * M.Qy;
     }
+    // interior stencil rows / columns are 2 .. gn-2; the pixel offset d moves
+    // the stencil by os * d grid samples
+    auto room = [](int lo, int hi, int os) { return lo < 0 || hi < 0 ? -1 : min(lo, hi) / os; };
+    e.ih = min(room(e.ly - 2, M.gny - 2 - e.ly, M.ios_y), room(e.lx - 2, M.gnx - 2 - e.lx, M.ios_x));
 }
 
 // Grid sample with the reference's stencil clamping, polyphase LDS layout.
@@ -137,6 +142,27 @@ __device__ __forceinline__ float poly_sample(const float* __restrict__ sg, int r
 }
 
 // ePSF value (no flux/bkg) of a star at pixel offset (dy, dx), fast path.
+// Interior evaluation (stencil rows / columns within 2 .. gn-2 for this
+// pixel): 16 LDS taps, no bounds tests.
+template <typename T, bool DERIV>
+__device__ __forceinline__ void epsf_interior(const ModelConst<T>& M, const EpsfRec<T>& e, const float* __restrict__ sg,
+                                              int dy, int dx, T& val, T& dpdv, T& dpdu) {
+    T row[4], drow[4];
+    const int off = dy + dx * M.Qy;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const float* rp = sg + e.ra[a] + off;
+        const T g0 = (T)rp[e.ca[0]], g1 = (T)rp[e.ca[1]], g2 = (T)rp[e.ca[2]], g3 = (T)rp[e.ca[3]];
+        row[a] = e.wx[0] * g0 + e.wx[1] * g1 + e.wx[2] * g2 + e.wx[3] * g3;
+        if (DERIV) drow[a] = e.dwx[0] * g0 + e.dwx[1] * g1 + e.dwx[2] * g2 + e.dwx[3] * g3;
+    }
+    val = e.wy[0] * row[0] + e.wy[1] * row[1] + e.wy[2] * row[2] + e.wy[3] * row[3];
+    if (DERIV) {
+        dpdv = e.dwy[0] * row[0] + e.dwy[1] * row[1] + e.dwy[2] * row[2] + e.dwy[3] * row[3];
+        dpdu = e.wy[0] * drow[0] + e.wy[1] * drow[1] + e.wy[2] * drow[2] + e.wy[3] * drow[3];
+    }
+}
+
 template <typename T, bool DERIV>
 __device__ __forceinline__ void epsf_eval(const ModelConst<T>& M, const EpsfRec<T>& e, const float* __restrict__ sg,
                                           int dy, int dx, T& val, T& dpdv, T& dpdu) {
@@ -151,14 +177,8 @@ __device__ __forceinline__ void epsf_eval(const ModelConst<T>& M, const EpsfRec<
     }
     T row[4], drow[4];
     if (ly >= 2 && ly <= M.gny - 2 && lx >= 2 && lx <= M.gnx - 2) {
-        const int off = dy + dx * M.Qy;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const float* rp = sg + e.ra[a] + off;
-            const T g0 = (T)rp[e.ca[0]], g1 = (T)rp[e.ca[1]], g2 = (T)rp[e.ca[2]], g3 = (T)rp[e.ca[3]];
-            row[a] = e.wx[0] * g0 + e.wx[1] * g1 + e.wx[2] * g2 + e.wx[3] * g3;
-            if (DERIV) drow[a] = e.dwx[0] * g0 + e.dwx[1] * g1 + e.dwx[2] * g2 + e.dwx[3] * g3;
-        }
+        epsf_interior<T, DERIV>(M, e, sg, dy, dx, val, dpdv, dpdu);
+        return;
     } else {
 #pragma unroll
         for (int a = 0; a < 4; ++a) {

@@ -30,7 +30,6 @@ constexpr int TILE_PX = 1024;           // pixels per tile
 constexpr int PPT = TILE_PX / BLOCK;    // pixels per thread
 constexpr int RBLOCK = 1024;            // render / fill blocks share one staged ePSF grid
 constexpr int RWAVE = RBLOCK / 64;
-constexpr int RPPT = TILE_PX / RBLOCK;  // pixels per thread in the render
 constexpr int FBLOCK = 1024;            // fill blocks: two per CU with the ePSF grid in LDS
 constexpr int FWAVE = FBLOCK / 64;
 constexpr int MAX_EPL = 16;             // stamp entries per lane (S^2 <= 1024)
@@ -467,94 +466,123 @@ static __global__ void k_star_rec(int na, const double* __restrict__ theta, Free
 //  _freeze_star! :739-754, final render :1344-1356)
 // ---------------------------------------------------------------------------
 constexpr int RCHUNK = 64;  // star records staged per LDS chunk in the render
+constexpr int RT = 64;      // render tile side (rows = columns)
+constexpr int RSB = 16;     // sub-block side: one 16 x 16 sub-block per wave, 4 pixels per lane
+static_assert((RT / RSB) * (RT / RSB) == RWAVE, "one sub-block per wave");
 
-// Model value of star j (staged in LDS) at pixel offset (dy, dx).
-template <typename T, int KIND, bool LDSG>
-__device__ __forceinline__ T staged_value(const ModelConst<T>& M, const StarRec<T>& r, const EpsfRec<T>* er,
-                                          int dy, int dx, const float* grid, const CornerRec<T>* cr) {
-    if constexpr (KIND == KIND_EPSF && LDSG) {
-        T p, d0, d1;
-        epsf_eval<T, false>(M, *er, grid, dy, dx, p, d0, d1);
-        return fma(r.flux, p, M.bkg);
-    } else {
-        return star_value<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, cr);
-    }
-}
-
+// tile-gather render.  Each 1024-thread block walks a contiguous range of
+// 64 x 64 tiles; wave w owns the 16 x 16 sub-block (w % 4, w / 4) and each
+// lane the pixels (row lane % 16, columns 4k + lane / 16), k = 0..3.  The
+// tile's star list is staged in chunks of 64 by one wave: only the stars this
+// mode renders (live / newly frozen / all), compacted in list order, each with
+// the mask of sub-blocks its model box touches, so a wave skips every other
+// star with one LDS read.  Star records are then read with wave-uniform
+// addresses (scalar loads, kept in SGPRs).  Per-pixel sums run over stars in
+// list (Morton) order: deterministic and the reference's summation order.
+// float renders fit 8 waves / SIMD: two 1024-thread blocks per CU
 template <typename T, int KIND, int MODE, bool LDSG>
-__global__ __launch_bounds__(RBLOCK) void k_render(
+__global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
     const CornerRec<T>* __restrict__ crs, const int* __restrict__ tstart, const int* __restrict__ tend,
     const int* __restrict__ list, TileGeo g, int ntiles, int tiles_per_block, const uint8_t* __restrict__ umask,
     T* __restrict__ dw, const T* __restrict__ w, const T* __restrict__ data, T* __restrict__ wt_resid,
     T* __restrict__ resid, double* __restrict__ partials, Scal* S) {
+    constexpr bool EPSF_LDS = KIND == KIND_EPSF && LDSG;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[RWAVE];
     __shared__ int s_nuse;
     static_assert(RCHUNK == 64, "one wave stages a chunk");
     const int ngrid = LDSG ? M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
-    StarRec<T>* srec = reinterpret_cast<StarRec<T>*>(smem + ((ngrid * 4 + 15) & ~15));
-    EpsfRec<T>* serec = reinterpret_cast<EpsfRec<T>*>(srec + RCHUNK);
-    int* sid = reinterpret_cast<int*>(serec + ((KIND == KIND_EPSF && LDSG) ? RCHUNK : 0));
+    int* sid = reinterpret_cast<int*>(smem + ((ngrid * 4 + 15) & ~15));
+    int* smask = sid + RCHUNK;
     if (LDSG) stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, RBLOCK);
     const float* grid = LDSG ? sgrid : M.grids;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int sbr = (wv % (RT / RSB)) * RSB, sbc = (wv / (RT / RSB)) * RSB;
     double acc = 0;
     const int t0 = xcd_remap(blockIdx.x, gridDim.x) * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
     for (int tile = t0; tile < t1; ++tile) {
         const int ty = tile % g.nty, tx = tile / g.nty;
-        const int gy0 = ty * g.TY + 1, gx0 = tx * g.TX + 1;
-        T val[RPPT];
-        int gyv[RPPT], gxv[RPPT];
-        bool act[RPPT];
+        const int ty0 = ty * RT + 1, tx0 = tx * RT + 1;       // tile origin (1-based)
+        const int gy0 = ty0 + sbr, gx0 = tx0 + sbc;           // this wave's sub-block origin
+        const int gy = gy0 + lane % 16;
+        T val[4];
+        int gxv[4];
+        bool act[4], touched[4];
 #pragma unroll
-        for (int k = 0; k < RPPT; ++k) {
-            int lin = threadIdx.x + k * RBLOCK;
-            gyv[k] = gy0 + lin % g.TY;
-            gxv[k] = gx0 + lin / g.TY;
-            act[k] = gyv[k] <= g.ny && gxv[k] <= g.nx;
-            size_t q = (size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny;
+        for (int k = 0; k < 4; ++k) {
+            gxv[k] = gx0 + 4 * k + lane / 16;
+            act[k] = gy <= g.ny && gxv[k] <= g.nx;
+            const size_t q = (size_t)(gy - 1) + (size_t)(gxv[k] - 1) * g.ny;
             if ((MODE == MODE_RESID || MODE == MODE_COST) && act[k]) act[k] = umask[q] != 0;
             val[k] = T(0);
+            touched[k] = false;
             if (MODE == MODE_FREEZE && act[k]) val[k] = dw[q];
         }
         const int s0 = tstart[tile], s1 = tend[tile];
-        bool touched = false;
         for (int c0 = s0; c0 < s1; c0 += RCHUNK) {
-            // the first wave stages the chunk's stars that this mode renders
-            // (live / newly frozen / all), compacted in list order
             __syncthreads();
             if (threadIdx.x < RCHUNK) {
                 const int i = threadIdx.x;
-                bool use = false;
-                int a = 0;
-                StarRec<T> r;
+                int a = 0, mask = 0;
                 if (c0 + i < s1) {
                     a = list[c0 + i];
-                    r = recs[a];
-                    use = MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (r.flags & 2) != 0 : (r.flags & 1) != 0);
+                    const StarRec<T> r = recs[a];
+                    const bool use =
+                        MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (r.flags & 2) != 0 : (r.flags & 1) != 0);
+                    if (use) {
+                        // sub-blocks (sr, sc) touched by the model box, bit sc * 4 + sr
+#pragma unroll
+                        for (int sc = 0; sc < RT / RSB; ++sc) {
+                            const int c_lo = tx0 + sc * RSB;
+                            const bool cov = r.ax - r.mR <= c_lo + RSB - 1 && r.ax + r.mR >= c_lo;
+#pragma unroll
+                            for (int sr = 0; sr < RT / RSB; ++sr) {
+                                const int r_lo = ty0 + sr * RSB;
+                                const bool rov = r.ay - r.mR <= r_lo + RSB - 1 && r.ay + r.mR >= r_lo;
+                                if (cov && rov) mask |= 1 << (sc * (RT / RSB) + sr);
+                            }
+                        }
+                    }
                 }
-                const unsigned long long m = __ballot(use);
+                const unsigned long long m = __ballot(mask != 0);
                 const int slot = __popcll(m & ((1ull << i) - 1ull));
-                if (use) {
-                    srec[slot] = r;
-                    if (KIND == KIND_EPSF && LDSG) serec[slot] = ers[a];
+                if (mask != 0) {
                     sid[slot] = a;
+                    smask[slot] = mask;
                 }
                 if (i == 0) s_nuse = __popcll(m);
             }
             __syncthreads();
             const int cn = s_nuse;
             for (int j = 0; j < cn; ++j) {
-                const StarRec<T> r = srec[j];
+                const int mj = __builtin_amdgcn_readfirstlane(smask[j]);
+                if (!((mj >> wv) & 1)) continue;   // one LDS read per star
+                const int a = __builtin_amdgcn_readfirstlane(sid[j]);
+                const StarRec<T> r = recs[a];
+                const int dy = gy - r.ay;
+                const bool rin = abs(dy) <= r.mR;
+                bool fast = false;
+                if constexpr (EPSF_LDS) fast = ers[a].ih >= r.mR;
 #pragma unroll
-                for (int k = 0; k < RPPT; ++k) {
-                    int dy = gyv[k] - r.ay, dx = gxv[k] - r.ax;
-                    if (act[k] && abs(dy) <= r.mR && abs(dx) <= r.mR) {
-                        T v = staged_value<T, KIND, LDSG>(M, r, serec + j, dy, dx, grid, crs + sid[j]);
+                for (int k = 0; k < 4; ++k) {
+                    const int c_lo = gx0 + 4 * k;
+                    if (r.ax + r.mR < c_lo || r.ax - r.mR > c_lo + 3) continue;   // uniform
+                    const int dx = gxv[k] - r.ax;
+                    if (act[k] && rin && abs(dx) <= r.mR) {
+                        T v;
+                        if constexpr (EPSF_LDS) {
+                            T p, d0, d1;
+                            if (fast) epsf_interior<T, false>(M, ers[a], grid, dy, dx, p, d0, d1);
+                            else epsf_eval<T, false>(M, ers[a], grid, dy, dx, p, d0, d1);
+                            v = fma(r.flux, p, M.bkg);
+                        } else {
+                            v = star_value<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a);
+                        }
                         if (MODE == MODE_FREEZE) {
                             val[k] -= v;
-                            touched = true;
+                            touched[k] = true;
                         } else {
                             val[k] += v;
                         }
@@ -563,16 +591,16 @@ __global__ __launch_bounds__(RBLOCK) void k_render(
             }
         }
 #pragma unroll
-        for (int k = 0; k < RPPT; ++k) {
+        for (int k = 0; k < 4; ++k) {
             if (!act[k]) continue;
-            size_t q = (size_t)(gyv[k] - 1) + (size_t)(gxv[k] - 1) * g.ny;
+            const size_t q = (size_t)(gy - 1) + (size_t)(gxv[k] - 1) * g.ny;
             if (MODE == MODE_RESID || MODE == MODE_COST) {
                 T r = val[k] - dw[q];
                 T wr = sqrt(w[q]) * r;
                 wt_resid[q] = wr;
                 acc += (double)wr * (double)wr;
             } else if (MODE == MODE_FREEZE) {
-                if (touched) dw[q] = val[k];
+                if (touched[k]) dw[q] = val[k];
             } else {
                 resid[q] = data[q] - val[k];
             }
@@ -641,7 +669,8 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
             T g3[3];
             if constexpr (KIND == KIND_EPSF && LDSG) {
                 T pv, pdv, pdu;
-                epsf_eval<T, true>(M, er, grid, dy, dx, pv, pdv, pdu);
+                if (er.ih >= R) epsf_interior<T, true>(M, er, grid, dy, dx, pv, pdv, pdu);   // uniform per star
+                else epsf_eval<T, true>(M, er, grid, dy, dx, pv, pdv, pdu);
                 g3[0] = r.flux * (-(M.osy * pdv));
                 g3[1] = r.flux * (-(M.osx * pdu));
                 g3[2] = pv;

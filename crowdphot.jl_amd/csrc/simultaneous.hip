@@ -433,10 +433,9 @@ struct Sim {
         CP_HIP(hipMemcpyAsync(&Sc->dof, &dof, 8, hipMemcpyHostToDevice, st));
 
         // --- tiles: fit list (stamp box) and model list (model box)
-        // model-box tile lists (render passes) and the pixel-major stamp structure (J v)
-        int TYm = 16;
-        if (o.tile == 8 || o.tile == 16 || o.tile == 32 || o.tile == 64 || o.tile == 128) TYm = o.tile;
-        TileGeo g{TYm, TILE_PX / TYm, (ny + TYm - 1) / TYm, (nx + TILE_PX / TYm - 1) / (TILE_PX / TYm), ny, nx};
+        // model-box tile lists (render passes, RT x RT tiles) and the pixel-major
+        // stamp structure (J v)
+        TileGeo g{RT, RT, (ny + RT - 1) / RT, (nx + RT - 1) / RT, ny, nx};
         const int ntiles = g.nty * g.ntx;
         TileLists Lmod;
         PixCSR csr;
@@ -484,8 +483,7 @@ struct Sim {
         const size_t poly_words = single_epsf ? (size_t)M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
         const bool ldsg = single_epsf && poly_words * 4 <= 44 * 1024;
         const size_t grid_sh = ldsg ? ((poly_words * 4 + 15) & ~(size_t)15) : 0;
-        const size_t render_sh =
-            grid_sh + RCHUNK * (sizeof(StarRec<T>) + (ldsg ? sizeof(EpsfRec<T>) : 0) + 4);
+        const size_t render_sh = grid_sh + RCHUNK * 2 * sizeof(int);
         const size_t fill_sh = grid_sh;
 
         // two 1024-thread render / fill blocks fit one CU (LDS and wave slots)

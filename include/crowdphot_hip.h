@@ -100,7 +100,7 @@ typedef struct cp_sim_opts {
     int32_t free_mask;         /* CP_FREE_* bits; fixed members take fixed_* values */
     double fixed_y, fixed_x, fixed_flux, fixed_bkg;  /* fixed values (bkg is always fixed) */
     int32_t precision;         /* CP_PREC_* */
-    int32_t tile;              /* image tile side in px for the star bins (0 = autotuned) */
+    int32_t tile;              /* reserved, ignored: the render bins stars into 64 x 64 px tiles */
 } cp_sim_opts;
 
 /* Caller-allocated outputs, catalogue order (length n_stars unless noted).
