@@ -776,8 +776,25 @@ static __global__ void k_lsqr_init(Scal* S, double lam, int itmax, double atol, 
 // product where each pixel reads only its own rows of the star-major V.
 // ---------------------------------------------------------------------------
 constexpr unsigned int CSR_NONE = 0xffffffffu;
+// Union pixels are ordered in CTY x CTX pixel tiles (tiles column-major over
+// the image, pixels column-major inside a tile): one star's 7 x 7 stamp then
+// falls in one or two neighbouring blocks of J v, which reuse its V rows and
+// its LSQR vector from cache.
+constexpr int CTY = 32, CTX = 8;
 
-// key = image index of stamp entry i = a S^2 + m (CSR_NONE off-image or w == 0).
+__host__ __device__ __forceinline__ unsigned int csr_key(int gy, int gx, int ntly) {
+    const int y = gy - 1, x = gx - 1;
+    const unsigned int t = (unsigned int)(x / CTX) * ntly + (unsigned int)(y / CTY);
+    return (t * CTX + (unsigned int)(x % CTX)) * CTY + (unsigned int)(y % CTY);
+}
+
+__host__ __device__ __forceinline__ int csr_q(unsigned int key, int ntly, int ny) {
+    const unsigned int iy = key % CTY, ix = (key / CTY) % CTX, t = key / (CTY * CTX);
+    const int y = (int)((t % ntly) * CTY + iy), x = (int)((t / ntly) * CTX + ix);
+    return y + x * ny;
+}
+
+// key = tile-order position of stamp entry i = a S^2 + m (CSR_NONE off-image or w == 0).
 template <typename T>
 static __global__ void k_csr_pairs(int na, const int* __restrict__ ay, const int* __restrict__ ax, int R, int S,
                                    const T* __restrict__ w, int ny, int nx, unsigned int* __restrict__ key,
@@ -790,7 +807,7 @@ static __global__ void k_csr_pairs(int na, const int* __restrict__ ay, const int
         unsigned int k = CSR_NONE;
         if (gy >= 1 && gy <= ny && gx >= 1 && gx <= nx) {
             const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * ny;
-            if (w[q] > T(0)) k = (unsigned int)q;
+            if (w[q] > T(0)) k = csr_key(gy, gx, (ny + CTY - 1) / CTY);
         }
         key[i] = k;
         val[i] = (int)i;
@@ -808,9 +825,9 @@ static __global__ void k_csr_heads(const unsigned int* __restrict__ key, int64_t
 }
 
 static __global__ void k_csr_pix(const unsigned int* __restrict__ key, int* __restrict__ ptr, int U,
-                                 const int* __restrict__ E, int* __restrict__ pix) {
+                                 const int* __restrict__ E, int ny, int* __restrict__ pix) {
     const int u = blockIdx.x * BLOCK + threadIdx.x;
-    if (u < U) pix[u] = (int)key[ptr[u]];
+    if (u < U) pix[u] = csr_q(key[ptr[u]], (ny + CTY - 1) / CTY, ny);
     if (u == 0) ptr[U] = *E;
 }
 

@@ -136,8 +136,12 @@ void build_pixel_csr(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* 
     const unsigned int gpairs = (unsigned int)std::min<int64_t>(cdiv(n, BLOCK), 65536);
     launch(k_csr_pairs<T>, dim3(gpairs), dim3(BLOCK), 0, st, na, d_ay, d_ax, R, S, d_w, ny, nx,
            k1.as<unsigned int>(), v1.as<int>());
-    // q <= ny nx - 1 < 2^end_bit - 1, so CSR_NONE (all ones in the sorted bits) sorts last
-    const int end_bit = std::min(32, bit_length((int64_t)ny * nx));
+    // keys < ntiles * CTY * CTX <= 2^end_bit - 1, so CSR_NONE (all ones in the
+    // sorted bits) sorts last
+    const int64_t ntly = (ny + CTY - 1) / CTY, ntlx = (nx + CTX - 1) / CTX;
+    const int64_t kmax = ntly * ntlx * CTY * CTX;
+    CP_ARG(kmax < (int64_t)UINT32_MAX, "image too large for one fit (tile-ordered pixel keys exceed 32 bits)");
+    const int end_bit = std::min(32, bit_length(kmax));
     size_t b = 0;
     CP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, k1.as<unsigned int>(), k2.as<unsigned int>(),
                                               v1.as<int>(), C.ent.as<int>(), (int)n, 0, end_bit, st));
@@ -162,7 +166,7 @@ void build_pixel_csr(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* 
     C.U = hv[0];
     C.E = hv[1];
     launch(k_csr_pix, dim3(cdiv(std::max(C.U, 1), BLOCK)), dim3(BLOCK), 0, st,
-           (const unsigned int*)k2.as<unsigned int>(), C.ptr.as<int>(), C.U, (const int*)dE.as<int>(),
+           (const unsigned int*)k2.as<unsigned int>(), C.ptr.as<int>(), C.U, (const int*)dE.as<int>(), ny,
            C.pix.as<int>());
 }
 
