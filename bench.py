@@ -47,19 +47,36 @@ def load_pkg():
     return mod
 
 
-def workload_bytes(kernel, S2, p, U, P, trace, na):
-    """Algorithmic HBM bytes of all launches of `kernel` in one fit (b_f = 4,
-    b_d = 8): J v reads V and v, reads and writes u over the union; J^T u reads
-    V and u over the union and updates the per-star float64 vectors."""
-    rows = trace
+def workload_bytes(kernel, S2, p, U, trace):
+    """Algorithmic HBM bytes of all launches of `kernel` in one fit, from the
+    unit costs b_f = b_i = 4 (pixels, stamp values, stamp indices) and
+    b_d = 8 (parameter vectors) over the live stars N of every trial:
+    J v reads the p stamp columns, the stamp indices and v (p N S^2 b_f +
+    N S^2 b_i + p N b_d) and writes u over the union (U b_f); J^T u reads the
+    columns, the indices and u at the stamp pixels (p N S^2 b_f + N S^2 (b_i +
+    b_f)) and writes the per-star vector (p N b_d)."""
     total = 0.0
-    for r in rows:
+    for r in trace:
         n_live, its = r[5], r[6]
         if kernel == "k_Jv":
-            total += its * (4 * p * S2 * n_live + 8 * p * n_live + 8 * U)
+            total += its * (4 * p * S2 * n_live + 4 * S2 * n_live + 8 * p * n_live + 4 * U)
         elif kernel == "k_JTu":
-            total += its * (4 * p * S2 * n_live + 4 * U + 48 * p * n_live)
+            total += its * (4 * p * S2 * n_live + 8 * S2 * n_live + 8 * p * n_live)
     return total
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy of known size),
+    or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_crowded_epsf_4096x4096_200k_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -72,6 +89,9 @@ def main():
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--stars", type=int, default=200_000)
     ap.add_argument("--profile-only", action="store_true", help="one fit, for rocprofv3 runs")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -198,14 +218,18 @@ def main():
     if cand:
         dom = max(cand, key=lambda k: cand[k][1])
         cnt, tot_ms = cand[dom]
-        bytes_total = workload_bytes(dom, S2, p, U, ny * nx, tracep, sp.n_active)
+        bytes_total = workload_bytes(dom, S2, p, U, tracep)
         avg_ms = tot_ms / cnt
         per_launch = bytes_total / cnt
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                 "avg_launch_us": round(avg_ms * 1e3, 2), "launches": cnt,
                 "algorithmic_bytes_per_launch": int(per_launch)}
+
+    serving = None
+    if not args.no_serving:
+        serving = serving_bench(pkg, args.serve_requests, args.serve_slots)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -233,9 +257,154 @@ def main():
                      "median_rel_flux_vs_truth": float(np.median(truth_rel))},
         "roofline": roof,
         "kernels_ms": {k: round(v[1], 3) for k, v in kernels.items()},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def serving_bench(pkg, n_req, slots, seed=5):
+    """Serving workload: uneven image tiles (side 512 / 1024 / 2048 / 4096,
+    side^2 / 80 stars clamped to 1000..200000, Gaussian or ePSF model) with
+    their images, weights and catalogues resident in HBM (C order, transposed
+    on the device), fitted (a) one at a time on one context ("plain") and (b)
+    through the continuous-batching server: closed loop for capacity, then
+    Poisson arrivals at 70 % of it for latency.  Every served result is
+    compared with the plain one."""
+    import torch
+    abi = pkg._abi
+    sim = pkg.simulation
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    sides = (512, 1024, 2048, 4096)
+    t0 = time.time()
+    pool = {}
+    for i, sd in enumerate(sides):
+        for j, m in enumerate(("gauss", "epsf")):
+            t = sim.serving_tile(seed * 100 + i * 2 + j, sd, m)
+            model, mask = pkg.photometry._sim_psf_and_mask(t["psf"], t["fixed"])
+            o = abi.sim_opts_default()
+            o.model_rad = t["model_rad"]
+            o.max_iter = t["max_iter"]
+            o.free_mask = mask
+            o.fixed_bkg = 0.0
+            o.layout = abi.CP_LAYOUT_ROWMAJOR
+            src = t["sources"]
+            pool[(sd, m)] = dict(
+                n=t["n"], ny=sd, nx=sd, fit_rad=float(t["fit_rad"]), desc=model.desc(), opts=o,
+                img=torch.from_numpy(np.ascontiguousarray(t["image"], np.float32).ravel()).to(dev),
+                iv=torch.from_numpy(np.ascontiguousarray(t["inv_var"], np.float32).ravel()).to(dev),
+                y=torch.from_numpy(np.ascontiguousarray(src["y"])).to(dev),
+                x=torch.from_numpy(np.ascontiguousarray(src["x"])).to(dev),
+                f=torch.from_numpy(np.ascontiguousarray(src["flux"])).to(dev))
+    torch.cuda.synchronize()
+    keys = [(sides[rng.integers(4)], ("gauss", "epsf")[rng.integers(2)]) for _ in range(n_req)]
+    log(f"[serve] {len(pool)} tiles generated and uploaded in {time.time() - t0:.1f}s; {n_req} requests, "
+        f"{slots} slots")
+
+    def new_out(n):
+        arrs = {k: np.zeros(n) for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq")}
+        arrs.update(converged=np.zeros(n, np.uint8), valid=np.zeros(n, np.uint8), n_iter=np.zeros(n, np.int32))
+        r = abi.SimResult()
+        for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq"):
+            setattr(r, k, arrs[k].ctypes.data_as(abi.f64p))
+        r.converged = arrs["converged"].ctypes.data_as(abi.u8p)
+        r.valid = arrs["valid"].ctypes.data_as(abi.u8p)
+        r.n_iter = arrs["n_iter"].ctypes.data_as(abi.i32p)
+        return r, arrs
+
+    stars = sum(pool[k]["n"] for k in keys)
+    ctx = pkg.Context(0)
+    lib = ctx.lib
+
+    def fit_plain(k):
+        t = pool[k]
+        r, arrs = new_out(t["n"])
+        rc = lib.cp_fit_simultaneous_device(ctx.handle, C.byref(t["desc"]), C.c_void_p(t["img"].data_ptr()),
+                                            C.c_void_p(t["iv"].data_ptr()), t["ny"], t["nx"],
+                                            C.c_void_p(t["y"].data_ptr()), C.c_void_p(t["x"].data_ptr()),
+                                            C.c_void_p(t["f"].data_ptr()), t["n"], t["fit_rad"],
+                                            C.byref(t["opts"]), C.byref(r))
+        ctx.check(rc)
+        return arrs
+
+    for k in pool:   # warm each shape once
+        fit_plain(k)
+    plain, lat_plain = [], []
+    tp = time.perf_counter()
+    for k in keys:
+        a = time.perf_counter()
+        plain.append(fit_plain(k))
+        lat_plain.append(time.perf_counter() - a)
+    tp = time.perf_counter() - tp
+
+    def run_server(srv, arrivals):
+        keep, tickets = [], []
+        t_first = time.perf_counter()
+        for k, g in zip(keys, arrivals):
+            dt = t_first + g - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            t = pool[k]
+            r, arrs = new_out(t["n"])
+            q = abi.Request()
+            q.psf = C.pointer(t["desc"])
+            q.image, q.inv_var = t["img"].data_ptr(), t["iv"].data_ptr()
+            q.ny, q.nx = t["ny"], t["nx"]
+            q.y0, q.x0, q.flux0 = t["y"].data_ptr(), t["x"].data_ptr(), t["f"].data_ptr()
+            q.n_stars, q.fit_rad = t["n"], t["fit_rad"]
+            q.opts = C.pointer(t["opts"])
+            q.out = C.pointer(r)
+            q.device_inputs = 1
+            tk = C.c_uint64()
+            rc = lib.cp_server_submit(srv, C.byref(q), C.byref(tk))
+            if rc != 0:
+                raise RuntimeError(lib.cp_server_last_error(srv).decode())
+            keep.append((r, arrs, q))
+            tickets.append(tk.value)
+        stats = []
+        for tk in tickets:
+            st = abi.RequestStat()
+            rc = lib.cp_server_wait(srv, tk, C.byref(st), None, 0)
+            if rc != 0:
+                raise RuntimeError(f"served fit failed ({rc})")
+            stats.append(st)
+        return [x[1] for x in keep], stats
+
+    h = C.c_void_p()
+    if lib.cp_server_create(C.byref(h), 0, 200 << 30, slots) != 0:
+        raise RuntimeError("cp_server_create failed")
+    try:
+        run_server(h, np.zeros(len(pool)))   # warm the slots
+        tc = time.perf_counter()
+        closed, cst = run_server(h, np.zeros(n_req))
+        tc = time.perf_counter() - tc
+        rate = 0.7 * n_req / tc
+        opened, ost = run_server(h, np.cumsum(rng.exponential(1.0 / rate, n_req)))
+    finally:
+        lib.cp_server_destroy(h)
+    t_open = max(st.t_end for st in ost) - min(st.t_submit for st in ost)
+    same = all(all(np.array_equal(a[f], b[f]) for f in ("y", "x", "flux", "flux_err", "n_iter"))
+               for a, b in zip(plain, closed)) and \
+        all(np.array_equal(a["flux"], b["flux"]) for a, b in zip(plain, opened))
+    lat_open = np.array([st.t_end - st.t_submit for st in ost])
+    res = {
+        "workload": "image tiles resident in HBM, side uniform over {512, 1024, 2048, 4096}, side^2/80 stars "
+                    "(1000..200000), Gaussian or ePSF per request, max_iter 10",
+        "requests": n_req, "stars": int(stars), "slots": slots, "hbm_budget_gb": 200,
+        "plain": {"stars_per_s": round(stars / tp, 1), "p50_ms": round(float(np.median(lat_plain)) * 1e3, 2),
+                  "p99_ms": round(float(np.quantile(lat_plain, 0.99)) * 1e3, 2)},
+        "served_closed_loop": {"stars_per_s": round(stars / tc, 1)},
+        "served": {"arrival_rate_per_s": round(rate, 2), "stars_per_s": round(stars / t_open, 1),
+                   "p50_ms": round(float(np.median(lat_open)) * 1e3, 2),
+                   "p99_ms": round(float(np.quantile(lat_open, 0.99)) * 1e3, 2)},
+        "speedup_closed_loop_vs_plain": round(tp / tc, 3),
+        "results_identical_to_plain": bool(same),
+    }
+    log(f"[serve] plain {stars / tp:.4g} stars/s; served closed loop {stars / tc:.4g} stars/s "
+        f"({tp / tc:.2f}x); open loop at {rate:.1f} req/s: p50 {res['served']['p50_ms']} ms, "
+        f"p99 {res['served']['p99_ms']} ms; identical {same}")
+    return res
 
 
 def cpu_baseline(pkg, crop):

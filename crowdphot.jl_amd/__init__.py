@@ -11,4 +11,4 @@ from .photometry import (ArgumentError, Context, LMResult, MultiPassPhotResult, 
                          fit_star, fit_star_batch)
 from .psf import (CircularGaussianPSF, CircularMoffatPSF, GriddedPSFModel,  # noqa: F401
                   ImagePSF, moffat_epsf, pixel_response_kernel)
-from . import simulation  # noqa: F401
+from . import serving, simulation  # noqa: F401

@@ -26,6 +26,8 @@ CP_SOLVER_CGLS = 1
 
 CP_PREC_MIXED = 0
 CP_PREC_FP64 = 1
+CP_LAYOUT_COLMAJOR = 0
+CP_LAYOUT_ROWMAJOR = 1
 
 CP_COV_DEFAULT = 0
 CP_COV_KNOWN = 1
@@ -96,6 +98,7 @@ class SimOpts(C.Structure):
         ("fixed_bkg", C.c_double),
         ("precision", C.c_int32),
         ("tile", C.c_int32),
+        ("layout", C.c_int32),
     ]
 
 
@@ -175,6 +178,7 @@ def sim_opts_default():
     o.free_mask = CP_FREE_Y | CP_FREE_X | CP_FREE_FLUX
     o.fixed_y = o.fixed_x = o.fixed_flux = o.fixed_bkg = 0.0
     o.precision = CP_PREC_MIXED
+    o.layout = CP_LAYOUT_COLMAJOR
     o.tile = 0
     return o
 
@@ -195,6 +199,30 @@ def lm_opts_default():
     o.min_diagonal = 1e-6
     o.covariance = CP_COV_DEFAULT
     return o
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("psf", C.POINTER(PsfDesc)),
+        ("image", C.c_void_p),
+        ("inv_var", C.c_void_p),
+        ("ny", C.c_int32), ("nx", C.c_int32),
+        ("y0", C.c_void_p), ("x0", C.c_void_p), ("flux0", C.c_void_p),
+        ("n_stars", C.c_int32),
+        ("fit_rad", C.c_double),
+        ("opts", C.POINTER(SimOpts)),
+        ("out", C.POINTER(SimResult)),
+        ("device_inputs", C.c_int32),
+    ]
+
+
+class RequestStat(C.Structure):
+    _fields_ = [
+        ("rc", C.c_int32),
+        ("slot", C.c_int32),
+        ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
+        ("hbm_bytes", C.c_uint64),
+    ]
 
 
 # Every symbol include/crowdphot_hip.h declares, with (restype, argtypes).
@@ -234,6 +262,14 @@ SIGNATURES = {
     "cp_render_sources": (C.c_int, [C.POINTER(PsfDesc), C.c_int32, C.c_int32, f64p, f64p, f64p,
                                     C.c_int32, C.c_double, f64p]),
     "cp_add_noise": (C.c_int, [C.c_uint64, f64p, C.c_int64, C.c_double, C.c_double]),
+    "cp_request_bytes": (C.c_uint64, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                      C.c_int32]),
+    "cp_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_uint64, C.c_int32]),
+    "cp_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "cp_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
+    "cp_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStat), C.c_char_p, C.c_int32]),
+    "cp_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "cp_server_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None

@@ -15,11 +15,11 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["context.hip", "simultaneous.hip", "lm_batch.hip"]
-CPP_SOURCES = ["simulate.cpp"]
+CPP_SOURCES = ["simulate.cpp", "server.cpp"]
 # packed FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature",
              "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics", "-Wno-unused-result"]
-CPP_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-march=x86-64-v3"]
+CPP_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-pthread", "-march=x86-64-v3"]
 
 
 def _run(cmd):
@@ -47,7 +47,7 @@ def build(jobs=8, verbose=False):
             if verbose and err:
                 sys.stderr.write(err)
     tmp = OUT + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, OUT)
     return OUT
 

@@ -1311,6 +1311,25 @@ static __global__ void k_chisq(int na, int p, const double* __restrict__ theta, 
     if (den > 0) chisq[a] = num / den;
 }
 
+// dst (cols x rows, row-major) = transpose of src (rows x cols, row-major):
+// turns a C-order image into the column-major layout and back.  32 x 32 LDS
+// tiles (one padding column against bank conflicts), block 32 x 8.
+template <typename TS, typename TD>
+static __global__ __launch_bounds__(256) void k_transpose(const TS* __restrict__ src, int rows, int cols,
+                                                          TD* __restrict__ dst) {
+    __shared__ TD tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int i = threadIdx.y; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) tile[i][threadIdx.x] = (TD)src[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.y; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) dst[(size_t)c * rows + r] = tile[threadIdx.x][i];
+    }
+}
+
 template <typename T>
 static __global__ void k_to_float(const T* __restrict__ src, float* __restrict__ dst, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)

@@ -688,7 +688,14 @@ struct Sim {
         CP_HIP(hipMemcpyAsync(hfz.data(), frozen_at.get(), (size_t)na * 4, hipMemcpyDeviceToHost, st));
         if (out->chisq) CP_HIP(hipMemcpyAsync(hchi.data(), chis.get(), (size_t)na * 8, hipMemcpyDeviceToHost, st));
         if (out->residual) {
-            if (sizeof(T) == 4) {
+            if (o.layout == CP_LAYOUT_ROWMAJOR) {
+                // column-major ny x nx is row-major nx x ny: transpose to ny x nx
+                DevBuf rf(ctx, Pn * 4);
+                launch(k_transpose<T, float>, dim3(cdiv(ny, 32), cdiv(nx, 32)), dim3(32, 8), 0, st,
+                       (const T*)resid, nx, ny, rf.as<float>());
+                CP_HIP(hipMemcpyAsync(out->residual, rf.get(), Pn * 4, hipMemcpyDeviceToHost, st));
+                CP_HIP(hipStreamSynchronize(st));
+            } else if (sizeof(T) == 4) {
                 CP_HIP(hipMemcpyAsync(out->residual, resid, Pn * 4, hipMemcpyDeviceToHost, st));
             } else {
                 DevBuf rf(ctx, Pn * 4);
@@ -756,10 +763,26 @@ void fit_sim_device(cp_ctx* ctx, const cp_psf_desc* psfd, const float* d_img, co
     CP_ARG(n >= 0, "negative star count");
     const cp_sim_opts& o = *po;
     validate_sim(o, fit_rad);
+    CP_ARG(o.layout == CP_LAYOUT_COLMAJOR || o.layout == CP_LAYOUT_ROWMAJOR, "layout must be CP_LAYOUT_*");
     HostPsf P = make_host_psf(psfd);
     CP_HIP(hipSetDevice(ctx->device));
     DevPsf DP;
     upload_psf(ctx, psfd, P, DP);
+    DevBuf t_img, t_iv;
+    if (o.layout == CP_LAYOUT_ROWMAJOR) {
+        // C-order inputs: transpose into the column-major working layout
+        const size_t Pn = (size_t)ny * nx;
+        t_img.alloc(ctx, Pn * 4);
+        launch(k_transpose<float, float>, dim3(cdiv(nx, 32), cdiv(ny, 32)), dim3(32, 8), 0, ctx->stream, d_img, ny, nx,
+               t_img.as<float>());
+        d_img = t_img.as<float>();
+        if (d_iv) {
+            t_iv.alloc(ctx, Pn * 4);
+            launch(k_transpose<float, float>, dim3(cdiv(nx, 32), cdiv(ny, 32)), dim3(32, 8), 0, ctx->stream, d_iv,
+                   ny, nx, t_iv.as<float>());
+            d_iv = t_iv.as<float>();
+        }
+    }
     std::vector<double> hy(n), hx(n), hf(n);
     if (n > 0) {
         CP_HIP(hipMemcpyAsync(hy.data(), d_y0, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));

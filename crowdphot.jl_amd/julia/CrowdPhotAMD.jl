@@ -71,6 +71,7 @@ mutable struct CpSimOpts
     fixed_bkg::Float64
     precision::Int32
     tile::Int32
+    layout::Int32
 end
 
 mutable struct CpSimResult
@@ -170,7 +171,7 @@ function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad
                   model_rad_max, model_rad_nsigma, max_step, max_trials, max_iter, x_tol, f_tol, g_tol,
                   λ_init, λ_up, λ_down, λ_min, λ_max, 0, mask,
                   Float64(get(fixed, :y, 0)), Float64(get(fixed, :x, 0)), Float64(get(fixed, :flux, 0)),
-                  Float64(get(fixed, :bkg, 0)), precision === :fp64 ? 1 : 0, 0)
+                  Float64(get(fixed, :bkg, 0)), precision === :fp64 ? 1 : 0, 0, 0)
     out = (y = zeros(n), x = zeros(n), flux = zeros(n), bkg = zeros(n), y_err = zeros(n), x_err = zeros(n),
            flux_err = zeros(n), bkg_err = zeros(n), chisq = zeros(n), converged = zeros(UInt8, n),
            valid = zeros(UInt8, n), n_iter = zeros(Int32, n), active = zeros(UInt8, n),

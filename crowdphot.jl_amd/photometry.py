@@ -76,6 +76,15 @@ def _p(a, t):
     return None if a is None else a.ctypes.data_as(t)
 
 
+def _rowmajor32(img):
+    """float32 C-order view or copy: the library transposes it on the device
+    (CP_LAYOUT_ROWMAJOR), so no host-side transpose."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    if a.ndim != 2:
+        raise ArgumentError("image must be a 2-D array")
+    return a.ravel()
+
+
 def _colmajor32(img):
     a = np.asarray(img, dtype=np.float32)
     if a.ndim != 2:
@@ -144,124 +153,149 @@ def _sim_psf_and_mask(psf, fixed):
     return model, mask
 
 
-def fit_all_stars_simultaneous(image, psf, sources, fit_rad, *, fixed=None, inv_var=None,
-                               spread_model_fwhm=None, solver="lsqr", inner_iterations=10,
-                               linear_tol=1e-6, model_rad="auto", model_rad_max=None,
-                               model_rad_nsigma=1.0, max_step=0.25, max_trials=8, max_iter=40,
-                               x_tol=1e-5, f_tol=1e-4, g_tol=1e-4, lambda_init=1e-4, lambda_up=10.0,
-                               lambda_down=10.0, lambda_min=1e-12, lambda_max=1e12, damping="marquardt",
-                               show_trace=False, covariance_estimator=None, precision="mixed",
-                               residual=True, trace=0, tile=0, ctx=None):
+class _SimCall:
+    """Validated inputs, options and caller-owned outputs of one simultaneous fit
+    (argument contract of psf_photometry_simultaneous.jl:984-1040,1060-1066)."""
+
+    def __init__(self, image, psf, sources, fit_rad, *, fixed=None, inv_var=None, spread_model_fwhm=None,
+                 solver="lsqr", inner_iterations=10, linear_tol=1e-6, model_rad="auto", model_rad_max=None,
+                 model_rad_nsigma=1.0, max_step=0.25, max_trials=8, max_iter=40, x_tol=1e-5, f_tol=1e-4,
+                 g_tol=1e-4, lambda_init=1e-4, lambda_up=10.0, lambda_down=10.0, lambda_min=1e-12,
+                 lambda_max=1e12, damping="marquardt", show_trace=False, covariance_estimator=None,
+                 precision="mixed", residual=True, trace=0, tile=0):
+        fixed = dict(fixed or {})
+        if max_iter <= 0:
+            raise ArgumentError("max_iter must be positive")
+        if inner_iterations <= 0:
+            raise ArgumentError("inner_iterations must be positive")
+        if not linear_tol > 0:
+            raise ArgumentError("linear_tol must be positive")
+        if max_trials <= 0:
+            raise ArgumentError("max_trials must be positive")
+        if not (model_rad == "auto" or (np.isscalar(model_rad) and model_rad > 0)):
+            raise ArgumentError(f"model_rad must be :auto or a positive real, got {model_rad!r}")
+        if model_rad_max is None:
+            model_rad_max = 10 * fit_rad
+        if not model_rad_max >= fit_rad:
+            raise ArgumentError("model_rad_max must be >= fit_rad")
+        if not model_rad_nsigma > 0:
+            raise ArgumentError("model_rad_nsigma must be positive")
+        if solver not in ("lsqr", "lsmr"):
+            raise ArgumentError(f"solver must be :lsqr or :lsmr, got {solver!r}")
+        if solver == "lsmr":
+            raise ArgumentError("solver :lsmr is not built on the GPU; use :lsqr")
+        if damping == "levenberg":
+            raise ArgumentError("LevenbergDamping is not supported by fit_all_stars_simultaneous: it conflicts "
+                                "with the column-equilibrated x_tol scaling. Use MarquardtDamping.")
+        if not fit_rad > 0:
+            raise ArgumentError("fit_rad must be positive")
+        img = np.asarray(image)
+        if img.ndim != 2:
+            raise ArgumentError("image must be a 2-D array")
+        ny, nx = img.shape
+        if inv_var is not None and np.shape(inv_var) != img.shape:
+            raise ArgumentError("`inv_var` must be the same size as `image`")
+        if model_rad == "auto" and inv_var is None:
+            raise ArgumentError("model_rad=:auto requires inv_var (it needs a noise scale); pass inv_var or an "
+                                "explicit scalar model_rad")
+        model, mask = _sim_psf_and_mask(psf, fixed)
+        self.y0, self.x0, self.f0 = _catalog(sources)
+        n = len(self.y0)
+        o = _abi.sim_opts_default()
+        o.solver = _abi.CP_SOLVER_LSQR
+        o.inner_iterations = inner_iterations
+        o.linear_tol = linear_tol
+        o.model_rad = 0.0 if model_rad == "auto" else float(model_rad)
+        o.model_rad_max = float(model_rad_max)
+        o.model_rad_nsigma = model_rad_nsigma
+        o.max_step = max_step
+        o.max_trials = max_trials
+        o.max_iter = int(max_iter)
+        o.x_tol, o.f_tol, o.g_tol = x_tol, f_tol, g_tol
+        o.lambda_init, o.lambda_up, o.lambda_down = lambda_init, lambda_up, lambda_down
+        o.lambda_min, o.lambda_max = lambda_min, lambda_max
+        o.covariance = {None: _abi.CP_COV_DEFAULT, "known": _abi.CP_COV_KNOWN,
+                        "reweighted": _abi.CP_COV_REWEIGHTED, "none": _abi.CP_COV_NONE}[covariance_estimator]
+        o.free_mask = mask
+        o.fixed_y = float(fixed.get("y", 0.0))
+        o.fixed_x = float(fixed.get("x", 0.0))
+        o.fixed_flux = float(fixed.get("flux", 0.0))
+        o.fixed_bkg = float(fixed.get("bkg", 0.0))
+        o.precision = {"mixed": _abi.CP_PREC_MIXED, "fp64": _abi.CP_PREC_FP64}[precision]
+        o.tile = int(tile)
+        o.layout = _abi.CP_LAYOUT_ROWMAJOR
+        self.opts = o
+        self.ny, self.nx, self.n = ny, nx, n
+        self.fit_rad = float(fit_rad)
+        self.show_trace = show_trace
+        self.out = {k: np.zeros(n) for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err",
+                                             "chisq")}
+        self.conv = np.zeros(n, np.uint8)
+        self.valid = np.zeros(n, np.uint8)
+        self.active = np.zeros(n, np.uint8)
+        self.n_iter = np.zeros(n, np.int32)
+        self.res = np.zeros(ny * nx, np.float32) if residual else None
+        self.tr = np.zeros(max(int(trace), 1) * 8)
+        s = _abi.SimResult()
+        for k, v in self.out.items():
+            setattr(s, k, _p(v, _abi.f64p))
+        s.converged = _p(self.conv, _abi.u8p)
+        s.valid = _p(self.valid, _abi.u8p)
+        s.active = _p(self.active, _abi.u8p)
+        s.n_iter = _p(self.n_iter, _abi.i32p)
+        s.residual = _p(self.res, _abi.f32p)
+        s.trace = _p(self.tr, _abi.f64p)
+        s.trace_cap = int(trace)
+        self.result = s
+        self.desc = model.desc()
+        self.im = _rowmajor32(img)
+        self.iv = None if inv_var is None else _rowmajor32(inv_var)
+
+    def run(self, ctx):
+        rc = ctx.lib.cp_fit_simultaneous(ctx.handle, C.byref(self.desc), _p(self.im, _abi.f32p),
+                                         _p(self.iv, _abi.f32p), self.ny, self.nx, _p(self.y0, _abi.f64p),
+                                         _p(self.x0, _abi.f64p), _p(self.f0, _abi.f64p), self.n, self.fit_rad,
+                                         C.byref(self.opts), C.byref(self.result))
+        ctx.check(rc)
+
+    def finish(self):
+        n, s, tr = self.n, self.result, self.tr
+        nanv = np.full(n, np.nan)
+        msgs = [f"star {i + 1}: excluded before the solve (no usable pixels or non-finite initial parameters)"
+                for i in range(n) if not self.active[i]]
+        resid = None if self.res is None else self.res.reshape(self.ny, self.nx)
+        if self.show_trace:
+            for row in tr[: 8 * s.trace_len].reshape(-1, 8):
+                print(f"Iter {int(row[0]):4d} trial {int(row[1]):2d} | cost = {row[2]:.4e} -> {row[3]:.4e} | "
+                      f"lambda = {row[4]:.2e} | {'accepted' if row[3] < row[2] else 'rejected'}")
+        out = self.out
+        return MultiPassPhotResult(
+            y=out["y"], x=out["x"], y_err=out["y_err"], x_err=out["x_err"], flux=out["flux"],
+            flux_err=out["flux_err"], bkg=out["bkg"], bkg_err=out["bkg_err"], converged=self.conv.astype(bool),
+            valid=self.valid.astype(bool), chisq=out["chisq"], qfit=nanv.copy(), qfit_expected=nanv.copy(),
+            qfit_z=nanv.copy(), crowding=nanv.copy(), spread_model=nanv.copy(), spread_model_err=nanv.copy(),
+            n_iter=self.n_iter.astype(np.int64), n_passes=s.n_passes, n_failed=s.n_failed, failure_msgs=msgs,
+            residual=resid,
+            stats=dict(n_active=s.n_active, n_union=s.n_union, cost=s.cost_final, n_trials=s.n_trials_total,
+                       n_inner=s.n_inner_total, trace=tr[: 8 * s.trace_len].reshape(-1, 8).copy()))
+
+
+def fit_all_stars_simultaneous(image, psf, sources, fit_rad, *, ctx=None, **kw):
     """Simultaneous damped Gauss-Newton PSF photometry on the GPU, with the
-    reference's keyword arguments and result (psf_photometry_simultaneous.jl:829-952).
+    reference's keyword arguments and result (psf_photometry_simultaneous.jl:829-952):
+    fixed, inv_var, solver, inner_iterations, linear_tol, model_rad, model_rad_max,
+    model_rad_nsigma, max_step, max_trials, max_iter, x_tol, f_tol, g_tol, lambda_init,
+    lambda_up, lambda_down, lambda_min, lambda_max, damping, show_trace,
+    covariance_estimator, spread_model_fwhm.
 
     Build additions: ``precision`` ("mixed": float32 pixels/Jacobian/residuals
     with float64 reductions and star vectors; "fp64"), ``trace`` (rows of
-    per-trial (iter, trial, cost, cost_cand, lambda) to keep), ``tile``.
+    per-trial (iter, trial, cost, cost_cand, lambda, n_live, inner its) to keep),
+    ``residual`` (False skips the residual image), ``ctx``.
     """
-    fixed = dict(fixed or {})
-    if max_iter <= 0:
-        raise ArgumentError("max_iter must be positive")
-    if inner_iterations <= 0:
-        raise ArgumentError("inner_iterations must be positive")
-    if not linear_tol > 0:
-        raise ArgumentError("linear_tol must be positive")
-    if max_trials <= 0:
-        raise ArgumentError("max_trials must be positive")
-    if not (model_rad == "auto" or (np.isscalar(model_rad) and model_rad > 0)):
-        raise ArgumentError(f"model_rad must be :auto or a positive real, got {model_rad!r}")
-    if model_rad_max is None:
-        model_rad_max = 10 * fit_rad
-    if not model_rad_max >= fit_rad:
-        raise ArgumentError("model_rad_max must be >= fit_rad")
-    if not model_rad_nsigma > 0:
-        raise ArgumentError("model_rad_nsigma must be positive")
-    if solver not in ("lsqr", "lsmr"):
-        raise ArgumentError(f"solver must be :lsqr or :lsmr, got {solver!r}")
-    if solver == "lsmr":
-        raise ArgumentError("solver :lsmr is not built on the GPU; use :lsqr")
-    if damping == "levenberg":
-        raise ArgumentError("LevenbergDamping is not supported by fit_all_stars_simultaneous: it conflicts "
-                            "with the column-equilibrated x_tol scaling. Use MarquardtDamping.")
-    if not fit_rad > 0:
-        raise ArgumentError("fit_rad must be positive")
-    img = np.asarray(image)
-    if img.ndim != 2:
-        raise ArgumentError("image must be a 2-D array")
-    ny, nx = img.shape
-    if inv_var is not None and np.shape(inv_var) != img.shape:
-        raise ArgumentError("`inv_var` must be the same size as `image`")
-    if model_rad == "auto" and inv_var is None:
-        raise ArgumentError("model_rad=:auto requires inv_var (it needs a noise scale); pass inv_var or an "
-                            "explicit scalar model_rad")
-    model, mask = _sim_psf_and_mask(psf, fixed)
-    y0, x0, f0 = _catalog(sources)
-    n = len(y0)
-    o = _abi.sim_opts_default()
-    o.solver = _abi.CP_SOLVER_LSQR
-    o.inner_iterations = inner_iterations
-    o.linear_tol = linear_tol
-    o.model_rad = 0.0 if model_rad == "auto" else float(model_rad)
-    o.model_rad_max = float(model_rad_max)
-    o.model_rad_nsigma = model_rad_nsigma
-    o.max_step = max_step
-    o.max_trials = max_trials
-    o.max_iter = int(max_iter)
-    o.x_tol, o.f_tol, o.g_tol = x_tol, f_tol, g_tol
-    o.lambda_init, o.lambda_up, o.lambda_down = lambda_init, lambda_up, lambda_down
-    o.lambda_min, o.lambda_max = lambda_min, lambda_max
-    o.covariance = {None: _abi.CP_COV_DEFAULT, "known": _abi.CP_COV_KNOWN,
-                    "reweighted": _abi.CP_COV_REWEIGHTED, "none": _abi.CP_COV_NONE}[covariance_estimator]
-    o.free_mask = mask
-    o.fixed_y = float(fixed.get("y", 0.0))
-    o.fixed_x = float(fixed.get("x", 0.0))
-    o.fixed_flux = float(fixed.get("flux", 0.0))
-    o.fixed_bkg = float(fixed.get("bkg", 0.0))
-    o.precision = {"mixed": _abi.CP_PREC_MIXED, "fp64": _abi.CP_PREC_FP64}[precision]
-    o.tile = int(tile)
-
-    out = {k: np.zeros(n) for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq")}
-    conv = np.zeros(n, np.uint8)
-    valid = np.zeros(n, np.uint8)
-    active = np.zeros(n, np.uint8)
-    n_iter = np.zeros(n, np.int32)
-    res = np.zeros(ny * nx, np.float32) if residual else None
-    tr = np.zeros(max(int(trace), 1) * 8)
-    s = _abi.SimResult()
-    for k, v in out.items():
-        setattr(s, k, _p(v, _abi.f64p))
-    s.converged = _p(conv, _abi.u8p)
-    s.valid = _p(valid, _abi.u8p)
-    s.active = _p(active, _abi.u8p)
-    s.n_iter = _p(n_iter, _abi.i32p)
-    s.residual = _p(res, _abi.f32p)
-    s.trace = _p(tr, _abi.f64p)
-    s.trace_cap = int(trace)
-    ctx = ctx or default_context()
-    d = model.desc()
-    im = _colmajor32(img)
-    iv = None if inv_var is None else _colmajor32(inv_var)
-    rc = ctx.lib.cp_fit_simultaneous(ctx.handle, C.byref(d), _p(im, _abi.f32p), _p(iv, _abi.f32p), ny, nx,
-                                     _p(y0, _abi.f64p), _p(x0, _abi.f64p), _p(f0, _abi.f64p), n,
-                                     float(fit_rad), C.byref(o), C.byref(s))
-    ctx.check(rc)
-    nanv = np.full(n, np.nan)
-    msgs = [f"star {i + 1}: excluded before the solve (no usable pixels or non-finite initial parameters)"
-            for i in range(n) if not active[i]]
-    resid = None if res is None else res.reshape(nx, ny).T
-    if show_trace:
-        for row in tr[: 8 * s.trace_len].reshape(-1, 8):
-            print(f"Iter {int(row[0]):4d} trial {int(row[1]):2d} | cost = {row[2]:.4e} -> {row[3]:.4e} | "
-                  f"lambda = {row[4]:.2e} | {'accepted' if row[3] < row[2] else 'rejected'}")
-    return MultiPassPhotResult(
-        y=out["y"], x=out["x"], y_err=out["y_err"], x_err=out["x_err"], flux=out["flux"],
-        flux_err=out["flux_err"], bkg=out["bkg"], bkg_err=out["bkg_err"], converged=conv.astype(bool),
-        valid=valid.astype(bool), chisq=out["chisq"], qfit=nanv.copy(), qfit_expected=nanv.copy(),
-        qfit_z=nanv.copy(), crowding=nanv.copy(), spread_model=nanv.copy(), spread_model_err=nanv.copy(),
-        n_iter=n_iter.astype(np.int64), n_passes=s.n_passes, n_failed=s.n_failed, failure_msgs=msgs,
-        residual=resid,
-        stats=dict(n_active=s.n_active, n_union=s.n_union, cost=s.cost_final, n_trials=s.n_trials_total,
-                   n_inner=s.n_inner_total, trace=tr[: 8 * s.trace_len].reshape(-1, 8).copy()))
+    call = _SimCall(image, psf, sources, fit_rad, **kw)
+    call.run(ctx or default_context())
+    return call.finish()
 
 
 # ---------------------------------------------------------------------------

@@ -114,6 +114,27 @@ def config_epsf_crowded(seed=2, shape=(4096, 4096), n=200_000, max_iter=10):
                 max_iter=max_iter)
 
 
+def serving_tile(seed, side, model="epsf", max_iter=10):
+    """One request of the serving workload: a side x side tile with
+    min(200000, max(1000, side^2 / 80)) stars, background 50 subtracted
+    exactly, Poisson + N(0, 4^2), power-law fluxes 1e2-1e6; model "epsf"
+    (the crowded-field ePSF, fit_rad 3, model_rad 12) or "gauss"
+    (CircularGaussianPSF fwhm 2.5, fit_rad 3, model_rad 7)."""
+    n = int(min(200_000, max(1000, side * side // 80)))
+    bkg, rn = 50.0, 4.0
+    y, x, f = simulate_sources(seed, n, (8.0, side - 8.0), (8.0, side - 8.0), (1e2, 1e6), "powerlaw", 1.5)
+    if model == "epsf":
+        psf, fixed, mr = epsf_c2(), {"bkg": 0.0}, 12.0
+    else:
+        psf, fixed, mr = CircularGaussianPSF(fwhm=2.5), {"fwhm": 2.5, "bkg": 0.0}, 7.0
+    truth = render_sources(psf, (side, side), y, x, f, mr) + bkg
+    img = (add_noise(truth, seed, 1.0, rn) - bkg).astype(np.float32)
+    inv_var = np.full((side, side), 1.0 / (bkg + rn * rn), np.float32)
+    y0, x0, f0 = _perturb(seed, y, x, f)
+    return dict(image=img, inv_var=inv_var, psf=psf, fixed=fixed, sources={"y": y0, "x": x0, "flux": f0},
+                truth=(y, x, f), fit_rad=3, model_rad=mr, max_iter=max_iter, n=n, side=side, model=model)
+
+
 def config_plummer(seed=4, shape=(8192, 8192), n=1_000_000, scale=1500.0, max_iter=10):
     """As the crowded ePSF field at 8192^2 with a Plummer-clustered population."""
     ny, nx = shape

@@ -75,6 +75,9 @@ typedef struct cp_psf_desc {
 #define CP_PREC_MIXED 0        /* FP32 pixels/Jacobian/residuals, FP64 reductions + star vectors */
 #define CP_PREC_FP64 1         /* everything FP64 (parity runs) */
 
+#define CP_LAYOUT_COLMAJOR 0   /* image[(gy-1) + (gx-1)*ny]: Julia's own layout */
+#define CP_LAYOUT_ROWMAJOR 1   /* image[(gy-1)*nx + (gx-1)]: C / numpy order, transposed on the device */
+
 #define CP_COV_DEFAULT 0       /* known weights if inv_var given, else reweighted */
 #define CP_COV_KNOWN 1
 #define CP_COV_REWEIGHTED 2
@@ -101,6 +104,7 @@ typedef struct cp_sim_opts {
     double fixed_y, fixed_x, fixed_flux, fixed_bkg;  /* fixed values (bkg is always fixed) */
     int32_t precision;         /* CP_PREC_* */
     int32_t tile;              /* reserved, ignored: the render bins stars into 64 x 64 px tiles */
+    int32_t layout;            /* CP_LAYOUT_*: image, inv_var and residual layout (default column-major) */
 } cp_sim_opts;
 
 /* Caller-allocated outputs, catalogue order (length n_stars unless noted).
@@ -251,6 +255,50 @@ int cp_render_sources(const cp_psf_desc* psf, int32_t ny, int32_t nx, const doub
                       const double* x, const double* flux, int32_t n, double model_radius,
                       double* image);
 int cp_add_noise(uint64_t seed, double* image, int64_t npix, double gain, double read_noise);
+
+/* ---- serving: continuous batching of simultaneous fits ---------------------
+ * No counterpart in the reference, which fits one image per call: for callers
+ * that stream many image tiles.  A server owns n_slots contexts (one HIP
+ * stream and device-block cache each) on one device.  Requests are admitted
+ * in arrival order as soon as a slot is free and their HBM estimate
+ * (cp_request_bytes) fits beside the running ones within hbm_budget (0 = no
+ * cap); admitted fits overlap on the device.  Each result is the one the
+ * request gets alone (its own buffers and stream; fixed-order reductions).
+ * The caller keeps every array of a request (inputs, the psf grids and node
+ * coordinates, the result arrays) alive until cp_server_wait returns for it.
+ * cp_server_destroy finishes running requests and drops queued ones; call it
+ * only when no thread is waiting. */
+typedef struct cp_server cp_server;
+
+typedef struct cp_request {
+    const cp_psf_desc* psf;     /* copied at submit */
+    const float* image;         /* ny*nx column-major */
+    const float* inv_var;       /* may be NULL */
+    int32_t ny, nx;
+    const double *y0, *x0, *flux0;
+    int32_t n_stars;
+    double fit_rad;
+    const cp_sim_opts* opts;    /* copied at submit */
+    cp_sim_result* out;         /* filled by the time cp_server_wait returns */
+    int32_t device_inputs;      /* 1: image, inv_var, y0, x0, flux0 are device pointers */
+} cp_request;
+
+typedef struct cp_request_stat {
+    int32_t rc;                 /* the fit's return code */
+    int32_t slot;               /* worker slot that ran it */
+    double t_submit, t_start, t_end;   /* seconds, one monotonic host clock */
+    uint64_t hbm_bytes;         /* admission estimate charged against the budget */
+} cp_request_stat;
+
+uint64_t cp_request_bytes(int32_t ny, int32_t nx, int32_t n_stars, double fit_rad, double model_rad,
+                          int32_t precision, int32_t device_inputs);
+int cp_server_create(cp_server** out, int32_t device, uint64_t hbm_budget, int32_t n_slots);
+int cp_server_submit(cp_server* s, const cp_request* r, uint64_t* ticket);
+int cp_server_poll(cp_server* s, uint64_t ticket, int32_t* done);
+/* Blocks until the request finished; returns its rc (message in msg). */
+int cp_server_wait(cp_server* s, uint64_t ticket, cp_request_stat* st, char* msg, int32_t msg_cap);
+const char* cp_server_last_error(const cp_server* s);
+void cp_server_destroy(cp_server* s);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     structs = {"cp_psf_desc": abi.PsfDesc, "cp_sim_opts": abi.SimOpts, "cp_sim_result": abi.SimResult,
-               "cp_lm_opts": abi.LMOpts, "cp_lm_stat": abi.LMStat}
+               "cp_lm_opts": abi.LMOpts, "cp_lm_stat": abi.LMStat, "cp_request": abi.Request,
+               "cp_request_stat": abi.RequestStat}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -67,6 +68,27 @@ def test_defaults_match_reference_keywords():
     assert lib.cp_psf_nparams(abi.CP_PSF_CIRCULAR_GAUSSIAN) == 5
     assert lib.cp_psf_nparams(abi.CP_PSF_CIRCULAR_MOFFAT) == 6
     assert lib.cp_psf_nparams(abi.CP_PSF_GRIDDED_IMAGE) == 4
+
+
+def test_request_bytes_estimate():
+    lib = abi.load_library()
+    b32 = lib.cp_request_bytes(4096, 4096, 200_000, 3.0, 12.0, abi.CP_PREC_MIXED, 1)
+    b64 = lib.cp_request_bytes(4096, 4096, 200_000, 3.0, 12.0, abi.CP_PREC_FP64, 1)
+    host = lib.cp_request_bytes(4096, 4096, 200_000, 3.0, 12.0, abi.CP_PREC_MIXED, 0)
+    # images (6 float32 planes + mask), stamp rows and the stamp structure dominate
+    assert 0.5e9 < b32 < 2e9 and b64 > b32 and host > b32
+    assert lib.cp_request_bytes(0, 10, 5, 3.0, 12.0, 0, 0) == 0
+    assert lib.cp_request_bytes(10, 10, 5, 0.0, 12.0, 0, 0) == 0
+
+
+def test_server_argument_errors_without_gpu():
+    lib = abi.load_library()
+    h = C.c_void_p()
+    assert lib.cp_server_create(C.byref(h), 0, 0, 0) == abi.CP_EARG
+    assert lib.cp_server_create(None, 0, 0, 1) == abi.CP_EARG
+    t = C.c_uint64()
+    assert lib.cp_server_submit(None, None, C.byref(t)) == abi.CP_ESTATE
+    assert lib.cp_server_poll(None, 1, None) == abi.CP_ESTATE
 
 
 def test_null_context_is_rejected_without_gpu():
