@@ -224,6 +224,43 @@ static std::vector<int> auto_radii(const HostPsf& P, double nsigma, int R_fit, i
 // ---------------------------------------------------------------------------
 // the fit
 // ---------------------------------------------------------------------------
+
+// Instantiated HIP graphs of one fit's trial body (inner solve, step,
+// candidate render and cost), keyed by the one argument that changes between
+// trials (which of the two residual buffers is current).
+struct TrialGraphs {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    const void* key[2] = {nullptr, nullptr};
+    ~TrialGraphs() {
+        for (auto& e : exec)
+            if (e) (void)hipGraphExecDestroy(e);
+    }
+    // Launch the graph for `key`, capturing `body` on `st` the first time.
+    template <typename F>
+    void run(hipStream_t st, const void* k, F&& body) {
+        int slot = key[0] == k ? 0 : (key[1] == k ? 1 : (key[0] == nullptr ? 0 : 1));
+        if (key[slot] != k) {
+            if (exec[slot]) CP_HIP(hipGraphExecDestroy(exec[slot]));
+            exec[slot] = nullptr;
+            hipGraph_t g = nullptr;
+            CP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                hipGraph_t junk = nullptr;
+                (void)hipStreamEndCapture(st, &junk);
+                if (junk) (void)hipGraphDestroy(junk);
+                throw;
+            }
+            CP_HIP(hipStreamEndCapture(st, &g));
+            hipError_t e = hipGraphInstantiate(&exec[slot], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            CP_HIP(e);
+            key[slot] = k;
+        }
+        CP_HIP(hipGraphLaunch(exec[slot], st));
+    }
+};
 template <typename T, int KIND>
 struct Sim {
     cp_ctx* ctx;
@@ -538,6 +575,8 @@ struct Sim {
         // iteration renders it separately.
         double lam = o.lambda_init, cost = 0;
         int n_run = 0, n_live_now = na;
+        TrialGraphs graphs;
+        const bool use_graphs = !ctx->prof;   // per-launch event timing needs plain launches
         bool D_init = false, have_resid = false;
         for (int iter = 1; iter <= o.max_iter; ++iter) {
             star_rec(theta.as<double>(), recs, crs, ers, 0, 0);
@@ -573,29 +612,34 @@ struct Sim {
                 const double lam_solve = std::sqrt(lam);
                 launch(k_lsqr_init, dim3(1), dim3(64), 0, st, Sc, lam_solve, o.inner_iterations, o.linear_tol,
                        o.linear_tol);
-                for (int it = 1; it <= o.inner_iterations; ++it) {
-                    const int first = it == 1;
-                    launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, csr.U, (const int*)csr.pix.as<int>(),
-                             (const int*)csr.ptr.as<int>(), (const int*)csr.ent.as<int>(), S2, (const T*)V.as<T>(),
-                             (const double*)(first ? bsc.as<double>() : tb.as<double>()),
-                             (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(), first, part.as<double>(), Sc);
-                    finish(k_finish<FIN_BETA>, gb_jv, it);
-                    jtu_launch(S2, dim3(gb_jtu), na, (const int*)ay.as<int>(), (const int*)ax.as<int>(),
-                               (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx, (const T*)V.as<T>(),
-                               (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(), tb.as<double>(),
-                               xs.as<double>(), wl.as<double>(), first, it, part.as<double>(), part2.as<double>(), Sc);
-                    finish(k_finish<FIN_LSQR>, gb_jtu, it);
-                }
-                launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
-                       (const double*)wl.as<double>(), xs.as<double>(), (const double*)colnorm.as<double>(),
-                       (const double*)theta.as<double>(), dsc.as<double>(), theta_c.as<double>(),
-                       (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
-                star_rec(theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
-                render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
-                                         ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
-                                         umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cand,
-                                         nullptr, part.as<double>(), Sc);
-                finish(k_finish<FIN_COST_CAND>, gb_render, 0);
+                // inner solve, capped step and candidate cost: one graph launch per trial
+                auto trial_body = [&]() {
+                    for (int it = 1; it <= o.inner_iterations; ++it) {
+                        const int first = it == 1;
+                        launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, csr.U, (const int*)csr.pix.as<int>(),
+                                 (const int*)csr.ptr.as<int>(), (const int*)csr.ent.as<int>(), S2, (const T*)V.as<T>(),
+                                 (const double*)(first ? bsc.as<double>() : tb.as<double>()),
+                                 (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(), first, part.as<double>(), Sc);
+                        finish(k_finish<FIN_BETA>, gb_jv, it);
+                        jtu_launch(S2, dim3(gb_jtu), na, (const int*)ay.as<int>(), (const int*)ax.as<int>(),
+                                   (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx, (const T*)V.as<T>(),
+                                   (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(), tb.as<double>(),
+                                   xs.as<double>(), wl.as<double>(), first, it, part.as<double>(), part2.as<double>(), Sc);
+                        finish(k_finish<FIN_LSQR>, gb_jtu, it);
+                    }
+                    launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
+                           (const double*)wl.as<double>(), xs.as<double>(), (const double*)colnorm.as<double>(),
+                           (const double*)theta.as<double>(), dsc.as<double>(), theta_c.as<double>(),
+                           (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
+                    star_rec(theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
+                    render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
+                                             ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
+                                             umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cand,
+                                             nullptr, part.as<double>(), Sc);
+                    finish(k_finish<FIN_COST_CAND>, gb_render, 0);
+                };
+                if (use_graphs) graphs.run(st, wres_cur, trial_body);
+                else trial_body();
                 read_S();
                 const double cost_c = hS->cost_cand;
                 out->n_trials_total += 1;
