@@ -479,14 +479,40 @@ static_assert((RT / RSB) * (RT / RSB) == RWAVE, "one sub-block per wave");
 // star with one LDS read.  Star records are then read with wave-uniform
 // addresses (scalar loads, kept in SGPRs).  Per-pixel sums run over stars in
 // list (Morton) order: deterministic and the reference's summation order.
+// Render cost of a tile: its star-list length plus a fixed per-tile term.
+constexpr int TILE_COST0 = 16;
+static __global__ void k_tile_cost(const int* __restrict__ tstart, const int* __restrict__ tend, int ntiles,
+                                   long long* __restrict__ cost) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t < ntiles) cost[t] = (long long)(tend[t] - tstart[t]) + TILE_COST0;
+}
+
+// Tiles [t0, t1) of logical block b of nb: equal shares of the inclusive cost
+// scan (tiles whose scan value falls in (total b / nb, total (b + 1) / nb]).
+__device__ __forceinline__ void tile_share(const long long* __restrict__ scan, int ntiles, int b, int nb, int& t0,
+                                           int& t1) {
+    const long long total = scan[ntiles - 1];
+    auto upper = [&](long long v) {   // number of tiles with scan <= v
+        int lo = 0, hi = ntiles;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (scan[mid] <= v) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    t0 = upper(total * b / nb);
+    t1 = upper(total * (b + 1) / nb);
+}
+
 // float renders fit 8 waves / SIMD: two 1024-thread blocks per CU
 template <typename T, int KIND, int MODE, bool LDSG>
 __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
     const CornerRec<T>* __restrict__ crs, const int* __restrict__ tstart, const int* __restrict__ tend,
-    const int* __restrict__ list, TileGeo g, int ntiles, int tiles_per_block, const uint8_t* __restrict__ umask,
-    T* __restrict__ dw, const T* __restrict__ w, const T* __restrict__ data, T* __restrict__ wt_resid,
-    T* __restrict__ resid, double* __restrict__ partials, Scal* S) {
+    const int* __restrict__ list, TileGeo g, int ntiles, const long long* __restrict__ tscan,
+    const uint8_t* __restrict__ umask, T* __restrict__ dw, const T* __restrict__ w, const T* __restrict__ data,
+    T* __restrict__ wt_resid, T* __restrict__ resid, double* __restrict__ partials, Scal* S) {
     constexpr bool EPSF_LDS = KIND == KIND_EPSF && LDSG;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[RWAVE];
@@ -501,7 +527,8 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int sbr = (wv % (RT / RSB)) * RSB, sbc = (wv / (RT / RSB)) * RSB;
     double acc = 0;
-    const int t0 = xcd_remap(blockIdx.x, gridDim.x) * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
+    int t0, t1;
+    tile_share(tscan, ntiles, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, t0, t1);
     for (int tile = t0; tile < t1; ++tile) {
         const int ty = tile % g.nty, tx = tile / g.nty;
         const int ty0 = ty * RT + 1, tx0 = tx * RT + 1;       // tile origin (1-based)

@@ -306,17 +306,17 @@ struct Sim {
     template <int MODE>
     void render_launch(bool ldsg, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
                        const EpsfRec<T>* ers, const CornerRec<T>* crs, const TileLists& L, const TileGeo& g,
-                       int ntiles, int tpb, const uint8_t* umask, T* dw, const T* w, const T* data, T* wres,
-                       T* resid, double* part, Scal* Sc) {
+                       int ntiles, const long long* tscan, const uint8_t* umask, T* dw, const T* w, const T* data,
+                       T* wres, T* resid, double* part, Scal* Sc) {
         static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
         if (ldsg)
             launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                     ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
+                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc);
         else
             launch_n(nm[MODE], k_render<T, KIND, MODE, false>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                     ntiles, tpb, umask, dw, w, data, wres, resid, part, Sc);
+                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc);
     }
 
     void run(const float* d_img, const float* d_iv, int ny, int nx, const double* d_y0, const double* d_x0,
@@ -483,6 +483,18 @@ struct Sim {
         build_pixel_csr<T>(ctx, cub, na, ay.as<int>(), ax.as<int>(), R, S, (const T*)w.as<T>(), ny, nx, csr);
         build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), o.model_rad > 0 ? nullptr : mR.as<int>(),
                     mR_scalar, g, Lmod);
+        // cost-balanced tile shares of the render blocks
+        DevBuf tcost(ctx, (size_t)ntiles * 8), tscan(ctx, (size_t)ntiles * 8);
+        launch(k_tile_cost, dim3(cdiv(ntiles, BLOCK)), dim3(BLOCK), 0, st, (const int*)Lmod.start.as<int>(),
+               (const int*)Lmod.end.as<int>(), ntiles, tcost.as<long long>());
+        {
+            size_t b = 0;
+            CP_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, b, tcost.as<long long>(), tscan.as<long long>(), ntiles, st));
+            cub.ensure(b);
+            b = cub.tmp.bytes();
+            CP_HIP(hipcub::DeviceScan::InclusiveSum(cub.tmp.get(), b, tcost.as<long long>(), tscan.as<long long>(),
+                                                    ntiles, st));
+        }
         cub.tmp.release();
 
         // --- working buffers
@@ -527,10 +539,9 @@ struct Sim {
         const size_t render_sh = grid_sh + RCHUNK * 2 * sizeof(int);
         const size_t fill_sh = grid_sh;
 
-        // two 1024-thread render / fill blocks fit one CU (LDS and wave slots)
-        const int nb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 2);
-        const int tpb_render = (int)cdiv(ntiles, nb_render);
-        const int gb_render = (int)cdiv(ntiles, tpb_render);
+        // two 1024-thread render / fill blocks fit one CU (LDS and wave slots);
+        // render blocks take equal shares of the tile-cost scan
+        const int gb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 4);
         const int gb_jv = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * 8));
         const int GLs = S2 <= 256 ? 16 : 64;
         const int gb_fill = (int)std::min<int64_t>(cdiv(na, FBLOCK / 64), (int64_t)n_cu * 2);
@@ -582,7 +593,7 @@ struct Sim {
             star_rec(theta.as<double>(), recs, crs, ers, 0, 0);
             if (!have_resid) {
                 render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
-                                          ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
+                                          ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, (const long long*)tscan.as<long long>(),
                                           umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
                                           part.as<double>(), Sc);
                 finish(k_finish<FIN_COST>, gb_render, 0);
@@ -633,7 +644,7 @@ struct Sim {
                            (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
                     star_rec(theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
                     render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
-                                             ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, tpb_render,
+                                             ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, (const long long*)tscan.as<long long>(),
                                              umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cand,
                                              nullptr, part.as<double>(), Sc);
                     finish(k_finish<FIN_COST_CAND>, gb_render, 0);
@@ -690,8 +701,9 @@ struct Sim {
                     star_rec(theta.as<double>(), recs, crs, ers, iter, 0);
                     render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
                                                ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles,
-                                               tpb_render, umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(),
-                                               wres_cur, nullptr, part.as<double>(), Sc);
+                                               (const long long*)tscan.as<long long>(), umask.as<uint8_t>(),
+                                               dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
+                                               part.as<double>(), Sc);
                 }
                 if (hS->n_live == 0) break;
             }
@@ -716,7 +728,7 @@ struct Sim {
         T* resid = ubuf.as<T>();
         render_launch<MODE_FINAL>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(), ers.as<EpsfRec<T>>(),
                                   crs.as<CornerRec<T>>(),
-                                  Lmod, g, ntiles, tpb_render, umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
+                                  Lmod, g, ntiles, (const long long*)tscan.as<long long>(), umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
                                   data.as<T>(), wres_cur, resid, part.as<double>(), Sc);
         DevBuf chis(ctx, (size_t)na * 8);
         if (out->chisq)
