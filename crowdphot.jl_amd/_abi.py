@@ -99,6 +99,7 @@ class SimOpts(C.Structure):
         ("precision", C.c_int32),
         ("tile", C.c_int32),
         ("layout", C.c_int32),
+        ("spread_model_fwhm", C.c_double),
     ]
 
 
@@ -118,6 +119,8 @@ class SimResult(C.Structure):
         ("n_trials_total", C.c_int32), ("n_inner_total", C.c_int32),
         ("trace", f64p),
         ("trace_cap", C.c_int32), ("trace_len", C.c_int32),
+        ("qfit", f64p), ("qfit_expected", f64p), ("qfit_z", f64p), ("crowding", f64p),
+        ("spread_model", f64p), ("spread_model_err", f64p),
     ]
 
 
@@ -262,6 +265,7 @@ SIGNATURES = {
     "cp_render_sources": (C.c_int, [C.POINTER(PsfDesc), C.c_int32, C.c_int32, f64p, f64p, f64p,
                                     C.c_int32, C.c_double, f64p]),
     "cp_add_noise": (C.c_int, [C.c_uint64, f64p, C.c_int64, C.c_double, C.c_double]),
+    "cp_spread_kernel": (C.c_int, [C.c_double, C.c_int32, f64p]),
     "cp_request_bytes": (C.c_uint64, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                       C.c_int32]),
     "cp_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_uint64, C.c_int32]),

@@ -1308,34 +1308,134 @@ static __global__ void k_errors(int na, int R, int S, const int* __restrict__ ay
     for (int k = 0; k < p; ++k) errs[(size_t)a * p + k] = sqrt(fmax(0.0, fac * cov[k * 3 + k]));
 }
 
-// Per-star reduced chi^2 over the +-fit_rad box at the fitted position (:1368-1397).
-template <typename T>
-static __global__ void k_chisq(int na, int p, const double* __restrict__ theta, FreeMap fm, double fit_rad,
-                        const T* __restrict__ resid, const float* __restrict__ iv, int ny, int nx,
-                        double* __restrict__ chisq) {
-    int a = blockIdx.x * BLOCK + threadIdx.x;
+// Post-fit per-star statistics over the +-fit_rad box at the fitted position
+// (psf_photometry_simultaneous.jl:1344-1397; _star_diagnostics!,
+// psf_photometry_diagnostics.jl:154-231): one wave per star.  The star's
+// model over the box is staged in LDS and correlated (zero border) with the
+// spread_model reference disk K of half-width kh (K = null: spread_model
+// stays NaN).  resid is the global residual (data minus every star's model),
+// data the working image, w the weights (inverse variance, 0 = masked; all
+// ones without inv_var).  out[a * 7 + ...] = chisq, qfit, qfit_expected,
+// qfit_z, crowding, spread_model, spread_model_err (NaN where the reference
+// leaves NaN, chisq 0).
+constexpr int DIAG_MAXB = 1024;   // box pixels: (2 ceil(fit_rad) + 2)^2 for fit_rad <= 15
+
+template <typename T, int KIND>
+static __global__ __launch_bounds__(64) void k_diag(int na, const double* __restrict__ theta, FreeMap fm,
+                                                    double fit_rad, ModelConst<T> M, double inv_norm_g64,
+                                                    double moffat_amp64, NodeGrid G, const T* __restrict__ resid,
+                                                    const T* __restrict__ data, const T* __restrict__ w,
+                                                    int has_iv, int ny, int nx, const double* __restrict__ K,
+                                                    int kh, double* __restrict__ out) {
+    __shared__ T sm[DIAG_MAXB];
+    const int a = blockIdx.x, lane = threadIdx.x;
     if (a >= na) return;
+    double* o = out + (size_t)a * 7;
     double Y, X, F;
-    star_yxf(fm, theta + (size_t)a * p, Y, X, F);
-    chisq[a] = 0;
-    if (!(isfinite(Y) && isfinite(X) && isfinite(F) && F > 0)) return;
-    int ylo = max((int)floor(Y - fit_rad), 1), yhi = min((int)ceil(Y + fit_rad), ny);
-    int xlo = max((int)floor(X - fit_rad), 1), xhi = min((int)ceil(X + fit_rad), nx);
-    if (ylo > yhi || xlo > xhi) return;
-    double num = 0;
-    int good = 0;
-    for (int gx = xlo; gx <= xhi; ++gx)
-        for (int gy = ylo; gy <= yhi; ++gy) {
-            size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * ny;
-            double ivv = iv ? (double)iv[q] : 1.0;
-            if (isfinite(ivv) && ivv > 0) {
-                double r = (double)resid[q];
-                num += ivv * r * r;
-                ++good;
-            }
+    star_yxf(fm, theta + (size_t)a * fm.p, Y, X, F);
+    const bool valid = isfinite(Y) && isfinite(X) && isfinite(F) && F > 0;
+    int ylo = 1, yhi = 0, xlo = 1, xhi = 0;
+    if (valid) {
+        ylo = max((int)floor(Y - fit_rad), 1);
+        yhi = min((int)ceil(Y + fit_rad), ny);
+        xlo = max((int)floor(X - fit_rad), 1);
+        xhi = min((int)ceil(X + fit_rad), nx);
+    }
+    const int by = yhi - ylo + 1, bx = xhi - xlo + 1;
+    if (!valid || by <= 0 || bx <= 0) {
+        if (lane == 0) {
+            o[0] = 0;
+            for (int k = 1; k < 7; ++k) o[k] = NAN;
         }
-    int den = good - p;
-    if (den > 0) chisq[a] = num / den;
+        return;
+    }
+    const int nb = by * bx;
+    StarRec<T> r{};
+    r.flux = (T)F;
+    if (KIND == KIND_GAUSS) r.amp = (T)(F * inv_norm_g64);
+    else if (KIND == KIND_MOFFAT) r.amp = (T)(F * moffat_amp64);
+    else r.amp = (T)F;
+    CornerRec<T> cr;
+    if (KIND == KIND_EPSF) grid_corners<T>(G, M.n_nodes, Y, X, cr);
+    for (int m = lane; m < nb; m += 64) {
+        const int iy = m % by, ix = m / by;
+        sm[m] = star_value<T, KIND>(M, r, (T)((ylo + iy) - Y), (T)((xlo + ix) - X), M.grids, &cr);
+    }
+    __syncthreads();
+    const double bkg = (double)M.bkg, inv_flux = 1.0 / F;
+    double chi = 0, qfit = 0, nclean = 0, ndirty = 0, dcrowd = 0, Sp = 0, Sq = 0, Sp0 = 0, Sq0 = 0, SqG = 0;
+    double ssum = 0, s2sum = 0, ngood = 0;
+    const int kw = 2 * kh + 1;
+    for (int m = lane; m < nb; m += 64) {
+        const int iy = m % by, ix = m / by;
+        const size_t q = (size_t)(ylo + iy - 1) + (size_t)(xlo + ix - 1) * ny;
+        const double wp = (double)w[q];
+        if (!(wp > 0)) continue;
+        const double rq = (double)resid[q], mv = (double)sm[m];
+        chi += wp * rq * rq;
+        ngood += 1;
+        qfit += fabs(rq);
+        const double Pp = (mv - bkg) * inv_flux, wP = wp * Pp, pi = rq + mv - bkg;
+        nclean += wP * pi;
+        ndirty += wP * ((double)data[q] - bkg);
+        dcrowd += wP * Pp;
+        if (K) {
+            double gm = 0;
+            for (int bb = -kh; bb <= kh; ++bb) {
+                const int jx = ix + bb;
+                if (jx < 0 || jx >= bx) continue;
+                for (int aa = -kh; aa <= kh; ++aa) {
+                    const int jy = iy + aa;
+                    if (jy < 0 || jy >= by) continue;
+                    gm += (double)sm[jy + jx * by] * K[(aa + kh) + (bb + kh) * kw];
+                }
+            }
+            const double gi = (gm - bkg) * inv_flux;
+            Sp += wP * pi;
+            Sq += wp * gi * pi;
+            Sp0 += wP * Pp;
+            Sq0 += wp * Pp * gi;
+            SqG += wp * gi * gi;
+        }
+        if (has_iv) {
+            const double sg = 1.0 / sqrt(wp);
+            ssum += sg;
+            s2sum += sg * sg;
+        }
+    }
+    chi = wave_sum(chi);
+    ngood = wave_sum(ngood);
+    qfit = wave_sum(qfit);
+    nclean = wave_sum(nclean);
+    ndirty = wave_sum(ndirty);
+    dcrowd = wave_sum(dcrowd);
+    Sp = wave_sum(Sp);
+    Sq = wave_sum(Sq);
+    Sp0 = wave_sum(Sp0);
+    Sq0 = wave_sum(Sq0);
+    SqG = wave_sum(SqG);
+    ssum = wave_sum(ssum);
+    s2sum = wave_sum(s2sum);
+    if (lane != 0) return;
+    const int p = fm.p;
+    o[0] = ngood - p > 0 ? chi / (ngood - p) : 0.0;
+    o[1] = qfit * inv_flux;
+    o[4] = (dcrowd > 0 && nclean > 0 && ndirty > 0) ? 2.5 * log10(ndirty / nclean) : NAN;
+    o[5] = (K && Sp > 0) ? Sq / Sp - Sq0 / Sp0 : NAN;
+    if (K && Sp > 0 && has_iv) {
+        const double v = Sp * Sp * SqG + Sq * Sq * Sp0 - 2 * Sq * Sp * Sq0;
+        o[6] = sqrt(fmax(0.0, v)) / (Sp * Sp);
+    } else {
+        o[6] = NAN;
+    }
+    const double c2pi = sqrt(2.0 / M_PI);
+    o[2] = has_iv ? c2pi * ssum * inv_flux : NAN;
+    if (has_iv && s2sum > 0 && ngood > p) {
+        const double dof = sqrt(1.0 - p / ngood);
+        o[3] = (qfit - c2pi * dof * ssum) / (sqrt((1 - 2 / M_PI) * s2sum) * dof);
+    } else {
+        o[3] = NAN;
+    }
 }
 
 // dst (cols x rows, row-major) = transpose of src (rows x cols, row-major):

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <vector>
 
+#include "diag_host.h"
 #include "sim_kernels.h"
 #include "psf_host.h"
 
@@ -268,7 +269,10 @@ struct Sim {
     DevPsf& DP;
     hipStream_t st;
 
-    Sim(cp_ctx* c, const HostPsf& p, DevPsf& d) : ctx(c), P(p), DP(d), st(c->stream) {}
+    const cp_psf_desc* psfd_;
+
+    Sim(cp_ctx* c, const HostPsf& p, DevPsf& d, const cp_psf_desc* pd)
+        : ctx(c), P(p), DP(d), st(c->stream), psfd_(pd) {}
 
     template <bool FINAL, bool LDSG>
     void fill_launch(int S2, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
@@ -365,6 +369,9 @@ struct Sim {
                 out->valid[i] = valid[i];
                 out->n_iter[i] = 0;
                 if (out->chisq) out->chisq[i] = 0;
+                for (double* d : {out->qfit, out->qfit_expected, out->qfit_z, out->crowding, out->spread_model,
+                                  out->spread_model_err})
+                    if (d) d[i] = NAN;
             }
         };
         out->trace_len = 0;
@@ -730,11 +737,7 @@ struct Sim {
                                   crs.as<CornerRec<T>>(),
                                   Lmod, g, ntiles, (const long long*)tscan.as<long long>(), umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
                                   data.as<T>(), wres_cur, resid, part.as<double>(), Sc);
-        DevBuf chis(ctx, (size_t)na * 8);
-        if (out->chisq)
-            launch(k_chisq<T>, dim3(gstar), dim3(BLOCK), 0, st, na, p, (const double*)theta.as<double>(), fm, fit_rad,
-                   (const T*)resid, d_iv, ny, nx, chis.as<double>());
-        std::vector<double> hth((size_t)na * p), herr((size_t)na * p), hchi(na, 0.0);
+        std::vector<double> hth((size_t)na * p), herr((size_t)na * p);
         std::vector<uint8_t> hlive(na), hconv(na);
         std::vector<int> hfz(na);
         CP_HIP(hipMemcpyAsync(hth.data(), theta.get(), nb8, hipMemcpyDeviceToHost, st));
@@ -742,7 +745,6 @@ struct Sim {
         CP_HIP(hipMemcpyAsync(hlive.data(), live.get(), na, hipMemcpyDeviceToHost, st));
         CP_HIP(hipMemcpyAsync(hconv.data(), conv.get(), na, hipMemcpyDeviceToHost, st));
         CP_HIP(hipMemcpyAsync(hfz.data(), frozen_at.get(), (size_t)na * 4, hipMemcpyDeviceToHost, st));
-        if (out->chisq) CP_HIP(hipMemcpyAsync(hchi.data(), chis.get(), (size_t)na * 8, hipMemcpyDeviceToHost, st));
         if (out->residual) {
             if (o.layout == CP_LAYOUT_ROWMAJOR) {
                 // column-major ny x nx is row-major nx x ny: transpose to ny x nx
@@ -779,7 +781,6 @@ struct Sim {
             out->flux[i] = F;
             out->converged[i] = hconv[j];
             out->n_iter[i] = hlive[j] ? n_run : hfz[j];
-            if (out->chisq) out->chisq[i] = hchi[j];
         }
         for (int i = 0; i < n; ++i) {
             if (fm.y_fixed) out->y[i] = o.fixed_y;
@@ -788,7 +789,39 @@ struct Sim {
             if (out->valid[i] && !(std::isfinite(out->y[i]) && std::isfinite(out->x[i]) &&
                                    std::isfinite(out->flux[i]) && out->flux[i] > 0))
                 out->valid[i] = 0;
-            if (!out->valid[i] && out->chisq) out->chisq[i] = 0;
+        }
+
+        // --- per-star diagnostics over the global residual (:1344-1397,
+        // psf_photometry_diagnostics.jl:154-231); the spread_model disk is
+        // sized from catalogue star 1's fitted position unless given
+        double* diag_out[7] = {out->chisq, out->qfit, out->qfit_expected, out->qfit_z, out->crowding,
+                               out->spread_model, out->spread_model_err};
+        bool any_diag = false;
+        for (double* d : diag_out) any_diag = any_diag || d;
+        if (any_diag) {
+            const double sfw = o.spread_model_fwhm > 0 ? o.spread_model_fwhm
+                                                       : spread_fwhm_host(psfd_, out->y[0], out->x[0]);
+            const int kh = (int)std::ceil(fit_rad);
+            DevBuf dK, dd(ctx, (size_t)na * 7 * 8);
+            if (std::isfinite(sfw) && sfw > 0) {
+                std::vector<double> hK((size_t)(2 * kh + 1) * (2 * kh + 1));
+                spread_kernel_host(sfw, kh, hK.data());
+                dK.alloc(ctx, hK.size() * 8);
+                CP_HIP(hipMemcpyAsync(dK.get(), hK.data(), hK.size() * 8, hipMemcpyHostToDevice, st));
+                CP_HIP(hipStreamSynchronize(st));   // hK is pageable and local
+            }
+            launch(k_diag<T, KIND>, dim3(na), dim3(64), 0, st, na, (const double*)theta.as<double>(), fm, fit_rad, M,
+                   inv_norm_g64, moffat_amp64, DP.G, (const T*)resid, (const T*)data.as<T>(), (const T*)w.as<T>(),
+                   d_iv ? 1 : 0, ny, nx, (const double*)(dK.get() ? dK.as<double>() : nullptr), kh, dd.as<double>());
+            std::vector<double> hd((size_t)na * 7);
+            CP_HIP(hipMemcpyAsync(hd.data(), dd.get(), hd.size() * 8, hipMemcpyDeviceToHost, st));
+            CP_HIP(hipStreamSynchronize(st));
+            for (int j = 0; j < na; ++j) {
+                const int i = hact[j];
+                if (!out->valid[i]) continue;
+                for (int k = 0; k < 7; ++k)
+                    if (diag_out[k]) diag_out[k][i] = hd[(size_t)j * 7 + k];
+            }
         }
         out->n_passes = n_run;
         out->n_active = na;
@@ -846,7 +879,7 @@ void fit_sim_device(cp_ctx* ctx, const cp_psf_desc* psfd, const float* d_img, co
         CP_HIP(hipMemcpyAsync(hf.data(), d_f0, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
         CP_HIP(hipStreamSynchronize(ctx->stream));
     }
-#define CP_RUN(TT, KK) Sim<TT, KK>(ctx, P, DP).run(d_img, d_iv, ny, nx, d_y0, d_x0, d_f0, n, fit_rad, o, out, hy, hx, hf)
+#define CP_RUN(TT, KK) Sim<TT, KK>(ctx, P, DP, psfd).run(d_img, d_iv, ny, nx, d_y0, d_x0, d_f0, n, fit_rad, o, out, hy, hx, hf)
     const bool f64 = o.precision == CP_PREC_FP64;
     switch (P.kind) {
         case CP_PSF_CIRCULAR_GAUSSIAN:

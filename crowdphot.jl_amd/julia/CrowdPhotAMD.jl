@@ -72,6 +72,7 @@ mutable struct CpSimOpts
     precision::Int32
     tile::Int32
     layout::Int32
+    spread_model_fwhm::Float64
 end
 
 mutable struct CpSimResult
@@ -89,6 +90,8 @@ mutable struct CpSimResult
     n_trials_total::Int32; n_inner_total::Int32
     trace::Ptr{Float64}
     trace_cap::Int32; trace_len::Int32
+    qfit::Ptr{Float64}; qfit_expected::Ptr{Float64}; qfit_z::Ptr{Float64}; crowding::Ptr{Float64}
+    spread_model::Ptr{Float64}; spread_model_err::Ptr{Float64}
 end
 
 mutable struct Context
@@ -149,7 +152,8 @@ function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad
         model_rad_nsigma::Real = 1.0, max_step::Real = 0.25, max_trials::Int = 8, max_iter::Integer = 40,
         x_tol::Real = 1.0e-5, f_tol::Real = 1.0e-4, g_tol::Real = 1.0e-4, λ_init::Real = 1.0e-4,
         λ_up::Real = 10.0, λ_down::Real = 10.0, λ_min::Real = 1.0e-12, λ_max::Real = 1.0e12,
-        precision::Symbol = :mixed, ctx::Context = Context(0), kws...)
+        precision::Symbol = :mixed, spread_model_fwhm::Union{Nothing, Real} = nothing,
+        ctx::Context = Context(0), kws...)
     solver in (:lsqr,) || throw(ArgumentError("solver must be :lsqr on the GPU, got $(repr(solver))"))
     (model_rad === :auto || (model_rad isa Real && model_rad > 0)) ||
         throw(ArgumentError("model_rad must be :auto or a positive real, got $(repr(model_rad))"))
@@ -171,15 +175,20 @@ function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad
                   model_rad_max, model_rad_nsigma, max_step, max_trials, max_iter, x_tol, f_tol, g_tol,
                   λ_init, λ_up, λ_down, λ_min, λ_max, 0, mask,
                   Float64(get(fixed, :y, 0)), Float64(get(fixed, :x, 0)), Float64(get(fixed, :flux, 0)),
-                  Float64(get(fixed, :bkg, 0)), precision === :fp64 ? 1 : 0, 0, 0)
+                  Float64(get(fixed, :bkg, 0)), precision === :fp64 ? 1 : 0, 0, 0,
+                  spread_model_fwhm === nothing ? 0.0 : Float64(spread_model_fwhm))
     out = (y = zeros(n), x = zeros(n), flux = zeros(n), bkg = zeros(n), y_err = zeros(n), x_err = zeros(n),
            flux_err = zeros(n), bkg_err = zeros(n), chisq = zeros(n), converged = zeros(UInt8, n),
            valid = zeros(UInt8, n), n_iter = zeros(Int32, n), active = zeros(UInt8, n),
+           qfit = zeros(n), qfit_expected = zeros(n), qfit_z = zeros(n), crowding = zeros(n),
+           spread_model = zeros(n), spread_model_err = zeros(n),
            residual = zeros(Float32, ny, nx))
     r = CpSimResult(pointer(out.y), pointer(out.x), pointer(out.flux), pointer(out.bkg), pointer(out.y_err),
                     pointer(out.x_err), pointer(out.flux_err), pointer(out.bkg_err), pointer(out.chisq),
                     pointer(out.converged), pointer(out.valid), pointer(out.n_iter), pointer(out.active),
-                    pointer(out.residual), 0, 0, 0, 0, 0.0, 0, 0, C_NULL, 0, 0)
+                    pointer(out.residual), 0, 0, 0, 0, 0.0, 0, 0, C_NULL, 0, 0,
+                    pointer(out.qfit), pointer(out.qfit_expected), pointer(out.qfit_z), pointer(out.crowding),
+                    pointer(out.spread_model), pointer(out.spread_model_err))
     GC.@preserve keep out img iv begin
         rc = ccall(sym(:cp_fit_simultaneous), Cint,
                    (Ptr{Cvoid}, Ref{CpPsfDesc}, Ptr{Float32}, Ptr{Float32}, Int32, Int32, Ptr{Float64}, Ptr{Float64},
@@ -187,13 +196,13 @@ function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad
                    ctx.handle, desc, img, iv === nothing ? C_NULL : iv, ny, nx, y0, x0, f0, n, fit_rad, o, r)
         check(ctx, rc)
     end
-    nanv = fill(NaN, n)
     msgs = ["star $i: excluded before the solve (no usable pixels or non-finite initial parameters)"
             for i in 1:n if out.active[i] == 0]
     return (y = out.y, x = out.x, y_err = out.y_err, x_err = out.x_err, flux = out.flux, flux_err = out.flux_err,
             bkg = out.bkg, bkg_err = out.bkg_err, converged = BitVector(out.converged .!= 0),
-            valid = BitVector(out.valid .!= 0), chisq = out.chisq, qfit = nanv, qfit_expected = copy(nanv),
-            qfit_z = copy(nanv), crowding = copy(nanv), spread_model = copy(nanv), spread_model_err = copy(nanv),
+            valid = BitVector(out.valid .!= 0), chisq = out.chisq, qfit = out.qfit, qfit_expected = out.qfit_expected,
+            qfit_z = out.qfit_z, crowding = out.crowding, spread_model = out.spread_model,
+            spread_model_err = out.spread_model_err,
             n_iter = Int.(out.n_iter), n_passes = Int(r.n_passes), n_failed = Int(r.n_failed),
             failure_msgs = msgs, residual = out.residual)
 end

@@ -224,12 +224,14 @@ class _SimCall:
         o.precision = {"mixed": _abi.CP_PREC_MIXED, "fp64": _abi.CP_PREC_FP64}[precision]
         o.tile = int(tile)
         o.layout = _abi.CP_LAYOUT_ROWMAJOR
+        o.spread_model_fwhm = 0.0 if spread_model_fwhm is None else float(spread_model_fwhm)
         self.opts = o
         self.ny, self.nx, self.n = ny, nx, n
         self.fit_rad = float(fit_rad)
         self.show_trace = show_trace
         self.out = {k: np.zeros(n) for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err",
-                                             "chisq")}
+                                             "chisq", "qfit", "qfit_expected", "qfit_z", "crowding",
+                                             "spread_model", "spread_model_err")}
         self.conv = np.zeros(n, np.uint8)
         self.valid = np.zeros(n, np.uint8)
         self.active = np.zeros(n, np.uint8)
@@ -260,7 +262,6 @@ class _SimCall:
 
     def finish(self):
         n, s, tr = self.n, self.result, self.tr
-        nanv = np.full(n, np.nan)
         msgs = [f"star {i + 1}: excluded before the solve (no usable pixels or non-finite initial parameters)"
                 for i in range(n) if not self.active[i]]
         resid = None if self.res is None else self.res.reshape(self.ny, self.nx)
@@ -272,8 +273,9 @@ class _SimCall:
         return MultiPassPhotResult(
             y=out["y"], x=out["x"], y_err=out["y_err"], x_err=out["x_err"], flux=out["flux"],
             flux_err=out["flux_err"], bkg=out["bkg"], bkg_err=out["bkg_err"], converged=self.conv.astype(bool),
-            valid=self.valid.astype(bool), chisq=out["chisq"], qfit=nanv.copy(), qfit_expected=nanv.copy(),
-            qfit_z=nanv.copy(), crowding=nanv.copy(), spread_model=nanv.copy(), spread_model_err=nanv.copy(),
+            valid=self.valid.astype(bool), chisq=out["chisq"], qfit=out["qfit"], qfit_expected=out["qfit_expected"],
+            qfit_z=out["qfit_z"], crowding=out["crowding"], spread_model=out["spread_model"],
+            spread_model_err=out["spread_model_err"],
             n_iter=self.n_iter.astype(np.int64), n_passes=s.n_passes, n_failed=s.n_failed, failure_msgs=msgs,
             residual=resid,
             stats=dict(n_active=s.n_active, n_union=s.n_union, cost=s.cost_final, n_trials=s.n_trials_total,

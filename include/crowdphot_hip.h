@@ -105,6 +105,8 @@ typedef struct cp_sim_opts {
     int32_t precision;         /* CP_PREC_* */
     int32_t tile;              /* reserved, ignored: the render bins stars into 64 x 64 px tiles */
     int32_t layout;            /* CP_LAYOUT_*: image, inv_var and residual layout (default column-major) */
+    double spread_model_fwhm;  /* > 0: FWHM of the spread_model reference disk; otherwise the
+                                  Gaussian-equivalent FWHM of the PSF at the first catalogue star */
 } cp_sim_opts;
 
 /* Caller-allocated outputs, catalogue order (length n_stars unless noted).
@@ -126,6 +128,9 @@ typedef struct cp_sim_result {
     double *trace;             /* optional: 8 doubles per trial (iter, trial, cost, cost_cand, lambda,
                                   n_live, inner iterations, 0) */
     int32_t trace_cap, trace_len;
+    /* post-fit diagnostics (psf_photometry_diagnostics.jl:154-231), each may be NULL;
+       NaN where the reference leaves NaN */
+    double *qfit, *qfit_expected, *qfit_z, *crowding, *spread_model, *spread_model_err;
 } cp_sim_result;
 
 /* ---- per-star Levenberg-Marquardt -----------------------------------------
@@ -255,6 +260,11 @@ int cp_render_sources(const cp_psf_desc* psf, int32_t ny, int32_t nx, const doub
                       const double* x, const double* flux, int32_t n, double model_radius,
                       double* image);
 int cp_add_noise(uint64_t seed, double* image, int64_t npix, double gain, double read_noise);
+
+/* The spread_model reference disk (_exp_disk_kernel_bandlimited,
+ * psf_photometry_diagnostics.jl:78-101): K[i + j (2 half + 1)], sum 1.
+ * Exposed so the diagnostics set-up can be checked on its own. */
+int cp_spread_kernel(double fwhm, int32_t half, double* K);
 
 /* ---- serving: continuous batching of simultaneous fits ---------------------
  * No counterpart in the reference, which fits one image per call: for callers

@@ -934,6 +934,93 @@ int or_inner_solve(int32_t solver, const double* V, const int32_t* pixels, int32
     return lsqr(op, b, lam, itmax, atol, btol, x_out);
 }
 
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// post-fit diagnostics (src/photometry/psf_photometry_diagnostics.jl)
+// ---------------------------------------------------------------------------
+
+// Mixture-of-Gaussians exponential disk (psf_photometry_diagnostics.jl:14-15).
+const double EXPGAL_AMP[6] = {1.99485977e-4, 2.61612679e-3, 1.89726655e-2,
+                              1.00186544e-1, 3.68534484e-1, 5.09490694e-1};
+const double EXPGAL_VAR[6] = {1.20078965e-3, 8.84526493e-3, 3.91463084e-2,
+                              1.39976817e-1, 4.60962500e-1, 1.50159566e0};
+
+// 64-point Gauss-Legendre nodes / weights on [-1, 1] (Newton on P_64).
+void gauss_legendre64(double* x, double* wgt) {
+    const int n = 64;
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1, p1 = 0;
+            for (int k = 1; k <= n; ++k) {
+                const double p2 = p1;
+                p1 = p0;
+                p0 = ((2 * k - 1) * z * p1 - (k - 1) * p2) / k;
+            }
+            dp = n * (z * p0 - p1) / (z * z - 1);
+            const double dz = p0 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        x[i] = z;
+        wgt[i] = 2 / ((1 - z * z) * dp * dp);
+    }
+}
+
+// I(m; v) of _exp_disk_kernel_bandlimited (:78-101): the integral
+// int_{-1/2}^{1/2} exp(-a k^2) cos(2 pi k m) dk (the reference's closed form
+// through the complex error function), by Gauss-Legendre on 8 panels.
+double band_profile_gl(int m, double a) {
+    if (M_PI * M_PI * (double)m * m / a > 690) return 0.0;   // the reference's underflow cut
+    static double X[64], W[64];
+    static bool init = false;
+    if (!init) {
+        gauss_legendre64(X, W);
+        init = true;
+    }
+    double s = 0;
+    for (int panel = 0; panel < 8; ++panel) {
+        const double lo = -0.5 + panel / 8.0, h = 1.0 / 16;   // panel centre lo + h, half-width h
+        for (int i = 0; i < 64; ++i) {
+            const double k = lo + h + h * X[i];
+            s += W[i] * h * std::exp(-a * k * k) * std::cos(2 * M_PI * k * m);
+        }
+    }
+    return s;
+}
+
+void spread_kernel(double fwhm, int half, std::vector<double>& K) {
+    const int n = 2 * half + 1;
+    const double re = fwhm / 16 * 1.67834699;
+    K.assign((size_t)n * n, 0.0);
+    std::vector<double> Iv(n);
+    for (int c = 0; c < 6; ++c) {
+        const double a = 2 * M_PI * M_PI * re * re * EXPGAL_VAR[c];
+        for (int m = -half; m <= half; ++m) Iv[m + half] = band_profile_gl(m, a);
+        for (int jj = 0; jj < n; ++jj)
+            for (int ii = 0; ii < n; ++ii) K[ii + jj * n] += EXPGAL_AMP[c] * Iv[ii] * Iv[jj];
+    }
+    double s = 0;
+    for (double v : K) s += v;
+    for (double& v : K) v /= s;
+}
+
+}  // namespace
+
+extern "C" {
+
+// The spread_model reference disk as the oracle computes it (for tests).
+int or_spread_kernel(double fwhm, int32_t half, double* K) {
+    if (!(fwhm > 0) || half < 0 || half > 31 || !K) return fail("bad spread kernel arguments");
+    std::vector<double> k;
+    spread_kernel(fwhm, half, k);
+    std::copy(k.begin(), k.end(), K);
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // fit_all_stars_simultaneous (psf_photometry_simultaneous.jl:953-1431)
 // ---------------------------------------------------------------------------
@@ -1032,6 +1119,9 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
         for (int i = 0; i < n_stars; ++i) {
             out->bkg[i] = 0; out->bkg_err[i] = 0; out->converged[i] = 0; out->n_iter[i] = 0;
             if (out->chisq) out->chisq[i] = 0;
+            for (double* d : {out->qfit, out->qfit_expected, out->qfit_z, out->crowding, out->spread_model,
+                              out->spread_model_err})
+                if (d) d[i] = NAN;
         }
         if (out->residual) std::memset(out->residual, 0, sizeof(float) * npix);
         out->n_passes = 0; out->n_failed = n_stars; out->n_active = 0; out->n_union = 0;
@@ -1290,6 +1380,134 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
         }
     }
 
+    // -- post-fit diagnostics (:1344-1397; _star_diagnostics!,
+    // psf_photometry_diagnostics.jl:154-231; _spread_fwhm :113-123)
+    auto diagnostics = [&](const std::vector<double>& Y, const std::vector<double>& X,
+                           const std::vector<double>& Fl, const std::vector<uint8_t>& vld) {
+        double* outs[6] = {out->qfit, out->qfit_expected, out->qfit_z, out->crowding, out->spread_model,
+                           out->spread_model_err};
+        for (double* d : outs)
+            if (d)
+                for (int i = 0; i < n_stars; ++i) d[i] = NAN;
+        bool any = false;
+        for (double* d : outs) any = any || d;
+        if (!any) return;
+        // reference disk: explicit FWHM, else the effective-area FWHM of a
+        // unit-flux render of the model at catalogue star 1's fitted position
+        double sfw = o->spread_model_fwhm;
+        if (!(sfw > 0)) {
+            sfw = NAN;
+            double prm[8];
+            for (int k = 0; k < np_full; ++k) prm[k] = tmpl[k];
+            prm[0] = Y[0];
+            prm[1] = X[0];
+            prm[row_flux] = 1.0;
+            prm[row_bkg] = 0.0;
+            double ylo, yhi, xlo, xhi;
+            if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN || P.kind == CP_PSF_CIRCULAR_MOFFAT) {
+                const double fw = P.kind == CP_PSF_CIRCULAR_GAUSSIAN
+                                      ? P.fwhm : 2 * P.alpha * std::sqrt(std::exp2(1 / P.beta) - 1);
+                ylo = Y[0] - 5 * fw / 2; yhi = Y[0] + 5 * fw / 2;
+                xlo = X[0] - 5 * fw / 2; xhi = X[0] + 5 * fw / 2;
+            } else {
+                ylo = Y[0] - (P.oy - 1) / P.os_y; yhi = Y[0] + (P.gny - P.oy) / P.os_y;
+                xlo = X[0] - (P.ox - 1) / P.os_x; xhi = X[0] + (P.gnx - P.ox) / P.os_x;
+            }
+            if (std::isfinite(Y[0]) && std::isfinite(X[0])) {
+                double s1 = 0, s2 = 0;
+                for (long gx = (long)std::floor(xlo); gx <= (long)std::ceil(xhi); ++gx)
+                    for (long gy = (long)std::floor(ylo); gy <= (long)std::ceil(yhi); ++gy) {
+                        const double g = evaluate(P, prm, (double)gy, (double)gx);
+                        s1 += g;
+                        s2 += g * g;
+                    }
+                if (s2 > 0) sfw = std::sqrt(s1 * s1 / s2 * 2 * std::log(2.0) / M_PI);
+            }
+        }
+        const int kh = (int)std::ceil(fit_rad), kw = 2 * kh + 1;
+        std::vector<double> K;
+        const bool do_spread = std::isfinite(sfw) && sfw > 0;
+        if (do_spread) spread_kernel(sfw, kh, K);
+        double prm[8];
+        for (int j = 0; j < na; ++j) {
+            const int i = active[j];
+            if (!vld[i]) continue;
+            for (int k = 0; k < np_full; ++k) prm[k] = tmpl[k];
+            prm[0] = Y[i];
+            prm[1] = X[i];
+            prm[row_flux] = Fl[i];
+            int ylo = std::max((int)std::floor(Y[i] - fit_rad), 1), yhi = std::min((int)std::ceil(Y[i] + fit_rad), (int)ny);
+            int xlo = std::max((int)std::floor(X[i] - fit_rad), 1), xhi = std::min((int)std::ceil(X[i] + fit_rad), (int)nx);
+            if (ylo > yhi || xlo > xhi) continue;
+            const int by = yhi - ylo + 1, bx = xhi - xlo + 1;
+            std::vector<double> ms((size_t)by * bx), gs((size_t)by * bx, 0.0);
+            for (int ix = 0; ix < bx; ++ix)
+                for (int iy = 0; iy < by; ++iy) ms[iy + ix * by] = evaluate(P, prm, ylo + iy, xlo + ix);
+            if (do_spread)   // correlate!(.., :zero), correlation.jl:122-140
+                for (int ix = 0; ix < bx; ++ix)
+                    for (int iy = 0; iy < by; ++iy) {
+                        double g = 0;
+                        for (int b = -kh; b <= kh; ++b)
+                            for (int a = -kh; a <= kh; ++a) {
+                                const int jy = iy + a, jx = ix + b;
+                                if (jy < 0 || jy >= by || jx < 0 || jx >= bx) continue;
+                                g += ms[jy + jx * by] * K[(a + kh) + (b + kh) * kw];
+                            }
+                        gs[iy + ix * by] = g;
+                    }
+            const double flux = Fl[i], bkg = prm[row_bkg];
+            if (!(flux > 0)) continue;
+            const double inv_flux = 1 / flux;
+            double qfit = 0, nclean = 0, ndirty = 0, dcrowd = 0, Sp = 0, Sq = 0, Sp0 = 0, Sq0 = 0, SqG = 0;
+            double ssum = 0, s2sum = 0;
+            int ngood = 0;
+            for (int ix = 0; ix < bx; ++ix)
+                for (int iy = 0; iy < by; ++iy) {
+                    const size_t q = (size_t)(ylo + iy - 1) + (size_t)(xlo + ix - 1) * ny;
+                    const double wp = inv_var ? inv_var[q] : 1.0;
+                    if (!(std::isfinite(wp) && wp > 0)) continue;
+                    const double rq = data[q] - model_img[q], mv = ms[iy + ix * by];
+                    qfit += std::fabs(rq);
+                    const double Pp = (mv - bkg) * inv_flux, wP = wp * Pp, pi = rq + mv - bkg;
+                    nclean += wP * pi;
+                    ndirty += wP * (data[q] - bkg);
+                    dcrowd += wP * Pp;
+                    if (do_spread) {
+                        const double gi = (gs[iy + ix * by] - bkg) * inv_flux;
+                        Sp += wP * pi;
+                        Sq += wp * gi * pi;
+                        Sp0 += wP * Pp;
+                        Sq0 += wp * Pp * gi;
+                        SqG += wp * gi * gi;
+                    }
+                    if (inv_var) {
+                        const double sg = 1 / std::sqrt(wp);
+                        ssum += sg;
+                        s2sum += sg * sg;
+                        ++ngood;
+                    }
+                }
+            if (out->qfit) out->qfit[i] = qfit * inv_flux;
+            if (out->crowding && dcrowd > 0 && nclean > 0 && ndirty > 0)
+                out->crowding[i] = 2.5 * std::log10(ndirty / nclean);
+            if (do_spread && Sp > 0) {
+                if (out->spread_model) out->spread_model[i] = Sq / Sp - Sq0 / Sp0;
+                if (inv_var && out->spread_model_err) {
+                    const double v = Sp * Sp * SqG + Sq * Sq * Sp0 - 2 * Sq * Sp * Sq0;
+                    out->spread_model_err[i] = std::sqrt(std::max(0.0, v)) / (Sp * Sp);
+                }
+            }
+            if (inv_var) {
+                const double c = std::sqrt(2 / M_PI);
+                if (out->qfit_expected) out->qfit_expected[i] = c * ssum * inv_flux;
+                if (out->qfit_z && s2sum > 0 && ngood > p) {
+                    const double dof_f = std::sqrt(1 - (double)p / ngood);
+                    out->qfit_z[i] = (qfit - c * dof_f * ssum) / (std::sqrt((1 - 2 / M_PI) * s2sum) * dof_f);
+                }
+            }
+        }
+    };
+
     // -- final bookkeeping and errors (:1289-1329)
     for (int j = 0; j < na; ++j) {
         int i = active[j];
@@ -1373,6 +1591,7 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
         int den = good - p;
         if (den > 0) out->chisq[i] = num / den;
     }
+    diagnostics(py, px, pf, valid);
     out->n_passes = n_run;
     out->n_failed = n_failed;
     out->n_active = na;

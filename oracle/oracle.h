@@ -25,6 +25,7 @@ int or_nparams(int32_t kind);
 int or_evaluate_fg(const cp_psf_desc* psf, const double* prm, double py, double px,
                    double* f, double* G);
 double or_evaluate(const cp_psf_desc* psf, const double* prm, double py, double px);
+int or_spread_kernel(double fwhm, int32_t half, double* K);
 void or_bicubic(const double* data, int32_t ny, int32_t nx, double y, double x, double fill,
                 double* out3);
 int or_grid_corners_dw(const cp_psf_desc* psf, double Y, double X, int32_t* idx4, double* w4,

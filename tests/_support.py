@@ -48,6 +48,7 @@ def oracle():
         "or_nparams": (C.c_int, [C.c_int32]),
         "or_evaluate_fg": (C.c_int, [P(abi.PsfDesc), f64p, C.c_double, C.c_double, f64p, f64p]),
         "or_evaluate": (C.c_double, [P(abi.PsfDesc), f64p, C.c_double, C.c_double]),
+        "or_spread_kernel": (C.c_int, [C.c_double, C.c_int32, f64p]),
         "or_bicubic": (None, [f64p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, f64p]),
         "or_grid_corners_dw": (C.c_int, [P(abi.PsfDesc), C.c_double, C.c_double, i32p, f64p, f64p, f64p]),
         "or_curve_of_growth": (C.c_double, [P(abi.PsfDesc), C.c_double]),

@@ -8,6 +8,10 @@ from _support import f64, load_pkg, oracle, ptr
 abi = load_pkg()._abi
 
 
+FIELDS = ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq",
+          "qfit", "qfit_expected", "qfit_z", "crowding", "spread_model", "spread_model_err")
+
+
 class Result:
     pass
 
@@ -27,7 +31,7 @@ def fit(psf_model, image, ys, xs, fs, fit_rad, inv_var=None, trace=0, **kw):
     img = f64(image.ravel(order="F"))
     iv = None if inv_var is None else f64(np.asarray(inv_var).ravel(order="F"))
     r = Result()
-    for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq"):
+    for k in FIELDS:
         setattr(r, k, np.zeros(n))
     r.converged = np.zeros(n, np.uint8)
     r.valid = np.zeros(n, np.uint8)
@@ -36,7 +40,7 @@ def fit(psf_model, image, ys, xs, fs, fit_rad, inv_var=None, trace=0, **kw):
     r.residual = np.zeros(ny * nx, np.float32)
     r.trace = np.zeros(max(trace, 1) * 8)
     s = abi.SimResult()
-    for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq"):
+    for k in FIELDS:
         setattr(s, k, ptr(getattr(r, k), abi.f64p))
     s.converged = ptr(r.converged, abi.u8p)
     s.valid = ptr(r.valid, abi.u8p)
