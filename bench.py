@@ -30,6 +30,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "crowdphot.jl_amd")
 HBM_PEAK_GBS = 8000.0
+# every per-star output of fit_all_stars_simultaneous's result
+# (MultiPassPhotResult, src/photometry/psf_photometry_single.jl:99-122)
+PER_STAR = ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq", "qfit", "qfit_expected",
+            "qfit_z", "crowding", "spread_model", "spread_model_err")
 
 
 def log(*a):
@@ -136,13 +140,13 @@ def main():
         return o
 
     def make_result(trace_cap=0):
-        r = {k: np.zeros(n) for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq")}
+        r = {k: np.zeros(n) for k in PER_STAR}
         r["converged"] = np.zeros(n, np.uint8)
         r["valid"] = np.zeros(n, np.uint8)
         r["n_iter"] = np.zeros(n, np.int32)
         r["trace"] = np.zeros(max(trace_cap, 1) * 8)
         s = abi.SimResult()
-        for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq", "trace"):
+        for k in PER_STAR + ("trace",):
             setattr(s, k, r[k].ctypes.data_as(abi.f64p))
         s.converged = r["converged"].ctypes.data_as(abi.u8p)
         s.valid = r["valid"].ctypes.data_as(abi.u8p)

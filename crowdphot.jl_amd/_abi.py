@@ -266,6 +266,8 @@ SIGNATURES = {
                                     C.c_int32, C.c_double, f64p]),
     "cp_add_noise": (C.c_int, [C.c_uint64, f64p, C.c_int64, C.c_double, C.c_double]),
     "cp_spread_kernel": (C.c_int, [C.c_double, C.c_int32, f64p]),
+    "cp_curve_of_growth": (C.c_int, [C.POINTER(PsfDesc), C.c_int32, f64p, f64p]),
+    "cp_pixel_overlap": (C.c_double, [C.c_double, C.c_double, C.c_double]),
     "cp_request_bytes": (C.c_uint64, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                       C.c_int32]),
     "cp_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_uint64, C.c_int32]),

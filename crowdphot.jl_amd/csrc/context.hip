@@ -1,4 +1,5 @@
 // context.hip -- context lifetime, defaults and the model-evaluation entry point.
+#include <algorithm>
 #include <cstring>
 
 #include "sim_kernels.h"
@@ -48,6 +49,54 @@ __global__ void k_evaluate(ModelConst<double> M, NodeGrid G, double Y, double X,
 
 thread_local cp_ctx* tl_ctx = nullptr;
 
+void staged_d2h(cp_ctx* c, hipStream_t st, const std::vector<D2HSeg>& segs) {
+    // chunks of at most STAGE_CHUNK bytes, each one contiguous run of device
+    // memory feeding one or more host destinations
+    struct Piece {
+        char* dst;
+        size_t off, bytes;   // offset inside the chunk
+    };
+    struct Chunk {
+        const char* src;
+        size_t bytes;
+        std::vector<Piece> pieces;
+    };
+    std::vector<Chunk> ch;
+    for (const D2HSeg& sg : segs) {
+        const char* src = static_cast<const char*>(sg.src);
+        char* dst = static_cast<char*>(sg.dst);
+        size_t left = sg.bytes;
+        while (left > 0) {
+            if (ch.empty() || ch.back().src + ch.back().bytes != src || ch.back().bytes == STAGE_CHUNK)
+                ch.push_back(Chunk{src, 0, {}});
+            Chunk& k = ch.back();
+            const size_t take = std::min(left, STAGE_CHUNK - k.bytes);
+            k.pieces.push_back(Piece{dst, k.bytes, take});
+            k.bytes += take;
+            src += take;
+            dst += take;
+            left -= take;
+        }
+    }
+    if (ch.empty()) return;
+    if (!c->stage) {
+        CP_HIP(hipHostMalloc(&c->stage, 2 * STAGE_CHUNK, hipHostMallocDefault));
+        for (auto& e : c->stage_ev) CP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    char* buf[2] = {static_cast<char*>(c->stage), static_cast<char*>(c->stage) + STAGE_CHUNK};
+    auto issue = [&](size_t k) {
+        CP_HIP(hipMemcpyAsync(buf[k & 1], ch[k].src, ch[k].bytes, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipEventRecord(c->stage_ev[k & 1], st));
+    };
+    issue(0);
+    for (size_t k = 0; k < ch.size(); ++k) {
+        // buffer (k + 1) & 1 last held chunk k - 1, already copied out
+        if (k + 1 < ch.size()) issue(k + 1);
+        CP_HIP(hipEventSynchronize(c->stage_ev[k & 1]));
+        for (const Piece& pc : ch[k].pieces) par_memcpy(pc.dst, buf[k & 1] + pc.off, pc.bytes);
+    }
+}
+
 }  // namespace cp
 
 using namespace cp;
@@ -85,6 +134,9 @@ void cp_destroy(cp_ctx* c) {
         (void)hipStreamDestroy(c->stream);
     }
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->stage) (void)hipHostFree(c->stage);
+    for (auto e : c->stage_ev)
+        if (e) (void)hipEventDestroy(e);
     cp::cache_trim(c, UINT64_MAX);
     for (auto& p : c->prof_pending) {
         (void)hipEventDestroy(p.a);

@@ -50,6 +50,9 @@ struct cp_ctx {
     // pinned host staging for small scalar read-backs
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // pinned double buffer (2 x STAGE_CHUNK) for result read-back
+    void* stage = nullptr;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int n_cu = 0;
     // per-kernel HIP-event timing (cp_profile)
     bool prof = false;
@@ -226,5 +229,21 @@ inline void* pinned(cp_ctx* c, size_t bytes) {
 }
 
 inline int popcount32(uint32_t v) { return __builtin_popcount(v); }
+
+// Host memcpy split over OpenMP threads (host_copy.cpp).
+void par_memcpy(void* dst, const void* src, size_t bytes);
+
+// One device-to-host piece of a result read-back.
+struct D2HSeg {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+constexpr size_t STAGE_CHUNK = size_t(4) << 20;
+// Copy every segment to its (pageable) host destination through the
+// context's pinned double buffer: the DMA of one chunk overlaps the
+// multi-threaded host copy of the previous one.  Segments adjacent on the
+// device share a chunk.  Returns when every destination is written.
+void staged_d2h(cp_ctx* c, hipStream_t st, const std::vector<D2HSeg>& segs);
 
 }  // namespace cp

@@ -53,29 +53,71 @@ static const double EXPGAL_VAR[6] = {1.20078965e-3, 8.84526493e-3, 3.91463084e-2
                                      1.39976817e-1, 4.60962500e-1, 1.50159566e0};
 
 // I(m; v) = int_{-1/2}^{1/2} exp(-a k^2) cos(2 pi k m) dk, a = 2 pi^2 re^2 v:
-// the band-limited 1-D profile of one mixture component.  The reference uses
-// its closed form through the complex error function; here the (analytic,
-// smooth) integrand is integrated by composite Simpson's rule on [0, 1/2]
-// with 4096 panels (error far below float64 rounding of the kernel).  The
-// reference's underflow cut (pi^2 m^2 / a > 690 -> 0) is kept.
-static double band_profile(int m, double a) {
-    if (M_PI * M_PI * (double)m * m / a > 690) return 0.0;
-    const int n = 4096;
-    const double h = 0.5 / n;
-    auto f = [&](double k) { return std::exp(-a * k * k) * std::cos(2 * M_PI * k * m); };
-    double s = f(0.0) + f(0.5);
-    for (int i = 1; i < n; ++i) s += (i & 1 ? 4.0 : 2.0) * f(i * h);
-    return 2 * s * h / 3;
+// the band-limited 1-D profile of one mixture component, for m = 0..half at
+// once.  The reference uses its closed form through the complex error
+// function; here the (analytic, smooth) even integrand is integrated over
+// [0, 1/2] by 16-point Gauss-Legendre on 32 equal panels (exact to float64
+// rounding for |m| <= 31: each panel spans under 1/3 of a period of the
+// fastest cosine), with cos(2 pi k m) from cos(2 pi k) by the Chebyshev
+// recurrence.  The reference's underflow cut (pi^2 m^2 / a > 690 -> 0) is
+// kept.
+struct GaussLegendre16 {
+    double x[16], w[16];
+    GaussLegendre16() {   // nodes / weights of P_16 on [-1, 1] by Newton's method
+        for (int i = 0; i < 16; ++i) {
+            double z = std::cos(M_PI * (i + 0.75) / 16.5), dp = 1;
+            for (int it = 0; it < 100; ++it) {
+                double p0 = 1, p1 = 0;
+                for (int k = 1; k <= 16; ++k) {
+                    const double p2 = p1;
+                    p1 = p0;
+                    p0 = ((2 * k - 1) * z * p1 - (k - 1) * p2) / k;
+                }
+                dp = 16 * (z * p0 - p1) / (z * z - 1);
+                const double dz = p0 / dp;
+                z -= dz;
+                if (std::fabs(dz) < 1e-16) break;
+            }
+            x[i] = z;
+            w[i] = 2 / ((1 - z * z) * dp * dp);
+        }
+    }
+};
+
+static void band_profiles(double a, int half, double* Iv) {
+    static const GaussLegendre16 gl;   // initialised once, thread-safe
+    const double* X = gl.x;
+    const double* Wt = gl.w;
+    for (int m = 0; m <= half; ++m) Iv[m] = 0;
+    const int panels = 32;
+    const double hw = 0.25 / panels;   // panel half-width on [0, 1/2]
+    for (int pnl = 0; pnl < panels; ++pnl) {
+        const double mid = (2 * pnl + 1) * hw;
+        for (int i = 0; i < 16; ++i) {
+            const double k = mid + hw * X[i];
+            const double e = Wt[i] * hw * std::exp(-a * k * k), c1 = std::cos(2 * M_PI * k);
+            double cm1 = 1, cm = c1;   // cos(2 pi k (m - 1)), cos(2 pi k m)
+            Iv[0] += e;
+            for (int m = 1; m <= half; ++m) {
+                Iv[m] += e * cm;
+                const double nx = 2 * c1 * cm - cm1;
+                cm1 = cm;
+                cm = nx;
+            }
+        }
+    }
+    for (int m = 0; m <= half; ++m) Iv[m] = M_PI * M_PI * (double)m * m / a > 690 ? 0.0 : 2 * Iv[m];
 }
 
 void spread_kernel_host(double fwhm, int half, double* K) {
     const int n = 2 * half + 1;
     const double re = fwhm / 16 * 1.67834699;
     for (int i = 0; i < n * n; ++i) K[i] = 0;
-    double Iv[64];
+    double Ih[32], Iv[64];
     for (int c = 0; c < 6; ++c) {
         const double a = 2 * M_PI * M_PI * re * re * EXPGAL_VAR[c];
-        for (int m = -half; m <= half; ++m) Iv[m + half] = band_profile(m, a);
+        band_profiles(a, half, Ih);
+        for (int m = -half; m <= half; ++m) Iv[m + half] = Ih[m < 0 ? -m : m];
         for (int jj = 0; jj < n; ++jj)
             for (int ii = 0; ii < n; ++ii) K[ii + jj * n] += EXPGAL_AMP[c] * Iv[ii] * Iv[jj];
     }

@@ -1315,10 +1315,77 @@ static __global__ void k_errors(int na, int R, int S, const int* __restrict__ ay
 // spread_model reference disk K of half-width kh (K = null: spread_model
 // stays NaN).  resid is the global residual (data minus every star's model),
 // data the working image, w the weights (inverse variance, 0 = masked; all
-// ones without inv_var).  out[a * 7 + ...] = chisq, qfit, qfit_expected,
-// qfit_z, crowding, spread_model, spread_model_err (NaN where the reference
-// leaves NaN, chisq 0).
+// ones without inv_var).  Valid stars write the result columns OUT_CHISQ ..
+// OUT_CHISQ + 6 (chisq, qfit, qfit_expected, qfit_z, crowding, spread_model,
+// spread_model_err; NaN where the reference leaves NaN) of catalogue star
+// act[a]; invalid stars keep what k_results wrote (chisq 0, the rest NaN).
 constexpr int DIAG_MAXB = 1024;   // box pixels: (2 ceil(fit_rad) + 2)^2 for fit_rad <= 15
+
+// Result columns (float64, catalogue order, n each), then n_iter (int32),
+// converged (u8) and valid (u8): the per-star fields of MultiPassPhotResult
+// (src/photometry/psf_photometry_single.jl:99-122).
+enum OutCol {
+    OUT_Y, OUT_X, OUT_FLUX, OUT_BKG, OUT_Y_ERR, OUT_X_ERR, OUT_FLUX_ERR, OUT_BKG_ERR, OUT_CHISQ,
+    OUT_QFIT, OUT_QFIT_EXP, OUT_QFIT_Z, OUT_CROWDING, OUT_SPREAD, OUT_SPREAD_ERR, OUT_F64
+};
+
+// slot[act[j]] = j: the active index of each catalogue star (slot is -1
+// beforehand for every star).
+static __global__ void k_slot_of(const int* __restrict__ act, int na, int* __restrict__ slot) {
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j < na) slot[act[j]] = j;
+}
+
+// Per-catalogue-star results (:1289-1300, :1331-1337, :1414-1430): an active
+// star (slot >= 0) takes its fitted free parameters and, with err_on, their
+// errors; every other value is the catalogue's (errors NaN for free, 0 for
+// fixed parameters); fixed parameters take their fixed value; valid = active
+// and finite y, x and positive finite flux (the final validity gate); n_iter
+// = n_run for a star still live at the end, else its freeze iteration.
+// chisq starts at 0 and the diagnostics at NaN (k_diag fills valid stars).
+static __global__ void k_results(int n, const int* __restrict__ slot, const double* __restrict__ y0,
+                                 const double* __restrict__ x0, const double* __restrict__ f0,
+                                 const double* __restrict__ theta, const double* __restrict__ errs, int err_on,
+                                 const uint8_t* __restrict__ live, const uint8_t* __restrict__ conv,
+                                 const int* __restrict__ frozen_at, int n_run, FreeMap fm, double fixed_bkg,
+                                 double* __restrict__ cols, int* __restrict__ n_iter, uint8_t* __restrict__ converged,
+                                 uint8_t* __restrict__ valid) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int a = slot[i];
+    double Y = y0[i], X = x0[i], F = f0[i];
+    double ey = fm.y_fixed ? 0.0 : NAN, ex = fm.x_fixed ? 0.0 : NAN, ef = fm.f_fixed ? 0.0 : NAN;
+    int it = 0;
+    uint8_t cv = 0;
+    if (a >= 0) {
+        for (int k = 0; k < fm.p; ++k) {
+            const int c = gc_of(fm, k);
+            const double v = theta[(size_t)a * fm.p + k], e = errs[(size_t)a * fm.p + k];
+            if (c == 1) { Y = v; if (err_on) ey = e; }
+            if (c == 2) { X = v; if (err_on) ex = e; }
+            if (c == 3) { F = v; if (err_on) ef = e; }
+        }
+        cv = conv[a];
+        it = live[a] ? n_run : frozen_at[a];
+    }
+    if (fm.y_fixed) Y = fm.fixed_y;
+    if (fm.x_fixed) X = fm.fixed_x;
+    if (fm.f_fixed) F = fm.fixed_f;
+    const size_t N = (size_t)n;
+    cols[OUT_Y * N + i] = Y;
+    cols[OUT_X * N + i] = X;
+    cols[OUT_FLUX * N + i] = F;
+    cols[OUT_BKG * N + i] = fixed_bkg;
+    cols[OUT_Y_ERR * N + i] = ey;
+    cols[OUT_X_ERR * N + i] = ex;
+    cols[OUT_FLUX_ERR * N + i] = ef;
+    cols[OUT_BKG_ERR * N + i] = 0.0;
+    cols[OUT_CHISQ * N + i] = 0.0;
+    for (int k = OUT_QFIT; k < OUT_F64; ++k) cols[k * N + i] = NAN;
+    n_iter[i] = it;
+    converged[i] = cv;
+    valid[i] = a >= 0 && isfinite(Y) && isfinite(X) && isfinite(F) && F > 0;
+}
 
 template <typename T, int KIND>
 static __global__ __launch_bounds__(64) void k_diag(int na, const double* __restrict__ theta, FreeMap fm,
@@ -1326,11 +1393,13 @@ static __global__ __launch_bounds__(64) void k_diag(int na, const double* __rest
                                                     double moffat_amp64, NodeGrid G, const T* __restrict__ resid,
                                                     const T* __restrict__ data, const T* __restrict__ w,
                                                     int has_iv, int ny, int nx, const double* __restrict__ K,
-                                                    int kh, double* __restrict__ out) {
+                                                    int kh, const int* __restrict__ act, int n,
+                                                    double* __restrict__ cols) {
     __shared__ T sm[DIAG_MAXB];
     const int a = blockIdx.x, lane = threadIdx.x;
     if (a >= na) return;
-    double* o = out + (size_t)a * 7;
+    const size_t N = (size_t)n;
+    double* o = cols + OUT_CHISQ * N + act[a];   // o[k * N]: column OUT_CHISQ + k
     double Y, X, F;
     star_yxf(fm, theta + (size_t)a * fm.p, Y, X, F);
     const bool valid = isfinite(Y) && isfinite(X) && isfinite(F) && F > 0;
@@ -1342,13 +1411,7 @@ static __global__ __launch_bounds__(64) void k_diag(int na, const double* __rest
         xhi = min((int)ceil(X + fit_rad), nx);
     }
     const int by = yhi - ylo + 1, bx = xhi - xlo + 1;
-    if (!valid || by <= 0 || bx <= 0) {
-        if (lane == 0) {
-            o[0] = 0;
-            for (int k = 1; k < 7; ++k) o[k] = NAN;
-        }
-        return;
-    }
+    if (!valid || by <= 0 || bx <= 0) return;
     const int nb = by * bx;
     StarRec<T> r{};
     r.flux = (T)F;
@@ -1419,22 +1482,22 @@ static __global__ __launch_bounds__(64) void k_diag(int na, const double* __rest
     if (lane != 0) return;
     const int p = fm.p;
     o[0] = ngood - p > 0 ? chi / (ngood - p) : 0.0;
-    o[1] = qfit * inv_flux;
-    o[4] = (dcrowd > 0 && nclean > 0 && ndirty > 0) ? 2.5 * log10(ndirty / nclean) : NAN;
-    o[5] = (K && Sp > 0) ? Sq / Sp - Sq0 / Sp0 : NAN;
+    o[N] = qfit * inv_flux;
+    o[4 * N] = (dcrowd > 0 && nclean > 0 && ndirty > 0) ? 2.5 * log10(ndirty / nclean) : NAN;
+    o[5 * N] = (K && Sp > 0) ? Sq / Sp - Sq0 / Sp0 : NAN;
     if (K && Sp > 0 && has_iv) {
         const double v = Sp * Sp * SqG + Sq * Sq * Sp0 - 2 * Sq * Sp * Sq0;
-        o[6] = sqrt(fmax(0.0, v)) / (Sp * Sp);
+        o[6 * N] = sqrt(fmax(0.0, v)) / (Sp * Sp);
     } else {
-        o[6] = NAN;
+        o[6 * N] = NAN;
     }
     const double c2pi = sqrt(2.0 / M_PI);
-    o[2] = has_iv ? c2pi * ssum * inv_flux : NAN;
+    o[2 * N] = has_iv ? c2pi * ssum * inv_flux : NAN;
     if (has_iv && s2sum > 0 && ngood > p) {
         const double dof = sqrt(1.0 - p / ngood);
-        o[3] = (qfit - c2pi * dof * ssum) / (sqrt((1 - 2 / M_PI) * s2sum) * dof);
+        o[3 * N] = (qfit - c2pi * dof * ssum) / (sqrt((1 - 2 / M_PI) * s2sum) * dof);
     } else {
-        o[3] = NAN;
+        o[3 * N] = NAN;
     }
 }
 

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <vector>
 
+#include "aperture_host.h"
 #include "diag_host.h"
 #include "sim_kernels.h"
 #include "psf_host.h"
@@ -187,26 +188,33 @@ static double quantile7(std::vector<double> v, double p) {
     return v[lo] + (h - lo) * (v[lo + 1] - v[lo]);
 }
 
-// _model_radii :auto for the analytic-CoG models (:209-243, curve_of_growth.jl:180-211).
-static std::vector<int> auto_radii(const HostPsf& P, double nsigma, int R_fit, int R_cap,
+// _model_radii :auto (:209-243): the curve of growth is closed-form for the
+// analytic models (curve_of_growth.jl:180-211) and pixel-integrated over exact
+// circular apertures for ImagePSF / GriddedPSFModel (curve_of_growth.jl:143-176).
+static std::vector<int> auto_radii(const HostPsf& P, const cp_psf_desc* psfd, double nsigma, int R_fit, int R_cap,
                                    const std::vector<double>& wsample, const std::vector<double>& flux) {
-    CP_ARG(P.kind != CP_PSF_GRIDDED_IMAGE,
-           "model_rad=:auto for ImagePSF/GriddedPSFModel needs the aperture curve of growth, not built; "
-           "pass a scalar model_rad");
     std::vector<double> ws;
     for (double v : wsample)
         if (v > 0 && std::isfinite(v)) ws.push_back(v);
     CP_ARG(!ws.empty(), "no positive finite weights for the :auto noise estimate");
     double wv = quantile7(ws, 0.84);
     double sigma_bg = wv > 0 ? std::sqrt(1 / wv) : 1.0;
-    auto cog = [&](double r) {
-        if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN) return 1 - std::exp(CP_GAUSS_PRE * r * r / (P.fwhm * P.fwhm));
-        return 1 - std::pow(1 + r * r / (P.alpha * P.alpha), 1 - P.beta);
-    };
+    std::vector<double> cg(R_cap);
+    if (P.kind == CP_PSF_GRIDDED_IMAGE) {
+        std::vector<double> radii(R_cap);
+        for (int r = 1; r <= R_cap; ++r) radii[r - 1] = r;
+        CP_ARG(model_cog_host(psfd, R_cap, radii.data(), cg.data()),
+               "model_rad=:auto: the curve of growth of this PSF model could not be computed");
+    } else {
+        for (int r = 1; r <= R_cap; ++r)
+            cg[r - 1] = P.kind == CP_PSF_CIRCULAR_GAUSSIAN
+                            ? 1 - std::exp(CP_GAUSS_PRE * (double)r * r / (P.fwhm * P.fwhm))
+                            : 1 - std::pow(1 + (double)r * r / (P.alpha * P.alpha), 1 - P.beta);
+    }
     std::vector<double> sb(R_cap);
-    double last = cog(R_cap), prev = 0;
+    double last = cg[R_cap - 1], prev = 0;
     for (int r = 1; r <= R_cap; ++r) {
-        double ee = cog(r) / last;
+        double ee = cg[r - 1] / last;
         sb[r - 1] = std::max((ee - prev) / (M_PI * (double)(r * r - (r - 1) * (r - 1))), 0.0);
         prev = ee;
     }
@@ -454,7 +462,7 @@ struct Sim {
             CP_HIP(hipStreamSynchronize(st));
             std::vector<double> wsd(hw.begin(), hw.end()), fl(na);
             for (int j = 0; j < na; ++j) fl[j] = hf0[hact[j]];
-            std::vector<int> rr = auto_radii(P, o.model_rad_nsigma, R, R_cap, wsd, fl);
+            std::vector<int> rr = auto_radii(P, psfd_, o.model_rad_nsigma, R, R_cap, wsd, fl);
             mR_src.alloc(ctx, (size_t)na * 4);
             CP_HIP(hipMemcpyAsync(mR_src.get(), rr.data(), (size_t)na * 4, hipMemcpyHostToDevice, st));
             CP_HIP(hipStreamSynchronize(st));
@@ -737,92 +745,85 @@ struct Sim {
                                   crs.as<CornerRec<T>>(),
                                   Lmod, g, ntiles, (const long long*)tscan.as<long long>(), umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
                                   data.as<T>(), wres_cur, resid, part.as<double>(), Sc);
-        std::vector<double> hth((size_t)na * p), herr((size_t)na * p);
-        std::vector<uint8_t> hlive(na), hconv(na);
-        std::vector<int> hfz(na);
-        CP_HIP(hipMemcpyAsync(hth.data(), theta.get(), nb8, hipMemcpyDeviceToHost, st));
-        CP_HIP(hipMemcpyAsync(herr.data(), errs.get(), nb8, hipMemcpyDeviceToHost, st));
-        CP_HIP(hipMemcpyAsync(hlive.data(), live.get(), na, hipMemcpyDeviceToHost, st));
-        CP_HIP(hipMemcpyAsync(hconv.data(), conv.get(), na, hipMemcpyDeviceToHost, st));
-        CP_HIP(hipMemcpyAsync(hfz.data(), frozen_at.get(), (size_t)na * 4, hipMemcpyDeviceToHost, st));
-        if (out->residual) {
-            if (o.layout == CP_LAYOUT_ROWMAJOR) {
-                // column-major ny x nx is row-major nx x ny: transpose to ny x nx
-                DevBuf rf(ctx, Pn * 4);
-                launch(k_transpose<T, float>, dim3(cdiv(ny, 32), cdiv(nx, 32)), dim3(32, 8), 0, st,
-                       (const T*)resid, nx, ny, rf.as<float>());
-                CP_HIP(hipMemcpyAsync(out->residual, rf.get(), Pn * 4, hipMemcpyDeviceToHost, st));
-                CP_HIP(hipStreamSynchronize(st));
-            } else if (sizeof(T) == 4) {
-                CP_HIP(hipMemcpyAsync(out->residual, resid, Pn * 4, hipMemcpyDeviceToHost, st));
-            } else {
-                DevBuf rf(ctx, Pn * 4);
-                launch(k_to_float<T>, dim3(std::min<int64_t>(cdiv(Pn, BLOCK), n_cu * 16)), dim3(BLOCK), 0, st,
-                       (const T*)resid, rf.as<float>(), Pn);
-                CP_HIP(hipMemcpyAsync(out->residual, rf.get(), Pn * 4, hipMemcpyDeviceToHost, st));
-                CP_HIP(hipStreamSynchronize(st));
-            }
-        }
-        CP_HIP(hipStreamSynchronize(st));
-
-        // --- assemble (:1289-1300, :1331-1337, :1414-1430)
-        write_initial(valid);
-        for (int j = 0; j < na; ++j) {
-            const int i = hact[j];
-            double Y = hy0[i], X = hx0[i], F = hf0[i];
-            for (int k = 0; k < p; ++k) {
-                double v = hth[(size_t)j * p + k], e = herr[(size_t)j * p + k];
-                if (gcs[k] == 1) { Y = v; if (cov_mode != CP_COV_NONE) out->y_err[i] = e; }
-                if (gcs[k] == 2) { X = v; if (cov_mode != CP_COV_NONE) out->x_err[i] = e; }
-                if (gcs[k] == 3) { F = v; if (cov_mode != CP_COV_NONE) out->flux_err[i] = e; }
-            }
-            out->y[i] = Y;
-            out->x[i] = X;
-            out->flux[i] = F;
-            out->converged[i] = hconv[j];
-            out->n_iter[i] = hlive[j] ? n_run : hfz[j];
-        }
-        for (int i = 0; i < n; ++i) {
-            if (fm.y_fixed) out->y[i] = o.fixed_y;
-            if (fm.x_fixed) out->x[i] = o.fixed_x;
-            if (fm.f_fixed) out->flux[i] = o.fixed_flux;
-            if (out->valid[i] && !(std::isfinite(out->y[i]) && std::isfinite(out->x[i]) &&
-                                   std::isfinite(out->flux[i]) && out->flux[i] > 0))
-                out->valid[i] = 0;
-        }
+        // --- per-catalogue-star results, assembled on the device (:1289-1300,
+        // :1331-1337, :1414-1430)
+        const size_t N = (size_t)n;
+        DevBuf slot(ctx, N * 4);
+        CP_HIP(hipMemsetAsync(slot.get(), 0xff, N * 4, st));   // -1: not active
+        launch(k_slot_of, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, (const int*)act.as<int>(), na, slot.as<int>());
+        DevBuf res(ctx, N * (OUT_F64 * 8 + 4 + 1 + 1));
+        double* cols = res.as<double>();
+        int* d_niter = reinterpret_cast<int*>(cols + OUT_F64 * N);
+        uint8_t* d_conv = reinterpret_cast<uint8_t*>(d_niter + N);
+        uint8_t* d_valid = d_conv + N;
+        launch(k_results, dim3(cdiv(n, BLOCK)), dim3(BLOCK), 0, st, n, (const int*)slot.as<int>(), d_y0, d_x0, d_f0,
+               (const double*)theta.as<double>(), (const double*)errs.as<double>(), cov_mode != CP_COV_NONE ? 1 : 0,
+               (const uint8_t*)live.as<uint8_t>(), (const uint8_t*)conv.as<uint8_t>(),
+               (const int*)frozen_at.as<int>(), n_run, fm, o.fixed_bkg, cols, d_niter, d_conv, d_valid);
 
         // --- per-star diagnostics over the global residual (:1344-1397,
         // psf_photometry_diagnostics.jl:154-231); the spread_model disk is
-        // sized from catalogue star 1's fitted position unless given
-        double* diag_out[7] = {out->chisq, out->qfit, out->qfit_expected, out->qfit_z, out->crowding,
-                               out->spread_model, out->spread_model_err};
+        // sized from catalogue star 1's reported position unless given
+        double* host_cols[OUT_F64] = {out->y,        out->x,         out->flux,   out->bkg,      out->y_err,
+                                      out->x_err,    out->flux_err,  out->bkg_err, out->chisq,   out->qfit,
+                                      out->qfit_expected, out->qfit_z, out->crowding, out->spread_model,
+                                      out->spread_model_err};
         bool any_diag = false;
-        for (double* d : diag_out) any_diag = any_diag || d;
+        for (int k = OUT_CHISQ; k < OUT_F64; ++k) any_diag = any_diag || host_cols[k];
+        DevBuf dK;
         if (any_diag) {
-            const double sfw = o.spread_model_fwhm > 0 ? o.spread_model_fwhm
-                                                       : spread_fwhm_host(psfd_, out->y[0], out->x[0]);
+            double sfw = NAN;
+            if (out->spread_model || out->spread_model_err) {
+                if (o.spread_model_fwhm > 0) {
+                    sfw = o.spread_model_fwhm;
+                } else {
+                    double yx[2];
+                    CP_HIP(hipMemcpyAsync(&yx[0], cols + OUT_Y * N, 8, hipMemcpyDeviceToHost, st));
+                    CP_HIP(hipMemcpyAsync(&yx[1], cols + OUT_X * N, 8, hipMemcpyDeviceToHost, st));
+                    CP_HIP(hipStreamSynchronize(st));
+                    sfw = spread_fwhm_host(psfd_, yx[0], yx[1]);
+                }
+            }
             const int kh = (int)std::ceil(fit_rad);
-            DevBuf dK, dd(ctx, (size_t)na * 7 * 8);
+            std::vector<double> hK;
             if (std::isfinite(sfw) && sfw > 0) {
-                std::vector<double> hK((size_t)(2 * kh + 1) * (2 * kh + 1));
+                hK.resize((size_t)(2 * kh + 1) * (2 * kh + 1));
                 spread_kernel_host(sfw, kh, hK.data());
                 dK.alloc(ctx, hK.size() * 8);
                 CP_HIP(hipMemcpyAsync(dK.get(), hK.data(), hK.size() * 8, hipMemcpyHostToDevice, st));
-                CP_HIP(hipStreamSynchronize(st));   // hK is pageable and local
             }
             launch(k_diag<T, KIND>, dim3(na), dim3(64), 0, st, na, (const double*)theta.as<double>(), fm, fit_rad, M,
                    inv_norm_g64, moffat_amp64, DP.G, (const T*)resid, (const T*)data.as<T>(), (const T*)w.as<T>(),
-                   d_iv ? 1 : 0, ny, nx, (const double*)(dK.get() ? dK.as<double>() : nullptr), kh, dd.as<double>());
-            std::vector<double> hd((size_t)na * 7);
-            CP_HIP(hipMemcpyAsync(hd.data(), dd.get(), hd.size() * 8, hipMemcpyDeviceToHost, st));
-            CP_HIP(hipStreamSynchronize(st));
-            for (int j = 0; j < na; ++j) {
-                const int i = hact[j];
-                if (!out->valid[i]) continue;
-                for (int k = 0; k < 7; ++k)
-                    if (diag_out[k]) diag_out[k][i] = hd[(size_t)j * 7 + k];
-            }
+                   d_iv ? 1 : 0, ny, nx, (const double*)(dK.get() ? dK.as<double>() : nullptr), kh,
+                   (const int*)act.as<int>(), n, cols);
+            if (!hK.empty()) CP_HIP(hipStreamSynchronize(st));   // hK is pageable and local
         }
+
+        // --- read-back: every requested column and the residual image
+        std::vector<D2HSeg> segs;
+        for (int k = 0; k < OUT_F64; ++k)
+            if (host_cols[k]) segs.push_back({host_cols[k], cols + k * N, N * 8});
+        if (out->n_iter) segs.push_back({out->n_iter, d_niter, N * 4});
+        if (out->converged) segs.push_back({out->converged, d_conv, N});
+        if (out->valid) segs.push_back({out->valid, d_valid, N});
+        DevBuf rf;
+        if (out->residual) {
+            const void* rsrc = resid;
+            if (o.layout == CP_LAYOUT_ROWMAJOR) {
+                // column-major ny x nx is row-major nx x ny: transpose to ny x nx
+                rf.alloc(ctx, Pn * 4);
+                launch(k_transpose<T, float>, dim3(cdiv(ny, 32), cdiv(nx, 32)), dim3(32, 8), 0, st,
+                       (const T*)resid, nx, ny, rf.as<float>());
+                rsrc = rf.get();
+            } else if (sizeof(T) != 4) {
+                rf.alloc(ctx, Pn * 4);
+                launch(k_to_float<T>, dim3(std::min<int64_t>(cdiv(Pn, BLOCK), n_cu * 16)), dim3(BLOCK), 0, st,
+                       (const T*)resid, rf.as<float>(), Pn);
+                rsrc = rf.get();
+            }
+            segs.push_back({out->residual, rsrc, (size_t)Pn * 4});
+        }
+        staged_d2h(ctx, st, segs);
         out->n_passes = n_run;
         out->n_active = na;
         out->n_union = U;

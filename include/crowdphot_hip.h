@@ -266,6 +266,17 @@ int cp_add_noise(uint64_t seed, double* image, int64_t npix, double gain, double
  * Exposed so the diagnostics set-up can be checked on its own. */
 int cp_spread_kernel(double fwhm, int32_t half, double* K);
 
+/* curve_of_growth(model, radii) (src/curve_of_growth.jl:143-176) of the
+ * unit-flux model centred at (0, 0) with exact pixel-circle overlap weights
+ * (src/apertures.jl:52-166): what model_rad = :auto uses for ImagePSF and
+ * GriddedPSFModel (_model_radii, psf_photometry_simultaneous.jl:209-243).
+ * radii strictly positive and increasing.  Exposed for checks. */
+int cp_curve_of_growth(const cp_psf_desc* psf, int32_t n_radii, const double* radii, double* flux);
+/* Exact overlap area of the unit pixel centred at (dx, dy) with the circle of
+ * radius r about the origin (aperture_weight(..., ExactOverlap()),
+ * src/apertures.jl:160-166).  Exposed for checks. */
+double cp_pixel_overlap(double dx, double dy, double r);
+
 /* ---- serving: continuous batching of simultaneous fits ---------------------
  * No counterpart in the reference, which fits one image per call: for callers
  * that stream many image tiles.  A server owns n_slots contexts (one HIP

@@ -298,11 +298,84 @@ void evaluate_fg(const Psf& P, const double* prm, double py, double px, double& 
 }
 
 // Unit-flux curve of growth for :auto model radii (curve_of_growth.jl:180-211).
+double cog_pixels(const Psf& P, double r);
+
 double cog_unit(const Psf& P, double r) {
     if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN) return 1 - std::exp(GAUSS_PRE * r * r / (P.fwhm * P.fwhm));
     if (P.kind == CP_PSF_CIRCULAR_MOFFAT)
         return 1 - std::pow(1 + r * r / (P.alpha * P.alpha), 1 - P.beta);
-    return std::numeric_limits<double>::quiet_NaN();
+    return cog_pixels(P, r);   // no closed form: the generic aperture sum
+}
+
+// Exact area of the rectangle [x0, x1] x [y0, y1] inside the circle of radius
+// r about the origin, as the integral over x of the covered vertical extent
+// max(0, min(y1, h) - max(y0, -h)), h = sqrt(r^2 - x^2).  The extent's form
+// changes only where h or -h crosses y0 or y1 (x = +-sqrt(r^2 - y^2)), so the
+// integral is split there and each piece, whose form its midpoint decides, is
+// integrated in closed form (int h dx = (x h + r^2 asin(x / r)) / 2).
+double rect_circle_area(double x0, double y0, double x1, double y1, double r) {
+    const double a = std::max(x0, -r), b = std::min(x1, r);
+    if (!(b > a)) return 0.0;
+    // x = 0 is a cut too: there h reaches r, so a bound at |y| = r touches
+    // the circle without crossing it and no piece has it as its midpoint
+    std::vector<double> cut = {a, b};
+    if (a < 0 && b > 0) cut.push_back(0.0);
+    for (double y : {y0, y1})
+        if (std::fabs(y) < r) {
+            const double s = std::sqrt((r - y) * (r + y));
+            for (double c : {-s, s})
+                if (c > a && c < b) cut.push_back(c);
+        }
+    std::sort(cut.begin(), cut.end());
+    auto hh = [&](double x) { return std::sqrt(std::max(0.0, (r - x) * (r + x))); };
+    auto Hi = [&](double x) { return 0.5 * (x * hh(x) + r * r * std::asin(std::min(1.0, std::max(-1.0, x / r)))); };
+    double area = 0;
+    for (size_t k = 0; k + 1 < cut.size(); ++k) {
+        const double u = cut[k], v = cut[k + 1];
+        if (!(v > u)) continue;
+        const double m = 0.5 * (u + v), hm = hh(m);
+        const bool top_is_h = hm < y1, bot_is_h = -hm > y0;
+        const double top = top_is_h ? hm : y1, bot = bot_is_h ? -hm : y0;
+        if (!(top > bot)) continue;
+        const double It = top_is_h ? Hi(v) - Hi(u) : y1 * (v - u);
+        const double Ib = bot_is_h ? -(Hi(v) - Hi(u)) : y0 * (v - u);
+        area += It - Ib;
+    }
+    return area;
+}
+
+// aperture_weight(ap, i, j, ExactOverlap()) (apertures.jl:93-166) for the
+// pixel centred at offset (dx, dy) from the aperture centre.
+double pixel_overlap(double dx, double dy, double r) {
+    const double d2 = dx * dx + dy * dy, dr = std::sqrt(2.0) / 2;
+    if (d2 > (r + dr) * (r + dr)) return 0.0;
+    if (r > dr && d2 < (r - dr) * (r - dr)) return 1.0;
+    return rect_circle_area(dx - 0.5, dy - 0.5, dx + 0.5, dy + 0.5, r);
+}
+
+// curve_of_growth(model, radii) generic method (curve_of_growth.jl:143-176)
+// for the unit model at (0, 0) over bounding_axes (apertures.jl:52-65).
+double cog_pixels(const Psf& P, double r) {
+    double prm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN) {
+        prm[2] = P.fwhm;
+        prm[3] = 1;
+    } else if (P.kind == CP_PSF_CIRCULAR_MOFFAT) {
+        prm[2] = P.alpha;
+        prm[3] = P.beta;
+        prm[4] = 1;
+    } else {
+        prm[2] = 1;
+    }
+    const double lo = -r - 0.5, hi = r - 0.5;
+    const long i0 = (long)(std::ceil(lo) == lo ? std::ceil(lo) + 1 : std::ceil(lo)), i1 = (long)std::ceil(hi);
+    double s = 0;
+    for (long j = i0; j <= i1; ++j)
+        for (long i = i0; i <= i1; ++i) {
+            const double w = pixel_overlap((double)j, (double)i, r);
+            if (w > 0) s += w * evaluate(P, prm, (double)i, (double)j);
+        }
+    return s;
 }
 
 // Julia default quantile (type 7) of an unsorted sample.
@@ -781,6 +854,16 @@ double or_curve_of_growth(const cp_psf_desc* psf, double r) {
     return cog_unit(P, r);
 }
 
+// The generic aperture-sum curve of growth for any model (checks against the
+// analytic forms) and the exact pixel overlap.
+double or_curve_of_growth_pixels(const cp_psf_desc* psf, double r) {
+    Psf P;
+    if (make_psf(psf, P) || !(r > 0)) return std::numeric_limits<double>::quiet_NaN();
+    return cog_pixels(P, r);
+}
+
+double or_pixel_overlap(double dx, double dy, double r) { return pixel_overlap(dx, dy, r); }
+
 // _morton2d (psf_photometry_simultaneous.jl:163-174).
 uint64_t or_morton2d(int64_t y, int64_t x) {
     auto spread = [](uint64_t v) {
@@ -810,7 +893,7 @@ int or_ring_offsets(int32_t R, int32_t* dy, int32_t* dx) {
 }
 
 // _model_radii (psf_photometry_simultaneous.jl:209-243).  model_rad > 0 is the
-// scalar path; model_rad <= 0 is :auto (analytic-CoG models only).
+// scalar path; model_rad <= 0 is :auto.
 int or_model_radii(const cp_psf_desc* psf, double model_rad, double nsigma, int32_t R_fit,
                    int32_t R_cap, const double* w, int64_t npix, const double* flux_init,
                    int32_t n, int32_t* out) {
@@ -822,8 +905,6 @@ int or_model_radii(const cp_psf_desc* psf, double model_rad, double nsigma, int3
     }
     Psf P;
     if (make_psf(psf, P)) return CP_EARG;
-    if (P.kind == CP_PSF_GRIDDED_IMAGE)
-        return fail("model_rad=:auto for ImagePSF/GriddedPSFModel needs the aperture curve of growth, not built");
     int64_t step = std::max<int64_t>(1, npix / 65536);
     std::vector<double> ws;
     for (int64_t i = 0; i < npix; i += step)

@@ -31,6 +31,8 @@ void or_bicubic(const double* data, int32_t ny, int32_t nx, double y, double x, 
 int or_grid_corners_dw(const cp_psf_desc* psf, double Y, double X, int32_t* idx4, double* w4,
                        double* dwdy4, double* dwdx4);
 double or_curve_of_growth(const cp_psf_desc* psf, double r);
+double or_curve_of_growth_pixels(const cp_psf_desc* psf, double r);
+double or_pixel_overlap(double dx, double dy, double r);
 
 /* binning / indexing */
 uint64_t or_morton2d(int64_t y, int64_t x);

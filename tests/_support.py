@@ -52,6 +52,8 @@ def oracle():
         "or_bicubic": (None, [f64p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, f64p]),
         "or_grid_corners_dw": (C.c_int, [P(abi.PsfDesc), C.c_double, C.c_double, i32p, f64p, f64p, f64p]),
         "or_curve_of_growth": (C.c_double, [P(abi.PsfDesc), C.c_double]),
+        "or_curve_of_growth_pixels": (C.c_double, [P(abi.PsfDesc), C.c_double]),
+        "or_pixel_overlap": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "or_morton2d": (C.c_uint64, [C.c_int64, C.c_int64]),
         "or_ring_offsets": (C.c_int, [C.c_int32, i32p, i32p]),
         "or_model_radii": (C.c_int, [P(abi.PsfDesc), C.c_double, C.c_double, C.c_int32, C.c_int32, f64p,

@@ -223,6 +223,23 @@ def test_auto_model_radius_parity(ctx):
     _compare(r64, ro, 1e-7, 1e-7)
 
 
+def test_auto_model_radius_parity_epsf(ctx):
+    """model_rad = :auto with a GriddedPSFModel: radii from the pixel-integrated
+    curve of growth over exact circular apertures."""
+    print("case: crowded epsf 192x192 model_rad=auto", flush=True)
+    n = int(round(2e5 * (192 * 192) / 4096 ** 2))
+    cfg = sim.config_epsf_crowded(seed=8, shape=(192, 192), n=n, max_iter=10)
+    cfg["image"] = np.asarray(cfg["image"], np.float32).astype(np.float64)
+    cfg["model_rad"] = 0.0
+    ro = _oracle_run(cfg)
+    cfg2 = dict(cfg)
+    cfg2["model_rad"] = "auto"
+    r64 = _gpu_run(ctx, cfg2, "fp64")
+    _compare(r64, ro, 1e-6, 1e-6, frac=0.99)
+    fixed = _gpu_run(ctx, dict(cfg, model_rad=cfg["fit_rad"]), "fp64")
+    assert not np.array_equal(fixed.residual, r64.residual)
+
+
 def test_frozen_stars_stable_on_gpu(ctx):
     cfg = sim.config_gauss_small(seed=9, shape=(200, 200), n=60)
     r1 = _gpu_run(ctx, dict(cfg, max_iter=6), "mixed")
