@@ -69,6 +69,11 @@ struct Scal {
     double sig, tau;        // pending x/w update of the last finished iteration
     double atol, btol;
     int iter, done, itmax, solver;
+    // J v visits only pixels with a live stamp entry (n_jv_pix of them); the
+    // other union pixels are zero rows of J, where u only scales: their
+    // share of |u|^2 is fscale^2 * sf
+    double sf, fscale;
+    int n_jv_pix, pad1;
     // CGLS
     double gamma, cg_alpha, cg_beta, pnorm2, qnorm2, s0norm;
     unsigned int counters[8];
@@ -858,17 +863,22 @@ static __global__ void k_csr_pix(const unsigned int* __restrict__ key, int* __re
     if (u == 0) ptr[U] = *E;
 }
 
-// u <- J v - alpha u over the union pixels (apply_J! :95-133 as a gather).
-// Row r = a S^2 + m of V belongs to star a = floor(r / S^2), computed as
-// floor(r * (1/S^2) + 0.5/S^2) in float64 (exact for r < 2^31).
+// u <- J v - alpha u over the union pixels that hold a live stamp entry
+// (apply_J! :95-133 as a gather over the live-compacted structure; the other
+// union pixels are zero rows, accounted for in k_finish<FIN_BETA>).  Row r =
+// a S^2 + m of V belongs to star a = floor(r / S^2), computed as
+// floor(r * (1/S^2) + 0.5/S^2) in float64 (exact for r < 2^31).  The first
+// call also sums |b|^2 over the visited pixels (partials2).
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_Jv(int U, const int* __restrict__ pix, const int* __restrict__ ptr,
+__global__ __launch_bounds__(BLOCK) void k_Jv(const int* __restrict__ pix, const int* __restrict__ ptr,
                                               const int* __restrict__ ent, int S2, const T* __restrict__ V,
                                               const double* __restrict__ tsrc,
                                               const T* __restrict__ usrc, T* __restrict__ ubuf, int first,
-                                              double* __restrict__ partials, Scal* Sc) {
+                                              double* __restrict__ partials, double* __restrict__ partials2,
+                                              Scal* Sc) {
     if (Sc->done) return;
     __shared__ double sred[NWAVE];
+    const int U = Sc->n_jv_pix;
     const double alpha = Sc->alpha;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
     double tfac, ufac;
@@ -882,7 +892,7 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(int U, const int* __restrict__ pix
     const double inv_s2 = 1.0 / S2, half_s2 = 0.5 / S2;
     const vec4_t<T>* V4 = reinterpret_cast<const vec4_t<T>*>(V);
     const double4* t4 = reinterpret_cast<const double4*>(tsrc);
-    double acc2 = 0;
+    double acc2 = 0, accb = 0;
     for (int u = blockIdx.x * BLOCK + threadIdx.x; u < U; u += gridDim.x * BLOCK) {
         const int q = pix[u];
         const int e0 = ptr[u], e1 = ptr[u + 1];
@@ -894,11 +904,53 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(int U, const int* __restrict__ pix
             const double4 t = t4[a];
             y += (double)v.x * t.x + (double)v.y * t.y + (double)v.z * t.z;
         }
-        const T uu = (T)(y * tfac - alpha * ((double)usrc[q] * ufac));
+        const double us = (double)usrc[q];
+        const T uu = (T)(y * tfac - alpha * (us * ufac));
         ubuf[q] = uu;
         acc2 += (double)uu * (double)uu;
+        accb += us * us;
     }
     block_store(acc2, partials, sred);
+    block_store(first ? accb : 0.0, partials2, sred);
+}
+
+// Live compaction of the pixel-major structure (rebuilt from the full one as
+// stars freeze): flag[e] = 1 for entries of live stars, flag[E] = 0.
+static __global__ void k_entry_live(int E, const int* __restrict__ ent, int S2, const uint8_t* __restrict__ live,
+                                    int* __restrict__ flag) {
+    const double inv_s2 = 1.0 / S2, half_s2 = 0.5 / S2;
+    for (int e = blockIdx.x * BLOCK + threadIdx.x; e <= E; e += gridDim.x * BLOCK)
+        flag[e] = e < E ? (int)live[(int)fma((double)ent[e], inv_s2, half_s2)] : 0;
+}
+
+// ent_l[pos[e]] = ent[e] for flagged entries (pos = exclusive scan of flag),
+// and pflag[u] = 1 for pixels with a flagged entry (pflag[U] = 0).
+static __global__ void k_entry_compact(int E, int U, const int* __restrict__ ent, const int* __restrict__ flag,
+                                       const int* __restrict__ pos, const int* __restrict__ ptr,
+                                       int* __restrict__ ent_l, int* __restrict__ pflag) {
+    const int stride = gridDim.x * BLOCK;
+    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < E; e += stride)
+        if (flag[e]) ent_l[pos[e]] = ent[e];
+    for (int u = blockIdx.x * BLOCK + threadIdx.x; u <= U; u += stride)
+        pflag[u] = u < U ? (int)(pos[ptr[u + 1]] > pos[ptr[u]]) : 0;
+}
+
+// pix_l / ptr_l of the pixels with live entries (ppos = exclusive scan of
+// pflag), ptr_l[L] = number of live entries, Sc->n_jv_pix = L.
+static __global__ void k_pixel_compact(int U, const int* __restrict__ pix, const int* __restrict__ ptr,
+                                       const int* __restrict__ pos, const int* __restrict__ pflag,
+                                       const int* __restrict__ ppos, int* __restrict__ pix_l,
+                                       int* __restrict__ ptr_l, Scal* Sc) {
+    for (int u = blockIdx.x * BLOCK + threadIdx.x; u < U; u += gridDim.x * BLOCK)
+        if (pflag[u]) {
+            const int k = ppos[u];
+            pix_l[k] = pix[u];
+            ptr_l[k] = pos[ptr[u]];
+        }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ptr_l[ppos[U]] = pos[ptr[U]];
+        Sc->n_jv_pix = ppos[U];
+    }
 }
 
 // v <- J^T u - beta v per star, with the pending x/w update of the previous
@@ -1057,7 +1109,7 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const dou
         a += part[i];
         if (OP == FIN_FILL) b = fmax(b, part[nb + i]);
         if (OP == FIN_ACCEPT) b += part[nb + i];
-        if (OP == FIN_LSQR) b += part2[i];
+        if (OP == FIN_LSQR || OP == FIN_BETA) b += part2[i];
     }
     a = wave_sum(a);
     b = OP == FIN_FILL ? wave_max(b) : wave_sum(b);
@@ -1078,7 +1130,22 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const dou
         Sc->bnorm2 = ta;
         Sc->gmax_bits = (unsigned long long)__double_as_longlong(tb);
     }
-    if (OP == FIN_BETA) Sc->beta_next = sqrt(ta);
+    if (OP == FIN_BETA) {
+        // union pixels without a live entry are zero rows of J, where the
+        // update is u <- -alpha ufac u (the factor k_Jv applies): their part
+        // of |u|^2 is fscale^2 times |b|^2 over them, which is the residual
+        // norm beta1^2 less the part k_Jv visited (tb, first call)
+        double ufac;
+        if (it == 1) {
+            ufac = -1.0 / Sc->beta1;
+            Sc->sf = fmax(0.0, Sc->beta1 * Sc->beta1 - tb);
+            Sc->fscale = 1.0;
+        } else {
+            ufac = Sc->beta_cur != 0 ? 1.0 / Sc->beta_cur : 1.0;
+        }
+        Sc->fscale *= -Sc->alpha * ufac;
+        Sc->beta_next = sqrt(ta + Sc->fscale * Sc->fscale * Sc->sf);
+    }
     if (OP == FIN_LSQR) lsqr_advance(Sc, ta, tb, it);
     if (OP == FIN_ACCEPT) {
         Sc->n_live = (int)ta;

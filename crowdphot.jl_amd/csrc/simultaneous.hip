@@ -594,6 +594,39 @@ struct Sim {
                    (const double*)part2.as<double>(), nb, it);
         };
 
+        // --- J v over live stars only: the pixel-major structure restricted
+        // to live stars' entries and the pixels that hold one, rebuilt from
+        // the full structure whenever the live set has shrunk by a tenth.
+        // The buffers never move (captured trial graphs keep their
+        // arguments); k_Jv reads the pixel count from Sc.
+        DevBuf ent_l(ctx, (size_t)std::max(csr.E, 1) * 4), ptr_l(ctx, (size_t)(csr.U + 1) * 4),
+            pix_l(ctx, (size_t)std::max(csr.U, 1) * 4);
+        DevBuf cflag(ctx, (size_t)(csr.E + 1) * 4), cpos(ctx, (size_t)(csr.E + 1) * 4);
+        DevBuf pflag(ctx, (size_t)(csr.U + 1) * 4), ppos(ctx, (size_t)(csr.U + 1) * 4);
+        int live_compacted = -1;
+        auto compact_jv = [&]() {
+            const int gbe = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)csr.E + 1, BLOCK), (int64_t)n_cu * 16));
+            launch(k_entry_live, dim3(gbe), dim3(BLOCK), 0, st, csr.E, (const int*)csr.ent.as<int>(), S2,
+                   (const uint8_t*)live.as<uint8_t>(), cflag.as<int>());
+            size_t b = 0;
+            CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, cflag.as<int>(), cpos.as<int>(), csr.E + 1, st));
+            cub.ensure(b);
+            b = cub.tmp.bytes();
+            CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, cflag.as<int>(), cpos.as<int>(), csr.E + 1, st));
+            launch(k_entry_compact, dim3(gbe), dim3(BLOCK), 0, st, csr.E, csr.U, (const int*)csr.ent.as<int>(),
+                   (const int*)cflag.as<int>(), (const int*)cpos.as<int>(), (const int*)csr.ptr.as<int>(),
+                   ent_l.as<int>(), pflag.as<int>());
+            b = 0;
+            CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, pflag.as<int>(), ppos.as<int>(), csr.U + 1, st));
+            cub.ensure(b);
+            b = cub.tmp.bytes();
+            CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, pflag.as<int>(), ppos.as<int>(), csr.U + 1, st));
+            const int gbp = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * 16));
+            launch(k_pixel_compact, dim3(gbp), dim3(BLOCK), 0, st, csr.U, (const int*)csr.pix.as<int>(),
+                   (const int*)csr.ptr.as<int>(), (const int*)cpos.as<int>(), (const int*)pflag.as<int>(),
+                   (const int*)ppos.as<int>(), pix_l.as<int>(), ptr_l.as<int>(), Sc);
+        };
+
         // --- outer damped Gauss-Newton loop (:1186-1284)
         // An accepted candidate's weighted residual (written by its cost
         // render) is the next iteration's residual: the live-star model minus
@@ -605,6 +638,10 @@ struct Sim {
         const bool use_graphs = !ctx->prof;   // per-launch event timing needs plain launches
         bool D_init = false, have_resid = false;
         for (int iter = 1; iter <= o.max_iter; ++iter) {
+            if (live_compacted < 0 || n_live_now < 0.9 * live_compacted) {
+                compact_jv();
+                live_compacted = n_live_now;
+            }
             star_rec(theta.as<double>(), recs, crs, ers, 0, 0);
             if (!have_resid) {
                 render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
@@ -642,10 +679,11 @@ struct Sim {
                 auto trial_body = [&]() {
                     for (int it = 1; it <= o.inner_iterations; ++it) {
                         const int first = it == 1;
-                        launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, csr.U, (const int*)csr.pix.as<int>(),
-                                 (const int*)csr.ptr.as<int>(), (const int*)csr.ent.as<int>(), S2, (const T*)V.as<T>(),
+                        launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, (const int*)pix_l.as<int>(),
+                                 (const int*)ptr_l.as<int>(), (const int*)ent_l.as<int>(), S2, (const T*)V.as<T>(),
                                  (const double*)(first ? bsc.as<double>() : tb.as<double>()),
-                                 (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(), first, part.as<double>(), Sc);
+                                 (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(), first, part.as<double>(),
+                                 part2.as<double>(), Sc);
                         finish(k_finish<FIN_BETA>, gb_jv, it);
                         jtu_launch(S2, dim3(gb_jtu), na, (const int*)ay.as<int>(), (const int*)ax.as<int>(),
                                    (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx, (const T*)V.as<T>(),
