@@ -34,22 +34,31 @@ struct ModelConst {
     const float* grids;    // device, n_nodes * gny * gnx, node-major, each column-major
     T bkg;                 // fixed background added by every star's model
     int ios_y, ios_x;      // integer oversampling factors
-    int Qy, Qx;            // polyphase plane sides: ceil(gny / os_y), ceil(gnx / os_x)
+    int Qy, Qx;            // polyphase plane sides of the edge-padded grid:
+                           // ceil((gny + 2 PAD) / os_y), ceil((gnx + 2 PAD) / os_x)
 };
+
+// Edge padding of the LDS copy of an ePSF grid: PAD samples on every side
+// repeat the nearest edge sample, so the reference's clamped stencil
+// (bicubic_interpolate, empirical_models.jl:253-292) becomes a plain 16-tap
+// read for every in-bounds pixel.
+constexpr int EPSF_PAD = 2;
 
 // Per-star constants of the single-node ePSF fast path.  With integer
 // oversampling every pixel of a star shares the fractional grid offset, so the
 // Keys cubic-convolution weights (the expansion of cubic4,
 // empirical_models.jl:185-196) are fixed per star: value = sum_ab wy[a] wx[b] g,
 // d/dv uses dwy, d/du uses dwx.  ra/ca are the polyphase LDS addresses of the
-// four stencil rows / columns at the anchor pixel; a pixel at offset (dy, dx)
-// adds dy + dx * Qy.
+// four stencil rows / columns at the anchor pixel in the edge-padded grid; a
+// pixel at offset (dy, dx) adds dy + dx * Qy.  Pixels outside [dylo, dyhi] x
+// [dxlo, dxhi] are outside the grid's closed range (value fill); ok = 0 when
+// no pixel is inside.
 template <typename T>
 struct EpsfRec {
     T wy[4], wx[4], dwy[4], dwx[4];
     int ly, lx;      // floor(v), floor(u) at the anchor pixel (1-based grid coordinates)
     int ra[4], ca[4];
-    int ih;          // every pixel with |dy|, |dx| <= ih evaluates on the unclamped interior stencil
+    int dylo, dyhi, dxlo, dxhi, ok;
 };
 
 // Per-star record used by the gather kernels (rebuilt whenever theta changes).
@@ -91,14 +100,22 @@ __host__ __device__ __forceinline__ int poly_addr(int r, int c, int osy, int osx
     return ((r % osy) * osx + (c % osx)) * (Qy * Qx) + (c / osx) * Qy + (r / osy);
 }
 
-// Stage one node grid (global, column-major) into LDS in the polyphase layout.
+// Stage one node grid (global, column-major) into LDS in the polyphase layout,
+// edge-padded by EPSF_PAD samples (padded sample (r, c) holds grid sample
+// (clamp(r - PAD), clamp(c - PAD))).
 __device__ __forceinline__ void stage_poly(const float* __restrict__ g, int gny, int gnx, int osy, int osx,
                                            int Qy, int Qx, float* __restrict__ sg, int tid, int nthr) {
-    for (int i = tid; i < gny * gnx; i += nthr) {
-        const int c = i / gny, r = i - c * gny;
-        sg[poly_addr(r, c, osy, osx, Qy, Qx)] = g[i];
+    const int py = gny + 2 * EPSF_PAD, px = gnx + 2 * EPSF_PAD;
+    for (int i = tid; i < py * px; i += nthr) {
+        const int c = i / py, r = i - c * py;
+        const int rs = min(max(r - EPSF_PAD, 0), gny - 1), cs = min(max(c - EPSF_PAD, 0), gnx - 1);
+        sg[poly_addr(r, c, osy, osx, Qy, Qx)] = g[rs + (size_t)cs * gny];
     }
 }
+
+// floor(a / b) and ceil(a / b) for b > 0
+__host__ __device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+__host__ __device__ __forceinline__ int ceil_div(int a, int b) { return -floor_div(-a, b); }
 
 // Build the per-star fast-path constants from its fractional position.
 template <typename T>
@@ -117,83 +134,60 @@ __device__ __forceinline__ void epsf_rec(const ModelConst<T>& M, double fy, doub
     for (int a = 0; a < 4; ++a) e.dwy[a] = wd[a];
     keys_weights<T>((T)(u0 - flx), e.wx, wd);
     for (int a = 0; a < 4; ++a) e.dwx[a] = wd[a];
+    // pixel offsets whose floor(v) is inside the closed range: [1, gny - 1],
+    // or gny exactly when v = gny (t = 0, wy[1] = 1)
+    const int osy = M.ios_y, osx = M.ios_x;
+    e.dylo = ceil_div(1 - e.ly, osy);
+    e.dyhi = (e.wy[1] == T(1) && floor_div(M.gny - e.ly, osy) * osy == M.gny - e.ly) ? floor_div(M.gny - e.ly, osy)
+                                                                                     : floor_div(M.gny - 1 - e.ly, osy);
+    e.dxlo = ceil_div(1 - e.lx, osx);
+    e.dxhi = (e.wx[1] == T(1) && floor_div(M.gnx - e.lx, osx) * osx == M.gnx - e.lx) ? floor_div(M.gnx - e.lx, osx)
+                                                                                     : floor_div(M.gnx - 1 - e.lx, osx);
+    e.ok = e.dylo <= e.dyhi && e.dxlo <= e.dxhi;
     const int QQ = M.Qy * M.Qx;
     for (int a = 0; a < 4; ++a) {
-        // 0-based row of stencil entry a is ly - 2 + a; floor division keeps
-        // the plane arithmetic exact for negative rows (used only off the fast path)
-        int r = e.ly - 2 + a, c = e.lx - 2 + a;
-        int pr = ((r % M.ios_y) + M.ios_y) % M.ios_y, qr = (r - pr) / M.ios_y;
-        int pc = ((c % M.ios_x) + M.ios_x) % M.ios_x, qc = (c - pc) / M.ios_x;
-        e.ra[a] = pr * M.ios_x * QQ + qr;
-        e.ca[a] = pc * QQ + qc * M.Qy;
+        // stencil entry a reads 0-based grid row ly - 2 + a, padded row ly + a;
+        // addresses are those of the in-range pixel (dylo, dxlo) shifted back
+        // by (dylo, dxlo), so every clamped pixel offset lands in the grid
+        const int r = e.ok ? e.ly - 2 + a + EPSF_PAD + osy * e.dylo : 0;
+        const int c = e.ok ? e.lx - 2 + a + EPSF_PAD + osx * e.dxlo : 0;
+        e.ra[a] = (r % osy) * osx * QQ + r / osy - (e.ok ? e.dylo : 0);
+        e.ca[a] = (c % osx) * QQ + (c / osx - (e.ok ? e.dxlo : 0)) * M.Qy;
     }
-    // interior stencil rows / columns are 2 .. gn-2; the pixel offset d moves
-    // the stencil by os * d grid samples
-    auto room = [](int lo, int hi, int os) { return lo < 0 || hi < 0 ? -1 : min(lo, hi) / os; };
-    e.ih = min(room(e.ly - 2, M.gny - 2 - e.ly, M.ios_y), room(e.lx - 2, M.gnx - 2 - e.lx, M.ios_x));
+    if (!e.ok) e.dylo = e.dyhi = e.dxlo = e.dxhi = 0;
 }
 
-// Grid sample with the reference's stencil clamping, polyphase LDS layout.
-__device__ __forceinline__ float poly_sample(const float* __restrict__ sg, int r, int c, int gny, int gnx, int osy,
-                                             int osx, int Qy, int Qx) {
-    r = min(max(r, 0), gny - 1);
-    c = min(max(c, 0), gnx - 1);
-    return sg[poly_addr(r, c, osy, osx, Qy, Qx)];
-}
-
-// ePSF value (no flux/bkg) of a star at pixel offset (dy, dx), fast path.
-// Interior evaluation (stencil rows / columns within 2 .. gn-2 for this
-// pixel): 16 LDS taps, no bounds tests.
-template <typename T, bool DERIV>
-__device__ __forceinline__ void epsf_interior(const ModelConst<T>& M, const EpsfRec<T>& e, const float* __restrict__ sg,
-                                              int dy, int dx, T& val, T& dpdv, T& dpdu) {
-    T row[4], drow[4];
-    const int off = dy + dx * M.Qy;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const float* rp = sg + e.ra[a] + off;
-        const T g0 = (T)rp[e.ca[0]], g1 = (T)rp[e.ca[1]], g2 = (T)rp[e.ca[2]], g3 = (T)rp[e.ca[3]];
-        row[a] = e.wx[0] * g0 + e.wx[1] * g1 + e.wx[2] * g2 + e.wx[3] * g3;
-        if (DERIV) drow[a] = e.dwx[0] * g0 + e.dwx[1] * g1 + e.dwx[2] * g2 + e.dwx[3] * g3;
-    }
-    val = e.wy[0] * row[0] + e.wy[1] * row[1] + e.wy[2] * row[2] + e.wy[3] * row[3];
-    if (DERIV) {
-        dpdv = e.dwy[0] * row[0] + e.dwy[1] * row[1] + e.dwy[2] * row[2] + e.dwy[3] * row[3];
-        dpdu = e.wy[0] * drow[0] + e.wy[1] * drow[1] + e.wy[2] * drow[2] + e.wy[3] * drow[3];
-    }
-}
-
+// ePSF value (no flux / bkg) of a star at pixel offset (dy, dx) and its
+// derivatives along v and u, from the edge-padded polyphase LDS grid: the
+// offset is clamped to the star's in-range window (so the 16 reads stay in
+// the grid), and pixels outside the window take the fill value.
 template <typename T, bool DERIV>
 __device__ __forceinline__ void epsf_eval(const ModelConst<T>& M, const EpsfRec<T>& e, const float* __restrict__ sg,
                                           int dy, int dx, T& val, T& dpdv, T& dpdu) {
-    const int ly = e.ly + M.ios_y * dy, lx = e.lx + M.ios_x * dx;
-    // closed-bounds test of bicubic_interpolate: floor(v) in [1, gny-1], or v == gny exactly
-    const bool inb = ly >= 1 && lx >= 1 && (ly < M.gny || (ly == M.gny && e.wy[1] == T(1))) &&
-                     (lx < M.gnx || (lx == M.gnx && e.wx[1] == T(1)));
-    if (!inb) {
-        val = M.fill;
-        dpdv = dpdu = T(0);
-        return;
-    }
+    const bool inb = e.ok && dy >= e.dylo && dy <= e.dyhi && dx >= e.dxlo && dx <= e.dxhi;
+    const int dyc = min(max(dy, e.dylo), e.dyhi), dxc = min(max(dx, e.dxlo), e.dxhi);
+    const float* sp = sg + (dyc + dxc * M.Qy);
+    float g[16];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) g[a * 4 + b] = sp[e.ra[a] + e.ca[b]];
     T row[4], drow[4];
-    if (ly >= 2 && ly <= M.gny - 2 && lx >= 2 && lx <= M.gnx - 2) {
-        epsf_interior<T, DERIV>(M, e, sg, dy, dx, val, dpdv, dpdu);
-        return;
-    } else {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            T g[4];
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-                g[b] = (T)poly_sample(sg, ly - 2 + a, lx - 2 + b, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx);
-            row[a] = e.wx[0] * g[0] + e.wx[1] * g[1] + e.wx[2] * g[2] + e.wx[3] * g[3];
-            if (DERIV) drow[a] = e.dwx[0] * g[0] + e.dwx[1] * g[1] + e.dwx[2] * g[2] + e.dwx[3] * g[3];
-        }
+    for (int a = 0; a < 4; ++a) {
+        row[a] = e.wx[0] * (T)g[a * 4] + e.wx[1] * (T)g[a * 4 + 1] + e.wx[2] * (T)g[a * 4 + 2] + e.wx[3] * (T)g[a * 4 + 3];
+        if (DERIV)
+            drow[a] = e.dwx[0] * (T)g[a * 4] + e.dwx[1] * (T)g[a * 4 + 1] + e.dwx[2] * (T)g[a * 4 + 2] +
+                      e.dwx[3] * (T)g[a * 4 + 3];
     }
     val = e.wy[0] * row[0] + e.wy[1] * row[1] + e.wy[2] * row[2] + e.wy[3] * row[3];
     if (DERIV) {
         dpdv = e.dwy[0] * row[0] + e.dwy[1] * row[1] + e.dwy[2] * row[2] + e.dwy[3] * row[3];
         dpdu = e.wy[0] * drow[0] + e.wy[1] * drow[1] + e.wy[2] * drow[2] + e.wy[3] * drow[3];
+    }
+    if (!inb) {
+        val = M.fill;
+        dpdv = dpdu = T(0);
     }
 }
 

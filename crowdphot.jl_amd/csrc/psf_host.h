@@ -162,8 +162,8 @@ ModelConst<T> model_const(const HostPsf& P, const DevPsf& D, double bkg) {
         M.grids = D.grids.as<float>();
         M.ios_y = (int)P.osy;
         M.ios_x = (int)P.osx;
-        M.Qy = (P.gny + M.ios_y - 1) / M.ios_y;
-        M.Qx = (P.gnx + M.ios_x - 1) / M.ios_x;
+        M.Qy = (P.gny + 2 * EPSF_PAD + M.ios_y - 1) / M.ios_y;
+        M.Qx = (P.gnx + 2 * EPSF_PAD + M.ios_x - 1) / M.ios_x;
     }
     M.bkg = (T)bkg;
     return M;

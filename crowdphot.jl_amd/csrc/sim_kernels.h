@@ -510,6 +510,21 @@ __device__ __forceinline__ void tile_share(const long long* __restrict__ scan, i
     t1 = upper(total * (b + 1) / nb);
 }
 
+// ceil(65536 / h) for h = 1..16: (p * RENDER_INV_H[h]) >> 16 == p / h for 0 <= p < 256
+__constant__ const int RENDER_INV_H[17] = {0,    65536, 32768, 21846, 16384, 13108, 10923, 9363, 8192,
+                                           7282, 6554,  5958,  5462,  5042,  4682,  4370,  4096};
+
+// Dynamic LDS of the render: the staged ePSF grid (ngrid floats), the staged
+// star ids and sub-block masks (RCHUNK each), then per wave a 16 x 16
+// accumulator of T and a 16 x 16 byte touch map.
+__host__ __device__ __forceinline__ size_t render_acc_offset(int ngrid) {
+    return (((size_t)ngrid * 4 + 15) & ~(size_t)15) + (size_t)RCHUNK * 2 * sizeof(int);
+}
+template <typename T>
+__host__ __device__ __forceinline__ size_t render_lds_bytes(int ngrid) {
+    return render_acc_offset(ngrid) + (size_t)RWAVE * RSB * RSB * (sizeof(T) + 1);
+}
+
 // float renders fit 8 waves / SIMD: two 1024-thread blocks per CU
 template <typename T, int KIND, int MODE, bool LDSG>
 __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
@@ -531,6 +546,10 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     const float* grid = LDSG ? sgrid : M.grids;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int sbr = (wv % (RT / RSB)) * RSB, sbc = (wv / (RT / RSB)) * RSB;
+    // this wave's sub-block accumulator (column-major 16 x 16) and, for the
+    // freeze render, which of its pixels a star touched
+    T* sacc = reinterpret_cast<T*>(smem + render_acc_offset(ngrid)) + wv * (RSB * RSB);
+    uint8_t* stouch = smem + render_acc_offset(ngrid) + RWAVE * RSB * RSB * sizeof(T) + wv * (RSB * RSB);
     double acc = 0;
     int t0, t1;
     tile_share(tscan, ntiles, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, t0, t1);
@@ -538,19 +557,19 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
         const int ty = tile % g.nty, tx = tile / g.nty;
         const int ty0 = ty * RT + 1, tx0 = tx * RT + 1;       // tile origin (1-based)
         const int gy0 = ty0 + sbr, gx0 = tx0 + sbc;           // this wave's sub-block origin
+        // lane owns pixels (row lane % 16, column 4 k + lane / 16) for the
+        // initial and final passes
         const int gy = gy0 + lane % 16;
-        T val[4];
-        int gxv[4];
-        bool act[4], touched[4];
+        bool act[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            gxv[k] = gx0 + 4 * k + lane / 16;
-            act[k] = gy <= g.ny && gxv[k] <= g.nx;
-            const size_t q = (size_t)(gy - 1) + (size_t)(gxv[k] - 1) * g.ny;
+            const int gx = gx0 + 4 * k + lane / 16;
+            act[k] = gy <= g.ny && gx <= g.nx;
+            const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * g.ny;
             if ((MODE == MODE_RESID || MODE == MODE_COST) && act[k]) act[k] = umask[q] != 0;
-            val[k] = T(0);
-            touched[k] = false;
-            if (MODE == MODE_FREEZE && act[k]) val[k] = dw[q];
+            const int idx = (4 * k + lane / 16) * RSB + lane % 16;
+            sacc[idx] = (MODE == MODE_FREEZE && act[k]) ? dw[q] : T(0);
+            if (MODE == MODE_FREEZE) stouch[idx] = 0;
         }
         const int s0 = tstart[tile], s1 = tend[tile];
         for (int c0 = s0; c0 < s1; c0 += RCHUNK) {
@@ -593,30 +612,65 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                 if (!((mj >> wv) & 1)) continue;   // one LDS read per star
                 const int a = __builtin_amdgcn_readfirstlane(sid[j]);
                 const StarRec<T> r = recs[a];
-                const int dy = gy - r.ay;
-                const bool rin = abs(dy) <= r.mR;
-                bool fast = false;
-                if constexpr (EPSF_LDS) fast = ers[a].ih >= r.mR;
+                // the model box clipped to the sub-block, in local rows / columns;
+                // its h x wd pixels are spread over the lanes (pixel p: column
+                // p / h, row p % h), so no lane idles outside the box
+                const int r_lo = max(r.ay - r.mR - gy0, 0), r_hi = min(r.ay + r.mR - gy0, RSB - 1);
+                const int c_lo = max(r.ax - r.mR - gx0, 0), c_hi = min(r.ax + r.mR - gx0, RSB - 1);
+                const int h = r_hi - r_lo + 1, n = h * (c_hi - c_lo + 1);
+                if (h <= 0 || c_hi < c_lo) continue;
+                const int inv_h = RENDER_INV_H[h];   // p / h == (p inv_h) >> 16 for p < 256
+                // this star's ePSF stencil (epsf_eval with the record in registers)
+                int toff[16], wlo[2], whi[2];
+                bool sok = false;
+                T wyv[4], wxv[4];
+                if constexpr (EPSF_LDS) {
+                    const EpsfRec<T>& er = ers[a];
+                    sok = er.ok;
+                    wlo[0] = er.dylo;
+                    whi[0] = er.dyhi;
+                    wlo[1] = er.dxlo;
+                    whi[1] = er.dxhi;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int c_lo = gx0 + 4 * k;
-                    if (r.ax + r.mR < c_lo || r.ax - r.mR > c_lo + 3) continue;   // uniform
-                    const int dx = gxv[k] - r.ax;
-                    if (act[k] && rin && abs(dx) <= r.mR) {
+                    for (int ta = 0; ta < 4; ++ta) {
+                        wyv[ta] = er.wy[ta];
+                        wxv[ta] = er.wx[ta];
+#pragma unroll
+                        for (int tb = 0; tb < 4; ++tb) toff[ta * 4 + tb] = er.ra[ta] + er.ca[tb];
+                    }
+                }
+                for (int base = 0; base < n; base += 64) {
+                    const int pi = base + lane;
+                    if (pi < n) {
+                        const int col = (pi * inv_h) >> 16, row = pi - col * h;
+                        const int lr = r_lo + row, lc = c_lo + col;
+                        const int dy = gy0 + lr - r.ay, dx = gx0 + lc - r.ax;
                         T v;
                         if constexpr (EPSF_LDS) {
-                            T p, d0, d1;
-                            if (fast) epsf_interior<T, false>(M, ers[a], grid, dy, dx, p, d0, d1);
-                            else epsf_eval<T, false>(M, ers[a], grid, dy, dx, p, d0, d1);
-                            v = fma(r.flux, p, M.bkg);
+                            // all 16 LDS taps issued before the first is used
+                            const bool inb = sok && dy >= wlo[0] && dy <= whi[0] && dx >= wlo[1] && dx <= whi[1];
+                            const int dyc = min(max(dy, wlo[0]), whi[0]), dxc = min(max(dx, wlo[1]), whi[1]);
+                            const float* sp = grid + (dyc + dxc * M.Qy);
+                            float gv[16];
+#pragma unroll
+                            for (int t = 0; t < 16; ++t) gv[t] = sp[toff[t]];
+                            T rowv[4];
+#pragma unroll
+                            for (int ta = 0; ta < 4; ++ta)
+                                rowv[ta] = wxv[0] * (T)gv[ta * 4] + wxv[1] * (T)gv[ta * 4 + 1] + wxv[2] * (T)gv[ta * 4 + 2] +
+                                           wxv[3] * (T)gv[ta * 4 + 3];
+                            const T pv = inb ? wyv[0] * rowv[0] + wyv[1] * rowv[1] + wyv[2] * rowv[2] + wyv[3] * rowv[3]
+                                             : M.fill;
+                            v = fma(r.flux, pv, M.bkg);
                         } else {
                             v = star_value<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a);
                         }
+                        const int idx = lc * RSB + lr;
                         if (MODE == MODE_FREEZE) {
-                            val[k] -= v;
-                            touched[k] = true;
+                            sacc[idx] -= v;
+                            stouch[idx] = 1;
                         } else {
-                            val[k] += v;
+                            sacc[idx] += v;
                         }
                     }
                 }
@@ -625,16 +679,19 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (!act[k]) continue;
-            const size_t q = (size_t)(gy - 1) + (size_t)(gxv[k] - 1) * g.ny;
+            const int gx = gx0 + 4 * k + lane / 16;
+            const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * g.ny;
+            const int idx = (4 * k + lane / 16) * RSB + lane % 16;
+            const T val = sacc[idx];
             if (MODE == MODE_RESID || MODE == MODE_COST) {
-                T r = val[k] - dw[q];
+                T r = val - dw[q];
                 T wr = sqrt(w[q]) * r;
                 wt_resid[q] = wr;
                 acc += (double)wr * (double)wr;
             } else if (MODE == MODE_FREEZE) {
-                if (touched[k]) dw[q] = val[k];
+                if (stouch[idx]) dw[q] = val;
             } else {
-                resid[q] = data[q] - val[k];
+                resid[q] = data[q] - val;
             }
         }
     }
@@ -701,8 +758,7 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
             T g3[3];
             if constexpr (KIND == KIND_EPSF && LDSG) {
                 T pv, pdv, pdu;
-                if (er.ih >= R) epsf_interior<T, true>(M, er, grid, dy, dx, pv, pdv, pdu);   // uniform per star
-                else epsf_eval<T, true>(M, er, grid, dy, dx, pv, pdv, pdu);
+                epsf_eval<T, true>(M, er, grid, dy, dx, pv, pdv, pdu);
                 g3[0] = r.flux * (-(M.osy * pdv));
                 g3[1] = r.flux * (-(M.osx * pdu));
                 g3[2] = pv;

@@ -549,9 +549,11 @@ struct Sim {
         const double inv_norm_g64 = (KIND == KIND_GAUSS) ? 1.0 / (-(M_PI * P.fwhm * P.fwhm / CP_GAUSS_PRE)) : 0.0;
         const double moffat_amp64 = (KIND == KIND_MOFFAT) ? (P.beta - 1) / (M_PI * P.alpha * P.alpha) : 0.0;
         const size_t poly_words = single_epsf ? (size_t)M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
-        const bool ldsg = single_epsf && poly_words * 4 <= 44 * 1024;
+        // the LDS grid copy (edge-padded, polyphase) leaves room for two float
+        // render / fill blocks per CU (one for float64)
+        const bool ldsg = single_epsf && poly_words * 4 <= (sizeof(T) == 4 ? 56 : 96) * 1024;
         const size_t grid_sh = ldsg ? ((poly_words * 4 + 15) & ~(size_t)15) : 0;
-        const size_t render_sh = grid_sh + RCHUNK * 2 * sizeof(int);
+        const size_t render_sh = render_lds_bytes<T>(ldsg ? (int)poly_words : 0);
         const size_t fill_sh = grid_sh;
 
         // two 1024-thread render / fill blocks fit one CU (LDS and wave slots);
