@@ -514,15 +514,31 @@ __device__ __forceinline__ void tile_share(const long long* __restrict__ scan, i
 __constant__ const int RENDER_INV_H[17] = {0,    65536, 32768, 21846, 16384, 13108, 10923, 9363, 8192,
                                            7282, 6554,  5958,  5462,  5042,  4682,  4370,  4096};
 
+// One staged star of the render chunk: its record and, for the LDS ePSF
+// path, epsf_eval's stencil ready to use (tap offsets ra + ca, weights,
+// in-range window).  Read by every wave whose sub-block the star touches.
+template <typename T>
+struct alignas(16) RStar {
+    StarRec<T> r;
+    int a;                       // active index (corner records of multi-node models)
+    int ok, wlo0, whi0, wlo1, whi1;
+    T wy[4], wx[4];
+    int toff[16];
+};
+
 // Dynamic LDS of the render: the staged ePSF grid (ngrid floats), the staged
-// star ids and sub-block masks (RCHUNK each), then per wave a 16 x 16
-// accumulator of T and a 16 x 16 byte touch map.
+// sub-block masks (RCHUNK ints) and stars (RCHUNK RStar), then per wave a
+// 16 x 16 accumulator of T and a 16 x 16 byte touch map.
+__host__ __device__ __forceinline__ size_t render_star_offset(int ngrid) {
+    return (((size_t)ngrid * 4 + 15) & ~(size_t)15) + (size_t)RCHUNK * sizeof(int);
+}
+template <typename T>
 __host__ __device__ __forceinline__ size_t render_acc_offset(int ngrid) {
-    return (((size_t)ngrid * 4 + 15) & ~(size_t)15) + (size_t)RCHUNK * 2 * sizeof(int);
+    return render_star_offset(ngrid) + (size_t)RCHUNK * sizeof(RStar<T>);
 }
 template <typename T>
 __host__ __device__ __forceinline__ size_t render_lds_bytes(int ngrid) {
-    return render_acc_offset(ngrid) + (size_t)RWAVE * RSB * RSB * (sizeof(T) + 1);
+    return render_acc_offset<T>(ngrid) + (size_t)RWAVE * RSB * RSB * (sizeof(T) + 1);
 }
 
 // float renders fit 8 waves / SIMD: two 1024-thread blocks per CU
@@ -540,16 +556,16 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     static_assert(RCHUNK == 64, "one wave stages a chunk");
     const int ngrid = LDSG ? M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
-    int* sid = reinterpret_cast<int*>(smem + ((ngrid * 4 + 15) & ~15));
-    int* smask = sid + RCHUNK;
+    int* smask = reinterpret_cast<int*>(smem + ((ngrid * 4 + 15) & ~15));
+    RStar<T>* sst = reinterpret_cast<RStar<T>*>(smem + render_star_offset(ngrid));
     if (LDSG) stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, RBLOCK);
     const float* grid = LDSG ? sgrid : M.grids;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int sbr = (wv % (RT / RSB)) * RSB, sbc = (wv / (RT / RSB)) * RSB;
     // this wave's sub-block accumulator (column-major 16 x 16) and, for the
     // freeze render, which of its pixels a star touched
-    T* sacc = reinterpret_cast<T*>(smem + render_acc_offset(ngrid)) + wv * (RSB * RSB);
-    uint8_t* stouch = smem + render_acc_offset(ngrid) + RWAVE * RSB * RSB * sizeof(T) + wv * (RSB * RSB);
+    T* sacc = reinterpret_cast<T*>(smem + render_acc_offset<T>(ngrid)) + wv * (RSB * RSB);
+    uint8_t* stouch = smem + render_acc_offset<T>(ngrid) + RWAVE * RSB * RSB * sizeof(T) + wv * (RSB * RSB);
     double acc = 0;
     int t0, t1;
     tile_share(tscan, ntiles, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, t0, t1);
@@ -577,9 +593,10 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
             if (threadIdx.x < RCHUNK) {
                 const int i = threadIdx.x;
                 int a = 0, mask = 0;
+                StarRec<T> r{};
                 if (c0 + i < s1) {
                     a = list[c0 + i];
-                    const StarRec<T> r = recs[a];
+                    r = recs[a];
                     const bool use =
                         MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (r.flags & 2) != 0 : (r.flags & 1) != 0);
                     if (use) {
@@ -600,18 +617,41 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                 const unsigned long long m = __ballot(mask != 0);
                 const int slot = __popcll(m & ((1ull << i) - 1ull));
                 if (mask != 0) {
-                    sid[slot] = a;
                     smask[slot] = mask;
+                    RStar<T>& st = sst[slot];
+                    st.r = r;
+                    st.a = a;
+                    if constexpr (EPSF_LDS) {
+                        const EpsfRec<T>& er = ers[a];
+                        st.ok = er.ok;
+                        st.wlo0 = er.dylo;
+                        st.whi0 = er.dyhi;
+                        st.wlo1 = er.dxlo;
+                        st.whi1 = er.dxhi;
+#pragma unroll
+                        for (int ta = 0; ta < 4; ++ta) {
+                            st.wy[ta] = er.wy[ta];
+                            st.wx[ta] = er.wx[ta];
+#pragma unroll
+                            for (int tb = 0; tb < 4; ++tb) st.toff[ta * 4 + tb] = er.ra[ta] + er.ca[tb];
+                        }
+                    }
                 }
                 if (i == 0) s_nuse = __popcll(m);
             }
             __syncthreads();
             const int cn = s_nuse;
-            for (int j = 0; j < cn; ++j) {
-                const int mj = __builtin_amdgcn_readfirstlane(smask[j]);
-                if (!((mj >> wv) & 1)) continue;   // one LDS read per star
-                const int a = __builtin_amdgcn_readfirstlane(sid[j]);
-                const StarRec<T> r = recs[a];
+            // the chunk's stars touching this wave's sub-block, in list order
+            unsigned long long mine = __ballot(lane < cn && ((smask[lane] >> wv) & 1));
+            while (mine) {
+                const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(mine));
+                mine &= mine - 1;
+                const RStar<T>& st = sst[j];
+                StarRec<T> r = st.r;
+                r.ay = __builtin_amdgcn_readfirstlane(r.ay);
+                r.ax = __builtin_amdgcn_readfirstlane(r.ax);
+                r.mR = __builtin_amdgcn_readfirstlane(r.mR);
+                const int a = __builtin_amdgcn_readfirstlane(st.a);
                 // the model box clipped to the sub-block, in local rows / columns;
                 // its h x wd pixels are spread over the lanes (pixel p: column
                 // p / h, row p % h), so no lane idles outside the box
@@ -625,19 +665,18 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                 bool sok = false;
                 T wyv[4], wxv[4];
                 if constexpr (EPSF_LDS) {
-                    const EpsfRec<T>& er = ers[a];
-                    sok = er.ok;
-                    wlo[0] = er.dylo;
-                    whi[0] = er.dyhi;
-                    wlo[1] = er.dxlo;
-                    whi[1] = er.dxhi;
+                    sok = __builtin_amdgcn_readfirstlane(st.ok) != 0;
+                    wlo[0] = __builtin_amdgcn_readfirstlane(st.wlo0);
+                    whi[0] = __builtin_amdgcn_readfirstlane(st.whi0);
+                    wlo[1] = __builtin_amdgcn_readfirstlane(st.wlo1);
+                    whi[1] = __builtin_amdgcn_readfirstlane(st.whi1);
 #pragma unroll
                     for (int ta = 0; ta < 4; ++ta) {
-                        wyv[ta] = er.wy[ta];
-                        wxv[ta] = er.wx[ta];
-#pragma unroll
-                        for (int tb = 0; tb < 4; ++tb) toff[ta * 4 + tb] = er.ra[ta] + er.ca[tb];
+                        wyv[ta] = st.wy[ta];
+                        wxv[ta] = st.wx[ta];
                     }
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) toff[t] = __builtin_amdgcn_readfirstlane(st.toff[t]);
                 }
                 for (int base = 0; base < n; base += 64) {
                     const int pi = base + lane;

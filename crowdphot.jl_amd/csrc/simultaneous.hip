@@ -554,6 +554,7 @@ struct Sim {
         const bool ldsg = single_epsf && poly_words * 4 <= (sizeof(T) == 4 ? 56 : 96) * 1024;
         const size_t grid_sh = ldsg ? ((poly_words * 4 + 15) & ~(size_t)15) : 0;
         const size_t render_sh = render_lds_bytes<T>(ldsg ? (int)poly_words : 0);
+        CP_ARG(render_sh <= 160 * 1024, "render LDS footprint above 160 KiB");
         const size_t fill_sh = grid_sh;
 
         // two 1024-thread render / fill blocks fit one CU (LDS and wave slots);
