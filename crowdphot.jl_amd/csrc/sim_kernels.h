@@ -470,7 +470,6 @@ static __global__ void k_star_rec(int na, const double* __restrict__ theta, Free
 // (_render_model! :545-673, _residual_cost! :675-684, _cost! :686-693,
 //  _freeze_star! :739-754, final render :1344-1356)
 // ---------------------------------------------------------------------------
-constexpr int RCHUNK = 64;  // star records staged per LDS chunk in the render
 constexpr int RT = 64;      // render tile side (rows = columns)
 constexpr int RSB = 16;     // sub-block side: one 16 x 16 sub-block per wave, 4 pixels per lane
 static_assert((RT / RSB) * (RT / RSB) == RWAVE, "one sub-block per wave");
@@ -514,27 +513,11 @@ __device__ __forceinline__ void tile_share(const long long* __restrict__ scan, i
 __constant__ const int RENDER_INV_H[17] = {0,    65536, 32768, 21846, 16384, 13108, 10923, 9363, 8192,
                                            7282, 6554,  5958,  5462,  5042,  4682,  4370,  4096};
 
-// One staged star of the render chunk: its record and, for the LDS ePSF
-// path, epsf_eval's stencil ready to use (tap offsets ra + ca, weights,
-// in-range window).  Read by every wave whose sub-block the star touches.
-template <typename T>
-struct alignas(16) RStar {
-    StarRec<T> r;
-    int a;                       // active index (corner records of multi-node models)
-    int ok, wlo0, whi0, wlo1, whi1;
-    T wy[4], wx[4];
-    int toff[16];
-};
-
-// Dynamic LDS of the render: the staged ePSF grid (ngrid floats), the staged
-// sub-block masks (RCHUNK ints) and stars (RCHUNK RStar), then per wave a
-// 16 x 16 accumulator of T and a 16 x 16 byte touch map.
-__host__ __device__ __forceinline__ size_t render_star_offset(int ngrid) {
-    return (((size_t)ngrid * 4 + 15) & ~(size_t)15) + (size_t)RCHUNK * sizeof(int);
-}
+// Dynamic LDS of the render: the staged ePSF grid (ngrid floats), then per
+// wave a 16 x 16 accumulator of T and a 16 x 16 byte touch map.
 template <typename T>
 __host__ __device__ __forceinline__ size_t render_acc_offset(int ngrid) {
-    return render_star_offset(ngrid) + (size_t)RCHUNK * sizeof(RStar<T>);
+    return ((size_t)ngrid * 4 + 15) & ~(size_t)15;
 }
 template <typename T>
 __host__ __device__ __forceinline__ size_t render_lds_bytes(int ngrid) {
@@ -552,13 +535,12 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     constexpr bool EPSF_LDS = KIND == KIND_EPSF && LDSG;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[RWAVE];
-    __shared__ int s_nuse;
-    static_assert(RCHUNK == 64, "one wave stages a chunk");
     const int ngrid = LDSG ? M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
-    int* smask = reinterpret_cast<int*>(smem + ((ngrid * 4 + 15) & ~15));
-    RStar<T>* sst = reinterpret_cast<RStar<T>*>(smem + render_star_offset(ngrid));
-    if (LDSG) stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, RBLOCK);
+    if (LDSG) {
+        stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, RBLOCK);
+        __syncthreads();   // the only block barrier: every wave reads the whole grid
+    }
     const float* grid = LDSG ? sgrid : M.grids;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int sbr = (wv % (RT / RSB)) * RSB, sbc = (wv / (RT / RSB)) * RSB;
@@ -588,95 +570,51 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
             if (MODE == MODE_FREEZE) stouch[idx] = 0;
         }
         const int s0 = tstart[tile], s1 = tend[tile];
-        for (int c0 = s0; c0 < s1; c0 += RCHUNK) {
-            __syncthreads();
-            if (threadIdx.x < RCHUNK) {
-                const int i = threadIdx.x;
-                int a = 0, mask = 0;
-                StarRec<T> r{};
-                if (c0 + i < s1) {
-                    a = list[c0 + i];
-                    r = recs[a];
-                    const bool use =
-                        MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (r.flags & 2) != 0 : (r.flags & 1) != 0);
-                    if (use) {
-                        // sub-blocks (sr, sc) touched by the model box, bit sc * 4 + sr
-#pragma unroll
-                        for (int sc = 0; sc < RT / RSB; ++sc) {
-                            const int c_lo = tx0 + sc * RSB;
-                            const bool cov = r.ax - r.mR <= c_lo + RSB - 1 && r.ax + r.mR >= c_lo;
-#pragma unroll
-                            for (int sr = 0; sr < RT / RSB; ++sr) {
-                                const int r_lo = ty0 + sr * RSB;
-                                const bool rov = r.ay - r.mR <= r_lo + RSB - 1 && r.ay + r.mR >= r_lo;
-                                if (cov && rov) mask |= 1 << (sc * (RT / RSB) + sr);
-                            }
-                        }
-                    }
-                }
-                const unsigned long long m = __ballot(mask != 0);
-                const int slot = __popcll(m & ((1ull << i) - 1ull));
-                if (mask != 0) {
-                    smask[slot] = mask;
-                    RStar<T>& st = sst[slot];
-                    st.r = r;
-                    st.a = a;
-                    if constexpr (EPSF_LDS) {
-                        const EpsfRec<T>& er = ers[a];
-                        st.ok = er.ok;
-                        st.wlo0 = er.dylo;
-                        st.whi0 = er.dyhi;
-                        st.wlo1 = er.dxlo;
-                        st.whi1 = er.dxhi;
-#pragma unroll
-                        for (int ta = 0; ta < 4; ++ta) {
-                            st.wy[ta] = er.wy[ta];
-                            st.wx[ta] = er.wx[ta];
-#pragma unroll
-                            for (int tb = 0; tb < 4; ++tb) st.toff[ta * 4 + tb] = er.ra[ta] + er.ca[tb];
-                        }
-                    }
-                }
-                if (i == 0) s_nuse = __popcll(m);
+        for (int c0 = s0; c0 < s1; c0 += 64) {
+            // each lane inspects one list entry; the wave then takes the stars
+            // touching its sub-block in list order.  No block barrier: the 16
+            // waves of a tile run independently.
+            int ai = 0;
+            bool touch = false;
+            if (c0 + lane < s1) {
+                ai = list[c0 + lane];
+                const StarRec<T> ri = recs[ai];
+                const bool use =
+                    MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (ri.flags & 2) != 0 : (ri.flags & 1) != 0);
+                touch = use && ri.ay - ri.mR <= gy0 + RSB - 1 && ri.ay + ri.mR >= gy0 &&
+                        ri.ax - ri.mR <= gx0 + RSB - 1 && ri.ax + ri.mR >= gx0;
             }
-            __syncthreads();
-            const int cn = s_nuse;
-            // the chunk's stars touching this wave's sub-block, in list order
-            unsigned long long mine = __ballot(lane < cn && ((smask[lane] >> wv) & 1));
+            unsigned long long mine = __ballot(touch);
             while (mine) {
                 const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(mine));
                 mine &= mine - 1;
-                const RStar<T>& st = sst[j];
-                StarRec<T> r = st.r;
-                r.ay = __builtin_amdgcn_readfirstlane(r.ay);
-                r.ax = __builtin_amdgcn_readfirstlane(r.ax);
-                r.mR = __builtin_amdgcn_readfirstlane(r.mR);
-                const int a = __builtin_amdgcn_readfirstlane(st.a);
+                const int a = __builtin_amdgcn_readlane(ai, j);
+                const StarRec<T> r = recs[a];
                 // the model box clipped to the sub-block, in local rows / columns;
                 // its h x wd pixels are spread over the lanes (pixel p: column
                 // p / h, row p % h), so no lane idles outside the box
                 const int r_lo = max(r.ay - r.mR - gy0, 0), r_hi = min(r.ay + r.mR - gy0, RSB - 1);
                 const int c_lo = max(r.ax - r.mR - gx0, 0), c_hi = min(r.ax + r.mR - gx0, RSB - 1);
                 const int h = r_hi - r_lo + 1, n = h * (c_hi - c_lo + 1);
-                if (h <= 0 || c_hi < c_lo) continue;
                 const int inv_h = RENDER_INV_H[h];   // p / h == (p inv_h) >> 16 for p < 256
                 // this star's ePSF stencil (epsf_eval with the record in registers)
                 int toff[16], wlo[2], whi[2];
                 bool sok = false;
                 T wyv[4], wxv[4];
                 if constexpr (EPSF_LDS) {
-                    sok = __builtin_amdgcn_readfirstlane(st.ok) != 0;
-                    wlo[0] = __builtin_amdgcn_readfirstlane(st.wlo0);
-                    whi[0] = __builtin_amdgcn_readfirstlane(st.whi0);
-                    wlo[1] = __builtin_amdgcn_readfirstlane(st.wlo1);
-                    whi[1] = __builtin_amdgcn_readfirstlane(st.whi1);
+                    const EpsfRec<T>& er = ers[a];
+                    sok = er.ok;
+                    wlo[0] = er.dylo;
+                    whi[0] = er.dyhi;
+                    wlo[1] = er.dxlo;
+                    whi[1] = er.dxhi;
 #pragma unroll
                     for (int ta = 0; ta < 4; ++ta) {
-                        wyv[ta] = st.wy[ta];
-                        wxv[ta] = st.wx[ta];
-                    }
+                        wyv[ta] = er.wy[ta];
+                        wxv[ta] = er.wx[ta];
 #pragma unroll
-                    for (int t = 0; t < 16; ++t) toff[t] = __builtin_amdgcn_readfirstlane(st.toff[t]);
+                        for (int tb = 0; tb < 4; ++tb) toff[ta * 4 + tb] = er.ra[ta] + er.ca[tb];
+                    }
                 }
                 for (int base = 0; base < n; base += 64) {
                     const int pi = base + lane;
@@ -693,13 +631,16 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                             float gv[16];
 #pragma unroll
                             for (int t = 0; t < 16; ++t) gv[t] = sp[toff[t]];
+                            // schedule: the 16 LDS reads back to back, then the arithmetic
+                            __builtin_amdgcn_sched_group_barrier(0x0100, 16, 0);
+                            __builtin_amdgcn_sched_group_barrier(0x0002, 40, 0);
                             T rowv[4];
 #pragma unroll
                             for (int ta = 0; ta < 4; ++ta)
                                 rowv[ta] = wxv[0] * (T)gv[ta * 4] + wxv[1] * (T)gv[ta * 4 + 1] + wxv[2] * (T)gv[ta * 4 + 2] +
                                            wxv[3] * (T)gv[ta * 4 + 3];
-                            const T pv = inb ? wyv[0] * rowv[0] + wyv[1] * rowv[1] + wyv[2] * rowv[2] + wyv[3] * rowv[3]
-                                             : M.fill;
+                            const T pv4 = wyv[0] * rowv[0] + wyv[1] * rowv[1] + wyv[2] * rowv[2] + wyv[3] * rowv[3];
+                            const T pv = inb ? pv4 : M.fill;
                             v = fma(r.flux, pv, M.bkg);
                         } else {
                             v = star_value<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a);
