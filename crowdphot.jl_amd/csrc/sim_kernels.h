@@ -73,7 +73,7 @@ struct Scal {
     // other union pixels are zero rows of J, where u only scales: their
     // share of |u|^2 is fscale^2 * sf
     double sf, fscale;
-    int n_jv_pix, pad1;
+    int n_jv_pix, n_jtu_stars;
     // CGLS
     double gamma, cg_alpha, cg_beta, pnorm2, qnorm2, s0norm;
     unsigned int counters[8];
@@ -695,10 +695,10 @@ __device__ __forceinline__ double group_sum(double v) {
 template <typename T, int KIND, bool FINAL, bool LDSG, int EPL, int GL>
 __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_fill(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
-    const CornerRec<T>* __restrict__ crs, int na, int R, int S, FreeMap fm, const T* __restrict__ w, int ny,
-    int nx, const T* __restrict__ wt_resid, T* __restrict__ V, double* __restrict__ colnorm,
-    double* __restrict__ b_scaled, uint8_t* __restrict__ gloc, double g_tol, double* __restrict__ partials,
-    Scal* Sc) {
+    const CornerRec<T>* __restrict__ crs, const int* __restrict__ lst, int na, int R, int S, FreeMap fm,
+    const T* __restrict__ w, int ny, int nx, const T* __restrict__ wt_resid, T* __restrict__ V,
+    double* __restrict__ colnorm, double* __restrict__ b_scaled, uint8_t* __restrict__ gloc, double g_tol,
+    double* __restrict__ partials, Scal* Sc) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[FWAVE];
     float* sgrid = reinterpret_cast<float*>(smem);
@@ -713,7 +713,11 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
     const int p = fm.p;
     const double C_r = FINAL ? 0.0 : Sc->cost / Sc->dof;
     double bsq = 0, gmax = 0;
-    for (int a = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL; a < na; a += gridDim.x * GPB) {
+    // stars: the list lst (Sc->n_jtu_stars entries: the live stars of the last
+    // compaction; stars outside it are frozen and keep their columns), or all
+    const int cnt = lst ? Sc->n_jtu_stars : na;
+    for (int li = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL; li < cnt; li += gridDim.x * GPB) {
+        const int a = lst ? lst[li] : li;
         const StarRec<T> r = recs[a];
         if (!FINAL && !(r.flags & 1)) {   // uniform within the group
             if (sub < VP) b_scaled[(size_t)a * VP + sub] = 0.0;
@@ -990,11 +994,13 @@ static __global__ void k_pixel_compact(int U, const int* __restrict__ pix, const
 }
 
 // v <- J^T u - beta v per star, with the pending x/w update of the previous
-// iteration folded in; the last block advances the LSQR recurrences.  GL
-// lanes per star (GL = 16 for stamps up to 16 x 16, else 64).
+// iteration folded in.  GL lanes per star (GL = 16 for stamps up to 16 x 16,
+// else 64), grid-stride over the star list lst (Sc->n_jtu_stars entries: the
+// stars live when the J v structure was last compacted, the others are
+// neither read by J v nor moved by the step).
 template <typename T, int EPL, int GL>
 __global__ __launch_bounds__(BLOCK) void k_JTu(
-    int na, const int* __restrict__ ay, const int* __restrict__ ax, const uint8_t* __restrict__ live,
+    const int* __restrict__ lst, const int* __restrict__ ay, const int* __restrict__ ax, const uint8_t* __restrict__ live,
     int R, int S, int p, int ny, int nx, const T* __restrict__ V, const T* __restrict__ ubuf,
     const double* __restrict__ b_scaled, double* __restrict__ tbuf, double* __restrict__ x,
     double* __restrict__ wv, int first, int it, double* __restrict__ partials, double* __restrict__ partials2,
@@ -1010,8 +1016,9 @@ __global__ __launch_bounds__(BLOCK) void k_JTu(
     const int sub = threadIdx.x % GL;
     const int S2 = S * S;
     double tsq = 0, wsq = 0;
-    const int a = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL;
-    if (a < na) {
+    const int nl = Sc->n_jtu_stars;
+    for (int li = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL; li < nl; li += gridDim.x * GPB) {
+        const int a = lst[li];
         const size_t base = (size_t)a * p;
         if (!live[a]) {
             if (first && sub < p) {
@@ -1431,6 +1438,29 @@ enum OutCol {
     OUT_Y, OUT_X, OUT_FLUX, OUT_BKG, OUT_Y_ERR, OUT_X_ERR, OUT_FLUX_ERR, OUT_BKG_ERR, OUT_CHISQ,
     OUT_QFIT, OUT_QFIT_EXP, OUT_QFIT_Z, OUT_CROWDING, OUT_SPREAD, OUT_SPREAD_ERR, OUT_F64
 };
+
+// Live compaction of the model-box tile lists: flag[i] = live[list[i]] for
+// i < n, flag[n] = 0.
+static __global__ void k_list_live(int n, const int* __restrict__ list, const uint8_t* __restrict__ live,
+                                   int* __restrict__ flag) {
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i <= n; i += gridDim.x * BLOCK)
+        flag[i] = i < n ? (int)live[list[i]] : 0;
+}
+
+// list_l[pos[i]] = list[i] for flagged entries (pos = exclusive scan of flag)
+// and the tiles' ranges in the compacted list.
+static __global__ void k_list_compact(int n, const int* __restrict__ list, const int* __restrict__ flag,
+                                      const int* __restrict__ pos, int* __restrict__ list_l, int ntiles,
+                                      const int* __restrict__ start, const int* __restrict__ end,
+                                      int* __restrict__ start_l, int* __restrict__ end_l) {
+    const int stride = gridDim.x * BLOCK;
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride)
+        if (flag[i]) list_l[pos[i]] = list[i];
+    for (int t = blockIdx.x * BLOCK + threadIdx.x; t < ntiles; t += stride) {
+        start_l[t] = pos[start[t]];
+        end_l[t] = pos[end[t]];
+    }
+}
 
 // slot[act[j]] = j: the active index of each catalogue star (slot is -1
 // beforehand for every star).

@@ -284,12 +284,12 @@ struct Sim {
 
     template <bool FINAL, bool LDSG>
     void fill_launch(int S2, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
-                     const EpsfRec<T>* ers, const CornerRec<T>* crs, int na, int R, int S, const FreeMap& fm,
-                     const T* w, int ny, int nx, const T* wres, T* V, double* cn, double* b, uint8_t* gloc,
-                     double gtol, double* part, Scal* Sc) {
+                     const EpsfRec<T>* ers, const CornerRec<T>* crs, const int* lst, int na, int R, int S,
+                     const FreeMap& fm, const T* w, int ny, int nx, const T* wres, T* V, double* cn, double* b,
+                     uint8_t* gloc, double gtol, double* part, Scal* Sc) {
         const char* nm = FINAL ? "k_fill_final" : "k_fill";
 #define CP_FILL(E, G)                                                                                        \
-    launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G>, grid, dim3(FBLOCK), sh, st, M, recs, ers, crs, na, R, S, fm, \
+    launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G>, grid, dim3(FBLOCK), sh, st, M, recs, ers, crs, lst, na, R, S, fm, \
              w, ny, nx, wres, V, cn, b, gloc, gtol, part, Sc)
         if (S2 <= 64) CP_FILL(1, 64);
         else if (S2 <= 128) CP_FILL(2, 64);
@@ -299,11 +299,11 @@ struct Sim {
 #undef CP_FILL
     }
 
-    void jtu_launch(int S2, dim3 grid, int na, const int* ay, const int* ax, const uint8_t* live, int R, int S, int p,
+    void jtu_launch(int S2, dim3 grid, const int* lst, const int* ay, const int* ax, const uint8_t* live, int R, int S, int p,
                     int ny, int nx, const T* V, const T* u, const double* b, double* t, double* x, double* wv,
                     int first, int it, double* part, double* part2, Scal* Sc) {
 #define CP_JTU(E, G)                                                                                          \
-    launch_n("k_JTu", k_JTu<T, E, G>, grid, dim3(BLOCK), 0, st, na, ay, ax, live, R, S, p, ny, nx, V, u, b, t, x, wv, \
+    launch_n("k_JTu", k_JTu<T, E, G>, grid, dim3(BLOCK), 0, st, lst, ay, ax, live, R, S, p, ny, nx, V, u, b, t, x, wv, \
              first, it, part, part2, Sc)
         if (S2 <= 16) CP_JTU(1, 16);
         else if (S2 <= 32) CP_JTU(2, 16);
@@ -563,7 +563,7 @@ struct Sim {
         const int gb_jv = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * 8));
         const int GLs = S2 <= 256 ? 16 : 64;
         const int gb_fill = (int)std::min<int64_t>(cdiv(na, FBLOCK / 64), (int64_t)n_cu * 2);
-        const int gb_jtu = (int)cdiv(na, BLOCK / GLs);
+        const int gb_jtu = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(na, BLOCK / GLs), (int64_t)n_cu * 8));
         const unsigned int gstar = cdiv(na, BLOCK);
         // per-block partials of the widest launch (two sets for fill / accept)
         const int maxpb = (int)std::max<int64_t>(
@@ -576,20 +576,30 @@ struct Sim {
                    (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), frozen_iter, all_live,
                    r.as<StarRec<T>>(), DP.G, c.as<CornerRec<T>>(), single_epsf ? e.as<EpsfRec<T>>() : nullptr);
         };
+        // live stars of the last compaction (fill and J^T u walk them) and the
+        // model-box tile lists restricted to them (cost and freeze renders)
+        DevBuf jlist(ctx, (size_t)na * 4);
+        TileLists Llive;
+        Llive.npairs = Lmod.npairs;
+        Llive.start.alloc(ctx, (size_t)ntiles * 4);
+        Llive.end.alloc(ctx, (size_t)ntiles * 4);
+        Llive.list.alloc(ctx, (size_t)std::max(Lmod.npairs, 1) * 4);
+        DevBuf tflag(ctx, (size_t)(Lmod.npairs + 1) * 4), tpos(ctx, (size_t)(Lmod.npairs + 1) * 4);
+        DevBuf tcost_l(ctx, (size_t)ntiles * 8), tscan_l(ctx, (size_t)ntiles * 8);
         auto fill = [&](bool final_mode) {
             const StarRec<T>* rp = recs.as<StarRec<T>>();
             const EpsfRec<T>* ep = ers.as<EpsfRec<T>>();
             const CornerRec<T>* cp = crs.as<CornerRec<T>>();
             if (ldsg) {
                 if (final_mode)
-                    fill_launch<true, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<true, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, nullptr, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
                 else
-                    fill_launch<false, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<false, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, (const int*)jlist.as<int>(), na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             } else {
                 if (final_mode)
-                    fill_launch<true, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<true, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, nullptr, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
                 else
-                    fill_launch<false, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<false, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, (const int*)jlist.as<int>(), na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             }
         };
         auto finish = [&](auto kern, int nb, int it) {
@@ -608,6 +618,17 @@ struct Sim {
         DevBuf pflag(ctx, (size_t)(csr.U + 1) * 4), ppos(ctx, (size_t)(csr.U + 1) * 4);
         int live_compacted = -1;
         auto compact_jv = [&]() {
+            // the live stars, in active order, for J^T u (count into Sc)
+            {
+                hipcub::CountingInputIterator<int> cit(0);
+                size_t b = 0;
+                CP_HIP(hipcub::DeviceSelect::Flagged(nullptr, b, cit, (const uint8_t*)live.as<uint8_t>(), jlist.as<int>(),
+                                                     &Sc->n_jtu_stars, na, st));
+                cub.ensure(b);
+                b = cub.tmp.bytes();
+                CP_HIP(hipcub::DeviceSelect::Flagged(cub.tmp.get(), b, cit, (const uint8_t*)live.as<uint8_t>(),
+                                                     jlist.as<int>(), &Sc->n_jtu_stars, na, st));
+            }
             const int gbe = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)csr.E + 1, BLOCK), (int64_t)n_cu * 16));
             launch(k_entry_live, dim3(gbe), dim3(BLOCK), 0, st, csr.E, (const int*)csr.ent.as<int>(), S2,
                    (const uint8_t*)live.as<uint8_t>(), cflag.as<int>());
@@ -624,6 +645,31 @@ struct Sim {
             cub.ensure(b);
             b = cub.tmp.bytes();
             CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, pflag.as<int>(), ppos.as<int>(), csr.U + 1, st));
+            // tile lists of the live stars, and their cost scan
+            {
+                const int np = Lmod.npairs;
+                const int gbl = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)np + 1, BLOCK), (int64_t)n_cu * 16));
+                launch(k_list_live, dim3(gbl), dim3(BLOCK), 0, st, np, (const int*)Lmod.list.as<int>(),
+                       (const uint8_t*)live.as<uint8_t>(), tflag.as<int>());
+                size_t b2 = 0;
+                CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, tflag.as<int>(), tpos.as<int>(), np + 1, st));
+                cub.ensure(b2);
+                b2 = cub.tmp.bytes();
+                CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b2, tflag.as<int>(), tpos.as<int>(), np + 1, st));
+                launch(k_list_compact, dim3(gbl), dim3(BLOCK), 0, st, np, (const int*)Lmod.list.as<int>(),
+                       (const int*)tflag.as<int>(), (const int*)tpos.as<int>(), Llive.list.as<int>(), ntiles,
+                       (const int*)Lmod.start.as<int>(), (const int*)Lmod.end.as<int>(), Llive.start.as<int>(),
+                       Llive.end.as<int>());
+                launch(k_tile_cost, dim3(cdiv(ntiles, BLOCK)), dim3(BLOCK), 0, st, (const int*)Llive.start.as<int>(),
+                       (const int*)Llive.end.as<int>(), ntiles, tcost_l.as<long long>());
+                b2 = 0;
+                CP_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, b2, tcost_l.as<long long>(), tscan_l.as<long long>(),
+                                                        ntiles, st));
+                cub.ensure(b2);
+                b2 = cub.tmp.bytes();
+                CP_HIP(hipcub::DeviceScan::InclusiveSum(cub.tmp.get(), b2, tcost_l.as<long long>(),
+                                                        tscan_l.as<long long>(), ntiles, st));
+            }
             const int gbp = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * 16));
             launch(k_pixel_compact, dim3(gbp), dim3(BLOCK), 0, st, csr.U, (const int*)csr.pix.as<int>(),
                    (const int*)csr.ptr.as<int>(), (const int*)cpos.as<int>(), (const int*)pflag.as<int>(),
@@ -688,7 +734,7 @@ struct Sim {
                                  (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(), first, part.as<double>(),
                                  part2.as<double>(), Sc);
                         finish(k_finish<FIN_BETA>, gb_jv, it);
-                        jtu_launch(S2, dim3(gb_jtu), na, (const int*)ay.as<int>(), (const int*)ax.as<int>(),
+                        jtu_launch(S2, dim3(gb_jtu), (const int*)jlist.as<int>(), (const int*)ay.as<int>(), (const int*)ax.as<int>(),
                                    (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx, (const T*)V.as<T>(),
                                    (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(), tb.as<double>(),
                                    xs.as<double>(), wl.as<double>(), first, it, part.as<double>(), part2.as<double>(), Sc);
@@ -700,7 +746,7 @@ struct Sim {
                            (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
                     star_rec(theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
                     render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
-                                             ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Lmod, g, ntiles, (const long long*)tscan.as<long long>(),
+                                             ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Llive, g, ntiles, (const long long*)tscan_l.as<long long>(),
                                              umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cand,
                                              nullptr, part.as<double>(), Sc);
                     finish(k_finish<FIN_COST_CAND>, gb_render, 0);
@@ -756,8 +802,8 @@ struct Sim {
                 if (hS->n_frozen_now > 0) {
                     star_rec(theta.as<double>(), recs, crs, ers, iter, 0);
                     render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
-                                               ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles,
-                                               (const long long*)tscan.as<long long>(), umask.as<uint8_t>(),
+                                               ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Llive, g, ntiles,
+                                               (const long long*)tscan_l.as<long long>(), umask.as<uint8_t>(),
                                                dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
                                                part.as<double>(), Sc);
                 }
