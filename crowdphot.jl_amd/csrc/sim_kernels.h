@@ -907,13 +907,16 @@ static __global__ void k_csr_pix(const unsigned int* __restrict__ key, int* __re
 // (apply_J! :95-133 as a gather over the live-compacted structure; the other
 // union pixels are zero rows, accounted for in k_finish<FIN_BETA>).  Row r =
 // a S^2 + m of V belongs to star a = floor(r / S^2), computed as
-// floor(r * (1/S^2) + 0.5/S^2) in float64 (exact for r < 2^31).  The first
-// call also sums |b|^2 over the visited pixels (partials2).
+// floor(r * (1/S^2) + 0.5/S^2) in float64 (exact for r < 2^31).  u is kept in
+// the compacted pixel order (ucsr; the first call starts from the image-
+// layout residual b = wres) and copied to every stamp row that reads the
+// pixel (ustamp[r]), so J^T u streams it per star.  The first call also sums
+// |b|^2 over the visited pixels (partials2).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_Jv(const int* __restrict__ pix, const int* __restrict__ ptr,
                                               const int* __restrict__ ent, int S2, const T* __restrict__ V,
-                                              const double* __restrict__ tsrc,
-                                              const T* __restrict__ usrc, T* __restrict__ ubuf, int first,
+                                              const double* __restrict__ tsrc, const T* __restrict__ wres,
+                                              T* __restrict__ ucsr, T* __restrict__ ustamp, int first,
                                               double* __restrict__ partials, double* __restrict__ partials2,
                                               Scal* Sc) {
     if (Sc->done) return;
@@ -934,7 +937,6 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(const int* __restrict__ pix, const
     const double4* t4 = reinterpret_cast<const double4*>(tsrc);
     double acc2 = 0, accb = 0;
     for (int u = blockIdx.x * BLOCK + threadIdx.x; u < U; u += gridDim.x * BLOCK) {
-        const int q = pix[u];
         const int e0 = ptr[u], e1 = ptr[u + 1];
         double y = 0;
         for (int e = e0; e < e1; ++e) {
@@ -944,9 +946,10 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(const int* __restrict__ pix, const
             const double4 t = t4[a];
             y += (double)v.x * t.x + (double)v.y * t.y + (double)v.z * t.z;
         }
-        const double us = (double)usrc[q];
+        const double us = first ? (double)wres[pix[u]] : (double)ucsr[u];
         const T uu = (T)(y * tfac - alpha * (us * ufac));
-        ubuf[q] = uu;
+        ucsr[u] = uu;
+        for (int e = e0; e < e1; ++e) ustamp[ent[e]] = uu;
         acc2 += (double)uu * (double)uu;
         accb += us * us;
     }
@@ -1001,7 +1004,7 @@ static __global__ void k_pixel_compact(int U, const int* __restrict__ pix, const
 template <typename T, int EPL, int GL>
 __global__ __launch_bounds__(BLOCK) void k_JTu(
     const int* __restrict__ lst, const int* __restrict__ ay, const int* __restrict__ ax, const uint8_t* __restrict__ live,
-    int R, int S, int p, int ny, int nx, const T* __restrict__ V, const T* __restrict__ ubuf,
+    int R, int S, int p, int ny, int nx, const T* __restrict__ V, const T* __restrict__ ustamp,
     const double* __restrict__ b_scaled, double* __restrict__ tbuf, double* __restrict__ x,
     double* __restrict__ wv, int first, int it, double* __restrict__ partials, double* __restrict__ partials2,
     Scal* Sc) {
@@ -1027,15 +1030,14 @@ __global__ __launch_bounds__(BLOCK) void k_JTu(
                 tbuf[(size_t)a * VP + sub] = 0;
             }
         } else {
-            const int sy = ay[a], sx = ax[a];
             double acc0 = 0, acc1 = 0, acc2 = 0;
             T uv[EPL], v0[EPL], v1[EPL], v2[EPL];
 #pragma unroll
             for (int e = 0; e < EPL; ++e) {
+                // u at the star's stamp rows (zero where the row is off the
+                // image or masked: such rows have V = 0 and no J v entry)
                 const int m = sub + GL * e;
-                const int gy = sy + (m % S) - R, gx = sx + (m / S) - R;
-                const bool on = m < S2 && gy >= 1 && gy <= ny && gx >= 1 && gx <= nx;
-                uv[e] = on ? ubuf[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] : T(0);
+                uv[e] = m < S2 ? ustamp[(size_t)a * S2 + m] : T(0);
                 const vec4_t<T> vr = reinterpret_cast<const vec4_t<T>*>(V)[(size_t)a * S2 + (m < S2 ? m : 0)];
                 v0[e] = vr.x;
                 v1[e] = vr.y;

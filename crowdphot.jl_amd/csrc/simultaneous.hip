@@ -515,6 +515,9 @@ struct Sim {
         // --- working buffers
         // weighted residual of the current point and of the trial candidate
         DevBuf wres_a(ctx, Pn * sizeof(T)), wres_b(ctx, Pn * sizeof(T)), ubuf(ctx, Pn * sizeof(T));
+        // LSQR u in compacted pixel order and copied to the stamp rows
+        DevBuf ucsr(ctx, (size_t)std::max(csr.U, 1) * sizeof(T)), ustamp(ctx, (size_t)na * S2 * sizeof(T));
+        CP_HIP(hipMemsetAsync(ustamp.get(), 0, (size_t)na * S2 * sizeof(T), st));
         CP_HIP(hipMemsetAsync(wres_a.get(), 0, Pn * sizeof(T), st));
         CP_HIP(hipMemsetAsync(wres_b.get(), 0, Pn * sizeof(T), st));
         T* wres_cur = wres_a.as<T>();
@@ -731,12 +734,12 @@ struct Sim {
                         launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, (const int*)pix_l.as<int>(),
                                  (const int*)ptr_l.as<int>(), (const int*)ent_l.as<int>(), S2, (const T*)V.as<T>(),
                                  (const double*)(first ? bsc.as<double>() : tb.as<double>()),
-                                 (const T*)(first ? wres_cur : ubuf.as<T>()), ubuf.as<T>(), first, part.as<double>(),
+                                 (const T*)wres_cur, ucsr.as<T>(), ustamp.as<T>(), first, part.as<double>(),
                                  part2.as<double>(), Sc);
                         finish(k_finish<FIN_BETA>, gb_jv, it);
                         jtu_launch(S2, dim3(gb_jtu), (const int*)jlist.as<int>(), (const int*)ay.as<int>(), (const int*)ax.as<int>(),
                                    (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx, (const T*)V.as<T>(),
-                                   (const T*)ubuf.as<T>(), (const double*)bsc.as<double>(), tb.as<double>(),
+                                   (const T*)ustamp.as<T>(), (const double*)bsc.as<double>(), tb.as<double>(),
                                    xs.as<double>(), wl.as<double>(), first, it, part.as<double>(), part2.as<double>(), Sc);
                         finish(k_finish<FIN_LSQR>, gb_jtu, it);
                     }
