@@ -36,6 +36,7 @@ struct ModelConst {
     int ios_y, ios_x;      // integer oversampling factors
     int Qy, Qx;            // polyphase plane sides of the edge-padded grid:
                            // ceil((gny + 2 PAD) / os_y), ceil((gnx + 2 PAD) / os_x)
+    const float* poly;     // device: node 0 already in the padded polyphase LDS layout (or null)
 };
 
 // Edge padding of the LDS copy of an ePSF grid: PAD samples on every side
@@ -111,6 +112,16 @@ __device__ __forceinline__ void stage_poly(const float* __restrict__ g, int gny,
         const int rs = min(max(r - EPSF_PAD, 0), gny - 1), cs = min(max(c - EPSF_PAD, 0), gnx - 1);
         sg[poly_addr(r, c, osy, osx, Qy, Qx)] = g[rs + (size_t)cs * gny];
     }
+}
+
+// Copy the pre-staged grid (global, already in the LDS layout) into LDS.
+__device__ __forceinline__ void load_poly(const float* __restrict__ src, float* __restrict__ dst, int n, int tid,
+                                          int nthr) {
+    const int n4 = n >> 2;
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    for (int i = tid; i < n4; i += nthr) d4[i] = s4[i];
+    for (int i = (n4 << 2) + tid; i < n; i += nthr) dst[i] = src[i];
 }
 
 // floor(a / b) and ceil(a / b) for b > 0

@@ -538,7 +538,7 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     const int ngrid = LDSG ? M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
     if (LDSG) {
-        stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, RBLOCK);
+        load_poly(M.poly, sgrid, ngrid, threadIdx.x, RBLOCK);
         __syncthreads();   // the only block barrier: every wave reads the whole grid
     }
     const float* grid = LDSG ? sgrid : M.grids;
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
     __shared__ double sred[FWAVE];
     float* sgrid = reinterpret_cast<float*>(smem);
     if (LDSG) {
-        stage_poly(M.grids, M.gny, M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, sgrid, threadIdx.x, FBLOCK);
+        load_poly(M.poly, sgrid, M.ios_y * M.ios_x * M.Qy * M.Qx, threadIdx.x, FBLOCK);
         __syncthreads();
     }
     const float* grid = LDSG ? sgrid : M.grids;
@@ -1462,6 +1462,13 @@ static __global__ void k_list_compact(int n, const int* __restrict__ list, const
         start_l[t] = pos[start[t]];
         end_l[t] = pos[end[t]];
     }
+}
+
+// The single-node ePSF grid in the edge-padded polyphase layout the LDS
+// copies use (built once per fit; kernels copy it linearly).
+static __global__ void k_poly_grid(const float* __restrict__ g, int gny, int gnx, int osy, int osx, int Qy, int Qx,
+                                   float* __restrict__ out) {
+    stage_poly(g, gny, gnx, osy, osx, Qy, Qx, out, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // slot[act[j]] = j: the active index of each catalogue star (slot is -1

@@ -548,7 +548,8 @@ struct Sim {
             crs_c.alloc(ctx, sizeof(CornerRec<T>));
         }
 
-        const ModelConst<T> M = model_const<T>(P, DP, o.fixed_bkg);
+        ModelConst<T> M = model_const<T>(P, DP, o.fixed_bkg);
+        M.poly = nullptr;
         const double inv_norm_g64 = (KIND == KIND_GAUSS) ? 1.0 / (-(M_PI * P.fwhm * P.fwhm / CP_GAUSS_PRE)) : 0.0;
         const double moffat_amp64 = (KIND == KIND_MOFFAT) ? (P.beta - 1) / (M_PI * P.alpha * P.alpha) : 0.0;
         const size_t poly_words = single_epsf ? (size_t)M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
@@ -557,6 +558,14 @@ struct Sim {
         const bool ldsg = single_epsf && poly_words * 4 <= (sizeof(T) == 4 ? 56 : 96) * 1024;
         const size_t grid_sh = ldsg ? ((poly_words * 4 + 15) & ~(size_t)15) : 0;
         const size_t render_sh = render_lds_bytes<T>(ldsg ? (int)poly_words : 0);
+        DevBuf polyg;
+        if (ldsg) {
+            polyg.alloc(ctx, poly_words * 4);
+            CP_HIP(hipMemsetAsync(polyg.get(), 0, poly_words * 4, st));
+            launch(k_poly_grid, dim3(cdiv((int64_t)poly_words, BLOCK)), dim3(BLOCK), 0, st, (const float*)M.grids, M.gny,
+                   M.gnx, M.ios_y, M.ios_x, M.Qy, M.Qx, polyg.as<float>());
+            M.poly = polyg.as<float>();
+        }
         CP_ARG(render_sh <= 160 * 1024, "render LDS footprint above 160 KiB");
         const size_t fill_sh = grid_sh;
 
