@@ -250,6 +250,9 @@ SIGNATURES = {
                                              C.POINTER(SimResult)]),
     "cp_build_stamps": (C.c_int, [C.c_void_p, f32p, C.c_int32, C.c_int32, f64p, f64p, f64p, C.c_int32,
                                   C.c_double, i32p, i32p, i32p, i32p, i32p, u8p]),
+    "cp_fit_all_stars": (C.c_int, [C.c_void_p, C.POINTER(PsfDesc), f32p, f32p, C.c_int32, C.c_int32, f64p,
+                                   C.c_int32, C.c_double, C.c_int32, C.c_uint32, f64p, C.POINTER(LMOpts),
+                                   C.c_double, C.c_int32, C.POINTER(SimResult)]),
     "cp_fit_star_batch": (C.c_int, [C.c_void_p, C.POINTER(PsfDesc), f32p, i64p, i32p, i32p, f32p, f64p,
                                     C.c_uint32, C.c_int32, C.POINTER(LMOpts), f64p, f64p,
                                     C.POINTER(LMStat)]),

@@ -1,6 +1,7 @@
 """Host mirror of CrowdPhot.jl's fitting entry points over libcrowdphot_hip.so.
 
   fit_all_stars_simultaneous   src/photometry/psf_photometry_simultaneous.jl:953-1431
+  fit_all_stars                src/photometry/psf_photometry_single.jl:267-469
   MultiPassPhotResult          src/photometry/psf_photometry_single.jl:99-122
   fit_star / LMResult          src/psf/psf_fitting.jl:229-308, src/levenberg_marquardt.jl:28-41
   fit_star_batch               many fit_star calls as one batched GPU launch
@@ -298,6 +299,92 @@ def fit_all_stars_simultaneous(image, psf, sources, fit_rad, *, ctx=None, **kw):
     call = _SimCall(image, psf, sources, fit_rad, **kw)
     call.run(ctx or default_context())
     return call.finish()
+
+
+def fit_all_stars(image, psf, sources, fit_rad, *, n_passes=3, fixed=None, inv_var=None, spread_model_fwhm=None,
+                  precision="mixed", residual=True, ctx=None, **kw):
+    """Sequential multi-pass PSF photometry on the GPU (psf_photometry_single.jl:225-469):
+    stars brightest first, each fitted with fit_star on the progressively
+    subtracted residual over its +-fit_rad box, n_passes times.  Keyword
+    arguments as the reference's (fixed, inv_var, spread_model_fwhm and
+    fit_star's lm_irls keywords: lambda_init, ..., max_iter, x_tol, f_tol,
+    g_tol, damping, covariance_estimator).  ``precision`` ("mixed": float32
+    model and gradient with float64 sums and residual image; "fp64")."""
+    if n_passes <= 0:
+        raise ArgumentError(f"n_passes must be positive, got {n_passes}")
+    if not isinstance(psf, (CircularGaussianPSF, CircularMoffatPSF)):
+        raise ArgumentError("fit_all_stars on the GPU supports CircularGaussianPSF and CircularMoffatPSF")
+    if not fit_rad > 0:
+        raise ArgumentError("fit_rad must be positive")
+    fixed = dict(fixed or {})
+    img = np.asarray(image)
+    if img.ndim != 2:
+        raise ArgumentError("image must be a 2-D array")
+    if inv_var is not None and np.shape(inv_var) != img.shape:
+        raise ArgumentError("`inv_var` must be the same size as `image`")
+    ny, nx = img.shape
+    names = psf.props
+    for k in fixed:
+        if k not in names:
+            raise ArgumentError(f"{type(psf).__name__} has no parameter {k!r}")
+    mask = 0
+    for k, nme in enumerate(names):
+        if nme not in fixed:
+            mask |= 1 << k
+    if mask == 0:
+        raise ArgumentError("all model parameters are fixed; nothing to fit")
+    fixv = np.array([float(fixed[k]) if k in fixed else np.nan for k in names])
+    # _extract_source_catalog / _build_params_matrix (:152-197): y, x, flux,
+    # bkg from the catalogue, shape parameters from the template
+    y, x, flux = _catalog(sources)
+    get = (lambda k: sources[k]) if isinstance(sources, dict) else (lambda k: getattr(sources, k))
+    has = (lambda k: k in sources) if isinstance(sources, dict) else (lambda k: hasattr(sources, k))
+    bkg = np.ascontiguousarray(get("bkg"), np.float64) if has("bkg") else np.zeros_like(y)
+    n = len(y)
+    th0 = np.tile(psf.params(), (max(n, 1), 1))[:n].copy()
+    cols = {"y": y, "x": x, "flux": flux, "bkg": bkg}
+    for k, nme in enumerate(names):
+        if nme in cols:
+            th0[:, k] = cols[nme]
+    th0 = np.ascontiguousarray(th0)
+    out = {k: np.zeros(n) for k in ("y", "x", "flux", "bkg", "y_err", "x_err", "flux_err", "bkg_err", "chisq",
+                                     "qfit", "qfit_expected", "qfit_z", "crowding", "spread_model",
+                                     "spread_model_err")}
+    conv = np.zeros(n, np.uint8)
+    valid = np.zeros(n, np.uint8)
+    nit = np.zeros(n, np.int32)
+    res = np.zeros(ny * nx, np.float32) if residual else None
+    r = _abi.SimResult()
+    for k, v in out.items():
+        setattr(r, k, _p(v, _abi.f64p))
+    r.converged = _p(conv, _abi.u8p)
+    r.valid = _p(valid, _abi.u8p)
+    r.n_iter = _p(nit, _abi.i32p)
+    r.residual = _p(res, _abi.f32p)
+    o = _lm_opts(kw)
+    im = _colmajor32(img)
+    iv = None if inv_var is None else _colmajor32(inv_var)
+    prec = {"mixed": _abi.CP_PREC_MIXED, "fp64": _abi.CP_PREC_FP64}[precision]
+    ctx = ctx or default_context()
+    rc = ctx.lib.cp_fit_all_stars(ctx.handle, C.byref(psf.desc()), _p(im, _abi.f32p), _p(iv, _abi.f32p), ny, nx,
+                                  _p(th0, _abi.f64p), n, float(fit_rad), int(n_passes), mask, _p(fixv, _abi.f64p),
+                                  C.byref(o), 0.0 if spread_model_fwhm is None else float(spread_model_fwhm), prec,
+                                  C.byref(r))
+    ctx.check(rc)
+    if n == 0:
+        resid = np.zeros((0, 0))
+    else:
+        resid = None if res is None else res.reshape(nx, ny).T
+    msgs = []
+    if r.n_failed:
+        msgs = [f"{r.n_failed} / {n} star fits threw exceptions (too few valid inv_var pixels) and were marked "
+                "invalid"]
+    return MultiPassPhotResult(
+        y=out["y"], x=out["x"], y_err=out["y_err"], x_err=out["x_err"], flux=out["flux"], flux_err=out["flux_err"],
+        bkg=out["bkg"], bkg_err=out["bkg_err"], converged=conv.astype(bool), valid=valid.astype(bool),
+        chisq=out["chisq"], qfit=out["qfit"], qfit_expected=out["qfit_expected"], qfit_z=out["qfit_z"],
+        crowding=out["crowding"], spread_model=out["spread_model"], spread_model_err=out["spread_model_err"],
+        n_iter=nit.astype(np.int64), n_passes=r.n_passes, n_failed=r.n_failed, failure_msgs=msgs, residual=resid)
 
 
 # ---------------------------------------------------------------------------

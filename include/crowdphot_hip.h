@@ -239,6 +239,30 @@ int cp_fit_star_batch_device(cp_ctx* ctx, const cp_psf_desc* psf,
                              int32_t n_cutouts, const cp_lm_opts* opts,
                              double* d_theta_out, double* d_cov_out, cp_lm_stat* d_stats);
 
+/* ---- sequential multi-pass photometry: fit_all_stars ---------------------
+ * fit_all_stars, src/photometry/psf_photometry_single.jl:267-469 (replaces
+ * that Julia function for CircularGaussianPSF / CircularMoffatPSF).
+ * image / inv_var (NULL = unweighted): host float32 ny*nx column-major.
+ * theta0: n_stars x nparams(kind) initial parameters in getproperties order
+ * (_extract_source_catalog, :152-197: the catalogue's y, x, flux, bkg with
+ * the template PSF's shape parameters).  free_mask: free parameters (bit k =
+ * parameter k); a fixed parameter k fits at fixed_values[k] (the `fixed`
+ * NamedTuple), or at the star's own value where fixed_values is NULL or NaN.
+ * spread_model_fwhm <= 0: from the PSF at star 1 (_spread_fwhm).  precision:
+ * CP_PREC_MIXED (float32 model / gradient, float64 sums and residual image)
+ * or CP_PREC_FP64.  Fills y, x, flux, bkg, their errors (NaN for a free
+ * parameter never fitted, 0 for a fixed one), chisq, converged, valid, n_iter
+ * (summed over passes), the six diagnostics (NaN where not computed),
+ * n_passes (0 for an empty catalogue), n_failed and, when set, residual.
+ * Stars whose fit throws in the reference (too few valid inv_var pixels) are
+ * marked invalid and counted in n_failed; a valid star with a non-finite
+ * position fails the call with CP_EARG (the reference's InexactError). */
+int cp_fit_all_stars(cp_ctx* ctx, const cp_psf_desc* psf, const float* image, const float* inv_var,
+                     int32_t ny, int32_t nx, const double* theta0, int32_t n_stars, double fit_rad,
+                     int32_t n_passes, uint32_t free_mask, const double* fixed_values,
+                     const cp_lm_opts* opts, double spread_model_fwhm, int32_t precision,
+                     cp_sim_result* out);
+
 /* ---- synthetic inputs: simulate_image (src/simulation.jl:211-498) ----------
  * Seeded xoshiro256** streams; host arrays.
  * cp_sim_sources: n positions uniform in [lo, hi] per axis and fluxes from

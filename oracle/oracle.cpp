@@ -1089,6 +1089,113 @@ void spread_kernel(double fwhm, int half, std::vector<double>& K) {
     for (double& v : K) v /= s;
 }
 
+// _spread_fwhm (psf_photometry_diagnostics.jl:113-123): effective-area FWHM of
+// a unit-flux render of the model (prm: full parameters at the star) over its
+// integer extent (PSF.jl:285-291 with fwhm_factor 5; ImagePSF
+// empirical_models.jl:167-176).  NaN when undefined.
+double spread_fwhm(const Psf& P, const double* prm_star, int row_flux, int row_bkg) {
+    double prm[8];
+    for (int k = 0; k < 8; ++k) prm[k] = prm_star[k];
+    prm[row_flux] = 1.0;
+    prm[row_bkg] = 0.0;
+    const double Y = prm[0], X = prm[1];
+    if (!(std::isfinite(Y) && std::isfinite(X))) return NAN;
+    double ylo, yhi, xlo, xhi;
+    if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN || P.kind == CP_PSF_CIRCULAR_MOFFAT) {
+        const double fw = P.kind == CP_PSF_CIRCULAR_GAUSSIAN ? prm[2] : 2 * prm[2] * std::sqrt(std::exp2(1 / prm[3]) - 1);
+        ylo = Y - 5 * fw / 2; yhi = Y + 5 * fw / 2;
+        xlo = X - 5 * fw / 2; xhi = X + 5 * fw / 2;
+    } else {
+        ylo = Y - (P.oy - 1) / P.os_y; yhi = Y + (P.gny - P.oy) / P.os_y;
+        xlo = X - (P.ox - 1) / P.os_x; xhi = X + (P.gnx - P.ox) / P.os_x;
+    }
+    double s1 = 0, s2 = 0;
+    for (long gx = (long)std::floor(xlo); gx <= (long)std::ceil(xhi); ++gx)
+        for (long gy = (long)std::floor(ylo); gy <= (long)std::ceil(yhi); ++gy) {
+            const double g = evaluate(P, prm, (double)gy, (double)gx);
+            s1 += g;
+            s2 += g * g;
+        }
+    return s2 > 0 ? std::sqrt(s1 * s1 / s2 * 2 * std::log(2.0) / M_PI) : NAN;
+}
+
+// _star_diagnostics! (psf_photometry_diagnostics.jl:154-231) for one star
+// (prm: its full parameters) over the box [ylo, yhi] x [xlo, xhi] (1-based,
+// inside the image): data is the original image, resid the residual with this
+// star's model (and its neighbours') removed, K the spread_model disk (empty:
+// spread_model left NaN), p the free-parameter count.  Writes outs[k][i].
+void star_diagnostics(const Psf& P, const double* prm, int row_flux, int row_bkg, int ylo, int yhi, int xlo,
+                      int xhi, const double* data, const double* resid, const double* inv_var, int ny,
+                      const std::vector<double>& K, int kh, int p, double* const outs[6], int i) {
+    if (ylo > yhi || xlo > xhi) return;
+    const int by = yhi - ylo + 1, bx = xhi - xlo + 1, kw = 2 * kh + 1;
+    const bool do_spread = !K.empty();
+    std::vector<double> ms((size_t)by * bx), gs((size_t)by * bx, 0.0);
+    for (int ix = 0; ix < bx; ++ix)
+        for (int iy = 0; iy < by; ++iy) ms[iy + ix * by] = evaluate(P, prm, ylo + iy, xlo + ix);
+    if (do_spread)   // correlate!(.., :zero), correlation.jl:122-140
+        for (int ix = 0; ix < bx; ++ix)
+            for (int iy = 0; iy < by; ++iy) {
+                double g = 0;
+                for (int b = -kh; b <= kh; ++b)
+                    for (int a = -kh; a <= kh; ++a) {
+                        const int jy = iy + a, jx = ix + b;
+                        if (jy < 0 || jy >= by || jx < 0 || jx >= bx) continue;
+                        g += ms[jy + jx * by] * K[(a + kh) + (b + kh) * kw];
+                    }
+                gs[iy + ix * by] = g;
+            }
+    const double flux = prm[row_flux], bkg = prm[row_bkg];
+    if (!(flux > 0)) return;
+    const double inv_flux = 1 / flux;
+    double qfit = 0, nclean = 0, ndirty = 0, dcrowd = 0, Sp = 0, Sq = 0, Sp0 = 0, Sq0 = 0, SqG = 0;
+    double ssum = 0, s2sum = 0;
+    int ngood = 0;
+    for (int ix = 0; ix < bx; ++ix)
+        for (int iy = 0; iy < by; ++iy) {
+            const size_t q = (size_t)(ylo + iy - 1) + (size_t)(xlo + ix - 1) * ny;
+            const double wp = inv_var ? inv_var[q] : 1.0;
+            if (!(std::isfinite(wp) && wp > 0)) continue;
+            const double rq = resid[q], mv = ms[iy + ix * by];
+            qfit += std::fabs(rq);
+            const double Pp = (mv - bkg) * inv_flux, wP = wp * Pp, pi = rq + mv - bkg;
+            nclean += wP * pi;
+            ndirty += wP * (data[q] - bkg);
+            dcrowd += wP * Pp;
+            if (do_spread) {
+                const double gi = (gs[iy + ix * by] - bkg) * inv_flux;
+                Sp += wP * pi;
+                Sq += wp * gi * pi;
+                Sp0 += wP * Pp;
+                Sq0 += wp * Pp * gi;
+                SqG += wp * gi * gi;
+            }
+            if (inv_var) {
+                const double sg = 1 / std::sqrt(wp);
+                ssum += sg;
+                s2sum += sg * sg;
+                ++ngood;
+            }
+        }
+    if (outs[0]) outs[0][i] = qfit * inv_flux;
+    if (outs[3] && dcrowd > 0 && nclean > 0 && ndirty > 0) outs[3][i] = 2.5 * std::log10(ndirty / nclean);
+    if (do_spread && Sp > 0) {
+        if (outs[4]) outs[4][i] = Sq / Sp - Sq0 / Sp0;
+        if (inv_var && outs[5]) {
+            const double v = Sp * Sp * SqG + Sq * Sq * Sp0 - 2 * Sq * Sp * Sq0;
+            outs[5][i] = std::sqrt(std::max(0.0, v)) / (Sp * Sp);
+        }
+    }
+    if (inv_var) {
+        const double c = std::sqrt(2 / M_PI);
+        if (outs[1]) outs[1][i] = c * ssum * inv_flux;
+        if (outs[2] && s2sum > 0 && ngood > p) {
+            const double dof_f = std::sqrt(1 - (double)p / ngood);
+            outs[2][i] = (qfit - c * dof_f * ssum) / (std::sqrt((1 - 2 / M_PI) * s2sum) * dof_f);
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1475,40 +1582,16 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
         if (!any) return;
         // reference disk: explicit FWHM, else the effective-area FWHM of a
         // unit-flux render of the model at catalogue star 1's fitted position
-        double sfw = o->spread_model_fwhm;
-        if (!(sfw > 0)) {
-            sfw = NAN;
-            double prm[8];
-            for (int k = 0; k < np_full; ++k) prm[k] = tmpl[k];
-            prm[0] = Y[0];
-            prm[1] = X[0];
-            prm[row_flux] = 1.0;
-            prm[row_bkg] = 0.0;
-            double ylo, yhi, xlo, xhi;
-            if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN || P.kind == CP_PSF_CIRCULAR_MOFFAT) {
-                const double fw = P.kind == CP_PSF_CIRCULAR_GAUSSIAN
-                                      ? P.fwhm : 2 * P.alpha * std::sqrt(std::exp2(1 / P.beta) - 1);
-                ylo = Y[0] - 5 * fw / 2; yhi = Y[0] + 5 * fw / 2;
-                xlo = X[0] - 5 * fw / 2; xhi = X[0] + 5 * fw / 2;
-            } else {
-                ylo = Y[0] - (P.oy - 1) / P.os_y; yhi = Y[0] + (P.gny - P.oy) / P.os_y;
-                xlo = X[0] - (P.ox - 1) / P.os_x; xhi = X[0] + (P.gnx - P.ox) / P.os_x;
-            }
-            if (std::isfinite(Y[0]) && std::isfinite(X[0])) {
-                double s1 = 0, s2 = 0;
-                for (long gx = (long)std::floor(xlo); gx <= (long)std::ceil(xhi); ++gx)
-                    for (long gy = (long)std::floor(ylo); gy <= (long)std::ceil(yhi); ++gy) {
-                        const double g = evaluate(P, prm, (double)gy, (double)gx);
-                        s1 += g;
-                        s2 += g * g;
-                    }
-                if (s2 > 0) sfw = std::sqrt(s1 * s1 / s2 * 2 * std::log(2.0) / M_PI);
-            }
-        }
-        const int kh = (int)std::ceil(fit_rad), kw = 2 * kh + 1;
+        double prm1[8];
+        for (int k = 0; k < np_full; ++k) prm1[k] = tmpl[k];
+        prm1[0] = Y[0];
+        prm1[1] = X[0];
+        const double sfw = o->spread_model_fwhm > 0 ? o->spread_model_fwhm : spread_fwhm(P, prm1, row_flux, row_bkg);
+        const int kh = (int)std::ceil(fit_rad);
         std::vector<double> K;
-        const bool do_spread = std::isfinite(sfw) && sfw > 0;
-        if (do_spread) spread_kernel(sfw, kh, K);
+        if (std::isfinite(sfw) && sfw > 0) spread_kernel(sfw, kh, K);
+        std::vector<double> resid((size_t)ny * nx);
+        for (size_t q = 0; q < resid.size(); ++q) resid[q] = data[q] - model_img[q];
         double prm[8];
         for (int j = 0; j < na; ++j) {
             const int i = active[j];
@@ -1517,75 +1600,13 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
             prm[0] = Y[i];
             prm[1] = X[i];
             prm[row_flux] = Fl[i];
-            int ylo = std::max((int)std::floor(Y[i] - fit_rad), 1), yhi = std::min((int)std::ceil(Y[i] + fit_rad), (int)ny);
-            int xlo = std::max((int)std::floor(X[i] - fit_rad), 1), xhi = std::min((int)std::ceil(X[i] + fit_rad), (int)nx);
-            if (ylo > yhi || xlo > xhi) continue;
-            const int by = yhi - ylo + 1, bx = xhi - xlo + 1;
-            std::vector<double> ms((size_t)by * bx), gs((size_t)by * bx, 0.0);
-            for (int ix = 0; ix < bx; ++ix)
-                for (int iy = 0; iy < by; ++iy) ms[iy + ix * by] = evaluate(P, prm, ylo + iy, xlo + ix);
-            if (do_spread)   // correlate!(.., :zero), correlation.jl:122-140
-                for (int ix = 0; ix < bx; ++ix)
-                    for (int iy = 0; iy < by; ++iy) {
-                        double g = 0;
-                        for (int b = -kh; b <= kh; ++b)
-                            for (int a = -kh; a <= kh; ++a) {
-                                const int jy = iy + a, jx = ix + b;
-                                if (jy < 0 || jy >= by || jx < 0 || jx >= bx) continue;
-                                g += ms[jy + jx * by] * K[(a + kh) + (b + kh) * kw];
-                            }
-                        gs[iy + ix * by] = g;
-                    }
-            const double flux = Fl[i], bkg = prm[row_bkg];
-            if (!(flux > 0)) continue;
-            const double inv_flux = 1 / flux;
-            double qfit = 0, nclean = 0, ndirty = 0, dcrowd = 0, Sp = 0, Sq = 0, Sp0 = 0, Sq0 = 0, SqG = 0;
-            double ssum = 0, s2sum = 0;
-            int ngood = 0;
-            for (int ix = 0; ix < bx; ++ix)
-                for (int iy = 0; iy < by; ++iy) {
-                    const size_t q = (size_t)(ylo + iy - 1) + (size_t)(xlo + ix - 1) * ny;
-                    const double wp = inv_var ? inv_var[q] : 1.0;
-                    if (!(std::isfinite(wp) && wp > 0)) continue;
-                    const double rq = data[q] - model_img[q], mv = ms[iy + ix * by];
-                    qfit += std::fabs(rq);
-                    const double Pp = (mv - bkg) * inv_flux, wP = wp * Pp, pi = rq + mv - bkg;
-                    nclean += wP * pi;
-                    ndirty += wP * (data[q] - bkg);
-                    dcrowd += wP * Pp;
-                    if (do_spread) {
-                        const double gi = (gs[iy + ix * by] - bkg) * inv_flux;
-                        Sp += wP * pi;
-                        Sq += wp * gi * pi;
-                        Sp0 += wP * Pp;
-                        Sq0 += wp * Pp * gi;
-                        SqG += wp * gi * gi;
-                    }
-                    if (inv_var) {
-                        const double sg = 1 / std::sqrt(wp);
-                        ssum += sg;
-                        s2sum += sg * sg;
-                        ++ngood;
-                    }
-                }
-            if (out->qfit) out->qfit[i] = qfit * inv_flux;
-            if (out->crowding && dcrowd > 0 && nclean > 0 && ndirty > 0)
-                out->crowding[i] = 2.5 * std::log10(ndirty / nclean);
-            if (do_spread && Sp > 0) {
-                if (out->spread_model) out->spread_model[i] = Sq / Sp - Sq0 / Sp0;
-                if (inv_var && out->spread_model_err) {
-                    const double v = Sp * Sp * SqG + Sq * Sq * Sp0 - 2 * Sq * Sp * Sq0;
-                    out->spread_model_err[i] = std::sqrt(std::max(0.0, v)) / (Sp * Sp);
-                }
-            }
-            if (inv_var) {
-                const double c = std::sqrt(2 / M_PI);
-                if (out->qfit_expected) out->qfit_expected[i] = c * ssum * inv_flux;
-                if (out->qfit_z && s2sum > 0 && ngood > p) {
-                    const double dof_f = std::sqrt(1 - (double)p / ngood);
-                    out->qfit_z[i] = (qfit - c * dof_f * ssum) / (std::sqrt((1 - 2 / M_PI) * s2sum) * dof_f);
-                }
-            }
+            // +-fit_rad box at the fitted position, clamped (:1361-1365)
+            const int ylo = std::max((int)std::floor(Y[i] - fit_rad), 1);
+            const int yhi = std::min((int)std::ceil(Y[i] + fit_rad), (int)ny);
+            const int xlo = std::max((int)std::floor(X[i] - fit_rad), 1);
+            const int xhi = std::min((int)std::ceil(X[i] + fit_rad), (int)nx);
+            star_diagnostics(P, prm, row_flux, row_bkg, ylo, yhi, xlo, xhi, data.data(), resid.data(), inv_var, ny, K,
+                             kh, p, outs, i);
         }
     };
 
@@ -1720,11 +1741,13 @@ int or_lm_line(const double* xs, const double* ys, int32_t n, const double* x0,
     return rc;
 }
 
-// fit_star over a whole cutout (psf_fitting.jl:229-308 with
+namespace {
+// fit_star over a column-major ny x nx block whose pixel (i, j) sits at image
+// coordinates (oy + i, ox + j) (psf_fitting.jl:229-308 with
 // _prepare_fit_star_inputs :36-101); generic accum! over evaluate_fg.
-int or_fit_star(const cp_psf_desc* psfd, const double* pix, int32_t ny, int32_t nx,
-                const double* inv_var, const double* theta0, uint32_t free_mask,
-                const cp_lm_opts* o, double* theta_out, double* cov_out, cp_lm_stat* st) {
+int fit_star_block(const cp_psf_desc* psfd, const double* pix, int ny, int nx, int oy, int ox,
+                   const double* inv_var, const double* theta0, uint32_t free_mask, const cp_lm_opts* o,
+                   double* theta_out, double* cov_out, cp_lm_stat* st) {
     Psf P;
     if (make_psf(psfd, P)) return CP_EARG;
     const int npf = nparams_kind(P.kind);
@@ -1759,7 +1782,7 @@ int or_fit_star(const cp_psf_desc* psfd, const double* pix, int32_t ny, int32_t 
         for (int j = 1; j <= nx; ++j)
             for (int i = 1; i <= ny; ++i, ++q) {
                 double wv = inv_var ? wts[q] : 1.0;
-                evaluate_fg(P, prm.data(), (double)i, (double)j, f, G);
+                evaluate_fg(P, prm.data(), (double)(oy + i), (double)(ox + j), f, G);
                 double r = f - pix[q];
                 double wr = wv * r;
                 cost = std::fma(wr, r, cost);
@@ -1775,6 +1798,148 @@ int or_fit_star(const cp_psf_desc* psfd, const double* pix, int32_t ny, int32_t 
     if (rc) return rc;
     for (int k = 0; k < npf; ++k) theta_out[k] = full[k];
     for (int k = 0; k < p; ++k) theta_out[free_idx[k]] = x[k];
+    return 0;
+}
+}  // namespace
+
+int or_fit_star(const cp_psf_desc* psfd, const double* pix, int32_t ny, int32_t nx, const double* inv_var,
+                const double* theta0, uint32_t free_mask, const cp_lm_opts* o, double* theta_out, double* cov_out,
+                cp_lm_stat* st) {
+    return fit_star_block(psfd, pix, ny, nx, 0, 0, inv_var, theta0, free_mask, o, theta_out, cov_out, st);
+}
+
+// fit_all_stars (psf_photometry_single.jl:267-469): stars sorted by flux
+// (stable, brightest first) are fitted one at a time with fit_star on the
+// progressively subtracted residual over their +-fit_rad box; passes after
+// the first add each star's current model back before refitting it, and the
+// last pass computes the per-star diagnostics right after the star's
+// subtraction.  theta0: n x npf initial parameters (getproperties order),
+// free_mask: free parameters; a fixed parameter k fits at fixed_values[k]
+// (model_from_vector with `fixed`), or at the star's own value when
+// fixed_values is NULL or holds NaN there.  Fills y, x, flux, bkg, their
+// errors, converged, valid, chisq, n_iter, the diagnostics and the residual.
+int or_fit_all_stars(const cp_psf_desc* psfd, const double* image, const double* inv_var, int32_t ny, int32_t nx,
+                     const double* theta0, int32_t n, double fit_rad, int32_t n_passes, uint32_t free_mask,
+                     const double* fixed_values, const cp_lm_opts* o, double spread_model_fwhm,
+                     cp_sim_result* out) {
+    Psf P;
+    if (make_psf(psfd, P)) return CP_EARG;
+    if (n_passes <= 0) return fail("n_passes must be positive");
+    if (!(fit_rad > 0)) return fail("fit_rad must be positive");
+    const int npf = nparams_kind(P.kind), rf = flux_slot(P.kind), rb = bkg_slot(P.kind);
+    std::vector<int> free_idx;
+    for (int k = 0; k < npf; ++k)
+        if (free_mask & (1u << k)) free_idx.push_back(k);
+    const int p = (int)free_idx.size();
+    if (p == 0) return fail("all model parameters are fixed; nothing to fit");
+    if (n == 0) {   // the empty result (:285-288): no passes, 0 x 0 residual
+        out->n_passes = 0;
+        out->n_failed = 0;
+        return 0;
+    }
+    std::vector<double> prm(theta0, theta0 + (size_t)n * npf), err((size_t)n * npf, NAN);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < npf; ++k)
+            if (!(free_mask & (1u << k))) err[(size_t)i * npf + k] = 0.0;
+    std::vector<double> R(image, image + (size_t)ny * nx);
+    std::vector<uint8_t> valid(n, 1), conv(n, 0);
+    std::vector<double> chisq(n, 0.0);
+    std::vector<int> niter(n, 0);
+    double* outs[6] = {out->qfit, out->qfit_expected, out->qfit_z, out->crowding, out->spread_model,
+                       out->spread_model_err};
+    for (double* d : outs)
+        if (d)
+            for (int i = 0; i < n; ++i) d[i] = NAN;
+    const int kh = (int)std::ceil(fit_rad);
+    std::vector<double> K;
+    if (n > 0) {
+        const double sfw = spread_model_fwhm > 0 ? spread_model_fwhm : spread_fwhm(P, &prm[0], rf, rb);
+        if (std::isfinite(sfw) && sfw > 0) spread_kernel(sfw, kh, K);
+    }
+    int n_failed = 0;
+    std::vector<int> order(n);
+    std::vector<double> cut, ivc, th(npf), th0(npf), cov((size_t)p * p);
+    auto render = [&](const double* m, int ylo, int yhi, int xlo, int xhi, double sign) {
+        for (int gx = xlo; gx <= xhi; ++gx)
+            for (int gy = ylo; gy <= yhi; ++gy)
+                R[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] += sign * evaluate(P, m, gy, gx);
+    };
+    for (int pass = 1; pass <= n_passes; ++pass) {
+        for (int i = 0; i < n; ++i) order[i] = i;
+        auto key = [&](int i) {   // sortperm(rev=true) puts NaN first
+            const double f = prm[(size_t)i * npf + rf];
+            return std::isnan(f) ? INFINITY : f;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) > key(b); });
+        for (int idx : order) {
+            if (!valid[idx]) continue;
+            double* m = &prm[(size_t)idx * npf];
+            // floor(Int, .) of a non-finite or huge position throws outside the try
+            if (!(std::fabs(m[0]) < 1e9 && std::fabs(m[1]) < 1e9)) return fail("non-finite star position");
+            const int ylo = std::max((int)std::floor(m[0] - fit_rad), 1), yhi = std::min((int)std::ceil(m[0] + fit_rad), (int)ny);
+            const int xlo = std::max((int)std::floor(m[1] - fit_rad), 1), xhi = std::min((int)std::ceil(m[1] + fit_rad), (int)nx);
+            const long by = (long)yhi - ylo + 1, bx = (long)xhi - xlo + 1;
+            if (by <= 0 || bx <= 0 || by * bx < 3) {
+                valid[idx] = 0;
+                continue;
+            }
+            if (pass > 1) render(m, ylo, yhi, xlo, xhi, 1.0);
+            cut.assign((size_t)by * bx, 0.0);
+            if (inv_var) ivc.assign((size_t)by * bx, 0.0);
+            for (int ix = 0; ix < bx; ++ix)
+                for (int iy = 0; iy < by; ++iy) {
+                    const size_t q = (size_t)(ylo + iy - 1) + (size_t)(xlo + ix - 1) * ny;
+                    cut[iy + ix * by] = R[q];
+                    if (inv_var) ivc[iy + ix * by] = inv_var[q];
+                }
+            for (int k = 0; k < npf; ++k)
+                th0[k] = (!(free_mask & (1u << k)) && fixed_values && !std::isnan(fixed_values[k])) ? fixed_values[k]
+                                                                                                    : m[k];
+            cp_lm_stat st{};
+            const int rc = fit_star_block(psfd, cut.data(), (int)by, (int)bx, ylo - 1, xlo - 1,
+                                          inv_var ? ivc.data() : nullptr, th0.data(), free_mask, o, th.data(),
+                                          cov.data(), &st);
+            if (rc != 0) {   // the reference's catch: undo the add-back, mark invalid
+                if (pass > 1) render(m, ylo, yhi, xlo, xhi, -1.0);
+                valid[idx] = 0;
+                ++n_failed;
+                continue;
+            }
+            for (int k = 0; k < npf; ++k) m[k] = th[k];
+            for (int j = 0; j < p; ++j) err[(size_t)idx * npf + free_idx[j]] = std::sqrt(std::max(0.0, cov[(size_t)j * p + j]));
+            conv[idx] = st.converged;
+            chisq[idx] = st.chisq;
+            niter[idx] += st.iterations;
+            render(m, ylo, yhi, xlo, xhi, -1.0);
+            if (pass == n_passes)
+                star_diagnostics(P, m, rf, rb, ylo, yhi, xlo, xhi, image, R.data(), inv_var, ny, K, kh, p, outs, idx);
+        }
+        for (int i = 0; i < n; ++i) {   // between-pass validation
+            const double* m = &prm[(size_t)i * npf];
+            valid[i] = valid[i] && conv[i] && std::isfinite(m[0]) && std::isfinite(m[1]) && std::isfinite(m[rf]) &&
+                       m[rf] > 0;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        const double* m = &prm[(size_t)i * npf];
+        const double* e = &err[(size_t)i * npf];
+        out->y[i] = m[0];
+        out->x[i] = m[1];
+        out->flux[i] = m[rf];
+        out->bkg[i] = m[rb];
+        out->y_err[i] = e[0];
+        out->x_err[i] = e[1];
+        out->flux_err[i] = e[rf];
+        out->bkg_err[i] = e[rb];
+        out->converged[i] = conv[i];
+        out->valid[i] = valid[i];
+        if (out->chisq) out->chisq[i] = chisq[i];
+        if (out->n_iter) out->n_iter[i] = niter[i];
+    }
+    if (out->residual)
+        for (size_t q = 0; q < R.size(); ++q) out->residual[q] = (float)R[q];
+    out->n_passes = n_passes;
+    out->n_failed = n_failed;
     return 0;
 }
 

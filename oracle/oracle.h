@@ -70,6 +70,10 @@ int or_lm_line(const double* xs, const double* ys, int32_t n, const double* x0,
 int or_fit_star(const cp_psf_desc* psf, const double* pix, int32_t ny, int32_t nx,
                 const double* inv_var, const double* theta0, uint32_t free_mask,
                 const cp_lm_opts* o, double* theta_out, double* cov_out, cp_lm_stat* st);
+int or_fit_all_stars(const cp_psf_desc* psf, const double* image, const double* inv_var, int32_t ny, int32_t nx,
+                     const double* theta0, int32_t n, double fit_rad, int32_t n_passes, uint32_t free_mask,
+                     const double* fixed_values, const cp_lm_opts* o, double spread_model_fwhm,
+                     cp_sim_result* out);
 /* batched over float32 cutouts packed as in cp_fit_star_batch; nthreads <= 0 = all */
 int or_fit_star_batch(const cp_psf_desc* psf, const float* pix, const int64_t* offsets,
                       const int32_t* ny, const int32_t* nx, const float* inv_var,

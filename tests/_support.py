@@ -72,6 +72,9 @@ def oracle():
                                   P(abi.LMOpts), f64p, f64p, P(abi.LMStat)]),
         "or_fit_star_batch": (C.c_int, [P(abi.PsfDesc), f32p, i64p, i32p, i32p, f32p, f64p, C.c_uint32,
                                         C.c_int32, P(abi.LMOpts), f64p, f64p, P(abi.LMStat), C.c_int32]),
+        "or_fit_all_stars": (C.c_int, [P(abi.PsfDesc), f64p, f64p, C.c_int32, C.c_int32, f64p, C.c_int32,
+                                       C.c_double, C.c_int32, C.c_uint32, f64p, P(abi.LMOpts), C.c_double,
+                                       P(abi.SimResult)]),
         "or_extract_errors": (C.c_int, [f64p, C.c_int32, i32p, C.c_int32, f64p]),
     }
     for k, (r, a) in sig.items():

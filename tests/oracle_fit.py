@@ -63,3 +63,53 @@ def fit(psf_model, image, ys, xs, fs, fit_rad, inv_var=None, trace=0, **kw):
     r.valid = r.valid.astype(bool)
     r.converged = r.converged.astype(bool)
     return r
+
+
+def fit_all(psf_model, image, sources, fit_rad, n_passes=3, fixed=None, inv_var=None, spread_model_fwhm=None,
+            **lm_kw):
+    """Oracle fit_all_stars with the package function's arguments: image
+    [ny, nx] (row = y), sources a dict with y, x and optional flux, bkg."""
+    lib = oracle()
+    fixed = dict(fixed or {})
+    ny, nx = np.shape(image)
+    names = psf_model.props
+    y = f64(sources["y"])
+    n = len(y)
+    cols = {"y": y, "x": f64(sources["x"]), "flux": f64(sources.get("flux", np.ones(n))),
+            "bkg": f64(sources.get("bkg", np.zeros(n)))}
+    th0 = np.tile(psf_model.params(), (max(n, 1), 1))[:n].copy()
+    for k, nme in enumerate(names):
+        if nme in cols:
+            th0[:, k] = cols[nme]
+    th0 = f64(th0)
+    mask = sum(1 << k for k, nme in enumerate(names) if nme not in fixed)
+    fixv = f64([float(fixed[k]) if k in fixed else np.nan for k in names])
+    o = abi.lm_opts_default()
+    for k, v in lm_kw.items():
+        setattr(o, k, v)
+    r = Result()
+    for k in FIELDS:
+        setattr(r, k, np.zeros(n))
+    r.converged = np.zeros(n, np.uint8)
+    r.valid = np.zeros(n, np.uint8)
+    r.n_iter = np.zeros(n, np.int32)
+    r.residual = np.zeros(ny * nx, np.float32)
+    s = abi.SimResult()
+    for k in FIELDS:
+        setattr(s, k, ptr(getattr(r, k), abi.f64p))
+    s.converged = ptr(r.converged, abi.u8p)
+    s.valid = ptr(r.valid, abi.u8p)
+    s.n_iter = ptr(r.n_iter, abi.i32p)
+    s.residual = ptr(r.residual, abi.f32p)
+    img = f64(np.asarray(image).ravel(order="F"))
+    iv = None if inv_var is None else f64(np.asarray(inv_var).ravel(order="F"))
+    rc = lib.or_fit_all_stars(psf_model.desc(), ptr(img, abi.f64p), ptr(iv, abi.f64p), ny, nx, ptr(th0, abi.f64p),
+                              n, float(fit_rad), int(n_passes), mask, ptr(fixv, abi.f64p), C.byref(o),
+                              0.0 if spread_model_fwhm is None else float(spread_model_fwhm), C.byref(s))
+    if rc != 0:
+        raise ValueError(lib.or_last_error().decode())
+    r.n_passes, r.n_failed = s.n_passes, s.n_failed
+    r.residual = r.residual.reshape(nx, ny).T
+    r.valid = r.valid.astype(bool)
+    r.converged = r.converged.astype(bool)
+    return r

@@ -31,6 +31,26 @@ def test_simultaneous_argument_contract():
         f(img, psf, cat, 5, fixed={"fwhm": 2.0, "bkg": 0.0}, inv_var=np.ones((3, 3)), model_rad=5)
 
 
+def test_multipass_argument_contract():
+    """fit_all_stars' argument errors are raised on the host, before any
+    device work (psf_photometry_single.jl:279,305)."""
+    psf = pkg.CircularGaussianPSF(fwhm=2.0)
+    img = np.zeros((64, 64))
+    cat = {"y": [32.0], "x": [32.0], "flux": [500.0]}
+    f = pkg.fit_all_stars
+    with pytest.raises(pkg.ArgumentError, match="n_passes"):
+        f(img, psf, cat, 5, n_passes=0)
+    with pytest.raises(pkg.ArgumentError, match="all model parameters are fixed"):
+        f(img, psf, cat, 5, fixed={"y": 1, "x": 1, "flux": 1, "fwhm": 2.0, "bkg": 0.0})
+    with pytest.raises(pkg.ArgumentError, match="same size"):
+        f(img, psf, cat, 5, inv_var=np.ones((3, 3)))
+    with pytest.raises(pkg.ArgumentError, match="no parameter"):
+        f(img, psf, cat, 5, fixed={"alpha": 1.0})
+    epsf = pkg.ImagePSF(np.ones((9, 9)), oversampling=1)
+    with pytest.raises(pkg.ArgumentError, match="CircularGaussianPSF"):
+        f(img, epsf, cat, 5)
+
+
 def test_generator_is_seeded_and_follows_distributions():
     y, x, fl = sim.simulate_sources(7, 20000, (8, 248), (8, 248), (1e2, 1e5), "loguniform")
     y2, x2, fl2 = sim.simulate_sources(7, 20000, (8, 248), (8, 248), (1e2, 1e5), "loguniform")
