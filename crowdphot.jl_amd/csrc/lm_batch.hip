@@ -8,6 +8,7 @@
 // References: fit_star src/psf/psf_fitting.jl:229-308; lm_irls
 // src/levenberg_marquardt.jl:431-680.
 #include <cstring>
+#include <type_traits>
 
 #include "lm_core.h"
 #include "psf_host.h"
@@ -15,7 +16,7 @@
 namespace cp {
 
 template <typename T, int KIND, int NP>
-__global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, const float* __restrict__ pix,
+__global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, LMModel<T> Md, const float* __restrict__ pix,
                                                     const int64_t* __restrict__ offsets,
                                                     const int32_t* __restrict__ cny, const int32_t* __restrict__ cnx,
                                                     const float* __restrict__ iv, const double* __restrict__ theta0,
@@ -36,7 +37,7 @@ __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, const float* __re
             if (k < C.npf) full0[k] = theta0[(size_t)c * C.npf + k];
         const CutoutSrc S{pix + off, C.have_iv ? iv + off : nullptr, ny, ny * nx};
         LMOut<NP> R;
-        lm_fit<T, KIND, NP>(C, full0, S, R);
+        lm_fit<T, KIND, NP>(C, Md, full0, S, R);
         // Lane 0 writes inside an if that falls through to the loop end (no
         // divergent `continue`): the loop then has one back edge and the
         // whole wave is converged again when the next __shfl reads lane 0.
@@ -53,10 +54,10 @@ __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, const float* __re
 }
 
 template <typename T, int KIND>
-void launch_lm(int np, dim3 g, hipStream_t s, const LMConst& C, const float* pix, const int64_t* off,
-               const int32_t* ny, const int32_t* nx, const float* iv, const double* th0, int n, double* th,
-               double* cov, cp_lm_stat* st, int* work) {
-#define CP_LM(NPV) launch_n("k_lm_batch", k_lm_batch<T, KIND, NPV>, g, dim3(BLOCK), 0, s, C, pix, off, ny, nx, iv, th0, n, th, cov, st, work)
+void launch_lm(int np, dim3 g, hipStream_t s, const LMConst& C, const LMModel<T>& Md, const float* pix,
+               const int64_t* off, const int32_t* ny, const int32_t* nx, const float* iv, const double* th0, int n,
+               double* th, double* cov, cp_lm_stat* st, int* work) {
+#define CP_LM(NPV) launch_n("k_lm_batch", k_lm_batch<T, KIND, NPV>, g, dim3(BLOCK), 0, s, C, Md, pix, off, ny, nx, iv, th0, n, th, cov, st, work)
     switch (np) {
         case 1: CP_LM(1); break;
         case 2: CP_LM(2); break;
@@ -79,20 +80,25 @@ void lm_batch_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_pix, co
     CP_ARG(lm_const(psf, free_mask, o, d_iv != nullptr, C, np, &why), why);
     if (n == 0) return;
     hipStream_t st = ctx->stream;
+    const HostPsf P = make_host_psf(psf);
+    DevPsf DP;
+    upload_psf(ctx, psf, P, DP);
     DevBuf work(ctx, 4);
     CP_HIP(hipMemsetAsync(work.get(), 0, 4, st));
     dim3 g((unsigned)std::min<int64_t>((n + NWAVE - 1) / NWAVE, (int64_t)std::max(ctx->n_cu, 1) * 8));
-    if (psf->kind == CP_PSF_CIRCULAR_GAUSSIAN) {
-        if (precision == CP_PREC_FP64)
-            launch_lm<double, KIND_GAUSS>(np, g, st, C, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work.as<int>());
-        else
-            launch_lm<float, KIND_GAUSS>(np, g, st, C, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work.as<int>());
-    } else {
-        if (precision == CP_PREC_FP64)
-            launch_lm<double, KIND_MOFFAT>(np, g, st, C, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work.as<int>());
-        else
-            launch_lm<float, KIND_MOFFAT>(np, g, st, C, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work.as<int>());
-    }
+    auto go = [&](auto tag, auto kind) {
+        using T = decltype(tag);
+        constexpr int K = decltype(kind)::value;
+        const LMModel<T> Md{model_const<T>(P, DP, 0.0), DP.G};
+        launch_lm<T, K>(np, g, st, C, Md, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work.as<int>());
+    };
+    using KG = std::integral_constant<int, KIND_GAUSS>;
+    using KM = std::integral_constant<int, KIND_MOFFAT>;
+    using KE = std::integral_constant<int, KIND_EPSF>;
+    const bool f64 = precision == CP_PREC_FP64;
+    if (psf->kind == CP_PSF_CIRCULAR_GAUSSIAN) f64 ? go(0.0, KG{}) : go(0.0f, KG{});
+    else if (psf->kind == CP_PSF_CIRCULAR_MOFFAT) f64 ? go(0.0, KM{}) : go(0.0f, KM{});
+    else f64 ? go(0.0, KE{}) : go(0.0f, KE{});
     CP_HIP(hipStreamSynchronize(st));
 }
 
@@ -144,7 +150,7 @@ int cp_fit_star_batch(cp_ctx* ctx, const cp_psf_desc* psf, const float* pix, con
         if (n == 0) return;
         int64_t total = 0;
         for (int c = 0; c < n; ++c) total = std::max<int64_t>(total, offsets[c] + (int64_t)ny[c] * nx[c]);
-        const int npf = psf->kind == CP_PSF_CIRCULAR_GAUSSIAN ? 5 : 6;
+        const int npf = lm_nparams(psf->kind);
         int np = 0;
         for (int k = 0; k < npf; ++k) np += (free_mask >> k) & 1u;
         DevBuf dp(ctx, total * 4), dof(ctx, (size_t)n * 8), dny(ctx, (size_t)n * 4), dnx(ctx, (size_t)n * 4),

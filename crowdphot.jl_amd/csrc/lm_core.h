@@ -9,8 +9,9 @@
 // lm_irls step (float64 NP x NP Cholesky), so control flow stays uniform.
 //
 // References: fit_star src/psf/psf_fitting.jl:229-308 with the specialised
-// accumulators :314-451 (CircularGaussianPSF) and :772-862 (CircularMoffatPSF);
-// lm_irls src/levenberg_marquardt.jl:431-680; damp! :74-99; covariance! :366-388.
+// accumulators :314-451 (CircularGaussianPSF), :772-862 (CircularMoffatPSF),
+// :1086-1173 (ImagePSF) and :1177-1210 (GriddedPSFModel); lm_irls
+// src/levenberg_marquardt.jl:431-680; damp! :74-99; covariance! :366-388.
 #pragma once
 
 #include "sim_kernels.h"
@@ -78,6 +79,46 @@ __device__ __forceinline__ void model_fg(const T* prm, T ddy, T ddx, T& f, T G[6
     }
 }
 
+// The ePSF grids and node layout of a CP_PSF_GRIDDED_IMAGE model (unused by
+// the analytic kinds).  M.bkg is 0: the background is a fitted parameter.
+template <typename T>
+struct LMModel {
+    ModelConst<T> M;
+    NodeGrid G;
+};
+
+// Bilinear node weights of a multi-node ePSF at the star's current position
+// (GriddedPSFModel evaluate_fg, gridded_psf.jl:252-271), once per parameter
+// vector; nothing for the other kinds.
+template <typename T, int KIND>
+__device__ __forceinline__ void model_corners(const LMModel<T>& Md, const double* full, CornerRec<T>& cr) {
+    if constexpr (KIND == KIND_EPSF) grid_corners<T>(Md.G, Md.M.n_nodes, full[0], full[1], cr);
+}
+
+// f and the getproperties-ordered gradient at offset (ddy, ddx) from the
+// centre: the analytic formulas above, or for the ePSF (y, x, flux, bkg)
+// f = flux s + bkg with G = (flux ds/dy, flux ds/dx, s, 1) through the same
+// bicubic / node-blend device code the simultaneous fit uses.
+template <typename T, int KIND>
+__device__ __forceinline__ void model_eval(const LMModel<T>& Md, const CornerRec<T>& cr, const T* prm, T ddy,
+                                           T ddx, T& f, T G[6]) {
+    if constexpr (KIND == KIND_EPSF) {
+        StarRec<T> r{};
+        r.flux = prm[2];
+        r.amp = prm[2];
+        T gy, gx, gf;
+        f = star_value<T, KIND_EPSF>(Md.M, r, ddy, ddx, Md.M.grids, &cr) + prm[3];
+        star_grad<T, KIND_EPSF>(Md.M, r, ddy, ddx, Md.M.grids, &cr, gy, gx, gf);
+        G[0] = gy;
+        G[1] = gx;
+        G[2] = gf;
+        G[3] = T(1);
+        G[4] = G[5] = T(0);
+    } else {
+        model_fg<T, KIND>(prm, ddy, ddx, f, G);
+    }
+}
+
 __device__ __forceinline__ double pick6(const double* v, int k) {
     return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : k == 3 ? v[3] : k == 4 ? v[4] : v[5];
 }
@@ -108,11 +149,14 @@ struct CutoutSrc {
 // then rounded, so absolute image coordinates keep their precision in the
 // float32 mode.
 template <typename T, int KIND, int NP, typename Src>
-__device__ __forceinline__ void accum(const LMConst& C, const double* full, const Src& S, NE<NP>& out) {
+__device__ __forceinline__ void accum(const LMConst& C, const LMModel<T>& Md, const double* full, const Src& S,
+                                      NE<NP>& out) {
     const int lane = threadIdx.x & 63;
     T prm[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) prm[k] = (T)full[k];
+    CornerRec<T> cr;
+    model_corners<T, KIND>(Md, full, cr);
     NE<NP> acc;
 #pragma unroll
     for (int i = 0; i < NP * (NP + 1) / 2; ++i) acc.A[i] = 0;
@@ -123,7 +167,7 @@ __device__ __forceinline__ void accum(const LMConst& C, const double* full, cons
         double py, px, v, w;
         S.at(m, py, px, v, w);
         T f, G[6];
-        model_fg<T, KIND>(prm, (T)(py - full[0]), (T)(px - full[1]), f, G);
+        model_eval<T, KIND>(Md, cr, prm, (T)(py - full[0]), (T)(px - full[1]), f, G);
         const T r = f - (T)v;
         const double wr = w * (double)r;
         acc.cost = fma(wr, (double)r, acc.cost);
@@ -283,7 +327,7 @@ struct LMOut {
 // and the lm_irls iteration from the full parameter vector full0 over the
 // pixel source S.  Wave-uniform: every lane returns the same R.
 template <typename T, int KIND, int NP, typename Src>
-__device__ void lm_fit(const LMConst& C, const double* full0, const Src& S, LMOut<NP>& R) {
+__device__ void lm_fit(const LMConst& C, const LMModel<T>& Md, const double* full0, const Src& S, LMOut<NP>& R) {
     const int lane = threadIdx.x & 63;
     double full[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -327,7 +371,7 @@ __device__ void lm_fit(const LMConst& C, const double* full0, const Src& S, LMOu
     NE<NP>& cur = R.cur;
     double fx[6];
     to_full(x, fx);
-    accum<T, KIND, NP>(C, fx, S, cur);
+    accum<T, KIND, NP>(C, Md, fx, S, cur);
     const double cost_init = cur.cost;
     double D[NP];
 #pragma unroll
@@ -374,7 +418,7 @@ __device__ void lm_fit(const LMConst& C, const double* full0, const Src& S, LMOu
 #pragma unroll
         for (int i = 0; i < NP; ++i) xc[i] = x[i] + dlt[i];
         to_full(xc, fx);
-        accum<T, KIND, NP>(C, fx, S, cand);
+        accum<T, KIND, NP>(C, Md, fx, S, cand);
         const bool accepted = cand.cost < cur.cost;
         const double dcost = cur.cost - cand.cost;
         const double ratio = prered > 0 ? dcost / prered : 0.0;
@@ -430,19 +474,24 @@ __device__ void lm_cov(const LMConst& C, const LMOut<NP>& R, int npx, double* cv
     for (int i = 0; i < NP * NP; ++i) cv[i] *= fac;
 }
 
+inline int lm_nparams(int kind) {
+    return kind == CP_PSF_CIRCULAR_GAUSSIAN ? 5 : kind == CP_PSF_CIRCULAR_MOFFAT ? 6 : 4;
+}
+
 // LMConst from the options; false (with the reason in *why) for a model the
 // kernel does not cover or a bad free mask.
 inline bool lm_const(const cp_psf_desc* psf, uint32_t free_mask, const cp_lm_opts* o, bool have_iv, LMConst& C,
                      int& np, const char** why) {
-    if (!(psf->kind == CP_PSF_CIRCULAR_GAUSSIAN || psf->kind == CP_PSF_CIRCULAR_MOFFAT)) {
-        *why = "batched fit_star supports CircularGaussianPSF and CircularMoffatPSF";
+    if (!(psf->kind == CP_PSF_CIRCULAR_GAUSSIAN || psf->kind == CP_PSF_CIRCULAR_MOFFAT ||
+          psf->kind == CP_PSF_GRIDDED_IMAGE)) {
+        *why = "unknown PSF kind";
         return false;
     }
     if (o->max_iter < 0) {
         *why = "max_iter must be non-negative";
         return false;
     }
-    const int npf = psf->kind == CP_PSF_CIRCULAR_GAUSSIAN ? 5 : 6;
+    const int npf = lm_nparams(psf->kind);
     C = LMConst{};
     C.kind = psf->kind;
     C.npf = npf;

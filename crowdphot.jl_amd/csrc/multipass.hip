@@ -76,14 +76,15 @@ struct BoxSrc {
 };
 
 template <typename T, int KIND>
-__device__ __forceinline__ double model_value(const T* p, const double* full, double py, double px) {
+__device__ __forceinline__ double model_value(const LMModel<T>& Md, const CornerRec<T>& cr, const T* p,
+                                              const double* full, double py, double px) {
     T f, G[6];
-    model_fg<T, KIND>(p, (T)(py - full[0]), (T)(px - full[1]), f, G);
+    model_eval<T, KIND>(Md, cr, p, (T)(py - full[0]), (T)(px - full[1]), f, G);
     return (double)f;
 }
 
 template <typename T, int KIND, int NP>
-__global__ __launch_bounds__(64) void k_mp_level(MPArgs A, const int* __restrict__ list, int cnt) {
+__global__ __launch_bounds__(64) void k_mp_level(MPArgs A, LMModel<T> Md, const int* __restrict__ list, int cnt) {
     extern __shared__ double lds[];
     if ((int)blockIdx.x >= cnt) return;
     const int lane = threadIdx.x;
@@ -105,25 +106,27 @@ __global__ __launch_bounds__(64) void k_mp_level(MPArgs A, const int* __restrict
         full0[k] = fixed_here ? A.fixv[k] : cur[k];
         pc[k] = (T)cur[k];
     }
+    CornerRec<T> cc;
+    model_corners<T, KIND>(Md, cur, cc);
     // stage the box; passes after the first add the star's model back (add_star!)
     for (int m = lane; m < nb; m += 64) {
         const int col = m / by, gy = ylo + m - col * by, gx = xlo + col;
         const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * NY;
         double v = A.R[q];
-        if (!A.first) v += model_value<T, KIND>(pc, cur, gy, gx);
+        if (!A.first) v += model_value<T, KIND>(Md, cc, pc, cur, gy, gx);
         cut[m] = v;
         if (A.iv) ivs[m] = A.iv[q];
     }
     __syncthreads();
     const BoxSrc S{cut, A.iv ? ivs : nullptr, by, ylo, xlo, nb};
     LMOut<NP> R;
-    lm_fit<T, KIND, NP>(A.C, full0, S, R);
+    lm_fit<T, KIND, NP>(A.C, Md, full0, S, R);
     if (R.st.status != 0) {
         // the reference's catch: undo the add-back, mark the star invalid
         if (!A.first)
             for (int m = lane; m < nb; m += 64) {
                 const int col = m / by, gy = ylo + m - col * by, gx = xlo + col;
-                A.R[(size_t)(gy - 1) + (size_t)(gx - 1) * NY] = cut[m] - model_value<T, KIND>(pc, cur, gy, gx);
+                A.R[(size_t)(gy - 1) + (size_t)(gx - 1) * NY] = cut[m] - model_value<T, KIND>(Md, cc, pc, cur, gy, gx);
             }
         if (lane == 0) {
             A.valid[s] = 0;
@@ -135,9 +138,11 @@ __global__ __launch_bounds__(64) void k_mp_level(MPArgs A, const int* __restrict
     T pn[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) pn[k] = (T)R.full[k];
+    CornerRec<T> cn;
+    model_corners<T, KIND>(Md, R.full, cn);
     for (int m = lane; m < nb; m += 64) {
         const int col = m / by, gy = ylo + m - col * by, gx = xlo + col;
-        const double f = model_value<T, KIND>(pn, R.full, gy, gx);
+        const double f = model_value<T, KIND>(Md, cn, pn, R.full, gy, gx);
         const double r = cut[m] - f;
         A.R[(size_t)(gy - 1) + (size_t)(gx - 1) * NY] = r;
         cut[m] = r;
@@ -258,8 +263,9 @@ static __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, flo
 }
 
 template <typename T, int KIND>
-void launch_level(int np, size_t lds, hipStream_t s, const MPArgs& A, const int* list, int cnt) {
-#define CP_MP(NPV) launch_n("k_mp_level", k_mp_level<T, KIND, NPV>, dim3(cnt), dim3(64), lds, s, A, list, cnt)
+void launch_level(int np, size_t lds, hipStream_t s, const MPArgs& A, const LMModel<T>& Md, const int* list,
+                  int cnt) {
+#define CP_MP(NPV) launch_n("k_mp_level", k_mp_level<T, KIND, NPV>, dim3(cnt), dim3(64), lds, s, A, Md, list, cnt)
     switch (np) {
         case 1: CP_MP(1); break;
         case 2: CP_MP(2); break;
@@ -344,6 +350,12 @@ void fit_all_stars_impl(cp_ctx* ctx, const cp_psf_desc* psf, const float* image,
         dK.alloc(ctx, hK.size() * 8);
         CP_HIP(hipMemcpyAsync(dK.get(), hK.data(), hK.size() * 8, hipMemcpyHostToDevice, st));
     }
+
+    const HostPsf HP = make_host_psf(psf);
+    DevPsf DP;
+    upload_psf(ctx, psf, HP, DP);
+    const LMModel<double> Md64{model_const<double>(HP, DP, 0.0), DP.G};
+    const LMModel<float> Md32{model_const<float>(HP, DP, 0.0), DP.G};
 
     MPArgs A{};
     A.C = C;
@@ -430,12 +442,16 @@ void fit_all_stars_impl(cp_ctx* ctx, const cp_psf_desc* psf, const float* image,
         A.last = pass == n_passes;
         for (int l = 1; l <= nlev; ++l) {
             const int* lst = dlist.as<int>() + off[l];
+            const bool f64 = precision == CP_PREC_FP64;
             if (psf->kind == CP_PSF_CIRCULAR_GAUSSIAN) {
-                if (precision == CP_PREC_FP64) launch_level<double, KIND_GAUSS>(np, lds, st, A, lst, cnt[l]);
-                else launch_level<float, KIND_GAUSS>(np, lds, st, A, lst, cnt[l]);
+                if (f64) launch_level<double, KIND_GAUSS>(np, lds, st, A, Md64, lst, cnt[l]);
+                else launch_level<float, KIND_GAUSS>(np, lds, st, A, Md32, lst, cnt[l]);
+            } else if (psf->kind == CP_PSF_CIRCULAR_MOFFAT) {
+                if (f64) launch_level<double, KIND_MOFFAT>(np, lds, st, A, Md64, lst, cnt[l]);
+                else launch_level<float, KIND_MOFFAT>(np, lds, st, A, Md32, lst, cnt[l]);
             } else {
-                if (precision == CP_PREC_FP64) launch_level<double, KIND_MOFFAT>(np, lds, st, A, lst, cnt[l]);
-                else launch_level<float, KIND_MOFFAT>(np, lds, st, A, lst, cnt[l]);
+                if (f64) launch_level<double, KIND_EPSF>(np, lds, st, A, Md64, lst, cnt[l]);
+                else launch_level<float, KIND_EPSF>(np, lds, st, A, Md32, lst, cnt[l]);
             }
         }
         launch(k_mp_validate, dim3(cdiv(n, BLOCK)), dim3(BLOCK), 0, st, n, npf, (const double*)A.prm,

@@ -312,8 +312,8 @@ def fit_all_stars(image, psf, sources, fit_rad, *, n_passes=3, fixed=None, inv_v
     model and gradient with float64 sums and residual image; "fp64")."""
     if n_passes <= 0:
         raise ArgumentError(f"n_passes must be positive, got {n_passes}")
-    if not isinstance(psf, (CircularGaussianPSF, CircularMoffatPSF)):
-        raise ArgumentError("fit_all_stars on the GPU supports CircularGaussianPSF and CircularMoffatPSF")
+    if not isinstance(psf, (CircularGaussianPSF, CircularMoffatPSF, ImagePSF, GriddedPSFModel)):
+        raise ArgumentError(f"unsupported PSF model {type(psf).__name__}")
     if not fit_rad > 0:
         raise ArgumentError("fit_rad must be positive")
     fixed = dict(fixed or {})
@@ -437,8 +437,8 @@ def fit_star_batch(model, cutouts, theta0=None, *, fixed=None, inv_var=None, ctx
     getproperties order (default: the model's own for every cutout).  fixed:
     names of the parameters held at their theta0 values.  Returns
     (theta [n, nparams], cov [n, p, p], stats structured list)."""
-    if not isinstance(model, (CircularGaussianPSF, CircularMoffatPSF)):
-        raise ArgumentError("batched fit_star supports CircularGaussianPSF and CircularMoffatPSF")
+    if not isinstance(model, (CircularGaussianPSF, CircularMoffatPSF, ImagePSF, GriddedPSFModel)):
+        raise ArgumentError(f"unsupported PSF model {type(model).__name__}")
     fixed = set(fixed or ())
     names = model.props
     npf = len(names)

@@ -218,8 +218,9 @@ int cp_build_stamps(cp_ctx* ctx, const float* w, int32_t ny, int32_t nx,
                     int32_t* active, int32_t* n_active, int32_t* pixels, uint8_t* union_mask);
 
 /* ---- batched per-star LM: fit_star over many cutouts ----------------------
- * fit_star src/psf/psf_fitting.jl:229-308 (+ the CircularGaussian / CircularMoffat
- * specialisations :314-451, :772-862) driving lm_irls src/levenberg_marquardt.jl:431-680.
+ * fit_star src/psf/psf_fitting.jl:229-308 (+ the CircularGaussian / CircularMoffat /
+ * ImagePSF / GriddedPSFModel specialisations :314-451, :772-862, :1086-1173,
+ * :1177-1210) driving lm_irls src/levenberg_marquardt.jl:431-680.
  * Cutout c occupies pix[offsets[c] .. offsets[c] + ny[c]*nx[c]) column-major,
  * fit over all its pixels (inds = axes(cutout)), coordinates 1-based within
  * the cutout.  inv_var: NULL or packed like pix.  theta0 / theta_out:
@@ -241,7 +242,7 @@ int cp_fit_star_batch_device(cp_ctx* ctx, const cp_psf_desc* psf,
 
 /* ---- sequential multi-pass photometry: fit_all_stars ---------------------
  * fit_all_stars, src/photometry/psf_photometry_single.jl:267-469 (replaces
- * that Julia function for CircularGaussianPSF / CircularMoffatPSF).
+ * that Julia function; every model kind above).
  * image / inv_var (NULL = unweighted): host float32 ny*nx column-major.
  * theta0: n_stars x nparams(kind) initial parameters in getproperties order
  * (_extract_source_catalog, :152-197: the catalogue's y, x, flux, bkg with

@@ -198,3 +198,101 @@ def test_failures_empty_and_out_of_image(ctx):
     assert r.n_passes == 0 and r.residual.shape == (0, 0)
     with pytest.raises(pkg.ArgumentError, match="position"):
         pkg.fit_all_stars(img, G2, dict(y=[np.nan], x=[3.0]), 5, ctx=ctx)
+
+
+def _epsf(n_nodes):
+    e = pkg.moffat_epsf(fwhm=2.2, beta=2.5, size=41, oversampling=4)
+    if n_nodes == 1:
+        return pkg.GriddedPSFModel([e], [0.0], [0.0])
+    e2 = pkg.moffat_epsf(fwhm=2.7, beta=3.0, size=41, oversampling=4)
+    return pkg.GriddedPSFModel([e, e2, e2, e], [1.0, 1.0, 192.0, 192.0], [1.0, 160.0, 1.0, 160.0])
+
+
+@pytest.mark.parametrize("n_nodes", [1, 4])
+def test_crowded_epsf_matches_oracle(ctx, n_nodes):
+    """ImagePSF / GriddedPSFModel stars (y, x, flux, bkg free), fp64 against
+    the oracle to summation-order rounding, and the mixed path against the
+    oracle's own self-agreement."""
+    psf = _epsf(n_nodes)
+    img, cat = _crowded(psf, seed=17)
+    iv = (1.0 / np.maximum(img, 1.0)).astype(np.float32).astype(np.float64)
+    kw = dict(n_passes=2, inv_var=iv, max_iter=100)
+    o = oracle_fit.fit_all(psf, img, cat, 4, **kw)
+    g = pkg.fit_all_stars(img, psf, cat, 4, precision="fp64", ctx=ctx, **kw)
+    print(f"epsf nodes {n_nodes}: valid {g.valid.sum()}/{len(g.valid)} n_iter {g.n_iter.sum()} vs "
+          f"{o.n_iter.sum()}", flush=True)
+    _compare(g, o, 1e-6)
+    np.testing.assert_allclose(g.residual, o.residual, rtol=1e-5, atol=1e-3)
+    kw = dict(n_passes=2, fixed=dict(bkg=25.0), max_iter=100)
+    o = oracle_fit.fit_all(psf, img, cat, 4, **kw)
+    g = pkg.fit_all_stars(img, psf, cat, 4, ctx=ctx, **kw)
+    rng = np.random.default_rng(0)
+    o3 = oracle_fit.fit_all(psf, img * (1 + 1e-7 * rng.standard_normal(img.shape)), cat, 4, **kw)
+    both = o.valid & o3.valid
+    self_frac = _agree(o3, o, 1e-4, 0.02)[both].mean()
+    diag_frac = {k: (_diag_close(o3, o, k, both).mean() if _diag_close(o3, o, k, both).size else 1.0) - 0.02
+                 for k in DIAG}
+    _compare(g, o, 1e-4, frac=min(0.995, self_frac - 0.01), sig=0.02, diag_frac=diag_frac)
+
+
+def _gridded_gauss():
+    g = np.arange(-5, 6, dtype=float)
+    node = np.exp(pkg.psf.GAUSS_PRE * (g[:, None] ** 2 + g[None, :] ** 2) / 4.0)
+    return pkg.GriddedPSFModel([pkg.ImagePSF(node, oversampling=1, normalize=True)], [0.0], [0.0])
+
+
+@pytest.mark.parametrize("kind", ["gauss", "gridded"])
+def test_isolated_stars_sequential_equals_simultaneous(ctx, kind):
+    """test/photometry/psf_photometry_simultaneous.jl:219-253 on the GPU, both
+    paths in float64."""
+    psf, fixed = (G2, dict(fwhm=2.0, bkg=0.0)) if kind == "gauss" else (_gridded_gauss(), dict(bkg=0.0))
+    ys = np.array([20.3, 40.7, 64.2, 90.6, 110.1])
+    xs = np.array([30.4, 95.2, 60.8, 20.5, 100.9])
+    fl = np.array([650.0, 720.0, 880.0, 610.0, 790.0])
+    img = sim.render_sources(psf, (128, 128), ys, xs, fl, 5)
+    cat = dict(y=ys, x=xs, flux=np.full(5, 400.0))
+    seq = pkg.fit_all_stars(img, psf, cat, 5, n_passes=1, fixed=fixed, max_iter=200, precision="fp64", ctx=ctx)
+    simr = pkg.fit_all_stars_simultaneous(img, psf, cat, 5, fixed=fixed, max_iter=40, inner_iterations=10,
+                                          x_tol=1e-8, model_rad=5, precision="fp64", ctx=ctx)
+    assert simr.valid.all() and seq.valid.all()
+    np.testing.assert_allclose(simr.flux, seq.flux, rtol=1e-7)    # the image travels as float32
+    np.testing.assert_allclose(simr.y, seq.y, rtol=0, atol=1e-7)
+    np.testing.assert_allclose(seq.flux, fl, rtol=1e-6)
+
+
+@pytest.mark.parametrize("n_nodes", [1, 4])
+def test_fit_star_batch_epsf_matches_oracle(ctx, n_nodes):
+    """fit_star over ePSF cutouts (psf_fitting.jl:1086-1210): each cutout
+    holds one star of the model at its own position in the frame of the node
+    grid; (y, x, flux, bkg) free; both precisions against the oracle."""
+    import ctypes as C
+    from _support import f64, oracle, ptr
+    psf = _epsf(n_nodes)
+    rng = np.random.default_rng(5)
+    n = 40
+    cuts, th0 = [], []
+    for i in range(n):
+        s = int(rng.choice([11, 13, 15]))
+        y, x = (s + 1) / 2 + rng.uniform(-0.5, 0.5), (s + 1) / 2 + rng.uniform(-0.5, 0.5)
+        f, b = rng.uniform(500, 5000), rng.uniform(5, 50)
+        c = sim.render_sources(psf, (s, s), [y], [x], [f], 40) + b
+        cuts.append(sim.add_noise(c, 100 + i).astype(np.float32).astype(np.float64))
+        th0.append([y + rng.normal(0, 0.2), x + rng.normal(0, 0.2), f * rng.uniform(0.8, 1.2), b])
+    th0 = np.array(th0)
+    abi = pkg._abi
+    ny = np.array([c.shape[0] for c in cuts], np.int32)
+    off = np.zeros(n, np.int64)
+    off[1:] = np.cumsum(ny.astype(np.int64) ** 2)[:-1]
+    pix = np.concatenate([c.T.ravel() for c in cuts]).astype(np.float32)
+    tho = np.zeros((n, 4))
+    sto = (abi.LMStat * n)()
+    o = abi.lm_opts_default()
+    oracle().or_fit_star_batch(psf.desc(), ptr(pix, abi.f32p), ptr(off, abi.i64p), ptr(ny, abi.i32p),
+                               ptr(ny, abi.i32p), None, ptr(f64(th0), abi.f64p), 0b1111, n, C.byref(o),
+                               ptr(tho, abi.f64p), None, sto, 0)
+    th, cov, st = pkg.fit_star_batch(psf, cuts, th0, ctx=ctx)
+    assert all(s.status == 0 for s in st) and all(s.status == 0 for s in sto)
+    rel = np.abs(th[:, 2] - tho[:, 2]) / tho[:, 2]
+    dp = np.maximum(np.abs(th[:, 0] - tho[:, 0]), np.abs(th[:, 1] - tho[:, 1]))
+    assert np.mean((rel <= 1e-4) & (dp <= 1e-3)) >= 0.95, (rel.max(), dp.max())
+    assert np.median(rel) < 1e-6

@@ -46,9 +46,8 @@ def test_multipass_argument_contract():
         f(img, psf, cat, 5, inv_var=np.ones((3, 3)))
     with pytest.raises(pkg.ArgumentError, match="no parameter"):
         f(img, psf, cat, 5, fixed={"alpha": 1.0})
-    epsf = pkg.ImagePSF(np.ones((9, 9)), oversampling=1)
-    with pytest.raises(pkg.ArgumentError, match="CircularGaussianPSF"):
-        f(img, epsf, cat, 5)
+    with pytest.raises(pkg.ArgumentError, match="unsupported PSF model"):
+        f(img, object(), cat, 5)
 
 
 def test_generator_is_seeded_and_follows_distributions():

@@ -303,3 +303,50 @@ def test_empty_catalogue_and_argument_errors():
         oracle_fit.fit_all(G2, img, dict(y=[5.0], x=[5.0]), 3, fixed=dict(y=1, x=1, fwhm=2, flux=1, bkg=0))
     with pytest.raises(ValueError, match="position"):
         oracle_fit.fit_all(G2, img, dict(y=[np.nan], x=[5.0]), 3)
+
+
+# --------------------------------------------------------------------------
+# the sequential path against the simultaneous one
+# (test/photometry/psf_photometry_simultaneous.jl:219-253, 356-383)
+# --------------------------------------------------------------------------
+def _gridded_gauss():
+    """The reference's single-node ePSF of CircularGaussianPSF(fwhm=2):
+    render(psf) over its extent (+-5 px), unit oversampling, normalised."""
+    g = np.arange(-5, 6, dtype=float)
+    node = np.exp(pkg.psf.GAUSS_PRE * (g[:, None] ** 2 + g[None, :] ** 2) / 4.0)
+    return pkg.GriddedPSFModel([pkg.ImagePSF(node, oversampling=1, normalize=True)], [0.0], [0.0])
+
+
+@pytest.mark.parametrize("kind", ["gauss", "gridded"])
+def test_isolated_stars_sequential_equals_simultaneous(kind):
+    psf, fixed = (G2, dict(fwhm=2.0, bkg=0.0)) if kind == "gauss" else (_gridded_gauss(), dict(bkg=0.0))
+    img, src = _field((128, 128), 5, 15, 10, (600.0, 900.0), seed=42, psf=psf, bkg=20.0, model_radius=5)
+    sub = img - 20.0
+    cat = dict(y=src["y"], x=src["x"], flux=np.full(5, 400.0))
+    seq = oracle_fit.fit_all(psf, sub, cat, 5, n_passes=1, fixed=fixed, max_iter=200)
+    sim_ = oracle_fit.fit(psf, sub, cat["y"], cat["x"], cat["flux"], 5, max_iter=40, inner_iterations=10, x_tol=1e-8,
+                          model_rad=5.0, fixed_bkg=0.0)
+    assert sim_.valid.all() and seq.valid.all()
+    np.testing.assert_allclose(sim_.flux, seq.flux, rtol=1e-10)
+    np.testing.assert_allclose(sim_.y, seq.y, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(sim_.x, seq.x, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(seq.flux, src["flux"], rtol=1e-10)
+
+
+def test_spread_model_sequential_matches_simultaneous():
+    psf = pkg.CircularGaussianPSF(fwhm=2.5)
+    pts = [(20.5, 20.5), (20.5, 68.5), (68.5, 20.5)]
+    ys = [p[0] for p in pts] + [68.5]
+    xs = [p[1] for p in pts] + [68.5]
+    img = sim.render_sources(psf, (90, 90), ys[:3], xs[:3], [4000.0] * 3, 1000)
+    img += sim.render_sources(pkg.CircularGaussianPSF(fwhm=5.0), (90, 90), [68.5], [68.5], [4000.0], 1000)
+    iv = np.full(img.shape, 1 / 20.0)
+    cat = dict(y=ys, x=xs, flux=[4000.0] * 4)
+    seq = oracle_fit.fit_all(psf, img, cat, 7, n_passes=3, fixed=dict(fwhm=2.5, bkg=0.0), max_iter=100, inv_var=iv)
+    simr = oracle_fit.fit(psf, img, ys, xs, [4000.0] * 4, 7, inv_var=iv, max_iter=40, fixed_bkg=0.0, model_rad=0.0)
+    for i in range(3):
+        assert seq.valid[i] and simr.valid[i]
+        assert abs(seq.spread_model[i]) < 3e-3 and abs(simr.spread_model[i]) < 3e-3
+        assert abs(seq.spread_model[i] - simr.spread_model[i]) < 2e-3
+    assert seq.spread_model[3] > 5e-3 and simr.spread_model[3] > 5e-3
+    assert seq.spread_model[3] == pytest.approx(simr.spread_model[3], rel=0.25)
