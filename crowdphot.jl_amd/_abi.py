@@ -250,6 +250,9 @@ SIGNATURES = {
                                              C.POINTER(SimResult)]),
     "cp_build_stamps": (C.c_int, [C.c_void_p, f32p, C.c_int32, C.c_int32, f64p, f64p, f64p, C.c_int32,
                                   C.c_double, i32p, i32p, i32p, i32p, i32p, u8p]),
+    "cp_fit_all_stars_device": (C.c_int, [C.c_void_p, C.POINTER(PsfDesc), C.c_void_p, C.c_void_p, C.c_int32,
+                                          C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_uint32,
+                                          f64p, C.POINTER(LMOpts), C.c_double, C.c_int32, C.POINTER(SimResult)]),
     "cp_fit_all_stars": (C.c_int, [C.c_void_p, C.POINTER(PsfDesc), f32p, f32p, C.c_int32, C.c_int32, f64p,
                                    C.c_int32, C.c_double, C.c_int32, C.c_uint32, f64p, C.POINTER(LMOpts),
                                    C.c_double, C.c_int32, C.POINTER(SimResult)]),

@@ -107,8 +107,9 @@ __device__ __forceinline__ void model_eval(const LMModel<T>& Md, const CornerRec
         r.flux = prm[2];
         r.amp = prm[2];
         T gy, gx, gf;
-        f = star_value<T, KIND_EPSF>(Md.M, r, ddy, ddx, Md.M.grids, &cr) + prm[3];
+        // one bicubic pass gives the value too: gf is the unit-flux profile s
         star_grad<T, KIND_EPSF>(Md.M, r, ddy, ddx, Md.M.grids, &cr, gy, gx, gf);
+        f = fma(prm[2], gf, prm[3]);
         G[0] = gy;
         G[1] = gx;
         G[2] = gf;

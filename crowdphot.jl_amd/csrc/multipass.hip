@@ -21,6 +21,9 @@
 #include <numeric>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
+#include "cp_internal.h"
 #include "diag_host.h"
 #include "lm_core.h"
 #include "psf_host.h"
@@ -277,31 +280,180 @@ void launch_level(int np, size_t lds, hipStream_t s, const MPArgs& A, const LMMo
 #undef CP_MP
 }
 
-// Box of a star centred at (y, x): floor(y - r):ceil(y + r) clamped to the
-// image (psf_photometry_single.jl:353-357, _clamp_inds).
-struct HostBox {
-    long ylo, yhi, xlo, xhi;
-};
-inline HostBox star_box(double y, double x, double r, int ny, int nx) {
-    return {std::max((long)std::floor(y - r), 1L), std::min((long)std::ceil(y + r), (long)ny),
-            std::max((long)std::floor(x - r), 1L), std::min((long)std::ceil(x + r), (long)nx)};
+// ---------------------------------------------------------------------------
+// Per-pass schedule, built on the device
+// ---------------------------------------------------------------------------
+// Box of every valid star (floor(y - r):ceil(y + r) clamped to the image,
+// psf_photometry_single.jl:353-357) and its flux-order key: an unsigned
+// integer that orders like Julia's isless on the flux with NaN above
+// everything, so a stable descending sort is sortperm(flux; rev = true).
+// Stars whose box has fewer than 3 pixels become invalid (:358); invalid
+// stars take key 0, below every valid key.  *bad = 1 + a star whose position
+// is not finite (the reference's floor(Int, NaN) throws).
+static __global__ void k_mp_boxes(int n, int npf, const double* __restrict__ prm, uint8_t* __restrict__ valid,
+                                  double r, int ny, int nx, int4* __restrict__ box,
+                                  unsigned long long* __restrict__ key, int* __restrict__ bad) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    key[i] = 0ull;
+    if (!valid[i]) return;
+    const double* m = prm + (size_t)i * npf;
+    const double y = m[0], x = m[1];
+    if (!(fabs(y) < 1e9 && fabs(x) < 1e9)) {
+        *bad = i + 1;
+        return;
+    }
+    const long ylo = max((long)floor(y - r), 1L), yhi = min((long)ceil(y + r), (long)ny);
+    const long xlo = max((long)floor(x - r), 1L), xhi = min((long)ceil(x + r), (long)nx);
+    const long by = yhi - ylo + 1, bx = xhi - xlo + 1;
+    if (by <= 0 || bx <= 0 || by * bx < 3) {
+        valid[i] = 0;
+        return;
+    }
+    box[i] = make_int4((int)ylo, (int)yhi, (int)xlo, (int)xhi);
+    const double f = m[npf - 2];
+    const unsigned long long b = (unsigned long long)__double_as_longlong(f);
+    key[i] = isnan(f) ? ~0ull : ((b >> 63) ? ~b : (b | (1ull << 63)));
 }
 
-void fit_all_stars_impl(cp_ctx* ctx, const cp_psf_desc* psf, const float* image, const float* inv_var, int ny,
-                        int nx, const double* theta0, int n, double fit_rad, int n_passes, uint32_t free_mask,
-                        const double* fixed_values, const cp_lm_opts* o, double spread_model_fwhm, int precision,
-                        cp_sim_result* out) {
-    CP_ARG(psf && o && out && image, "NULL argument");
+// Cell of the star at flux rank k: the W x W cell of its box's lower corner
+// (boxes are at most W pixels a side, so overlapping boxes sit in the same or
+// adjacent cells); invalid ranks take the sentinel ncell.  Level 1 to start.
+static __global__ void k_mp_cells(int n, const int* __restrict__ order, const unsigned long long* __restrict__ skey,
+                                  const int4* __restrict__ box, int W, int ncy, int ncell, int* __restrict__ cell,
+                                  int* __restrict__ rank_seq, int* __restrict__ lev) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    rank_seq[k] = k;
+    if (skey[k] == 0ull) {
+        cell[k] = ncell;
+        lev[k] = 0;
+        return;
+    }
+    const int4 b = box[order[k]];
+    cell[k] = (b.x - 1) / W + ((b.z - 1) / W) * ncy;
+    lev[k] = 1;
+}
+
+// Start / end of each cell's run in the cell-sorted rank list.
+static __global__ void k_mp_cell_bounds(int n, const int* __restrict__ cs, int ncell, int* __restrict__ cstart,
+                                        int* __restrict__ cend) {
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const int c = cs[s];
+    if (c >= ncell) return;
+    if (s == 0 || cs[s - 1] != c) cstart[c] = s;
+    if (s == n - 1 || cs[s + 1] != c) cend[c] = s + 1;
+}
+
+// One relaxation sweep of level(k) = 1 + max level of the earlier-ranked
+// stars whose boxes overlap k's.  Levels only grow towards that longest-path
+// value, so sweeps may read neighbours' levels from before or after their
+// update in the same sweep; a sweep that changes nothing has reached it.
+static __global__ void k_mp_relax(int n, const int* __restrict__ order, const int4* __restrict__ box,
+                                  const int* __restrict__ cell, const int* __restrict__ cstart,
+                                  const int* __restrict__ cend, const int* __restrict__ cranks, int ncy, int ncx,
+                                  int* lev, int* __restrict__ changed) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const int c = cell[k];
+    if (c >= ncy * ncx) return;
+    const int4 bj = box[order[k]];
+    const int cy = c % ncy, cx = c / ncy;
+    int best = 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = cx + dx;
+        if (xx < 0 || xx >= ncx) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = cy + dy;
+            if (yy < 0 || yy >= ncy) continue;
+            const int cc = yy + xx * ncy;
+            const int e = cend[cc];
+            for (int s = cstart[cc]; s < e; ++s) {
+                const int r = cranks[s];
+                if (r >= k) break;   // ranks ascend within a cell
+                const int4 bi = box[order[r]];
+                if (bi.y < bj.x || bj.y < bi.x || bi.w < bj.z || bj.w < bi.z) continue;
+                best = max(best, __atomic_load_n(lev + r, __ATOMIC_RELAXED));
+            }
+        }
+    }
+    if (best + 1 > lev[k]) {
+        __atomic_store_n(lev + k, best + 1, __ATOMIC_RELAXED);
+        *changed = 1;
+    }
+}
+
+// First position of each level in the level-sorted star list.
+static __global__ void k_mp_level_bounds(int n, const int* __restrict__ ls, int* __restrict__ lstart) {
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const int l = ls[s];
+    if (l > 0 && (s == 0 || ls[s - 1] != l)) lstart[l] = s;
+}
+
+static __global__ void k_iota(int n, int* __restrict__ v) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
+// Result columns from the parameter and error rows.
+static __global__ void k_mp_results(int n, int npf, const double* __restrict__ prm, const double* __restrict__ err,
+                                    double* __restrict__ cols) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const size_t N = (size_t)n;
+    const double* m = prm + (size_t)i * npf;
+    const double* e = err + (size_t)i * npf;
+    cols[OUT_Y * N + i] = m[0];
+    cols[OUT_X * N + i] = m[1];
+    cols[OUT_FLUX * N + i] = m[npf - 2];
+    cols[OUT_BKG * N + i] = m[npf - 1];
+    cols[OUT_Y_ERR * N + i] = e[0];
+    cols[OUT_X_ERR * N + i] = e[1];
+    cols[OUT_FLUX_ERR * N + i] = e[npf - 2];
+    cols[OUT_BKG_ERR * N + i] = e[npf - 1];
+}
+
+// Initial errors (NaN free, 0 fixed) and state.
+static __global__ void k_mp_init(int n, int npf, uint32_t free_mask, double* __restrict__ err,
+                                 uint8_t* __restrict__ valid, uint8_t* __restrict__ conv, int* __restrict__ n_iter,
+                                 double* __restrict__ cols) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < npf; ++k) err[(size_t)i * npf + k] = ((free_mask >> k) & 1u) ? NAN : 0.0;
+    valid[i] = 1;
+    conv[i] = 0;
+    n_iter[i] = 0;
+    const size_t N = (size_t)n;
+    cols[OUT_CHISQ * N + i] = 0.0;
+    for (int k = OUT_QFIT; k < OUT_F64; ++k) cols[k * N + i] = NAN;
+}
+
+struct Sorter {
+    cp_ctx* ctx;
+    DevBuf tmp;
+    void* get(size_t b) {
+        if (tmp.bytes() < b) tmp.alloc(ctx, std::max<size_t>(b, 1 << 20));
+        return tmp.get();
+    }
+};
+
+void fit_all_stars_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_image, const float* d_iv, int ny,
+                          int nx, const double* d_theta0, int n, double fit_rad, int n_passes, uint32_t free_mask,
+                          const double* fixed_values, const cp_lm_opts* o, double spread_model_fwhm, int precision,
+                          cp_sim_result* out) {
+    CP_ARG(psf && o && out && d_image, "NULL argument");
     CP_ARG(ny > 0 && nx > 0, "image must be non-empty");
-    CP_ARG(n >= 0 && (theta0 || n == 0), "bad star catalogue");
+    CP_ARG(n >= 0 && (d_theta0 || n == 0), "bad star catalogue");
     CP_ARG(n_passes > 0, "n_passes must be positive, got " + std::to_string(n_passes));
     CP_ARG(fit_rad > 0 && std::isfinite(fit_rad), "fit_rad must be positive");
     CP_ARG(precision == CP_PREC_MIXED || precision == CP_PREC_FP64, "unknown precision");
     LMConst C;
     int np = 0;
     const char* why = "";
-    CP_ARG(lm_const(psf, free_mask, o, inv_var != nullptr, C, np, &why), why);
-    const int npf = C.npf, rf = npf - 2, rb = npf - 1;
+    CP_ARG(lm_const(psf, free_mask, o, d_iv != nullptr, C, np, &why), why);
+    const int npf = C.npf;
     const long W = 2 * (long)std::ceil(fit_rad) + 2;
     CP_ARG(W * W * MP_BYTES_PER_PIXEL <= MP_LDS_MAX, "fit_rad too large for the per-star box (at most " +
                                                          std::to_string((int)std::sqrt(MP_LDS_MAX / MP_BYTES_PER_PIXEL) / 2 - 1) + ")");
@@ -310,39 +462,40 @@ void fit_all_stars_impl(cp_ctx* ctx, const cp_psf_desc* psf, const float* image,
         out->n_passes = 0;
         return;
     }
+    const HostPsf HP = make_host_psf(psf);
+    DevPsf DP;
+    upload_psf(ctx, psf, HP, DP);
+    const LMModel<double> Md64{model_const<double>(HP, DP, 0.0), DP.G};
+    const LMModel<float> Md32{model_const<float>(HP, DP, 0.0), DP.G};
     const int maxb = (int)(W * W);
     const size_t lds = (size_t)maxb * MP_BYTES_PER_PIXEL;
     hipStream_t st = ctx->stream;
-    const size_t NP_ = (size_t)ny * nx;
+    const size_t NP_ = (size_t)ny * nx, N = (size_t)n;
+    const unsigned gn = cdiv(n, BLOCK);
 
-    // host state mirrored from the device between passes
-    std::vector<double> prm(theta0, theta0 + (size_t)n * npf), err((size_t)n * npf);
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < npf; ++k) err[(size_t)i * npf + k] = ((free_mask >> k) & 1u) ? NAN : 0.0;
-    std::vector<uint8_t> valid(n, 1);
-    std::vector<double> diag0((size_t)6 * n, NAN);
-
-    DevBuf dR(ctx, NP_ * 8), dimg(ctx, NP_ * 4), div, dprm(ctx, prm.size() * 8), derr(ctx, err.size() * 8),
-        dbox(ctx, (size_t)n * sizeof(int4)), dvalid(ctx, n), dconv(ctx, n), dchi(ctx, (size_t)n * 8),
-        dnit(ctx, (size_t)n * 4), dfail(ctx, 4), ddiag(ctx, (size_t)6 * n * 8), dlist(ctx, (size_t)n * 4), dK;
-    CP_HIP(hipMemcpyAsync(dimg.get(), image, NP_ * 4, hipMemcpyHostToDevice, st));
-    if (inv_var) {
-        div.alloc(ctx, NP_ * 4);
-        CP_HIP(hipMemcpyAsync(div.get(), inv_var, NP_ * 4, hipMemcpyHostToDevice, st));
-    }
+    DevBuf dR(ctx, NP_ * 8), dprm(ctx, N * npf * 8), derr(ctx, N * npf * 8), dbox(ctx, N * sizeof(int4)),
+        dvalid(ctx, N), dconv(ctx, N), dnit(ctx, N * 4), dfail(ctx, 8), dcols(ctx, N * OUT_F64 * 8), dK;
+    // schedule buffers
+    const int Wi = (int)W, ncy = (ny + Wi - 1) / Wi, ncx = (nx + Wi - 1) / Wi, ncell = ncy * ncx;
+    DevBuf dkey(ctx, N * 8), dskey(ctx, N * 8), diota(ctx, N * 4), dorder(ctx, N * 4), dcell(ctx, N * 4),
+        dcell_s(ctx, N * 4), drank(ctx, N * 4), dcrank(ctx, N * 4), dlev(ctx, N * 4), dlev_s(ctx, N * 4),
+        dlist(ctx, N * 4), dcstart(ctx, (size_t)ncell * 4), dcend(ctx, (size_t)ncell * 4),
+        dlstart(ctx, (N + 2) * 4), dflag(ctx, 8);
+    Sorter sorter{ctx, {}};
+    launch(k_iota, dim3(gn), dim3(BLOCK), 0, st, n, diota.as<int>());
     launch(k_f32_to_f64, dim3((unsigned)std::min<size_t>(cdiv((int64_t)NP_, BLOCK), 65536)), dim3(BLOCK), 0, st,
-           (int64_t)NP_, (const float*)dimg.as<float>(), dR.as<double>());
-    CP_HIP(hipMemcpyAsync(dprm.get(), prm.data(), prm.size() * 8, hipMemcpyHostToDevice, st));
-    CP_HIP(hipMemcpyAsync(derr.get(), err.data(), err.size() * 8, hipMemcpyHostToDevice, st));
-    CP_HIP(hipMemcpyAsync(ddiag.get(), diag0.data(), diag0.size() * 8, hipMemcpyHostToDevice, st));
-    CP_HIP(hipMemsetAsync(dconv.get(), 0, n, st));
-    CP_HIP(hipMemsetAsync(dchi.get(), 0, (size_t)n * 8, st));
-    CP_HIP(hipMemsetAsync(dnit.get(), 0, (size_t)n * 4, st));
-    CP_HIP(hipMemsetAsync(dfail.get(), 0, 4, st));
+           (int64_t)NP_, d_image, dR.as<double>());
+    CP_HIP(hipMemcpyAsync(dprm.get(), d_theta0, N * npf * 8, hipMemcpyDeviceToDevice, st));
+    launch(k_mp_init, dim3(gn), dim3(BLOCK), 0, st, n, npf, free_mask, derr.as<double>(), dvalid.as<uint8_t>(),
+           dconv.as<uint8_t>(), dnit.as<int>(), dcols.as<double>());
+    CP_HIP(hipMemsetAsync(dfail.get(), 0, 8, st));
 
     // spread_model reference disk, sized from star 1 (:320-323)
     const int kh = (int)std::ceil(fit_rad);
-    const double sfw = spread_model_fwhm > 0 ? spread_model_fwhm : spread_fwhm_host(psf, theta0[0], theta0[1]);
+    double yx0[2];
+    CP_HIP(hipMemcpyAsync(yx0, d_theta0, 16, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipStreamSynchronize(st));
+    const double sfw = spread_model_fwhm > 0 ? spread_model_fwhm : spread_fwhm_host(psf, yx0[0], yx0[1]);
     std::vector<double> hK;
     if (std::isfinite(sfw) && sfw > 0) {
         hK.resize((size_t)(2 * kh + 1) * (2 * kh + 1));
@@ -351,19 +504,13 @@ void fit_all_stars_impl(cp_ctx* ctx, const cp_psf_desc* psf, const float* image,
         CP_HIP(hipMemcpyAsync(dK.get(), hK.data(), hK.size() * 8, hipMemcpyHostToDevice, st));
     }
 
-    const HostPsf HP = make_host_psf(psf);
-    DevPsf DP;
-    upload_psf(ctx, psf, HP, DP);
-    const LMModel<double> Md64{model_const<double>(HP, DP, 0.0), DP.G};
-    const LMModel<float> Md32{model_const<float>(HP, DP, 0.0), DP.G};
-
     MPArgs A{};
     A.C = C;
     A.free_mask = free_mask;
     for (int k = 0; k < 6; ++k) A.fixv[k] = (fixed_values && k < npf) ? fixed_values[k] : NAN;
     A.R = dR.as<double>();
-    A.image = dimg.as<float>();
-    A.iv = inv_var ? div.as<float>() : nullptr;
+    A.image = d_image;
+    A.iv = d_iv;
     A.ny = ny;
     A.n = n;
     A.npf = npf;
@@ -372,135 +519,123 @@ void fit_all_stars_impl(cp_ctx* ctx, const cp_psf_desc* psf, const float* image,
     A.box = dbox.as<int4>();
     A.valid = dvalid.as<uint8_t>();
     A.conv = dconv.as<uint8_t>();
-    A.chisq = dchi.as<double>();
+    A.chisq = dcols.as<double>() + OUT_CHISQ * N;
     A.n_iter = dnit.as<int>();
     A.n_failed = dfail.as<int>();
     A.K = hK.empty() ? nullptr : dK.as<double>();
     A.kh = kh;
-    A.diag = ddiag.as<double>();
+    A.diag = dcols.as<double>() + OUT_QFIT * N;
     A.maxb = maxb;
 
-    std::vector<int> order(n), level(n), lvmap(NP_, 0), cnt, list(n);
-    std::vector<int4> box(n);
+    int cell_bits = 1;
+    while ((1 << cell_bits) <= ncell) ++cell_bits;
+    std::vector<int> lstart;
     for (int pass = 1; pass <= n_passes; ++pass) {
-        if (pass > 1) {
-            CP_HIP(hipMemcpyAsync(prm.data(), dprm.get(), prm.size() * 8, hipMemcpyDeviceToHost, st));
-            CP_HIP(hipMemcpyAsync(valid.data(), dvalid.get(), n, hipMemcpyDeviceToHost, st));
+        // boxes, flux order (stable descending sort of the keys)
+        CP_HIP(hipMemsetAsync(dflag.get(), 0, 8, st));
+        launch(k_mp_boxes, dim3(gn), dim3(BLOCK), 0, st, n, npf, (const double*)A.prm, A.valid, fit_rad, ny, nx,
+               dbox.as<int4>(), dkey.as<unsigned long long>(), dflag.as<int>() + 1);
+        size_t b = 0;
+        CP_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b, dkey.as<unsigned long long>(),
+                                                            dskey.as<unsigned long long>(), diota.as<int>(),
+                                                            dorder.as<int>(), n, 0, 64, st));
+        CP_HIP(hipcub::DeviceRadixSort::SortPairsDescending(sorter.get(b), b, dkey.as<unsigned long long>(),
+                                                            dskey.as<unsigned long long>(), diota.as<int>(),
+                                                            dorder.as<int>(), n, 0, 64, st));
+        // cells of the ordered stars, grouped by cell with ranks ascending
+        launch(k_mp_cells, dim3(gn), dim3(BLOCK), 0, st, n, (const int*)dorder.as<int>(),
+               (const unsigned long long*)dskey.as<unsigned long long>(), (const int4*)A.box, Wi, ncy, ncell,
+               dcell.as<int>(), drank.as<int>(), dlev.as<int>());
+        b = 0;
+        CP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, dcell.as<int>(), dcell_s.as<int>(), drank.as<int>(),
+                                                  dcrank.as<int>(), n, 0, cell_bits, st));
+        CP_HIP(hipcub::DeviceRadixSort::SortPairs(sorter.get(b), b, dcell.as<int>(), dcell_s.as<int>(),
+                                                  drank.as<int>(), dcrank.as<int>(), n, 0, cell_bits, st));
+        CP_HIP(hipMemsetAsync(dcstart.get(), 0, (size_t)ncell * 4, st));
+        CP_HIP(hipMemsetAsync(dcend.get(), 0, (size_t)ncell * 4, st));
+        launch(k_mp_cell_bounds, dim3(gn), dim3(BLOCK), 0, st, n, (const int*)dcell_s.as<int>(), ncell,
+               dcstart.as<int>(), dcend.as<int>());
+        int bad = 0;
+        CP_HIP(hipMemcpyAsync(&bad, dflag.as<int>() + 1, 4, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipStreamSynchronize(st));
+        CP_ARG(bad == 0, "star " + std::to_string(bad) + " has a non-finite or out-of-range position");
+        // levels: relaxation sweeps until nothing changes
+        for (int sweep = 0;; ++sweep) {
+            CP_ARG(sweep <= n, "level relaxation did not settle");
+            CP_HIP(hipMemsetAsync(dflag.get(), 0, 4, st));
+            launch(k_mp_relax, dim3(gn), dim3(BLOCK), 0, st, n, (const int*)dorder.as<int>(), (const int4*)A.box,
+                   (const int*)dcell.as<int>(), (const int*)dcstart.as<int>(), (const int*)dcend.as<int>(),
+                   (const int*)dcrank.as<int>(), ncy, ncx, dlev.as<int>(), dflag.as<int>());
+            int changed = 0;
+            CP_HIP(hipMemcpyAsync(&changed, dflag.get(), 4, hipMemcpyDeviceToHost, st));
+            CP_HIP(hipStreamSynchronize(st));
+            if (!changed) break;
+        }
+        // stars grouped by level (level 0: invalid, first)
+        int lbits = 1;
+        while ((1 << lbits) <= n + 1) ++lbits;
+        b = 0;
+        CP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, dlev.as<int>(), dlev_s.as<int>(), dorder.as<int>(),
+                                                  dlist.as<int>(), n, 0, lbits, st));
+        CP_HIP(hipcub::DeviceRadixSort::SortPairs(sorter.get(b), b, dlev.as<int>(), dlev_s.as<int>(),
+                                                  dorder.as<int>(), dlist.as<int>(), n, 0, lbits, st));
+        launch(k_mp_level_bounds, dim3(gn), dim3(BLOCK), 0, st, n, (const int*)dlev_s.as<int>(),
+               dlstart.as<int>());
+        int nlev = 0;
+        CP_HIP(hipMemcpyAsync(&nlev, dlev_s.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipStreamSynchronize(st));
+        lstart.assign(nlev + 2, n);
+        if (nlev > 0) {
+            CP_HIP(hipMemcpyAsync(lstart.data() + 1, dlstart.as<int>() + 1, (size_t)nlev * 4, hipMemcpyDeviceToHost,
+                                  st));
             CP_HIP(hipStreamSynchronize(st));
         }
-        // flux order, brightest first; sortperm(rev = true) puts NaN first
-        std::iota(order.begin(), order.end(), 0);
-        auto key = [&](int i) {
-            const double f = prm[(size_t)i * npf + rf];
-            return std::isnan(f) ? INFINITY : f;
-        };
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) > key(b); });
-        // levels over a per-pixel map of the highest level that covers it
-        int nlev = 0;
-        for (int idx : order) {
-            level[idx] = 0;
-            if (!valid[idx]) continue;
-            const double y = prm[(size_t)idx * npf], x = prm[(size_t)idx * npf + 1];
-            CP_ARG(std::fabs(y) < 1e9 && std::fabs(x) < 1e9,
-                   "star " + std::to_string(idx + 1) + " has a non-finite or out-of-range position");
-            const HostBox b = star_box(y, x, fit_rad, ny, nx);
-            const long by = b.yhi - b.ylo + 1, bx = b.xhi - b.xlo + 1;
-            if (by <= 0 || bx <= 0 || by * bx < 3) {   // too few pixels (:358)
-                valid[idx] = 0;
-                continue;
-            }
-            int L = 0;
-            for (long gx = b.xlo; gx <= b.xhi; ++gx)
-                for (long gy = b.ylo; gy <= b.yhi; ++gy) L = std::max(L, lvmap[(size_t)(gy - 1) + (size_t)(gx - 1) * ny]);
-            ++L;
-            for (long gx = b.xlo; gx <= b.xhi; ++gx)
-                for (long gy = b.ylo; gy <= b.yhi; ++gy) lvmap[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] = L;
-            level[idx] = L;
-            box[idx] = make_int4((int)b.ylo, (int)b.yhi, (int)b.xlo, (int)b.xhi);
-            nlev = std::max(nlev, L);
-        }
-        for (int idx = 0; idx < n; ++idx)   // clear the map for the next pass
-            if (level[idx]) {
-                const int4 b = box[idx];
-                for (int gx = b.z; gx <= b.w; ++gx)
-                    for (int gy = b.x; gy <= b.y; ++gy) lvmap[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] = 0;
-            }
-        cnt.assign(nlev + 2, 0);
-        for (int idx = 0; idx < n; ++idx) cnt[level[idx]]++;
-        std::vector<int> off(nlev + 2, 0);
-        for (int l = 1; l <= nlev; ++l) off[l + 1] = off[l] + cnt[l];
-        {
-            std::vector<int> fill(off);
-            for (int idx = 0; idx < n; ++idx)
-                if (level[idx]) list[fill[level[idx]]++] = idx;
-        }
-        const int total = nlev ? off[nlev + 1] : 0;
-        CP_HIP(hipMemcpyAsync(dbox.get(), box.data(), (size_t)n * sizeof(int4), hipMemcpyHostToDevice, st));
-        CP_HIP(hipMemcpyAsync(dvalid.get(), valid.data(), n, hipMemcpyHostToDevice, st));
-        if (total) CP_HIP(hipMemcpyAsync(dlist.get(), list.data(), (size_t)total * 4, hipMemcpyHostToDevice, st));
+        lstart[nlev + 1] = n;
         A.first = pass == 1;
         A.last = pass == n_passes;
+        const bool f64 = precision == CP_PREC_FP64;
         for (int l = 1; l <= nlev; ++l) {
-            const int* lst = dlist.as<int>() + off[l];
-            const bool f64 = precision == CP_PREC_FP64;
+            const int* lst = dlist.as<int>() + lstart[l];
+            const int cnt = lstart[l + 1] - lstart[l];
+            if (cnt <= 0) continue;
             if (psf->kind == CP_PSF_CIRCULAR_GAUSSIAN) {
-                if (f64) launch_level<double, KIND_GAUSS>(np, lds, st, A, Md64, lst, cnt[l]);
-                else launch_level<float, KIND_GAUSS>(np, lds, st, A, Md32, lst, cnt[l]);
+                if (f64) launch_level<double, KIND_GAUSS>(np, lds, st, A, Md64, lst, cnt);
+                else launch_level<float, KIND_GAUSS>(np, lds, st, A, Md32, lst, cnt);
             } else if (psf->kind == CP_PSF_CIRCULAR_MOFFAT) {
-                if (f64) launch_level<double, KIND_MOFFAT>(np, lds, st, A, Md64, lst, cnt[l]);
-                else launch_level<float, KIND_MOFFAT>(np, lds, st, A, Md32, lst, cnt[l]);
+                if (f64) launch_level<double, KIND_MOFFAT>(np, lds, st, A, Md64, lst, cnt);
+                else launch_level<float, KIND_MOFFAT>(np, lds, st, A, Md32, lst, cnt);
             } else {
-                if (f64) launch_level<double, KIND_EPSF>(np, lds, st, A, Md64, lst, cnt[l]);
-                else launch_level<float, KIND_EPSF>(np, lds, st, A, Md32, lst, cnt[l]);
+                if (f64) launch_level<double, KIND_EPSF>(np, lds, st, A, Md64, lst, cnt);
+                else launch_level<float, KIND_EPSF>(np, lds, st, A, Md32, lst, cnt);
             }
         }
-        launch(k_mp_validate, dim3(cdiv(n, BLOCK)), dim3(BLOCK), 0, st, n, npf, (const double*)A.prm,
-               (const uint8_t*)A.conv, A.valid);
-        // the host copies above are pageable and reused next pass
-        CP_HIP(hipStreamSynchronize(st));
+        launch(k_mp_validate, dim3(gn), dim3(BLOCK), 0, st, n, npf, (const double*)A.prm, (const uint8_t*)A.conv,
+               A.valid);
     }
 
-    // results
-    std::vector<uint8_t> conv(n);
-    std::vector<double> chisq(n), diag((size_t)6 * n);
-    std::vector<int> nit(n);
-    int nfail = 0;
-    CP_HIP(hipMemcpyAsync(prm.data(), dprm.get(), prm.size() * 8, hipMemcpyDeviceToHost, st));
-    CP_HIP(hipMemcpyAsync(err.data(), derr.get(), err.size() * 8, hipMemcpyDeviceToHost, st));
-    CP_HIP(hipMemcpyAsync(valid.data(), dvalid.get(), n, hipMemcpyDeviceToHost, st));
-    CP_HIP(hipMemcpyAsync(conv.data(), dconv.get(), n, hipMemcpyDeviceToHost, st));
-    CP_HIP(hipMemcpyAsync(chisq.data(), dchi.get(), (size_t)n * 8, hipMemcpyDeviceToHost, st));
-    CP_HIP(hipMemcpyAsync(nit.data(), dnit.get(), (size_t)n * 4, hipMemcpyDeviceToHost, st));
-    CP_HIP(hipMemcpyAsync(diag.data(), ddiag.get(), diag.size() * 8, hipMemcpyDeviceToHost, st));
-    CP_HIP(hipMemcpyAsync(&nfail, dfail.get(), 4, hipMemcpyDeviceToHost, st));
+    // results: columns on the device, one staged read-back
+    launch(k_mp_results, dim3(gn), dim3(BLOCK), 0, st, n, npf, (const double*)A.prm, (const double*)A.err,
+           dcols.as<double>());
+    double* host_cols[OUT_F64] = {out->y,        out->x,         out->flux,    out->bkg,     out->y_err,
+                                  out->x_err,    out->flux_err,  out->bkg_err, out->chisq,   out->qfit,
+                                  out->qfit_expected, out->qfit_z, out->crowding, out->spread_model,
+                                  out->spread_model_err};
+    std::vector<D2HSeg> segs;
+    for (int k = 0; k < OUT_F64; ++k)
+        if (host_cols[k]) segs.push_back({host_cols[k], dcols.as<double>() + k * N, N * 8});
+    if (out->n_iter) segs.push_back({out->n_iter, dnit.get(), N * 4});
+    if (out->converged) segs.push_back({out->converged, dconv.get(), N});
+    if (out->valid) segs.push_back({out->valid, dvalid.get(), N});
+    DevBuf d32;
     if (out->residual) {
-        DevBuf d32(ctx, NP_ * 4);
+        d32.alloc(ctx, NP_ * 4);
         launch(k_f64_to_f32, dim3((unsigned)std::min<size_t>(cdiv((int64_t)NP_, BLOCK), 65536)), dim3(BLOCK), 0,
                st, (int64_t)NP_, (const double*)dR.as<double>(), d32.as<float>());
-        CP_HIP(hipMemcpyAsync(out->residual, d32.get(), NP_ * 4, hipMemcpyDeviceToHost, st));
-        CP_HIP(hipStreamSynchronize(st));
+        segs.push_back({out->residual, d32.get(), NP_ * 4});
     }
-    CP_HIP(hipStreamSynchronize(st));
-    double* dcols[6] = {out->qfit, out->qfit_expected, out->qfit_z, out->crowding, out->spread_model,
-                        out->spread_model_err};
-    for (int i = 0; i < n; ++i) {
-        const double* m = &prm[(size_t)i * npf];
-        const double* e = &err[(size_t)i * npf];
-        if (out->y) out->y[i] = m[0];
-        if (out->x) out->x[i] = m[1];
-        if (out->flux) out->flux[i] = m[rf];
-        if (out->bkg) out->bkg[i] = m[rb];
-        if (out->y_err) out->y_err[i] = e[0];
-        if (out->x_err) out->x_err[i] = e[1];
-        if (out->flux_err) out->flux_err[i] = e[rf];
-        if (out->bkg_err) out->bkg_err[i] = e[rb];
-        if (out->chisq) out->chisq[i] = chisq[i];
-        if (out->converged) out->converged[i] = conv[i];
-        if (out->valid) out->valid[i] = valid[i];
-        if (out->n_iter) out->n_iter[i] = nit[i];
-        for (int k = 0; k < 6; ++k)
-            if (dcols[k]) dcols[k][i] = diag[(size_t)k * n + i];
-    }
+    int nfail = 0;
+    segs.push_back({&nfail, dfail.get(), 4});
+    staged_d2h(ctx, st, segs);
     out->n_passes = n_passes;
     out->n_failed = nfail;
 }
@@ -510,15 +645,44 @@ void fit_all_stars_impl(cp_ctx* ctx, const cp_psf_desc* psf, const float* image,
 
 using namespace cp;
 
-extern "C" int cp_fit_all_stars(cp_ctx* ctx, const cp_psf_desc* psf, const float* image, const float* inv_var,
-                                int32_t ny, int32_t nx, const double* theta0, int32_t n_stars, double fit_rad,
-                                int32_t n_passes, uint32_t free_mask, const double* fixed_values,
-                                const cp_lm_opts* opts, double spread_model_fwhm, int32_t precision,
-                                cp_sim_result* out) {
+extern "C" {
+
+int cp_fit_all_stars_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_image, const float* d_inv_var,
+                            int32_t ny, int32_t nx, const double* d_theta0, int32_t n_stars, double fit_rad,
+                            int32_t n_passes, uint32_t free_mask, const double* fixed_values,
+                            const cp_lm_opts* opts, double spread_model_fwhm, int32_t precision,
+                            cp_sim_result* out) {
     if (!ctx) return CP_ESTATE;
     return guarded(ctx, [&] {
         CP_HIP(hipSetDevice(ctx->device));
-        fit_all_stars_impl(ctx, psf, image, inv_var, ny, nx, theta0, n_stars, fit_rad, n_passes, free_mask,
-                           fixed_values, opts, spread_model_fwhm, precision, out);
+        fit_all_stars_device(ctx, psf, d_image, d_inv_var, ny, nx, d_theta0, n_stars, fit_rad, n_passes, free_mask,
+                             fixed_values, opts, spread_model_fwhm, precision, out);
     });
 }
+
+int cp_fit_all_stars(cp_ctx* ctx, const cp_psf_desc* psf, const float* image, const float* inv_var, int32_t ny,
+                     int32_t nx, const double* theta0, int32_t n_stars, double fit_rad, int32_t n_passes,
+                     uint32_t free_mask, const double* fixed_values, const cp_lm_opts* opts,
+                     double spread_model_fwhm, int32_t precision, cp_sim_result* out) {
+    if (!ctx) return CP_ESTATE;
+    return guarded(ctx, [&] {
+        CP_ARG(image && ny > 0 && nx > 0 && n_stars >= 0 && (theta0 || n_stars == 0), "bad arguments");
+        CP_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const size_t P = (size_t)ny * nx;
+        const int npf = lm_nparams(psf ? psf->kind : 0);
+        DevBuf dimg(ctx, P * 4), div, dth(ctx, (size_t)std::max(n_stars, 1) * npf * 8);
+        CP_HIP(hipMemcpyAsync(dimg.get(), image, P * 4, hipMemcpyHostToDevice, st));
+        if (inv_var) {
+            div.alloc(ctx, P * 4);
+            CP_HIP(hipMemcpyAsync(div.get(), inv_var, P * 4, hipMemcpyHostToDevice, st));
+        }
+        if (n_stars) CP_HIP(hipMemcpyAsync(dth.get(), theta0, (size_t)n_stars * npf * 8, hipMemcpyHostToDevice, st));
+        fit_all_stars_device(ctx, psf, dimg.as<float>(), inv_var ? div.as<float>() : nullptr, ny, nx,
+                             dth.as<double>(), n_stars, fit_rad, n_passes, free_mask, fixed_values, opts,
+                             spread_model_fwhm, precision, out);
+        CP_HIP(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"

@@ -264,6 +264,14 @@ int cp_fit_all_stars(cp_ctx* ctx, const cp_psf_desc* psf, const float* image, co
                      const cp_lm_opts* opts, double spread_model_fwhm, int32_t precision,
                      cp_sim_result* out);
 
+/* Same, with image / inv_var / theta0 already resident in device memory
+ * (device pointers); outputs in cp_sim_result are host pointers. */
+int cp_fit_all_stars_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_image,
+                            const float* d_inv_var, int32_t ny, int32_t nx, const double* d_theta0,
+                            int32_t n_stars, double fit_rad, int32_t n_passes, uint32_t free_mask,
+                            const double* fixed_values, const cp_lm_opts* opts,
+                            double spread_model_fwhm, int32_t precision, cp_sim_result* out);
+
 /* ---- synthetic inputs: simulate_image (src/simulation.jl:211-498) ----------
  * Seeded xoshiro256** streams; host arrays.
  * cp_sim_sources: n positions uniform in [lo, hi] per axis and fluxes from
