@@ -3,6 +3,7 @@
 Host-side mirror of the reference Julia API over libcrowdphot_hip.so:
   fit_all_stars_simultaneous  (src/photometry/psf_photometry_simultaneous.jl:953)
   fit_all_stars                (src/photometry/psf_photometry_single.jl:267, sequential multi-pass)
+  matched_filter, findlocalmaxima, correlate (src/detection.jl, src/correlation.jl)
   fit_star / fit_star_batch    (src/psf/psf_fitting.jl:229, lm_irls src/levenberg_marquardt.jl:431)
   CircularGaussianPSF, CircularMoffatPSF, ImagePSF, GriddedPSFModel
 """
@@ -12,4 +13,6 @@ from .photometry import (ArgumentError, Context, LMResult, MultiPassPhotResult, 
                          fit_all_stars_simultaneous, fit_star, fit_star_batch)
 from .psf import (CircularGaussianPSF, CircularMoffatPSF, GriddedPSFModel,  # noqa: F401
                   ImagePSF, moffat_epsf, pixel_response_kernel)
+from .detection import (CircularGaussianPRF, GaussianPRF, MatchedFilterResult, correlate,  # noqa: F401
+                        findlocalmaxima, matched_filter, render)
 from . import serving, simulation  # noqa: F401

@@ -25,6 +25,8 @@ CP_SOLVER_LSQR = 0
 CP_SOLVER_CGLS = 1
 
 CP_PREC_MIXED = 0
+CP_BORDER_REPLICATE = 0
+CP_BORDER_ZERO = 1
 CP_PREC_FP64 = 1
 CP_LAYOUT_COLMAJOR = 0
 CP_LAYOUT_ROWMAJOR = 1
@@ -121,6 +123,16 @@ class SimResult(C.Structure):
         ("trace_cap", C.c_int32), ("trace_len", C.c_int32),
         ("qfit", f64p), ("qfit_expected", f64p), ("qfit_z", f64p), ("crowding", f64p),
         ("spread_model", f64p), ("spread_model_err", f64p),
+    ]
+
+
+class MFResult(C.Structure):
+    _fields_ = [
+        ("smoothed", f64p), ("smoothed_inv_var", f64p), ("significance", f64p), ("kernel", f64p),
+        ("kernel_norm", C.c_double),
+        ("peaks_cap", C.c_int32), ("n_peaks", C.c_int32),
+        ("peak_y", i32p), ("peak_x", i32p),
+        ("peak_significance", f64p), ("peak_flux", f64p),
     ]
 
 
@@ -263,6 +275,17 @@ SIGNATURES = {
                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                            C.c_int32, C.POINTER(LMOpts), C.c_void_p, C.c_void_p,
                                            C.c_void_p]),
+    "cp_correlate": (C.c_int, [C.c_void_p, f64p, C.c_int32, C.c_int32, f64p, C.c_int32, C.c_int32, C.c_int32,
+                               f64p]),
+    "cp_correlate_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, f64p, C.c_int32, C.c_int32,
+                                      C.c_int32, C.c_void_p]),
+    "cp_findlocalmaxima": (C.c_int, [C.c_void_p, f64p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, i32p, i32p,
+                                     C.POINTER(C.c_int32)]),
+    "cp_matched_filter": (C.c_int, [C.c_void_p, f32p, f32p, C.c_int32, C.c_int32, f64p, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_double, C.c_int32, C.POINTER(MFResult)]),
+    "cp_matched_filter_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, f64p,
+                                           C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                           C.POINTER(MFResult)]),
     "cp_sim_sources": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_int32, C.c_double, C.c_double, C.c_double, f64p, f64p, f64p]),
     "cp_sim_sources_plummer": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, C.c_double,

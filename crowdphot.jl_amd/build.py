@@ -14,7 +14,7 @@ OUT = os.path.join(HERE, "libcrowdphot_hip.so")
 OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["context.hip", "simultaneous.hip", "lm_batch.hip", "multipass.hip"]
+HIP_SOURCES = ["context.hip", "simultaneous.hip", "lm_batch.hip", "multipass.hip", "detection.hip"]
 CPP_SOURCES = ["simulate.cpp", "server.cpp", "diagnostics.cpp", "aperture.cpp", "host_copy.cpp"]
 # packed FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature",

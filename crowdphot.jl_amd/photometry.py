@@ -121,8 +121,13 @@ class MultiPassPhotResult:
 
 
 def _catalog(sources):
-    """_extract_source_catalog for a NamedTuple-like source table
-    (psf_photometry_single.jl:175-182): y, x required; flux defaults to 1."""
+    """_extract_source_catalog (psf_photometry_single.jl:152-197): a
+    MatchedFilterResult gives the peak pixels and peak fluxes; a
+    NamedTuple-like table y, x (required) and flux (default 1)."""
+    if hasattr(sources, "peaks") and hasattr(sources, "peak_fluxes"):
+        pk = np.asarray(sources.peaks, dtype=np.float64).reshape(-1, 2)
+        return (np.ascontiguousarray(pk[:, 0]), np.ascontiguousarray(pk[:, 1]),
+                np.ascontiguousarray(sources.peak_fluxes, dtype=np.float64))
     get = (lambda k: sources[k]) if isinstance(sources, dict) else (lambda k: getattr(sources, k))
     has = (lambda k: k in sources) if isinstance(sources, dict) else (lambda k: hasattr(sources, k))
     y = np.ascontiguousarray(get("y"), dtype=np.float64)

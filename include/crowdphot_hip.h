@@ -272,6 +272,51 @@ int cp_fit_all_stars_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_
                             const double* fixed_values, const cp_lm_opts* opts,
                             double spread_model_fwhm, int32_t precision, cp_sim_result* out);
 
+/* ---- detection: correlate, findlocalmaxima, matched_filter ----------------
+ * correlate / correlate! (src/correlation.jl:108-140), findlocalmaxima and
+ * matched_filter (src/detection.jl:87-324): the catalogue-building step before
+ * the fits (_extract_source_catalog takes a MatchedFilterResult,
+ * src/photometry/psf_photometry_single.jl:190-197).  Images are column-major
+ * (row = y fastest); peak coordinates are 1-based (row, column). */
+#define CP_BORDER_REPLICATE 0
+#define CP_BORDER_ZERO 1
+
+typedef struct cp_mf_result {
+    double *smoothed;          /* ny*nx: image correlated with the normalised kernel; may be NULL */
+    double *smoothed_inv_var;  /* ny*nx: inv_var correlated with the squared kernel (inv_var given); may be NULL */
+    double *significance;      /* ny*nx significance map; may be NULL */
+    double *kernel;            /* kr*kc normalised kernel; may be NULL */
+    double kernel_norm;        /* out */
+    int32_t peaks_cap;         /* capacity of the four peak arrays */
+    int32_t n_peaks;           /* out: peaks found; only min(n_peaks, peaks_cap) are written */
+    int32_t *peak_y, *peak_x;  /* 1-based row (y) and column (x) */
+    double *peak_significance, *peak_flux;
+} cp_mf_result;
+
+/* out = img correlated with the kr x kc kernel (odd sides, column-major;
+ * no flip), border CP_BORDER_REPLICATE or CP_BORDER_ZERO.  A kernel that is
+ * numerically rank 1 (every singular value past the first below
+ * sqrt(eps)) runs as two 1-D passes, as the reference's _tryfactor does.
+ * Host float64 arrays; the _device variant takes device img / out. */
+int cp_correlate(cp_ctx* ctx, const double* img, int32_t ny, int32_t nx, const double* kernel,
+                 int32_t kr, int32_t kc, int32_t border, double* out);
+int cp_correlate_device(cp_ctx* ctx, const double* d_img, int32_t ny, int32_t nx,
+                        const double* kernel, int32_t kr, int32_t kc, int32_t border, double* d_out);
+/* Pixels strictly greater than their 8 neighbours (3x3 window), in the
+ * reference's order; edges = 0 skips the image boundary.  Host img; peak
+ * arrays of capacity cap; *n_peaks = peaks found. */
+int cp_findlocalmaxima(cp_ctx* ctx, const double* img, int32_t ny, int32_t nx, int32_t edges,
+                       int32_t cap, int32_t* peak_y, int32_t* peak_x, int32_t* n_peaks);
+/* matched_filter(image, kernel; inv_var, normalize_zerosum, sigma, border).
+ * image / inv_var (NULL = uniform weights): float32 ny*nx; kernel: host
+ * float64 kr*kc.  The _device variant takes device image / inv_var. */
+int cp_matched_filter(cp_ctx* ctx, const float* image, const float* inv_var, int32_t ny, int32_t nx,
+                      const double* kernel, int32_t kr, int32_t kc, int32_t normalize_zerosum,
+                      double sigma, int32_t border, cp_mf_result* out);
+int cp_matched_filter_device(cp_ctx* ctx, const float* d_image, const float* d_inv_var, int32_t ny,
+                             int32_t nx, const double* kernel, int32_t kr, int32_t kc,
+                             int32_t normalize_zerosum, double sigma, int32_t border, cp_mf_result* out);
+
 /* ---- synthetic inputs: simulate_image (src/simulation.jl:211-498) ----------
  * Seeded xoshiro256** streams; host arrays.
  * cp_sim_sources: n positions uniform in [lo, hi] per axis and fluxes from

@@ -1977,3 +1977,171 @@ int or_fit_star_batch(const cp_psf_desc* psf, const float* pix, const int64_t* o
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Detection: correlate (src/correlation.jl), findlocalmaxima and
+// matched_filter (src/detection.jl)
+// ---------------------------------------------------------------------------
+namespace {
+
+// out[row, col] = sum_{j, i} img[row + i - r - 1, col + j - c - 1] K[i, j]
+// (the direct form of correlation.jl:329-346 that its tests take as ground
+// truth), out-of-image pixels replicated or zero.  Kernel-order sums.
+void correlate_direct(const double* img, int ny, int nx, const double* K, int kr, int kc, int border, double* out) {
+    const int r = kr / 2, c = kc / 2;
+    for (int col = 1; col <= nx; ++col)
+        for (int row = 1; row <= ny; ++row) {
+            double acc = 0;
+            for (int j = 1; j <= kc; ++j)
+                for (int i = 1; i <= kr; ++i) {
+                    int ir = row + i - r - 1, ic = col + j - c - 1;
+                    double v;
+                    if (border == CP_BORDER_REPLICATE) {
+                        ir = std::min(std::max(ir, 1), ny);
+                        ic = std::min(std::max(ic, 1), nx);
+                        v = img[(size_t)(ir - 1) + (size_t)(ic - 1) * ny];
+                    } else {
+                        v = (ir >= 1 && ir <= ny && ic >= 1 && ic <= nx) ? img[(size_t)(ir - 1) + (size_t)(ic - 1) * ny]
+                                                                        : 0.0;
+                    }
+                    acc += v * K[(size_t)(i - 1) + (size_t)(j - 1) * kr];
+                }
+            out[(size_t)(row - 1) + (size_t)(col - 1) * ny] = acc;
+        }
+}
+
+// findlocalmaxima (detection.jl:240-324): interior pixels column by column,
+// then (edges) the top / bottom rows, the left / right columns and the four
+// corners, each against its in-image neighbours, strictly.  -1 for the
+// reference's BoundsError (a 1-pixel-wide image of length >= 3 with edges).
+int local_maxima(const double* a, int H, int W, bool edges, std::vector<std::pair<int, int>>& out) {
+    out.clear();
+    if (H < 3 && W < 3) return 0;
+    auto v = [&](int r, int c) { return a[(size_t)(r - 1) + (size_t)(c - 1) * H]; };
+    for (int col = 2; col <= W - 1; ++col)
+        for (int row = 2; row <= H - 1; ++row) {
+            const double x = v(row, col);
+            if (x > v(row - 1, col - 1) && x > v(row - 1, col) && x > v(row - 1, col + 1) && x > v(row, col - 1) &&
+                x > v(row, col + 1) && x > v(row + 1, col - 1) && x > v(row + 1, col) && x > v(row + 1, col + 1))
+                out.push_back({row, col});
+        }
+    if (!edges) return 0;
+    if ((H == 1 && W >= 3) || (W == 1 && H >= 3)) return -1;
+    for (int col = 2; col <= W - 1; ++col) {
+        double x = v(1, col);
+        if (x > v(1, col - 1) && x > v(1, col + 1) && x > v(2, col - 1) && x > v(2, col) && x > v(2, col + 1))
+            out.push_back({1, col});
+        x = v(H, col);
+        if (x > v(H - 1, col - 1) && x > v(H - 1, col) && x > v(H - 1, col + 1) && x > v(H, col - 1) &&
+            x > v(H, col + 1))
+            out.push_back({H, col});
+    }
+    for (int row = 2; row <= H - 1; ++row) {
+        double x = v(row, 1);
+        if (x > v(row - 1, 1) && x > v(row - 1, 2) && x > v(row, 2) && x > v(row + 1, 1) && x > v(row + 1, 2))
+            out.push_back({row, 1});
+        x = v(row, W);
+        if (x > v(row - 1, W - 1) && x > v(row - 1, W) && x > v(row, W - 1) && x > v(row + 1, W - 1) &&
+            x > v(row + 1, W))
+            out.push_back({row, W});
+    }
+    if (H >= 2 && W >= 2) {
+        if (v(1, 1) > v(1, 2) && v(1, 1) > v(2, 1) && v(1, 1) > v(2, 2)) out.push_back({1, 1});
+        if (v(1, W) > v(1, W - 1) && v(1, W) > v(2, W - 1) && v(1, W) > v(2, W)) out.push_back({1, W});
+        if (v(H, 1) > v(H - 1, 1) && v(H, 1) > v(H - 1, 2) && v(H, 1) > v(H, 2)) out.push_back({H, 1});
+        if (v(H, W) > v(H - 1, W - 1) && v(H, W) > v(H - 1, W) && v(H, W) > v(H, W - 1)) out.push_back({H, W});
+    }
+    return 0;
+}
+
+}  // namespace
+
+int or_correlate(const double* img, int32_t ny, int32_t nx, const double* kernel, int32_t kr, int32_t kc,
+                 int32_t border, double* out) {
+    if (!(kr % 2 == 1 && kc % 2 == 1)) return fail("kernel dimensions must be odd");
+    if (border != CP_BORDER_REPLICATE && border != CP_BORDER_ZERO) return fail("unknown border mode");
+    correlate_direct(img, ny, nx, kernel, kr, kc, border, out);
+    return 0;
+}
+
+int or_findlocalmaxima(const double* img, int32_t ny, int32_t nx, int32_t edges, int32_t cap, int32_t* py,
+                       int32_t* px, int32_t* n) {
+    std::vector<std::pair<int, int>> pk;
+    if (local_maxima(img, ny, nx, edges != 0, pk)) return fail("image too narrow for edge maxima");
+    *n = (int32_t)pk.size();
+    for (int k = 0; k < (int)pk.size() && k < cap; ++k) {
+        py[k] = pk[k].first;
+        px[k] = pk[k].second;
+    }
+    return 0;
+}
+
+// matched_filter (detection.jl:87-205): kernel normalisation (zero-sum
+// (P - mean) / (sum P^2 - (sum P)^2 / N) or P / sum P^2, degenerate
+// denominators keep P with norm 1), smoothed = correlate(image, K); with
+// inv_var (non-finite -> 0): num = correlate(w v, K) (w = 0 -> 0),
+// den = correlate(w, K^2), significance = num / sqrt(den) (0 where den <= 0);
+// without: significance = smoothed * kernel_norm.  Peaks: local maxima of the
+// significance away from the edges with significance >= sigma; flux
+// smoothed[p], or significance / (sqrt(den) kernel_norm^2) with inv_var.
+int or_matched_filter(const double* image, const double* inv_var, int32_t ny, int32_t nx, const double* kernel,
+                      int32_t kr, int32_t kc, int32_t normalize_zerosum, double sigma, int32_t border,
+                      cp_mf_result* out) {
+    if (!(kr % 2 == 1 && kc % 2 == 1)) return fail("kernel dimensions must be odd");
+    if (border != CP_BORDER_REPLICATE && border != CP_BORDER_ZERO) return fail("unknown border mode");
+    const size_t P = (size_t)ny * nx, NK = (size_t)kr * kc;
+    double sumP = 0, sumP2 = 0;
+    for (size_t k = 0; k < NK; ++k) {
+        sumP += kernel[k];
+        sumP2 += kernel[k] * kernel[k];
+    }
+    std::vector<double> K(kernel, kernel + NK);
+    double knorm = 1.0;
+    if (normalize_zerosum) {
+        const double denom = sumP2 - sumP * sumP / (double)NK;
+        if (denom > 0) {
+            for (size_t k = 0; k < NK; ++k) K[k] = (kernel[k] - sumP / (double)NK) / denom;
+            knorm = std::sqrt(denom);
+        }
+    } else if (sumP2 > 0) {
+        for (size_t k = 0; k < NK; ++k) K[k] = kernel[k] / sumP2;
+        knorm = std::sqrt(sumP2);
+    }
+    std::vector<double> sm(P), sig(P), den;
+    correlate_direct(image, ny, nx, K.data(), kr, kc, border, sm.data());
+    if (inv_var) {
+        std::vector<double> w(P), wv(P), K2(NK), num(P);
+        for (size_t q = 0; q < P; ++q) {
+            w[q] = std::isfinite(inv_var[q]) ? inv_var[q] : 0.0;
+            wv[q] = w[q] == 0 ? 0.0 : w[q] * image[q];
+        }
+        for (size_t k = 0; k < NK; ++k) K2[k] = K[k] * K[k];
+        correlate_direct(wv.data(), ny, nx, K.data(), kr, kc, border, num.data());
+        den.resize(P);
+        correlate_direct(w.data(), ny, nx, K2.data(), kr, kc, border, den.data());
+        for (size_t q = 0; q < P; ++q) sig[q] = den[q] > 0 ? num[q] / std::sqrt(den[q]) : 0.0;
+    } else {
+        for (size_t q = 0; q < P; ++q) sig[q] = sm[q] * knorm;
+    }
+    std::vector<std::pair<int, int>> pk;
+    local_maxima(sig.data(), ny, nx, false, pk);
+    int np = 0;
+    for (auto& p : pk) {
+        const size_t q = (size_t)(p.first - 1) + (size_t)(p.second - 1) * ny;
+        if (!(sig[q] >= sigma)) continue;
+        if (np < out->peaks_cap) {
+            out->peak_y[np] = p.first;
+            out->peak_x[np] = p.second;
+            out->peak_significance[np] = sig[q];
+            out->peak_flux[np] = inv_var ? (den[q] > 0 ? sig[q] / (std::sqrt(den[q]) * knorm * knorm) : 0.0) : sm[q];
+        }
+        ++np;
+    }
+    out->n_peaks = np;
+    out->kernel_norm = knorm;
+    if (out->kernel) std::copy(K.begin(), K.end(), out->kernel);
+    if (out->smoothed) std::copy(sm.begin(), sm.end(), out->smoothed);
+    if (out->significance) std::copy(sig.begin(), sig.end(), out->significance);
+    if (out->smoothed_inv_var && inv_var) std::copy(den.begin(), den.end(), out->smoothed_inv_var);
+    return 0;
+}

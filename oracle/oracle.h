@@ -83,6 +83,15 @@ int or_fit_star_batch(const cp_psf_desc* psf, const float* pix, const int64_t* o
 int or_extract_errors(const double* cov, int32_t p, const int32_t* free_idx, int32_t n_params,
                       double* errors_col);
 
+/* detection */
+int or_correlate(const double* img, int32_t ny, int32_t nx, const double* kernel, int32_t kr, int32_t kc,
+                 int32_t border, double* out);
+int or_findlocalmaxima(const double* img, int32_t ny, int32_t nx, int32_t edges, int32_t cap, int32_t* py,
+                       int32_t* px, int32_t* n);
+int or_matched_filter(const double* image, const double* inv_var, int32_t ny, int32_t nx, const double* kernel,
+                      int32_t kr, int32_t kc, int32_t normalize_zerosum, double sigma, int32_t border,
+                      cp_mf_result* out);
+
 #ifdef __cplusplus
 }
 #endif

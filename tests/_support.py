@@ -75,6 +75,11 @@ def oracle():
         "or_fit_all_stars": (C.c_int, [P(abi.PsfDesc), f64p, f64p, C.c_int32, C.c_int32, f64p, C.c_int32,
                                        C.c_double, C.c_int32, C.c_uint32, f64p, P(abi.LMOpts), C.c_double,
                                        P(abi.SimResult)]),
+        "or_correlate": (C.c_int, [f64p, C.c_int32, C.c_int32, f64p, C.c_int32, C.c_int32, C.c_int32, f64p]),
+        "or_findlocalmaxima": (C.c_int, [f64p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, i32p, i32p,
+                                         C.POINTER(C.c_int32)]),
+        "or_matched_filter": (C.c_int, [f64p, f64p, C.c_int32, C.c_int32, f64p, C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_double, C.c_int32, P(abi.MFResult)]),
         "or_extract_errors": (C.c_int, [f64p, C.c_int32, i32p, C.c_int32, f64p]),
     }
     for k, (r, a) in sig.items():
