@@ -6,10 +6,11 @@
 // _tryfactor correlation.jl:196-226) runs as a column pass then a row pass,
 // each thread one output pixel reading its taps straight from L1/L2 (the
 // reads of one tap are coalesced down a column).  Any other kernel runs as a
-// 2-D pass over 16 x 16 output tiles: the block stages the tile's input
-// window (border rule applied) and the kernel in LDS as float64 and every
-// thread sums its kr x kc taps from LDS; windows that would not fit LDS read
-// the image through the cache instead.  Sums are float64 in kernel order.
+// 2-D pass over (16 RY) x 16 output tiles: the block stages the tile's input
+// window (border rule applied) in LDS as float64, the kernel taps come
+// through the scalar cache, and every thread sums RY output rows, one LDS
+// window read feeding RY FMAs; windows that would not fit LDS read the image
+// through the cache instead.  Sums are float64 in kernel order.
 //
 // Peaks.  One thread per pixel tests the strict 3 x 3 maximum; the flagged
 // pixels are compacted in the reference's order (column-major interior,
@@ -18,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "cp_internal.h"
@@ -39,33 +41,174 @@ __device__ __forceinline__ double pix(const TI* __restrict__ img, int ny, int nx
     return (r >= 0 && r < ny && c >= 0 && c < nx) ? (double)img[(size_t)r + (size_t)c * ny] : 0.0;
 }
 
-// 2-D pass, window staged in LDS.  Block CT x CT threads (y fastest).
-template <typename TI>
+// 2-D pass, window staged in LDS.  Block CT x CT threads (y fastest); each
+// thread sums RY consecutive output rows of one column, so one LDS read of
+// the window feeds RY fused multiply-adds: for each kernel column the RY
+// running window values rotate through registers (the i loop is unrolled by
+// RY so the rotation is register renaming).  Kernel taps are wave-uniform
+// loads through the scalar cache.  Each output sums its taps column by
+// column, rows ascending.
+template <typename TI, int RY>
 __global__ __launch_bounds__(CT* CT) void k_corr2d_lds(const TI* __restrict__ img, int ny, int nx,
                                                        const double* __restrict__ K, int kr, int kc, int border,
                                                        double* __restrict__ out) {
-    extern __shared__ double sh[];
+    extern __shared__ double win[];
+    constexpr int TY = CT * RY;
     const int r = kr / 2, c = kc / 2;
-    const int wy = CT + kr - 1, wx = CT + kc - 1;
-    double* win = sh;
-    double* ks = sh + (size_t)wy * wx;
-    const int row0 = blockIdx.x * CT, col0 = blockIdx.y * CT;
-    for (int i = threadIdx.x; i < kr * kc; i += CT * CT) ks[i] = K[i];
+    const int wy = TY + kr - 1, wx = CT + kc - 1;
+    const int row0 = blockIdx.x * TY, col0 = blockIdx.y * CT;
     for (int i = threadIdx.x; i < wy * wx; i += CT * CT) {
         const int b = i / wy, a = i - b * wy;
         win[i] = pix(img, ny, nx, row0 + a - r, col0 + b - c, border);
     }
     __syncthreads();
     const int ty = threadIdx.x % CT, tx = threadIdx.x / CT;
-    const int row = row0 + ty, col = col0 + tx;
-    if (row >= ny || col >= nx) return;
-    double acc = 0.0;
+    const int rb = ty * RY, col = col0 + tx;
+    double acc[RY];
+#pragma unroll
+    for (int q = 0; q < RY; ++q) acc[q] = 0.0;
+    const int kmain = kr - kr % RY;
     for (int j = 0; j < kc; ++j) {
-        const double* w = win + ty + (size_t)(tx + j) * wy;
-        const double* k = ks + (size_t)j * kr;
-        for (int i = 0; i < kr; ++i) acc = fma(w[i], k[i], acc);
+        const double* w = win + rb + (size_t)(tx + j) * wy;
+        const double* kj = K + (size_t)j * kr;
+        double v[RY];
+#pragma unroll
+        for (int q = 0; q < RY - 1; ++q) v[q] = w[q];
+        int i = 0;
+        for (; i < kmain; i += RY) {
+#pragma unroll
+            for (int u = 0; u < RY; ++u) {
+                v[(u + RY - 1) % RY] = w[i + u + RY - 1];
+                const double k = kj[i + u];
+#pragma unroll
+                for (int q = 0; q < RY; ++q) acc[q] = fma(v[(u + q) % RY], k, acc[q]);
+            }
+        }
+        // remainder taps (kr not a multiple of RY): rebuild the window per tap
+        for (; i < kr; ++i) {
+            const double k = kj[i];
+#pragma unroll
+            for (int q = 0; q < RY; ++q) acc[q] = fma(w[i + q], k, acc[q]);
+        }
     }
-    out[(size_t)row + (size_t)col * ny] = acc;
+    if (col >= nx) return;
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        const int row = row0 + rb + q;
+        if (row < ny) out[(size_t)row + (size_t)col * ny] = acc[q];
+    }
+}
+
+template <int RY>
+constexpr size_t corr_lds_bytes(int kr, int kc) {
+    return (size_t)(CT * RY + kr - 1) * (CT + kc - 1) * 8;
+}
+
+// matched_filter's three correlations in one pass: smoothed =
+// correlate(image, K) and, with weights, num = correlate(w v, K) and
+// den = correlate(w, K^2), then the significance num / sqrt(den) (0 where
+// den <= 0) or smoothed * knorm.  The block stages the image and (weighted)
+// the weights w = finite(inv_var) ? inv_var : 0 as float32 planes (exact
+// copies of the float32 inputs, border rule applied), and each window value
+// read feeds RY outputs of every sum; w v is formed in float64 at read time
+// (exact for float32 operands; 0 where w = 0).  Same tap order as correlate.
+template <int RY, bool WEIGHTED>
+__global__ __launch_bounds__(CT* CT) void k_mf2d(const float* __restrict__ img, const float* __restrict__ iv, int ny,
+                                                 int nx, const double* __restrict__ K, const double* __restrict__ K2,
+                                                 int kr, int kc, int border, double knorm, double* __restrict__ sm,
+                                                 double* __restrict__ den_out, double* __restrict__ sig) {
+    extern __shared__ float fsh[];
+    constexpr int TY = CT * RY;
+    const int r = kr / 2, c = kc / 2;
+    const int wy = TY + kr - 1, wx = CT + kc - 1;
+    float* pv = fsh;
+    float* pw = fsh + (size_t)wy * wx;
+    const int row0 = blockIdx.x * TY, col0 = blockIdx.y * CT;
+    for (int i = threadIdx.x; i < wy * wx; i += CT * CT) {
+        const int b = i / wy, a = i - b * wy;
+        int rr = row0 + a - r, cc = col0 + b - c;
+        bool in = true;
+        if (border == CP_BORDER_REPLICATE) {
+            rr = min(max(rr, 0), ny - 1);
+            cc = min(max(cc, 0), nx - 1);
+        } else {
+            in = rr >= 0 && rr < ny && cc >= 0 && cc < nx;
+        }
+        const size_t q = in ? (size_t)rr + (size_t)cc * ny : 0;
+        pv[i] = in ? img[q] : 0.0f;
+        if (WEIGHTED) {
+            const float a_ = in ? iv[q] : 0.0f;
+            pw[i] = isfinite(a_) ? a_ : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int ty = threadIdx.x % CT, tx = threadIdx.x / CT;
+    const int rb = ty * RY, col = col0 + tx;
+    double as[RY], an[RY], ad[RY];
+#pragma unroll
+    for (int q = 0; q < RY; ++q) as[q] = an[q] = ad[q] = 0.0;
+    // one tap for all RY outputs from the rotating registers xv / xw / xp
+    // (image, weight, weighted image) at rotation u
+    auto tap = [&](const double* xv, const double* xw, const double* xp, int u, double k, double k2) {
+#pragma unroll
+        for (int q = 0; q < RY; ++q) {
+            const int e = (u + q) % RY;
+            as[q] = fma(xv[e], k, as[q]);
+            if (WEIGHTED) {
+                an[q] = fma(xp[e], k, an[q]);
+                ad[q] = fma(xw[e], k2, ad[q]);
+            }
+        }
+    };
+    const int kmain = kr - kr % RY;
+    for (int j = 0; j < kc; ++j) {
+        const float* v = pv + rb + (size_t)(tx + j) * wy;
+        const float* w = pw + rb + (size_t)(tx + j) * wy;
+        const double* kj = K + (size_t)j * kr;
+        const double* k2j = K2 + (size_t)j * kr;
+        double xv[RY], xw[RY], xp[RY];
+        auto load = [&](int slot, int at) {
+            xv[slot] = (double)v[at];
+            if (WEIGHTED) {
+                xw[slot] = (double)w[at];
+                xp[slot] = xw[slot] == 0.0 ? 0.0 : xw[slot] * xv[slot];
+            }
+        };
+#pragma unroll
+        for (int q = 0; q < RY - 1; ++q) load(q, q);
+        int i = 0;
+        for (; i < kmain; i += RY) {
+#pragma unroll
+            for (int u = 0; u < RY; ++u) {
+                load((u + RY - 1) % RY, i + u + RY - 1);
+                tap(xv, xw, xp, u, kj[i + u], WEIGHTED ? k2j[i + u] : 0.0);
+            }
+        }
+        for (; i < kr; ++i) {   // remainder taps: reload the RY window values
+#pragma unroll
+            for (int q = 0; q < RY; ++q) load(q, i + q);
+            tap(xv, xw, xp, 0, kj[i], WEIGHTED ? k2j[i] : 0.0);
+        }
+    }
+    if (col >= nx) return;
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        const int row = row0 + rb + q;
+        if (row >= ny) continue;
+        const size_t o = (size_t)row + (size_t)col * ny;
+        sm[o] = as[q];
+        if (WEIGHTED) {
+            den_out[o] = ad[q];
+            sig[o] = ad[q] > 0 ? an[q] / sqrt(ad[q]) : 0.0;
+        } else {
+            sig[o] = as[q] * knorm;
+        }
+    }
+}
+
+template <int RY>
+constexpr size_t mf_lds_bytes(int kr, int kc, bool weighted) {
+    return (size_t)(CT * RY + kr - 1) * (CT + kc - 1) * 4 * (weighted ? 2 : 1);
 }
 
 // 2-D pass through the cache, for kernels whose window overflows LDS.
@@ -291,11 +434,16 @@ void run_corr(cp_ctx* ctx, const CorrPlan& pl, const TI* d_img, int ny, int nx, 
                  ny, nx, (const double*)pl.k2.as<double>(), pl.kc, border, d_out);
         return;
     }
-    const size_t lds = ((size_t)(CT + pl.kr - 1) * (CT + pl.kc - 1) + (size_t)pl.kr * pl.kc) * 8;
-    if (lds <= CORR_LDS_MAX) {
-        dim3 gg((unsigned)((ny + CT - 1) / CT), (unsigned)((nx + CT - 1) / CT));
-        launch_n("k_corr2d", k_corr2d_lds<TI>, gg, dim3(CT * CT), lds, st, d_img, ny, nx,
-                 (const double*)pl.K.as<double>(), pl.kr, pl.kc, border, d_out);
+    auto go = [&](auto ry) {
+        constexpr int RY = decltype(ry)::value;
+        dim3 gg((unsigned)((ny + CT * RY - 1) / (CT * RY)), (unsigned)((nx + CT - 1) / CT));
+        launch_n("k_corr2d", k_corr2d_lds<TI, RY>, gg, dim3(CT * CT), corr_lds_bytes<RY>(pl.kr, pl.kc), st, d_img,
+                 ny, nx, (const double*)pl.K.as<double>(), pl.kr, pl.kc, border, d_out);
+    };
+    if (corr_lds_bytes<8>(pl.kr, pl.kc) <= CORR_LDS_MAX) {
+        go(std::integral_constant<int, 8>{});
+    } else if (corr_lds_bytes<2>(pl.kr, pl.kc) <= CORR_LDS_MAX) {
+        go(std::integral_constant<int, 2>{});
     } else {
         launch_n("k_corr2d", k_corr2d_global<TI>, dim3(g), dim3(BLOCK), 0, st, d_img, ny, nx,
                  (const double*)pl.K.as<double>(), pl.kr, pl.kc, border, d_out);
@@ -378,23 +526,42 @@ void matched_filter_device(cp_ctx* ctx, const float* d_image, const float* d_iv,
     // 2. correlations (:139-168)
     const int64_t P = (int64_t)ny * nx;
     const unsigned g = (unsigned)((P + BLOCK - 1) / BLOCK);
-    CorrPlan pk, pk2;
-    plan_corr(ctx, K.data(), kr, kc, pk);
     DevBuf sm(ctx, (size_t)P * 8), sig(ctx, (size_t)P * 8), tmp, w, wv, num, den;
-    run_corr<float>(ctx, pk, d_image, ny, nx, border, sm.as<double>(), tmp);
-    if (d_iv) {
-        plan_corr(ctx, K2.data(), kr, kc, pk2);
-        w.alloc(ctx, (size_t)P * 8);
-        wv.alloc(ctx, (size_t)P * 8);
-        num.alloc(ctx, (size_t)P * 8);
-        den.alloc(ctx, (size_t)P * 8);
-        launch(k_mf_weights, dim3(g), dim3(BLOCK), 0, st, P, d_image, d_iv, w.as<double>(), wv.as<double>());
-        run_corr<double>(ctx, pk, wv.as<double>(), ny, nx, border, num.as<double>(), tmp);
-        run_corr<double>(ctx, pk2, w.as<double>(), ny, nx, border, den.as<double>(), tmp);
+    if (d_iv) den.alloc(ctx, (size_t)P * 8);
+    if (mf_lds_bytes<4>(kr, kc, d_iv != nullptr) <= CORR_LDS_MAX) {
+        // one fused pass over LDS-staged float32 planes
+        DevBuf dK(ctx, NK * 8), dK2(ctx, NK * 8);
+        CP_HIP(hipMemcpyAsync(dK.get(), K.data(), NK * 8, hipMemcpyHostToDevice, st));
+        CP_HIP(hipMemcpyAsync(dK2.get(), K2.data(), NK * 8, hipMemcpyHostToDevice, st));
+        constexpr int RY = 4;
+        dim3 gg((unsigned)((ny + CT * RY - 1) / (CT * RY)), (unsigned)((nx + CT - 1) / CT));
+        if (d_iv)
+            launch_n("k_mf2d", k_mf2d<RY, true>, gg, dim3(CT * CT), mf_lds_bytes<RY>(kr, kc, true), st, d_image, d_iv,
+                     ny, nx, (const double*)dK.as<double>(), (const double*)dK2.as<double>(), kr, kc, border, knorm,
+                     sm.as<double>(), den.as<double>(), sig.as<double>());
+        else
+            launch_n("k_mf2d", k_mf2d<RY, false>, gg, dim3(CT * CT), mf_lds_bytes<RY>(kr, kc, false), st, d_image,
+                     (const float*)nullptr, ny, nx, (const double*)dK.as<double>(),
+                     (const double*)dK2.as<double>(), kr, kc, border, knorm, sm.as<double>(), (double*)nullptr,
+                     sig.as<double>());
+        CP_HIP(hipStreamSynchronize(st));   // K, K2 are local
+    } else {
+        CorrPlan pk, pk2;
+        plan_corr(ctx, K.data(), kr, kc, pk);
+        run_corr<float>(ctx, pk, d_image, ny, nx, border, sm.as<double>(), tmp);
+        if (d_iv) {
+            plan_corr(ctx, K2.data(), kr, kc, pk2);
+            w.alloc(ctx, (size_t)P * 8);
+            wv.alloc(ctx, (size_t)P * 8);
+            num.alloc(ctx, (size_t)P * 8);
+            launch(k_mf_weights, dim3(g), dim3(BLOCK), 0, st, P, d_image, d_iv, w.as<double>(), wv.as<double>());
+            run_corr<double>(ctx, pk, wv.as<double>(), ny, nx, border, num.as<double>(), tmp);
+            run_corr<double>(ctx, pk2, w.as<double>(), ny, nx, border, den.as<double>(), tmp);
+        }
+        launch(k_mf_significance, dim3(g), dim3(BLOCK), 0, st, P, (const double*)num.as<double>(),
+               (const double*)(d_iv ? den.as<double>() : nullptr), (const double*)sm.as<double>(), knorm,
+               sig.as<double>());
     }
-    launch(k_mf_significance, dim3(g), dim3(BLOCK), 0, st, P, (const double*)num.as<double>(),
-           (const double*)(d_iv ? den.as<double>() : nullptr), (const double*)sm.as<double>(), knorm,
-           sig.as<double>());
     // 3-4. peaks: interior local maxima of the significance at or above sigma
     DevBuf idx;
     const int n = select_maxima(ctx, sig.as<double>(), ny, nx, true, sigma, idx);
