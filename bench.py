@@ -563,7 +563,7 @@ def detection_bench(pkg, ctx, cfg, d_img, d_iv, no_cpu, steps=5):
     if not no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_detect
-        crop = 512
+        crop = 2048
         img = cfg["image"][:crop, :crop].astype(np.float64)
         iv = cfg["inv_var"][:crop, :crop].astype(np.float64)
         t = time.perf_counter()

@@ -6,7 +6,8 @@ keyword arguments and result types of CrowdPhot.jl's fitting entry points and
 forwards the work to the HIP library through `ccall`:
 
 - `fit_all_stars_simultaneous`  (src/photometry/psf_photometry_simultaneous.jl:953)
-- `fit_star_batch`              (batched `fit_star`, src/psf/psf_fitting.jl:229)
+- `fit_all_stars`               (sequential multi-pass, src/photometry/psf_photometry_single.jl:267)
+- `matched_filter`              (src/detection.jl:87)
 
 Depends on the standard library only (`Libdl`), so it loads without `Pkg`
 network access.  The PSF model types are CrowdPhot.jl's own
@@ -92,6 +93,23 @@ mutable struct CpSimResult
     trace_cap::Int32; trace_len::Int32
     qfit::Ptr{Float64}; qfit_expected::Ptr{Float64}; qfit_z::Ptr{Float64}; crowding::Ptr{Float64}
     spread_model::Ptr{Float64}; spread_model_err::Ptr{Float64}
+end
+
+mutable struct CpLMOpts
+    lambda_init::Float64; lambda_up::Float64; lambda_down::Float64; lambda_min::Float64; lambda_max::Float64
+    max_iter::Int32
+    x_tol::Float64; f_tol::Float64; g_tol::Float64
+    damping::Int32
+    min_diagonal::Float64
+    covariance::Int32
+end
+
+mutable struct CpMFResult
+    smoothed::Ptr{Float64}; smoothed_inv_var::Ptr{Float64}; significance::Ptr{Float64}; kernel::Ptr{Float64}
+    kernel_norm::Float64
+    peaks_cap::Int32; n_peaks::Int32
+    peak_y::Ptr{Int32}; peak_x::Ptr{Int32}
+    peak_significance::Ptr{Float64}; peak_flux::Ptr{Float64}
 end
 
 mutable struct Context
@@ -205,6 +223,109 @@ function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad
             spread_model_err = out.spread_model_err,
             n_iter = Int.(out.n_iter), n_passes = Int(r.n_passes), n_failed = Int(r.n_failed),
             failure_msgs = msgs, residual = out.residual)
+end
+
+"""
+    fit_all_stars(image, psf, sources, fit_rad; n_passes = 3, fixed = (;), inv_var = nothing,
+                  spread_model_fwhm = nothing, kws...)
+
+Sequential multi-pass photometry (psf_photometry_single.jl:267-469) on the GPU,
+with fit_star's lm_irls keywords (`max_iter`, `x_tol`, `f_tol`, `g_tol`, `λ_*`).
+`sources` is a NamedTuple (y, x[, flux, bkg]).  Returns a NamedTuple with the
+MultiPassPhotResult fields.
+"""
+function fit_all_stars(image::AbstractMatrix, psf, sources, fit_rad::Real; n_passes::Integer = 3,
+        fixed::NamedTuple = (;), inv_var = nothing, spread_model_fwhm::Union{Nothing, Real} = nothing,
+        max_iter::Integer = 200, x_tol::Real = 1.0e-8, f_tol::Real = 1.0e-4, g_tol::Real = 1.0e-4,
+        λ_init::Real = 1.0e-4, λ_up::Real = 10.0, λ_down::Real = 10.0, λ_min::Real = 1.0e-12,
+        λ_max::Real = 1.0e12, precision::Symbol = :mixed, ctx::Context = Context(0), kws...)
+    n_passes > 0 || throw(ArgumentError("n_passes must be positive, got $n_passes"))
+    props = propertynames_fit(psf)
+    free = Tuple(k for k in props if !haskey(fixed, k))
+    isempty(free) && throw(ArgumentError("all model parameters are fixed; nothing to fit"))
+    mask = UInt32(sum(k in free ? (1 << (i - 1)) : 0 for (i, k) in enumerate(props)))
+    fixv = Float64[haskey(fixed, k) ? Float64(getfield(fixed, k)) : NaN for k in props]
+    n = length(sources.y)
+    theta0 = zeros(length(props), n)                       # column per star = row-major n x npf
+    for (i, k) in enumerate(props)
+        theta0[i, :] .= k === :y ? Float64.(sources.y) : k === :x ? Float64.(sources.x) :
+                        k === :flux ? (haskey(sources, :flux) ? Float64.(sources.flux) : ones(n)) :
+                        k === :bkg ? (haskey(sources, :bkg) ? Float64.(sources.bkg) : zeros(n)) :
+                        fill(Float64(getproperty(psf, k)), n)
+    end
+    ny, nx = size(image)
+    img = Matrix{Float32}(image)
+    iv = inv_var === nothing ? nothing : Matrix{Float32}(inv_var)
+    keep = Any[]
+    desc = psf_desc(psf, (;), keep)
+    o = CpLMOpts(λ_init, λ_up, λ_down, λ_min, λ_max, max_iter, x_tol, f_tol, g_tol, 0, 1.0e-6, 0)
+    out = (y = zeros(n), x = zeros(n), flux = zeros(n), bkg = zeros(n), y_err = zeros(n), x_err = zeros(n),
+           flux_err = zeros(n), bkg_err = zeros(n), chisq = zeros(n), converged = zeros(UInt8, n),
+           valid = zeros(UInt8, n), n_iter = zeros(Int32, n), qfit = zeros(n), qfit_expected = zeros(n),
+           qfit_z = zeros(n), crowding = zeros(n), spread_model = zeros(n), spread_model_err = zeros(n),
+           residual = zeros(Float32, ny, nx))
+    r = CpSimResult(pointer(out.y), pointer(out.x), pointer(out.flux), pointer(out.bkg), pointer(out.y_err),
+                    pointer(out.x_err), pointer(out.flux_err), pointer(out.bkg_err), pointer(out.chisq),
+                    pointer(out.converged), pointer(out.valid), pointer(out.n_iter), C_NULL,
+                    pointer(out.residual), 0, 0, 0, 0, 0.0, 0, 0, C_NULL, 0, 0,
+                    pointer(out.qfit), pointer(out.qfit_expected), pointer(out.qfit_z), pointer(out.crowding),
+                    pointer(out.spread_model), pointer(out.spread_model_err))
+    GC.@preserve keep out img iv theta0 fixv begin
+        rc = ccall(sym(:cp_fit_all_stars), Cint,
+                   (Ptr{Cvoid}, Ref{CpPsfDesc}, Ptr{Float32}, Ptr{Float32}, Int32, Int32, Ptr{Float64}, Int32,
+                    Float64, Int32, UInt32, Ptr{Float64}, Ref{CpLMOpts}, Float64, Int32, Ref{CpSimResult}),
+                   ctx.handle, desc, img, iv === nothing ? C_NULL : iv, ny, nx, theta0, n, fit_rad, n_passes, mask,
+                   fixv, o, spread_model_fwhm === nothing ? 0.0 : Float64(spread_model_fwhm),
+                   precision === :fp64 ? 1 : 0, r)
+        check(ctx, rc)
+    end
+    return (y = out.y, x = out.x, y_err = out.y_err, x_err = out.x_err, flux = out.flux, flux_err = out.flux_err,
+            bkg = out.bkg, bkg_err = out.bkg_err, converged = BitVector(out.converged .!= 0),
+            valid = BitVector(out.valid .!= 0), chisq = out.chisq, qfit = out.qfit, qfit_expected = out.qfit_expected,
+            qfit_z = out.qfit_z, crowding = out.crowding, spread_model = out.spread_model,
+            spread_model_err = out.spread_model_err, n_iter = Int.(out.n_iter), n_passes = Int(r.n_passes),
+            n_failed = Int(r.n_failed), failure_msgs = String[],
+            residual = n == 0 ? Matrix{Float32}(undef, 0, 0) : out.residual)
+end
+
+"""
+    matched_filter(image, kernel::AbstractMatrix; inv_var = nothing, normalize_zerosum = true,
+                   sigma = 5.0, border = :replicate)
+
+Matched-filter detection (detection.jl:87-205) on the GPU.  Returns a
+NamedTuple with the MatchedFilterResult fields; `peaks` are CartesianIndex.
+"""
+function matched_filter(image::AbstractMatrix, kernel::AbstractMatrix; inv_var = nothing,
+        normalize_zerosum::Bool = true, sigma::Real = 5.0, border::Symbol = :replicate, ctx::Context = Context(0))
+    border in (:replicate, :zero) || throw(ArgumentError("unknown border mode :$border; use :replicate or :zero"))
+    inv_var === nothing || size(inv_var) == size(image) ||
+        throw(DimensionMismatch("inv_var size $(size(inv_var)) must match image size $(size(image))"))
+    ny, nx = size(image)
+    kr, kc = size(kernel)
+    img = Matrix{Float32}(image)
+    iv = inv_var === nothing ? nothing : Matrix{Float32}(map(v -> isfinite(v) ? v : zero(v), inv_var))
+    K = Matrix{Float64}(kernel)
+    cap = (div(ny + 1, 2) + 1) * (div(nx + 1, 2) + 1)
+    sm, sig = zeros(ny, nx), zeros(ny, nx)
+    den = inv_var === nothing ? nothing : zeros(ny, nx)
+    kout = zeros(kr, kc)
+    py, px = zeros(Int32, cap), zeros(Int32, cap)
+    ps, pf = zeros(cap), zeros(cap)
+    r = CpMFResult(pointer(sm), den === nothing ? C_NULL : pointer(den), pointer(sig), pointer(kout), 0.0,
+                   cap, 0, pointer(py), pointer(px), pointer(ps), pointer(pf))
+    GC.@preserve img iv K sm sig den kout py px ps pf begin
+        rc = ccall(sym(:cp_matched_filter), Cint,
+                   (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Int32, Int32, Ptr{Float64}, Int32, Int32, Int32,
+                    Float64, Int32, Ref{CpMFResult}),
+                   ctx.handle, img, iv === nothing ? C_NULL : iv, ny, nx, K, kr, kc, normalize_zerosum, sigma,
+                   border === :zero ? 1 : 0, r)
+        check(ctx, rc)
+    end
+    m = Int(r.n_peaks)
+    return (image = image, inv_var = iv, smoothed_image = sm, smoothed_inv_var = den, significance_map = sig,
+            kernel = kout, kernel_norm = r.kernel_norm,
+            peaks = [CartesianIndex(Int(py[i]), Int(px[i])) for i in 1:m],
+            peak_significances = ps[1:m], peak_fluxes = pf[1:m])
 end
 
 propertynames_fit(psf) = begin
