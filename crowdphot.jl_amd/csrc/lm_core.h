@@ -60,21 +60,27 @@ __device__ __forceinline__ void model_fg(const T* prm, T ddy, T ddx, T& f, T G[6
         G[5] = T(0);
     } else {
         // (y, x, alpha, beta, flux, bkg), evaluate_fg parametric_models.jl:704-724
+        // The parameter-only factors (1/alpha^2, 1/norm, ...) are loop
+        // invariant in accum! and hoisted out of the pixel loop; per pixel
+        // one reciprocal of u = 1 + r^2/alpha^2 serves all four gradients.
         T a = prm[2], be = prm[3], flux = prm[4], bkg = prm[5];
         T a2 = a * a;
+        T ia = T(1) / a, ia2 = ia * ia;
         T r2 = ddx * ddx + ddy * ddy;
-        T u = T(1) + r2 / a2;
+        T u = fma(r2, ia2, T(1));
+        T iu = T(1) / u;
         T lu = log(u);
         T prof = exp(-be * lu);
-        T norm = T(M_PI) * a2 / (be - T(1));
-        T amp = flux / norm;
+        T inorm = (be - T(1)) / (T(M_PI) * a2);
+        T amp = flux * inorm;
         T Ag = amp * prof;
         f = fma(amp, prof, bkg);
-        G[0] = T(2) * Ag * be * ddy / (a2 * u);
-        G[1] = T(2) * Ag * be * ddx / (a2 * u);
-        G[2] = Ag * (T(-2) / a + T(2) * be * r2 / (a * a2 * u));
+        T c = T(2) * Ag * be * ia2 * iu;   // 2 A g beta / (alpha^2 u)
+        G[0] = c * ddy;
+        G[1] = c * ddx;
+        G[2] = Ag * (T(-2) * ia) + c * r2 * ia;
         G[3] = Ag * (T(1) / (be - T(1)) - lu);
-        G[4] = prof / norm;
+        G[4] = prof * inorm;
         G[5] = T(1);
     }
 }
@@ -121,6 +127,10 @@ __device__ __forceinline__ void model_eval(const LMModel<T>& Md, const CornerRec
 }
 
 __device__ __forceinline__ double pick6(const double* v, int k) {
+    return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : k == 3 ? v[3] : k == 4 ? v[4] : v[5];
+}
+template <typename T>
+__device__ __forceinline__ T pick6t(const T* v, int k) {
     return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : k == 3 ? v[3] : k == 4 ? v[4] : v[5];
 }
 
@@ -172,15 +182,18 @@ __device__ __forceinline__ void accum(const LMConst& C, const LMModel<T>& Md, co
         const T r = f - (T)v;
         const double wr = w * (double)r;
         acc.cost = fma(wr, (double)r, acc.cost);
-        const double G6[6] = {(double)G[0], (double)G[1], (double)G[2], (double)G[3], (double)G[4], (double)G[5]};
-        double g[NP];
+        // the free gradients, selected in the storage precision, then widened
+        double g[NP], wg[NP];
 #pragma unroll
-        for (int j = 0; j < NP; ++j) g[j] = pick6(G6, C.fidx[j]);
+        for (int j = 0; j < NP; ++j) {
+            g[j] = (double)pick6t<T>(G, C.fidx[j]);
+            wg[j] = w * g[j];
+        }
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
             acc.b[j] = fma(wr, g[j], acc.b[j]);
 #pragma unroll
-            for (int i = 0; i <= j; ++i) acc.A[tri(i, j)] = fma(w * g[i], g[j], acc.A[tri(i, j)]);
+            for (int i = 0; i <= j; ++i) acc.A[tri(i, j)] = fma(wg[i], g[j], acc.A[tri(i, j)]);
         }
     }
 #pragma unroll
