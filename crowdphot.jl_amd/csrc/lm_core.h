@@ -45,7 +45,8 @@ __device__ __forceinline__ void model_fg(const T* prm, T ddy, T ddx, T& f, T G[6
         // (y, x, fwhm, flux, bkg), evaluate_fg parametric_models.jl:92-113
         T fwhm = prm[2], flux = prm[3], bkg = prm[4];
         T f2 = fwhm * fwhm;
-        T sq = (ddx * ddx + ddy * ddy) / f2;
+        T if2 = T(1) / f2, ifw = T(1) / fwhm;   // loop invariant in accum!
+        T sq = (ddx * ddx + ddy * ddy) * if2;
         T norm = -(T(M_PI) * f2 / T(CP_GAUSS_PRE));
         T amp = flux / norm;
         T g = exp(T(CP_GAUSS_PRE) * sq);
@@ -54,7 +55,7 @@ __device__ __forceinline__ void model_fg(const T* prm, T ddy, T ddx, T& f, T G[6
         T gf2 = T(CP_GAUSS_PRE) / f2;
         G[0] = T(-2) * Ag * gf2 * ddy;
         G[1] = T(-2) * Ag * gf2 * ddx;
-        G[2] = T(-2) * Ag * (T(1) + T(CP_GAUSS_PRE) * sq) / fwhm;
+        G[2] = T(-2) * Ag * (T(1) + T(CP_GAUSS_PRE) * sq) * ifw;
         G[3] = g / norm;
         G[4] = T(1);
         G[5] = T(0);
@@ -209,6 +210,8 @@ __device__ __forceinline__ double Aget(const NE<NP>& e, int i, int j) {
 }
 
 // Cholesky of a packed symmetric NP x NP; false if not positive definite.
+// The diagonal of L holds the reciprocals of the pivots (one division per
+// column; chol_solve multiplies by them).
 template <int NP>
 __device__ bool chol(const double* Ap, double L[NP][NP]) {
 #pragma unroll
@@ -217,14 +220,14 @@ __device__ bool chol(const double* Ap, double L[NP][NP]) {
 #pragma unroll
         for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
         if (!(s > 0)) return false;
-        double dd = sqrt(s);
-        L[j][j] = dd;
+        const double idd = 1.0 / sqrt(s);
+        L[j][j] = idd;
 #pragma unroll
         for (int i = j + 1; i < NP; ++i) {
             double t = Ap[tri(j, i)];
 #pragma unroll
             for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
-            L[i][j] = t / dd;
+            L[i][j] = t * idd;
         }
     }
     return true;
@@ -238,14 +241,14 @@ __device__ void chol_solve(const double L[NP][NP], const double* b, double* x) {
         double s = b[i];
 #pragma unroll
         for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
-        y[i] = s / L[i][i];
+        y[i] = s * L[i][i];
     }
 #pragma unroll
     for (int i = NP - 1; i >= 0; --i) {
         double s = y[i];
 #pragma unroll
         for (int k = i + 1; k < NP; ++k) s -= L[k][i] * x[k];
-        x[i] = s / L[i][i];
+        x[i] = s * L[i][i];
     }
 }
 
