@@ -210,14 +210,17 @@ int cp_evaluate_fg(cp_ctx* ctx, const cp_psf_desc* psf, const double* prm, const
             CP_ARG(prm[2] > 0, "fwhm must be positive");
             double f2 = prm[2] * prm[2];
             M.fwhm2 = f2;
+            M.inv_fwhm2 = 1.0 / f2;
             M.inv_norm_g = 1.0 / (-(M_PI * f2 / CP_GAUSS_PRE));
             hipLaunchKernelGGL(k_evaluate<KIND_GAUSS>, g, dim3(BLOCK), 0, st, M, DP.G, prm[0], prm[1], F, prm[2],
                                0.0, 0.0, dy.as<double>(), dx.as<double>(), n, df.as<double>(), dG.as<double>());
         } else if (P.kind == CP_PSF_CIRCULAR_MOFFAT) {
             double a = prm[2], b = prm[3];
             M.alpha2 = a * a;
+            M.inv_alpha2 = 1.0 / (a * a);
             M.beta = b;
             M.norm_m = M_PI * a * a / (b - 1);
+            M.inv_norm_m = (b - 1) / (M_PI * a * a);
             hipLaunchKernelGGL(k_evaluate<KIND_MOFFAT>, g, dim3(BLOCK), 0, st, M, DP.G, prm[0], prm[1], F, 0.0, a,
                                b, dy.as<double>(), dx.as<double>(), n, df.as<double>(), dG.as<double>());
         } else {

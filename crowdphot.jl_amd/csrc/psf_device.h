@@ -28,6 +28,7 @@ template <typename T>
 struct ModelConst {
     T fwhm2, inv_norm_g;   // gauss: fwhm^2 and 1/norm with norm = -(pi fwhm^2 / GAUSS_PRE)
     T alpha2, beta, norm_m;  // moffat: alpha^2, beta, norm = pi alpha^2 / (beta - 1)
+    T inv_fwhm2, inv_alpha2, inv_norm_m;   // their reciprocals (no division per pixel)
     // ePSF geometry (all nodes share it)
     int gny, gnx, n_nodes;
     T osx, osy, ox, oy, fill;
@@ -293,11 +294,11 @@ __device__ __forceinline__ T star_value(const ModelConst<T>& M, const StarRec<T>
                                         const float* __restrict__ grid,
                                         const CornerRec<T>* __restrict__ cr) {
     if constexpr (KIND == KIND_GAUSS) {
-        T sq = (ddx * ddx + ddy * ddy) / M.fwhm2;
+        T sq = (ddx * ddx + ddy * ddy) * M.inv_fwhm2;
         return fma(r.amp, exp(T(CP_GAUSS_PRE) * sq), M.bkg);
     } else if constexpr (KIND == KIND_MOFFAT) {
         T r2 = ddx * ddx + ddy * ddy;
-        return fma(r.amp, pow(T(1) + r2 / M.alpha2, -M.beta), M.bkg);
+        return fma(r.amp, pow(fma(r2, M.inv_alpha2, T(1)), -M.beta), M.bkg);
     } else {
         T u = M.osx * ddx + M.ox;
         T v = M.osy * ddy + M.oy;
@@ -323,22 +324,23 @@ __device__ __forceinline__ void star_grad(const ModelConst<T>& M, const StarRec<
                                           const float* __restrict__ grid,
                                           const CornerRec<T>* __restrict__ cr, T& gy, T& gx, T& gf) {
     if constexpr (KIND == KIND_GAUSS) {
-        T sq = (ddx * ddx + ddy * ddy) / M.fwhm2;
+        T sq = (ddx * ddx + ddy * ddy) * M.inv_fwhm2;
         T g = exp(T(CP_GAUSS_PRE) * sq);
         T Ag = r.amp * g;
-        T gf2 = T(CP_GAUSS_PRE) / M.fwhm2;
+        T gf2 = T(CP_GAUSS_PRE) * M.inv_fwhm2;
         gy = T(-2) * Ag * gf2 * ddy;
         gx = T(-2) * Ag * gf2 * ddx;
         gf = g * M.inv_norm_g;
     } else if constexpr (KIND == KIND_MOFFAT) {
         T r2 = ddx * ddx + ddy * ddy;
-        T u = T(1) + r2 / M.alpha2;
+        T u = fma(r2, M.inv_alpha2, T(1));
         T prof = pow(u, -M.beta);
-        T amp = r.flux / M.norm_m;
+        T amp = r.flux * M.inv_norm_m;
         T Ag = amp * prof;
-        gy = T(2) * Ag * M.beta * ddy / (M.alpha2 * u);
-        gx = T(2) * Ag * M.beta * ddx / (M.alpha2 * u);
-        gf = prof / M.norm_m;
+        T c = T(2) * Ag * M.beta * M.inv_alpha2 / u;   // 2 A g beta / (alpha^2 u)
+        gy = c * ddy;
+        gx = c * ddx;
+        gf = prof * M.inv_norm_m;
     } else {
         T u = M.osx * ddx + M.ox;
         T v = M.osy * ddy + M.oy;

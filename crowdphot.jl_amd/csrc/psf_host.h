@@ -145,11 +145,14 @@ ModelConst<T> model_const(const HostPsf& P, const DevPsf& D, double bkg) {
         double f2 = P.fwhm * P.fwhm;
         double norm = -(M_PI * f2 / GP);
         M.fwhm2 = (T)f2;
+        M.inv_fwhm2 = (T)(1.0 / f2);
         M.inv_norm_g = (T)(1.0 / norm);
     } else if (P.kind == CP_PSF_CIRCULAR_MOFFAT) {
         M.alpha2 = (T)(P.alpha * P.alpha);
+        M.inv_alpha2 = (T)(1.0 / (P.alpha * P.alpha));
         M.beta = (T)P.beta;
         M.norm_m = (T)(M_PI * P.alpha * P.alpha / (P.beta - 1));
+        M.inv_norm_m = (T)((P.beta - 1) / (M_PI * P.alpha * P.alpha));
     } else {
         M.gny = P.gny;
         M.gnx = P.gnx;
