@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--lm-cutouts", type=int, default=1_000_000,
                     help="batched per-star LM cutouts (0 skips)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes instead of tuning them on the workload first")
     ap.add_argument("--no-plummer", action="store_true",
                     help="skip the 8192^2 field with 1e6 Plummer-clustered stars")
     ap.add_argument("--no-sequential", action="store_true", help="skip the sequential fit_all_stars section")
@@ -191,6 +193,13 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] profile-only fit done: passes {s32.n_passes}")
         return
+    tuning = None
+    if not args.no_autotune:
+        # launch shapes (blocks per CU of J v, J^T u and the renders) picked by
+        # timing whole fits of this workload, before the timed region
+        t0 = time.time()
+        tuning, tbest = pkg.autotune.tune(ctx, lambda: fit(o32, s32), repeats=2)
+        log(f"[bench] autotuned launch shapes in {time.time() - t0:.1f}s: {tuning} ({tbest * 1e3:.2f} ms)")
     for i in range(args.warmup):
         fit(o32, s32)
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
@@ -252,7 +261,8 @@ def main():
     serving = None
     if not args.no_serving:
         serving = serving_bench(pkg, args.serve_requests, args.serve_slots)
-    batched_lm = lm_bench(pkg, ctx, args.lm_cutouts, args.no_cpu_baseline) if args.lm_cutouts > 0 else None
+    batched_lm = (lm_bench(pkg, ctx, args.lm_cutouts, args.no_cpu_baseline, not args.no_autotune)
+                  if args.lm_cutouts > 0 else None)
     detection = None
     if not args.no_detection:
         detection = detection_bench(pkg, ctx, cfg, d_img, d_iv, args.no_cpu_baseline)
@@ -286,6 +296,7 @@ def main():
         "config": {"workload": f"crowded ePSF field {ny}x{nx}, {n} stars, ePSF 101x101 os4, fit_rad 3, "
                                f"model_rad 12, 10 outer GN iterations, LSQR 10 inner",
                    "stars": n, "pixels": ny * nx, "outer_iterations": int(s32.n_passes),
+                   "launch_shapes": tuning or "built-in defaults",
                    "pixels_per_s": round(ny * nx * args.steps / (t_end - t_start), 1)},
         "accuracy": {"reference": "float64 device run, same inputs", "frac_within_1e-4_flux_1e-3_px": within,
                      "median_rel_flux": float(np.median(rel)), "p99_rel_flux": float(np.quantile(rel, 0.99)),
@@ -515,7 +526,7 @@ def plummer_bench(pkg, ctx, steps=2):
     return res
 
 
-def lm_bench(pkg, ctx, n, no_cpu, steps=3):
+def lm_bench(pkg, ctx, n, no_cpu, autotune, steps=3):
     """Batched per-star Levenberg-Marquardt over uneven isolated-star cutouts
     (odd sides 9..31, CircularMoffat with FWHM U(1.8, 3.0), beta 2.5 fixed,
     free y, x, flux, bkg, Poisson noise), packed and resident in HBM.
@@ -552,6 +563,13 @@ def lm_bench(pkg, ctx, n, no_cpu, steps=3):
 
     fit()
     torch.cuda.synchronize()
+    shape = None
+    if autotune:
+        def run():
+            fit()
+            torch.cuda.synchronize()
+        shape, _ = pkg.autotune.tune(ctx, run, space=pkg.autotune.LM_SPACE, repeats=2)
+        log(f"[lm] autotuned lm_blocks_per_cu = {shape['lm_blocks_per_cu']}")
     t = time.perf_counter()
     for _ in range(steps):
         fit()
@@ -570,6 +588,7 @@ def lm_bench(pkg, ctx, n, no_cpu, steps=3):
     res = {"workload": f"{n} isolated-star cutouts, odd sides 9..31 (mean {s2.mean():.1f} px), CircularMoffat "
                        f"FWHM U(1.8, 3.0) beta 2.5, free (y, x, flux, bkg), Poisson, unweighted",
            "fits_per_s": round(n / dt, 1), "ms_per_batch": round(dt * 1e3, 3),
+           "lm_blocks_per_cu": shape["lm_blocks_per_cu"] if shape else "built-in default (8)",
            "pixels_per_s": round(float(s2.sum()) / dt, 1),
            "mean_iterations": round(float(st["iterations"].mean()), 3),
            "converged_frac": round(float(st["converged"].mean()), 5),

@@ -16,5 +16,5 @@ from .psf import (CircularGaussianPSF, CircularMoffatPSF, GriddedPSFModel,  # no
                   ImagePSF, moffat_epsf, pixel_response_kernel)
 from .detection import (CircularGaussianPRF, GaussianPRF, MatchedFilterResult, correlate,  # noqa: F401
                         findlocalmaxima, matched_filter, render)
-from . import roman_crds, serving, simulation  # noqa: F401
+from . import autotune, roman_crds, serving, simulation  # noqa: F401
 from .roman_crds import read_asdf, roman_crds_gridded_epsf  # noqa: F401

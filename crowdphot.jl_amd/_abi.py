@@ -153,6 +153,16 @@ class LMOpts(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """cp_tuning: blocks per compute unit of the grid-stride kernels (0 = default)."""
+    _fields_ = [
+        ("jv_blocks_per_cu", C.c_int32),
+        ("jtu_blocks_per_cu", C.c_int32),
+        ("render_blocks_per_cu", C.c_int32),
+        ("lm_blocks_per_cu", C.c_int32),
+    ]
+
+
 class LMStat(C.Structure):
     _fields_ = [
         ("minimum", C.c_double),
@@ -248,6 +258,8 @@ SIGNATURES = {
     "cp_stream": (C.c_void_p, [C.c_void_p]),
     "cp_synchronize": (C.c_int, [C.c_void_p]),
     "cp_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "cp_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "cp_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "cp_profile_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "cp_sim_opts_default": (None, [C.POINTER(SimOpts)]),
     "cp_lm_opts_default": (None, [C.POINTER(LMOpts)]),

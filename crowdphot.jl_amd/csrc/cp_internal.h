@@ -16,6 +16,9 @@
 
 namespace cp {
 
+// Blocks per CU of a launch: the context's tuned value, else the default.
+inline int64_t tuned_bpc(int32_t v, int64_t dflt) { return v > 0 ? (int64_t)v : dflt; }
+
 // Thrown inside the library, converted to a return code at the C boundary.
 struct Error : std::runtime_error {
     int code;
@@ -54,6 +57,7 @@ struct cp_ctx {
     void* stage = nullptr;
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int n_cu = 0;
+    cp_tuning tuning{};   // launch shapes (0 = default), cp_set_tuning
     // per-kernel HIP-event timing (cp_profile)
     bool prof = false;
     struct Pending {

@@ -85,7 +85,8 @@ void lm_batch_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_pix, co
     upload_psf(ctx, psf, P, DP);
     DevBuf work(ctx, 4);
     CP_HIP(hipMemsetAsync(work.get(), 0, 4, st));
-    dim3 g((unsigned)std::min<int64_t>((n + NWAVE - 1) / NWAVE, (int64_t)std::max(ctx->n_cu, 1) * 8));
+    dim3 g((unsigned)std::min<int64_t>((n + NWAVE - 1) / NWAVE,
+                                       (int64_t)std::max(ctx->n_cu, 1) * tuned_bpc(ctx->tuning.lm_blocks_per_cu, 8)));
     auto go = [&](auto tag, auto kind) {
         using T = decltype(tag);
         constexpr int K = decltype(kind)::value;

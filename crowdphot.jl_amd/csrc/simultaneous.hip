@@ -571,11 +571,15 @@ struct Sim {
 
         // two 1024-thread render / fill blocks fit one CU (LDS and wave slots);
         // render blocks take equal shares of the tile-cost scan
-        const int gb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * 4);
-        const int gb_jv = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * 8));
+        // (blocks per CU: the context's tuned launch shapes, else these defaults)
+        const cp_tuning& tu = ctx->tuning;
+        const int gb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * tuned_bpc(tu.render_blocks_per_cu, 4));
+        const int gb_jv = (int)std::max<int64_t>(
+            1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * tuned_bpc(tu.jv_blocks_per_cu, 8)));
         const int GLs = S2 <= 256 ? 16 : 64;
         const int gb_fill = (int)std::min<int64_t>(cdiv(na, FBLOCK / 64), (int64_t)n_cu * 2);
-        const int gb_jtu = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(na, BLOCK / GLs), (int64_t)n_cu * 8));
+        const int gb_jtu = (int)std::max<int64_t>(
+            1, std::min<int64_t>(cdiv(na, BLOCK / GLs), (int64_t)n_cu * tuned_bpc(tu.jtu_blocks_per_cu, 8)));
         const unsigned int gstar = cdiv(na, BLOCK);
         // per-block partials of the widest launch (two sets for fill / accept)
         const int maxpb = (int)std::max<int64_t>(

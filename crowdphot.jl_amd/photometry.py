@@ -51,6 +51,22 @@ class Context:
     def stream(self):
         return self.lib.cp_stream(self.handle)
 
+    def get_tuning(self):
+        """The context's launch shapes (cp_tuning) as a dict; 0 = built-in default."""
+        t = _abi.Tuning()
+        self.check(self.lib.cp_get_tuning(self.handle, C.byref(t)))
+        return {name: int(getattr(t, name)) for name, _ in _abi.Tuning._fields_}
+
+    def set_tuning(self, **shape):
+        """Set launch shapes by cp_tuning field name (blocks per CU, 0 = default);
+        fields not given keep their current value."""
+        cur = self.get_tuning()
+        unknown = set(shape) - set(cur)
+        if unknown:
+            raise ArgumentError(f"unknown tuning fields {sorted(unknown)}; known: {sorted(cur)}")
+        cur.update({k: int(v) for k, v in shape.items()})
+        self.check(self.lib.cp_set_tuning(self.handle, C.byref(_abi.Tuning(**cur))))
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.cp_destroy(self.handle)

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     structs = {"cp_psf_desc": abi.PsfDesc, "cp_sim_opts": abi.SimOpts, "cp_sim_result": abi.SimResult,
                "cp_lm_opts": abi.LMOpts, "cp_lm_stat": abi.LMStat, "cp_request": abi.Request,
-               "cp_request_stat": abi.RequestStat}
+               "cp_request_stat": abi.RequestStat, "cp_tuning": abi.Tuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -94,4 +94,7 @@ def test_server_argument_errors_without_gpu():
 def test_null_context_is_rejected_without_gpu():
     lib = abi.load_library()
     assert lib.cp_synchronize(None) == abi.CP_ESTATE
+    t = abi.Tuning()
+    assert lib.cp_get_tuning(None, C.byref(t)) == abi.CP_ESTATE
+    assert lib.cp_set_tuning(None, C.byref(t)) == abi.CP_ESTATE
     assert lib.cp_fit_simultaneous(None, None, None, None, 0, 0, None, None, None, 0, 1.0, None, None) == abi.CP_ESTATE

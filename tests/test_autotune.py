@@ -1,0 +1,62 @@
+"""The launch-shape autotuner's search and cache (CPU; a stand-in context
+records the shapes, and the timed call sleeps for a cost that depends on the
+shape, so the search has a known optimum)."""
+import json
+import time
+
+from _support import load_pkg
+
+pkg = load_pkg()
+at = pkg.autotune
+
+
+class FakeCtx:
+    def __init__(self):
+        self.t = {k: 0 for k in ("jv_blocks_per_cu", "jtu_blocks_per_cu", "render_blocks_per_cu",
+                                 "lm_blocks_per_cu")}
+        self.history = []
+
+    def get_tuning(self):
+        return dict(self.t)
+
+    def set_tuning(self, **kw):
+        self.t.update(kw)
+        self.history.append(dict(self.t))
+
+
+def _run_for(ctx, best):
+    def run():
+        cost = sum(abs(ctx.t[k] - v) for k, v in best.items())
+        time.sleep(0.002 + 0.004 * cost / 16)
+    return run
+
+
+def test_coordinate_descent_finds_the_known_optimum():
+    ctx = FakeCtx()
+    best = {"jv_blocks_per_cu": 16, "jtu_blocks_per_cu": 4, "render_blocks_per_cu": 4}
+    shape, t = at.tune(ctx, _run_for(ctx, best), repeats=2)
+    for k, v in best.items():
+        assert shape[k] == v, (k, shape)
+    assert ctx.get_tuning()["jv_blocks_per_cu"] == 16   # left set on the context
+    assert t < 0.01
+
+
+def test_defaults_are_the_starting_point_and_kept_without_gain():
+    ctx = FakeCtx()
+    best = dict(at.DEFAULTS)
+    shape, _ = at.tune(ctx, _run_for(ctx, {k: best[k] for k in at.LM_SPACE}), space=at.LM_SPACE, repeats=2)
+    assert shape["lm_blocks_per_cu"] == at.DEFAULTS["lm_blocks_per_cu"]
+    assert ctx.history[0]["lm_blocks_per_cu"] == at.DEFAULTS["lm_blocks_per_cu"]
+
+
+def test_cache_round_trip(tmp_path):
+    path = str(tmp_path / "tuning.json")
+    ctx = FakeCtx()
+    best = {"jv_blocks_per_cu": 4, "jtu_blocks_per_cu": 16, "render_blocks_per_cu": 8}
+    shape = at.cached(ctx, "crowded/4096", _run_for(ctx, best), path)
+    assert {k: shape[k] for k in best} == best
+    assert json.load(open(path))["crowded/4096"] == shape
+    ctx2 = FakeCtx()
+    calls = []
+    shape2 = at.cached(ctx2, "crowded/4096", lambda: calls.append(1), path)
+    assert shape2 == shape and calls == [] and ctx2.get_tuning() == shape

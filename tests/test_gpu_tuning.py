@@ -1,0 +1,79 @@
+"""Launch shapes (cp_tuning) on the GPU: range checks, the batched per-star
+LM unchanged bit for bit under another grid, the simultaneous fit under
+another shape still on the FP64 oracle, and the autotuner on a real call."""
+import numpy as np
+import pytest
+
+from _support import load_pkg
+
+pkg = load_pkg()
+sim = pkg.simulation
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture()
+def ctx():
+    c = pkg.Context(0)
+    yield c
+    c.close()
+
+
+def test_tuning_round_trip_and_range(ctx):
+    assert set(ctx.get_tuning().values()) == {0}
+    ctx.set_tuning(jv_blocks_per_cu=4, lm_blocks_per_cu=2)
+    t = ctx.get_tuning()
+    assert t["jv_blocks_per_cu"] == 4 and t["lm_blocks_per_cu"] == 2 and t["jtu_blocks_per_cu"] == 0
+    with pytest.raises(pkg.ArgumentError, match="blocks per CU"):
+        ctx.set_tuning(render_blocks_per_cu=65)
+    with pytest.raises(pkg.ArgumentError, match="unknown tuning"):
+        ctx.set_tuning(tile=3)
+    assert ctx.get_tuning() == t
+
+
+def test_batched_lm_identical_under_other_grids(ctx):
+    c = sim.config_cutouts(seed=3, n=3000)
+    model = pkg.CircularMoffatPSF(alpha=1.0, beta=c["beta"])
+    cut = [c["pix"][o:o + s * s].reshape(s, s).T for o, s in zip(c["offsets"], c["ny"])]
+    ref = pkg.fit_star_batch(model, cut, c["theta0"], fixed=("alpha", "beta"), ctx=ctx)
+    for bpc in (1, 2, 16):
+        ctx.set_tuning(lm_blocks_per_cu=bpc)
+        got = pkg.fit_star_batch(model, cut, c["theta0"], fixed=("alpha", "beta"), ctx=ctx)
+        np.testing.assert_array_equal(got[0], ref[0])
+        np.testing.assert_array_equal(got[1], ref[1])
+
+
+def test_simultaneous_under_other_shapes_matches_oracle(ctx):
+    """The Gaussian-field parity case (256x256, 200 stars) and a crowded ePSF
+    crop under launch shapes far from the defaults: the same bars as with the
+    default shape."""
+    from test_gpu_parity import FLUX_RTOL, POS_ATOL, _compare, _gpu_run, _oracle_run
+    cfg = sim.config_gauss_small(seed=1)
+    cfg["image"] = cfg["image"].astype(np.float32).astype(np.float64)
+    ro = _oracle_run(cfg)
+    n = int(round(2e5 * (384 * 384) / 4096 ** 2))
+    crowd = sim.config_epsf_crowded(seed=5, shape=(384, 384), n=n, max_iter=10)
+    ref = _gpu_run(ctx, crowd, "mixed")
+    for shape in ({"jv_blocks_per_cu": 1, "jtu_blocks_per_cu": 16, "render_blocks_per_cu": 1},
+                  {"jv_blocks_per_cu": 32, "jtu_blocks_per_cu": 2, "render_blocks_per_cu": 8}):
+        ctx.set_tuning(**shape)
+        _compare(_gpu_run(ctx, cfg, "fp64"), ro, 1e-7, 1e-7)
+        _compare(_gpu_run(ctx, cfg, "mixed"), ro, FLUX_RTOL, POS_ATOL)
+        # crowded field: against the default shape's own run
+        _compare(_gpu_run(ctx, crowd, "mixed"), ref, FLUX_RTOL, POS_ATOL, frac=0.98)
+
+
+def test_autotune_on_a_real_call(ctx):
+    c = sim.config_cutouts(seed=5, n=2000)
+    model = pkg.CircularMoffatPSF(alpha=1.0, beta=c["beta"])
+    cut = [c["pix"][o:o + s * s].reshape(s, s).T for o, s in zip(c["offsets"], c["ny"])]
+    ref = pkg.fit_star_batch(model, cut, c["theta0"], fixed=("alpha", "beta"), ctx=ctx)
+
+    def run():
+        pkg.fit_star_batch(model, cut, c["theta0"], fixed=("alpha", "beta"), ctx=ctx)
+
+    shape, t = pkg.autotune.tune(ctx, run, space=pkg.autotune.LM_SPACE, repeats=1)
+    assert shape["lm_blocks_per_cu"] in pkg.autotune.LM_SPACE["lm_blocks_per_cu"] and t > 0
+    assert ctx.get_tuning()["lm_blocks_per_cu"] == shape["lm_blocks_per_cu"]
+    got = pkg.fit_star_batch(model, cut, c["theta0"], fixed=("alpha", "beta"), ctx=ctx)
+    np.testing.assert_array_equal(got[0], ref[0])
