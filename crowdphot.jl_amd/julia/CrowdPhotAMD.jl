@@ -132,6 +132,21 @@ function check(ctx::Context, rc)
     error("libcrowdphot_hip error $rc: $msg")
 end
 
+# Launch shapes of a context (cp_tuning: blocks per CU, 0 = built-in default).
+mutable struct CpTuning
+    jv_blocks_per_cu::Int32; jtu_blocks_per_cu::Int32; render_blocks_per_cu::Int32; lm_blocks_per_cu::Int32
+end
+CpTuning() = CpTuning(0, 0, 0, 0)
+
+function get_tuning(ctx::Context)
+    t = CpTuning()
+    check(ctx, ccall(sym(:cp_get_tuning), Cint, (Ptr{Cvoid}, Ref{CpTuning}), ctx.handle, t))
+    return t
+end
+
+set_tuning!(ctx::Context, t::CpTuning) =
+    check(ctx, ccall(sym(:cp_set_tuning), Cint, (Ptr{Cvoid}, Ref{CpTuning}), ctx.handle, t))
+
 # PSF descriptor of a CrowdPhot model (fields read by name; `fixed` shape
 # values override the model's own, as model_from_vector does).
 function psf_desc(psf, fixed, keep)
