@@ -1288,26 +1288,42 @@ static __global__ void k_freeze_all(int na, int iter, uint8_t* __restrict__ live
 
 // Per-star error block, covariance and errors (:1305-1329, covariance!
 // levenberg_marquardt.jl:366-388).  cov_mode 1 known, 2 reweighted, -1 none.
+// ERR_GL lanes per star stride over its stamp rows (consecutive rows, so a
+// group's loads are contiguous) and sum the block's distinct entries over the
+// group; every lane of the group then runs the same small inverse and the
+// group's first lane writes.
+constexpr int ERR_GL = 16;
 template <typename T, int P>
-static __global__ void k_errors(int na, int R, int S, const int* __restrict__ ay, const int* __restrict__ ax,
-                         const T* __restrict__ w, int ny, int nx, const T* __restrict__ V,
-                         const double* __restrict__ colnorm, int cov_mode, double fac_rw,
-                         double* __restrict__ errs) {
-    int a = blockIdx.x * BLOCK + threadIdx.x;
-    if (a >= na) return;
+static __global__ __launch_bounds__(BLOCK) void k_errors(int na, int R, int S, const int* __restrict__ ay,
+                                                         const int* __restrict__ ax, const T* __restrict__ w,
+                                                         int ny, int nx, const T* __restrict__ V,
+                                                         const double* __restrict__ colnorm, int cov_mode,
+                                                         double fac_rw, double* __restrict__ errs) {
+    const int a = blockIdx.x * (BLOCK / ERR_GL) + threadIdx.x / ERR_GL;
+    const int sub = threadIdx.x % ERR_GL;
+    if (a >= na) return;   // uniform over the group: the group sums stay inside it
     constexpr int p = P;
     const int S2 = S * S;
     double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int m = 0; m < S2; ++m) {
-        int gy = ay[a] + (m % S) - R, gx = ax[a] + (m / S) - R;
+    const int ya = ay[a], xa = ax[a];
+    for (int m = sub; m < S2; m += ERR_GL) {
+        const int dx = m / S;
+        const int gy = ya + (m - dx * S) - R, gx = xa + dx - R;
         if (gy < 1 || gy > ny || gx < 1 || gx > nx) continue;
         if (!(w[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] > T(0))) continue;
         const T* Vm = V + ((size_t)a * S2 + m) * VP;
 #pragma unroll
         for (int k = 0; k < p; ++k)
 #pragma unroll
-            for (int l = 0; l < p; ++l) blk[k * 3 + l] += (double)Vm[k] * (double)Vm[l];
+            for (int l = k; l < p; ++l) blk[k * 3 + l] += (double)Vm[k] * (double)Vm[l];
     }
+#pragma unroll
+    for (int k = 0; k < p; ++k)
+#pragma unroll
+        for (int l = k; l < p; ++l) {
+            blk[k * 3 + l] = group_sum<ERR_GL>(blk[k * 3 + l]);
+            blk[l * 3 + k] = blk[k * 3 + l];
+        }
     const double* cn = colnorm + (size_t)a * p;
     double tr = 0;
 #pragma unroll
@@ -1416,8 +1432,9 @@ static __global__ void k_errors(int na, int R, int S, const int* __restrict__ ay
             }
     }
     double fac = cov_mode == 2 ? fac_rw : 1.0;
+    if (sub == 0)
 #pragma unroll
-    for (int k = 0; k < p; ++k) errs[(size_t)a * p + k] = sqrt(fmax(0.0, fac * cov[k * 3 + k]));
+        for (int k = 0; k < p; ++k) errs[(size_t)a * p + k] = sqrt(fmax(0.0, fac * cov[k * 3 + k]));
 }
 
 // Post-fit per-star statistics over the +-fit_rad box at the fitted position

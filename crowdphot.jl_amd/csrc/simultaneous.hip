@@ -835,7 +835,7 @@ struct Sim {
         CP_HIP(hipMemsetAsync(errs.get(), 0, nb8, st));
         if (cov_mode != CP_COV_NONE) {
 #define CP_ERR(PP)                                                                                             \
-    launch(k_errors<T, PP>, dim3(gstar), dim3(BLOCK), 0, st, na, R, S, (const int*)ay.as<int>(),               \
+    launch(k_errors<T, PP>, dim3(cdiv(na, BLOCK / ERR_GL)), dim3(BLOCK), 0, st, na, R, S, (const int*)ay.as<int>(), \
            (const int*)ax.as<int>(), (const T*)w.as<T>(), ny, nx, (const T*)V.as<T>(),                          \
            (const double*)colnorm.as<double>(), cov_mode, cost / dof, errs.as<double>())
             if (p == 1) CP_ERR(1);
