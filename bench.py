@@ -108,6 +108,8 @@ def main():
                     help="one matched_filter call, for rocprofv3 runs")
     ap.add_argument("--profile-sequential", action="store_true",
                     help="one sequential 3-pass fit_all_stars, for rocprofv3 runs")
+    ap.add_argument("--profile-lm", action="store_true",
+                    help="one batched per-star LM call over the --lm-cutouts batch, for rocprofv3 runs")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -180,6 +182,10 @@ def main():
     if args.profile_sequential:
         sequential_bench(pkg, ctx, cfg, d_img, d_iv, True, steps=0)
         log("[bench] profile-only sequential fit done")
+        return
+    if args.profile_lm:
+        lm_bench(pkg, ctx, args.lm_cutouts, True, False, steps=0)
+        log("[bench] profile-only batched LM done")
         return
     if args.profile_only:
         # calibration: one plain device copy of 256 MiB (known read and write
@@ -563,6 +569,8 @@ def lm_bench(pkg, ctx, n, no_cpu, autotune, steps=3):
 
     fit()
     torch.cuda.synchronize()
+    if steps == 0:
+        return None
     shape = None
     if autotune:
         def run():
