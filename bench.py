@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-crop", type=int, default=1536)
+    ap.add_argument("--cpu-crop", type=int, default=2048)
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--stars", type=int, default=200_000)
     ap.add_argument("--profile-only", action="store_true", help="one fit, for rocprofv3 runs")

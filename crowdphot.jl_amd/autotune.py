@@ -43,29 +43,34 @@ def tune(ctx, run, space=None, repeats=2, rounds=1, log=None):
     performs one complete, synchronous call.  Returns (shape, seconds) with
     the shape set on ctx."""
     space = DEFAULT_SPACE if space is None else space
-    shape = ctx.get_tuning()
+    before = ctx.get_tuning()
+    shape = dict(before)
     for k in space:
         if shape.get(k, 0) == 0:
             shape[k] = DEFAULTS[k]
-    ctx.set_tuning(**shape)
-    run()   # warm-up: allocations, code loading
-    best_t = _time(run, repeats)
-    for _ in range(rounds):
-        changed = False
-        for field, cands in space.items():
-            for v in cands:
-                if v == shape[field]:
-                    continue
-                trial = dict(shape, **{field: v})
-                ctx.set_tuning(**trial)
-                t = _time(run, repeats)
-                if log:
-                    log(f"[tune] {field}={v}: {t * 1e3:.2f} ms (best {best_t * 1e3:.2f} ms)")
-                if t < best_t:
-                    best_t, shape, changed = t, trial, True
-            ctx.set_tuning(**shape)
-        if not changed:
-            break
+    try:
+        ctx.set_tuning(**shape)
+        run()   # warm-up: allocations, code loading
+        best_t = _time(run, repeats)
+        for _ in range(rounds):
+            changed = False
+            for field, cands in space.items():
+                for v in cands:
+                    if v == shape[field]:
+                        continue
+                    trial = dict(shape, **{field: v})
+                    ctx.set_tuning(**trial)
+                    t = _time(run, repeats)
+                    if log:
+                        log(f"[tune] {field}={v}: {t * 1e3:.2f} ms (best {best_t * 1e3:.2f} ms)")
+                    if t < best_t:
+                        best_t, shape, changed = t, trial, True
+                ctx.set_tuning(**shape)
+            if not changed:
+                break
+    except BaseException:
+        ctx.set_tuning(**before)   # a failing call leaves the context as it was
+        raise
     ctx.set_tuning(**shape)
     return shape, best_t
 

@@ -60,3 +60,23 @@ def test_cache_round_trip(tmp_path):
     calls = []
     shape2 = at.cached(ctx2, "crowded/4096", lambda: calls.append(1), path)
     assert shape2 == shape and calls == [] and ctx2.get_tuning() == shape
+
+
+def test_failing_call_restores_the_shape():
+    ctx = FakeCtx()
+    ctx.set_tuning(jv_blocks_per_cu=16)
+    before = ctx.get_tuning()
+    n = []
+
+    def run():
+        n.append(1)
+        if len(n) > 3:
+            raise RuntimeError("device error")
+
+    try:
+        at.tune(ctx, run)
+    except RuntimeError:
+        pass
+    else:
+        raise AssertionError("the call's error must propagate")
+    assert ctx.get_tuning() == before
