@@ -1448,6 +1448,7 @@ static __global__ __launch_bounds__(BLOCK) void k_errors(int na, int R, int S, c
 // OUT_CHISQ + 6 (chisq, qfit, qfit_expected, qfit_z, crowding, spread_model,
 // spread_model_err; NaN where the reference leaves NaN) of catalogue star
 // act[a]; invalid stars keep what k_results wrote (chisq 0, the rest NaN).
+constexpr int DIAG_GL = 16;      // lanes per star in k_diag
 constexpr int DIAG_MAXB = 1024;   // box pixels: (2 ceil(fit_rad) + 2)^2 for fit_rad <= 15
 
 // Result columns (float64, catalogue order, n each), then n_iter (int32),
@@ -1552,16 +1553,19 @@ static __global__ __launch_bounds__(64) void k_diag(int na, const double* __rest
                                                     double moffat_amp64, NodeGrid G, const T* __restrict__ resid,
                                                     const T* __restrict__ data, const T* __restrict__ w,
                                                     int has_iv, int ny, int nx, const double* __restrict__ K,
-                                                    int kh, const int* __restrict__ act, int n,
+                                                    int kh, const int* __restrict__ act, int n, int maxb,
                                                     double* __restrict__ cols) {
-    __shared__ T sm[DIAG_MAXB];
-    const int a = blockIdx.x, lane = threadIdx.x;
-    if (a >= na) return;
+    // DIAG_GL lanes per star, 64 / DIAG_GL stars per block, each with its
+    // own box of model values in LDS (maxb entries).  The only barrier comes
+    // before any lane leaves; group sums stay inside a group.
+    extern __shared__ __align__(16) unsigned char dsm[];
+    const int grp = threadIdx.x / DIAG_GL, lane = threadIdx.x % DIAG_GL;
+    const int a = blockIdx.x * (64 / DIAG_GL) + grp;
+    T* sm = reinterpret_cast<T*>(dsm) + grp * maxb;
     const size_t N = (size_t)n;
-    double* o = cols + OUT_CHISQ * N + act[a];   // o[k * N]: column OUT_CHISQ + k
-    double Y, X, F;
-    star_yxf(fm, theta + (size_t)a * fm.p, Y, X, F);
-    const bool valid = isfinite(Y) && isfinite(X) && isfinite(F) && F > 0;
+    double Y = 0, X = 0, F = 0;
+    if (a < na) star_yxf(fm, theta + (size_t)a * fm.p, Y, X, F);
+    const bool valid = a < na && isfinite(Y) && isfinite(X) && isfinite(F) && F > 0;
     int ylo = 1, yhi = 0, xlo = 1, xhi = 0;
     if (valid) {
         ylo = max((int)floor(Y - fit_rad), 1);
@@ -1570,25 +1574,27 @@ static __global__ __launch_bounds__(64) void k_diag(int na, const double* __rest
         xhi = min((int)ceil(X + fit_rad), nx);
     }
     const int by = yhi - ylo + 1, bx = xhi - xlo + 1;
-    if (!valid || by <= 0 || bx <= 0) return;
-    const int nb = by * bx;
+    const bool go = valid && by > 0 && bx > 0;
+    const int nb = go ? by * bx : 0;
     StarRec<T> r{};
     r.flux = (T)F;
     if (KIND == KIND_GAUSS) r.amp = (T)(F * inv_norm_g64);
     else if (KIND == KIND_MOFFAT) r.amp = (T)(F * moffat_amp64);
     else r.amp = (T)F;
     CornerRec<T> cr;
-    if (KIND == KIND_EPSF) grid_corners<T>(G, M.n_nodes, Y, X, cr);
-    for (int m = lane; m < nb; m += 64) {
+    if (KIND == KIND_EPSF && go) grid_corners<T>(G, M.n_nodes, Y, X, cr);
+    for (int m = lane; m < nb; m += DIAG_GL) {
         const int iy = m % by, ix = m / by;
         sm[m] = star_value<T, KIND>(M, r, (T)((ylo + iy) - Y), (T)((xlo + ix) - X), M.grids, &cr);
     }
     __syncthreads();
+    if (!go) return;
+    double* o = cols + OUT_CHISQ * N + act[a];   // o[k * N]: column OUT_CHISQ + k
     const double bkg = (double)M.bkg, inv_flux = 1.0 / F;
     double chi = 0, qfit = 0, nclean = 0, ndirty = 0, dcrowd = 0, Sp = 0, Sq = 0, Sp0 = 0, Sq0 = 0, SqG = 0;
     double ssum = 0, s2sum = 0, ngood = 0;
     const int kw = 2 * kh + 1;
-    for (int m = lane; m < nb; m += 64) {
+    for (int m = lane; m < nb; m += DIAG_GL) {
         const int iy = m % by, ix = m / by;
         const size_t q = (size_t)(ylo + iy - 1) + (size_t)(xlo + ix - 1) * ny;
         const double wp = (double)w[q];
@@ -1625,19 +1631,19 @@ static __global__ __launch_bounds__(64) void k_diag(int na, const double* __rest
             s2sum += sg * sg;
         }
     }
-    chi = wave_sum(chi);
-    ngood = wave_sum(ngood);
-    qfit = wave_sum(qfit);
-    nclean = wave_sum(nclean);
-    ndirty = wave_sum(ndirty);
-    dcrowd = wave_sum(dcrowd);
-    Sp = wave_sum(Sp);
-    Sq = wave_sum(Sq);
-    Sp0 = wave_sum(Sp0);
-    Sq0 = wave_sum(Sq0);
-    SqG = wave_sum(SqG);
-    ssum = wave_sum(ssum);
-    s2sum = wave_sum(s2sum);
+    chi = group_sum<DIAG_GL>(chi);
+    ngood = group_sum<DIAG_GL>(ngood);
+    qfit = group_sum<DIAG_GL>(qfit);
+    nclean = group_sum<DIAG_GL>(nclean);
+    ndirty = group_sum<DIAG_GL>(ndirty);
+    dcrowd = group_sum<DIAG_GL>(dcrowd);
+    Sp = group_sum<DIAG_GL>(Sp);
+    Sq = group_sum<DIAG_GL>(Sq);
+    Sp0 = group_sum<DIAG_GL>(Sp0);
+    Sq0 = group_sum<DIAG_GL>(Sq0);
+    SqG = group_sum<DIAG_GL>(SqG);
+    ssum = group_sum<DIAG_GL>(ssum);
+    s2sum = group_sum<DIAG_GL>(s2sum);
     if (lane != 0) return;
     const int p = fm.p;
     o[0] = ngood - p > 0 ? chi / (ngood - p) : 0.0;

@@ -895,10 +895,12 @@ struct Sim {
                 dK.alloc(ctx, hK.size() * 8);
                 CP_HIP(hipMemcpyAsync(dK.get(), hK.data(), hK.size() * 8, hipMemcpyHostToDevice, st));
             }
-            launch(k_diag<T, KIND>, dim3(na), dim3(64), 0, st, na, (const double*)theta.as<double>(), fm, fit_rad, M,
-                   inv_norm_g64, moffat_amp64, DP.G, (const T*)resid, (const T*)data.as<T>(), (const T*)w.as<T>(),
-                   d_iv ? 1 : 0, ny, nx, (const double*)(dK.get() ? dK.as<double>() : nullptr), kh,
-                   (const int*)act.as<int>(), n, cols);
+            const int dmaxb = (2 * (int)std::ceil(fit_rad) + 2) * (2 * (int)std::ceil(fit_rad) + 2);
+            launch(k_diag<T, KIND>, dim3(cdiv(na, 64 / DIAG_GL)), dim3(64),
+                   (size_t)(64 / DIAG_GL) * dmaxb * sizeof(T), st, na, (const double*)theta.as<double>(), fm,
+                   fit_rad, M, inv_norm_g64, moffat_amp64, DP.G, (const T*)resid, (const T*)data.as<T>(),
+                   (const T*)w.as<T>(), d_iv ? 1 : 0, ny, nx, (const double*)(dK.get() ? dK.as<double>() : nullptr),
+                   kh, (const int*)act.as<int>(), n, dmaxb, cols);
             if (!hK.empty()) CP_HIP(hipStreamSynchronize(st));   // hK is pageable and local
         }
 
