@@ -795,6 +795,8 @@ def sequential_bench(pkg, ctx, cfg, d_img, d_iv, no_cpu, steps=3):
                        f"(fit_rad 3, bkg fixed at 0, fit_star defaults: max_iter 200)",
            "stars_per_s": round(n / dt, 1), "ms_per_fit": round(dt * 1e3, 2),
            "valid": int(valid.sum()), "mean_iterations_per_star": round(float(nit.mean()), 3),
+           "iterations_per_star_p50_p99_max": [int(np.percentile(nit, 50)), int(np.percentile(nit, 99)),
+                                               int(nit.max())],
            "kernels_ms": {k: round(v[1], 3) for k, v in kernels.items()},
            "fit_kernel_launches": km[0],
            "accuracy": {"reference": "float64 device run, same inputs",
