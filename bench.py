@@ -13,7 +13,7 @@ a fit).  Extra fields: roofline of the dominant kernel (HIP-event timing of
 every launch during one instrumented fit), accuracy against the float64
 device run, and the CPU baseline (the FP64 oracle on a bounded crop).
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
 """
 import argparse
 import ctypes as C
@@ -86,8 +86,8 @@ def measured_traffic(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-crop", type=int, default=2048)
     ap.add_argument("--side", type=int, default=4096)
