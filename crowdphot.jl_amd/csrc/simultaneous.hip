@@ -291,7 +291,7 @@ struct Sim {
 #define CP_FILL(E, G)                                                                                        \
     launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G>, grid, dim3(FBLOCK), sh, st, M, recs, ers, crs, lst, na, R, S, fm, \
              w, ny, nx, wres, V, cn, b, gloc, gtol, part, Sc)
-        if (S2 <= 64) CP_FILL(1, 64);
+        if (S2 <= 64) CP_FILL(4, 16);
         else if (S2 <= 128) CP_FILL(2, 64);
         else if (S2 <= 256) CP_FILL(4, 64);
         else if (S2 <= 512) CP_FILL(8, 64);
