@@ -247,7 +247,12 @@ class RequestStat(C.Structure):
         ("slot", C.c_int32),
         ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
         ("hbm_bytes", C.c_uint64),
+        ("hbm_peak", C.c_uint64),
     ]
+
+
+class MemStats(C.Structure):
+    _fields_ = [("budget", C.c_uint64), ("in_use", C.c_uint64), ("cached", C.c_uint64), ("peak", C.c_uint64)]
 
 
 # Every symbol include/crowdphot_hip.h declares, with (restype, argtypes).
@@ -261,6 +266,9 @@ SIGNATURES = {
     "cp_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "cp_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "cp_profile_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "cp_mem_stats_get": (C.c_int, [C.c_void_p, C.POINTER(MemStats)]),
+    "cp_set_budget": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "cp_release_cached": (C.c_int, [C.c_void_p]),
     "cp_sim_opts_default": (None, [C.POINTER(SimOpts)]),
     "cp_lm_opts_default": (None, [C.POINTER(LMOpts)]),
     "cp_psf_nparams": (C.c_int, [C.c_int32]),

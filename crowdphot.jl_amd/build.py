@@ -18,7 +18,7 @@ HIP_SOURCES = ["context.hip", "simultaneous.hip", "lm_batch.hip", "multipass.hip
 CPP_SOURCES = ["simulate.cpp", "server.cpp", "diagnostics.cpp", "aperture.cpp", "host_copy.cpp"]
 # packed FP32 instructions are kept out of every HIP object
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature",
-             "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics", "-Wno-unused-result"]
+             "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 CPP_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-pthread", "-march=x86-64-v3"]
 
 

@@ -173,6 +173,41 @@ int cp_set_tuning(cp_ctx* c, const cp_tuning* t) {
     });
 }
 
+int cp_mem_stats_get(cp_ctx* c, cp_mem_stats* out) {
+    if (!c) return CP_ESTATE;
+    return guarded(c, [&] {
+        CP_ARG(out != nullptr, "NULL argument");
+        out->budget = c->budget;
+        out->in_use = c->in_use;
+        out->cached = c->cached;
+        out->peak = c->peak;
+    });
+}
+
+int cp_set_budget(cp_ctx* c, uint64_t budget) {
+    if (!c) return CP_ESTATE;
+    return guarded(c, [&] {
+        CP_HIP(hipSetDevice(c->device));
+        c->budget = budget;
+        // cached blocks beyond the new cap go back to the device; the peak
+        // restarts from what is in use now
+        if (budget && c->in_use + c->cached > budget) {
+            CP_HIP(hipStreamSynchronize(c->stream));
+            cache_trim(c, 0);
+        }
+        c->peak = c->in_use;
+    });
+}
+
+int cp_release_cached(cp_ctx* c) {
+    if (!c) return CP_ESTATE;
+    return guarded(c, [&] {
+        CP_HIP(hipSetDevice(c->device));
+        CP_HIP(hipStreamSynchronize(c->stream));
+        cache_trim(c, UINT64_MAX);
+    });
+}
+
 int cp_profile(cp_ctx* c, int enable) {
     if (!c) return CP_ESTATE;
     return guarded(c, [&] {

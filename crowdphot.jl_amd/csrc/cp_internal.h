@@ -137,6 +137,13 @@ class DevBuf {
                                        " left of " + std::to_string(c->budget));
         size_t got = 0;
         void* p = cache_take(c, g, &got);
+        if (p && c->budget && c->in_use + got > c->budget) {
+            // the cached block is larger than what the budget leaves: put it
+            // back and allocate the granule itself (the trim may free it)
+            c->cache.emplace(got, p);
+            c->cached += got;
+            p = nullptr;
+        }
         if (p) {
             g = got;   // charge the block's real size
         } else {

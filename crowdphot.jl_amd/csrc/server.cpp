@@ -36,8 +36,10 @@ struct Job {
     int state = 0;   // 0 queued, 1 running, 2 done
     int rc = CP_OK;
     int slot = -1;
+    bool waited = false;   // a thread is blocked in cp_server_wait on it
     std::string msg;
     double t_submit = 0, t_start = 0, t_end = 0;
+    uint64_t peak = 0;
 };
 
 }  // namespace
@@ -46,13 +48,17 @@ struct cp_server {
     int device = 0;
     uint64_t budget = 0;
     std::vector<cp_ctx*> ctxs;
+    // per slot: running, and the bytes it holds against the budget (the
+    // running request's estimate, or an idle slot's cached blocks)
+    std::vector<uint8_t> busy;
+    std::vector<uint64_t> charge;
     std::vector<std::thread> workers;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<uint64_t> queue;
     std::unordered_map<uint64_t, Job> jobs;
     uint64_t next_ticket = 1;
-    uint64_t in_flight = 0;
+    int n_waiters = 0;
     bool stop = false;
     std::string err;
 };
@@ -60,20 +66,52 @@ struct cp_server {
 extern "C" uint64_t cp_request_bytes(int32_t ny, int32_t nx, int32_t n_stars, double fit_rad, double model_rad,
                                      int32_t precision, int32_t device_inputs) {
     if (ny <= 0 || nx <= 0 || n_stars < 0 || !(fit_rad > 0)) return 0;
+    // an upper bound on the device blocks one simultaneous fit holds at once
+    // (its largest phase is the LSQR trials), by buffer family
     const double T = precision == CP_PREC_FP64 ? 8.0 : 4.0;
     const double P = (double)ny * nx;
     const double R = std::ceil(fit_rad), S2 = (2 * R + 1) * (2 * R + 1);
     const double mR = model_rad > 0 ? std::ceil(std::max(model_rad, fit_rad)) : 10.0 * R;
-    const double n = n_stars;
-    double b = P * (6 * T + 1);                               // data, data_work, w, 2 residuals, u, union mask
+    const double n = n_stars, E = n * S2, U = std::min(P, E);
+    double b = P * (6 * T + 1 + 4);                           // data, data_work, w, 2 residuals, u; union mask;
+                                                              // float32 residual staging
+    b += P * 8;                                               // row-major inputs transposed on the device
     if (!device_inputs) b += P * 8 + n * 24;                 // staged image, weights, catalogue
-    b += n * S2 * 4 * T;                                      // stamp Jacobian rows
-    b += n * S2 * 25;                                         // pixel-major stamp structure + its sort buffers
+    b += E * (4 * T + T);                                     // stamp Jacobian rows, u at the stamp rows
+    b += E * 12 + U * 4;                                      // pixel-major stamp structure (pix, ptr, ent)
+    b += E * 12 + U * 20;                                     // its live compaction (flags, scans) and u
     const double tiles = std::pow((2 * mR + 1) / 64.0 + 1.0, 2);
-    b += n * tiles * 12 * 2;                                  // model-box tile lists + sort buffers
-    b += n * 800 + (4 << 20);                                 // per-star records and vectors, partials
-    return (uint64_t)(b * 1.1);
+    b += n * tiles * 16 + 4 * P / 4096 * 24;                  // model-box tile lists, live copies, scans
+    b += n * 1100 + (8 << 20);                                // per-star records and vectors, partials, results
+    return (uint64_t)(b * 1.15);
 }
+
+namespace {
+
+// Bytes the other slots hold against the budget.
+uint64_t held_by_others(const cp_server* s, int slot) {
+    uint64_t o = 0;
+    for (int j = 0; j < (int)s->ctxs.size(); ++j)
+        if (j != slot) o += s->charge[j];
+    return o;
+}
+
+// Whether the request at the head of the queue may start on `slot` (called
+// with the server lock held).  Idle slots release their cached blocks when
+// the request would not fit beside them.
+bool admissible(cp_server* s, int slot, uint64_t need) {
+    if (!s->budget) return true;
+    if (held_by_others(s, slot) + need <= s->budget) return true;
+    for (int j = 0; j < (int)s->ctxs.size(); ++j)
+        if (j != slot && !s->busy[j] && s->charge[j]) {
+            (void)cp_release_cached(s->ctxs[j]);
+            cp_mem_stats m{};
+            s->charge[j] = cp_mem_stats_get(s->ctxs[j], &m) == CP_OK ? m.cached : 0;
+        }
+    return held_by_others(s, slot) + need <= s->budget;
+}
+
+}  // namespace
 
 static void worker_main(cp_server* s, int slot) {
     cp_ctx* ctx = s->ctxs[slot];
@@ -85,17 +123,20 @@ static void worker_main(cp_server* s, int slot) {
             s->cv_work.wait(lk, [&] {
                 if (s->stop) return true;
                 if (s->queue.empty()) return false;
-                const Job& h = s->jobs[s->queue.front()];
-                return s->in_flight == 0 || !s->budget || s->in_flight + h.bytes <= s->budget;
+                return admissible(s, slot, s->jobs[s->queue.front()].bytes);
             });
             if (s->stop) return;
             t = s->queue.front();
             s->queue.pop_front();
-            job = &s->jobs[t];
+            job = &s->jobs[t];   // element references survive rehashing
             job->state = 1;
             job->slot = slot;
             job->t_start = now_s();
-            s->in_flight += job->bytes;
+            s->busy[slot] = 1;
+            s->charge[slot] = job->bytes;
+            // the slot's blocks (cached ones included) stay within the estimate
+            if (s->budget) (void)cp_set_budget(ctx, job->bytes);
+            else (void)cp_set_budget(ctx, 0);
             // the next queued request may fit beside this one
             s->cv_work.notify_one();
         }
@@ -107,13 +148,17 @@ static void worker_main(cp_server* s, int slot) {
         else
             rc = cp_fit_simultaneous(ctx, &job->psf, r.image, r.inv_var, r.ny, r.nx, r.y0, r.x0, r.flux0,
                                      r.n_stars, r.fit_rad, &job->opts, r.out);
+        cp_mem_stats m{};
+        (void)cp_mem_stats_get(ctx, &m);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             job->rc = rc;
             if (rc != CP_OK) job->msg = cp_last_error(ctx);
             job->t_end = now_s();
+            job->peak = m.peak;
             job->state = 2;
-            s->in_flight -= job->bytes;
+            s->busy[slot] = 0;
+            s->charge[slot] = s->budget ? m.cached : 0;
         }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
@@ -139,6 +184,8 @@ int cp_server_create(cp_server** out, int32_t device, uint64_t hbm_budget, int32
         }
         s->ctxs.push_back(c);
     }
+    s->busy.assign(n_slots, 0);
+    s->charge.assign(n_slots, 0);
     for (int i = 0; i < n_slots; ++i) s->workers.emplace_back(worker_main, s, i);
     *out = s;
     return CP_OK;
@@ -197,8 +244,14 @@ int cp_server_wait(cp_server* s, uint64_t ticket, cp_request_stat* st, char* msg
         s->err = "cp_server_wait: unknown ticket";
         return CP_EARG;
     }
-    s->cv_done.wait(lk, [&] { return it->second.state == 2; });
-    const Job& j = it->second;
+    Job& j = it->second;   // a reference, not the iterator: submits may rehash the map
+    if (j.waited) {
+        s->err = "cp_server_wait: another thread is already waiting on this ticket";
+        return CP_ESTATE;
+    }
+    j.waited = true;
+    ++s->n_waiters;
+    s->cv_done.wait(lk, [&] { return j.state == 2; });
     if (st) {
         st->rc = j.rc;
         st->slot = j.slot;
@@ -206,13 +259,16 @@ int cp_server_wait(cp_server* s, uint64_t ticket, cp_request_stat* st, char* msg
         st->t_start = j.t_start;
         st->t_end = j.t_end;
         st->hbm_bytes = j.bytes;
+        st->hbm_peak = j.peak;
     }
     if (msg && msg_cap > 0) {
         std::strncpy(msg, j.msg.c_str(), (size_t)msg_cap - 1);
         msg[msg_cap - 1] = 0;
     }
     const int rc = j.rc;
-    s->jobs.erase(it);
+    s->jobs.erase(ticket);
+    --s->n_waiters;
+    if (s->stop) s->cv_done.notify_all();   // cp_server_destroy waits for the last waiter
     return rc;
 }
 
@@ -223,10 +279,23 @@ void cp_server_destroy(cp_server* s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
+        // queued requests never run: complete them so their waiters return
+        for (uint64_t t : s->queue) {
+            Job& j = s->jobs[t];
+            j.state = 2;
+            j.rc = CP_ESTATE;
+            j.msg = "the server was destroyed before this request ran";
+            j.t_start = j.t_end = now_s();
+        }
         s->queue.clear();
     }
     s->cv_work.notify_all();
-    for (auto& w : s->workers) w.join();
+    s->cv_done.notify_all();
+    for (auto& w : s->workers) w.join();   // running requests finish and notify
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return s->n_waiters == 0; });
+    }
     for (auto* c : s->ctxs) cp_destroy(c);
     delete s;
 }

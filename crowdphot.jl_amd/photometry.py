@@ -67,6 +67,17 @@ class Context:
         cur.update({k: int(v) for k, v in shape.items()})
         self.check(self.lib.cp_set_tuning(self.handle, C.byref(_abi.Tuning(**cur))))
 
+    def mem_stats(self):
+        """Device-block accounting of the context (cp_mem_stats): budget, in_use,
+        cached and peak bytes."""
+        m = _abi.MemStats()
+        self.check(self.lib.cp_mem_stats_get(self.handle, C.byref(m)))
+        return {name: int(getattr(m, name)) for name, _ in _abi.MemStats._fields_}
+
+    def set_budget(self, hbm_budget):
+        """Cap the context's device blocks at hbm_budget bytes (0 = no cap); restarts the peak."""
+        self.check(self.lib.cp_set_budget(self.handle, int(hbm_budget)))
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.cp_destroy(self.handle)

@@ -153,6 +153,37 @@ def config_plummer(seed=4, shape=(8192, 8192), n=1_000_000, scale=1500.0, max_it
                 max_iter=max_iter)
 
 
+def config_plummer_crop(seed=4, side=512, offset=(0.0, 0.0), n_full=1_000_000, full=8192, scale=1500.0,
+                        max_iter=10):
+    """A side x side crop of the Plummer-clustered 8192^2 field (1e6 stars,
+    scale radius 1500 px) at the same local density: the crop's lower corner
+    sits at the field centre minus side / 2, shifted by `offset` pixels.  Stars
+    of the full population whose light reaches the crop are rendered into it;
+    the catalogue holds those inside [8, side - 8] of the crop (crop
+    coordinates), perturbed like every other workload."""
+    rn = 4.0
+    c = (full + 1) / 2
+    y, x, f = plummer_sources(seed, n_full, scale, (c, c), 8.0, full - 8.0)
+    y0c = math.floor(c - side / 2 + offset[0])
+    x0c = math.floor(c - side / 2 + offset[1])
+    yy, xx = y - y0c, x - x0c
+    near = (yy > -13) & (yy < side + 13) & (xx > -13) & (xx < side + 13)
+    yy, xx, ff = yy[near], xx[near], f[near]
+    psf = epsf_c2()
+    gy = np.arange(1, side + 1)[:, None] + y0c
+    gx = np.arange(1, side + 1)[None, :] + x0c
+    bkg = 50.0 + 0.01 * (gx + gy)
+    truth = render_sources(psf, (side, side), yy, xx, ff, 12.0) + bkg
+    img = (add_noise(truth, seed, 1.0, rn) - bkg).astype(np.float32)
+    inv_var = (1.0 / (bkg + rn * rn)).astype(np.float32)
+    inside = (yy >= 8) & (yy <= side - 8) & (xx >= 8) & (xx <= side - 8)
+    ty, tx, tf = yy[inside], xx[inside], ff[inside]
+    py, px, pf = _perturb(seed, ty, tx, tf)
+    return dict(image=img, inv_var=inv_var, psf=psf, fixed={"bkg": 0.0},
+                sources={"y": py, "x": px, "flux": pf}, truth=(ty, tx, tf), fit_rad=3, model_rad=12.0,
+                max_iter=max_iter, n_rendered=int(near.sum()))
+
+
 def config_cutouts(seed=3, n=1_000_000, beta=2.5):
     """Isolated-star cutouts with odd sides uniform over 9..31, CircularMoffat
     (FWHM ~ U(1.8, 3.0), beta fixed), flux log-uniform 1e2-1e5, background

@@ -188,6 +188,20 @@ typedef struct cp_tuning {
 int cp_get_tuning(cp_ctx* ctx, cp_tuning* out);
 int cp_set_tuning(cp_ctx* ctx, const cp_tuning* t);
 
+/* Device-memory accounting, no reference counterpart (Julia's GC owns the
+ * reference's arrays): every buffer of a call comes from the context's
+ * cache of device blocks.  in_use = blocks held by a running call, cached =
+ * released blocks kept for reuse, peak = the most in_use since the last
+ * cp_set_budget.  cp_set_budget changes the cap (0 = none), frees cached
+ * blocks beyond it and restarts peak; cp_release_cached frees every cached
+ * block.  The server uses them to keep each slot within its admission. */
+typedef struct cp_mem_stats {
+    uint64_t budget, in_use, cached, peak;
+} cp_mem_stats;
+int cp_mem_stats_get(cp_ctx* ctx, cp_mem_stats* out);
+int cp_set_budget(cp_ctx* ctx, uint64_t hbm_budget_bytes);
+int cp_release_cached(cp_ctx* ctx);
+
 void cp_sim_opts_default(cp_sim_opts* o);
 void cp_lm_opts_default(cp_lm_opts* o);
 
@@ -380,8 +394,14 @@ double cp_pixel_overlap(double dx, double dy, double r);
  * request gets alone (its own buffers and stream; fixed-order reductions).
  * The caller keeps every array of a request (inputs, the psf grids and node
  * coordinates, the result arrays) alive until cp_server_wait returns for it.
- * cp_server_destroy finishes running requests and drops queued ones; call it
- * only when no thread is waiting. */
+ * HBM: while a request runs, its slot context is capped at the request's
+ * estimate (cp_request_bytes, an upper bound on what the fit allocates), and
+ * an idle slot is charged for the device blocks it keeps cached; admission
+ * keeps the sum over slots within hbm_budget, releasing idle slots' caches
+ * when the next request needs the room.  cp_server_destroy finishes running
+ * requests, completes queued ones with CP_ESTATE and returns once every
+ * thread blocked in cp_server_wait has been woken; at most one thread may
+ * wait on a ticket. */
 typedef struct cp_server cp_server;
 
 typedef struct cp_request {
@@ -402,6 +422,7 @@ typedef struct cp_request_stat {
     int32_t slot;               /* worker slot that ran it */
     double t_submit, t_start, t_end;   /* seconds, one monotonic host clock */
     uint64_t hbm_bytes;         /* admission estimate charged against the budget */
+    uint64_t hbm_peak;          /* device bytes the fit actually held at most */
 } cp_request_stat;
 
 uint64_t cp_request_bytes(int32_t ny, int32_t nx, int32_t n_stars, double fit_rad, double model_rad,

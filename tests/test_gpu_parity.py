@@ -182,9 +182,9 @@ def test_crowded_epsf_crop_parity(ctx):
     cfg = sim.config_epsf_crowded(seed=5, shape=(384, 384), n=n, max_iter=10)
     ro = _oracle_run(cfg)
     r64 = _gpu_run(ctx, cfg, "fp64")
-    _compare(r64, ro, 1e-6, 1e-6, frac=0.99)
+    _compare(r64, ro, 1e-6, 1e-6)            # every star, same validity
     r32 = _gpu_run(ctx, cfg, "mixed")
-    _compare(r32, ro, FLUX_RTOL, POS_ATOL, frac=0.98)
+    _compare(r32, ro, FLUX_RTOL, POS_ATOL, frac=0.99)
 
 
 def test_deterministic_repeat(ctx):

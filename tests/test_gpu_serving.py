@@ -83,3 +83,42 @@ def test_submit_errors():
             srv.submit(t["image"], t["psf"], t["sources"], t["fit_rad"], **_kw(t))
         with pytest.raises(pkg.ArgumentError):
             srv.submit(t["image"], t["psf"], t["sources"], -1.0, **_kw(t))
+
+
+def test_admitted_requests_stay_within_their_estimate():
+    reqs = _requests()
+    with pkg.serving.Server(0, 1 << 40, slots=3) as srv:
+        tickets = [srv.submit(t["image"], t["psf"], t["sources"], t["fit_rad"], **_kw(t)) for t in reqs]
+        for k in tickets:
+            _, st = srv.wait(k)
+            assert 0 < st["hbm_peak"] <= st["hbm_bytes"], (st["hbm_peak"], st["hbm_bytes"])
+
+
+def test_destroy_completes_queued_requests_and_wakes_waiters():
+    import ctypes as C
+    import threading
+    reqs = _requests() * 2   # a queue long enough that some requests are still waiting at destroy
+    srv = pkg.serving.Server(0, 0, slots=1)
+    tickets = [srv.submit(t["image"], t["psf"], t["sources"], t["fit_rad"], **_kw(t)) for t in reqs]
+    rcs = {}
+
+    def waiter(k):
+        st = abi.RequestStat()
+        msg = C.create_string_buffer(256)
+        rcs[k] = srv.lib.cp_server_wait(srv.handle, k, C.byref(st), msg, 256)
+
+    th = [threading.Thread(target=waiter, args=(k,)) for k in tickets[1:]]
+    for t in th:
+        t.start()
+    # a second waiter on a ticket that already has one is refused at once
+    import time
+    time.sleep(0.05)
+    st = abi.RequestStat()
+    assert srv.lib.cp_server_wait(srv.handle, tickets[-1], C.byref(st), None, 0) == abi.CP_ESTATE
+    srv.lib.cp_server_destroy(srv.handle)   # returns only after every waiter woke
+    srv.handle = None
+    for t in th:
+        t.join(timeout=60)
+        assert not t.is_alive()
+    assert all(rc in (abi.CP_OK, abi.CP_ESTATE) for rc in rcs.values()), rcs
+    assert abi.CP_ESTATE in rcs.values(), "no queued request was left for destroy to complete"
