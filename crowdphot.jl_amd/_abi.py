@@ -23,6 +23,7 @@ CP_PSF_GRIDDED_IMAGE = 2
 
 CP_SOLVER_LSQR = 0
 CP_SOLVER_CGLS = 1
+CP_SOLVER_LSMR = 2
 
 CP_PREC_MIXED = 0
 CP_BORDER_REPLICATE = 0

@@ -70,7 +70,8 @@ typedef struct cp_psf_desc {
  * own solver / precision choices.  cp_sim_opts_default fills the reference
  * defaults. */
 #define CP_SOLVER_LSQR 0       /* reference default (Krylov.jl :lsqr) */
-#define CP_SOLVER_CGLS 1       /* CG on the damped normal equations */
+#define CP_SOLVER_CGLS 1       /* CG on the damped normal equations (the build's own; J'J v fused) */
+#define CP_SOLVER_LSMR 2       /* Krylov.jl :lsmr (Fong & Saunders 2011) */
 
 #define CP_PREC_MIXED 0        /* FP32 pixels/Jacobian/residuals, FP64 reductions + star vectors */
 #define CP_PREC_FP64 1         /* everything FP64 (parity runs) */

@@ -10,8 +10,10 @@
 //   binning / Morton / ring / model radii .... pinned (closed-form definitions,
 //                                             reference invariants)
 //   J, J^T ................................... pinned (dense-J identities)
-//   LSQR / CGLS .............................. pinned to the damped normal
-//       equations and to the exact Krylov-subspace minimiser at every k;
+//   LSQR / LSMR / CG ......................... pinned to the damped normal
+//       equations, to the exact Krylov-subspace minimisers at every k (LSQR
+//       and CG: the residual norm; LSMR: the normal-equation residual) and,
+//       for LSQR and LSMR, to scipy's implementations iterate by iterate;
 //       Krylov.jl 0.10's own stopping-rule trajectory is parity unpinned
 //       (the package is not available here).
 //   outer Gauss-Newton loop .................. pinned by noiseless recovery and
@@ -528,15 +530,120 @@ int lsqr(const Op& A, const double* b, double lam, int itmax, double atol, doubl
     return iter;
 }
 
-// CGLS (Hestenes-Stiefel CG on (J'J + lam^2 I) x = J'b, Bjorck 1996 s7.4).
-// The build's stopping rule (CGLS has no reference counterpart): at least one
-// step, then stop when ||s_k|| <= atol * ||s_0|| or after itmax steps.
+// LSMR, Fong & Saunders, SIAM J. Sci. Comput. 33(5):2950-2971 (2011), with
+// damping lam and the stopping tests of the Krylov.jl implementation the
+// reference calls (solver = :lsmr, psf_photometry_simultaneous.jl:1164,
+// 1221-1224): atol on ||A'r||, btol / axtol on ||r||, conlim, and the 5-deep
+// forward-error window on zeta.  ||x|| is the norm of the current iterate.
+int lsmr(const Op& A, const double* b, double lam, int itmax, double atol, double btol, double* x) {
+    const int64_t m = A.m, n = A.n;
+    const double axtol = std::sqrt(EPS64), etol = std::sqrt(EPS64);
+    const double ctol = 1.0 / (1.0 / std::sqrt(EPS64));
+    const int window = 5;
+    std::vector<double> u(b, b + m), v(n), h(n), hbar(n, 0.0), Av(m), Atu(n), err_vec(window, 0.0);
+    std::fill(x, x + n, 0.0);
+    double beta1 = nrm2(u.data(), m);
+    if (beta1 == 0) return 0;
+    for (auto& t : u) t /= beta1;
+    A.JT(u.data(), v.data());
+    double alpha = nrm2(v.data(), n);
+    if (alpha == 0) return 0;
+    for (auto& t : v) t /= alpha;
+    h = v;
+    double zetabar = alpha * beta1, alphabar = alpha, rho = 1, rhobar = 1, cbar = 1, sbar = 0;
+    double betadd = beta1, betad = 0, rhodold = 1, tautildeold = 0, thetatilde = 0, zeta = 0, d = 0;
+    double Anorm2 = alpha * alpha, maxrbar = 0, minrbar = 1e100;
+    double xENorm2 = 0, err_lbnd = 0;
+    int iter = 0;
+    bool done = false;
+    while (!done) {
+        ++iter;
+        // Golub-Kahan step: beta u = A v - alpha u, alpha v = A' u - beta v
+        A.J(v.data(), Av.data());
+        for (int64_t i = 0; i < m; ++i) u[i] = Av[i] - alpha * u[i];
+        double beta = nrm2(u.data(), m);
+        if (beta != 0) {
+            for (auto& t : u) t /= beta;
+            A.JT(u.data(), Atu.data());
+            for (int64_t i = 0; i < n; ++i) v[i] = Atu[i] - beta * v[i];
+            alpha = nrm2(v.data(), n);
+            if (alpha != 0)
+                for (auto& t : v) t /= alpha;
+        }
+        // the damping rotation, then the two QR rotations
+        double chat, shat, alphahat;
+        sym_givens(alphabar, lam, chat, shat, alphahat);
+        const double rhoold = rho;
+        double c, s;
+        sym_givens(alphahat, beta, c, s, rho);
+        const double thetanew = s * alpha;
+        alphabar = c * alpha;
+        const double rhobarold = rhobar, zetaold = zeta;
+        const double thetabar = sbar * rho, rhotemp = cbar * rho;
+        sym_givens(rhotemp, thetanew, cbar, sbar, rhobar);
+        zeta = cbar * zetabar;
+        zetabar = -sbar * zetabar;
+        xENorm2 += zeta * zeta;
+        err_vec[iter % window] = zeta;
+        if (iter >= window) err_lbnd = nrm2(err_vec.data(), window);
+        // h, hbar and x
+        const double delta = thetabar * rho / (rhoold * rhobarold);
+        const double sigma = zeta / (rho * rhobar), eta = thetanew / rho;
+        for (int64_t i = 0; i < n; ++i) {
+            hbar[i] = h[i] - delta * hbar[i];
+            x[i] += sigma * hbar[i];
+            h[i] = v[i] - eta * h[i];
+        }
+        // ||r|| estimate
+        const double betaacute = chat * betadd, betacheck = -shat * betadd;
+        const double betahat = c * betaacute;
+        betadd = -s * betaacute;
+        const double thetatildeold = thetatilde;
+        double ctildeold, stildeold, rhotildeold;
+        sym_givens(rhodold, thetabar, ctildeold, stildeold, rhotildeold);
+        thetatilde = stildeold * rhobar;
+        rhodold = ctildeold * rhobar;
+        betad = -stildeold * betad + ctildeold * betahat;
+        tautildeold = (zetaold - thetatildeold * tautildeold) / rhotildeold;
+        const double taud = (zeta - thetatilde * tautildeold) / rhodold;
+        d += betacheck * betacheck;
+        const double rNorm = std::sqrt(d + (betad - taud) * (betad - taud) + betadd * betadd);
+        // ||A|| and cond(A)
+        Anorm2 += beta * beta;
+        const double Anorm = std::sqrt(Anorm2);
+        Anorm2 += alpha * alpha;
+        maxrbar = std::max(maxrbar, rhobarold);
+        if (iter > 1) minrbar = std::min(minrbar, rhobarold);
+        const double Acond = std::max(maxrbar, rhotemp) / std::min(minrbar, rhotemp);
+        // tests
+        const double ArNorm = std::fabs(zetabar);
+        const double xNorm = nrm2(x, n);
+        const double test1 = rNorm / beta1;
+        const double test2 = ArNorm / (Anorm * rNorm);
+        const double test3 = 1 / Acond;
+        const double t1 = test1 / (1 + Anorm * xNorm / beta1);
+        const double rNormtol = btol + axtol * Anorm * xNorm / beta1;
+        const bool ill = (1 + test3 <= 1) || (test3 <= ctol);
+        const bool solved = (1 + test2 <= 1) || (test2 <= axtol) || (ArNorm <= atol) || (1 + t1 <= 1) ||
+                            (test1 <= rNormtol) || (iter >= window && err_lbnd <= etol * std::sqrt(xENorm2));
+        done = solved || ill || iter >= itmax;
+    }
+    return iter;
+}
+
+// CG (Hestenes-Stiefel) on the damped normal equations (J'J + lam^2 I) x = J'b:
+// the search direction p, its image q = J p, z = J'q + lam^2 p = N p, the
+// normal-equation residual s updated by s <- s - a z (the recurrence the
+// device runs), gamma = |s|^2 computed from s itself.  The build's stopping
+// rule (CG has no reference counterpart): at least one step, then stop when
+// ||s_k|| <= atol * ||s_0|| or after itmax steps; a step with p'Np <= 0 ends
+// the solve before it is taken.
 int cgls(const Op& A, const double* b, double lam, int itmax, double atol, double /*btol*/,
          double* x) {
     const int64_t m = A.m, n = A.n;
-    std::vector<double> r(b, b + m), s(n), p(n), q(m);
+    std::vector<double> s(n), p(n), q(m), z(n);
     std::fill(x, x + n, 0.0);
-    A.JT(r.data(), s.data());
+    A.JT(b, s.data());
     p = s;
     double gamma = dot(s.data(), s.data(), n);
     const double stop = atol * std::sqrt(gamma);
@@ -546,13 +653,15 @@ int cgls(const Op& A, const double* b, double lam, int itmax, double atol, doubl
     while (iter < itmax) {
         ++iter;
         A.J(p.data(), q.data());
+        A.JT(q.data(), z.data());
+        for (int64_t i = 0; i < n; ++i) z[i] += lam2 * p[i];
         double delta = dot(q.data(), q.data(), m) + lam2 * dot(p.data(), p.data(), n);
         if (!(delta > 0)) break;
         double a = gamma / delta;
-        for (int64_t i = 0; i < n; ++i) x[i] += a * p[i];
-        for (int64_t i = 0; i < m; ++i) r[i] -= a * q[i];
-        A.JT(r.data(), s.data());
-        for (int64_t i = 0; i < n; ++i) s[i] -= lam2 * x[i];
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] += a * p[i];
+            s[i] -= a * z[i];
+        }
         double gnew = dot(s.data(), s.data(), n);
         if (std::sqrt(gnew) <= stop) break;
         double bet = gnew / gamma;
@@ -1012,6 +1121,7 @@ int or_inner_solve(int32_t solver, const double* V, const int32_t* pixels, int32
     op.J = [&](const double* v, double* out) { apply_J_impl(out, npix, V, pixels, p, S2, n_active, v, live); };
     op.JT = [&](const double* u, double* out) { apply_JT_impl(out, V, pixels, p, S2, n_active, u, live); };
     if (solver == CP_SOLVER_CGLS) return cgls(op, b, lam, itmax, atol, btol, x_out);
+    if (solver == CP_SOLVER_LSMR) return lsmr(op, b, lam, itmax, atol, btol, x_out);
     return lsqr(op, b, lam, itmax, atol, btol, x_out);
 }
 
@@ -1224,7 +1334,8 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
     double model_rad_max = o->model_rad_max > 0 ? o->model_rad_max : 10 * fit_rad;
     if (!(model_rad_max >= fit_rad)) return fail("model_rad_max must be >= fit_rad");
     if (!(o->model_rad_nsigma > 0)) return fail("model_rad_nsigma must be positive");
-    if (o->solver != CP_SOLVER_LSQR && o->solver != CP_SOLVER_CGLS) return fail("unknown solver");
+    if (o->solver != CP_SOLVER_LSQR && o->solver != CP_SOLVER_CGLS && o->solver != CP_SOLVER_LSMR)
+        return fail("unknown solver");
     if (!(fit_rad > 0)) return fail("fit_rad must be positive");
     Psf P;
     if (make_psf(psfd, P)) return CP_EARG;
@@ -1485,9 +1596,8 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
         bool accepted = false, f_conv = false;
         for (int trial = 1; trial <= o->max_trials; ++trial) {
             double lam_solve = std::sqrt(lam);
-            int its = (o->solver == CP_SOLVER_CGLS)
-                          ? cgls(op, mrhs.data(), lam_solve, o->inner_iterations, o->linear_tol, o->linear_tol, dsc.data())
-                          : lsqr(op, mrhs.data(), lam_solve, o->inner_iterations, o->linear_tol, o->linear_tol, dsc.data());
+            const auto solve = o->solver == CP_SOLVER_CGLS ? cgls : (o->solver == CP_SOLVER_LSMR ? lsmr : lsqr);
+            int its = solve(op, mrhs.data(), lam_solve, o->inner_iterations, o->linear_tol, o->linear_tol, dsc.data());
             out->n_inner_total += its;
             out->n_trials_total += 1;
             for (int64_t i = 0; i < n; ++i) dlt[i] = dsc[i] / colnorm[i];

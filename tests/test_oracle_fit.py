@@ -218,7 +218,7 @@ def _solve(solver, V, pix, p, S2, na, npix, b, lam, itmax, tol, live=None):
     return x, its
 
 
-@pytest.mark.parametrize("solver", [abi.CP_SOLVER_LSQR, abi.CP_SOLVER_CGLS])
+@pytest.mark.parametrize("solver", [abi.CP_SOLVER_LSQR, abi.CP_SOLVER_CGLS, abi.CP_SOLVER_LSMR])
 def test_inner_solve_matches_dense_damped_normal_equations(solver):
     c = G["operator_fixtures"]["lsqr_dense"]
     p, S2, na, npix = c["p"], c["S2"], c["n_active"], c["npix"]
@@ -263,6 +263,88 @@ def test_truncated_iterate_is_krylov_subspace_minimiser(solver):
         x, its = _solve(solver, V, pix, p, S2, na, npix, b, lam, k, 0.0)
         assert its == k
         np.testing.assert_allclose(x, want, rtol=1e-8, atol=1e-10)
+
+
+def test_lsmr_iterate_minimises_normal_equation_residual():
+    """LSMR's x_k minimises ||J'(b - J x) - lam^2 x|| over K_k(J'J + lam^2 I, J'b)
+    (Fong & Saunders 2011, s2): pins every truncated LSMR iteration exactly,
+    and it differs from the LSQR iterate (which minimises the residual)."""
+    c = G["operator_fixtures"]["lsqr_dense"]
+    p, S2, na, npix = c["p"], c["S2"], c["n_active"], c["npix"]
+    rng = np.random.default_rng(11)
+    V = _equilibrate(rng.standard_normal((na, S2, p)))
+    pix = _pixel_map(2, p, S2, na, npix)
+    J = _dense(V, pix, p, S2, na, npix)
+    b = rng.standard_normal(npix)
+    lam = 0.1
+    H = J.T @ J + lam * lam * np.eye(na * p)
+    g = J.T @ b
+    for k in range(1, 7):
+        K = np.zeros((na * p, k))
+        v = g.copy()
+        for j in range(k):
+            K[:, j] = v
+            v = H @ v
+        Q, _ = np.linalg.qr(K)
+        y = np.linalg.lstsq(H @ Q, g, rcond=None)[0]
+        want = Q @ y
+        x, its = _solve(abi.CP_SOLVER_LSMR, V, pix, p, S2, na, npix, b, lam, k, 0.0)
+        assert its == k
+        np.testing.assert_allclose(x, want, rtol=1e-8, atol=1e-10)
+        xq, _ = _solve(abi.CP_SOLVER_LSQR, V, pix, p, S2, na, npix, b, lam, k, 0.0)
+        if k > 1:
+            assert np.max(np.abs(xq - x)) > 1e-6
+
+
+@pytest.mark.parametrize("solver", ["lsqr", "lsmr"])
+def test_inner_solvers_match_scipy_iterate_by_iterate(solver):
+    """The oracle's LSQR / LSMR against scipy.sparse.linalg's implementations of
+    the same published methods (damp = lam, x0 = 0) at every truncated
+    iteration count, on the stamp operator written out as a dense matrix."""
+    sla = pytest.importorskip("scipy.sparse.linalg")
+    c = G["operator_fixtures"]["lsqr_dense"]
+    p, S2, na, npix = c["p"], c["S2"], c["n_active"], c["npix"]
+    rng = np.random.default_rng(21)
+    V = _equilibrate(rng.standard_normal((na, S2, p)))
+    pix = _pixel_map(3, p, S2, na, npix)
+    J = _dense(V, pix, p, S2, na, npix)
+    b = rng.standard_normal(npix)
+    lam = 0.05
+    code = abi.CP_SOLVER_LSQR if solver == "lsqr" else abi.CP_SOLVER_LSMR
+    fn = sla.lsqr if solver == "lsqr" else sla.lsmr
+    for k in range(1, 5):   # below the 5-deep forward-error window scipy does not have
+        want = fn(J, b, damp=lam, atol=0.0, btol=0.0, conlim=0.0, iter_lim=k)[0] if solver == "lsqr" else \
+            fn(J, b, damp=lam, atol=0.0, btol=0.0, conlim=0.0, maxiter=k)[0]
+        x, its = _solve(code, V, pix, p, S2, na, npix, b, lam, k, 0.0)
+        assert its == k
+        np.testing.assert_allclose(x, want, rtol=1e-10, atol=1e-12)
+
+
+def test_cg_iterates_equal_lsqr_in_exact_arithmetic_and_stop_rule():
+    """CG on the damped normal equations and LSQR span the same Krylov space and
+    minimise the same quadratic, so their k-th iterates agree to rounding; CG
+    stops at the first k with ||s_k|| <= atol ||s_0||, which the dense normal
+    equations give independently."""
+    c = G["operator_fixtures"]["lsqr_dense"]
+    p, S2, na, npix = c["p"], c["S2"], c["n_active"], c["npix"]
+    rng = np.random.default_rng(31)
+    V = _equilibrate(rng.standard_normal((na, S2, p)))
+    pix = _pixel_map(2, p, S2, na, npix)
+    J = _dense(V, pix, p, S2, na, npix)
+    b = rng.standard_normal(npix)
+    lam = 0.2
+    for k in range(1, 8):
+        xc, _ = _solve(abi.CP_SOLVER_CGLS, V, pix, p, S2, na, npix, b, lam, k, 0.0)
+        xq, _ = _solve(abi.CP_SOLVER_LSQR, V, pix, p, S2, na, npix, b, lam, k, 0.0)
+        np.testing.assert_allclose(xc, xq, rtol=1e-8, atol=1e-10)
+    H = J.T @ J + lam * lam * np.eye(na * p)
+    g = J.T @ b
+    atol = 1e-3
+    _, its = _solve(abi.CP_SOLVER_CGLS, V, pix, p, S2, na, npix, b, lam, 200, atol)
+    for k in range(1, its + 1):
+        xk, _ = _solve(abi.CP_SOLVER_CGLS, V, pix, p, S2, na, npix, b, lam, k, 0.0)
+        r = np.linalg.norm(g - H @ xk)
+        assert (r <= atol * np.linalg.norm(g) * (1 + 1e-9)) == (k == its), (k, its, r)
 
 
 # ------------------------------------------------------------------- whole fit
