@@ -248,12 +248,15 @@ class RequestStat(C.Structure):
         ("slot", C.c_int32),
         ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
         ("hbm_bytes", C.c_uint64),
+        ("hbm_charged", C.c_uint64),
         ("hbm_peak", C.c_uint64),
+        ("hbm_resident_peak", C.c_uint64),
     ]
 
 
 class MemStats(C.Structure):
-    _fields_ = [("budget", C.c_uint64), ("in_use", C.c_uint64), ("cached", C.c_uint64), ("peak", C.c_uint64)]
+    _fields_ = [("budget", C.c_uint64), ("in_use", C.c_uint64), ("cached", C.c_uint64), ("peak", C.c_uint64),
+                ("peak_resident", C.c_uint64)]
 
 
 # Every symbol include/crowdphot_hip.h declares, with (restype, argtypes).

@@ -181,6 +181,7 @@ int cp_mem_stats_get(cp_ctx* c, cp_mem_stats* out) {
         out->in_use = c->in_use;
         out->cached = c->cached;
         out->peak = c->peak;
+        out->peak_resident = c->peak_resident;
     });
 }
 
@@ -196,6 +197,7 @@ int cp_set_budget(cp_ctx* c, uint64_t budget) {
             cache_trim(c, 0);
         }
         c->peak = c->in_use;
+        c->peak_resident = c->in_use + c->cached;
     });
 }
 

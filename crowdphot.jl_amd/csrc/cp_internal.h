@@ -45,6 +45,7 @@ struct cp_ctx {
     uint64_t budget = 0;   // 0 = no cap
     uint64_t in_use = 0;   // bytes currently allocated through DevBuf
     uint64_t peak = 0;
+    uint64_t peak_resident = 0;   // most in_use + cached
     // released device blocks kept for reuse (size -> pointer); every use is
     // ordered on `stream`, so a block can be handed out again immediately
     std::multimap<size_t, void*> cache;
@@ -164,6 +165,7 @@ class DevBuf {
         gbytes_ = p ? g : 0;
         c->in_use += g;
         if (c->in_use > c->peak) c->peak = c->in_use;
+        if (c->in_use + c->cached > c->peak_resident) c->peak_resident = c->in_use + c->cached;
     }
     void release() {
         if (ptr_) {

@@ -39,7 +39,7 @@ struct Job {
     bool waited = false;   // a thread is blocked in cp_server_wait on it
     std::string msg;
     double t_submit = 0, t_start = 0, t_end = 0;
-    uint64_t peak = 0;
+    uint64_t charged = 0, peak = 0, peak_resident = 0;
 };
 
 }  // namespace
@@ -96,19 +96,37 @@ uint64_t held_by_others(const cp_server* s, int slot) {
     return o;
 }
 
-// Whether the request at the head of the queue may start on `slot` (called
-// with the server lock held).  Idle slots release their cached blocks when
-// the request would not fit beside them.
-bool admissible(cp_server* s, int slot, uint64_t need) {
-    if (!s->budget) return true;
-    if (held_by_others(s, slot) + need <= s->budget) return true;
-    for (int j = 0; j < (int)s->ctxs.size(); ++j)
-        if (j != slot && !s->busy[j] && s->charge[j]) {
-            (void)cp_release_cached(s->ctxs[j]);
-            cp_mem_stats m{};
-            s->charge[j] = cp_mem_stats_get(s->ctxs[j], &m) == CP_OK ? m.cached : 0;
-        }
-    return held_by_others(s, slot) + need <= s->budget;
+uint64_t cached_of(cp_ctx* c) {
+    cp_mem_stats m{};
+    return cp_mem_stats_get(c, &m) == CP_OK ? m.cached : 0;
+}
+
+// Whether the request at the head of the queue may start on the idle `slot`
+// (called with the server lock held), and what it is charged: `need` plus the
+// blocks the slot keeps cached.  Cached blocks are released only when the
+// request would not fit beside them: first the other idle slots', then the
+// slot's own.
+bool admissible(cp_server* s, int slot, uint64_t need, uint64_t* charge) {
+    cp_ctx* own = s->ctxs[slot];
+    if (!s->budget) {
+        *charge = need;
+        return true;
+    }
+    auto fits = [&] { return held_by_others(s, slot) + s->charge[slot] + need <= s->budget; };
+    if (!fits()) {
+        for (int j = 0; j < (int)s->ctxs.size(); ++j)
+            if (j != slot && !s->busy[j] && s->charge[j]) {
+                (void)cp_release_cached(s->ctxs[j]);
+                s->charge[j] = cached_of(s->ctxs[j]);
+            }
+    }
+    if (!fits() && s->charge[slot]) {
+        (void)cp_release_cached(own);
+        s->charge[slot] = cached_of(own);
+    }
+    if (!fits()) return false;
+    *charge = s->charge[slot] + need;
+    return true;
 }
 
 }  // namespace
@@ -120,10 +138,11 @@ static void worker_main(cp_server* s, int slot) {
         Job* job;
         {
             std::unique_lock<std::mutex> lk(s->mu);
+            uint64_t charge = 0;
             s->cv_work.wait(lk, [&] {
                 if (s->stop) return true;
                 if (s->queue.empty()) return false;
-                return admissible(s, slot, s->jobs[s->queue.front()].bytes);
+                return admissible(s, slot, s->jobs[s->queue.front()].bytes, &charge);
             });
             if (s->stop) return;
             t = s->queue.front();
@@ -132,11 +151,11 @@ static void worker_main(cp_server* s, int slot) {
             job->state = 1;
             job->slot = slot;
             job->t_start = now_s();
+            job->charged = charge;
             s->busy[slot] = 1;
-            s->charge[slot] = job->bytes;
-            // the slot's blocks (cached ones included) stay within the estimate
-            if (s->budget) (void)cp_set_budget(ctx, job->bytes);
-            else (void)cp_set_budget(ctx, 0);
+            s->charge[slot] = charge;
+            // the slot's blocks (reused cached ones included) stay within its charge
+            (void)cp_set_budget(ctx, s->budget ? charge : 0);
             // the next queued request may fit beside this one
             s->cv_work.notify_one();
         }
@@ -156,6 +175,7 @@ static void worker_main(cp_server* s, int slot) {
             if (rc != CP_OK) job->msg = cp_last_error(ctx);
             job->t_end = now_s();
             job->peak = m.peak;
+            job->peak_resident = m.peak_resident;
             job->state = 2;
             s->busy[slot] = 0;
             s->charge[slot] = s->budget ? m.cached : 0;
@@ -259,7 +279,9 @@ int cp_server_wait(cp_server* s, uint64_t ticket, cp_request_stat* st, char* msg
         st->t_start = j.t_start;
         st->t_end = j.t_end;
         st->hbm_bytes = j.bytes;
+        st->hbm_charged = j.charged;
         st->hbm_peak = j.peak;
+        st->hbm_resident_peak = j.peak_resident;
     }
     if (msg && msg_cap > 0) {
         std::strncpy(msg, j.msg.c_str(), (size_t)msg_cap - 1);

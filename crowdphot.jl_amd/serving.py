@@ -73,7 +73,8 @@ class Server:
                 raise MemoryError(text)
             raise RuntimeError(f"libcrowdphot_hip error {rc}: {text}")
         stat = dict(slot=st.slot, t_submit=st.t_submit, t_start=st.t_start, t_end=st.t_end,
-                    latency=st.t_end - st.t_submit, hbm_bytes=st.hbm_bytes, hbm_peak=st.hbm_peak)
+                    latency=st.t_end - st.t_submit, hbm_bytes=st.hbm_bytes, hbm_charged=st.hbm_charged,
+                    hbm_peak=st.hbm_peak, hbm_resident_peak=st.hbm_resident_peak)
         return call.finish(), stat
 
     def close(self):

@@ -192,12 +192,13 @@ int cp_set_tuning(cp_ctx* ctx, const cp_tuning* t);
 /* Device-memory accounting, no reference counterpart (Julia's GC owns the
  * reference's arrays): every buffer of a call comes from the context's
  * cache of device blocks.  in_use = blocks held by a running call, cached =
- * released blocks kept for reuse, peak = the most in_use since the last
- * cp_set_budget.  cp_set_budget changes the cap (0 = none), frees cached
- * blocks beyond it and restarts peak; cp_release_cached frees every cached
- * block.  The server uses them to keep each slot within its admission. */
+ * released blocks kept for reuse, peak = the most in_use and peak_resident =
+ * the most in_use + cached since the last cp_set_budget.  cp_set_budget
+ * changes the cap on in_use + cached (0 = none), frees cached blocks beyond
+ * it and restarts both peaks; cp_release_cached frees every cached block.
+ * The server uses them to keep each slot within its admission. */
 typedef struct cp_mem_stats {
-    uint64_t budget, in_use, cached, peak;
+    uint64_t budget, in_use, cached, peak, peak_resident;
 } cp_mem_stats;
 int cp_mem_stats_get(cp_ctx* ctx, cp_mem_stats* out);
 int cp_set_budget(cp_ctx* ctx, uint64_t hbm_budget_bytes);
@@ -395,11 +396,12 @@ double cp_pixel_overlap(double dx, double dy, double r);
  * request gets alone (its own buffers and stream; fixed-order reductions).
  * The caller keeps every array of a request (inputs, the psf grids and node
  * coordinates, the result arrays) alive until cp_server_wait returns for it.
- * HBM: while a request runs, its slot context is capped at the request's
- * estimate (cp_request_bytes, an upper bound on what the fit allocates), and
- * an idle slot is charged for the device blocks it keeps cached; admission
- * keeps the sum over slots within hbm_budget, releasing idle slots' caches
- * when the next request needs the room.  cp_server_destroy finishes running
+ * HBM: a running request is charged its estimate (cp_request_bytes, an upper
+ * bound on what the fit allocates) plus the blocks its slot kept cached from
+ * earlier requests (which the fit reuses), and its slot context is capped
+ * there; an idle slot is charged for its cached blocks.  Admission keeps the
+ * sum over slots within hbm_budget, releasing idle slots' caches (then the
+ * admitted slot's own) only when the next request needs the room.  cp_server_destroy finishes running
  * requests, completes queued ones with CP_ESTATE and returns once every
  * thread blocked in cp_server_wait has been woken; at most one thread may
  * wait on a ticket. */
@@ -422,8 +424,11 @@ typedef struct cp_request_stat {
     int32_t rc;                 /* the fit's return code */
     int32_t slot;               /* worker slot that ran it */
     double t_submit, t_start, t_end;   /* seconds, one monotonic host clock */
-    uint64_t hbm_bytes;         /* admission estimate charged against the budget */
-    uint64_t hbm_peak;          /* device bytes the fit actually held at most */
+    uint64_t hbm_bytes;         /* admission estimate (cp_request_bytes) */
+    uint64_t hbm_charged;       /* bytes its slot held against the budget: the estimate plus the
+                                   blocks the slot kept cached from earlier requests */
+    uint64_t hbm_peak;          /* device blocks the fit held at most */
+    uint64_t hbm_resident_peak; /* its slot's blocks (held + cached) at most, <= hbm_charged */
 } cp_request_stat;
 
 uint64_t cp_request_bytes(int32_t ny, int32_t nx, int32_t n_stars, double fit_rad, double model_rad,

@@ -64,8 +64,10 @@ def test_budget_admits_one_request_at_a_time():
     with pkg.serving.Server(0, budget, slots=4) as srv:
         tickets = [srv.submit(t["image"], t["psf"], t["sources"], t["fit_rad"], **_kw(t)) for t in reqs]
         stats = [srv.wait(k)[1] for k in tickets]
-    # at any moment the admitted estimates fit the budget
-    events = sorted([(s["t_start"], e) for s, e in zip(stats, est)] + [(s["t_end"], -e) for s, e in zip(stats, est)],
+    # at any moment the running requests' charges (estimate plus the blocks
+    # their slot kept cached) fit the budget, and no slot exceeded its charge
+    assert all(s["hbm_charged"] >= e and s["hbm_resident_peak"] <= s["hbm_charged"] for s, e in zip(stats, est))
+    events = sorted([(s["t_start"], s["hbm_charged"]) for s in stats] + [(s["t_end"], -s["hbm_charged"]) for s in stats],
                     key=lambda v: (v[0], v[1]))
     used = 0
     for _, d in events:
@@ -92,6 +94,7 @@ def test_admitted_requests_stay_within_their_estimate():
         for k in tickets:
             _, st = srv.wait(k)
             assert 0 < st["hbm_peak"] <= st["hbm_bytes"], (st["hbm_peak"], st["hbm_bytes"])
+            assert st["hbm_resident_peak"] <= st["hbm_charged"], st
 
 
 def test_destroy_completes_queued_requests_and_wakes_waiters():
