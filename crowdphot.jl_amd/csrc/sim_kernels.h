@@ -74,8 +74,14 @@ struct Scal {
     // share of |u|^2 is fscale^2 * sf
     double sf, fscale;
     int n_jv_pix, n_jtu_stars;
-    // CGLS
+    // CG on the damped normal equations: gamma = |s|^2, pp = |p|^2, the step a
+    // and the direction update beta of the current iteration, |s_0|
     double gamma, cg_alpha, cg_beta, pnorm2, qnorm2, s0norm;
+    // LSMR (Fong & Saunders 2011): the QR rotations, the ||r|| estimate, the
+    // cond(A) bounds, ||x_k||^2, and del, the hbar factor of the pending
+    // update (0 for LSQR, whose pending update is LSMR's with del = 0)
+    double alphabar, rho, rhobar_m, cbar, sbar, zeta, zetabar;
+    double betadd, betad, rhodold, tautildeold, thetatilde, dsum, maxrbar, minrbar, del;
     unsigned int counters[8];
 };
 
@@ -817,28 +823,59 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
 // ---------------------------------------------------------------------------
 // inner solver: LSQR (Paige & Saunders 1982) on the stamp operator
 // ---------------------------------------------------------------------------
-static __global__ void k_lsqr_init(Scal* S, double lam, int itmax, double atol, double btol) {
+static __global__ void k_lsqr_init(Scal* S, double lam, int itmax, double atol, double btol, int solver) {
     if (threadIdx.x != 0) return;
     S->lam = lam;
     S->itmax = itmax;
     S->atol = atol;
     S->btol = btol;
+    S->solver = solver;
+    S->sig = S->tau = S->del = 0;
+    S->iter = 0;
+    S->xnorm2 = 0;
+    for (int i = 0; i < 5; ++i) S->errv[i] = 0;
+    S->xenorm2 = 0;
+    if (solver == CP_SOLVER_CGLS) {
+        // s_0 = J' b - lam^2 x_0 = -b_scaled, p_0 = s_0
+        S->gamma = S->bnorm2;
+        S->pnorm2 = S->bnorm2;
+        S->s0norm = sqrt(S->bnorm2);
+        S->cg_beta = 0;
+        S->cg_alpha = 0;
+        S->done = !(S->bnorm2 > 0);
+        return;
+    }
     double beta1 = sqrt(S->cost);
     double alpha = beta1 > 0 ? sqrt(S->bnorm2) / beta1 : 0.0;
     S->beta1 = beta1;
     S->alpha = alpha;
     S->beta_cur = beta1;
     S->beta_next = 0;
+    S->anorm2 = alpha * alpha;
+    S->done = (beta1 == 0 || alpha == 0);
+    if (solver == CP_SOLVER_LSMR) {
+        S->zetabar = alpha * beta1;
+        S->alphabar = alpha;
+        S->rho = 1;
+        S->rhobar_m = 1;
+        S->cbar = 1;
+        S->sbar = 0;
+        S->betadd = beta1;
+        S->betad = 0;
+        S->rhodold = 1;
+        S->tautildeold = 0;
+        S->thetatilde = 0;
+        S->zeta = 0;
+        S->dsum = 0;
+        S->maxrbar = 0;
+        S->minrbar = 1e100;
+        return;
+    }
     S->rhobar = alpha;
     S->phibar = beta1;
-    S->anorm2 = alpha * alpha;
-    S->dnorm2 = S->xnorm2 = S->z = S->res2 = S->xenorm2 = 0;
-    for (int i = 0; i < 5; ++i) S->errv[i] = 0;
+    S->dnorm2 = S->z = S->res2 = 0;
     S->c2 = -1;
     S->s2 = 0;
-    S->sig = S->tau = 0;
-    S->iter = 0;
-    S->done = (beta1 == 0 || alpha == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -922,15 +959,24 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(const int* __restrict__ pix, const
     if (Sc->done) return;
     __shared__ double sred[NWAVE];
     const int U = Sc->n_jv_pix;
-    const double alpha = Sc->alpha;
-    const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
-    double tfac, ufac;
-    if (first) {
-        tfac = -ia / Sc->beta1;
-        ufac = -1.0 / Sc->beta1;
+    // new u = tfac J t + ucoef u_old.  LSQR / LSMR: u <- J v - alpha u with
+    // v = t / alpha and u_old = ucsr / beta (the first call: t = -b_scaled,
+    // u_1 = -wres / beta1).  CG: q <- J s + beta q_old (p = s + beta p_old,
+    // so J p = J s + beta J p_old; the first call: q_0 = J s_0 = -J b_scaled).
+    double tfac, ucoef;
+    if (Sc->solver == CP_SOLVER_CGLS) {
+        tfac = first ? -1.0 : 1.0;
+        ucoef = first ? 0.0 : Sc->cg_beta;
     } else {
-        tfac = ia;
-        ufac = Sc->beta_cur != 0 ? 1.0 / Sc->beta_cur : 1.0;
+        const double alpha = Sc->alpha;
+        const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
+        if (first) {
+            tfac = -ia / Sc->beta1;
+            ucoef = alpha / Sc->beta1;
+        } else {
+            tfac = ia;
+            ucoef = -alpha * (Sc->beta_cur != 0 ? 1.0 / Sc->beta_cur : 1.0);
+        }
     }
     const double inv_s2 = 1.0 / S2, half_s2 = 0.5 / S2;
     const vec4_t<T>* V4 = reinterpret_cast<const vec4_t<T>*>(V);
@@ -947,7 +993,7 @@ __global__ __launch_bounds__(BLOCK) void k_Jv(const int* __restrict__ pix, const
             y += (double)v.x * t.x + (double)v.y * t.y + (double)v.z * t.z;
         }
         const double us = first ? (double)wres[pix[u]] : (double)ucsr[u];
-        const T uu = (T)(y * tfac - alpha * (us * ufac));
+        const T uu = (T)(y * tfac + ucoef * us);
         ucsr[u] = uu;
         for (int e = e0; e < e1; ++e) ustamp[ent[e]] = uu;
         acc2 += (double)uu * (double)uu;
@@ -996,29 +1042,42 @@ static __global__ void k_pixel_compact(int U, const int* __restrict__ pix, const
     }
 }
 
-// v <- J^T u - beta v per star, with the pending x/w update of the previous
-// iteration folded in.  GL lanes per star (GL = 16 for stamps up to 16 x 16,
-// else 64), grid-stride over the star list lst (Sc->n_jtu_stars entries: the
-// stars live when the J v structure was last compacted, the others are
-// neither read by J v nor moved by the step).
-template <typename T, int EPL, int GL>
+// Per star, GL lanes (GL = 16 for stamps up to 16 x 16, else 64),
+// grid-stride over the star list lst (Sc->n_jtu_stars entries: the stars live
+// when the J v structure was last compacted, the others are neither read by
+// J v nor moved by the step).  g = J^T u over the star's stamp rows, then by
+// solver:
+//  * LSQR / LSMR: t <- g / beta - beta v, with the pending update of the
+//    previous iteration folded in (hbar <- w - del hbar, x <- x + sig hbar,
+//    w <- v - tau w; LSQR is del = 0 with w its search vector, LSMR keeps
+//    hbar and calls w h).  Partials: |t|^2 and |w|^2, and for LSMR the
+//    products |x|^2, x.w, x.hbar, w.hbar, |hbar|^2 that give |x_k| after
+//    this iteration's scalars are known.
+//  * CG: p <- s + beta p, z = g + lam^2 p, x <- x + a p, s <- s - a z
+//    (s in the padded t buffer, p in w).  Partials: |s|^2 and s.p_old-free
+//    s.p (the next |p|^2 follows from them).
+template <typename T, int EPL, int GL, int SOLVER>
 __global__ __launch_bounds__(BLOCK) void k_JTu(
     const int* __restrict__ lst, const int* __restrict__ ay, const int* __restrict__ ax, const uint8_t* __restrict__ live,
     int R, int S, int p, int ny, int nx, const T* __restrict__ V, const T* __restrict__ ustamp,
     const double* __restrict__ b_scaled, double* __restrict__ tbuf, double* __restrict__ x,
-    double* __restrict__ wv, int first, int it, double* __restrict__ partials, double* __restrict__ partials2,
-    Scal* Sc) {
+    double* __restrict__ wv, double* __restrict__ hb, int first, int it, double* __restrict__ partials,
+    double* __restrict__ partials2, double* __restrict__ partials3, Scal* Sc) {
     if (Sc->done) return;
     __shared__ double sred[NWAVE];
+    constexpr bool CG = SOLVER == CP_SOLVER_CGLS, LSMR = SOLVER == CP_SOLVER_LSMR;
     const double alpha = Sc->alpha, beta = Sc->beta_next;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
     const double ib = beta != 0 ? 1.0 / beta : 1.0;
-    const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau;
-    const double ib1 = first ? 1.0 / Sc->beta1 : 0.0;
+    const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau, del = first ? 0.0 : Sc->del;
+    const double ib1 = first || CG ? (CG ? 0.0 : 1.0 / Sc->beta1) : 0.0;
+    const double cg_a = CG ? Sc->cg_alpha : 0.0, cg_b = CG && !first ? Sc->cg_beta : 0.0;
+    const double lam2 = CG ? Sc->lam * Sc->lam : 0.0;
     constexpr int GPB = BLOCK / GL;
     const int sub = threadIdx.x % GL;
     const int S2 = S * S;
     double tsq = 0, wsq = 0;
+    double xx = 0, xw = 0, xh = 0, wh = 0, hh = 0;
     const int nl = Sc->n_jtu_stars;
     for (int li = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL; li < nl; li += gridDim.x * GPB) {
         const int a = lst[li];
@@ -1027,6 +1086,7 @@ __global__ __launch_bounds__(BLOCK) void k_JTu(
             if (first && sub < p) {
                 x[base + sub] = 0;
                 wv[base + sub] = 0;
+                if (LSMR) hb[base + sub] = 0;
                 tbuf[(size_t)a * VP + sub] = 0;
             }
         } else {
@@ -1057,22 +1117,53 @@ __global__ __launch_bounds__(BLOCK) void k_JTu(
                 const int k = sub;
                 const double acck = k == 0 ? acc0 : (k == 1 ? acc1 : acc2);
                 const size_t tk = (size_t)a * VP + k;
-                const double told = first ? -b_scaled[tk] * ib1 : tbuf[tk];
-                const double v = told * ia;
-                const double xo = first ? 0.0 : x[base + k];
-                const double wo = first ? 0.0 : wv[base + k];
-                x[base + k] = xo + sig * wo;
-                const double wn = v - tau * wo;
-                wv[base + k] = wn;
-                wsq += wn * wn;
-                const double tn = beta != 0 ? acck * ib - beta * v : told;
-                tbuf[tk] = tn;
-                tsq += tn * tn;
+                if (CG) {
+                    const double sk = first ? -b_scaled[tk] : tbuf[tk];
+                    const double pk = first ? sk : sk + cg_b * wv[base + k];
+                    const double zk = acck + lam2 * pk;
+                    x[base + k] = (first ? 0.0 : x[base + k]) + cg_a * pk;
+                    wv[base + k] = pk;
+                    const double sn = sk - cg_a * zk;
+                    tbuf[tk] = sn;
+                    tsq += sn * sn;
+                    wsq += sn * pk;
+                } else {
+                    const double told = first ? -b_scaled[tk] * ib1 : tbuf[tk];
+                    const double v = told * ia;
+                    const double xo = first ? 0.0 : x[base + k];
+                    const double wo = first ? 0.0 : wv[base + k];
+                    const double ho = (LSMR && !first) ? hb[base + k] : 0.0;
+                    const double hn = wo - del * ho;
+                    const double xn = xo + sig * hn;
+                    x[base + k] = xn;
+                    const double wn = v - tau * wo;
+                    wv[base + k] = wn;
+                    if (LSMR) {
+                        hb[base + k] = hn;
+                        xx += xn * xn;
+                        xw += xn * wn;
+                        xh += xn * hn;
+                        wh += wn * hn;
+                        hh += hn * hn;
+                    }
+                    wsq += wn * wn;
+                    const double tn = beta != 0 ? acck * ib - beta * v : told;
+                    tbuf[tk] = tn;
+                    tsq += tn * tn;
+                }
             }
         }
     }
     block_store(tsq, partials, sred);
     block_store(wsq, partials2, sred);
+    if (LSMR) {
+        const int nb = gridDim.x;
+        block_store(xx, partials3, sred);
+        block_store(xw, partials3 + nb, sred);
+        block_store(xh, partials3 + 2 * nb, sred);
+        block_store(wh, partials3 + 3 * nb, sred);
+        block_store(hh, partials3 + 4 * nb, sred);
+    }
 }
 
 // LSQR scalar recurrences of iteration `it` (one thread), from tt = |J^T u -
@@ -1137,24 +1228,108 @@ __device__ void lsqr_advance(Scal* Sc, double tt, double ww, int it) {
     Sc->done = solved || ill || it >= Sc->itmax;
 }
 
+// LSMR scalar recurrences of iteration `it` (one thread), Fong & Saunders
+// 2011 with the Krylov.jl stopping tests, from tt = |t|^2 (t = alpha_new
+// v_new) and the products q[0..5] = |x|^2, x.h, x.hbar, |h|^2, h.hbar,
+// |hbar|^2 of the vectors after the previous iteration's update, which give
+// |x_it| once this iteration's del and sigma are known.
+__device__ void lsmr_advance(Scal* Sc, double tt, const double* q, int it) {
+    const double beta = Sc->beta_next, lam = Sc->lam;
+    double alpha = Sc->alpha;
+    if (beta != 0) alpha = sqrt(tt);
+    double chat, shat, alphahat;
+    sym_givens(Sc->alphabar, lam, chat, shat, alphahat);
+    const double rhoold = Sc->rho;
+    double c, s, rho;
+    sym_givens(alphahat, beta, c, s, rho);
+    const double thetanew = s * alpha;
+    Sc->alphabar = c * alpha;
+    const double rhobarold = Sc->rhobar_m, zetaold = Sc->zeta;
+    const double thetabar = Sc->sbar * rho, rhotemp = Sc->cbar * rho;
+    double cbar, sbar, rhobar;
+    sym_givens(rhotemp, thetanew, cbar, sbar, rhobar);
+    Sc->cbar = cbar;
+    Sc->sbar = sbar;
+    Sc->rhobar_m = rhobar;
+    Sc->rho = rho;
+    const double zeta = cbar * Sc->zetabar;
+    Sc->zeta = zeta;
+    Sc->zetabar = -sbar * Sc->zetabar;
+    Sc->xenorm2 += zeta * zeta;
+    Sc->errv[it % 5] = zeta;
+    double err_lbnd = 0;
+    if (it >= 5) {
+        double e2 = 0;
+        for (int i = 0; i < 5; ++i) e2 += Sc->errv[i] * Sc->errv[i];
+        err_lbnd = sqrt(e2);
+    }
+    // pending update: hbar <- h - del hbar, x <- x + sig hbar, h <- v - tau h
+    const double del = thetabar * rho / (rhoold * rhobarold);
+    const double sig = zeta / (rho * rhobar);
+    Sc->del = del;
+    Sc->sig = sig;
+    Sc->tau = thetanew / rho;
+    const double hb2 = q[3] - 2 * del * q[4] + del * del * q[5];
+    const double xhb = q[1] - del * q[2];
+    const double xnorm = sqrt(fmax(0.0, q[0] + 2 * sig * xhb + sig * sig * hb2));
+    // ||r|| estimate
+    const double betaacute = chat * Sc->betadd, betacheck = -shat * Sc->betadd;
+    const double betahat = c * betaacute;
+    Sc->betadd = -s * betaacute;
+    const double thetatildeold = Sc->thetatilde;
+    double ctildeold, stildeold, rhotildeold;
+    sym_givens(Sc->rhodold, thetabar, ctildeold, stildeold, rhotildeold);
+    Sc->thetatilde = stildeold * rhobar;
+    Sc->rhodold = ctildeold * rhobar;
+    Sc->betad = -stildeold * Sc->betad + ctildeold * betahat;
+    Sc->tautildeold = (zetaold - thetatildeold * Sc->tautildeold) / rhotildeold;
+    const double taud = (zeta - Sc->thetatilde * Sc->tautildeold) / Sc->rhodold;
+    Sc->dsum += betacheck * betacheck;
+    const double bd = Sc->betad - taud;
+    const double rnorm = sqrt(Sc->dsum + bd * bd + Sc->betadd * Sc->betadd);
+    // ||A|| and cond(A)
+    Sc->anorm2 += beta * beta;
+    const double anorm = sqrt(Sc->anorm2);
+    Sc->anorm2 += alpha * alpha;
+    Sc->maxrbar = fmax(Sc->maxrbar, rhobarold);
+    if (it > 1) Sc->minrbar = fmin(Sc->minrbar, rhobarold);
+    const double acond = fmax(Sc->maxrbar, rhotemp) / fmin(Sc->minrbar, rhotemp);
+    const double arnorm = fabs(Sc->zetabar);
+    const double beta1 = Sc->beta1;
+    const double test1 = rnorm / beta1, test2 = arnorm / (anorm * rnorm), test3 = 1 / acond;
+    const double t1 = test1 / (1 + anorm * xnorm / beta1);
+    const double axtol = 1.4901161193847656e-08, etol = axtol, ctol = axtol;
+    const double rtol = Sc->btol + axtol * anorm * xnorm / beta1;
+    const bool ill = (1 + test3 <= 1) || (test3 <= ctol);
+    const bool solved = (1 + test2 <= 1) || (test2 <= axtol) || (arnorm <= Sc->atol) || (1 + t1 <= 1) ||
+                        (test1 <= rtol) || (it >= 5 && err_lbnd <= etol * sqrt(Sc->xenorm2));
+    Sc->alpha = alpha;
+    Sc->beta_cur = beta;
+    Sc->iter = it;
+    Sc->done = solved || ill || it >= Sc->itmax;
+}
+
 // ---------------------------------------------------------------------------
 // k_finish: one block sums the per-block partials of the preceding launch in
 // block order and updates the device scalars
 // ---------------------------------------------------------------------------
-enum : int { FIN_COST = 0, FIN_COST_CAND = 1, FIN_FILL = 2, FIN_BETA = 3, FIN_LSQR = 4, FIN_ACCEPT = 5, FIN_COUNT = 6 };
+enum : int {
+    FIN_COST = 0, FIN_COST_CAND = 1, FIN_FILL = 2, FIN_BETA = 3, FIN_LSQR = 4, FIN_ACCEPT = 5, FIN_COUNT = 6,
+    FIN_CGQ = 7, FIN_CGS = 8
+};
 constexpr int FIN_BLOCK = 1024;
 
 template <int OP>
 static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const double* __restrict__ part,
                                                              const double* __restrict__ part2, int nb, int it) {
-    if ((OP == FIN_BETA || OP == FIN_LSQR) && Sc->done) return;
+    if ((OP == FIN_BETA || OP == FIN_LSQR || OP == FIN_CGQ || OP == FIN_CGS) && Sc->done) return;
     __shared__ double sh[2][FIN_BLOCK / 64];
     double a = 0, b = 0;
     for (int i = threadIdx.x; i < nb; i += FIN_BLOCK) {
         a += part[i];
         if (OP == FIN_FILL) b = fmax(b, part[nb + i]);
         if (OP == FIN_ACCEPT) b += part[nb + i];
-        if (OP == FIN_LSQR || OP == FIN_BETA) b += part2[i];
+        if (OP == FIN_LSQR || OP == FIN_BETA || OP == FIN_CGS) b += part2[i];
     }
     a = wave_sum(a);
     b = OP == FIN_FILL ? wave_max(b) : wave_sum(b);
@@ -1192,11 +1367,59 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const dou
         Sc->beta_next = sqrt(ta + Sc->fscale * Sc->fscale * Sc->sf);
     }
     if (OP == FIN_LSQR) lsqr_advance(Sc, ta, tb, it);
+    if (OP == FIN_CGQ) {
+        // ta = |q|^2, q = J p: the step a = gamma / (|q|^2 + lam^2 |p|^2)
+        const double delta = ta + Sc->lam * Sc->lam * Sc->pnorm2;
+        if (!(delta > 0)) {
+            Sc->done = 1;
+        } else {
+            Sc->cg_alpha = Sc->gamma / delta;
+        }
+    }
+    if (OP == FIN_CGS) {
+        // ta = |s_new|^2, tb = s_new . p: the next direction p + = s_new + beta p
+        const double beta = ta / Sc->gamma;
+        Sc->pnorm2 = ta + 2 * beta * tb + beta * beta * Sc->pnorm2;
+        Sc->cg_beta = beta;
+        Sc->gamma = ta;
+        Sc->iter = it;
+        Sc->done = sqrt(ta) <= Sc->atol * Sc->s0norm || it >= Sc->itmax;
+    }
     if (OP == FIN_ACCEPT) {
         Sc->n_live = (int)ta;
         Sc->n_frozen_now = (int)tb;
     }
     if (OP == FIN_COUNT) Sc->n_union = (unsigned long long)ta;
+}
+
+// One-block sum of LSMR's seven per-block partials of k_JTu (|t|^2, |h|^2 and
+// the five products in part3) and its scalar recurrences.
+static __global__ __launch_bounds__(FIN_BLOCK) void k_finish_lsmr(Scal* Sc, const double* __restrict__ part,
+                                                                  const double* __restrict__ part2,
+                                                                  const double* __restrict__ part3, int nb, int it) {
+    if (Sc->done) return;
+    __shared__ double sh[7][FIN_BLOCK / 64];
+    double a[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nb; i += FIN_BLOCK) {
+        a[0] += part[i];
+        a[1] += part3[i];
+        a[2] += part3[nb + i];
+        a[3] += part3[2 * nb + i];
+        a[4] += part2[i];
+        a[5] += part3[3 * nb + i];
+        a[6] += part3[4 * nb + i];
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) a[k] = wave_sum(a[k]);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 7; ++k) sh[k][threadIdx.x >> 6] = a[k];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double t[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 7; ++k)
+        for (int i = 0; i < FIN_BLOCK / 64; ++i) t[k] += sh[k][i];
+    // t[1..6] = |x|^2, x.h, x.hbar, |h|^2, h.hbar, |hbar|^2
+    lsmr_advance(Sc, t[0], t + 1, it);
 }
 
 // The accepted candidate's cost becomes the current cost (its weighted
@@ -1206,18 +1429,21 @@ static __global__ void k_take_candidate(Scal* Sc) {
 }
 
 // Final x update, step capping and candidate theta (:1224-1230, _cap_position_step! :700-716).
+// The last iteration's pending update is x + sig (w - del hbar) (LSQR: del = 0;
+// CG: sig = 0, x is already final).
 static __global__ void k_step(int na, int p, int k_y, int k_x, double max_step, const double* __restrict__ wv,
-                       double* __restrict__ x, const double* __restrict__ colnorm,
+                       const double* __restrict__ hb, double* __restrict__ x, const double* __restrict__ colnorm,
                        const double* __restrict__ theta, double* __restrict__ dsc, double* __restrict__ theta_c,
                        const uint8_t* __restrict__ live, const Scal* Sc) {
     int a = blockIdx.x * BLOCK + threadIdx.x;
     if (a >= na) return;
     const size_t base = (size_t)a * p;
     const bool any_iter = Sc->iter > 0;
-    const double sig = Sc->sig;
+    const double sig = Sc->sig, del = Sc->del;
     double d[3];
     for (int k = 0; k < p; ++k) {
-        double xv = (any_iter && live[a]) ? x[base + k] + sig * wv[base + k] : 0.0;
+        const double h = hb ? wv[base + k] - del * hb[base + k] : wv[base + k];
+        double xv = (any_iter && live[a]) ? x[base + k] + sig * h : 0.0;
         x[base + k] = xv;
         d[k] = xv / colnorm[base + k];
     }

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -299,12 +300,13 @@ struct Sim {
 #undef CP_FILL
     }
 
+    template <int SOLVER>
     void jtu_launch(int S2, dim3 grid, const int* lst, const int* ay, const int* ax, const uint8_t* live, int R, int S, int p,
                     int ny, int nx, const T* V, const T* u, const double* b, double* t, double* x, double* wv,
-                    int first, int it, double* part, double* part2, Scal* Sc) {
+                    double* hb, int first, int it, double* part, double* part2, double* part3, Scal* Sc) {
 #define CP_JTU(E, G)                                                                                          \
-    launch_n("k_JTu", k_JTu<T, E, G>, grid, dim3(BLOCK), 0, st, lst, ay, ax, live, R, S, p, ny, nx, V, u, b, t, x, wv, \
-             first, it, part, part2, Sc)
+    launch_n("k_JTu", k_JTu<T, E, G, SOLVER>, grid, dim3(BLOCK), 0, st, lst, ay, ax, live, R, S, p, ny, nx, V, u, b, t, \
+             x, wv, hb, first, it, part, part2, part3, Sc)
         if (S2 <= 16) CP_JTU(1, 16);
         else if (S2 <= 32) CP_JTU(2, 16);
         else if (S2 <= 64) CP_JTU(4, 16);
@@ -527,8 +529,9 @@ struct Sim {
         CP_HIP(hipMemsetAsync(V.get(), 0, V.bytes(), st));
         const size_t nb8 = (size_t)na * p * 8;
         const size_t nb32 = (size_t)na * VP * 8;   // padded float64 LSQR vectors read by J v
+        const int solver = o.solver;
         DevBuf colnorm(ctx, nb8), D(ctx, nb8), bsc(ctx, nb32), xs(ctx, nb8), wl(ctx, nb8), tb(ctx, nb32), dsc(ctx, nb8),
-            theta_c(ctx, nb8);
+            theta_c(ctx, nb8), hbar(ctx, solver == CP_SOLVER_LSMR ? nb8 : 0);
         CP_HIP(hipMemsetAsync(colnorm.get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(xs.get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(wl.get(), 0, nb8, st));
@@ -584,7 +587,8 @@ struct Sim {
         // per-block partials of the widest launch (two sets for fill / accept)
         const int maxpb = (int)std::max<int64_t>(
             {(int64_t)gb_render, (int64_t)gb_fill, (int64_t)gb_jv, (int64_t)gb_jtu, (int64_t)gstar});
-        DevBuf part(ctx, (size_t)2 * maxpb * 8), part2(ctx, (size_t)maxpb * 8);
+        DevBuf part(ctx, (size_t)2 * maxpb * 8), part2(ctx, (size_t)maxpb * 8),
+            part3(ctx, solver == CP_SOLVER_LSMR ? (size_t)5 * maxpb * 8 : 0);
 
         auto star_rec = [&](const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter, int all_live) {
             launch(k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, th, fm, M, inv_norm_g64, moffat_amp64,
@@ -739,7 +743,7 @@ struct Sim {
             for (int trial = 1; trial <= o.max_trials; ++trial) {
                 const double lam_solve = std::sqrt(lam);
                 launch(k_lsqr_init, dim3(1), dim3(64), 0, st, Sc, lam_solve, o.inner_iterations, o.linear_tol,
-                       o.linear_tol);
+                       o.linear_tol, solver);
                 // inner solve, capped step and candidate cost: one graph launch per trial
                 auto trial_body = [&]() {
                     for (int it = 1; it <= o.inner_iterations; ++it) {
@@ -749,15 +753,32 @@ struct Sim {
                                  (const double*)(first ? bsc.as<double>() : tb.as<double>()),
                                  (const T*)wres_cur, ucsr.as<T>(), ustamp.as<T>(), first, part.as<double>(),
                                  part2.as<double>(), Sc);
-                        finish(k_finish<FIN_BETA>, gb_jv, it);
-                        jtu_launch(S2, dim3(gb_jtu), (const int*)jlist.as<int>(), (const int*)ay.as<int>(), (const int*)ax.as<int>(),
-                                   (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx, (const T*)V.as<T>(),
-                                   (const T*)ustamp.as<T>(), (const double*)bsc.as<double>(), tb.as<double>(),
-                                   xs.as<double>(), wl.as<double>(), first, it, part.as<double>(), part2.as<double>(), Sc);
-                        finish(k_finish<FIN_LSQR>, gb_jtu, it);
+                        if (solver == CP_SOLVER_CGLS) finish(k_finish<FIN_CGQ>, gb_jv, it);
+                        else finish(k_finish<FIN_BETA>, gb_jv, it);
+                        auto jtu = [&](auto sv) {
+                            jtu_launch<decltype(sv)::value>(
+                                S2, dim3(gb_jtu), (const int*)jlist.as<int>(), (const int*)ay.as<int>(),
+                                (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
+                                (const T*)V.as<T>(), (const T*)ustamp.as<T>(), (const double*)bsc.as<double>(),
+                                tb.as<double>(), xs.as<double>(), wl.as<double>(), hbar.as<double>(), first, it,
+                                part.as<double>(), part2.as<double>(), part3.as<double>(), Sc);
+                        };
+                        if (solver == CP_SOLVER_CGLS) {
+                            jtu(std::integral_constant<int, CP_SOLVER_CGLS>{});
+                            finish(k_finish<FIN_CGS>, gb_jtu, it);
+                        } else if (solver == CP_SOLVER_LSMR) {
+                            jtu(std::integral_constant<int, CP_SOLVER_LSMR>{});
+                            launch(k_finish_lsmr, dim3(1), dim3(FIN_BLOCK), 0, st, Sc, (const double*)part.as<double>(),
+                                   (const double*)part2.as<double>(), (const double*)part3.as<double>(), gb_jtu, it);
+                        } else {
+                            jtu(std::integral_constant<int, CP_SOLVER_LSQR>{});
+                            finish(k_finish<FIN_LSQR>, gb_jtu, it);
+                        }
                     }
                     launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
-                           (const double*)wl.as<double>(), xs.as<double>(), (const double*)colnorm.as<double>(),
+                           (const double*)wl.as<double>(),
+                           (const double*)(solver == CP_SOLVER_LSMR ? hbar.as<double>() : nullptr), xs.as<double>(),
+                           (const double*)colnorm.as<double>(),
                            (const double*)theta.as<double>(), dsc.as<double>(), theta_c.as<double>(),
                            (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
                     star_rec(theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
@@ -944,7 +965,8 @@ void validate_sim(const cp_sim_opts& o, double fit_rad) {
     double mrm = o.model_rad_max > 0 ? o.model_rad_max : 10 * fit_rad;
     CP_ARG(mrm >= fit_rad, "model_rad_max must be >= fit_rad");
     CP_ARG(o.model_rad_nsigma > 0, "model_rad_nsigma must be positive");
-    CP_ARG(o.solver == CP_SOLVER_LSQR, "solver must be LSQR in this build (CGLS runs in the oracle only)");
+    CP_ARG(o.solver == CP_SOLVER_LSQR || o.solver == CP_SOLVER_LSMR || o.solver == CP_SOLVER_CGLS,
+           "solver must be CP_SOLVER_LSQR, CP_SOLVER_LSMR or CP_SOLVER_CGLS");
     CP_ARG(fit_rad > 0 && std::isfinite(fit_rad), "fit_rad must be positive");
     CP_ARG((o.free_mask & 7) != 0, "all model parameters are fixed; nothing to fit");
     CP_ARG(o.precision == CP_PREC_MIXED || o.precision == CP_PREC_FP64, "unknown precision mode");

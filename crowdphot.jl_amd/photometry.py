@@ -147,14 +147,29 @@ class MultiPassPhotResult:
     stats: dict = field(default_factory=dict)
 
 
+def _field(rec, name):
+    return rec[name] if isinstance(rec, dict) else getattr(rec, name)
+
+
 def _catalog(sources):
     """_extract_source_catalog (psf_photometry_single.jl:152-197): a
-    MatchedFilterResult gives the peak pixels and peak fluxes; a
-    NamedTuple-like table y, x (required) and flux (default 1)."""
+    MatchedFilterResult gives the peak pixels and peak fluxes; a sequence of
+    per-star records (the Vector{NamedTuple} of measure_star_shapes) gives
+    centroid.y, centroid.x and matched_filter_flux; a NamedTuple-like table
+    gives y, x (required) and flux (default 1)."""
     if hasattr(sources, "peaks") and hasattr(sources, "peak_fluxes"):
         pk = np.asarray(sources.peaks, dtype=np.float64).reshape(-1, 2)
         return (np.ascontiguousarray(pk[:, 0]), np.ascontiguousarray(pk[:, 1]),
                 np.ascontiguousarray(sources.peak_fluxes, dtype=np.float64))
+    if isinstance(sources, (list, tuple)) and not (isinstance(sources, tuple) and hasattr(sources, "_fields")):
+        try:
+            y = [float(_field(_field(s, "centroid"), "y")) for s in sources]
+            x = [float(_field(_field(s, "centroid"), "x")) for s in sources]
+            f = [float(_field(s, "matched_filter_flux")) for s in sources]
+        except (KeyError, AttributeError, TypeError) as e:
+            raise ArgumentError("a per-star source list needs centroid.y, centroid.x and matched_filter_flux "
+                                f"in every record ({e})") from None
+        return (np.asarray(y, np.float64), np.asarray(x, np.float64), np.asarray(f, np.float64))
     get = (lambda k: sources[k]) if isinstance(sources, dict) else (lambda k: getattr(sources, k))
     has = (lambda k: k in sources) if isinstance(sources, dict) else (lambda k: hasattr(sources, k))
     y = np.ascontiguousarray(get("y"), dtype=np.float64)
@@ -213,10 +228,8 @@ class _SimCall:
             raise ArgumentError("model_rad_max must be >= fit_rad")
         if not model_rad_nsigma > 0:
             raise ArgumentError("model_rad_nsigma must be positive")
-        if solver not in ("lsqr", "lsmr"):
-            raise ArgumentError(f"solver must be :lsqr or :lsmr, got {solver!r}")
-        if solver == "lsmr":
-            raise ArgumentError("solver :lsmr is not built on the GPU; use :lsqr")
+        if solver not in ("lsqr", "lsmr", "cg"):
+            raise ArgumentError(f"solver must be :lsqr or :lsmr (or the build's :cg), got {solver!r}")
         if damping == "levenberg":
             raise ArgumentError("LevenbergDamping is not supported by fit_all_stars_simultaneous: it conflicts "
                                 "with the column-equilibrated x_tol scaling. Use MarquardtDamping.")
@@ -235,7 +248,7 @@ class _SimCall:
         self.y0, self.x0, self.f0 = _catalog(sources)
         n = len(self.y0)
         o = _abi.sim_opts_default()
-        o.solver = _abi.CP_SOLVER_LSQR
+        o.solver = {"lsqr": _abi.CP_SOLVER_LSQR, "lsmr": _abi.CP_SOLVER_LSMR, "cg": _abi.CP_SOLVER_CGLS}[solver]
         o.inner_iterations = inner_iterations
         o.linear_tol = linear_tol
         o.model_rad = 0.0 if model_rad == "auto" else float(model_rad)

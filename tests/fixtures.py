@@ -34,13 +34,24 @@ def separated_positions(rng, n, shape, border, min_sep):
 
 
 def crowded_field(seed, shape=(150, 150), n=200, fwhm=2.0, bkg=50.0, read_noise=2.0,
-                  flux=(100.0, 3000.0), radius=6, border=8):
+                  flux=(100.0, 3000.0), radius=6, border=8, min_sep=0.0):
     """Crowded noisy field in the manner of the reference crowded fixture
     (test/photometry/psf_photometry_simultaneous.jl:255-270): Poisson + Gaussian
-    noise, power-law fluxes (slope 2), background subtracted."""
+    noise, power-law fluxes (slope 2), background subtracted; candidate centres
+    closer than min_sep to an accepted one are redrawn (simulate_sources,
+    src/simulation.jl:240-273)."""
     rng = np.random.default_rng(seed)
-    ys = rng.uniform(border, shape[0] - border, n)
-    xs = rng.uniform(border, shape[1] - border, n)
+    if min_sep > 0:
+        ys, xs = np.zeros(n), np.zeros(n)
+        k = 0
+        while k < n:
+            y, x = rng.uniform(border, shape[0] - border), rng.uniform(border, shape[1] - border)
+            if k == 0 or np.min((ys[:k] - y) ** 2 + (xs[:k] - x) ** 2) >= min_sep * min_sep:
+                ys[k], xs[k] = y, x
+                k += 1
+    else:
+        ys = rng.uniform(border, shape[0] - border, n)
+        xs = rng.uniform(border, shape[1] - border, n)
     lo, hi = flux
     u = rng.random(n)
     q = 1 - 2.0

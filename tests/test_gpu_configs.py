@@ -82,13 +82,15 @@ def test_plummer_core_crop_parity(ctx):
     frac32, sf32, sp32 = tail_report("plummer core mixed", r32, ro, FLUX_RTOL, POS_ATOL)
     # float64 device mode: every star within 1e-6 of the oracle, same validity
     assert np.array_equal(r64.valid, ro.valid.astype(bool)) and frac64 == 1.0
-    # mixed precision: at least 99 % of stars inside the bar; the rest are
-    # stars still live at max_iter on both sides (not converged, so the float32
-    # rounding of pixels and Jacobian moves them along a shallow direction),
-    # typically at 1e-3 of their own sigma
+    # mixed precision: at least 99 % of stars inside the bar; each of the rest
+    # is still live at max_iter on both sides (not converged, so the float32
+    # rounding of pixels and Jacobian moves it along a shallow direction) or
+    # froze at a different iteration on the two sides, typically at 1e-3 of
+    # its own sigma
     assert frac32 >= 0.99
     out = _out_of_bar(r32, ro, FLUX_RTOL, POS_ATOL)
-    assert np.all(r32.n_iter[out] == cfg["max_iter"]) and np.all(ro.n_iter[out] == cfg["max_iter"])
+    mi = cfg["max_iter"]
+    assert np.all(((r32.n_iter[out] == mi) & (ro.n_iter[out] == mi)) | (r32.n_iter[out] != ro.n_iter[out]))
     assert np.median(sf32) < 0.01 and sf32.max() < 3 and sp32.max() < 3
 
 
@@ -178,13 +180,16 @@ def test_serving_large_tiles_alone_equal_mix():
         np.testing.assert_array_equal(alone[i].valid, res.valid)
         np.testing.assert_array_equal(alone[i].residual, res.residual)
         assert alone[i].n_passes == res.n_passes
-        # the fit held no more device memory than it was admitted with
+        # the fit held no more device memory than its estimate, and its slot no
+        # more than it was charged (the estimate plus the blocks it kept cached)
         assert 0 < st["hbm_peak"] <= st["hbm_bytes"], (spec[i], st["hbm_peak"], st["hbm_bytes"])
+        assert st["hbm_resident_peak"] <= st["hbm_charged"], (spec[i], st)
         print(f"  {spec[i]}: {reqs[i]['n']} stars, peak {st['hbm_peak'] / 2**20:.0f} MiB of "
-              f"{st['hbm_bytes'] / 2**20:.0f} MiB admitted, latency {st['latency'] * 1e3:.1f} ms", flush=True)
-    # at any moment the admitted requests fit the budget
-    ev = sorted([(st["t_start"], e) for (_, st), e in zip(served, est)] +
-                [(st["t_end"], -e) for (_, st), e in zip(served, est)], key=lambda v: (v[0], v[1]))
+              f"{st['hbm_bytes'] / 2**20:.0f} MiB estimated, slot {st['hbm_resident_peak'] / 2**20:.0f} MiB of "
+              f"{st['hbm_charged'] / 2**20:.0f} MiB charged, latency {st['latency'] * 1e3:.1f} ms", flush=True)
+    # at any moment the running requests' charges fit the budget
+    ev = sorted([(st["t_start"], st["hbm_charged"]) for _, st in served] +
+                [(st["t_end"], -st["hbm_charged"]) for _, st in served], key=lambda v: (v[0], v[1]))
     used = 0
     for _, d in ev:
         used += d
