@@ -299,6 +299,10 @@ SIGNATURES = {
                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                            C.c_int32, C.POINTER(LMOpts), C.c_void_p, C.c_void_p,
                                            C.c_void_p]),
+    "cp_fit_star_batch_device_noalloc": (C.c_int, [C.c_void_p, C.POINTER(PsfDesc), C.c_void_p, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                   C.c_int32, C.POINTER(LMOpts), C.c_void_p, C.c_void_p,
+                                                   C.c_void_p]),
     "cp_correlate": (C.c_int, [C.c_void_p, f64p, C.c_int32, C.c_int32, f64p, C.c_int32, C.c_int32, C.c_int32,
                                f64p]),
     "cp_correlate_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, f64p, C.c_int32, C.c_int32,

@@ -8,6 +8,7 @@
 // References: fit_star src/psf/psf_fitting.jl:229-308; lm_irls
 // src/levenberg_marquardt.jl:431-680.
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "lm_core.h"
@@ -69,6 +70,31 @@ void launch_lm(int np, dim3 g, hipStream_t s, const LMConst& C, const LMModel<T>
 #undef CP_LM
 }
 
+// Launches of one batched call; the work counter and the ePSF node grids are
+// already in device memory (G and grids unused for the analytic models).
+void lm_batch_launch(cp_ctx* ctx, const cp_psf_desc* psf, const HostPsf& P, const LMConst& C, int np,
+                     const float* grids, const NodeGrid& G, int* work, const float* d_pix, const int64_t* d_off,
+                     const int32_t* d_ny, const int32_t* d_nx, const float* d_iv, const double* d_th0, int n,
+                     double* d_th, double* d_cov, cp_lm_stat* d_st, int precision) {
+    hipStream_t st = ctx->stream;
+    dim3 g((unsigned)std::min<int64_t>((n + NWAVE - 1) / NWAVE,
+                                       (int64_t)std::max(ctx->n_cu, 1) * tuned_bpc(ctx->tuning.lm_blocks_per_cu, 8)));
+    auto go = [&](auto tag, auto kind) {
+        using T = decltype(tag);
+        constexpr int K = decltype(kind)::value;
+        const LMModel<T> Md{model_const<T>(P, grids, 0.0), G};
+        launch_lm<T, K>(np, g, st, C, Md, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work);
+    };
+    using KG = std::integral_constant<int, KIND_GAUSS>;
+    using KM = std::integral_constant<int, KIND_MOFFAT>;
+    using KE = std::integral_constant<int, KIND_EPSF>;
+    const bool f64 = precision == CP_PREC_FP64;
+    if (psf->kind == CP_PSF_CIRCULAR_GAUSSIAN) f64 ? go(0.0, KG{}) : go(0.0f, KG{});
+    else if (psf->kind == CP_PSF_CIRCULAR_MOFFAT) f64 ? go(0.0, KM{}) : go(0.0f, KM{});
+    else f64 ? go(0.0, KE{}) : go(0.0f, KE{});
+    CP_HIP(hipStreamSynchronize(st));
+}
+
 void lm_batch_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_pix, const int64_t* d_off,
                      const int32_t* d_ny, const int32_t* d_nx, const float* d_iv, const double* d_th0,
                      uint32_t free_mask, int n, const cp_lm_opts* o, double* d_th, double* d_cov,
@@ -79,28 +105,90 @@ void lm_batch_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_pix, co
     const char* why = "";
     CP_ARG(lm_const(psf, free_mask, o, d_iv != nullptr, C, np, &why), why);
     if (n == 0) return;
-    hipStream_t st = ctx->stream;
     const HostPsf P = make_host_psf(psf);
     DevPsf DP;
     upload_psf(ctx, psf, P, DP);
     DevBuf work(ctx, 4);
-    CP_HIP(hipMemsetAsync(work.get(), 0, 4, st));
-    dim3 g((unsigned)std::min<int64_t>((n + NWAVE - 1) / NWAVE,
-                                       (int64_t)std::max(ctx->n_cu, 1) * tuned_bpc(ctx->tuning.lm_blocks_per_cu, 8)));
-    auto go = [&](auto tag, auto kind) {
-        using T = decltype(tag);
-        constexpr int K = decltype(kind)::value;
-        const LMModel<T> Md{model_const<T>(P, DP, 0.0), DP.G};
-        launch_lm<T, K>(np, g, st, C, Md, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work.as<int>());
-    };
-    using KG = std::integral_constant<int, KIND_GAUSS>;
-    using KM = std::integral_constant<int, KIND_MOFFAT>;
-    using KE = std::integral_constant<int, KIND_EPSF>;
-    const bool f64 = precision == CP_PREC_FP64;
-    if (psf->kind == CP_PSF_CIRCULAR_GAUSSIAN) f64 ? go(0.0, KG{}) : go(0.0f, KG{});
-    else if (psf->kind == CP_PSF_CIRCULAR_MOFFAT) f64 ? go(0.0, KM{}) : go(0.0f, KM{});
-    else f64 ? go(0.0, KE{}) : go(0.0f, KE{});
-    CP_HIP(hipStreamSynchronize(st));
+    CP_HIP(hipMemsetAsync(work.get(), 0, 4, ctx->stream));
+    lm_batch_launch(ctx, psf, P, C, np, DP.grids.as<float>(), DP.G, work.as<int>(), d_pix, d_off, d_ny, d_nx, d_iv,
+                    d_th0, n, d_th, d_cov, d_st, precision);
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free variant: the work counter and the ePSF node grids live
+// in one file-scope device array per GPU instead of context blocks.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[CP_NOALLOC_SCRATCH_BYTES];
+
+// scratch's device address on each GPU (a host-side use of the symbol itself
+// is not a device address), looked up once per device with it current.
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char*> g_scratch_ptr;
+
+static unsigned char* scratch_on(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        CP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
+static __global__ void k_zero_int(int* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+void lm_batch_noalloc(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_pix, const int64_t* d_off,
+                      const int32_t* d_ny, const int32_t* d_nx, const float* d_iv, const double* d_th0,
+                      uint32_t free_mask, int n, const cp_lm_opts* o, double* d_th, double* d_cov,
+                      cp_lm_stat* d_st) {
+    CP_ARG(psf && o, "NULL argument");
+    LMConst C;
+    int np = 0;
+    const char* why = "";
+    CP_ARG(lm_const(psf, free_mask, o, d_iv != nullptr, C, np, &why), why);
+    const HostPsf P = make_host_psf(psf);
+    // layout: [work counter | node grids | grid_y | grid_x | index_grid], 256-byte aligned pieces
+    const bool epsf = P.kind == CP_PSF_GRIDDED_IMAGE;
+    const bool multi = epsf && P.n_nodes > 1;
+    const size_t ng = epsf ? (size_t)P.n_nodes * P.gny * P.gnx : 0;
+    const size_t o_grids = 256, o_y = o_grids + round256(ng * 4), o_x = o_y + round256(multi ? P.ygrid.size() * 8 : 0),
+                 o_idx = o_x + round256(multi ? P.xgrid.size() * 8 : 0),
+                 end = o_idx + round256(multi ? P.index_grid.size() * 4 : 0);
+    if (end > CP_NOALLOC_SCRATCH_BYTES)
+        throw Error(CP_ENOMEM, "cp_fit_star_batch_device_noalloc: the PSF needs " + std::to_string(end) +
+                                   " bytes of device scratch, more than its " +
+                                   std::to_string((size_t)CP_NOALLOC_SCRATCH_BYTES) + "-byte maximum");
+    if (n == 0) return;
+    hipStream_t st = ctx->stream;
+    unsigned char* base = scratch_on(ctx->device);
+    int* work = reinterpret_cast<int*>(base);
+    // a fill kernel, not a memset node, so the call can sit inside a captured graph
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, work);
+    CP_HIP(hipGetLastError());
+    NodeGrid G{};
+    const float* grids = nullptr;
+    if (epsf) {
+        CP_HIP(hipMemcpyAsync(base + o_grids, psf->grids, ng * 4, hipMemcpyHostToDevice, st));
+        grids = reinterpret_cast<const float*>(base + o_grids);
+        G.ngy = (int)P.ygrid.size();
+        G.ngx = (int)P.xgrid.size();
+        if (multi) {
+            CP_HIP(hipMemcpyAsync(base + o_y, P.ygrid.data(), P.ygrid.size() * 8, hipMemcpyHostToDevice, st));
+            CP_HIP(hipMemcpyAsync(base + o_x, P.xgrid.data(), P.xgrid.size() * 8, hipMemcpyHostToDevice, st));
+            CP_HIP(hipMemcpyAsync(base + o_idx, P.index_grid.data(), P.index_grid.size() * 4, hipMemcpyHostToDevice,
+                                  st));
+            G.ygrid = reinterpret_cast<const double*>(base + o_y);
+            G.xgrid = reinterpret_cast<const double*>(base + o_x);
+            G.index_grid = reinterpret_cast<const int*>(base + o_idx);
+        }
+    }
+    lm_batch_launch(ctx, psf, P, C, np, grids, G, work, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st,
+                    CP_PREC_MIXED);
 }
 
 }  // namespace cp
@@ -135,6 +223,20 @@ int cp_fit_star_batch_device(cp_ctx* ctx, const cp_psf_desc* psf, const float* d
         CP_HIP(hipSetDevice(ctx->device));
         lm_batch_device(ctx, psf, d_pix, d_offsets, d_ny, d_nx, d_inv_var, d_theta0, free_mask, n_cutouts, opts,
                         d_theta_out, d_cov_out, d_stats, CP_PREC_MIXED);
+    });
+}
+
+int cp_fit_star_batch_device_noalloc(cp_ctx* ctx, const cp_psf_desc* psf, const float* d_pix,
+                                     const int64_t* d_offsets, const int32_t* d_ny, const int32_t* d_nx,
+                                     const float* d_inv_var, const double* d_theta0, uint32_t free_mask,
+                                     int32_t n_cutouts, const cp_lm_opts* opts, double* d_theta_out,
+                                     double* d_cov_out, cp_lm_stat* d_stats) {
+    if (!ctx) return CP_ESTATE;
+    return guarded(ctx, [&] {
+        CP_ARG(n_cutouts >= 0, "negative cutout count");
+        CP_HIP(hipSetDevice(ctx->device));
+        lm_batch_noalloc(ctx, psf, d_pix, d_offsets, d_ny, d_nx, d_inv_var, d_theta0, free_mask, n_cutouts, opts,
+                         d_theta_out, d_cov_out, d_stats);
     });
 }
 

@@ -137,8 +137,10 @@ inline void upload_psf(cp_ctx* ctx, const cp_psf_desc* d, const HostPsf& P, DevP
     }
 }
 
+// Model constants over node grids already in device memory (grids may be
+// null for the analytic models).
 template <typename T>
-ModelConst<T> model_const(const HostPsf& P, const DevPsf& D, double bkg) {
+ModelConst<T> model_const(const HostPsf& P, const float* grids, double bkg) {
     ModelConst<T> M{};
     const double GP = CP_GAUSS_PRE;
     if (P.kind == CP_PSF_CIRCULAR_GAUSSIAN) {
@@ -162,7 +164,7 @@ ModelConst<T> model_const(const HostPsf& P, const DevPsf& D, double bkg) {
         M.ox = (T)P.ox;
         M.oy = (T)P.oy;
         M.fill = (T)P.fill;
-        M.grids = D.grids.as<float>();
+        M.grids = grids;
         M.ios_y = (int)P.osy;
         M.ios_x = (int)P.osx;
         M.Qy = (P.gny + 2 * EPSF_PAD + M.ios_y - 1) / M.ios_y;
@@ -170,6 +172,11 @@ ModelConst<T> model_const(const HostPsf& P, const DevPsf& D, double bkg) {
     }
     M.bkg = (T)bkg;
     return M;
+}
+
+template <typename T>
+ModelConst<T> model_const(const HostPsf& P, const DevPsf& D, double bkg) {
+    return model_const<T>(P, D.grids.as<float>(), bkg);
 }
 
 }  // namespace cp

@@ -271,6 +271,25 @@ int cp_fit_star_batch_device(cp_ctx* ctx, const cp_psf_desc* psf,
                              int32_t n_cutouts, const cp_lm_opts* opts,
                              double* d_theta_out, double* d_cov_out, cp_lm_stat* d_stats);
 
+/* Allocation-free form of cp_fit_star_batch_device.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, same results (the same kernels); the call
+ * makes no device allocation: its work counter and the ePSF node grids (and,
+ * for several nodes, their grid coordinates and index grid) go to one static
+ * device array per GPU of CP_NOALLOC_SCRATCH_BYTES, whose address is looked up
+ * once per device.  A PSF whose grids do not fit is rejected with CP_ENOMEM.
+ * Every call on one GPU shares that array, so calls that overlap on one GPU
+ * (other threads, other contexts' streams) race; serialise them.  Returns
+ * when the results are in device memory. */
+#define CP_NOALLOC_SCRATCH_BYTES (8u << 20)
+int cp_fit_star_batch_device_noalloc(cp_ctx* ctx, const cp_psf_desc* psf,
+                                     const float* d_pix, const int64_t* d_offsets,
+                                     const int32_t* d_ny, const int32_t* d_nx,
+                                     const float* d_inv_var, const double* d_theta0, uint32_t free_mask,
+                                     int32_t n_cutouts, const cp_lm_opts* opts,
+                                     double* d_theta_out, double* d_cov_out, cp_lm_stat* d_stats);
+
 /* ---- sequential multi-pass photometry: fit_all_stars ---------------------
  * fit_all_stars, src/photometry/psf_photometry_single.jl:267-469 (replaces
  * that Julia function; every model kind above).

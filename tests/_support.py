@@ -96,3 +96,55 @@ def ptr(a, t):
 
 def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+_hip = None
+
+
+def hip():
+    """ctypes handle on the HIP runtime the product library itself links
+    (libamdhip64.so.7), for tests that hand device buffers to the C ABI."""
+    global _hip
+    if _hip is None:
+        load_pkg()._abi.load_library()   # resolves the same libamdhip64 the library uses
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipDeviceSynchronize.argtypes = []
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """A device copy of a numpy array (hipMalloc + hipMemcpy); .ptr is the
+    device address, .get() copies it back; freed on close / garbage collection."""
+
+    def __init__(self, a=None, nbytes=None):
+        h = hip()
+        self.nbytes = int(a.nbytes if a is not None else nbytes)
+        p = C.c_void_p()
+        if h.hipMalloc(C.byref(p), max(self.nbytes, 1)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        self.ptr = p
+        if a is not None and self.nbytes:
+            src = np.ascontiguousarray(a)
+            if h.hipMemcpy(self.ptr, src.ctypes.data, self.nbytes, 1) != 0:   # host to device
+                raise RuntimeError("hipMemcpy failed")
+
+    def get(self, dtype, shape=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        if self.nbytes and hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) != 0:   # device to host
+            raise RuntimeError("hipMemcpy failed")
+        return out if shape is None else out.reshape(shape)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
