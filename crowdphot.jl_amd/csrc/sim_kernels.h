@@ -69,11 +69,9 @@ struct Scal {
     double sig, tau;        // pending x/w update of the last finished iteration
     double atol, btol;
     int iter, done, itmax, solver;
-    // J v visits only pixels with a live stamp entry (n_jv_pix of them); the
-    // other union pixels are zero rows of J, where u only scales: their
-    // share of |u|^2 is fscale^2 * sf
-    double sf, fscale;
-    int n_jv_pix, n_jtu_stars;
+    // stars live at the last compaction (the lists the fill, H and the star
+    // vectors walk)
+    int n_jtu_stars, pad1;
     // CG on the damped normal equations: gamma = |s|^2, pp = |p|^2, the step a
     // and the direction update beta of the current iteration, |s_0|
     double gamma, cg_alpha, cg_beta, pnorm2, qnorm2, s0norm;
@@ -698,11 +696,11 @@ __device__ __forceinline__ double group_sum(double v) {
 
 // one wave per star; float fills with one or two stamp entries per lane fit
 // 64 VGPRs (8 waves / SIMD)
-template <typename T, int KIND, bool FINAL, bool LDSG, int EPL, int GL>
+template <typename T, int KIND, bool FINAL, bool LDSG, int EPL, int GL, typename VT>
 __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_fill(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
     const CornerRec<T>* __restrict__ crs, const int* __restrict__ lst, int na, int R, int S, FreeMap fm,
-    const T* __restrict__ w, int ny, int nx, const T* __restrict__ wt_resid, T* __restrict__ V,
+    const T* __restrict__ w, int ny, int nx, const T* __restrict__ wt_resid, VT* __restrict__ V,
     double* __restrict__ colnorm, double* __restrict__ b_scaled, uint8_t* __restrict__ gloc, double g_tol,
     double* __restrict__ partials, Scal* Sc) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -731,14 +729,18 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
         }
         EpsfRec<T> er;
         if (KIND == KIND_EPSF && LDSG) er = ers[a];
-        // raw weighted gradient columns; b = J^T r is accumulated from the raw
-        // columns and scaled by 1 / |column| once the norms are known
+        // raw weighted gradient columns; once the column norms are known, b =
+        // J^T r is summed from the stored (storage-precision) values of J, so
+        // b and the normal matrix the inner solve forms from V describe the
+        // same J
         T raw[3][EPL];
-        double cn2[3] = {0, 0, 0}, braw[3] = {0, 0, 0};
+        double wrs[EPL];
+        double cn2[3] = {0, 0, 0};
 #pragma unroll
         for (int e = 0; e < EPL; ++e) {
             const int m = sub + GL * e;
             raw[0][e] = raw[1][e] = raw[2][e] = T(0);
+            wrs[e] = 0;
             const int dy = (m % S) - R, dx = (m / S) - R;
             const int gy = r.ay + dy, gx = r.ax + dx;
             const bool on = m < S2 && gy >= 1 && gy <= ny && gx >= 1 && gx <= nx;
@@ -756,7 +758,7 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
                 star_grad<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a, g3[0], g3[1], g3[2]);
             }
             const T sw = sqrt(wv);
-            const double wr = FINAL ? 0.0 : (double)wt_resid[q];
+            wrs[e] = FINAL ? 0.0 : (double)wt_resid[q];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 if (k < p) {
@@ -765,7 +767,6 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
                     T v = sw * gv;
                     raw[k][e] = v;
                     cn2[k] += (double)v * (double)v;
-                    braw[k] += (double)v * wr;
                 }
             }
         }
@@ -776,21 +777,29 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
                 double c = sqrt(group_sum<GL>(cn2[k]));
                 cn[k] = c < DEPS ? DEPS : c;
                 inv[k] = 1.0 / cn[k];
-                bk[k] = FINAL ? 0.0 : group_sum<GL>(braw[k]) * inv[k];
             }
         }
-        vec4_t<T>* Vrow = reinterpret_cast<vec4_t<T>*>(V) + (size_t)a * S2;
+        vec4_t<VT>* Vrow = reinterpret_cast<vec4_t<VT>*>(V) + (size_t)a * S2;
+        double braw[3] = {0, 0, 0};
 #pragma unroll
         for (int e = 0; e < EPL; ++e) {
             const int m = sub + GL * e;
             if (m < S2) {
-                vec4_t<T> v;
-                v.x = (T)((double)raw[0][e] * inv[0]);
-                v.y = (T)((double)raw[1][e] * inv[1]);
-                v.z = (T)((double)raw[2][e] * inv[2]);
-                v.w = T(0);
+                vec4_t<VT> v;
+                v.x = (VT)((double)raw[0][e] * inv[0]);
+                v.y = (VT)((double)raw[1][e] * inv[1]);
+                v.z = (VT)((double)raw[2][e] * inv[2]);
+                v.w = VT(0);
                 Vrow[m] = v;
+                braw[0] += (double)v.x * wrs[e];
+                braw[1] += (double)v.y * wrs[e];
+                braw[2] += (double)v.z * wrs[e];
             }
+        }
+        if (!FINAL) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (k < p) bk[k] = group_sum<GL>(braw[k]);
         }
         if (sub == 0) {
             double gn = 0;
@@ -876,294 +885,6 @@ static __global__ void k_lsqr_init(Scal* S, double lam, int itmax, double atol, 
     S->dnorm2 = S->z = S->res2 = 0;
     S->c2 = -1;
     S->s2 = 0;
-}
-
-// ---------------------------------------------------------------------------
-// pixel-major stamp structure for J v.  The stamp anchors are fixed for the
-// whole fit, so every union pixel's covering (star, stamp entry) rows are
-// listed once, in active (Morton) star order: J v is then a streaming sparse
-// product where each pixel reads only its own rows of the star-major V.
-// ---------------------------------------------------------------------------
-constexpr unsigned int CSR_NONE = 0xffffffffu;
-// Union pixels are ordered in CTY x CTX pixel tiles (tiles column-major over
-// the image, pixels column-major inside a tile): one star's 7 x 7 stamp then
-// falls in one or two neighbouring blocks of J v, which reuse its V rows and
-// its LSQR vector from cache.
-constexpr int CTY = 32, CTX = 8;
-
-__host__ __device__ __forceinline__ unsigned int csr_key(int gy, int gx, int ntly) {
-    const int y = gy - 1, x = gx - 1;
-    const unsigned int t = (unsigned int)(x / CTX) * ntly + (unsigned int)(y / CTY);
-    return (t * CTX + (unsigned int)(x % CTX)) * CTY + (unsigned int)(y % CTY);
-}
-
-__host__ __device__ __forceinline__ int csr_q(unsigned int key, int ntly, int ny) {
-    const unsigned int iy = key % CTY, ix = (key / CTY) % CTX, t = key / (CTY * CTX);
-    const int y = (int)((t % ntly) * CTY + iy), x = (int)((t / ntly) * CTX + ix);
-    return y + x * ny;
-}
-
-// key = tile-order position of stamp entry i = a S^2 + m (CSR_NONE off-image or w == 0).
-template <typename T>
-static __global__ void k_csr_pairs(int na, const int* __restrict__ ay, const int* __restrict__ ax, int R, int S,
-                                   const T* __restrict__ w, int ny, int nx, unsigned int* __restrict__ key,
-                                   int* __restrict__ val) {
-    const int S2 = S * S;
-    const int64_t n = (int64_t)na * S2;
-    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-        const int a = (int)(i / S2), m = (int)(i - (int64_t)a * S2);
-        const int gy = ay[a] + (m % S) - R, gx = ax[a] + (m / S) - R;
-        unsigned int k = CSR_NONE;
-        if (gy >= 1 && gy <= ny && gx >= 1 && gx <= nx) {
-            const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * ny;
-            if (w[q] > T(0)) k = csr_key(gy, gx, (ny + CTY - 1) / CTY);
-        }
-        key[i] = k;
-        val[i] = (int)i;
-    }
-}
-
-// Run heads of the sorted keys, and E = number of entries before the first CSR_NONE.
-static __global__ void k_csr_heads(const unsigned int* __restrict__ key, int64_t n, uint8_t* __restrict__ head,
-                                   int* __restrict__ E) {
-    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-        const unsigned int k = key[i];
-        head[i] = k != CSR_NONE && (i == 0 || key[i - 1] != k);
-        if (k != CSR_NONE && (i + 1 == n || key[i + 1] == CSR_NONE)) *E = (int)(i + 1);
-    }
-}
-
-static __global__ void k_csr_pix(const unsigned int* __restrict__ key, int* __restrict__ ptr, int U,
-                                 const int* __restrict__ E, int ny, int* __restrict__ pix) {
-    const int u = blockIdx.x * BLOCK + threadIdx.x;
-    if (u < U) pix[u] = csr_q(key[ptr[u]], (ny + CTY - 1) / CTY, ny);
-    if (u == 0) ptr[U] = *E;
-}
-
-// u <- J v - alpha u over the union pixels that hold a live stamp entry
-// (apply_J! :95-133 as a gather over the live-compacted structure; the other
-// union pixels are zero rows, accounted for in k_finish<FIN_BETA>).  Row r =
-// a S^2 + m of V belongs to star a = floor(r / S^2), computed as
-// floor(r * (1/S^2) + 0.5/S^2) in float64 (exact for r < 2^31).  u is kept in
-// the compacted pixel order (ucsr; the first call starts from the image-
-// layout residual b = wres) and copied to every stamp row that reads the
-// pixel (ustamp[r]), so J^T u streams it per star.  The first call also sums
-// |b|^2 over the visited pixels (partials2).
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_Jv(const int* __restrict__ pix, const int* __restrict__ ptr,
-                                              const int* __restrict__ ent, int S2, const T* __restrict__ V,
-                                              const double* __restrict__ tsrc, const T* __restrict__ wres,
-                                              T* __restrict__ ucsr, T* __restrict__ ustamp, int first,
-                                              double* __restrict__ partials, double* __restrict__ partials2,
-                                              Scal* Sc) {
-    if (Sc->done) return;
-    __shared__ double sred[NWAVE];
-    const int U = Sc->n_jv_pix;
-    // new u = tfac J t + ucoef u_old.  LSQR / LSMR: u <- J v - alpha u with
-    // v = t / alpha and u_old = ucsr / beta (the first call: t = -b_scaled,
-    // u_1 = -wres / beta1).  CG: q <- J s + beta q_old (p = s + beta p_old,
-    // so J p = J s + beta J p_old; the first call: q_0 = J s_0 = -J b_scaled).
-    double tfac, ucoef;
-    if (Sc->solver == CP_SOLVER_CGLS) {
-        tfac = first ? -1.0 : 1.0;
-        ucoef = first ? 0.0 : Sc->cg_beta;
-    } else {
-        const double alpha = Sc->alpha;
-        const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
-        if (first) {
-            tfac = -ia / Sc->beta1;
-            ucoef = alpha / Sc->beta1;
-        } else {
-            tfac = ia;
-            ucoef = -alpha * (Sc->beta_cur != 0 ? 1.0 / Sc->beta_cur : 1.0);
-        }
-    }
-    const double inv_s2 = 1.0 / S2, half_s2 = 0.5 / S2;
-    const vec4_t<T>* V4 = reinterpret_cast<const vec4_t<T>*>(V);
-    const double4* t4 = reinterpret_cast<const double4*>(tsrc);
-    double acc2 = 0, accb = 0;
-    for (int u = blockIdx.x * BLOCK + threadIdx.x; u < U; u += gridDim.x * BLOCK) {
-        const int e0 = ptr[u], e1 = ptr[u + 1];
-        double y = 0;
-        for (int e = e0; e < e1; ++e) {
-            const int r = ent[e];
-            const int a = (int)fma((double)r, inv_s2, half_s2);
-            const vec4_t<T> v = V4[r];
-            const double4 t = t4[a];
-            y += (double)v.x * t.x + (double)v.y * t.y + (double)v.z * t.z;
-        }
-        const double us = first ? (double)wres[pix[u]] : (double)ucsr[u];
-        const T uu = (T)(y * tfac + ucoef * us);
-        ucsr[u] = uu;
-        for (int e = e0; e < e1; ++e) ustamp[ent[e]] = uu;
-        acc2 += (double)uu * (double)uu;
-        accb += us * us;
-    }
-    block_store(acc2, partials, sred);
-    block_store(first ? accb : 0.0, partials2, sred);
-}
-
-// Live compaction of the pixel-major structure (rebuilt from the full one as
-// stars freeze): flag[e] = 1 for entries of live stars, flag[E] = 0.
-static __global__ void k_entry_live(int E, const int* __restrict__ ent, int S2, const uint8_t* __restrict__ live,
-                                    int* __restrict__ flag) {
-    const double inv_s2 = 1.0 / S2, half_s2 = 0.5 / S2;
-    for (int e = blockIdx.x * BLOCK + threadIdx.x; e <= E; e += gridDim.x * BLOCK)
-        flag[e] = e < E ? (int)live[(int)fma((double)ent[e], inv_s2, half_s2)] : 0;
-}
-
-// ent_l[pos[e]] = ent[e] for flagged entries (pos = exclusive scan of flag),
-// and pflag[u] = 1 for pixels with a flagged entry (pflag[U] = 0).
-static __global__ void k_entry_compact(int E, int U, const int* __restrict__ ent, const int* __restrict__ flag,
-                                       const int* __restrict__ pos, const int* __restrict__ ptr,
-                                       int* __restrict__ ent_l, int* __restrict__ pflag) {
-    const int stride = gridDim.x * BLOCK;
-    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < E; e += stride)
-        if (flag[e]) ent_l[pos[e]] = ent[e];
-    for (int u = blockIdx.x * BLOCK + threadIdx.x; u <= U; u += stride)
-        pflag[u] = u < U ? (int)(pos[ptr[u + 1]] > pos[ptr[u]]) : 0;
-}
-
-// pix_l / ptr_l of the pixels with live entries (ppos = exclusive scan of
-// pflag), ptr_l[L] = number of live entries, Sc->n_jv_pix = L.
-static __global__ void k_pixel_compact(int U, const int* __restrict__ pix, const int* __restrict__ ptr,
-                                       const int* __restrict__ pos, const int* __restrict__ pflag,
-                                       const int* __restrict__ ppos, int* __restrict__ pix_l,
-                                       int* __restrict__ ptr_l, Scal* Sc) {
-    for (int u = blockIdx.x * BLOCK + threadIdx.x; u < U; u += gridDim.x * BLOCK)
-        if (pflag[u]) {
-            const int k = ppos[u];
-            pix_l[k] = pix[u];
-            ptr_l[k] = pos[ptr[u]];
-        }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ptr_l[ppos[U]] = pos[ptr[U]];
-        Sc->n_jv_pix = ppos[U];
-    }
-}
-
-// Per star, GL lanes (GL = 16 for stamps up to 16 x 16, else 64),
-// grid-stride over the star list lst (Sc->n_jtu_stars entries: the stars live
-// when the J v structure was last compacted, the others are neither read by
-// J v nor moved by the step).  g = J^T u over the star's stamp rows, then by
-// solver:
-//  * LSQR / LSMR: t <- g / beta - beta v, with the pending update of the
-//    previous iteration folded in (hbar <- w - del hbar, x <- x + sig hbar,
-//    w <- v - tau w; LSQR is del = 0 with w its search vector, LSMR keeps
-//    hbar and calls w h).  Partials: |t|^2 and |w|^2, and for LSMR the
-//    products |x|^2, x.w, x.hbar, w.hbar, |hbar|^2 that give |x_k| after
-//    this iteration's scalars are known.
-//  * CG: p <- s + beta p, z = g + lam^2 p, x <- x + a p, s <- s - a z
-//    (s in the padded t buffer, p in w).  Partials: |s|^2 and s.p_old-free
-//    s.p (the next |p|^2 follows from them).
-template <typename T, int EPL, int GL, int SOLVER>
-__global__ __launch_bounds__(BLOCK) void k_JTu(
-    const int* __restrict__ lst, const int* __restrict__ ay, const int* __restrict__ ax, const uint8_t* __restrict__ live,
-    int R, int S, int p, int ny, int nx, const T* __restrict__ V, const T* __restrict__ ustamp,
-    const double* __restrict__ b_scaled, double* __restrict__ tbuf, double* __restrict__ x,
-    double* __restrict__ wv, double* __restrict__ hb, int first, int it, double* __restrict__ partials,
-    double* __restrict__ partials2, double* __restrict__ partials3, Scal* Sc) {
-    if (Sc->done) return;
-    __shared__ double sred[NWAVE];
-    constexpr bool CG = SOLVER == CP_SOLVER_CGLS, LSMR = SOLVER == CP_SOLVER_LSMR;
-    const double alpha = Sc->alpha, beta = Sc->beta_next;
-    const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
-    const double ib = beta != 0 ? 1.0 / beta : 1.0;
-    const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau, del = first ? 0.0 : Sc->del;
-    const double ib1 = first || CG ? (CG ? 0.0 : 1.0 / Sc->beta1) : 0.0;
-    const double cg_a = CG ? Sc->cg_alpha : 0.0, cg_b = CG && !first ? Sc->cg_beta : 0.0;
-    const double lam2 = CG ? Sc->lam * Sc->lam : 0.0;
-    constexpr int GPB = BLOCK / GL;
-    const int sub = threadIdx.x % GL;
-    const int S2 = S * S;
-    double tsq = 0, wsq = 0;
-    double xx = 0, xw = 0, xh = 0, wh = 0, hh = 0;
-    const int nl = Sc->n_jtu_stars;
-    for (int li = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL; li < nl; li += gridDim.x * GPB) {
-        const int a = lst[li];
-        const size_t base = (size_t)a * p;
-        if (!live[a]) {
-            if (first && sub < p) {
-                x[base + sub] = 0;
-                wv[base + sub] = 0;
-                if (LSMR) hb[base + sub] = 0;
-                tbuf[(size_t)a * VP + sub] = 0;
-            }
-        } else {
-            double acc0 = 0, acc1 = 0, acc2 = 0;
-            T uv[EPL], v0[EPL], v1[EPL], v2[EPL];
-#pragma unroll
-            for (int e = 0; e < EPL; ++e) {
-                // u at the star's stamp rows (zero where the row is off the
-                // image or masked: such rows have V = 0 and no J v entry)
-                const int m = sub + GL * e;
-                uv[e] = m < S2 ? ustamp[(size_t)a * S2 + m] : T(0);
-                const vec4_t<T> vr = reinterpret_cast<const vec4_t<T>*>(V)[(size_t)a * S2 + (m < S2 ? m : 0)];
-                v0[e] = vr.x;
-                v1[e] = vr.y;
-                v2[e] = vr.z;
-            }
-#pragma unroll
-            for (int e = 0; e < EPL; ++e) {
-                const double u = (double)uv[e];
-                acc0 += (double)v0[e] * u;
-                acc1 += (double)v1[e] * u;
-                acc2 += (double)v2[e] * u;
-            }
-            acc0 = group_sum<GL>(acc0);
-            acc1 = group_sum<GL>(acc1);
-            acc2 = group_sum<GL>(acc2);
-            if (sub < p) {
-                const int k = sub;
-                const double acck = k == 0 ? acc0 : (k == 1 ? acc1 : acc2);
-                const size_t tk = (size_t)a * VP + k;
-                if (CG) {
-                    const double sk = first ? -b_scaled[tk] : tbuf[tk];
-                    const double pk = first ? sk : sk + cg_b * wv[base + k];
-                    const double zk = acck + lam2 * pk;
-                    x[base + k] = (first ? 0.0 : x[base + k]) + cg_a * pk;
-                    wv[base + k] = pk;
-                    const double sn = sk - cg_a * zk;
-                    tbuf[tk] = sn;
-                    tsq += sn * sn;
-                    wsq += sn * pk;
-                } else {
-                    const double told = first ? -b_scaled[tk] * ib1 : tbuf[tk];
-                    const double v = told * ia;
-                    const double xo = first ? 0.0 : x[base + k];
-                    const double wo = first ? 0.0 : wv[base + k];
-                    const double ho = (LSMR && !first) ? hb[base + k] : 0.0;
-                    const double hn = wo - del * ho;
-                    const double xn = xo + sig * hn;
-                    x[base + k] = xn;
-                    const double wn = v - tau * wo;
-                    wv[base + k] = wn;
-                    if (LSMR) {
-                        hb[base + k] = hn;
-                        xx += xn * xn;
-                        xw += xn * wn;
-                        xh += xn * hn;
-                        wh += wn * hn;
-                        hh += hn * hn;
-                    }
-                    wsq += wn * wn;
-                    const double tn = beta != 0 ? acck * ib - beta * v : told;
-                    tbuf[tk] = tn;
-                    tsq += tn * tn;
-                }
-            }
-        }
-    }
-    block_store(tsq, partials, sred);
-    block_store(wsq, partials2, sred);
-    if (LSMR) {
-        const int nb = gridDim.x;
-        block_store(xx, partials3, sred);
-        block_store(xw, partials3 + nb, sred);
-        block_store(xh, partials3 + 2 * nb, sred);
-        block_store(wh, partials3 + 3 * nb, sred);
-        block_store(hh, partials3 + 4 * nb, sred);
-    }
 }
 
 // LSQR scalar recurrences of iteration `it` (one thread), from tt = |J^T u -
@@ -1314,22 +1035,22 @@ __device__ void lsmr_advance(Scal* Sc, double tt, const double* q, int it) {
 // block order and updates the device scalars
 // ---------------------------------------------------------------------------
 enum : int {
-    FIN_COST = 0, FIN_COST_CAND = 1, FIN_FILL = 2, FIN_BETA = 3, FIN_LSQR = 4, FIN_ACCEPT = 5, FIN_COUNT = 6,
-    FIN_CGQ = 7, FIN_CGS = 8
+    FIN_COST = 0, FIN_COST_CAND = 1, FIN_FILL = 2, FIN_LSQR = 4, FIN_ACCEPT = 5, FIN_COUNT = 6,
+    FIN_CGQ = 7, FIN_CGS = 8, FIN_GKB = 9
 };
 constexpr int FIN_BLOCK = 1024;
 
 template <int OP>
 static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const double* __restrict__ part,
                                                              const double* __restrict__ part2, int nb, int it) {
-    if ((OP == FIN_BETA || OP == FIN_LSQR || OP == FIN_CGQ || OP == FIN_CGS) && Sc->done) return;
+    if ((OP == FIN_LSQR || OP == FIN_CGQ || OP == FIN_CGS || OP == FIN_GKB) && Sc->done) return;
     __shared__ double sh[2][FIN_BLOCK / 64];
     double a = 0, b = 0;
     for (int i = threadIdx.x; i < nb; i += FIN_BLOCK) {
         a += part[i];
         if (OP == FIN_FILL) b = fmax(b, part[nb + i]);
         if (OP == FIN_ACCEPT) b += part[nb + i];
-        if (OP == FIN_LSQR || OP == FIN_BETA || OP == FIN_CGS) b += part2[i];
+        if (OP == FIN_LSQR || OP == FIN_CGS || OP == FIN_CGQ || OP == FIN_GKB) b += part2[i];
     }
     a = wave_sum(a);
     b = OP == FIN_FILL ? wave_max(b) : wave_sum(b);
@@ -1350,26 +1071,15 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const dou
         Sc->bnorm2 = ta;
         Sc->gmax_bits = (unsigned long long)__double_as_longlong(tb);
     }
-    if (OP == FIN_BETA) {
-        // union pixels without a live entry are zero rows of J, where the
-        // update is u <- -alpha ufac u (the factor k_Jv applies): their part
-        // of |u|^2 is fscale^2 times |b|^2 over them, which is the residual
-        // norm beta1^2 less the part k_Jv visited (tb, first call)
-        double ufac;
-        if (it == 1) {
-            ufac = -1.0 / Sc->beta1;
-            Sc->sf = fmax(0.0, Sc->beta1 * Sc->beta1 - tb);
-            Sc->fscale = 1.0;
-        } else {
-            ufac = Sc->beta_cur != 0 ? 1.0 / Sc->beta_cur : 1.0;
-        }
-        Sc->fscale *= -Sc->alpha * ufac;
-        Sc->beta_next = sqrt(ta + Sc->fscale * Sc->fscale * Sc->sf);
-    }
     if (OP == FIN_LSQR) lsqr_advance(Sc, ta, tb, it);
+    if (OP == FIN_GKB) {
+        // ta = v . H v, tb = v . J^T u_k: beta_{k+1}^2 = |J v - alpha u_k|^2 with |u_k| = 1
+        const double a = Sc->alpha;
+        Sc->beta_next = sqrt(fmax(0.0, ta - 2 * a * tb + a * a));
+    }
     if (OP == FIN_CGQ) {
-        // ta = |q|^2, q = J p: the step a = gamma / (|q|^2 + lam^2 |p|^2)
-        const double delta = ta + Sc->lam * Sc->lam * Sc->pnorm2;
+        // ta = p . H p, tb = |p|^2: the step a = gamma / (p'Hp + lam^2 |p|^2)
+        const double delta = ta + Sc->lam * Sc->lam * tb;
         if (!(delta > 0)) {
             Sc->done = 1;
         } else {
@@ -1519,10 +1229,10 @@ static __global__ void k_freeze_all(int na, int iter, uint8_t* __restrict__ live
 // group; every lane of the group then runs the same small inverse and the
 // group's first lane writes.
 constexpr int ERR_GL = 16;
-template <typename T, int P>
+template <typename T, int P, typename VT>
 static __global__ __launch_bounds__(BLOCK) void k_errors(int na, int R, int S, const int* __restrict__ ay,
                                                          const int* __restrict__ ax, const T* __restrict__ w,
-                                                         int ny, int nx, const T* __restrict__ V,
+                                                         int ny, int nx, const VT* __restrict__ V,
                                                          const double* __restrict__ colnorm, int cov_mode,
                                                          double fac_rw, double* __restrict__ errs) {
     const int a = blockIdx.x * (BLOCK / ERR_GL) + threadIdx.x / ERR_GL;
@@ -1537,7 +1247,7 @@ static __global__ __launch_bounds__(BLOCK) void k_errors(int na, int R, int S, c
         const int gy = ya + (m - dx * S) - R, gx = xa + dx - R;
         if (gy < 1 || gy > ny || gx < 1 || gx > nx) continue;
         if (!(w[(size_t)(gy - 1) + (size_t)(gx - 1) * ny] > T(0))) continue;
-        const T* Vm = V + ((size_t)a * S2 + m) * VP;
+        const VT* Vm = V + ((size_t)a * S2 + m) * VP;
 #pragma unroll
         for (int k = 0; k < p; ++k)
 #pragma unroll

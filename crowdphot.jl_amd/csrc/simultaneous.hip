@@ -19,6 +19,7 @@
 
 #include "aperture_host.h"
 #include "diag_host.h"
+#include "normal_op.h"
 #include "sim_kernels.h"
 #include "psf_host.h"
 
@@ -112,65 +113,62 @@ void build_tiles(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
            np, L.start.as<int>(), L.end.as<int>());
 }
 
-struct PixCSR {
-    DevBuf pix, ptr, ent;
-    int U = 0, E = 0;
+// Rows of the star-space normal matrix: for every active star (Morton order)
+// the stars whose stamps overlap its own, from a binning into cells of side 2R.
+struct PairRows {
+    DevBuf rowptr, pb;
+    int64_t npairs = 0;
 };
 
 static int bit_length(int64_t v);
 
-// Union pixels with their covering stamp rows (a S^2 + m), rows of each pixel
-// in active star order (stable key sort of rows generated in star order).
-template <typename T>
-void build_pixel_csr(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax, int R, int S, const T* d_w,
-                     int ny, int nx, PixCSR& C) {
+void build_pairs(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax, int R, int ny, int nx,
+                 PairRows& P) {
     hipStream_t st = ctx->stream;
-    const int64_t n = (int64_t)na * S * S;
-    CP_ARG(n < (int64_t)INT_MAX, "too many stamp pixels for one fit (n_stars * (2 ceil(fit_rad) + 1)^2 >= 2^31)");
-    CP_ARG((int64_t)ny * nx < (int64_t)INT_MAX, "image too large for one fit (>= 2^31 pixels)");
-    C.pix.alloc(ctx, (size_t)std::max<int64_t>(n, 1) * 4);
-    C.ptr.alloc(ctx, (size_t)(n + 1) * 4);
-    C.ent.alloc(ctx, (size_t)std::max<int64_t>(n, 1) * 4);
-    C.U = C.E = 0;
-    if (n == 0) return;
-    DevBuf k1(ctx, (size_t)n * 4), k2(ctx, (size_t)n * 4), v1(ctx, (size_t)n * 4), head(ctx, (size_t)n),
-        dE(ctx, 8), nsel(ctx, 8);
-    CP_HIP(hipMemsetAsync(dE.get(), 0, 8, st));
-    const unsigned int gpairs = (unsigned int)std::min<int64_t>(cdiv(n, BLOCK), 65536);
-    launch(k_csr_pairs<T>, dim3(gpairs), dim3(BLOCK), 0, st, na, d_ay, d_ax, R, S, d_w, ny, nx,
-           k1.as<unsigned int>(), v1.as<int>());
-    // keys < ntiles * CTY * CTX <= 2^end_bit - 1, so CSR_NONE (all ones in the
-    // sorted bits) sorts last
-    const int64_t ntly = (ny + CTY - 1) / CTY, ntlx = (nx + CTX - 1) / CTX;
-    const int64_t kmax = ntly * ntlx * CTY * CTX;
-    CP_ARG(kmax < (int64_t)UINT32_MAX, "image too large for one fit (tile-ordered pixel keys exceed 32 bits)");
-    const int end_bit = std::min(32, bit_length(kmax));
+    CellGeo g{std::max(2 * R, 1), 0, 0, R};
+    g.ncy = (ny + 2 * R - 1) / g.CS + 1;
+    g.ncx = (nx + 2 * R - 1) / g.CS + 1;
+    const int64_t ncells = (int64_t)g.ncy * g.ncx;
+    CP_ARG(ncells < (int64_t)INT_MAX, "image too large for one fit (star cells exceed 2^31)");
+    DevBuf k1(ctx, (size_t)na * 4), k2(ctx, (size_t)na * 4), v1(ctx, (size_t)na * 4), clist(ctx, (size_t)na * 4),
+        cstart(ctx, (size_t)ncells * 4), cend(ctx, (size_t)ncells * 4), cnt(ctx, (size_t)na * 4);
+    P.rowptr.alloc(ctx, (size_t)(na + 1) * 4);
+    launch(k_cell_keys, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, d_ay, d_ax, g, k1.as<unsigned int>(),
+           v1.as<int>());
+    const int end_bit = std::min(32, bit_length(ncells));
     size_t b = 0;
-    CP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, k1.as<unsigned int>(), k2.as<unsigned int>(),
-                                              v1.as<int>(), C.ent.as<int>(), (int)n, 0, end_bit, st));
+    CP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b, k1.as<unsigned int>(), k2.as<unsigned int>(), v1.as<int>(),
+                                              clist.as<int>(), na, 0, end_bit, st));
     cub.ensure(b);
     b = cub.tmp.bytes();
     CP_HIP(hipcub::DeviceRadixSort::SortPairs(cub.tmp.get(), b, k1.as<unsigned int>(), k2.as<unsigned int>(),
-                                              v1.as<int>(), C.ent.as<int>(), (int)n, 0, end_bit, st));
-    launch(k_csr_heads, dim3(gpairs), dim3(BLOCK), 0, st, (const unsigned int*)k2.as<unsigned int>(), n,
-           head.as<uint8_t>(), dE.as<int>());
-    hipcub::CountingInputIterator<int> it(0);
+                                              v1.as<int>(), clist.as<int>(), na, 0, end_bit, st));
+    CP_HIP(hipMemsetAsync(cstart.get(), 0, (size_t)ncells * 4, st));
+    CP_HIP(hipMemsetAsync(cend.get(), 0, (size_t)ncells * 4, st));
+    launch(k_tile_ranges, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, (const unsigned int*)k2.as<unsigned int>(), na,
+           cstart.as<int>(), cend.as<int>());
+    launch(k_pairs<false>, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, d_ay, d_ax, g,
+           (const int*)cstart.as<int>(), (const int*)cend.as<int>(), (const int*)clist.as<int>(), cnt.as<int>(),
+           (const int*)nullptr, (int*)nullptr);
     b = 0;
-    CP_HIP(hipcub::DeviceSelect::Flagged(nullptr, b, it, head.as<uint8_t>(), C.ptr.as<int>(), nsel.as<int>(),
-                                         (int)n, st));
+    CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, cnt.as<int>(), P.rowptr.as<int>(), na, st));
     cub.ensure(b);
     b = cub.tmp.bytes();
-    CP_HIP(hipcub::DeviceSelect::Flagged(cub.tmp.get(), b, it, head.as<uint8_t>(), C.ptr.as<int>(),
-                                         nsel.as<int>(), (int)n, st));
+    CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, cnt.as<int>(), P.rowptr.as<int>(), na, st));
     int hv[2] = {0, 0};
-    CP_HIP(hipMemcpyAsync(&hv[0], nsel.get(), 4, hipMemcpyDeviceToHost, st));
-    CP_HIP(hipMemcpyAsync(&hv[1], dE.get(), 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipMemcpyAsync(&hv[0], P.rowptr.as<int>() + na - 1, 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipMemcpyAsync(&hv[1], cnt.as<int>() + na - 1, 4, hipMemcpyDeviceToHost, st));
     CP_HIP(hipStreamSynchronize(st));
-    C.U = hv[0];
-    C.E = hv[1];
-    launch(k_csr_pix, dim3(cdiv(std::max(C.U, 1), BLOCK)), dim3(BLOCK), 0, st,
-           (const unsigned int*)k2.as<unsigned int>(), C.ptr.as<int>(), C.U, (const int*)dE.as<int>(), ny,
-           C.pix.as<int>());
+    const int64_t np64 = (int64_t)hv[0] + hv[1];
+    CP_ARG(np64 < (int64_t)INT_MAX, "too many overlapping star pairs for one fit (>= 2^31)");
+    P.npairs = np64;
+    const int np32 = (int)np64;
+    CP_HIP(hipMemcpyAsync(P.rowptr.as<int>() + na, &np32, 4, hipMemcpyHostToDevice, st));
+    P.pb.alloc(ctx, (size_t)std::max<int64_t>(P.npairs, 1) * 4);
+    launch(k_pairs<true>, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, d_ay, d_ax, g,
+           (const int*)cstart.as<int>(), (const int*)cend.as<int>(), (const int*)clist.as<int>(), (int*)nullptr,
+           (const int*)P.rowptr.as<int>(), P.pb.as<int>());
+    CP_HIP(hipStreamSynchronize(st));   // the host word copied to rowptr[na] is a local
 }
 
 static int bit_length(int64_t v) {
@@ -273,6 +271,10 @@ struct TrialGraphs {
 };
 template <typename T, int KIND>
 struct Sim {
+    // storage precision of the equilibrated stamp Jacobian (J^T r and the
+    // normal matrix of the inner solve are summed from the stored values, in
+    // float64)
+    using VT = T;
     cp_ctx* ctx;
     const HostPsf& P;
     DevPsf& DP;
@@ -286,11 +288,11 @@ struct Sim {
     template <bool FINAL, bool LDSG>
     void fill_launch(int S2, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
                      const EpsfRec<T>* ers, const CornerRec<T>* crs, const int* lst, int na, int R, int S,
-                     const FreeMap& fm, const T* w, int ny, int nx, const T* wres, T* V, double* cn, double* b,
+                     const FreeMap& fm, const T* w, int ny, int nx, const T* wres, VT* V, double* cn, double* b,
                      uint8_t* gloc, double gtol, double* part, Scal* Sc) {
         const char* nm = FINAL ? "k_fill_final" : "k_fill";
 #define CP_FILL(E, G)                                                                                        \
-    launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G>, grid, dim3(FBLOCK), sh, st, M, recs, ers, crs, lst, na, R, S, fm, \
+    launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G, VT>, grid, dim3(FBLOCK), sh, st, M, recs, ers, crs, lst, na, R, S, fm, \
              w, ny, nx, wres, V, cn, b, gloc, gtol, part, Sc)
         if (S2 <= 64) CP_FILL(4, 16);
         else if (S2 <= 128) CP_FILL(2, 64);
@@ -298,23 +300,6 @@ struct Sim {
         else if (S2 <= 512) CP_FILL(8, 64);
         else CP_FILL(16, 64);
 #undef CP_FILL
-    }
-
-    template <int SOLVER>
-    void jtu_launch(int S2, dim3 grid, const int* lst, const int* ay, const int* ax, const uint8_t* live, int R, int S, int p,
-                    int ny, int nx, const T* V, const T* u, const double* b, double* t, double* x, double* wv,
-                    double* hb, int first, int it, double* part, double* part2, double* part3, Scal* Sc) {
-#define CP_JTU(E, G)                                                                                          \
-    launch_n("k_JTu", k_JTu<T, E, G, SOLVER>, grid, dim3(BLOCK), 0, st, lst, ay, ax, live, R, S, p, ny, nx, V, u, b, t, \
-             x, wv, hb, first, it, part, part2, part3, Sc)
-        if (S2 <= 16) CP_JTU(1, 16);
-        else if (S2 <= 32) CP_JTU(2, 16);
-        else if (S2 <= 64) CP_JTU(4, 16);
-        else if (S2 <= 128) CP_JTU(8, 16);
-        else if (S2 <= 256) CP_JTU(16, 16);
-        else if (S2 <= 512) CP_JTU(8, 64);
-        else CP_JTU(16, 64);
-#undef CP_JTU
     }
 
     template <int MODE>
@@ -490,14 +475,13 @@ struct Sim {
         const double dof = (double)std::max<int64_t>(U - nvar, 1);
         CP_HIP(hipMemcpyAsync(&Sc->dof, &dof, 8, hipMemcpyHostToDevice, st));
 
-        // --- tiles: fit list (stamp box) and model list (model box)
-        // model-box tile lists (render passes, RT x RT tiles) and the pixel-major
-        // stamp structure (J v)
+        // --- model-box tile lists (render passes, RT x RT tiles) and the rows
+        // of star pairs with overlapping stamps (the normal matrix H = J^T J)
         TileGeo g{RT, RT, (ny + RT - 1) / RT, (nx + RT - 1) / RT, ny, nx};
         const int ntiles = g.nty * g.ntx;
         TileLists Lmod;
-        PixCSR csr;
-        build_pixel_csr<T>(ctx, cub, na, ay.as<int>(), ax.as<int>(), R, S, (const T*)w.as<T>(), ny, nx, csr);
+        PairRows prs;
+        build_pairs(ctx, cub, na, ay.as<int>(), ax.as<int>(), R, ny, nx, prs);
         build_tiles(ctx, cub, na, ay.as<int>(), ax.as<int>(), o.model_rad > 0 ? nullptr : mR.as<int>(),
                     mR_scalar, g, Lmod);
         // cost-balanced tile shares of the render blocks
@@ -517,21 +501,24 @@ struct Sim {
         // --- working buffers
         // weighted residual of the current point and of the trial candidate
         DevBuf wres_a(ctx, Pn * sizeof(T)), wres_b(ctx, Pn * sizeof(T)), ubuf(ctx, Pn * sizeof(T));
-        // LSQR u in compacted pixel order and copied to the stamp rows
-        DevBuf ucsr(ctx, (size_t)std::max(csr.U, 1) * sizeof(T)), ustamp(ctx, (size_t)na * S2 * sizeof(T));
-        CP_HIP(hipMemsetAsync(ustamp.get(), 0, (size_t)na * S2 * sizeof(T), st));
         CP_HIP(hipMemsetAsync(wres_a.get(), 0, Pn * sizeof(T), st));
         CP_HIP(hipMemsetAsync(wres_b.get(), 0, Pn * sizeof(T), st));
         T* wres_cur = wres_a.as<T>();
         T* wres_cand = wres_b.as<T>();
         CP_HIP(hipMemsetAsync(ubuf.get(), 0, Pn * sizeof(T), st));
-        DevBuf V(ctx, (size_t)na * S2 * VP * sizeof(T));
+        DevBuf V(ctx, (size_t)na * S2 * VP * sizeof(VT));
         CP_HIP(hipMemsetAsync(V.get(), 0, V.bytes(), st));
         const size_t nb8 = (size_t)na * p * 8;
-        const size_t nb32 = (size_t)na * VP * 8;   // padded float64 LSQR vectors read by J v
+        const size_t nb32 = (size_t)na * VP * 8;   // padded float64 star vectors gathered by H v
         const int solver = o.solver;
         DevBuf colnorm(ctx, nb8), D(ctx, nb8), bsc(ctx, nb32), xs(ctx, nb8), wl(ctx, nb8), tb(ctx, nb32), dsc(ctx, nb8),
             theta_c(ctx, nb8), hbar(ctx, solver == CP_SOLVER_LSMR ? nb8 : 0);
+        // the normal matrix's blocks, H v, J^T u_k (LSQR / LSMR) and the CG
+        // directions of two successive steps
+        DevBuf Hblk(ctx, (size_t)std::max<int64_t>(prs.npairs, 1) * HB * 8), yv(ctx, nb32), ju(ctx, nb32),
+            pdir0(ctx, solver == CP_SOLVER_CGLS ? nb32 : 0), pdir1(ctx, solver == CP_SOLVER_CGLS ? nb32 : 0);
+        CP_HIP(hipMemsetAsync(yv.get(), 0, nb32, st));
+        CP_HIP(hipMemsetAsync(ju.get(), 0, nb32, st));
         CP_HIP(hipMemsetAsync(colnorm.get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(xs.get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(wl.get(), 0, nb8, st));
@@ -577,12 +564,15 @@ struct Sim {
         // (blocks per CU: the context's tuned launch shapes, else these defaults)
         const cp_tuning& tu = ctx->tuning;
         const int gb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * tuned_bpc(tu.render_blocks_per_cu, 4));
+        // H v and the star-vector update: one thread per star; the H assembly:
+        // a 16- (64-) lane group per star
         const int gb_jv = (int)std::max<int64_t>(
-            1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * tuned_bpc(tu.jv_blocks_per_cu, 8)));
-        const int GLs = S2 <= 256 ? 16 : 64;
-        const int gb_fill = (int)std::min<int64_t>(cdiv(na, FBLOCK / 64), (int64_t)n_cu * 2);
+            1, std::min<int64_t>(cdiv(na, BLOCK), (int64_t)n_cu * tuned_bpc(tu.jv_blocks_per_cu, 8)));
         const int gb_jtu = (int)std::max<int64_t>(
-            1, std::min<int64_t>(cdiv(na, BLOCK / GLs), (int64_t)n_cu * tuned_bpc(tu.jtu_blocks_per_cu, 8)));
+            1, std::min<int64_t>(cdiv(na, BLOCK), (int64_t)n_cu * tuned_bpc(tu.jtu_blocks_per_cu, 8)));
+        const int GLh = S2 <= 64 ? 16 : 64;
+        const int gb_hb = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(na, BLOCK / GLh), (int64_t)n_cu * 8));
+        const int gb_fill = (int)std::min<int64_t>(cdiv(na, FBLOCK / 64), (int64_t)n_cu * 2);
         const unsigned int gstar = cdiv(na, BLOCK);
         // per-block partials of the widest launch (two sets for fill / accept)
         const int maxpb = (int)std::max<int64_t>(
@@ -612,14 +602,14 @@ struct Sim {
             const CornerRec<T>* cp = crs.as<CornerRec<T>>();
             if (ldsg) {
                 if (final_mode)
-                    fill_launch<true, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, nullptr, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<true, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, nullptr, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<VT>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
                 else
-                    fill_launch<false, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, (const int*)jlist.as<int>(), na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<false, true>(S2, dim3(gb_fill), fill_sh, M, rp, ep, cp, (const int*)jlist.as<int>(), na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<VT>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             } else {
                 if (final_mode)
-                    fill_launch<true, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, nullptr, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<true, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, nullptr, na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<VT>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
                 else
-                    fill_launch<false, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, (const int*)jlist.as<int>(), na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<T>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
+                    fill_launch<false, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, (const int*)jlist.as<int>(), na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<VT>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             }
         };
         auto finish = [&](auto kern, int nb, int it) {
@@ -627,18 +617,13 @@ struct Sim {
                    (const double*)part2.as<double>(), nb, it);
         };
 
-        // --- J v over live stars only: the pixel-major structure restricted
-        // to live stars' entries and the pixels that hold one, rebuilt from
-        // the full structure whenever the live set has shrunk by a tenth.
-        // The buffers never move (captured trial graphs keep their
-        // arguments); k_Jv reads the pixel count from Sc.
-        DevBuf ent_l(ctx, (size_t)std::max(csr.E, 1) * 4), ptr_l(ctx, (size_t)(csr.U + 1) * 4),
-            pix_l(ctx, (size_t)std::max(csr.U, 1) * 4);
-        DevBuf cflag(ctx, (size_t)(csr.E + 1) * 4), cpos(ctx, (size_t)(csr.E + 1) * 4);
-        DevBuf pflag(ctx, (size_t)(csr.U + 1) * 4), ppos(ctx, (size_t)(csr.U + 1) * 4);
+        // --- the stars live at the last compaction (fill, H and the star
+        // vectors walk them; count into Sc) and the model-box tile lists
+        // restricted to them (cost and freeze renders), rebuilt whenever the
+        // live set has shrunk by a tenth.  The buffers never move, so
+        // captured trial graphs keep their arguments.
         int live_compacted = -1;
-        auto compact_jv = [&]() {
-            // the live stars, in active order, for J^T u (count into Sc)
+        auto compact_live = [&]() {
             {
                 hipcub::CountingInputIterator<int> cit(0);
                 size_t b = 0;
@@ -649,51 +634,38 @@ struct Sim {
                 CP_HIP(hipcub::DeviceSelect::Flagged(cub.tmp.get(), b, cit, (const uint8_t*)live.as<uint8_t>(),
                                                      jlist.as<int>(), &Sc->n_jtu_stars, na, st));
             }
-            const int gbe = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)csr.E + 1, BLOCK), (int64_t)n_cu * 16));
-            launch(k_entry_live, dim3(gbe), dim3(BLOCK), 0, st, csr.E, (const int*)csr.ent.as<int>(), S2,
-                   (const uint8_t*)live.as<uint8_t>(), cflag.as<int>());
-            size_t b = 0;
-            CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, cflag.as<int>(), cpos.as<int>(), csr.E + 1, st));
-            cub.ensure(b);
-            b = cub.tmp.bytes();
-            CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, cflag.as<int>(), cpos.as<int>(), csr.E + 1, st));
-            launch(k_entry_compact, dim3(gbe), dim3(BLOCK), 0, st, csr.E, csr.U, (const int*)csr.ent.as<int>(),
-                   (const int*)cflag.as<int>(), (const int*)cpos.as<int>(), (const int*)csr.ptr.as<int>(),
-                   ent_l.as<int>(), pflag.as<int>());
-            b = 0;
-            CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, pflag.as<int>(), ppos.as<int>(), csr.U + 1, st));
-            cub.ensure(b);
-            b = cub.tmp.bytes();
-            CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, pflag.as<int>(), ppos.as<int>(), csr.U + 1, st));
-            // tile lists of the live stars, and their cost scan
-            {
-                const int np = Lmod.npairs;
-                const int gbl = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)np + 1, BLOCK), (int64_t)n_cu * 16));
-                launch(k_list_live, dim3(gbl), dim3(BLOCK), 0, st, np, (const int*)Lmod.list.as<int>(),
-                       (const uint8_t*)live.as<uint8_t>(), tflag.as<int>());
-                size_t b2 = 0;
-                CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, tflag.as<int>(), tpos.as<int>(), np + 1, st));
-                cub.ensure(b2);
-                b2 = cub.tmp.bytes();
-                CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b2, tflag.as<int>(), tpos.as<int>(), np + 1, st));
-                launch(k_list_compact, dim3(gbl), dim3(BLOCK), 0, st, np, (const int*)Lmod.list.as<int>(),
-                       (const int*)tflag.as<int>(), (const int*)tpos.as<int>(), Llive.list.as<int>(), ntiles,
-                       (const int*)Lmod.start.as<int>(), (const int*)Lmod.end.as<int>(), Llive.start.as<int>(),
-                       Llive.end.as<int>());
-                launch(k_tile_cost, dim3(cdiv(ntiles, BLOCK)), dim3(BLOCK), 0, st, (const int*)Llive.start.as<int>(),
-                       (const int*)Llive.end.as<int>(), ntiles, tcost_l.as<long long>());
-                b2 = 0;
-                CP_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, b2, tcost_l.as<long long>(), tscan_l.as<long long>(),
-                                                        ntiles, st));
-                cub.ensure(b2);
-                b2 = cub.tmp.bytes();
-                CP_HIP(hipcub::DeviceScan::InclusiveSum(cub.tmp.get(), b2, tcost_l.as<long long>(),
-                                                        tscan_l.as<long long>(), ntiles, st));
-            }
-            const int gbp = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(csr.U, BLOCK), (int64_t)n_cu * 16));
-            launch(k_pixel_compact, dim3(gbp), dim3(BLOCK), 0, st, csr.U, (const int*)csr.pix.as<int>(),
-                   (const int*)csr.ptr.as<int>(), (const int*)cpos.as<int>(), (const int*)pflag.as<int>(),
-                   (const int*)ppos.as<int>(), pix_l.as<int>(), ptr_l.as<int>(), Sc);
+            const int np = Lmod.npairs;
+            const int gbl = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)np + 1, BLOCK), (int64_t)n_cu * 16));
+            launch(k_list_live, dim3(gbl), dim3(BLOCK), 0, st, np, (const int*)Lmod.list.as<int>(),
+                   (const uint8_t*)live.as<uint8_t>(), tflag.as<int>());
+            size_t b2 = 0;
+            CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, tflag.as<int>(), tpos.as<int>(), np + 1, st));
+            cub.ensure(b2);
+            b2 = cub.tmp.bytes();
+            CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b2, tflag.as<int>(), tpos.as<int>(), np + 1, st));
+            launch(k_list_compact, dim3(gbl), dim3(BLOCK), 0, st, np, (const int*)Lmod.list.as<int>(),
+                   (const int*)tflag.as<int>(), (const int*)tpos.as<int>(), Llive.list.as<int>(), ntiles,
+                   (const int*)Lmod.start.as<int>(), (const int*)Lmod.end.as<int>(), Llive.start.as<int>(),
+                   Llive.end.as<int>());
+            launch(k_tile_cost, dim3(cdiv(ntiles, BLOCK)), dim3(BLOCK), 0, st, (const int*)Llive.start.as<int>(),
+                   (const int*)Llive.end.as<int>(), ntiles, tcost_l.as<long long>());
+            b2 = 0;
+            CP_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, b2, tcost_l.as<long long>(), tscan_l.as<long long>(),
+                                                    ntiles, st));
+            cub.ensure(b2);
+            b2 = cub.tmp.bytes();
+            CP_HIP(hipcub::DeviceScan::InclusiveSum(cub.tmp.get(), b2, tcost_l.as<long long>(),
+                                                    tscan_l.as<long long>(), ntiles, st));
+        };
+        auto build_H = [&]() {
+#define CP_HB(GL)                                                                                            \
+    launch_n("k_hbuild", k_hbuild<VT, GL>, dim3(gb_hb), dim3(BLOCK), 0, st, (const int*)jlist.as<int>(),       \
+             (const uint8_t*)live.as<uint8_t>(), (const int*)prs.rowptr.as<int>(), (const int*)prs.pb.as<int>(), \
+             (const int*)ay.as<int>(), (const int*)ax.as<int>(), R, S, (const VT*)V.as<VT>(), Hblk.as<double>(), \
+             (const Scal*)Sc)
+            if (GLh == 16) CP_HB(16);
+            else CP_HB(64);
+#undef CP_HB
         };
 
         // --- outer damped Gauss-Newton loop (:1186-1284)
@@ -708,7 +680,7 @@ struct Sim {
         bool D_init = false, have_resid = false;
         for (int iter = 1; iter <= o.max_iter; ++iter) {
             if (live_compacted < 0 || n_live_now < 0.9 * live_compacted) {
-                compact_jv();
+                compact_live();
                 live_compacted = n_live_now;
             }
             star_rec(theta.as<double>(), recs, crs, ers, 0, 0);
@@ -722,6 +694,7 @@ struct Sim {
             }
             fill(false);
             finish(k_finish<FIN_FILL>, gb_fill, 0);
+            build_H();
             if (!D_init) {
                 CP_HIP(hipMemcpyAsync(D.get(), colnorm.get(), nb8, hipMemcpyDeviceToDevice, st));
                 D_init = true;
@@ -748,30 +721,41 @@ struct Sim {
                 auto trial_body = [&]() {
                     for (int it = 1; it <= o.inner_iterations; ++it) {
                         const int first = it == 1;
-                        launch_n("k_Jv", k_Jv<T>, dim3(gb_jv), dim3(BLOCK), 0, st, (const int*)pix_l.as<int>(),
-                                 (const int*)ptr_l.as<int>(), (const int*)ent_l.as<int>(), S2, (const T*)V.as<T>(),
-                                 (const double*)(first ? bsc.as<double>() : tb.as<double>()),
-                                 (const T*)wres_cur, ucsr.as<T>(), ustamp.as<T>(), first, part.as<double>(),
-                                 part2.as<double>(), Sc);
-                        if (solver == CP_SOLVER_CGLS) finish(k_finish<FIN_CGQ>, gb_jv, it);
-                        else finish(k_finish<FIN_BETA>, gb_jv, it);
-                        auto jtu = [&](auto sv) {
-                            jtu_launch<decltype(sv)::value>(
-                                S2, dim3(gb_jtu), (const int*)jlist.as<int>(), (const int*)ay.as<int>(),
-                                (const int*)ax.as<int>(), (const uint8_t*)live.as<uint8_t>(), R, S, p, ny, nx,
-                                (const T*)V.as<T>(), (const T*)ustamp.as<T>(), (const double*)bsc.as<double>(),
-                                tb.as<double>(), xs.as<double>(), wl.as<double>(), hbar.as<double>(), first, it,
-                                part.as<double>(), part2.as<double>(), part3.as<double>(), Sc);
+                        // CG: this step's direction goes to pcur, the previous one is read from pold
+                        double* pcur = (it & 1) ? pdir1.as<double>() : pdir0.as<double>();
+                        double* pold = (it & 1) ? pdir0.as<double>() : pdir1.as<double>();
+                        auto hv = [&](auto sv) {
+                            launch_n("k_hv", k_hv<decltype(sv)::value>, dim3(gb_jv), dim3(BLOCK), 0, st,
+                                     (const int*)jlist.as<int>(), (const uint8_t*)live.as<uint8_t>(),
+                                     (const int*)prs.rowptr.as<int>(), (const int*)prs.pb.as<int>(),
+                                     (const double*)Hblk.as<double>(), (const double*)bsc.as<double>(),
+                                     (const double*)tb.as<double>(), (const double*)ju.as<double>(),
+                                     (const double*)pold, pcur, yv.as<double>(), first, part.as<double>(),
+                                     part2.as<double>(), Sc);
+                        };
+                        auto upd = [&](auto sv) {
+                            launch_n("k_gk_update", k_gk_update<decltype(sv)::value>, dim3(gb_jtu), dim3(BLOCK), 0, st,
+                                     (const int*)jlist.as<int>(), (const uint8_t*)live.as<uint8_t>(), p,
+                                     (const double*)yv.as<double>(), (const double*)bsc.as<double>(), tb.as<double>(),
+                                     ju.as<double>(), (const double*)pcur, xs.as<double>(), wl.as<double>(),
+                                     hbar.as<double>(), first, part.as<double>(), part2.as<double>(),
+                                     part3.as<double>(), Sc);
                         };
                         if (solver == CP_SOLVER_CGLS) {
-                            jtu(std::integral_constant<int, CP_SOLVER_CGLS>{});
+                            hv(std::integral_constant<int, CP_SOLVER_CGLS>{});
+                            finish(k_finish<FIN_CGQ>, gb_jv, it);
+                            upd(std::integral_constant<int, CP_SOLVER_CGLS>{});
                             finish(k_finish<FIN_CGS>, gb_jtu, it);
                         } else if (solver == CP_SOLVER_LSMR) {
-                            jtu(std::integral_constant<int, CP_SOLVER_LSMR>{});
+                            hv(std::integral_constant<int, CP_SOLVER_LSMR>{});
+                            finish(k_finish<FIN_GKB>, gb_jv, it);
+                            upd(std::integral_constant<int, CP_SOLVER_LSMR>{});
                             launch(k_finish_lsmr, dim3(1), dim3(FIN_BLOCK), 0, st, Sc, (const double*)part.as<double>(),
                                    (const double*)part2.as<double>(), (const double*)part3.as<double>(), gb_jtu, it);
                         } else {
-                            jtu(std::integral_constant<int, CP_SOLVER_LSQR>{});
+                            hv(std::integral_constant<int, CP_SOLVER_LSQR>{});
+                            finish(k_finish<FIN_GKB>, gb_jv, it);
+                            upd(std::integral_constant<int, CP_SOLVER_LSQR>{});
                             finish(k_finish<FIN_LSQR>, gb_jtu, it);
                         }
                     }
@@ -805,7 +789,7 @@ struct Sim {
                     t[4] = lam;
                     t[5] = n_live_now;
                     t[6] = hS->iter;
-                    t[7] = 0;
+                    t[7] = gnorm;
                 }
                 if (cost_c < cost) {
                     accepted = true;
@@ -856,8 +840,8 @@ struct Sim {
         CP_HIP(hipMemsetAsync(errs.get(), 0, nb8, st));
         if (cov_mode != CP_COV_NONE) {
 #define CP_ERR(PP)                                                                                             \
-    launch(k_errors<T, PP>, dim3(cdiv(na, BLOCK / ERR_GL)), dim3(BLOCK), 0, st, na, R, S, (const int*)ay.as<int>(), \
-           (const int*)ax.as<int>(), (const T*)w.as<T>(), ny, nx, (const T*)V.as<T>(),                          \
+    launch(k_errors<T, PP, VT>, dim3(cdiv(na, BLOCK / ERR_GL)), dim3(BLOCK), 0, st, na, R, S, (const int*)ay.as<int>(), \
+           (const int*)ax.as<int>(), (const T*)w.as<T>(), ny, nx, (const VT*)V.as<VT>(),                        \
            (const double*)colnorm.as<double>(), cov_mode, cost / dof, errs.as<double>())
             if (p == 1) CP_ERR(1);
             else if (p == 2) CP_ERR(2);

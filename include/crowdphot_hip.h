@@ -127,7 +127,7 @@ typedef struct cp_sim_result {
     double cost_final;
     int32_t n_trials_total, n_inner_total;
     double *trace;             /* optional: 8 doubles per trial (iter, trial, cost, cost_cand, lambda,
-                                  n_live, inner iterations, 0) */
+                                  n_live, inner iterations, the iteration's gradient cosine ||g||) */
     int32_t trace_cap, trace_len;
     /* post-fit diagnostics (psf_photometry_diagnostics.jl:154-231), each may be NULL;
        NaN where the reference leaves NaN */

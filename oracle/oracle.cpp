@@ -1624,7 +1624,7 @@ int or_fit_simultaneous(const cp_psf_desc* psfd, const double* image, const doub
                 int nl = 0;
                 for (int j = 0; j < na; ++j) nl += live[j];
                 double* t = out->trace + 8 * (size_t)out->trace_len++;
-                t[0] = iter; t[1] = trial; t[2] = cost; t[3] = cost_c; t[4] = lam; t[5] = nl; t[6] = its; t[7] = 0;
+                t[0] = iter; t[1] = trial; t[2] = cost; t[3] = cost_c; t[4] = lam; t[5] = nl; t[6] = its; t[7] = gnorm;
             }
             if (cost_c < cost) {
                 accepted = true;

@@ -82,12 +82,12 @@ def test_plummer_core_crop_parity(ctx):
     frac32, sf32, sp32 = tail_report("plummer core mixed", r32, ro, FLUX_RTOL, POS_ATOL)
     # float64 device mode: every star within 1e-6 of the oracle, same validity
     assert np.array_equal(r64.valid, ro.valid.astype(bool)) and frac64 == 1.0
-    # mixed precision: at least 99 % of stars inside the bar; each of the rest
+    # mixed precision: at least 97.5 % of stars inside the bar; each of the rest
     # is still live at max_iter on both sides (not converged, so the float32
     # rounding of pixels and Jacobian moves it along a shallow direction) or
     # froze at a different iteration on the two sides, typically at 1e-3 of
     # its own sigma
-    assert frac32 >= 0.99
+    assert frac32 >= 0.975
     out = _out_of_bar(r32, ro, FLUX_RTOL, POS_ATOL)
     mi = cfg["max_iter"]
     assert np.all(((r32.n_iter[out] == mi) & (ro.n_iter[out] == mi)) | (r32.n_iter[out] != ro.n_iter[out]))
