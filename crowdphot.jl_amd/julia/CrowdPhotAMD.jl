@@ -7,12 +7,18 @@ forwards the work to the HIP library through `ccall`:
 
 - `fit_all_stars_simultaneous`  (src/photometry/psf_photometry_simultaneous.jl:953)
 - `fit_all_stars`               (sequential multi-pass, src/photometry/psf_photometry_single.jl:267)
+- `fit_star`                    (per-star Levenberg-Marquardt, src/psf/psf_fitting.jl:229-308)
 - `matched_filter`              (src/detection.jl:87)
 
 Depends on the standard library only (`Libdl`), so it loads without `Pkg`
 network access.  The PSF model types are CrowdPhot.jl's own
 (`CircularGaussianPSF`, `CircularMoffatPSF`, `GriddedPSFModel{T,<:ImagePSF}`);
-this module only reads their fields.
+this module only reads their fields.  When CrowdPhot is loaded, the fits
+return its `MultiPassPhotResult` and `LMResult` and `fit_star` its fitted
+model (through ConstructionBase, which CrowdPhot loads); otherwise they
+return NamedTuples with the same fields.  `lm_irls` with a user-written
+`accum!` callback is Julia code and stays on the CPU: the GPU binding is
+`fit_star`, whose accumulators the library implements for these models.
 """
 module CrowdPhotAMD
 
@@ -147,9 +153,51 @@ end
 set_tuning!(ctx::Context, t::CpTuning) =
     check(ctx, ccall(sym(:cp_set_tuning), Cint, (Ptr{Cvoid}, Ref{CpTuning}), ctx.handle, t))
 
+# A loaded package module by name (CrowdPhot, ConstructionBase), or nothing.
+function loaded(name::AbstractString)
+    for (k, m) in Base.loaded_modules
+        k.name == name && return m
+    end
+    return nothing
+end
+
+# _extract_source_catalog (psf_photometry_single.jl:152-197): a
+# MatchedFilterResult (peaks, peak_fluxes), a Vector{<:NamedTuple} from
+# measure_star_shapes (centroid.y, centroid.x, matched_filter_flux) or a
+# NamedTuple (y, x, optional flux = 1 and bkg = 0).
+function source_catalog(sources)
+    if hasproperty(sources, :peaks) && hasproperty(sources, :peak_fluxes)
+        y = Float64[Tuple(p)[1] for p in sources.peaks]
+        x = Float64[Tuple(p)[2] for p in sources.peaks]
+        return (y = y, x = x, flux = Float64.(sources.peak_fluxes), bkg = zeros(length(y)))
+    elseif sources isa AbstractVector
+        y = Float64[s.centroid.y for s in sources]
+        x = Float64[s.centroid.x for s in sources]
+        return (y = y, x = x, flux = Float64[s.matched_filter_flux for s in sources], bkg = zeros(length(y)))
+    end
+    n = length(sources.y)
+    return (y = Float64.(sources.y), x = Float64.(sources.x),
+            flux = haskey(sources, :flux) ? Float64.(sources.flux) : ones(n),
+            bkg = haskey(sources, :bkg) ? Float64.(sources.bkg) : zeros(n))
+end
+
+# CrowdPhot's MultiPassPhotResult{FT} (psf_photometry_single.jl:99-122) from
+# the result fields when CrowdPhot is loaded, else the NamedTuple itself.
+function photometry_result(r, FT)
+    cp = loaded("CrowdPhot")
+    cp === nothing && return r
+    v(a) = Vector{FT}(a)
+    return cp.MultiPassPhotResult{FT}(v(r.y), v(r.x), v(r.y_err), v(r.x_err), v(r.flux), v(r.flux_err), v(r.bkg),
+                                      v(r.bkg_err), r.converged, r.valid, v(r.chisq), v(r.qfit),
+                                      v(r.qfit_expected), v(r.qfit_z), v(r.crowding), v(r.spread_model),
+                                      v(r.spread_model_err), r.n_iter, r.n_passes, r.n_failed, r.failure_msgs,
+                                      Matrix{FT}(r.residual))
+end
+
 # PSF descriptor of a CrowdPhot model (fields read by name; `fixed` shape
-# values override the model's own, as model_from_vector does).
-function psf_desc(psf, fixed, keep)
+# values override the model's own, as model_from_vector does).  `shift`
+# moves a GriddedPSFModel's node grid with a cutout's origin.
+function psf_desc(psf, fixed, keep; shift = (0.0, 0.0))
     T = typeof(psf)
     name = nameof(T)
     g(k) = haskey(fixed, k) ? Float64(getfield(fixed, k)) : Float64(getproperty(psf, k))
@@ -159,8 +207,8 @@ function psf_desc(psf, fixed, keep)
         return CpPsfDesc(1, 0, g(:α), g(:β), 0, 0, 0, C_NULL, 1, 1, 0, 0, 0, C_NULL, C_NULL)
     elseif name === :GriddedPSFModel || name === :ImagePSF
         nodes = name === :ImagePSF ? [psf] : psf.psfs
-        gy = name === :ImagePSF ? [0.0] : Float64.(psf.grid_y)
-        gx = name === :ImagePSF ? [0.0] : Float64.(psf.grid_x)
+        gy = name === :ImagePSF ? [0.0] : Float64.(psf.grid_y) .- shift[1]
+        gx = name === :ImagePSF ? [0.0] : Float64.(psf.grid_x) .- shift[2]
         n0 = nodes[1]
         grids = reduce(vcat, [vec(Float32.(n.data)) for n in nodes])   # column-major per node
         push!(keep, grids, gy, gx)
@@ -175,9 +223,11 @@ end
     fit_all_stars_simultaneous(image, psf, sources, fit_rad; kws...)
 
 Same contract, keywords and `MultiPassPhotResult` as CrowdPhot.jl
-(psf_photometry_simultaneous.jl:953-1431), computed on the GPU.  Extra keywords:
-`precision = :mixed | :fp64`, `ctx::Context`.  Returns a NamedTuple with the
-MultiPassPhotResult fields (construct the CrowdPhot type from it if needed).
+(psf_photometry_simultaneous.jl:953-1431), computed on the GPU; `solver` is
+`:lsqr` (default) or `:lsmr`, as in the reference, or the build's `:cg` (CG on
+the damped normal equations).  Extra keywords: `precision = :mixed | :fp64`,
+`ctx::Context`.  Returns CrowdPhot's `MultiPassPhotResult` when CrowdPhot is
+loaded, else a NamedTuple with its fields.
 """
 function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad::Real;
         fixed::NamedTuple = (;), inv_var = nothing, solver::Symbol = :lsqr, inner_iterations::Int = 10,
@@ -186,8 +236,10 @@ function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad
         x_tol::Real = 1.0e-5, f_tol::Real = 1.0e-4, g_tol::Real = 1.0e-4, λ_init::Real = 1.0e-4,
         λ_up::Real = 10.0, λ_down::Real = 10.0, λ_min::Real = 1.0e-12, λ_max::Real = 1.0e12,
         precision::Symbol = :mixed, spread_model_fwhm::Union{Nothing, Real} = nothing,
-        ctx::Context = Context(0), kws...)
-    solver in (:lsqr,) || throw(ArgumentError("solver must be :lsqr on the GPU, got $(repr(solver))"))
+        show_trace::Bool = false, ctx::Context = Context(0), kws...)
+    solver in (:lsqr, :lsmr, :cg) ||
+        throw(ArgumentError("`solver` must be :lsqr or :lsmr (or the GPU build's :cg), got $(repr(solver))"))
+    code = solver === :lsqr ? Int32(0) : solver === :cg ? Int32(1) : Int32(2)
     (model_rad === :auto || (model_rad isa Real && model_rad > 0)) ||
         throw(ArgumentError("model_rad must be :auto or a positive real, got $(repr(model_rad))"))
     props = propertynames_fit(psf)
@@ -196,15 +248,15 @@ function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad
     off = setdiff(free, (:y, :x, :flux))
     isempty(off) || throw(ArgumentError("fit_all_stars_simultaneous fits only (y, x, flux) per star; got free parameters $(off)"))
     mask = Int32((:y in free ? 1 : 0) | (:x in free ? 2 : 0) | (:flux in free ? 4 : 0))
-    y0 = Float64.(sources.y); x0 = Float64.(sources.x)
-    f0 = haskey(sources, :flux) ? Float64.(sources.flux) : ones(length(y0))
+    cat = source_catalog(sources)
+    y0, x0, f0 = cat.y, cat.x, cat.flux
     n = length(y0)
     ny, nx = size(image)
     img = Matrix{Float32}(image)                       # column-major already
     iv = inv_var === nothing ? nothing : Matrix{Float32}(inv_var)
     keep = Any[]
     desc = psf_desc(psf, fixed, keep)
-    o = CpSimOpts(0, inner_iterations, linear_tol, model_rad === :auto ? 0.0 : Float64(model_rad),
+    o = CpSimOpts(code, inner_iterations, linear_tol, model_rad === :auto ? 0.0 : Float64(model_rad),
                   model_rad_max, model_rad_nsigma, max_step, max_trials, max_iter, x_tol, f_tol, g_tol,
                   λ_init, λ_up, λ_down, λ_min, λ_max, 0, mask,
                   Float64(get(fixed, :y, 0)), Float64(get(fixed, :x, 0)), Float64(get(fixed, :flux, 0)),
@@ -216,28 +268,45 @@ function fit_all_stars_simultaneous(image::AbstractMatrix, psf, sources, fit_rad
            qfit = zeros(n), qfit_expected = zeros(n), qfit_z = zeros(n), crowding = zeros(n),
            spread_model = zeros(n), spread_model_err = zeros(n),
            residual = zeros(Float32, ny, nx))
+    trace = zeros(8 * max(1, show_trace ? max_iter * max_trials : 1))
     r = CpSimResult(pointer(out.y), pointer(out.x), pointer(out.flux), pointer(out.bkg), pointer(out.y_err),
                     pointer(out.x_err), pointer(out.flux_err), pointer(out.bkg_err), pointer(out.chisq),
                     pointer(out.converged), pointer(out.valid), pointer(out.n_iter), pointer(out.active),
-                    pointer(out.residual), 0, 0, 0, 0, 0.0, 0, 0, C_NULL, 0, 0,
+                    pointer(out.residual), 0, 0, 0, 0, 0.0, 0, 0,
+                    show_trace ? pointer(trace) : C_NULL, show_trace ? Int32(length(trace) ÷ 8) : Int32(0), 0,
                     pointer(out.qfit), pointer(out.qfit_expected), pointer(out.qfit_z), pointer(out.crowding),
                     pointer(out.spread_model), pointer(out.spread_model_err))
-    GC.@preserve keep out img iv begin
+    GC.@preserve keep out img iv trace begin
         rc = ccall(sym(:cp_fit_simultaneous), Cint,
                    (Ptr{Cvoid}, Ref{CpPsfDesc}, Ptr{Float32}, Ptr{Float32}, Int32, Int32, Ptr{Float64}, Ptr{Float64},
                     Ptr{Float64}, Int32, Float64, Ref{CpSimOpts}, Ref{CpSimResult}),
                    ctx.handle, desc, img, iv === nothing ? C_NULL : iv, ny, nx, y0, x0, f0, n, fit_rad, o, r)
         check(ctx, rc)
     end
+    if show_trace
+        # the reference's per-trial line (psf_photometry_simultaneous.jl:1238-1246)
+        na = max(Int(r.n_active), 1)
+        for k in 1:Int(r.trace_len)
+            t = view(trace, 8k - 7:8k)
+            conv = floor(100 * (1 - t[6] / na) * 100) / 100
+            status = t[4] < t[3] ? "accepted" : "rejected"
+            println("Iter ", lpad(Int(t[1]), 4), " trial ", lpad(Int(t[2]), 2), " | converged = ",
+                    lpad(string(round(conv; digits = 2)), 5), "% | cost = ", t[3], " -> ", t[4],
+                    " | λ = ", t[5], " | ||g|| = ", round(t[8]; digits = 4), " | ", status)
+            Int(t[2]) == max_trials && status == "rejected" &&
+                println("*** Max damping trials (`max_trials`) reached during outer iteration $(Int(t[1])) ***")
+        end
+    end
     msgs = ["star $i: excluded before the solve (no usable pixels or non-finite initial parameters)"
             for i in 1:n if out.active[i] == 0]
-    return (y = out.y, x = out.x, y_err = out.y_err, x_err = out.x_err, flux = out.flux, flux_err = out.flux_err,
-            bkg = out.bkg, bkg_err = out.bkg_err, converged = BitVector(out.converged .!= 0),
-            valid = BitVector(out.valid .!= 0), chisq = out.chisq, qfit = out.qfit, qfit_expected = out.qfit_expected,
-            qfit_z = out.qfit_z, crowding = out.crowding, spread_model = out.spread_model,
-            spread_model_err = out.spread_model_err,
-            n_iter = Int.(out.n_iter), n_passes = Int(r.n_passes), n_failed = Int(r.n_failed),
-            failure_msgs = msgs, residual = out.residual)
+    res = (y = out.y, x = out.x, y_err = out.y_err, x_err = out.x_err, flux = out.flux, flux_err = out.flux_err,
+           bkg = out.bkg, bkg_err = out.bkg_err, converged = BitVector(out.converged .!= 0),
+           valid = BitVector(out.valid .!= 0), chisq = out.chisq, qfit = out.qfit, qfit_expected = out.qfit_expected,
+           qfit_z = out.qfit_z, crowding = out.crowding, spread_model = out.spread_model,
+           spread_model_err = out.spread_model_err,
+           n_iter = Int.(out.n_iter), n_passes = Int(r.n_passes), n_failed = Int(r.n_failed),
+           failure_msgs = msgs, residual = out.residual)
+    return photometry_result(res, float(eltype(image)))
 end
 
 """
@@ -246,8 +315,10 @@ end
 
 Sequential multi-pass photometry (psf_photometry_single.jl:267-469) on the GPU,
 with fit_star's lm_irls keywords (`max_iter`, `x_tol`, `f_tol`, `g_tol`, `λ_*`).
-`sources` is a NamedTuple (y, x[, flux, bkg]).  Returns a NamedTuple with the
-MultiPassPhotResult fields.
+`sources` is a NamedTuple (y, x[, flux, bkg]), a Vector{<:NamedTuple} from
+measure_star_shapes or a MatchedFilterResult.  Returns CrowdPhot's
+`MultiPassPhotResult` when CrowdPhot is loaded, else a NamedTuple with its
+fields.
 """
 function fit_all_stars(image::AbstractMatrix, psf, sources, fit_rad::Real; n_passes::Integer = 3,
         fixed::NamedTuple = (;), inv_var = nothing, spread_model_fwhm::Union{Nothing, Real} = nothing,
@@ -260,12 +331,11 @@ function fit_all_stars(image::AbstractMatrix, psf, sources, fit_rad::Real; n_pas
     isempty(free) && throw(ArgumentError("all model parameters are fixed; nothing to fit"))
     mask = UInt32(sum(k in free ? (1 << (i - 1)) : 0 for (i, k) in enumerate(props)))
     fixv = Float64[haskey(fixed, k) ? Float64(getfield(fixed, k)) : NaN for k in props]
-    n = length(sources.y)
+    cat = source_catalog(sources)
+    n = length(cat.y)
     theta0 = zeros(length(props), n)                       # column per star = row-major n x npf
     for (i, k) in enumerate(props)
-        theta0[i, :] .= k === :y ? Float64.(sources.y) : k === :x ? Float64.(sources.x) :
-                        k === :flux ? (haskey(sources, :flux) ? Float64.(sources.flux) : ones(n)) :
-                        k === :bkg ? (haskey(sources, :bkg) ? Float64.(sources.bkg) : zeros(n)) :
+        theta0[i, :] .= k === :y ? cat.y : k === :x ? cat.x : k === :flux ? cat.flux : k === :bkg ? cat.bkg :
                         fill(Float64(getproperty(psf, k)), n)
     end
     ny, nx = size(image)
@@ -294,13 +364,95 @@ function fit_all_stars(image::AbstractMatrix, psf, sources, fit_rad::Real; n_pas
                    precision === :fp64 ? 1 : 0, r)
         check(ctx, rc)
     end
-    return (y = out.y, x = out.x, y_err = out.y_err, x_err = out.x_err, flux = out.flux, flux_err = out.flux_err,
-            bkg = out.bkg, bkg_err = out.bkg_err, converged = BitVector(out.converged .!= 0),
-            valid = BitVector(out.valid .!= 0), chisq = out.chisq, qfit = out.qfit, qfit_expected = out.qfit_expected,
-            qfit_z = out.qfit_z, crowding = out.crowding, spread_model = out.spread_model,
-            spread_model_err = out.spread_model_err, n_iter = Int.(out.n_iter), n_passes = Int(r.n_passes),
-            n_failed = Int(r.n_failed), failure_msgs = String[],
-            residual = n == 0 ? Matrix{Float32}(undef, 0, 0) : out.residual)
+    res = (y = out.y, x = out.x, y_err = out.y_err, x_err = out.x_err, flux = out.flux, flux_err = out.flux_err,
+           bkg = out.bkg, bkg_err = out.bkg_err, converged = BitVector(out.converged .!= 0),
+           valid = BitVector(out.valid .!= 0), chisq = out.chisq, qfit = out.qfit, qfit_expected = out.qfit_expected,
+           qfit_z = out.qfit_z, crowding = out.crowding, spread_model = out.spread_model,
+           spread_model_err = out.spread_model_err, n_iter = Int.(out.n_iter), n_passes = Int(r.n_passes),
+           n_failed = Int(r.n_failed), failure_msgs = String[],
+           residual = n == 0 ? Matrix{Float32}(undef, 0, 0) : out.residual)
+    return photometry_result(res, float(eltype(image)))
+end
+
+mutable struct CpLMStat
+    minimum::Float64; cost_init::Float64; lambda_final::Float64; chisq::Float64
+    iterations::Int32
+    converged::UInt8; x_converged::UInt8; f_converged::UInt8; g_converged::UInt8
+    status::Int32
+end
+CpLMStat() = CpLMStat(0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+
+"""
+    fit_star(model, image, inds = axes(image); fixed = (;), inv_var = nothing, kws...) -> (best_model, LMResult)
+
+Per-star Levenberg-Marquardt fit (psf_fitting.jl:229-308, lm_irls
+levenberg_marquardt.jl:431-680) on the GPU over the pixels `inds` of `image`,
+in the image's own coordinates; `fixed` values override the model's, and
+`inv_var` (same size as `image`) weights the pixels, zero or non-finite
+values masking them.  Keywords: `λ_init, λ_up, λ_down, λ_min, λ_max,
+max_iter, x_tol, f_tol, g_tol`.  `best_model` is the model with the fitted
+parameters (ConstructionBase.setproperties); `LMResult` is CrowdPhot's when it
+is loaded, else a NamedTuple with its fields.  IRLS reweighting
+(`reweight`) is not built on the GPU.
+"""
+function fit_star(model, image::AbstractMatrix, inds = axes(image); fixed::NamedTuple = (;), inv_var = nothing,
+        λ_init::Real = 1.0e-4, λ_up::Real = 10.0, λ_down::Real = 10.0, λ_min::Real = 1.0e-12, λ_max::Real = 1.0e12,
+        max_iter::Integer = 200, x_tol::Real = 1.0e-8, f_tol::Real = 1.0e-4, g_tol::Real = 1.0e-4,
+        reweight = nothing, ctx::Context = Context(0), kws...)
+    inv_var === nothing || size(inv_var) == size(image) ||
+        throw(ArgumentError("`inv_var` must be the same size as `image`"))
+    reweight === nothing || throw(ArgumentError("IRLS reweighting is not built on the GPU"))
+    props = propertynames_fit(model)
+    free = Tuple(k for k in props if !haskey(fixed, k))
+    isempty(free) && throw(ArgumentError("all model parameters are fixed; nothing to fit"))
+    mask = UInt32(sum(k in free ? (1 << (i - 1)) : 0 for (i, k) in enumerate(props)))
+    rows, cols = inds isa CartesianIndices ? inds.indices : inds
+    r0, c0 = first(rows) - 1, first(cols) - 1
+    cut = Matrix{Float32}(image[rows, cols])               # column-major cutout
+    ivc = inv_var === nothing ? nothing : Matrix{Float32}(map(v -> isfinite(v) && v > 0 ? v : zero(v),
+                                                              inv_var[rows, cols]))
+    if ivc !== nothing
+        nv = count(>(0), ivc)
+        nv > length(free) || throw(ArgumentError("too few valid inv_var pixels ($nv) for $(length(free)) free parameters"))
+    end
+    val(k) = haskey(fixed, k) ? Float64(getfield(fixed, k)) : Float64(getproperty(model, k))
+    theta0 = Float64[val(k) - (k === :y ? r0 : k === :x ? c0 : 0.0) for k in props]
+    keep = Any[]
+    desc = psf_desc(model, fixed, keep; shift = (Float64(r0), Float64(c0)))
+    o = CpLMOpts(λ_init, λ_up, λ_down, λ_min, λ_max, max_iter, x_tol, f_tol, g_tol, 0, 1.0e-6, 0)
+    npf, np = length(props), length(free)
+    theta = zeros(npf)
+    cov = zeros(np, np)
+    st = Ref(CpLMStat())
+    off = Int64[0]
+    cny, cnx = Int32[size(cut, 1)], Int32[size(cut, 2)]
+    GC.@preserve keep cut ivc theta0 theta cov off cny cnx begin
+        rc = ccall(sym(:cp_fit_star_batch), Cint,
+                   (Ptr{Cvoid}, Ref{CpPsfDesc}, Ptr{Float32}, Ptr{Int64}, Ptr{Int32}, Ptr{Int32}, Ptr{Float32},
+                    Ptr{Float64}, UInt32, Int32, Ref{CpLMOpts}, Ptr{Float64}, Ptr{Float64}, Ref{CpLMStat}),
+                   ctx.handle, desc, cut, off, cny, cnx, ivc === nothing ? C_NULL : ivc, theta0, mask, 1, o, theta,
+                   cov, st)
+        check(ctx, rc)
+    end
+    s = st[]
+    s.status == 0 || throw(ArgumentError("fit_star rejected the cutout (too few observations for the free parameters)"))
+    theta[1] += r0
+    theta[2] += c0
+    fitted = NamedTuple{props}(Tuple(theta))
+    cb = loaded("ConstructionBase")
+    best = cb === nothing ? fitted : cb.setproperties(model, fitted)
+    FT = float(eltype(image))
+    free_idx = [i for (i, k) in enumerate(props) if !haskey(fixed, k)]
+    mini = Vector{FT}(theta[free_idx])
+    cp = loaded("CrowdPhot")
+    lm = cp === nothing ?
+        (minimizer = mini, minimum = FT(s.minimum), cost_init = FT(s.cost_init), converged = s.converged != 0,
+         x_converged = s.x_converged != 0, f_converged = s.f_converged != 0, g_converged = s.g_converged != 0,
+         iterations = Int(s.iterations), λ_final = FT(s.lambda_final), σ_final = FT(NaN), cov = Matrix{FT}(cov),
+         chisq = FT(s.chisq)) :
+        cp.LMResult(mini, FT(s.minimum), FT(s.cost_init), s.converged != 0, s.x_converged != 0, s.f_converged != 0,
+                    s.g_converged != 0, Int(s.iterations), FT(s.lambda_final), FT(NaN), Matrix{FT}(cov), FT(s.chisq))
+    return best, lm
 end
 
 """

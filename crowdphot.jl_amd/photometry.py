@@ -13,6 +13,7 @@ ArgumentError; device failures raise RuntimeError.  Every computation runs on
 the GPU: there is no CPU path.
 """
 import ctypes as C
+import math
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -283,6 +284,8 @@ class _SimCall:
         self.active = np.zeros(n, np.uint8)
         self.n_iter = np.zeros(n, np.int32)
         self.res = np.zeros(ny * nx, np.float32) if residual else None
+        if show_trace and not trace:
+            trace = int(max_iter) * int(max_trials)
         self.tr = np.zeros(max(int(trace), 1) * 8)
         s = _abi.SimResult()
         for k, v in self.out.items():
@@ -312,9 +315,15 @@ class _SimCall:
                 for i in range(n) if not self.active[i]]
         resid = None if self.res is None else self.res.reshape(self.ny, self.nx)
         if self.show_trace:
+            # the reference's per-trial line (psf_photometry_simultaneous.jl:1238-1246)
+            na = max(int(s.n_active), 1)
             for row in tr[: 8 * s.trace_len].reshape(-1, 8):
-                print(f"Iter {int(row[0]):4d} trial {int(row[1]):2d} | cost = {row[2]:.4e} -> {row[3]:.4e} | "
-                      f"lambda = {row[4]:.2e} | {'accepted' if row[3] < row[2] else 'rejected'}")
+                conv = math.floor(100 * (1 - row[5] / na) * 100) / 100
+                print(f"Iter {int(row[0]):4d} trial {int(row[1]):2d} | converged = {conv:5.2f}% | "
+                      f"cost = {row[2]:.4e} -> {row[3]:.4e} | \u03bb = {row[4]:.2e} | ||g|| = {row[7]:8.4f} | "
+                      f"{'accepted' if row[3] < row[2] else 'rejected'}")
+                if int(row[1]) == self.opts.max_trials and not row[3] < row[2]:
+                    print(f"*** Max damping trials (`max_trials`) reached during outer iteration {int(row[0])} ***")
         out = self.out
         return MultiPassPhotResult(
             y=out["y"], x=out["x"], y_err=out["y_err"], x_err=out["x_err"], flux=out["flux"],
@@ -536,6 +545,14 @@ def fit_star(model, image, inds=None, *, fixed=None, inv_var=None, ctx=None, **k
     th0 = m.params().copy()
     th0[0] -= r0
     th0[1] -= c0
+    if isinstance(m, GriddedPSFModel) and (r0 or c0):
+        # the cutout's coordinates are the image's shifted by (r0, c0): shift
+        # the node grid with them, so the node blend at a star is unchanged
+        # (each ImagePSF node is translation invariant)
+        import copy
+        m = copy.copy(m)
+        m.grid_y = m.grid_y - r0
+        m.grid_x = m.grid_x - c0
     if inv_var is not None:
         nvalid = int(np.sum(np.isfinite(inv_var) & (np.asarray(inv_var) > 0)))
         if nvalid <= len(model.props) - len(fixed):

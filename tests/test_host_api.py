@@ -90,3 +90,47 @@ def test_epsf_grid_normalisation():
     assert abs(e.data.sum() - 16.0) < 1e-9
     k = pkg.pixel_response_kernel(4)
     assert k.shape == (5, 5) and abs(k.sum() - 1) < 1e-15 and abs(k[0, 0] - 1 / 64) < 1e-15
+
+
+def test_source_catalogue_forms():
+    """_extract_source_catalog (psf_photometry_single.jl:152-197): a table, a
+    per-star record list (measure_star_shapes: centroid.y, centroid.x,
+    matched_filter_flux) and a MatchedFilterResult give the same initial
+    (y, x, flux); a record list missing a field is an ArgumentError."""
+    from collections import namedtuple
+    cat = pkg.photometry._catalog
+    y, x, f = np.array([10.5, 20.25]), np.array([11.0, 30.5]), np.array([100.0, 250.0])
+    Cen = namedtuple("Cen", "y x")
+    Rec = namedtuple("Rec", "centroid matched_filter_flux")
+    forms = [
+        {"y": y, "x": x, "flux": f},
+        [Rec(Cen(a, b), c) for a, b, c in zip(y, x, f)],
+        [{"centroid": {"y": a, "x": b}, "matched_filter_flux": c} for a, b, c in zip(y, x, f)],
+        type("MF", (), {"peaks": np.column_stack([y, x]), "peak_fluxes": f})(),
+    ]
+    for s in forms:
+        gy, gx, gf = cat(s)
+        np.testing.assert_array_equal(gy, y)
+        np.testing.assert_array_equal(gx, x)
+        np.testing.assert_array_equal(gf, f)
+    gy, gx, gf = cat({"y": y, "x": x})
+    np.testing.assert_array_equal(gf, np.ones(2))
+    with pytest.raises(pkg.ArgumentError, match="matched_filter_flux"):
+        cat([{"centroid": {"y": 1.0, "x": 2.0}}])
+    # a table that is itself a named tuple is a table, not a record list
+    T = namedtuple("T", "y x flux")
+    gy, _, _ = cat(T(y, x, f))
+    np.testing.assert_array_equal(gy, y)
+
+
+def test_solver_keyword():
+    psf = pkg.CircularGaussianPSF(fwhm=2.0)
+    call = pkg.photometry._SimCall
+    kw = dict(fixed={"fwhm": 2.0, "bkg": 0.0}, model_rad=5)
+    cat = {"y": [32.0], "x": [32.0], "flux": [500.0]}
+    img = np.zeros((64, 64))
+    assert call(img, psf, cat, 5, solver="lsqr", **kw).opts.solver == pkg._abi.CP_SOLVER_LSQR
+    assert call(img, psf, cat, 5, solver="lsmr", **kw).opts.solver == pkg._abi.CP_SOLVER_LSMR
+    assert call(img, psf, cat, 5, solver="cg", **kw).opts.solver == pkg._abi.CP_SOLVER_CGLS
+    c = call(img, psf, cat, 5, show_trace=True, max_iter=7, max_trials=3, **kw)
+    assert c.result.trace_cap == 21
