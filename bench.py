@@ -77,27 +77,28 @@ def workload_bytes(kernel, S2, p, pairs_per_star, trace):
     """Algorithmic HBM bytes of all launches of `kernel` in one fit, from the
     unit costs b_f = b_i = 4 (stamp values, indices) and b_d = 8 (star
     vectors, normal-matrix blocks) over the live stars N of every trial, with
-    N * pairs_per_star pairs (rows of the normal matrix) among them:
-      k_hv     reads the p x p blocks and the pair indices (pairs (p^2 b_d +
-               b_i)) and the star vectors t, J^T u, writes H v (3 p N b_d),
-               plus the row list, row offsets and live flags (N (2 b_i + 1));
-      k_hbuild reads every live star's stamp rows once (p N S^2 b_f) and
-               writes the blocks (pairs (p^2 b_d + b_i));
-      k_gk_update reads five and writes four p-vectors per star (9 p N b_d).
-    k_hv and k_gk_update run once per inner iteration, k_hbuild once per
-    outer iteration (its first trial)."""
+    E = N * pairs_per_star row entries (ordered pairs, each star with itself)
+    among them and (E + N) / 2 distinct p x p blocks:
+      k_gk     reads each distinct block once ((E + N) / 2 p^2 b_d), every row
+               entry's neighbour index, block slot and live flag (E (2 b_i +
+               1)), the row list, offsets and own live flag (N (2 b_i + 1)),
+               the previous step's g and v and the x and w vectors (4 p N b_d)
+               and writes g, v, x and w (4 p N b_d);
+      k_hbuild reads every live star's stamp rows once (p N S^2 b_f) and the
+               row entries (E b_i), and writes the distinct blocks.
+    k_gk runs once per inner (Golub-Kahan) step, k_hbuild once per outer
+    iteration (its first trial)."""
     total = 0.0
     seen = set()
     for r in trace:
         it, n_live, its = int(r[0]), r[5], r[6]
-        pairs = n_live * pairs_per_star
-        if kernel == "k_hv":
-            total += its * (pairs * (p * p * 8 + 4) + n_live * (3 * p * 8 + 9))
-        elif kernel == "k_gk_update":
-            total += its * 9 * p * 8 * n_live
+        E = n_live * pairs_per_star
+        blocks = (E + n_live) / 2 * p * p * 8
+        if kernel == "k_gk":
+            total += its * (blocks + E * 9 + n_live * 9 + 8 * p * 8 * n_live)
         elif kernel == "k_hbuild" and it not in seen:
             seen.add(it)
-            total += p * 4 * S2 * n_live + pairs * (p * p * 8 + 4)
+            total += p * 4 * S2 * n_live + E * 4 + blocks
     return total
 
 
@@ -294,7 +295,7 @@ def main():
     pairs = star_pairs(src["y"], src["x"], R)
     pps = pairs / n
     roof = None
-    cand = {k: v for k, v in kernels.items() if k in ("k_hv", "k_hbuild", "k_gk_update")}
+    cand = {k: v for k, v in kernels.items() if k in ("k_gk", "k_hbuild")}
     if cand:
         # the dominant HBM-bound kernel (the renders and the fill are bound by
         # ePSF-evaluation issue, not HBM)

@@ -17,9 +17,29 @@
 // which is the bidiagonalization u_{k+1} beta_{k+1} = J v_k - alpha_k u_k,
 // v_{k+1} alpha_{k+1} = J^T u_{k+1} - beta_{k+1} v_k with the pixel vector u
 // never formed (|u_k| = 1).  The scalar recurrences and the x / w (LSMR:
-// h, hbar) updates are those of the pixel-space method, unchanged.  Per
-// iteration the traffic is the blocks and a few star vectors instead of two
-// passes over every stamp row.
+// h, hbar) updates are those of the pixel-space method, unchanged.
+//
+// One launch per Golub-Kahan step (k_gk): each row forms the v_k of its
+// neighbours where it gathers them, from the g_{k-1} = H v_{k-1} - alpha_{k-1}
+// J^T u_{k-1} and v_{k-1} the previous step stored,
+//
+//   v_k = (g_{k-1} / beta_k - beta_k v_{k-1}) / alpha_k,
+//
+// and the scalars need only sums over the rows of this step: with
+// g_k = H v_k - alpha_k J^T u_k,
+//
+//   alpha_{k+1}^2 = |g_k|^2 / beta_{k+1}^2 - 2 g_k . v_k + beta_{k+1}^2 |v_k|^2,
+//   g_k . v_k = v_k . H v_k - alpha_k v_k . J^T u_k,
+//
+// so one block-order reduction (k_finish_gk) per step gives beta_{k+1},
+// alpha_{k+1} and the LSQR / LSMR rotations.  The expansion of alpha_{k+1}^2
+// loses about eps (alpha^2 + beta^2) / alpha^2 relative, far below the
+// tolerances the solve runs to.
+//
+// Storage: the blocks of one outer iteration live in the slots of the pair
+// rows; the block of a pair (a, b) is built once, in the slot of the row of
+// min(a, b), and the other row reads it transposed (pblk < 0 names the slot).
+// Star vectors are float64 at stride p.
 //
 // Neighbour lists: stars are binned into cells of side 2R (anchors shifted by
 // R - 1 so every active anchor is inside the grid), so the stars whose stamps
@@ -31,7 +51,13 @@
 
 namespace cp {
 
-constexpr int HB = 12;   // doubles per block: p <= 3 rows of 4 (the 4th column 0)
+// Calls f with the number of free parameters per star (1..3) as a constant.
+template <typename F>
+static inline void with_p(int p, F&& f) {
+    if (p == 1) f(std::integral_constant<int, 1>{});
+    else if (p == 2) f(std::integral_constant<int, 2>{});
+    else f(std::integral_constant<int, 3>{});
+}
 
 struct CellGeo {
     int CS, ncy, ncx, R;
@@ -76,17 +102,42 @@ static __global__ void k_pairs(int na, const int* __restrict__ ay, const int* __
     if (!EMIT) cnt[a] = o;
 }
 
-// H_ab for every pair of the live rows in lst (Sc->n_jtu_stars of them):
-// GL lanes per row, its pairs in turn, lanes striding over the pixels the two
-// stamps share; the nine sums are group reductions in float64.  Blocks of a
-// frozen neighbour are written as zeros (its vectors are not read).
-template <typename T, int GL>
+// The block slot of every row entry: its own (b >= a) or -(j' + 1) for the
+// entry j' of row b that lists a (b < a; the overlap test is symmetric, so it
+// exists).  Sets *bad if one is missing.
+static __global__ void k_pair_slots(int na, const int* __restrict__ rowptr, const int* __restrict__ pb,
+                                    int* __restrict__ pblk, int* __restrict__ bad) {
+    const int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    for (int j = rowptr[a]; j < rowptr[a + 1]; ++j) {
+        const int b = pb[j];
+        if (b >= a) {
+            pblk[j] = j;
+            continue;
+        }
+        int t = -1;
+        for (int i = rowptr[b]; i < rowptr[b + 1]; ++i)
+            if (pb[i] == a) {
+                t = i;
+                break;
+            }
+        if (t < 0) *bad = 1;
+        pblk[j] = t < 0 ? j : -(t + 1);
+    }
+}
+
+// H_ab (P x P, row-major: row k is star a's parameter k) for every pair of
+// live stars (a <= b) of the rows in lst (Sc->n_jtu_stars of them): GL lanes
+// per row, its pairs in turn, lanes striding over the pixels the two stamps
+// share; the P^2 sums are group reductions in float64.  Blocks with a frozen
+// star are not built (k_gk never reads them).
+template <typename T, int GL, int P>
 __global__ __launch_bounds__(BLOCK) void k_hbuild(const int* __restrict__ lst, const uint8_t* __restrict__ live,
                                                   const int* __restrict__ rowptr, const int* __restrict__ pb,
                                                   const int* __restrict__ ay, const int* __restrict__ ax, int R,
                                                   int S, const T* __restrict__ V, double* __restrict__ H,
                                                   const Scal* Sc) {
-    constexpr int GPB = BLOCK / GL;
+    constexpr int GPB = BLOCK / GL, PP = P * P;
     const int sub = threadIdx.x % GL;
     const int S2 = S * S;
     const int nl = Sc->n_jtu_stars;
@@ -97,217 +148,301 @@ __global__ __launch_bounds__(BLOCK) void k_hbuild(const int* __restrict__ lst, c
         const int ya = ay[a], xa = ax[a];
         for (int j = rowptr[a]; j < rowptr[a + 1]; ++j) {
             const int b = pb[j];
-            double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            if (live[b]) {
-                const int dy0 = ay[b] - ya, dx0 = ax[b] - xa;
-                const int ylo = max(-R, dy0 - R), yhi = min(R, dy0 + R);
-                const int xlo = max(-R, dx0 - R), xhi = min(R, dx0 + R);
-                const int hy = yhi - ylo + 1, nov = hy * (xhi - xlo + 1);
-                for (int i = sub; i < nov; i += GL) {
-                    const int dy = ylo + i % hy, dx = xlo + i / hy;
-                    const vec4_t<T> va = V4[(size_t)a * S2 + (dx + R) * S + (dy + R)];
-                    const vec4_t<T> vb = V4[(size_t)b * S2 + (dx - dx0 + R) * S + (dy - dy0 + R)];
-                    const double A[3] = {(double)va.x, (double)va.y, (double)va.z};
-                    const double B[3] = {(double)vb.x, (double)vb.y, (double)vb.z};
+            if (b < a || !live[b]) continue;
+            double h[PP];
 #pragma unroll
-                    for (int k = 0; k < 3; ++k)
+            for (int e = 0; e < PP; ++e) h[e] = 0;
+            const int dy0 = ay[b] - ya, dx0 = ax[b] - xa;
+            const int ylo = max(-R, dy0 - R), yhi = min(R, dy0 + R);
+            const int xlo = max(-R, dx0 - R), xhi = min(R, dx0 + R);
+            const int hy = yhi - ylo + 1, nov = hy * (xhi - xlo + 1);
+            for (int i = sub; i < nov; i += GL) {
+                const int dy = ylo + i % hy, dx = xlo + i / hy;
+                const vec4_t<T> va = V4[(size_t)a * S2 + (dx + R) * S + (dy + R)];
+                const vec4_t<T> vb = V4[(size_t)b * S2 + (dx - dx0 + R) * S + (dy - dy0 + R)];
+                const double A[3] = {(double)va.x, (double)va.y, (double)va.z};
+                const double B[3] = {(double)vb.x, (double)vb.y, (double)vb.z};
 #pragma unroll
-                        for (int l = 0; l < 3; ++l) h[3 * k + l] += A[k] * B[l];
-                }
+                for (int k = 0; k < P; ++k)
 #pragma unroll
-                for (int e = 0; e < 9; ++e) h[e] = group_sum<GL>(h[e]);
+                    for (int l = 0; l < P; ++l) h[P * k + l] += A[k] * B[l];
             }
-            // lane e < 12 writes slot e of the block (row k = e / 4, column e % 4)
-            if (sub < HB) {
-                const int k = sub >> 2, l = sub & 3;
+#pragma unroll
+            for (int e = 0; e < PP; ++e) h[e] = group_sum<GL>(h[e]);
+            // lane e < P^2 writes entry e of the block
+            if (sub < PP) {
                 double v = 0;
 #pragma unroll
-                for (int e = 0; e < 9; ++e)
-                    if (e == 3 * k + l && l < 3) v = h[e];
-                H[(size_t)j * HB + sub] = v;
+                for (int e = 0; e < PP; ++e)
+                    if (e == sub) v = h[e];
+                H[(size_t)j * PP + sub] = v;
             }
         }
     }
 }
 
-__device__ __forceinline__ double4 ld4(const double* p) { return *reinterpret_cast<const double4*>(p); }
+// Block sums of N values in a fixed order (wave shuffles, then thread k sums
+// the waves of value k); value k of this block goes to part[k * gridDim.x +
+// blockIdx.x] with a plain store, read by the one-block finish launch after the
+// kernel boundary.
+template <int N>
+__device__ __forceinline__ void block_store_n(double (&v)[N], double* __restrict__ part, double (*sh)[NWAVE]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) sh[k][wv] = v[k];
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double r = 0;
+        for (int i = 0; i < NWAVE; ++i) r += sh[threadIdx.x][i];
+        part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
+    }
+}
 
-// y_a = sum_b H_ab v_b for the rows in lst, one thread per row, with the
-// partial sums the next scalar step needs.
-//  * LSQR / LSMR: v = t / alpha (the first call: t_1 = -b_scaled / beta1);
-//    partials v . y and v . J^T u_k.
-//  * CG: the direction p = s + beta p_old of each row is formed where it is
-//    gathered (the first call: p_0 = s_0 = -b_scaled) and each row writes its
-//    own to pnew; partials p . H p and |p|^2.
-// Neighbours that are not live contribute nothing (their vectors may be stale).
-template <int SOLVER>
-__global__ __launch_bounds__(BLOCK) void k_hv(const int* __restrict__ lst, const uint8_t* __restrict__ live,
+// y += H_ab v_b over the live neighbours b of row a, with v_b = c1 s1[b] +
+// c2 s2[b] (s2 read only when TWO).
+template <int P, bool TWO>
+__device__ __forceinline__ void row_apply(int a, const uint8_t* __restrict__ live, const int* __restrict__ rowptr,
+                                          const int* __restrict__ pb, const int* __restrict__ pblk,
+                                          const double* __restrict__ H, const double* __restrict__ s1, int st1,
+                                          double c1, const double* __restrict__ s2, double c2, double (&y)[P]) {
+    constexpr int PP = P * P;
+    const int j1 = rowptr[a + 1];
+    for (int j = rowptr[a]; j < j1; ++j) {
+        const int b = pb[j];
+        const int sl = pblk[j];
+        if (!live[b]) continue;
+        double v[P];
+#pragma unroll
+        for (int l = 0; l < P; ++l) {
+            v[l] = c1 * s1[(size_t)b * st1 + l];
+            if (TWO) v[l] += c2 * s2[(size_t)b * P + l];
+        }
+        const double* h = H + (size_t)(sl >= 0 ? sl : -sl - 1) * PP;
+        double hv[PP];
+#pragma unroll
+        for (int e = 0; e < PP; ++e) hv[e] = h[e];
+        if (sl >= 0) {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+#pragma unroll
+                for (int l = 0; l < P; ++l) y[k] += hv[P * k + l] * v[l];
+        } else {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+#pragma unroll
+                for (int l = 0; l < P; ++l) y[k] += hv[P * l + k] * v[l];
+        }
+    }
+}
+
+// One Golub-Kahan step k of LSQR / LSMR for the rows in lst (one thread per
+// row; the first step forms v_1 = -b_scaled / (beta_1 alpha_1) and J^T u_1 =
+// -b_scaled / beta_1):
+//   v_k and J^T u_k from (g_{k-1}, v_{k-1}) in gp / vp, y = H v_k,
+//   g_k = y - alpha_k J^T u_k and v_k to gn / vn, and the pending update of
+//   step k - 1 (hbar <- w - del hbar, x <- x + sig hbar, w <- v_k - tau w;
+//   LSQR has del = 0).
+// Partials: v.y, v.J^T u, |g|^2, |v|^2, |w|^2 and, for LSMR, |x|^2, x.w,
+// x.hbar, w.hbar, |hbar|^2 of the updated vectors.
+template <int P, int SOLVER>
+__global__ __launch_bounds__(BLOCK) void k_gk(const int* __restrict__ lst, const uint8_t* __restrict__ live,
                                               const int* __restrict__ rowptr, const int* __restrict__ pb,
-                                              const double* __restrict__ H, const double* __restrict__ b_scaled,
-                                              const double* __restrict__ tbuf, const double* __restrict__ ju,
-                                              const double* __restrict__ pold, double* __restrict__ pnew,
-                                              double* __restrict__ y, int first, double* __restrict__ partials,
-                                              double* __restrict__ partials2, Scal* Sc) {
+                                              const int* __restrict__ pblk, const double* __restrict__ H,
+                                              const double* __restrict__ b_scaled, const double* __restrict__ gp,
+                                              const double* __restrict__ vp, double* __restrict__ gn,
+                                              double* __restrict__ vn, double* __restrict__ x,
+                                              double* __restrict__ wv, double* __restrict__ hb, int first,
+                                              double* __restrict__ part, Scal* Sc) {
+    constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
+    constexpr int NP = LSMR ? 10 : 5;
     if (Sc->done) return;
-    __shared__ double sred[NWAVE];
-    constexpr bool CG = SOLVER == CP_SOLVER_CGLS;
+    __shared__ double sred[NP][NWAVE];
     const double alpha = Sc->alpha;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
-    const double ib1 = (!CG && first) ? 1.0 / Sc->beta1 : 0.0;
-    const double fac = CG ? 1.0 : (first ? -ib1 * ia : ia);
-    const double cg_b = CG && !first ? Sc->cg_beta : 0.0;
-    const double* src = first ? b_scaled : tbuf;
-    const double sgn = (CG && first) ? -1.0 : 1.0;
-    const int nl = Sc->n_jtu_stars;
-    double s1 = 0, s2 = 0;
-    for (int li = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x; li < nl; li += gridDim.x * BLOCK) {
-        const int a = lst[li];
-        double4 acc = make_double4(0, 0, 0, 0);
-        if (live[a]) {
-            for (int j = rowptr[a]; j < rowptr[a + 1]; ++j) {
-                const int b = pb[j];
-                if (!live[b]) continue;
-                double4 v = ld4(src + (size_t)b * VP);
-                if (CG) {
-                    if (first) {
-                        v.x = -v.x; v.y = -v.y; v.z = -v.z;
-                    } else {
-                        const double4 po = ld4(pold + (size_t)b * VP);
-                        v.x += cg_b * po.x; v.y += cg_b * po.y; v.z += cg_b * po.z;
-                    }
-                } else {
-                    v.x *= fac; v.y *= fac; v.z *= fac;
-                }
-                const double* h = H + (size_t)j * HB;
-                const double4 h0 = ld4(h), h1 = ld4(h + 4), h2 = ld4(h + 8);
-                acc.x += h0.x * v.x + h0.y * v.y + h0.z * v.z;
-                acc.y += h1.x * v.x + h1.y * v.y + h1.z * v.z;
-                acc.z += h2.x * v.x + h2.y * v.y + h2.z * v.z;
-            }
-            double4 va = ld4(src + (size_t)a * VP);
-            if (CG) {
-                if (first) {
-                    va.x = -va.x; va.y = -va.y; va.z = -va.z;
-                } else {
-                    const double4 po = ld4(pold + (size_t)a * VP);
-                    va.x += cg_b * po.x; va.y += cg_b * po.y; va.z += cg_b * po.z;
-                }
-                *reinterpret_cast<double4*>(pnew + (size_t)a * VP) = make_double4(va.x, va.y, va.z, 0.0);
-                s1 += va.x * acc.x + va.y * acc.y + va.z * acc.z;
-                s2 += va.x * va.x + va.y * va.y + va.z * va.z;
-            } else {
-                va.x *= fac; va.y *= fac; va.z *= fac;
-                double4 wk;
-                if (first) {
-                    wk = ld4(b_scaled + (size_t)a * VP);
-                    wk.x *= -ib1; wk.y *= -ib1; wk.z *= -ib1;
-                } else {
-                    wk = ld4(ju + (size_t)a * VP);
-                }
-                s1 += va.x * acc.x + va.y * acc.y + va.z * acc.z;
-                s2 += va.x * wk.x + va.y * wk.y + va.z * wk.z;
-            }
-        } else if (CG) {
-            *reinterpret_cast<double4*>(pnew + (size_t)a * VP) = make_double4(0, 0, 0, 0);
-        }
-        (void)sgn;
-        *reinterpret_cast<double4*>(y + (size_t)a * VP) = acc;
-    }
-    block_store(s1, partials, sred);
-    block_store(s2, partials2, sred);
-}
-
-// Per star of lst (one thread each), after the scalars of this step:
-//  * LSQR / LSMR: g = y - alpha J^T u_k is beta J^T u_{k+1}; J^T u_{k+1} =
-//    g / beta and t <- J^T u_{k+1} - beta v, with the pending update of the
-//    previous iteration folded in (hbar <- w - del hbar, x <- x + sig hbar,
-//    w <- v - tau w; LSQR is del = 0).  Partials |t|^2, |w|^2 and, for LSMR,
-//    |x|^2, x.w, x.hbar, w.hbar, |hbar|^2.
-//  * CG: x <- x + a p, s <- s - a (H p + lam^2 p).  Partials |s|^2 and s . p.
-template <int SOLVER>
-__global__ __launch_bounds__(BLOCK) void k_gk_update(
-    const int* __restrict__ lst, const uint8_t* __restrict__ live, int p, const double* __restrict__ y,
-    const double* __restrict__ b_scaled, double* __restrict__ tbuf, double* __restrict__ ju,
-    const double* __restrict__ pcur, double* __restrict__ x, double* __restrict__ wv, double* __restrict__ hb,
-    int first, double* __restrict__ partials, double* __restrict__ partials2, double* __restrict__ partials3,
-    Scal* Sc) {
-    if (Sc->done) return;
-    __shared__ double sred[NWAVE];
-    constexpr bool CG = SOLVER == CP_SOLVER_CGLS, LSMR = SOLVER == CP_SOLVER_LSMR;
-    const double alpha = Sc->alpha, beta = Sc->beta_next;
-    const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
+    const double beta = first ? 0.0 : Sc->beta_cur;
     const double ib = beta != 0 ? 1.0 / beta : 1.0;
+    const double ib1 = first ? 1.0 / Sc->beta1 : 0.0;
+    // v_k = c1 s1 + c2 vp, J^T u_k = d1 s1
+    const double c1 = first ? -ib1 * ia : ib * ia, c2 = first ? 0.0 : -beta * ia, d1 = first ? -ib1 : ib;
+    const double* s1 = first ? b_scaled : gp;
+    const int st1 = first ? VP : P;
     const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau, del = first ? 0.0 : Sc->del;
-    const double ib1 = (!CG && first) ? 1.0 / Sc->beta1 : 0.0;
-    const double cg_a = CG ? Sc->cg_alpha : 0.0;
-    const double lam2 = CG ? Sc->lam * Sc->lam : 0.0;
-    double tsq = 0, wsq = 0, xx = 0, xw = 0, xh = 0, wh = 0, hh = 0;
+    double acc[NP];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) acc[e] = 0;
     const int nl = Sc->n_jtu_stars;
     for (int li = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x; li < nl; li += gridDim.x * BLOCK) {
         const int a = lst[li];
-        const size_t base = (size_t)a * p, b4 = (size_t)a * VP;
+        const size_t base = (size_t)a * P;
         if (!live[a]) {
             if (first)
-                for (int k = 0; k < p; ++k) {
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
                     x[base + k] = 0;
                     wv[base + k] = 0;
                     if (LSMR) hb[base + k] = 0;
-                    tbuf[b4 + k] = 0;
-                    ju[b4 + k] = 0;
                 }
             continue;
         }
-        for (int k = 0; k < p; ++k) {
-            const double yk = y[b4 + k];
-            if (CG) {
-                const double pk = pcur[b4 + k];
-                const double zk = yk + lam2 * pk;
-                x[base + k] = (first ? 0.0 : x[base + k]) + cg_a * pk;
-                const double sk = first ? -b_scaled[b4 + k] : tbuf[b4 + k];
-                const double sn = sk - cg_a * zk;
-                tbuf[b4 + k] = sn;
-                tsq += sn * sn;
-                wsq += sn * pk;
-            } else {
-                const double told = first ? -b_scaled[b4 + k] * ib1 : tbuf[b4 + k];
-                const double v = told * ia;
-                const double wk = first ? told : ju[b4 + k];   // J^T u_k (u_1 = -r / beta1)
-                const double g = yk - alpha * wk;
-                const double xo = first ? 0.0 : x[base + k];
-                const double wo = first ? 0.0 : wv[base + k];
-                const double ho = (LSMR && !first) ? hb[base + k] : 0.0;
-                const double hn = wo - del * ho;
-                const double xn = xo + sig * hn;
-                x[base + k] = xn;
-                const double wn = v - tau * wo;
-                wv[base + k] = wn;
-                if (LSMR) {
-                    hb[base + k] = hn;
-                    xx += xn * xn;
-                    xw += xn * wn;
-                    xh += xn * hn;
-                    wh += wn * hn;
-                    hh += hn * hn;
-                }
-                wsq += wn * wn;
-                const double jn = beta != 0 ? g * ib : wk;
-                ju[b4 + k] = jn;
-                const double tn = beta != 0 ? jn - beta * v : told;
-                tbuf[b4 + k] = tn;
-                tsq += tn * tn;
+        double y[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) y[k] = 0;
+        if (first) row_apply<P, false>(a, live, rowptr, pb, pblk, H, s1, st1, c1, vp, c2, y);
+        else row_apply<P, true>(a, live, rowptr, pb, pblk, H, s1, st1, c1, vp, c2, y);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const double sk = s1[(size_t)a * st1 + k];
+            const double vk = c1 * sk + (first ? 0.0 : c2 * vp[base + k]);
+            const double jk = d1 * sk;
+            const double gk = y[k] - alpha * jk;
+            gn[base + k] = gk;
+            vn[base + k] = vk;
+            acc[0] += vk * y[k];
+            acc[1] += vk * jk;
+            acc[2] += gk * gk;
+            acc[3] += vk * vk;
+            const double xo = first ? 0.0 : x[base + k];
+            const double wo = first ? 0.0 : wv[base + k];
+            const double ho = (LSMR && !first) ? hb[base + k] : 0.0;
+            const double hn = wo - del * ho;
+            const double xn = xo + sig * hn;
+            const double wn = vk - tau * wo;
+            x[base + k] = xn;
+            wv[base + k] = wn;
+            acc[4] += wn * wn;
+            if (LSMR) {
+                hb[base + k] = hn;
+                acc[5] += xn * xn;
+                acc[6] += xn * wn;
+                acc[7] += xn * hn;
+                acc[8] += wn * hn;
+                acc[9] += hn * hn;
             }
         }
     }
-    block_store(tsq, partials, sred);
-    block_store(wsq, partials2, sred);
-    if (LSMR) {
-        const int nb = gridDim.x;
-        block_store(xx, partials3, sred);
-        block_store(xw, partials3 + nb, sred);
-        block_store(xh, partials3 + 2 * nb, sred);
-        block_store(wh, partials3 + 3 * nb, sred);
-        block_store(hh, partials3 + 4 * nb, sred);
+    block_store_n<NP>(acc, part, sred);
+}
+
+// One-block sum of k_gk's partials (block order) and the step's scalars:
+// beta_{k+1}, alpha_{k+1} by the expansion above, then the LSQR or LSMR
+// recurrences of step it.
+template <int SOLVER>
+static __global__ __launch_bounds__(FIN_BLOCK) void k_finish_gk(Scal* Sc, const double* __restrict__ part, int nb,
+                                                                int it) {
+    constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
+    constexpr int NP = LSMR ? 10 : 5;
+    if (Sc->done) return;
+    __shared__ double sh[NP][FIN_BLOCK / 64];
+    double a[NP];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) a[e] = 0;
+    for (int i = threadIdx.x; i < nb; i += FIN_BLOCK)
+#pragma unroll
+        for (int e = 0; e < NP; ++e) a[e] += part[(size_t)e * nb + i];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) a[e] = wave_sum(a[e]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int e = 0; e < NP; ++e) sh[e][threadIdx.x >> 6] = a[e];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double t[NP];
+    for (int e = 0; e < NP; ++e) {
+        t[e] = 0;
+        for (int i = 0; i < FIN_BLOCK / 64; ++i) t[e] += sh[e][i];
     }
+    const double al = Sc->alpha;
+    const double beta = sqrt(fmax(0.0, t[0] - 2 * al * t[1] + al * al));
+    Sc->beta_next = beta;
+    const double gv = t[0] - al * t[1];
+    const double tt = beta > 0 ? fmax(0.0, t[2] / (beta * beta) - 2 * gv + beta * beta * t[3]) : 0.0;
+    if (LSMR) {
+        // |x|^2, x.h, x.hbar, |h|^2, h.hbar, |hbar|^2
+        const double q[6] = {t[5], t[6], t[7], t[4], t[8], t[9]};
+        lsmr_advance(Sc, tt, q, it);
+    } else {
+        lsqr_advance(Sc, tt, t[4], it);
+    }
+}
+
+// CG on the damped normal equations (J'J + lam^2 I) x = -b_scaled, the
+// direction half: q = H p for the rows in lst with p = s + beta p_old formed
+// where it is gathered (the first step: p_0 = s_0 = -b_scaled); each row
+// writes its own p to pn and q.  Partials p.q and |p|^2.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_cg_hv(const int* __restrict__ lst, const uint8_t* __restrict__ live,
+                                                 const int* __restrict__ rowptr, const int* __restrict__ pb,
+                                                 const int* __restrict__ pblk, const double* __restrict__ H,
+                                                 const double* __restrict__ b_scaled, const double* __restrict__ s,
+                                                 const double* __restrict__ po, double* __restrict__ pn,
+                                                 double* __restrict__ q, int first, double* __restrict__ part,
+                                                 Scal* Sc) {
+    if (Sc->done) return;
+    __shared__ double sred[2][NWAVE];
+    const double cb = first ? 0.0 : Sc->cg_beta;
+    const double* s1 = first ? b_scaled : s;
+    const int st1 = first ? VP : P;
+    const double c1 = first ? -1.0 : 1.0;
+    double acc[2] = {0, 0};
+    const int nl = Sc->n_jtu_stars;
+    for (int li = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x; li < nl; li += gridDim.x * BLOCK) {
+        const int a = lst[li];
+        const size_t base = (size_t)a * P;
+        if (!live[a]) continue;
+        double y[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) y[k] = 0;
+        if (first) row_apply<P, false>(a, live, rowptr, pb, pblk, H, s1, st1, c1, po, cb, y);
+        else row_apply<P, true>(a, live, rowptr, pb, pblk, H, s1, st1, c1, po, cb, y);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const double pk = c1 * s1[(size_t)a * st1 + k] + (first ? 0.0 : cb * po[base + k]);
+            pn[base + k] = pk;
+            q[base + k] = y[k];
+            acc[0] += pk * y[k];
+            acc[1] += pk * pk;
+        }
+    }
+    block_store_n<2>(acc, part, sred);
+}
+
+// CG, the update half (one thread per star of lst): x <- x + a p,
+// s <- s - a (q + lam^2 p) (s_0 = -b_scaled).  Partials |s|^2 and s.p.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_cg_update(const int* __restrict__ lst, const uint8_t* __restrict__ live,
+                                                     const double* __restrict__ b_scaled,
+                                                     const double* __restrict__ pc, const double* __restrict__ q,
+                                                     double* __restrict__ s, double* __restrict__ x, int first,
+                                                     double* __restrict__ part, Scal* Sc) {
+    if (Sc->done) return;
+    __shared__ double sred[2][NWAVE];
+    const double ca = Sc->cg_alpha, lam2 = Sc->lam * Sc->lam;
+    double acc[2] = {0, 0};
+    const int nl = Sc->n_jtu_stars;
+    for (int li = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x; li < nl; li += gridDim.x * BLOCK) {
+        const int a = lst[li];
+        const size_t base = (size_t)a * P;
+        if (!live[a]) {
+            if (first)
+#pragma unroll
+                for (int k = 0; k < P; ++k) x[base + k] = 0;
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const double pk = pc[base + k];
+            x[base + k] = (first ? 0.0 : x[base + k]) + ca * pk;
+            const double sk = first ? -b_scaled[(size_t)a * VP + k] : s[base + k];
+            const double sn = sk - ca * (q[base + k] + lam2 * pk);
+            s[base + k] = sn;
+            acc[0] += sn * sn;
+            acc[1] += sn * pk;
+        }
+    }
+    block_store_n<2>(acc, part, sred);
 }
 
 }  // namespace cp

@@ -72,14 +72,17 @@ extern "C" uint64_t cp_request_bytes(int32_t ny, int32_t nx, int32_t n_stars, do
     const double P = (double)ny * nx;
     const double R = std::ceil(fit_rad), S2 = (2 * R + 1) * (2 * R + 1);
     const double mR = model_rad > 0 ? std::ceil(std::max(model_rad, fit_rad)) : 10.0 * R;
-    const double n = n_stars, E = n * S2, U = std::min(P, E);
+    const double n = n_stars, E = n * S2;
     double b = P * (6 * T + 1 + 4);                           // data, data_work, w, 2 residuals, u; union mask;
                                                               // float32 residual staging
     b += P * 8;                                               // row-major inputs transposed on the device
     if (!device_inputs) b += P * 8 + n * 24;                 // staged image, weights, catalogue
-    b += E * (4 * T + T);                                     // stamp Jacobian rows, u at the stamp rows
-    b += E * 12 + U * 4;                                      // pixel-major stamp structure (pix, ptr, ent)
-    b += E * 12 + U * 20;                                     // its live compaction (flags, scans) and u
+    b += E * 4 * T;                                           // stamp Jacobian rows
+    // star-pair rows of the normal matrix (neighbour, block slot and a 3 x 3
+    // float64 block per pair): pairs per star at eight times the mean density
+    // of anchors within 2R, and at least 0.3 S^2
+    const double K = std::min(n, std::max(0.3 * S2, 1 + 8 * n * (4 * R + 1) * (4 * R + 1) / P));
+    b += n * K * 80;
     const double tiles = std::pow((2 * mR + 1) / 64.0 + 1.0, 2);
     b += n * tiles * 16 + 4 * P / 4096 * 24;                  // model-box tile lists, live copies, scans
     b += n * 1100 + (8 << 20);                                // per-star records and vectors, partials, results

@@ -5,12 +5,13 @@
 //  * image tiles of TY x TX = 1024 px each own two star lists (stars whose
 //    fit-box or model-box overlaps the tile), built once by a stable radix
 //    sort so each list is in the reference's Morton star order.  Every
-//    image-space product (model render, residual, cost, J*v, freezing
+//    image-space product (model render, residual, cost, freezing
 //    subtraction) is a per-pixel gather over the tile's list: no float
 //    atomics, and each pixel sums its stars in the same order as the
 //    reference's scatter-add.
-//  * per-star products (fill + J^T r, J^T u, steps, freeze tests) run one
-//    wavefront per star with wave-shuffle reductions in float64.
+//  * per-star products (fill + J^T r, steps, freeze tests) run one
+//    wavefront per star with wave-shuffle reductions in float64; the inner
+//    solve runs in star space on the normal matrix (normal_op.h).
 //  * grid-wide sums: every block writes its partial with a plain store and a
 //    one-block k_finish launch sums them in block order (deterministic run to
 //    run).  No same-address atomics and no grid-wide fences: on gfx950 both
@@ -38,9 +39,8 @@ constexpr double DEPS = 2.220446049250313e-16;
 enum : int { MODE_RESID = 0, MODE_COST = 1, MODE_FREEZE = 2, MODE_FINAL = 3 };
 
 // Stamp values V: one row of VP = 4 values (the p <= 3 free columns, zero
-// padded) per stamp pixel, so a row is a single 16-byte (float) load.  The
-// LSQR vectors read by J v (b and t, float64) use the same padded stride and
-// are zero for stars that are not live, so J v needs no live test.
+// padded) per stamp pixel, so a row is a single 16-byte (float) load.
+// b_scaled = J^T r (float64) keeps the same padded stride per star.
 constexpr int VP = 4;
 template <typename T> struct Vec4Of;
 template <> struct Vec4Of<float> { using type = float4; };
@@ -1035,22 +1035,21 @@ __device__ void lsmr_advance(Scal* Sc, double tt, const double* q, int it) {
 // block order and updates the device scalars
 // ---------------------------------------------------------------------------
 enum : int {
-    FIN_COST = 0, FIN_COST_CAND = 1, FIN_FILL = 2, FIN_LSQR = 4, FIN_ACCEPT = 5, FIN_COUNT = 6,
-    FIN_CGQ = 7, FIN_CGS = 8, FIN_GKB = 9
+    FIN_COST = 0, FIN_COST_CAND = 1, FIN_FILL = 2, FIN_ACCEPT = 5, FIN_COUNT = 6, FIN_CGQ = 7, FIN_CGS = 8
 };
 constexpr int FIN_BLOCK = 1024;
 
 template <int OP>
 static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const double* __restrict__ part,
                                                              const double* __restrict__ part2, int nb, int it) {
-    if ((OP == FIN_LSQR || OP == FIN_CGQ || OP == FIN_CGS || OP == FIN_GKB) && Sc->done) return;
+    if ((OP == FIN_CGQ || OP == FIN_CGS) && Sc->done) return;
     __shared__ double sh[2][FIN_BLOCK / 64];
     double a = 0, b = 0;
     for (int i = threadIdx.x; i < nb; i += FIN_BLOCK) {
         a += part[i];
         if (OP == FIN_FILL) b = fmax(b, part[nb + i]);
         if (OP == FIN_ACCEPT) b += part[nb + i];
-        if (OP == FIN_LSQR || OP == FIN_CGS || OP == FIN_CGQ || OP == FIN_GKB) b += part2[i];
+        if (OP == FIN_CGS || OP == FIN_CGQ) b += part2[i];
     }
     a = wave_sum(a);
     b = OP == FIN_FILL ? wave_max(b) : wave_sum(b);
@@ -1070,12 +1069,6 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const dou
     if (OP == FIN_FILL) {
         Sc->bnorm2 = ta;
         Sc->gmax_bits = (unsigned long long)__double_as_longlong(tb);
-    }
-    if (OP == FIN_LSQR) lsqr_advance(Sc, ta, tb, it);
-    if (OP == FIN_GKB) {
-        // ta = v . H v, tb = v . J^T u_k: beta_{k+1}^2 = |J v - alpha u_k|^2 with |u_k| = 1
-        const double a = Sc->alpha;
-        Sc->beta_next = sqrt(fmax(0.0, ta - 2 * a * tb + a * a));
     }
     if (OP == FIN_CGQ) {
         // ta = p . H p, tb = |p|^2: the step a = gamma / (p'Hp + lam^2 |p|^2)
@@ -1100,36 +1093,6 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const dou
         Sc->n_frozen_now = (int)tb;
     }
     if (OP == FIN_COUNT) Sc->n_union = (unsigned long long)ta;
-}
-
-// One-block sum of LSMR's seven per-block partials of k_JTu (|t|^2, |h|^2 and
-// the five products in part3) and its scalar recurrences.
-static __global__ __launch_bounds__(FIN_BLOCK) void k_finish_lsmr(Scal* Sc, const double* __restrict__ part,
-                                                                  const double* __restrict__ part2,
-                                                                  const double* __restrict__ part3, int nb, int it) {
-    if (Sc->done) return;
-    __shared__ double sh[7][FIN_BLOCK / 64];
-    double a[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < nb; i += FIN_BLOCK) {
-        a[0] += part[i];
-        a[1] += part3[i];
-        a[2] += part3[nb + i];
-        a[3] += part3[2 * nb + i];
-        a[4] += part2[i];
-        a[5] += part3[3 * nb + i];
-        a[6] += part3[4 * nb + i];
-    }
-#pragma unroll
-    for (int k = 0; k < 7; ++k) a[k] = wave_sum(a[k]);
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 7; ++k) sh[k][threadIdx.x >> 6] = a[k];
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double t[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < 7; ++k)
-        for (int i = 0; i < FIN_BLOCK / 64; ++i) t[k] += sh[k][i];
-    // t[1..6] = |x|^2, x.h, x.hbar, |h|^2, h.hbar, |hbar|^2
-    lsmr_advance(Sc, t[0], t + 1, it);
 }
 
 // The accepted candidate's cost becomes the current cost (its weighted

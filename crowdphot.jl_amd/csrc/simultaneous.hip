@@ -116,7 +116,7 @@ void build_tiles(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
 // Rows of the star-space normal matrix: for every active star (Morton order)
 // the stars whose stamps overlap its own, from a binning into cells of side 2R.
 struct PairRows {
-    DevBuf rowptr, pb;
+    DevBuf rowptr, pb, pblk;
     int64_t npairs = 0;
 };
 
@@ -168,7 +168,15 @@ void build_pairs(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
     launch(k_pairs<true>, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, d_ay, d_ax, g,
            (const int*)cstart.as<int>(), (const int*)cend.as<int>(), (const int*)clist.as<int>(), (int*)nullptr,
            (const int*)P.rowptr.as<int>(), P.pb.as<int>());
-    CP_HIP(hipStreamSynchronize(st));   // the host word copied to rowptr[na] is a local
+    // block slots: each pair's block is kept once, in the row of the lower index
+    P.pblk.alloc(ctx, (size_t)std::max<int64_t>(P.npairs, 1) * 4);
+    CP_HIP(hipMemsetAsync(cnt.get(), 0, 4, st));
+    launch(k_pair_slots, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, (const int*)P.rowptr.as<int>(),
+           (const int*)P.pb.as<int>(), P.pblk.as<int>(), cnt.as<int>());
+    int bad = 0;
+    CP_HIP(hipMemcpyAsync(&bad, cnt.get(), 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipStreamSynchronize(st));   // also: the host word copied to rowptr[na] is a local
+    if (bad) throw Error(CP_ESTATE, "star pair rows are not symmetric");
 }
 
 static int bit_length(int64_t v) {
@@ -509,20 +517,16 @@ struct Sim {
         DevBuf V(ctx, (size_t)na * S2 * VP * sizeof(VT));
         CP_HIP(hipMemsetAsync(V.get(), 0, V.bytes(), st));
         const size_t nb8 = (size_t)na * p * 8;
-        const size_t nb32 = (size_t)na * VP * 8;   // padded float64 star vectors gathered by H v
+        const size_t nb32 = (size_t)na * VP * 8;   // b_scaled: padded like the V rows it is summed from
         const int solver = o.solver;
-        DevBuf colnorm(ctx, nb8), D(ctx, nb8), bsc(ctx, nb32), xs(ctx, nb8), wl(ctx, nb8), tb(ctx, nb32), dsc(ctx, nb8),
+        DevBuf colnorm(ctx, nb8), D(ctx, nb8), bsc(ctx, nb32), xs(ctx, nb8), wl(ctx, nb8), dsc(ctx, nb8),
             theta_c(ctx, nb8), hbar(ctx, solver == CP_SOLVER_LSMR ? nb8 : 0);
-        // the normal matrix's blocks, H v, J^T u_k (LSQR / LSMR) and the CG
-        // directions of two successive steps
-        DevBuf Hblk(ctx, (size_t)std::max<int64_t>(prs.npairs, 1) * HB * 8), yv(ctx, nb32), ju(ctx, nb32),
-            pdir0(ctx, solver == CP_SOLVER_CGLS ? nb32 : 0), pdir1(ctx, solver == CP_SOLVER_CGLS ? nb32 : 0);
-        CP_HIP(hipMemsetAsync(yv.get(), 0, nb32, st));
-        CP_HIP(hipMemsetAsync(ju.get(), 0, nb32, st));
-        CP_HIP(hipMemsetAsync(colnorm.get(), 0, nb8, st));
-        CP_HIP(hipMemsetAsync(xs.get(), 0, nb8, st));
-        CP_HIP(hipMemsetAsync(wl.get(), 0, nb8, st));
-        CP_HIP(hipMemsetAsync(tb.get(), 0, nb32, st));
+        // the normal matrix's p x p blocks and two pairs of star vectors:
+        // LSQR / LSMR (g, v) of the previous and the current Golub-Kahan step,
+        // CG s, q = H p and the directions of two successive steps
+        DevBuf Hblk(ctx, (size_t)std::max<int64_t>(prs.npairs, 1) * p * p * 8), gv0(ctx, nb8), gv1(ctx, nb8),
+            vv0(ctx, nb8), vv1(ctx, nb8);
+        for (DevBuf* b : {&gv0, &gv1, &vv0, &vv1, &colnorm, &xs, &wl}) CP_HIP(hipMemsetAsync(b->get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(bsc.get(), 0, nb32, st));
         DevBuf gloc(ctx, na), recs(ctx, (size_t)na * sizeof(StarRec<T>)), recs_c(ctx, (size_t)na * sizeof(StarRec<T>));
         DevBuf crs, crs_c, ers, ers_c;
@@ -577,8 +581,9 @@ struct Sim {
         // per-block partials of the widest launch (two sets for fill / accept)
         const int maxpb = (int)std::max<int64_t>(
             {(int64_t)gb_render, (int64_t)gb_fill, (int64_t)gb_jv, (int64_t)gb_jtu, (int64_t)gstar});
-        DevBuf part(ctx, (size_t)2 * maxpb * 8), part2(ctx, (size_t)maxpb * 8),
-            part3(ctx, solver == CP_SOLVER_LSMR ? (size_t)5 * maxpb * 8 : 0);
+        DevBuf part(ctx, (size_t)2 * maxpb * 8), part2(ctx, (size_t)maxpb * 8);
+        // the Golub-Kahan step's partials (ten per block for LSMR, five for LSQR)
+        DevBuf partk(ctx, (size_t)10 * std::max(gb_jv, gb_jtu) * 8);
 
         auto star_rec = [&](const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter, int all_live) {
             launch(k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, th, fm, M, inv_norm_g64, moffat_amp64,
@@ -658,14 +663,17 @@ struct Sim {
                                                     tscan_l.as<long long>(), ntiles, st));
         };
         auto build_H = [&]() {
-#define CP_HB(GL)                                                                                            \
-    launch_n("k_hbuild", k_hbuild<VT, GL>, dim3(gb_hb), dim3(BLOCK), 0, st, (const int*)jlist.as<int>(),       \
-             (const uint8_t*)live.as<uint8_t>(), (const int*)prs.rowptr.as<int>(), (const int*)prs.pb.as<int>(), \
-             (const int*)ay.as<int>(), (const int*)ax.as<int>(), R, S, (const VT*)V.as<VT>(), Hblk.as<double>(), \
+            with_p(p, [&](auto pc) {
+                constexpr int PV = decltype(pc)::value;
+#define CP_HB(GL)                                                                                                 \
+    launch_n("k_hbuild", k_hbuild<VT, GL, PV>, dim3(gb_hb), dim3(BLOCK), 0, st, (const int*)jlist.as<int>(),        \
+             (const uint8_t*)live.as<uint8_t>(), (const int*)prs.rowptr.as<int>(), (const int*)prs.pb.as<int>(),      \
+             (const int*)ay.as<int>(), (const int*)ax.as<int>(), R, S, (const VT*)V.as<VT>(), Hblk.as<double>(),      \
              (const Scal*)Sc)
-            if (GLh == 16) CP_HB(16);
-            else CP_HB(64);
+                if (GLh == 16) CP_HB(16);
+                else CP_HB(64);
 #undef CP_HB
+            });
         };
 
         // --- outer damped Gauss-Newton loop (:1186-1284)
@@ -719,46 +727,54 @@ struct Sim {
                        o.linear_tol, solver);
                 // inner solve, capped step and candidate cost: one graph launch per trial
                 auto trial_body = [&]() {
-                    for (int it = 1; it <= o.inner_iterations; ++it) {
-                        const int first = it == 1;
-                        // CG: this step's direction goes to pcur, the previous one is read from pold
-                        double* pcur = (it & 1) ? pdir1.as<double>() : pdir0.as<double>();
-                        double* pold = (it & 1) ? pdir0.as<double>() : pdir1.as<double>();
-                        auto hv = [&](auto sv) {
-                            launch_n("k_hv", k_hv<decltype(sv)::value>, dim3(gb_jv), dim3(BLOCK), 0, st,
-                                     (const int*)jlist.as<int>(), (const uint8_t*)live.as<uint8_t>(),
-                                     (const int*)prs.rowptr.as<int>(), (const int*)prs.pb.as<int>(),
-                                     (const double*)Hblk.as<double>(), (const double*)bsc.as<double>(),
-                                     (const double*)tb.as<double>(), (const double*)ju.as<double>(),
-                                     (const double*)pold, pcur, yv.as<double>(), first, part.as<double>(),
-                                     part2.as<double>(), Sc);
-                        };
-                        auto upd = [&](auto sv) {
-                            launch_n("k_gk_update", k_gk_update<decltype(sv)::value>, dim3(gb_jtu), dim3(BLOCK), 0, st,
-                                     (const int*)jlist.as<int>(), (const uint8_t*)live.as<uint8_t>(), p,
-                                     (const double*)yv.as<double>(), (const double*)bsc.as<double>(), tb.as<double>(),
-                                     ju.as<double>(), (const double*)pcur, xs.as<double>(), wl.as<double>(),
-                                     hbar.as<double>(), first, part.as<double>(), part2.as<double>(),
-                                     part3.as<double>(), Sc);
-                        };
-                        if (solver == CP_SOLVER_CGLS) {
-                            hv(std::integral_constant<int, CP_SOLVER_CGLS>{});
-                            finish(k_finish<FIN_CGQ>, gb_jv, it);
-                            upd(std::integral_constant<int, CP_SOLVER_CGLS>{});
-                            finish(k_finish<FIN_CGS>, gb_jtu, it);
-                        } else if (solver == CP_SOLVER_LSMR) {
-                            hv(std::integral_constant<int, CP_SOLVER_LSMR>{});
-                            finish(k_finish<FIN_GKB>, gb_jv, it);
-                            upd(std::integral_constant<int, CP_SOLVER_LSMR>{});
-                            launch(k_finish_lsmr, dim3(1), dim3(FIN_BLOCK), 0, st, Sc, (const double*)part.as<double>(),
-                                   (const double*)part2.as<double>(), (const double*)part3.as<double>(), gb_jtu, it);
-                        } else {
-                            hv(std::integral_constant<int, CP_SOLVER_LSQR>{});
-                            finish(k_finish<FIN_GKB>, gb_jv, it);
-                            upd(std::integral_constant<int, CP_SOLVER_LSQR>{});
-                            finish(k_finish<FIN_LSQR>, gb_jtu, it);
+                    with_p(p, [&](auto pc) {
+                        constexpr int PV = decltype(pc)::value;
+                        const int* rp = (const int*)prs.rowptr.as<int>();
+                        const int* pbp = (const int*)prs.pb.as<int>();
+                        const int* psl = (const int*)prs.pblk.as<int>();
+                        const int* jl = (const int*)jlist.as<int>();
+                        const uint8_t* lv = (const uint8_t*)live.as<uint8_t>();
+                        const double* Hp = (const double*)Hblk.as<double>();
+                        const double* bs = (const double*)bsc.as<double>();
+                        for (int it = 1; it <= o.inner_iterations; ++it) {
+                            const int first = it == 1;
+                            if (solver == CP_SOLVER_CGLS) {
+                                // s in gv0, q in gv1; this step's direction goes to
+                                // pcur, the previous one is read from pold
+                                double* pcur = (it & 1) ? vv1.as<double>() : vv0.as<double>();
+                                double* pold = (it & 1) ? vv0.as<double>() : vv1.as<double>();
+                                launch_n("k_cg_hv", k_cg_hv<PV>, dim3(gb_jv), dim3(BLOCK), 0, st, jl, lv, rp, pbp, psl,
+                                         Hp, bs, (const double*)gv0.as<double>(), (const double*)pold, pcur,
+                                         gv1.as<double>(), first, partk.as<double>(), Sc);
+                                launch(k_finish<FIN_CGQ>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc,
+                                       (const double*)partk.as<double>(), (const double*)partk.as<double>() + gb_jv,
+                                       gb_jv, it);
+                                launch_n("k_cg_update", k_cg_update<PV>, dim3(gb_jtu), dim3(BLOCK), 0, st, jl, lv, bs,
+                                         (const double*)pcur, (const double*)gv1.as<double>(), gv0.as<double>(),
+                                         xs.as<double>(), first, partk.as<double>(), Sc);
+                                launch(k_finish<FIN_CGS>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc,
+                                       (const double*)partk.as<double>(), (const double*)partk.as<double>() + gb_jtu,
+                                       gb_jtu, it);
+                                continue;
+                            }
+                            // (g, v) of step it - 1 in buffer set (it - 1) & 1, step it writes set it & 1
+                            DevBuf& gprev = (it & 1) ? gv0 : gv1;
+                            DevBuf& vprev = (it & 1) ? vv0 : vv1;
+                            DevBuf& gnew = (it & 1) ? gv1 : gv0;
+                            DevBuf& vnew = (it & 1) ? vv1 : vv0;
+                            auto gk = [&](auto sv) {
+                                constexpr int SV = decltype(sv)::value;
+                                launch_n("k_gk", k_gk<PV, SV>, dim3(gb_jv), dim3(BLOCK), 0, st, jl, lv, rp, pbp, psl,
+                                         Hp, bs, (const double*)gprev.as<double>(), (const double*)vprev.as<double>(),
+                                         gnew.as<double>(), vnew.as<double>(), xs.as<double>(), wl.as<double>(),
+                                         hbar.as<double>(), first, partk.as<double>(), Sc);
+                                launch(k_finish_gk<SV>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc,
+                                       (const double*)partk.as<double>(), gb_jv, it);
+                            };
+                            if (solver == CP_SOLVER_LSMR) gk(std::integral_constant<int, CP_SOLVER_LSMR>{});
+                            else gk(std::integral_constant<int, CP_SOLVER_LSQR>{});
                         }
-                    }
+                    });
                     launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
                            (const double*)wl.as<double>(),
                            (const double*)(solver == CP_SOLVER_LSMR ? hbar.as<double>() : nullptr), xs.as<double>(),
