@@ -38,8 +38,10 @@
 //
 // Storage: the blocks of one outer iteration live in the slots of the pair
 // rows; the block of a pair (a, b) is built once, in the slot of the row of
-// min(a, b), and the other row reads it transposed (pblk < 0 names the slot).
-// Star vectors are float64 at stride p.
+// min(a, b) (its upper entry), and the other row reads it transposed.  Each
+// row entry's (neighbour, slot) word is rewritten with the blocks, so the
+// operator kernels read no live flags of neighbours.  Star vectors are
+// float64 at stride p.
 //
 // Neighbour lists: stars are binned into cells of side 2R (anchors shifted by
 // R - 1 so every active anchor is inside the grid), so the stars whose stamps
@@ -102,19 +104,21 @@ static __global__ void k_pairs(int na, const int* __restrict__ ay, const int* __
     if (!EMIT) cnt[a] = o;
 }
 
-// The block slot of every row entry: its own (b >= a) or -(j' + 1) for the
-// entry j' of row b that lists a (b < a; the overlap test is symmetric, so it
-// exists).  Sets *bad if one is missing.
-static __global__ void k_pair_slots(int na, const int* __restrict__ rowptr, const int* __restrict__ pb,
-                                    int* __restrict__ pblk, int* __restrict__ bad) {
+// Per row entry j (row a, neighbour b = pb[j]): its row, whether it holds the
+// pair's block (b >= a: the upper entries), and for an upper entry the entry
+// of row b that lists a (rev; itself on the diagonal).  The overlap test is
+// symmetric, so that entry exists; *bad is set if one is missing.
+static __global__ void k_pair_rev(int na, const int* __restrict__ rowptr, const int* __restrict__ pb,
+                                  int* __restrict__ rowof, uint8_t* __restrict__ upper, int* __restrict__ rev,
+                                  int* __restrict__ bad) {
     const int a = blockIdx.x * BLOCK + threadIdx.x;
     if (a >= na) return;
     for (int j = rowptr[a]; j < rowptr[a + 1]; ++j) {
         const int b = pb[j];
-        if (b >= a) {
-            pblk[j] = j;
-            continue;
-        }
+        rowof[j] = a;
+        upper[j] = b >= a;
+        if (b == a) rev[j] = j;
+        if (b >= a) continue;
         int t = -1;
         for (int i = rowptr[b]; i < rowptr[b + 1]; ++i)
             if (pb[i] == a) {
@@ -122,61 +126,75 @@ static __global__ void k_pair_slots(int na, const int* __restrict__ rowptr, cons
                 break;
             }
         if (t < 0) *bad = 1;
-        pblk[j] = t < 0 ? j : -(t + 1);
+        else rev[t] = j;   // row b's entry for a is the upper one
     }
 }
 
-// H_ab (P x P, row-major: row k is star a's parameter k) for every pair of
-// live stars (a <= b) of the rows in lst (Sc->n_jtu_stars of them): GL lanes
-// per row, its pairs in turn, lanes striding over the pixels the two stamps
-// share; the P^2 sums are group reductions in float64.  Blocks with a frozen
-// star are not built (k_gk never reads them).
-template <typename T, int GL, int P>
-__global__ __launch_bounds__(BLOCK) void k_hbuild(const int* __restrict__ lst, const uint8_t* __restrict__ live,
-                                                  const int* __restrict__ rowptr, const int* __restrict__ pb,
+// H_ab (P x P, row-major: row k is star a's parameter k) of every upper entry
+// j (row a, neighbour b >= a) whose two stars are live: GH lanes per entry
+// striding over the pixels the two stamps share, the P^2 sums as group
+// reductions in float64, stored in slot j.  Every upper entry also writes the
+// (neighbour, slot) words of both rows' entries for this outer iteration:
+// neighbour -1 where the neighbour is frozen (its vectors and this block are
+// not read), slot -(j + 1) where the row reads the block transposed.
+template <typename T, int GH, int P>
+__global__ __launch_bounds__(BLOCK) void k_hbuild(const int* __restrict__ ulist, int n_upper,
+                                                  const int* __restrict__ rowof, const int* __restrict__ pb,
+                                                  const int* __restrict__ rev, const uint8_t* __restrict__ live,
                                                   const int* __restrict__ ay, const int* __restrict__ ax, int R,
                                                   int S, const T* __restrict__ V, double* __restrict__ H,
-                                                  const Scal* Sc) {
-    constexpr int GPB = BLOCK / GL, PP = P * P;
-    const int sub = threadIdx.x % GL;
+                                                  int2* __restrict__ ent) {
+    constexpr int GPB = BLOCK / GH, PP = P * P;
+    const int sub = threadIdx.x % GH;
     const int S2 = S * S;
-    const int nl = Sc->n_jtu_stars;
     const vec4_t<T>* V4 = reinterpret_cast<const vec4_t<T>*>(V);
-    for (int li = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GL; li < nl; li += gridDim.x * GPB) {
-        const int a = lst[li];
-        if (!live[a]) continue;   // uniform within the group
-        const int ya = ay[a], xa = ax[a];
-        for (int j = rowptr[a]; j < rowptr[a + 1]; ++j) {
-            const int b = pb[j];
-            if (b < a || !live[b]) continue;
-            double h[PP];
+    for (int u = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GH; u < n_upper; u += gridDim.x * GPB) {
+        const int j = ulist[u];
+        const int a = rowof[j], b = pb[j];
+        const bool la = live[a], lb = live[b];
+        if (sub == 0) {
+            ent[j] = make_int2(lb ? b : -1, j);
+            if (b != a) ent[rev[j]] = make_int2(la ? a : -1, -(j + 1));
+        }
+        if (!(la && lb)) continue;   // uniform within the group
+        double h[PP];
 #pragma unroll
-            for (int e = 0; e < PP; ++e) h[e] = 0;
-            const int dy0 = ay[b] - ya, dx0 = ax[b] - xa;
-            const int ylo = max(-R, dy0 - R), yhi = min(R, dy0 + R);
-            const int xlo = max(-R, dx0 - R), xhi = min(R, dx0 + R);
-            const int hy = yhi - ylo + 1, nov = hy * (xhi - xlo + 1);
-            for (int i = sub; i < nov; i += GL) {
-                const int dy = ylo + i % hy, dx = xlo + i / hy;
-                const vec4_t<T> va = V4[(size_t)a * S2 + (dx + R) * S + (dy + R)];
-                const vec4_t<T> vb = V4[(size_t)b * S2 + (dx - dx0 + R) * S + (dy - dy0 + R)];
-                const double A[3] = {(double)va.x, (double)va.y, (double)va.z};
-                const double B[3] = {(double)vb.x, (double)vb.y, (double)vb.z};
+        for (int e = 0; e < PP; ++e) h[e] = 0;
+        const int dy0 = ay[b] - ay[a], dx0 = ax[b] - ax[a];
+        const int ylo = max(-R, dy0 - R), yhi = min(R, dy0 + R);
+        const int xlo = max(-R, dx0 - R), xhi = min(R, dx0 + R);
+        const int hy = yhi - ylo + 1, nov = hy * (xhi - xlo + 1);
+        // pixel i of the overlap is (ylo + i % hy, xlo + i / hy); step GH
+        const int qs = GH / hy, rs = GH % hy;
+        int oy = sub % hy, ox = sub / hy;
+        const vec4_t<T>* Va = V4 + (size_t)a * S2 + R * S + R;
+        const vec4_t<T>* Vb = V4 + (size_t)b * S2 + (R - dx0) * S + (R - dy0);
+        for (int i = sub; i < nov; i += GH) {
+            const int dy = ylo + oy, dx = xlo + ox;
+            const vec4_t<T> va = Va[dx * S + dy];
+            const vec4_t<T> vb = Vb[dx * S + dy];
+            const double A[3] = {(double)va.x, (double)va.y, (double)va.z};
+            const double B[3] = {(double)vb.x, (double)vb.y, (double)vb.z};
 #pragma unroll
-                for (int k = 0; k < P; ++k)
+            for (int k = 0; k < P; ++k)
 #pragma unroll
-                    for (int l = 0; l < P; ++l) h[P * k + l] += A[k] * B[l];
+                for (int l = 0; l < P; ++l) h[P * k + l] += A[k] * B[l];
+            oy += rs;
+            ox += qs;
+            if (oy >= hy) {
+                oy -= hy;
+                ++ox;
             }
+        }
 #pragma unroll
-            for (int e = 0; e < PP; ++e) h[e] = group_sum<GL>(h[e]);
-            // lane e < P^2 writes entry e of the block
-            if (sub < PP) {
-                double v = 0;
+        for (int e = 0; e < PP; ++e) h[e] = group_sum<GH>(h[e]);
+        // lane e < P^2 writes entry e of the block (GH < P^2: lanes write several)
+        for (int e0 = sub; e0 < PP; e0 += GH) {
+            double v = 0;
 #pragma unroll
-                for (int e = 0; e < PP; ++e)
-                    if (e == sub) v = h[e];
-                H[(size_t)j * PP + sub] = v;
-            }
+            for (int e = 0; e < PP; ++e)
+                if (e == e0) v = h[e];
+            H[(size_t)j * PP + e0] = v;
         }
     }
 }
@@ -202,18 +220,19 @@ __device__ __forceinline__ void block_store_n(double (&v)[N], double* __restrict
 }
 
 // y += H_ab v_b over the live neighbours b of row a, with v_b = c1 s1[b] +
-// c2 s2[b] (s2 read only when TWO).
-template <int P, bool TWO>
-__device__ __forceinline__ void row_apply(int a, const uint8_t* __restrict__ live, const int* __restrict__ rowptr,
-                                          const int* __restrict__ pb, const int* __restrict__ pblk,
-                                          const double* __restrict__ H, const double* __restrict__ s1, int st1,
-                                          double c1, const double* __restrict__ s2, double c2, double (&y)[P]) {
+// c2 s2[b] (s2 read only when TWO): GL lanes of a group take the row's
+// entries in turn and the group sums y (every lane gets it).
+template <int P, bool TWO, int GL>
+__device__ __forceinline__ void row_apply(int a, int sub, const int* __restrict__ rowptr,
+                                          const int2* __restrict__ ent, const double* __restrict__ H,
+                                          const double* __restrict__ s1, int st1, double c1,
+                                          const double* __restrict__ s2, double c2, double (&y)[P]) {
     constexpr int PP = P * P;
     const int j1 = rowptr[a + 1];
-    for (int j = rowptr[a]; j < j1; ++j) {
-        const int b = pb[j];
-        const int sl = pblk[j];
-        if (!live[b]) continue;
+    for (int j = rowptr[a] + sub; j < j1; j += GL) {
+        const int2 e = ent[j];
+        if (e.x < 0) continue;
+        const int b = e.x, sl = e.y;
         double v[P];
 #pragma unroll
         for (int l = 0; l < P; ++l) {
@@ -223,7 +242,7 @@ __device__ __forceinline__ void row_apply(int a, const uint8_t* __restrict__ liv
         const double* h = H + (size_t)(sl >= 0 ? sl : -sl - 1) * PP;
         double hv[PP];
 #pragma unroll
-        for (int e = 0; e < PP; ++e) hv[e] = h[e];
+        for (int q = 0; q < PP; ++q) hv[q] = h[q];
         if (sl >= 0) {
 #pragma unroll
             for (int k = 0; k < P; ++k)
@@ -236,11 +255,25 @@ __device__ __forceinline__ void row_apply(int a, const uint8_t* __restrict__ liv
                 for (int l = 0; l < P; ++l) y[k] += hv[P * l + k] * v[l];
         }
     }
+#pragma unroll
+    for (int k = 0; k < P; ++k) y[k] = group_sum<GL>(y[k]);
 }
 
-// One Golub-Kahan step k of LSQR / LSMR for the rows in lst (one thread per
-// row; the first step forms v_1 = -b_scaled / (beta_1 alpha_1) and J^T u_1 =
-// -b_scaled / beta_1):
+template <int P>
+__device__ __forceinline__ double pick(const double (&y)[P], int k) {
+    double r = y[0];
+#pragma unroll
+    for (int q = 1; q < P; ++q)
+        if (k == q) r = y[q];
+    return r;
+}
+
+// Lanes per row of the star-space operator kernels.
+constexpr int GKL = 4;
+
+// One Golub-Kahan step k of LSQR / LSMR for the rows in lst (GKL lanes per
+// row; lane k < p then owns the row's component k; the first step forms
+// v_1 = -b_scaled / (beta_1 alpha_1) and J^T u_1 = -b_scaled / beta_1):
 //   v_k and J^T u_k from (g_{k-1}, v_{k-1}) in gp / vp, y = H v_k,
 //   g_k = y - alpha_k J^T u_k and v_k to gn / vn, and the pending update of
 //   step k - 1 (hbar <- w - del hbar, x <- x + sig hbar, w <- v_k - tau w;
@@ -249,15 +282,15 @@ __device__ __forceinline__ void row_apply(int a, const uint8_t* __restrict__ liv
 // x.hbar, w.hbar, |hbar|^2 of the updated vectors.
 template <int P, int SOLVER>
 __global__ __launch_bounds__(BLOCK) void k_gk(const int* __restrict__ lst, const uint8_t* __restrict__ live,
-                                              const int* __restrict__ rowptr, const int* __restrict__ pb,
-                                              const int* __restrict__ pblk, const double* __restrict__ H,
-                                              const double* __restrict__ b_scaled, const double* __restrict__ gp,
-                                              const double* __restrict__ vp, double* __restrict__ gn,
-                                              double* __restrict__ vn, double* __restrict__ x,
-                                              double* __restrict__ wv, double* __restrict__ hb, int first,
-                                              double* __restrict__ part, Scal* Sc) {
+                                              const int* __restrict__ rowptr, const int2* __restrict__ ent,
+                                              const double* __restrict__ H, const double* __restrict__ b_scaled,
+                                              const double* __restrict__ gp, const double* __restrict__ vp,
+                                              double* __restrict__ gn, double* __restrict__ vn,
+                                              double* __restrict__ x, double* __restrict__ wv,
+                                              double* __restrict__ hb, int first, double* __restrict__ part,
+                                              Scal* Sc) {
     constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
-    constexpr int NP = LSMR ? 10 : 5;
+    constexpr int NP = LSMR ? 10 : 5, RPB = BLOCK / GKL;
     if (Sc->done) return;
     __shared__ double sred[NP][NWAVE];
     const double alpha = Sc->alpha;
@@ -270,51 +303,50 @@ __global__ __launch_bounds__(BLOCK) void k_gk(const int* __restrict__ lst, const
     const double* s1 = first ? b_scaled : gp;
     const int st1 = first ? VP : P;
     const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau, del = first ? 0.0 : Sc->del;
+    const int sub = threadIdx.x % GKL;
     double acc[NP];
 #pragma unroll
     for (int e = 0; e < NP; ++e) acc[e] = 0;
     const int nl = Sc->n_jtu_stars;
-    for (int li = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x; li < nl; li += gridDim.x * BLOCK) {
+    for (int li = xcd_remap(blockIdx.x, gridDim.x) * RPB + threadIdx.x / GKL; li < nl; li += gridDim.x * RPB) {
         const int a = lst[li];
-        const size_t base = (size_t)a * P;
-        if (!live[a]) {
-            if (first)
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    x[base + k] = 0;
-                    wv[base + k] = 0;
-                    if (LSMR) hb[base + k] = 0;
-                }
+        const size_t ck = (size_t)a * P + sub;   // this lane's component (sub < P)
+        if (!live[a]) {   // uniform within the group
+            if (first && sub < P) {
+                x[ck] = 0;
+                wv[ck] = 0;
+                if (LSMR) hb[ck] = 0;
+            }
             continue;
         }
         double y[P];
 #pragma unroll
         for (int k = 0; k < P; ++k) y[k] = 0;
-        if (first) row_apply<P, false>(a, live, rowptr, pb, pblk, H, s1, st1, c1, vp, c2, y);
-        else row_apply<P, true>(a, live, rowptr, pb, pblk, H, s1, st1, c1, vp, c2, y);
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const double sk = s1[(size_t)a * st1 + k];
-            const double vk = c1 * sk + (first ? 0.0 : c2 * vp[base + k]);
+        if (first) row_apply<P, false, GKL>(a, sub, rowptr, ent, H, s1, st1, c1, vp, c2, y);
+        else row_apply<P, true, GKL>(a, sub, rowptr, ent, H, s1, st1, c1, vp, c2, y);
+        if (sub < P) {
+            const double yk = pick<P>(y, sub);
+            const double sk = s1[(size_t)a * st1 + sub];
+            const double vk = c1 * sk + (first ? 0.0 : c2 * vp[ck]);
             const double jk = d1 * sk;
-            const double gk = y[k] - alpha * jk;
-            gn[base + k] = gk;
-            vn[base + k] = vk;
-            acc[0] += vk * y[k];
+            const double gk = yk - alpha * jk;
+            gn[ck] = gk;
+            vn[ck] = vk;
+            acc[0] += vk * yk;
             acc[1] += vk * jk;
             acc[2] += gk * gk;
             acc[3] += vk * vk;
-            const double xo = first ? 0.0 : x[base + k];
-            const double wo = first ? 0.0 : wv[base + k];
-            const double ho = (LSMR && !first) ? hb[base + k] : 0.0;
+            const double xo = first ? 0.0 : x[ck];
+            const double wo = first ? 0.0 : wv[ck];
+            const double ho = (LSMR && !first) ? hb[ck] : 0.0;
             const double hn = wo - del * ho;
             const double xn = xo + sig * hn;
             const double wn = vk - tau * wo;
-            x[base + k] = xn;
-            wv[base + k] = wn;
+            x[ck] = xn;
+            wv[ck] = wn;
             acc[4] += wn * wn;
             if (LSMR) {
-                hb[base + k] = hn;
+                hb[ck] = hn;
                 acc[5] += xn * xn;
                 acc[6] += xn * wn;
                 acc[7] += xn * hn;
@@ -369,40 +401,42 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish_gk(Scal* Sc, const 
 }
 
 // CG on the damped normal equations (J'J + lam^2 I) x = -b_scaled, the
-// direction half: q = H p for the rows in lst with p = s + beta p_old formed
-// where it is gathered (the first step: p_0 = s_0 = -b_scaled); each row
-// writes its own p to pn and q.  Partials p.q and |p|^2.
+// direction half: q = H p for the rows in lst (GKL lanes per row, lane k < p
+// owning component k) with p = s + beta p_old formed where it is gathered
+// (the first step: p_0 = s_0 = -b_scaled); each row writes its own p to pn
+// and q.  Partials p.q and |p|^2.
 template <int P>
 __global__ __launch_bounds__(BLOCK) void k_cg_hv(const int* __restrict__ lst, const uint8_t* __restrict__ live,
-                                                 const int* __restrict__ rowptr, const int* __restrict__ pb,
-                                                 const int* __restrict__ pblk, const double* __restrict__ H,
-                                                 const double* __restrict__ b_scaled, const double* __restrict__ s,
-                                                 const double* __restrict__ po, double* __restrict__ pn,
-                                                 double* __restrict__ q, int first, double* __restrict__ part,
-                                                 Scal* Sc) {
+                                                 const int* __restrict__ rowptr, const int2* __restrict__ ent,
+                                                 const double* __restrict__ H, const double* __restrict__ b_scaled,
+                                                 const double* __restrict__ s, const double* __restrict__ po,
+                                                 double* __restrict__ pn, double* __restrict__ q, int first,
+                                                 double* __restrict__ part, Scal* Sc) {
+    constexpr int RPB = BLOCK / GKL;
     if (Sc->done) return;
     __shared__ double sred[2][NWAVE];
     const double cb = first ? 0.0 : Sc->cg_beta;
     const double* s1 = first ? b_scaled : s;
     const int st1 = first ? VP : P;
     const double c1 = first ? -1.0 : 1.0;
+    const int sub = threadIdx.x % GKL;
     double acc[2] = {0, 0};
     const int nl = Sc->n_jtu_stars;
-    for (int li = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x; li < nl; li += gridDim.x * BLOCK) {
+    for (int li = xcd_remap(blockIdx.x, gridDim.x) * RPB + threadIdx.x / GKL; li < nl; li += gridDim.x * RPB) {
         const int a = lst[li];
-        const size_t base = (size_t)a * P;
+        const size_t ck = (size_t)a * P + sub;
         if (!live[a]) continue;
         double y[P];
 #pragma unroll
         for (int k = 0; k < P; ++k) y[k] = 0;
-        if (first) row_apply<P, false>(a, live, rowptr, pb, pblk, H, s1, st1, c1, po, cb, y);
-        else row_apply<P, true>(a, live, rowptr, pb, pblk, H, s1, st1, c1, po, cb, y);
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const double pk = c1 * s1[(size_t)a * st1 + k] + (first ? 0.0 : cb * po[base + k]);
-            pn[base + k] = pk;
-            q[base + k] = y[k];
-            acc[0] += pk * y[k];
+        if (first) row_apply<P, false, GKL>(a, sub, rowptr, ent, H, s1, st1, c1, po, cb, y);
+        else row_apply<P, true, GKL>(a, sub, rowptr, ent, H, s1, st1, c1, po, cb, y);
+        if (sub < P) {
+            const double yk = pick<P>(y, sub);
+            const double pk = c1 * s1[(size_t)a * st1 + sub] + (first ? 0.0 : cb * po[ck]);
+            pn[ck] = pk;
+            q[ck] = yk;
+            acc[0] += pk * yk;
             acc[1] += pk * pk;
         }
     }
