@@ -157,8 +157,8 @@ class LMOpts(C.Structure):
 class Tuning(C.Structure):
     """cp_tuning: blocks per compute unit of the grid-stride kernels (0 = default)."""
     _fields_ = [
-        ("jv_blocks_per_cu", C.c_int32),
-        ("jtu_blocks_per_cu", C.c_int32),
+        ("hbuild_blocks_per_cu", C.c_int32),
+        ("cg_blocks_per_cu", C.c_int32),
         ("render_blocks_per_cu", C.c_int32),
         ("lm_blocks_per_cu", C.c_int32),
     ]

@@ -1,8 +1,8 @@
 """Launch-shape autotuner for the grid-stride kernels.
 
 The tunable shapes are the cp_tuning fields of a context (blocks per compute
-unit of the J v gather, the per-star J^T u, the tile-gather renders and the
-batched per-star LM).  `tune` times complete calls of a workload -- whatever
+unit of the normal-matrix assembly, CG's per-star update, the tile-gather
+renders and the batched per-star LM).  `tune` times complete calls of a workload -- whatever
 `run` does, typically one device-resident fit -- over candidate values, one
 field at a time (coordinate descent from the current shape, best of `repeats`
 wall-clock timings per candidate, the call synchronous), leaves the best
@@ -11,22 +11,22 @@ a small JSON file keyed by a workload name, so a later process applies them
 without timing.
 
 Shapes change no result of the batched per-star LM (every cutout's fit is
-independent of the grid).  In the simultaneous fit they regroup the per-block
-partial sums of the LSQR norms, a rounding-level difference that is fixed for
-a fixed shape.
+independent of the grid) or of the normal-matrix assembly (one group per
+block, whatever the grid).  CG's update shape and the render shape regroup
+per-block partial sums (CG's norms, the costs), a rounding-level difference
+that is fixed for a fixed shape.
 """
 import json
 import os
 import time
 
 DEFAULT_SPACE = {
-    "jv_blocks_per_cu": (4, 8, 16),
-    "jtu_blocks_per_cu": (4, 8, 16),
+    "hbuild_blocks_per_cu": (8, 16, 32),
     "render_blocks_per_cu": (2, 4, 8),
 }
 LM_SPACE = {"lm_blocks_per_cu": (2, 4, 8, 16)}
 # the built-in defaults (what 0 stands for), as starting points
-DEFAULTS = {"jv_blocks_per_cu": 8, "jtu_blocks_per_cu": 8, "render_blocks_per_cu": 4, "lm_blocks_per_cu": 8}
+DEFAULTS = {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 8, "render_blocks_per_cu": 4, "lm_blocks_per_cu": 8}
 
 
 def _time(run, repeats):

@@ -167,7 +167,7 @@ int cp_set_tuning(cp_ctx* c, const cp_tuning* t) {
     if (!c) return CP_ESTATE;
     return guarded(c, [&] {
         CP_ARG(t != nullptr, "NULL argument");
-        for (int32_t v : {t->jv_blocks_per_cu, t->jtu_blocks_per_cu, t->render_blocks_per_cu, t->lm_blocks_per_cu})
+        for (int32_t v : {t->hbuild_blocks_per_cu, t->cg_blocks_per_cu, t->render_blocks_per_cu, t->lm_blocks_per_cu})
             CP_ARG(v >= 0 && v <= 64, "blocks per CU must be in [0, 64] (0 = default), got " + std::to_string(v));
         c->tuning = *t;
     });

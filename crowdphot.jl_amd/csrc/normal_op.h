@@ -130,37 +130,47 @@ static __global__ void k_pair_rev(int na, const int* __restrict__ rowptr, const 
     }
 }
 
+// The record of upper entry u (slot j = ulist[u]): (j, its row, its
+// neighbour, its twin entry).
+static __global__ void k_upper_rec(int n_upper, const int* __restrict__ ulist, const int* __restrict__ rowof,
+                                   const int* __restrict__ pb, const int* __restrict__ rev, int4* __restrict__ urec) {
+    const int u = blockIdx.x * BLOCK + threadIdx.x;
+    if (u >= n_upper) return;
+    const int j = ulist[u];
+    urec[u] = make_int4(j, rowof[j], pb[j], rev[j]);
+}
+
 // H_ab (P x P, row-major: row k is star a's parameter k) of every upper entry
-// j (row a, neighbour b >= a) whose two stars are live: GH lanes per entry
+// (its record: slot j, row a, neighbour b >= a, the twin entry of row b) whose
+// two stars are live: GH lanes per entry
 // striding over the pixels the two stamps share, the P^2 sums as group
 // reductions in float64, stored in slot j.  Every upper entry also writes the
 // (neighbour, slot) words of both rows' entries for this outer iteration:
 // neighbour -1 where the neighbour is frozen (its vectors and this block are
 // not read), slot -(j + 1) where the row reads the block transposed.
 template <typename T, int GH, int P>
-__global__ __launch_bounds__(BLOCK) void k_hbuild(const int* __restrict__ ulist, int n_upper,
-                                                  const int* __restrict__ rowof, const int* __restrict__ pb,
-                                                  const int* __restrict__ rev, const uint8_t* __restrict__ live,
-                                                  const int* __restrict__ ay, const int* __restrict__ ax, int R,
-                                                  int S, const T* __restrict__ V, double* __restrict__ H,
+__global__ __launch_bounds__(BLOCK) void k_hbuild(const int4* __restrict__ urec, int n_upper,
+                                                  const uint8_t* __restrict__ live, const int* __restrict__ ay,
+                                                  const int* __restrict__ ax, int R, int S,
+                                                  const T* __restrict__ V, double* __restrict__ H,
                                                   int2* __restrict__ ent) {
     constexpr int GPB = BLOCK / GH, PP = P * P;
     const int sub = threadIdx.x % GH;
     const int S2 = S * S;
     const vec4_t<T>* V4 = reinterpret_cast<const vec4_t<T>*>(V);
     for (int u = xcd_remap(blockIdx.x, gridDim.x) * GPB + threadIdx.x / GH; u < n_upper; u += gridDim.x * GPB) {
-        const int j = ulist[u];
-        const int a = rowof[j], b = pb[j];
+        const int4 ur = urec[u];   // (slot j, row a, neighbour b, twin entry)
+        const int j = ur.x, a = ur.y, b = ur.z;
         const bool la = live[a], lb = live[b];
+        const int dy0 = ay[b] - ay[a], dx0 = ax[b] - ax[a];
         if (sub == 0) {
             ent[j] = make_int2(lb ? b : -1, j);
-            if (b != a) ent[rev[j]] = make_int2(la ? a : -1, -(j + 1));
+            if (b != a) ent[ur.w] = make_int2(la ? a : -1, -(j + 1));
         }
         if (!(la && lb)) continue;   // uniform within the group
         double h[PP];
 #pragma unroll
         for (int e = 0; e < PP; ++e) h[e] = 0;
-        const int dy0 = ay[b] - ay[a], dx0 = ax[b] - ax[a];
         const int ylo = max(-R, dy0 - R), yhi = min(R, dy0 + R);
         const int xlo = max(-R, dx0 - R), xhi = min(R, dx0 + R);
         const int hy = yhi - ylo + 1, nov = hy * (xhi - xlo + 1);
@@ -219,69 +229,98 @@ __device__ __forceinline__ void block_store_n(double (&v)[N], double* __restrict
     }
 }
 
+// Sum over the 4 lanes of an aligned quad (DPP lane swaps; every lane gets
+// the same value).
+template <int CTRL>
+__device__ __forceinline__ double dpp_swap(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double quad_sum(double v) {
+    v += dpp_swap<0x4E>(v);   // quad_perm [2, 3, 0, 1]: lane ^ 2
+    v += dpp_swap<0xB1>(v);   // quad_perm [1, 0, 3, 2]: lane ^ 1
+    return v;
+}
+
 // y += H_ab v_b over the live neighbours b of row a, with v_b = c1 s1[b] +
-// c2 s2[b] (s2 read only when TWO): GL lanes of a group take the row's
-// entries in turn and the group sums y (every lane gets it).
-template <int P, bool TWO, int GL>
-__device__ __forceinline__ void row_apply(int a, int sub, const int* __restrict__ rowptr,
-                                          const int2* __restrict__ ent, const double* __restrict__ H,
-                                          const double* __restrict__ s1, int st1, double c1,
-                                          const double* __restrict__ s2, double c2, double (&y)[P]) {
+// c2 s2[b] (s2 read only when TWO): the 4 lanes of a quad take the row's
+// entries j = jb, jb + 4, ... below j1, and the quad sums y.  The row's own
+// entry (b = a) also yields s1[a] and v_a (sd, vd), so the row needs no
+// separate loads of them; every lane gets all three.
+template <int P, bool TWO>
+__device__ __forceinline__ void row_apply(int a, int jb, int j1, const int2* __restrict__ ent,
+                                          const double* __restrict__ H, const double* __restrict__ s1, int st1,
+                                          double c1, const double* __restrict__ s2, double c2, double (&y)[P],
+                                          double (&sd)[P], double (&vd)[P]) {
     constexpr int PP = P * P;
-    const int j1 = rowptr[a + 1];
-    for (int j = rowptr[a] + sub; j < j1; j += GL) {
-        const int2 e = ent[j];
-        if (e.x < 0) continue;
-        const int b = e.x, sl = e.y;
+    const unsigned long long* ew = reinterpret_cast<const unsigned long long*>(ent);
+#pragma unroll
+    for (int k = 0; k < P; ++k) y[k] = sd[k] = vd[k] = 0;
+    for (int j = jb; j < j1; j += 4) {
+        // one 8-byte load of (neighbour, slot); a frozen neighbour (-1) reads
+        // star 0 and its block slot (in bounds) and contributes nothing
+        const unsigned long long e = ew[j];
+        const int bn = (int)(unsigned)(e & 0xffffffffu), sl = (int)(unsigned)(e >> 32);
+        const bool on = bn >= 0, dg = bn == a;
+        const int b = on ? bn : 0;
+        const bool tr = sl < 0;
         double v[P];
 #pragma unroll
         for (int l = 0; l < P; ++l) {
-            v[l] = c1 * s1[(size_t)b * st1 + l];
+            const double sv = s1[(size_t)b * st1 + l];
+            v[l] = c1 * sv;
             if (TWO) v[l] += c2 * s2[(size_t)b * P + l];
+            sd[l] = dg ? sv : sd[l];
+            vd[l] = dg ? v[l] : vd[l];
         }
-        const double* h = H + (size_t)(sl >= 0 ? sl : -sl - 1) * PP;
+        const double* h = H + (size_t)(tr ? -sl - 1 : sl) * PP;
         double hv[PP];
 #pragma unroll
         for (int q = 0; q < PP; ++q) hv[q] = h[q];
-        if (sl >= 0) {
 #pragma unroll
-            for (int k = 0; k < P; ++k)
+        for (int k = 0; k < P; ++k) {
+            double t = 0;
 #pragma unroll
-                for (int l = 0; l < P; ++l) y[k] += hv[P * k + l] * v[l];
-        } else {
-#pragma unroll
-            for (int k = 0; k < P; ++k)
-#pragma unroll
-                for (int l = 0; l < P; ++l) y[k] += hv[P * l + k] * v[l];
+            for (int l = 0; l < P; ++l) t += (tr ? hv[P * l + k] : hv[P * k + l]) * v[l];
+            y[k] += on ? t : 0.0;
         }
     }
 #pragma unroll
-    for (int k = 0; k < P; ++k) y[k] = group_sum<GL>(y[k]);
+    for (int k = 0; k < P; ++k) {
+        y[k] = quad_sum(y[k]);
+        sd[k] = quad_sum(sd[k]);   // one lane holds it, the others 0: exact
+        vd[k] = quad_sum(vd[k]);
+    }
 }
 
+// y[k] for a lane-dependent k as explicit selects over registers (an indexed
+// read would move the array to LDS).
 template <int P>
 __device__ __forceinline__ double pick(const double (&y)[P], int k) {
-    double r = y[0];
-#pragma unroll
-    for (int q = 1; q < P; ++q)
-        if (k == q) r = y[q];
-    return r;
+    const double y0 = y[0], y1 = P > 1 ? y[1] : 0.0, y2 = P > 2 ? y[2] : 0.0;
+    return k == 0 ? y0 : (k == 1 ? y1 : y2);
 }
 
-// Lanes per row of the star-space operator kernels.
+// Lanes per row of the star-space operator kernels (a quad: row_apply's DPP sums).
 constexpr int GKL = 4;
 
-// One Golub-Kahan step k of LSQR / LSMR for the rows in lst (GKL lanes per
-// row; lane k < p then owns the row's component k; the first step forms
-// v_1 = -b_scaled / (beta_1 alpha_1) and J^T u_1 = -b_scaled / beta_1):
+// One Golub-Kahan step k of LSQR / LSMR, GKL = 4 lanes (a quad) per star row
+// (a quad for every star; rows of frozen stars do nothing; lane k < p owns
+// the row's component k; the first step forms v_1 = -b_scaled / (beta_1
+// alpha_1) and J^T u_1 = -b_scaled / beta_1):
 //   v_k and J^T u_k from (g_{k-1}, v_{k-1}) in gp / vp, y = H v_k,
 //   g_k = y - alpha_k J^T u_k and v_k to gn / vn, and the pending update of
 //   step k - 1 (hbar <- w - del hbar, x <- x + sig hbar, w <- v_k - tau w;
 //   LSQR has del = 0).
 // Partials: v.y, v.J^T u, |g|^2, |v|^2, |w|^2 and, for LSMR, |x|^2, x.w,
 // x.hbar, w.hbar, |hbar|^2 of the updated vectors.
+// Load order matters: vector loads return in issue order, so the dependent
+// chain (row range, entries, neighbours) goes first and the row's x / w /
+// hbar loads only after the gather.
 template <int P, int SOLVER>
-__global__ __launch_bounds__(BLOCK) void k_gk(const int* __restrict__ lst, const uint8_t* __restrict__ live,
+__global__ __launch_bounds__(BLOCK) void k_gk(int na, const uint8_t* __restrict__ live,
                                               const int* __restrict__ rowptr, const int2* __restrict__ ent,
                                               const double* __restrict__ H, const double* __restrict__ b_scaled,
                                               const double* __restrict__ gp, const double* __restrict__ vp,
@@ -291,8 +330,18 @@ __global__ __launch_bounds__(BLOCK) void k_gk(const int* __restrict__ lst, const
                                               Scal* Sc) {
     constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
     constexpr int NP = LSMR ? 10 : 5, RPB = BLOCK / GKL;
-    if (Sc->done) return;
     __shared__ double sred[NP][NWAVE];
+    const int sub = threadIdx.x % GKL;
+    const int a = xcd_remap(blockIdx.x, gridDim.x) * RPB + threadIdx.x / GKL;
+    const double* s1 = first ? b_scaled : gp;
+    const int st1 = first ? VP : P;
+    // the row's entry range, live flag and the solver's done flag, loaded
+    // together at clamped (in-bounds) indices and tested once (no early exit,
+    // so no load waits on another); a finished solve does no work
+    const int ac = a < na ? a : na - 1;
+    const size_t ck = (size_t)ac * P + (sub < P ? sub : P - 1);   // this lane's component (sub < P)
+    const int j0 = rowptr[ac], j1 = rowptr[ac + 1];
+    const bool row = (a < na) & (live[ac] != 0) & (j1 > j0) & (Sc->done == 0);
     const double alpha = Sc->alpha;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
     const double beta = first ? 0.0 : Sc->beta_cur;
@@ -300,35 +349,20 @@ __global__ __launch_bounds__(BLOCK) void k_gk(const int* __restrict__ lst, const
     const double ib1 = first ? 1.0 / Sc->beta1 : 0.0;
     // v_k = c1 s1 + c2 vp, J^T u_k = d1 s1
     const double c1 = first ? -ib1 * ia : ib * ia, c2 = first ? 0.0 : -beta * ia, d1 = first ? -ib1 : ib;
-    const double* s1 = first ? b_scaled : gp;
-    const int st1 = first ? VP : P;
     const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau, del = first ? 0.0 : Sc->del;
-    const int sub = threadIdx.x % GKL;
     double acc[NP];
 #pragma unroll
     for (int e = 0; e < NP; ++e) acc[e] = 0;
-    const int nl = Sc->n_jtu_stars;
-    for (int li = xcd_remap(blockIdx.x, gridDim.x) * RPB + threadIdx.x / GKL; li < nl; li += gridDim.x * RPB) {
-        const int a = lst[li];
-        const size_t ck = (size_t)a * P + sub;   // this lane's component (sub < P)
-        if (!live[a]) {   // uniform within the group
-            if (first && sub < P) {
-                x[ck] = 0;
-                wv[ck] = 0;
-                if (LSMR) hb[ck] = 0;
-            }
-            continue;
-        }
-        double y[P];
-#pragma unroll
-        for (int k = 0; k < P; ++k) y[k] = 0;
-        if (first) row_apply<P, false, GKL>(a, sub, rowptr, ent, H, s1, st1, c1, vp, c2, y);
-        else row_apply<P, true, GKL>(a, sub, rowptr, ent, H, s1, st1, c1, vp, c2, y);
+    if (row) {   // uniform within the quad
+        double y[P], sd[P], vd[P];
+        if (first) row_apply<P, false>(a, j0 + sub, j1, ent, H, s1, st1, c1, vp, c2, y, sd, vd);
+        else row_apply<P, true>(a, j0 + sub, j1, ent, H, s1, st1, c1, vp, c2, y, sd, vd);
         if (sub < P) {
-            const double yk = pick<P>(y, sub);
-            const double sk = s1[(size_t)a * st1 + sub];
-            const double vk = c1 * sk + (first ? 0.0 : c2 * vp[ck]);
-            const double jk = d1 * sk;
+            const double xo = first ? 0.0 : x[ck];
+            const double wo = first ? 0.0 : wv[ck];
+            const double ho = (LSMR && !first) ? hb[ck] : 0.0;
+            const double yk = pick<P>(y, sub), vk = pick<P>(vd, sub);
+            const double jk = d1 * pick<P>(sd, sub);
             const double gk = yk - alpha * jk;
             gn[ck] = gk;
             vn[ck] = vk;
@@ -336,9 +370,6 @@ __global__ __launch_bounds__(BLOCK) void k_gk(const int* __restrict__ lst, const
             acc[1] += vk * jk;
             acc[2] += gk * gk;
             acc[3] += vk * vk;
-            const double xo = first ? 0.0 : x[ck];
-            const double wo = first ? 0.0 : wv[ck];
-            const double ho = (LSMR && !first) ? hb[ck] : 0.0;
             const double hn = wo - del * ho;
             const double xn = xo + sig * hn;
             const double wn = vk - tau * wo;
@@ -401,39 +432,37 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish_gk(Scal* Sc, const 
 }
 
 // CG on the damped normal equations (J'J + lam^2 I) x = -b_scaled, the
-// direction half: q = H p for the rows in lst (GKL lanes per row, lane k < p
-// owning component k) with p = s + beta p_old formed where it is gathered
-// (the first step: p_0 = s_0 = -b_scaled); each row writes its own p to pn
-// and q.  Partials p.q and |p|^2.
+// direction half: q = H p, a quad of lanes per star row (a quad for every
+// star, lane k < p owning component k) with p = s + beta p_old formed where
+// it is gathered (the first step: p_0 = s_0 = -b_scaled); each live row
+// writes its own p to pn and q.  Partials p.q and |p|^2.
 template <int P>
-__global__ __launch_bounds__(BLOCK) void k_cg_hv(const int* __restrict__ lst, const uint8_t* __restrict__ live,
+__global__ __launch_bounds__(BLOCK) void k_cg_hv(int na, const uint8_t* __restrict__ live,
                                                  const int* __restrict__ rowptr, const int2* __restrict__ ent,
                                                  const double* __restrict__ H, const double* __restrict__ b_scaled,
                                                  const double* __restrict__ s, const double* __restrict__ po,
                                                  double* __restrict__ pn, double* __restrict__ q, int first,
                                                  double* __restrict__ part, Scal* Sc) {
     constexpr int RPB = BLOCK / GKL;
-    if (Sc->done) return;
     __shared__ double sred[2][NWAVE];
-    const double cb = first ? 0.0 : Sc->cg_beta;
+    const int sub = threadIdx.x % GKL;
+    const int a = xcd_remap(blockIdx.x, gridDim.x) * RPB + threadIdx.x / GKL;
     const double* s1 = first ? b_scaled : s;
     const int st1 = first ? VP : P;
+    // one combined row test over loads issued together, as in k_gk
+    const int ac = a < na ? a : na - 1;
+    const size_t ck = (size_t)ac * P + (sub < P ? sub : P - 1);
+    const int j0 = rowptr[ac], j1 = rowptr[ac + 1];
+    const bool row = (a < na) & (live[ac] != 0) & (j1 > j0) & (Sc->done == 0);
+    const double cb = first ? 0.0 : Sc->cg_beta;
     const double c1 = first ? -1.0 : 1.0;
-    const int sub = threadIdx.x % GKL;
     double acc[2] = {0, 0};
-    const int nl = Sc->n_jtu_stars;
-    for (int li = xcd_remap(blockIdx.x, gridDim.x) * RPB + threadIdx.x / GKL; li < nl; li += gridDim.x * RPB) {
-        const int a = lst[li];
-        const size_t ck = (size_t)a * P + sub;
-        if (!live[a]) continue;
-        double y[P];
-#pragma unroll
-        for (int k = 0; k < P; ++k) y[k] = 0;
-        if (first) row_apply<P, false, GKL>(a, sub, rowptr, ent, H, s1, st1, c1, po, cb, y);
-        else row_apply<P, true, GKL>(a, sub, rowptr, ent, H, s1, st1, c1, po, cb, y);
+    if (row) {
+        double y[P], sd[P], pd[P];
+        if (first) row_apply<P, false>(a, j0 + sub, j1, ent, H, s1, st1, c1, po, cb, y, sd, pd);
+        else row_apply<P, true>(a, j0 + sub, j1, ent, H, s1, st1, c1, po, cb, y, sd, pd);
         if (sub < P) {
-            const double yk = pick<P>(y, sub);
-            const double pk = c1 * s1[(size_t)a * st1 + sub] + (first ? 0.0 : cb * po[ck]);
+            const double yk = pick<P>(y, sub), pk = pick<P>(pd, sub);
             pn[ck] = pk;
             q[ck] = yk;
             acc[0] += pk * yk;

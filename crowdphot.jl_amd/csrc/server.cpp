@@ -78,11 +78,12 @@ extern "C" uint64_t cp_request_bytes(int32_t ny, int32_t nx, int32_t n_stars, do
     b += P * 8;                                               // row-major inputs transposed on the device
     if (!device_inputs) b += P * 8 + n * 24;                 // staged image, weights, catalogue
     b += E * 4 * T;                                           // stamp Jacobian rows
-    // star-pair rows of the normal matrix (neighbour, block slot and a 3 x 3
-    // float64 block per pair): pairs per star at eight times the mean density
-    // of anchors within 2R, and at least 0.3 S^2
+    // star-pair rows of the normal matrix (per entry: neighbour, the (neighbour,
+    // slot) word and a 3 x 3 float64 block; per upper entry a 16-byte record;
+    // build-time scratch): pairs per star at eight times the mean density of
+    // anchors within 2R, and at least 0.3 S^2
     const double K = std::min(n, std::max(0.3 * S2, 1 + 8 * n * (4 * R + 1) * (4 * R + 1) / P));
-    b += n * K * 80;
+    b += n * K * 100;
     const double tiles = std::pow((2 * mR + 1) / 64.0 + 1.0, 2);
     b += n * tiles * 16 + 4 * P / 4096 * 24;                  // model-box tile lists, live copies, scans
     b += n * 1100 + (8 << 20);                                // per-star records and vectors, partials, results

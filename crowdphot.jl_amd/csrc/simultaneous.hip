@@ -117,8 +117,8 @@ void build_tiles(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
 // the stars whose stamps overlap its own, from a binning into cells of side 2R.
 struct PairRows {
     DevBuf rowptr, pb;
-    DevBuf rowof, rev, ulist;   // per entry: its row, the twin of an upper entry; the upper entries
-    DevBuf ent;                 // per entry, per outer iteration: (live neighbour or -1, block slot)
+    DevBuf urec;   // per upper entry (b >= a): (slot, row, neighbour, twin entry of the other row)
+    DevBuf ent;    // per entry, per outer iteration: (live neighbour or -1, block slot)
     int64_t npairs = 0;
     int n_upper = 0;
 };
@@ -173,24 +173,21 @@ void build_pairs(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
            (const int*)P.rowptr.as<int>(), P.pb.as<int>());
     // each pair's block is kept once, in its upper entry (b >= a)
     const size_t ne = (size_t)std::max<int64_t>(P.npairs, 1);
-    P.rowof.alloc(ctx, ne * 4);
-    P.rev.alloc(ctx, ne * 4);
-    P.ulist.alloc(ctx, ne * 4);
+    DevBuf rowof(ctx, ne * 4), rev(ctx, ne * 4), ulist(ctx, ne * 4), upper(ctx, ne), nup(ctx, 4);
     P.ent.alloc(ctx, ne * 8);
-    DevBuf upper(ctx, ne), nup(ctx, 4);
     CP_HIP(hipMemsetAsync(cnt.get(), 0, 4, st));
     CP_HIP(hipMemsetAsync(upper.get(), 0, ne, st));
     launch(k_pair_rev, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, (const int*)P.rowptr.as<int>(),
-           (const int*)P.pb.as<int>(), P.rowof.as<int>(), upper.as<uint8_t>(), P.rev.as<int>(), cnt.as<int>());
+           (const int*)P.pb.as<int>(), rowof.as<int>(), upper.as<uint8_t>(), rev.as<int>(), cnt.as<int>());
     {
         hipcub::CountingInputIterator<int> cit(0);
         size_t b3 = 0;
-        CP_HIP(hipcub::DeviceSelect::Flagged(nullptr, b3, cit, (const uint8_t*)upper.as<uint8_t>(), P.ulist.as<int>(),
+        CP_HIP(hipcub::DeviceSelect::Flagged(nullptr, b3, cit, (const uint8_t*)upper.as<uint8_t>(), ulist.as<int>(),
                                              nup.as<int>(), np32, st));
         cub.ensure(b3);
         b3 = cub.tmp.bytes();
         CP_HIP(hipcub::DeviceSelect::Flagged(cub.tmp.get(), b3, cit, (const uint8_t*)upper.as<uint8_t>(),
-                                             P.ulist.as<int>(), nup.as<int>(), np32, st));
+                                             ulist.as<int>(), nup.as<int>(), np32, st));
     }
     int bad = 0;
     CP_HIP(hipMemcpyAsync(&bad, cnt.get(), 4, hipMemcpyDeviceToHost, st));
@@ -198,6 +195,10 @@ void build_pairs(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
     CP_HIP(hipStreamSynchronize(st));   // also: the host word copied to rowptr[na] is a local
     if (bad || 2 * (int64_t)P.n_upper != P.npairs + na)
         throw Error(CP_ESTATE, "star pair rows are not symmetric");
+    P.urec.alloc(ctx, (size_t)P.n_upper * 16);
+    launch(k_upper_rec, dim3(cdiv(P.n_upper, BLOCK)), dim3(BLOCK), 0, st, P.n_upper, (const int*)ulist.as<int>(),
+           (const int*)rowof.as<int>(), (const int*)P.pb.as<int>(), (const int*)rev.as<int>(), P.urec.as<int4>());
+    CP_HIP(hipStreamSynchronize(st));   // the local buffers above are released on return
 }
 
 static int bit_length(int64_t v) {
@@ -589,16 +590,15 @@ struct Sim {
         // (blocks per CU: the context's tuned launch shapes, else these defaults)
         const cp_tuning& tu = ctx->tuning;
         const int gb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * tuned_bpc(tu.render_blocks_per_cu, 4));
-        // the Golub-Kahan step and CG's H p: GKL lanes per row; CG's update:
-        // one thread per star
-        const int gb_jv = (int)std::max<int64_t>(
-            1, std::min<int64_t>(cdiv(na, BLOCK / GKL), (int64_t)n_cu * tuned_bpc(tu.jv_blocks_per_cu, 8)));
+        // the Golub-Kahan step and CG's H p: GKL lanes per row, a group for
+        // every star; CG's update: one thread per star
+        const int gb_jv = (int)std::max<int64_t>(1, cdiv(na, BLOCK / GKL));
         const int gb_jtu = (int)std::max<int64_t>(
-            1, std::min<int64_t>(cdiv(na, BLOCK), (int64_t)n_cu * tuned_bpc(tu.jtu_blocks_per_cu, 8)));
+            1, std::min<int64_t>(cdiv(na, BLOCK), (int64_t)n_cu * tuned_bpc(tu.cg_blocks_per_cu, 8)));
         // the H assembly: GHh lanes per stored block
         const int GHh = S2 <= 64 ? 4 : (S2 <= 256 ? 16 : 64);
         const int gb_hb = (int)std::max<int64_t>(
-            1, std::min<int64_t>(cdiv(prs.n_upper, BLOCK / GHh), (int64_t)n_cu * 32));
+            1, std::min<int64_t>(cdiv(prs.n_upper, BLOCK / GHh), (int64_t)n_cu * tuned_bpc(tu.hbuild_blocks_per_cu, 32)));
         const int gb_fill = (int)std::min<int64_t>(cdiv(na, FBLOCK / 64), (int64_t)n_cu * 2);
         const unsigned int gstar = cdiv(na, BLOCK);
         // per-block partials of the widest launch (two sets for fill / accept)
@@ -689,10 +689,9 @@ struct Sim {
             with_p(p, [&](auto pc) {
                 constexpr int PV = decltype(pc)::value;
 #define CP_HB(GH)                                                                                                  \
-    launch_n("k_hbuild", k_hbuild<VT, GH, PV>, dim3(gb_hb), dim3(BLOCK), 0, st, (const int*)prs.ulist.as<int>(),     \
-             prs.n_upper, (const int*)prs.rowof.as<int>(), (const int*)prs.pb.as<int>(),                              \
-             (const int*)prs.rev.as<int>(), (const uint8_t*)live.as<uint8_t>(), (const int*)ay.as<int>(),             \
-             (const int*)ax.as<int>(), R, S, (const VT*)V.as<VT>(), Hblk.as<double>(), prs.ent.as<int2>())
+    launch_n("k_hbuild", k_hbuild<VT, GH, PV>, dim3(gb_hb), dim3(BLOCK), 0, st, (const int4*)prs.urec.as<int4>(),    \
+             prs.n_upper, (const uint8_t*)live.as<uint8_t>(), (const int*)ay.as<int>(), (const int*)ax.as<int>(), R,   \
+             S, (const VT*)V.as<VT>(), Hblk.as<double>(), prs.ent.as<int2>())
                 if (GHh == 4) CP_HB(4);
                 else if (GHh == 16) CP_HB(16);
                 else CP_HB(64);
@@ -766,7 +765,7 @@ struct Sim {
                                 // pcur, the previous one is read from pold
                                 double* pcur = (it & 1) ? vv1.as<double>() : vv0.as<double>();
                                 double* pold = (it & 1) ? vv0.as<double>() : vv1.as<double>();
-                                launch_n("k_cg_hv", k_cg_hv<PV>, dim3(gb_jv), dim3(BLOCK), 0, st, jl, lv, rp, en, Hp, bs,
+                                launch_n("k_cg_hv", k_cg_hv<PV>, dim3(gb_jv), dim3(BLOCK), 0, st, na, lv, rp, en, Hp, bs,
                                          (const double*)gv0.as<double>(), (const double*)pold, pcur, gv1.as<double>(),
                                          first, partk.as<double>(), Sc);
                                 launch(k_finish<FIN_CGQ>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc,
@@ -787,7 +786,7 @@ struct Sim {
                             DevBuf& vnew = (it & 1) ? vv1 : vv0;
                             auto gk = [&](auto sv) {
                                 constexpr int SV = decltype(sv)::value;
-                                launch_n("k_gk", k_gk<PV, SV>, dim3(gb_jv), dim3(BLOCK), 0, st, jl, lv, rp, en, Hp, bs,
+                                launch_n("k_gk", k_gk<PV, SV>, dim3(gb_jv), dim3(BLOCK), 0, st, na, lv, rp, en, Hp, bs,
                                          (const double*)gprev.as<double>(), (const double*)vprev.as<double>(),
                                          gnew.as<double>(), vnew.as<double>(), xs.as<double>(), wl.as<double>(),
                                          hbar.as<double>(), first, partk.as<double>(), Sc);

@@ -140,7 +140,7 @@ end
 
 # Launch shapes of a context (cp_tuning: blocks per CU, 0 = built-in default).
 mutable struct CpTuning
-    jv_blocks_per_cu::Int32; jtu_blocks_per_cu::Int32; render_blocks_per_cu::Int32; lm_blocks_per_cu::Int32
+    hbuild_blocks_per_cu::Int32; cg_blocks_per_cu::Int32; render_blocks_per_cu::Int32; lm_blocks_per_cu::Int32
 end
 CpTuning() = CpTuning(0, 0, 0, 0)
 

@@ -177,12 +177,12 @@ int cp_profile_report(cp_ctx* ctx, char* buf, int32_t cap);
 /* Launch shapes, no reference counterpart: blocks per compute unit of the
  * grid-stride kernels (0 = the built-in default).  The autotuner
  * (crowdphot.jl_amd.autotune) picks them by timing whole calls.  They change
- * no result of the batched per-star LM; in the simultaneous fit they regroup
- * the per-block partial sums of the LSQR norms (rounding-level differences,
- * fixed for a fixed shape).  Applies to calls made after cp_set_tuning. */
+ * no result of the batched per-star LM or of the normal-matrix assembly; the
+ * CG update's shape regroups the per-block partial sums of CG's norms
+ * (rounding-level differences, fixed for a fixed shape).  Applies to calls made after cp_set_tuning. */
 typedef struct cp_tuning {
-    int32_t jv_blocks_per_cu;      /* J v gather, 256-thread blocks (default 8) */
-    int32_t jtu_blocks_per_cu;     /* J^T u per star, 256-thread blocks (default 8) */
+    int32_t hbuild_blocks_per_cu;  /* normal-matrix assembly, 256-thread blocks (default 32) */
+    int32_t cg_blocks_per_cu;      /* CG step update per star, 256-thread blocks (default 8) */
     int32_t render_blocks_per_cu;  /* tile-gather renders, 1024-thread blocks (default 4) */
     int32_t lm_blocks_per_cu;      /* batched per-star LM, 256-thread blocks (default 8) */
 } cp_tuning;

@@ -12,7 +12,7 @@ at = pkg.autotune
 
 class FakeCtx:
     def __init__(self):
-        self.t = {k: 0 for k in ("jv_blocks_per_cu", "jtu_blocks_per_cu", "render_blocks_per_cu",
+        self.t = {k: 0 for k in ("hbuild_blocks_per_cu", "cg_blocks_per_cu", "render_blocks_per_cu",
                                  "lm_blocks_per_cu")}
         self.history = []
 
@@ -33,11 +33,11 @@ def _run_for(ctx, best):
 
 def test_coordinate_descent_finds_the_known_optimum():
     ctx = FakeCtx()
-    best = {"jv_blocks_per_cu": 16, "jtu_blocks_per_cu": 4, "render_blocks_per_cu": 4}
+    best = {"hbuild_blocks_per_cu": 16, "render_blocks_per_cu": 2}
     shape, t = at.tune(ctx, _run_for(ctx, best), repeats=2)
     for k, v in best.items():
         assert shape[k] == v, (k, shape)
-    assert ctx.get_tuning()["jv_blocks_per_cu"] == 16   # left set on the context
+    assert ctx.get_tuning()["hbuild_blocks_per_cu"] == 16   # left set on the context
     assert t < 0.01
 
 
@@ -52,7 +52,7 @@ def test_defaults_are_the_starting_point_and_kept_without_gain():
 def test_cache_round_trip(tmp_path):
     path = str(tmp_path / "tuning.json")
     ctx = FakeCtx()
-    best = {"jv_blocks_per_cu": 4, "jtu_blocks_per_cu": 16, "render_blocks_per_cu": 8}
+    best = {"hbuild_blocks_per_cu": 8, "render_blocks_per_cu": 8}
     shape = at.cached(ctx, "crowded/4096", _run_for(ctx, best), path)
     assert {k: shape[k] for k in best} == best
     assert json.load(open(path))["crowded/4096"] == shape
@@ -64,7 +64,7 @@ def test_cache_round_trip(tmp_path):
 
 def test_failing_call_restores_the_shape():
     ctx = FakeCtx()
-    ctx.set_tuning(jv_blocks_per_cu=16)
+    ctx.set_tuning(hbuild_blocks_per_cu=16)
     before = ctx.get_tuning()
     n = []
 

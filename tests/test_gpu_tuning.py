@@ -21,9 +21,9 @@ def ctx():
 
 def test_tuning_round_trip_and_range(ctx):
     assert set(ctx.get_tuning().values()) == {0}
-    ctx.set_tuning(jv_blocks_per_cu=4, lm_blocks_per_cu=2)
+    ctx.set_tuning(hbuild_blocks_per_cu=4, lm_blocks_per_cu=2)
     t = ctx.get_tuning()
-    assert t["jv_blocks_per_cu"] == 4 and t["lm_blocks_per_cu"] == 2 and t["jtu_blocks_per_cu"] == 0
+    assert t["hbuild_blocks_per_cu"] == 4 and t["lm_blocks_per_cu"] == 2 and t["cg_blocks_per_cu"] == 0
     with pytest.raises(pkg.ArgumentError, match="blocks per CU"):
         ctx.set_tuning(render_blocks_per_cu=65)
     with pytest.raises(pkg.ArgumentError, match="unknown tuning"):
@@ -54,8 +54,8 @@ def test_simultaneous_under_other_shapes_matches_oracle(ctx):
     n = int(round(2e5 * (384 * 384) / 4096 ** 2))
     crowd = sim.config_epsf_crowded(seed=5, shape=(384, 384), n=n, max_iter=10)
     ref = _gpu_run(ctx, crowd, "mixed")
-    for shape in ({"jv_blocks_per_cu": 1, "jtu_blocks_per_cu": 16, "render_blocks_per_cu": 1},
-                  {"jv_blocks_per_cu": 32, "jtu_blocks_per_cu": 2, "render_blocks_per_cu": 8}):
+    for shape in ({"hbuild_blocks_per_cu": 1, "cg_blocks_per_cu": 16, "render_blocks_per_cu": 1},
+                  {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 2, "render_blocks_per_cu": 8}):
         ctx.set_tuning(**shape)
         _compare(_gpu_run(ctx, cfg, "fp64"), ro, 1e-7, 1e-7)
         _compare(_gpu_run(ctx, cfg, "mixed"), ro, FLUX_RTOL, POS_ATOL)
