@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--stars", type=int, default=200_000)
     ap.add_argument("--profile-only", action="store_true", help="one fit, for rocprofv3 runs")
+    ap.add_argument("--profile-fits", type=int, default=1,
+                    help="fits run by --profile-only (the last one is the steady state)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving headline (the c2 fit becomes it)")
     ap.add_argument("--serve-per-step", type=int, default=64, help="requests per timed serving step")
     ap.add_argument("--serve-open", type=int, default=1000, help="open-loop requests for p50 / p99 latency")
@@ -237,9 +239,10 @@ def main():
         b_cal.copy_(a_cal)
         torch.cuda.synchronize()
         del a_cal, b_cal
-        fit(o32, s32)
+        for _ in range(max(args.profile_fits, 1)):
+            fit(o32, s32)
         torch.cuda.synchronize()
-        log(f"[bench] profile-only fit done: passes {s32.n_passes}")
+        log(f"[bench] profile-only fits done: passes {s32.n_passes}")
         return
     tuning = None
     if not args.no_autotune:

@@ -435,17 +435,20 @@ __device__ void grid_corners(const NodeGrid& G, int n_nodes, double Y, double X,
     }
 }
 
-// StarRec (and the single-node ePSF constants) from theta.  frozen_iter >= 1
+// StarRec (and the single-node ePSF constants) from theta, for the stars of
+// lst (Sc->n_jtu_stars of them) or, with lst null, for all.  frozen_iter >= 1
 // marks stars frozen at that iteration.
 template <typename T, int KIND>
-static __global__ void k_star_rec(int na, const double* __restrict__ theta, FreeMap fm, ModelConst<T> M,
+static __global__ void k_star_rec(int na, const int* __restrict__ lst, const Scal* Sc,
+                                  const double* __restrict__ theta, FreeMap fm, ModelConst<T> M,
                                   double inv_norm_g64, double moffat_amp64, const int* __restrict__ ay,
                                   const int* __restrict__ ax, const int* __restrict__ mR,
                                   const uint8_t* __restrict__ live, const int* __restrict__ frozen_at,
                                   int frozen_iter, int all_live, StarRec<T>* __restrict__ recs, NodeGrid G,
                                   CornerRec<T>* __restrict__ crs, EpsfRec<T>* __restrict__ ers) {
-    int a = blockIdx.x * BLOCK + threadIdx.x;
-    if (a >= na) return;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= (lst ? Sc->n_jtu_stars : na)) return;
+    const int a = lst ? lst[i] : i;
     double Y, X, F;
     star_yxf(fm, theta + (size_t)a * fm.p, Y, X, F);
     StarRec<T> r;
@@ -464,9 +467,40 @@ static __global__ void k_star_rec(int na, const double* __restrict__ theta, Free
     r.flags = fl;
     recs[a] = r;
     if (KIND == KIND_EPSF) {
-        if (M.n_nodes > 1) grid_corners<T>(G, M.n_nodes, Y, X, crs[a]);
-        else if (ers) epsf_rec<T>(M, fy, fx, ers[a]);
+        if (M.n_nodes > 1) {
+            grid_corners<T>(G, M.n_nodes, Y, X, crs[a]);
+        } else if (ers) {
+            EpsfRec<T> e;   // built in registers, stored whole
+            epsf_rec<T>(M, fy, fx, e);
+            ers[a] = e;
+        }
     }
+}
+
+// Tiles touched by the model box of a star of lst that froze at iteration
+// iter get mark iter (plain stores of the same value; marks of earlier
+// iterations need no clearing): the freeze render visits only those.
+static __global__ void k_mark_frozen(const int* __restrict__ lst, const Scal* Sc, const uint8_t* __restrict__ live,
+                                     const int* __restrict__ frozen_at, int iter, const int* __restrict__ ay,
+                                     const int* __restrict__ ax, const int* __restrict__ mR, TileGeo g,
+                                     int* __restrict__ tmark) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= Sc->n_jtu_stars) return;
+    const int a = lst[i];
+    if (live[a] || frozen_at[a] != iter) return;
+    int ty0, ty1, tx0, tx1;
+    if (!tile_range(g, ay[a], ax[a], mR[a], ty0, ty1, tx0, tx1)) return;
+    for (int tx = tx0; tx <= tx1; ++tx)
+        for (int ty = ty0; ty <= ty1; ++ty) tmark[tx * g.nty + ty] = iter;
+}
+
+// The device scalars copied to pinned host memory (vector stores; read by
+// the host after a stream synchronize).
+static __global__ void k_mirror(const Scal* __restrict__ Sc, Scal* __restrict__ hS) {
+    static_assert(sizeof(Scal) % 8 == 0, "Scal copied as 8-byte words");
+    const unsigned long long* s = reinterpret_cast<const unsigned long long*>(Sc);
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(hS);
+    for (int i = threadIdx.x; i < (int)(sizeof(Scal) / 8); i += blockDim.x) d[i] = s[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -535,7 +569,8 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     const CornerRec<T>* __restrict__ crs, const int* __restrict__ tstart, const int* __restrict__ tend,
     const int* __restrict__ list, TileGeo g, int ntiles, const long long* __restrict__ tscan,
     const uint8_t* __restrict__ umask, T* __restrict__ dw, const T* __restrict__ w, const T* __restrict__ data,
-    T* __restrict__ wt_resid, T* __restrict__ resid, double* __restrict__ partials, Scal* S) {
+    T* __restrict__ wt_resid, T* __restrict__ resid, double* __restrict__ partials, Scal* S,
+    const int* __restrict__ tmark, int mark_iter) {
     constexpr bool EPSF_LDS = KIND == KIND_EPSF && LDSG;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double sred[RWAVE];
@@ -556,6 +591,8 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     int t0, t1;
     tile_share(tscan, ntiles, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, t0, t1);
     for (int tile = t0; tile < t1; ++tile) {
+        // the freeze render: only tiles a newly frozen star touches
+        if (MODE == MODE_FREEZE && tmark[tile] != mark_iter) continue;
         const int ty = tile % g.nty, tx = tile / g.nty;
         const int ty0 = ty * RT + 1, tx0 = tx * RT + 1;       // tile origin (1-based)
         const int gy0 = ty0 + sbr, gx0 = tx0 + sbc;           // this wave's sub-block origin

@@ -336,16 +336,16 @@ struct Sim {
     void render_launch(bool ldsg, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
                        const EpsfRec<T>* ers, const CornerRec<T>* crs, const TileLists& L, const TileGeo& g,
                        int ntiles, const long long* tscan, const uint8_t* umask, T* dw, const T* w, const T* data,
-                       T* wres, T* resid, double* part, Scal* Sc) {
+                       T* wres, T* resid, double* part, Scal* Sc, const int* tmark = nullptr, int mark_iter = 0) {
         static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
         if (ldsg)
             launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc);
+                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
         else
             launch_n(nm[MODE], k_render<T, KIND, MODE, false>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc);
+                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
     }
 
     void run(const float* d_img, const float* d_iv, int ny, int nx, const double* d_y0, const double* d_x0,
@@ -416,8 +416,10 @@ struct Sim {
         Scal* Sc = dS.as<Scal>();
         Scal* hS = static_cast<Scal*>(pinned(ctx, sizeof(Scal)));
         CP_HIP(hipMemsetAsync(Sc, 0, sizeof(Scal), st));
+        // the scalars reach the host through a one-block copy kernel into the
+        // pinned mirror (cheaper than a copy-engine transfer of a few hundred bytes)
         auto read_S = [&]() {
-            CP_HIP(hipMemcpyAsync(hS, Sc, sizeof(Scal), hipMemcpyDeviceToHost, st));
+            launch(k_mirror, dim3(1), dim3(64), 0, st, (const Scal*)Sc, hS);
             CP_HIP(hipStreamSynchronize(st));
         };
 
@@ -608,8 +610,11 @@ struct Sim {
         // the Golub-Kahan step's partials (ten per block for LSMR, five for LSQR)
         DevBuf partk(ctx, (size_t)10 * std::max(gb_jv, gb_jtu) * 8);
 
-        auto star_rec = [&](const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter, int all_live) {
-            launch(k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, th, fm, M, inv_norm_g64, moffat_amp64,
+        // star records of the stars in lst (null: all)
+        auto star_rec = [&](const int* lst, const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter,
+                            int all_live) {
+            launch(k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, lst, (const Scal*)Sc, th, fm, M,
+                   inv_norm_g64, moffat_amp64,
                    (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const int*)mR.as<int>(),
                    (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), frozen_iter, all_live,
                    r.as<StarRec<T>>(), DP.G, c.as<CornerRec<T>>(), single_epsf ? e.as<EpsfRec<T>>() : nullptr);
@@ -624,6 +629,9 @@ struct Sim {
         Llive.list.alloc(ctx, (size_t)std::max(Lmod.npairs, 1) * 4);
         DevBuf tflag(ctx, (size_t)(Lmod.npairs + 1) * 4), tpos(ctx, (size_t)(Lmod.npairs + 1) * 4);
         DevBuf tcost_l(ctx, (size_t)ntiles * 8), tscan_l(ctx, (size_t)ntiles * 8);
+        // per tile: the last iteration at which a star touching it froze
+        DevBuf tmark(ctx, (size_t)ntiles * 4);
+        CP_HIP(hipMemsetAsync(tmark.get(), 0, (size_t)ntiles * 4, st));
         auto fill = [&](bool final_mode) {
             const StarRec<T>* rp = recs.as<StarRec<T>>();
             const EpsfRec<T>* ep = ers.as<EpsfRec<T>>();
@@ -714,7 +722,7 @@ struct Sim {
                 compact_live();
                 live_compacted = n_live_now;
             }
-            star_rec(theta.as<double>(), recs, crs, ers, 0, 0);
+            star_rec(jlist.as<int>(), theta.as<double>(), recs, crs, ers, 0, 0);
             if (!have_resid) {
                 render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
                                           ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, (const long long*)tscan.as<long long>(),
@@ -730,19 +738,13 @@ struct Sim {
                 CP_HIP(hipMemcpyAsync(D.get(), colnorm.get(), nb8, hipMemcpyDeviceToDevice, st));
                 D_init = true;
             }
-            read_S();
-            cost = hS->cost;
-            double gnorm;
-            {
-                unsigned long long bits = hS->gmax_bits;
-                std::memcpy(&gnorm, &bits, 8);
-            }
-            if (gnorm <= o.g_tol) {
-                launch(k_freeze_all, dim3(gstar), dim3(BLOCK), 0, st, na, iter - 1, live.as<uint8_t>(),
-                       frozen_at.as<int>(), conv.as<uint8_t>());
-                n_run = iter - 1;
-                break;
-            }
+            // The cost and the gradient cosine of this iteration are read with
+            // the first trial's result: the trial is launched without waiting
+            // for them and discarded when the g test stops the fit (a trial
+            // changes none of the state the stop keeps: theta, the residual,
+            // the live set).
+            double gnorm = 0;
+            bool g_stop = false;
             bool accepted = false, f_conv = false;
             for (int trial = 1; trial <= o.max_trials; ++trial) {
                 const double lam_solve = std::sqrt(lam);
@@ -803,7 +805,7 @@ struct Sim {
                            (const double*)colnorm.as<double>(),
                            (const double*)theta.as<double>(), dsc.as<double>(), theta_c.as<double>(),
                            (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
-                    star_rec(theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
+                    star_rec(jlist.as<int>(), theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
                     render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
                                              ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Llive, g, ntiles, (const long long*)tscan_l.as<long long>(),
                                              umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cand,
@@ -813,6 +815,15 @@ struct Sim {
                 if (use_graphs) graphs.run(st, wres_cur, trial_body);
                 else trial_body();
                 read_S();
+                if (trial == 1) {
+                    cost = hS->cost;
+                    const unsigned long long bits = hS->gmax_bits;
+                    std::memcpy(&gnorm, &bits, 8);
+                    if (gnorm <= o.g_tol) {
+                        g_stop = true;
+                        break;
+                    }
+                }
                 const double cost_c = hS->cost_cand;
                 out->n_trials_total += 1;
                 out->n_inner_total += hS->iter;
@@ -837,6 +848,12 @@ struct Sim {
                 }
                 lam = std::min(lam * o.lambda_up, o.lambda_max);
             }
+            if (g_stop) {
+                launch(k_freeze_all, dim3(gstar), dim3(BLOCK), 0, st, na, iter - 1, live.as<uint8_t>(),
+                       frozen_at.as<int>(), conv.as<uint8_t>());
+                n_run = iter - 1;
+                break;
+            }
             n_run = iter;
             if (f_conv) {
                 // theta <- theta + delta for the accepted step, then freeze everyone
@@ -859,12 +876,17 @@ struct Sim {
                 read_S();
                 n_live_now = hS->n_live;
                 if (hS->n_frozen_now > 0) {
-                    star_rec(theta.as<double>(), recs, crs, ers, iter, 0);
+                    // newly frozen stars are in the live list of the last compaction
+                    star_rec(jlist.as<int>(), theta.as<double>(), recs, crs, ers, iter, 0);
+                    launch(k_mark_frozen, dim3(gstar), dim3(BLOCK), 0, st, (const int*)jlist.as<int>(), (const Scal*)Sc,
+                           (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), iter,
+                           (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const int*)mR.as<int>(), g,
+                           tmark.as<int>());
                     render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
                                                ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Llive, g, ntiles,
                                                (const long long*)tscan_l.as<long long>(), umask.as<uint8_t>(),
                                                dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
-                                               part.as<double>(), Sc);
+                                               part.as<double>(), Sc, (const int*)tmark.as<int>(), iter);
                 }
                 if (hS->n_live == 0) break;
             }
@@ -872,7 +894,7 @@ struct Sim {
 
         // --- epilogue: errors (:1302-1329), final render and chisq (:1344-1397)
         const int cov_mode = o.covariance == CP_COV_DEFAULT ? (d_iv ? CP_COV_KNOWN : CP_COV_REWEIGHTED) : o.covariance;
-        star_rec(theta.as<double>(), recs, crs, ers, 0, 1);
+        star_rec(nullptr, theta.as<double>(), recs, crs, ers, 0, 1);
         fill(true);
         DevBuf errs(ctx, nb8);
         CP_HIP(hipMemsetAsync(errs.get(), 0, nb8, st));
