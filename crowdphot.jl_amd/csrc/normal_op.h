@@ -391,18 +391,27 @@ __global__ __launch_bounds__(BLOCK) void k_gk(int na, const uint8_t* __restrict_
 
 // One-block sum of k_gk's partials (block order) and the step's scalars:
 // beta_{k+1}, alpha_{k+1} by the expansion above, then the LSQR or LSMR
-// recurrences of step it.
+// recurrences of step it.  The scalar state is staged in LDS while the
+// partials load, so the serial recurrences read and write LDS, not global
+// memory, and it is written back whole at the end.
+// (256 threads: the LSMR recurrences need more than the 128 registers a
+// 1024-thread block leaves each lane)
+constexpr int FINGK_BLOCK = 256;
 template <int SOLVER>
-static __global__ __launch_bounds__(FIN_BLOCK) void k_finish_gk(Scal* Sc, const double* __restrict__ part, int nb,
-                                                                int it) {
+static __global__ __launch_bounds__(FINGK_BLOCK) void k_finish_gk(Scal* Sc, const double* __restrict__ part, int nb,
+                                                                  int it) {
     constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
-    constexpr int NP = LSMR ? 10 : 5;
-    if (Sc->done) return;
-    __shared__ double sh[NP][FIN_BLOCK / 64];
+    constexpr int NP = LSMR ? 10 : 5, NW = sizeof(Scal) / 8;
+    static_assert(sizeof(Scal) % 8 == 0 && NW <= FINGK_BLOCK, "Scal staged as 8-byte words");
+    __shared__ Scal Ls;
+    __shared__ double sh[NP][FINGK_BLOCK / 64];
+    unsigned long long* lw = reinterpret_cast<unsigned long long*>(&Ls);
+    unsigned long long* gw = reinterpret_cast<unsigned long long*>(Sc);
+    if (threadIdx.x < NW) lw[threadIdx.x] = gw[threadIdx.x];
     double a[NP];
 #pragma unroll
     for (int e = 0; e < NP; ++e) a[e] = 0;
-    for (int i = threadIdx.x; i < nb; i += FIN_BLOCK)
+    for (int i = threadIdx.x; i < nb; i += FINGK_BLOCK)
 #pragma unroll
         for (int e = 0; e < NP; ++e) a[e] += part[(size_t)e * nb + i];
 #pragma unroll
@@ -411,24 +420,28 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish_gk(Scal* Sc, const 
 #pragma unroll
         for (int e = 0; e < NP; ++e) sh[e][threadIdx.x >> 6] = a[e];
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    double t[NP];
-    for (int e = 0; e < NP; ++e) {
-        t[e] = 0;
-        for (int i = 0; i < FIN_BLOCK / 64; ++i) t[e] += sh[e][i];
+    if (Ls.done) return;   // uniform
+    if (threadIdx.x == 0) {
+        double t[NP];
+        for (int e = 0; e < NP; ++e) {
+            t[e] = 0;
+            for (int i = 0; i < FINGK_BLOCK / 64; ++i) t[e] += sh[e][i];
+        }
+        const double al = Ls.alpha;
+        const double beta = sqrt(fmax(0.0, t[0] - 2 * al * t[1] + al * al));
+        Ls.beta_next = beta;
+        const double gv = t[0] - al * t[1];
+        const double tt = beta > 0 ? fmax(0.0, t[2] / (beta * beta) - 2 * gv + beta * beta * t[3]) : 0.0;
+        if (LSMR) {
+            // |x|^2, x.h, x.hbar, |h|^2, h.hbar, |hbar|^2
+            const double q[6] = {t[5], t[6], t[7], t[4], t[8], t[9]};
+            lsmr_advance(&Ls, tt, q, it);
+        } else {
+            lsqr_advance(&Ls, tt, t[4], it);
+        }
     }
-    const double al = Sc->alpha;
-    const double beta = sqrt(fmax(0.0, t[0] - 2 * al * t[1] + al * al));
-    Sc->beta_next = beta;
-    const double gv = t[0] - al * t[1];
-    const double tt = beta > 0 ? fmax(0.0, t[2] / (beta * beta) - 2 * gv + beta * beta * t[3]) : 0.0;
-    if (LSMR) {
-        // |x|^2, x.h, x.hbar, |h|^2, h.hbar, |hbar|^2
-        const double q[6] = {t[5], t[6], t[7], t[4], t[8], t[9]};
-        lsmr_advance(Sc, tt, q, it);
-    } else {
-        lsqr_advance(Sc, tt, t[4], it);
-    }
+    __syncthreads();
+    if (threadIdx.x < NW) gw[threadIdx.x] = lw[threadIdx.x];
 }
 
 // CG on the damped normal equations (J'J + lam^2 I) x = -b_scaled, the

@@ -56,9 +56,8 @@ constexpr int EPSF_PAD = 2;
 // [dxlo, dxhi] are outside the grid's closed range (value fill); ok = 0 when
 // no pixel is inside.
 template <typename T>
-struct EpsfRec {
+struct alignas(16) EpsfRec {   // 16-byte aligned: stored and loaded as whole 16-byte words
     T wy[4], wx[4], dwy[4], dwx[4];
-    int ly, lx;      // floor(v), floor(u) at the anchor pixel (1-based grid coordinates)
     int ra[4], ca[4];
     int dylo, dyhi, dxlo, dxhi, ok;
 };
@@ -139,8 +138,7 @@ __device__ __forceinline__ void epsf_rec(const ModelConst<T>& M, double fy, doub
         u0 = -1e8;
     }
     const double fly = floor(v0), flx = floor(u0);
-    e.ly = (int)fly;
-    e.lx = (int)flx;
+    const int ly = (int)fly, lx = (int)flx;   // floor(v), floor(u) at the anchor pixel (1-based grid coordinates)
     T wd[4];
     keys_weights<T>((T)(v0 - fly), e.wy, wd);
     for (int a = 0; a < 4; ++a) e.dwy[a] = wd[a];
@@ -149,20 +147,20 @@ __device__ __forceinline__ void epsf_rec(const ModelConst<T>& M, double fy, doub
     // pixel offsets whose floor(v) is inside the closed range: [1, gny - 1],
     // or gny exactly when v = gny (t = 0, wy[1] = 1)
     const int osy = M.ios_y, osx = M.ios_x;
-    e.dylo = ceil_div(1 - e.ly, osy);
-    e.dyhi = (e.wy[1] == T(1) && floor_div(M.gny - e.ly, osy) * osy == M.gny - e.ly) ? floor_div(M.gny - e.ly, osy)
-                                                                                     : floor_div(M.gny - 1 - e.ly, osy);
-    e.dxlo = ceil_div(1 - e.lx, osx);
-    e.dxhi = (e.wx[1] == T(1) && floor_div(M.gnx - e.lx, osx) * osx == M.gnx - e.lx) ? floor_div(M.gnx - e.lx, osx)
-                                                                                     : floor_div(M.gnx - 1 - e.lx, osx);
+    e.dylo = ceil_div(1 - ly, osy);
+    e.dyhi = (e.wy[1] == T(1) && floor_div(M.gny - ly, osy) * osy == M.gny - ly) ? floor_div(M.gny - ly, osy)
+                                                                                 : floor_div(M.gny - 1 - ly, osy);
+    e.dxlo = ceil_div(1 - lx, osx);
+    e.dxhi = (e.wx[1] == T(1) && floor_div(M.gnx - lx, osx) * osx == M.gnx - lx) ? floor_div(M.gnx - lx, osx)
+                                                                                 : floor_div(M.gnx - 1 - lx, osx);
     e.ok = e.dylo <= e.dyhi && e.dxlo <= e.dxhi;
     const int QQ = M.Qy * M.Qx;
     for (int a = 0; a < 4; ++a) {
         // stencil entry a reads 0-based grid row ly - 2 + a, padded row ly + a;
         // addresses are those of the in-range pixel (dylo, dxlo) shifted back
         // by (dylo, dxlo), so every clamped pixel offset lands in the grid
-        const int r = e.ok ? e.ly - 2 + a + EPSF_PAD + osy * e.dylo : 0;
-        const int c = e.ok ? e.lx - 2 + a + EPSF_PAD + osx * e.dxlo : 0;
+        const int r = e.ok ? ly - 2 + a + EPSF_PAD + osy * e.dylo : 0;
+        const int c = e.ok ? lx - 2 + a + EPSF_PAD + osx * e.dxlo : 0;
         e.ra[a] = (r % osy) * osx * QQ + r / osy - (e.ok ? e.dylo : 0);
         e.ca[a] = (c % osx) * QQ + (c / osx - (e.ok ? e.dxlo : 0)) * M.Qy;
     }

@@ -425,7 +425,7 @@ struct Sim {
 
         // --- data and weights
         DevBuf data(ctx, Pn * sizeof(T)), dw(ctx, Pn * sizeof(T)), w(ctx, Pn * sizeof(T));
-        launch(k_prep_pixels<T>, dim3(std::min<int64_t>(cdiv(Pn, BLOCK), n_cu * 16)), dim3(BLOCK), 0, st, d_img,
+        launch_n("k_prep_pixels", k_prep_pixels<T>, dim3(std::min<int64_t>(cdiv(Pn, BLOCK), n_cu * 16)), dim3(BLOCK), 0, st, d_img,
                d_iv, Pn, data.as<T>(), dw.as<T>(), w.as<T>(), Sc);
         read_S();
         CP_ARG(hS->n_nonfinite == 0 || d_iv != nullptr,
@@ -491,7 +491,7 @@ struct Sim {
                fm.gc2, (const int*)(o.model_rad > 0 ? nullptr : mR_src.as<int>()), mR_scalar, ay.as<int>(),
                ax.as<int>(), mR.as<int>(), theta.as<double>(), live.as<uint8_t>(), frozen_at.as<int>(),
                conv.as<uint8_t>());
-        launch(k_union<T>, dim3(cdiv((int64_t)na * S2, BLOCK)), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
+        launch_n("k_union", k_union<T>, dim3(cdiv((int64_t)na * S2, BLOCK)), dim3(BLOCK), 0, st, na, (const int*)ay.as<int>(),
                (const int*)ax.as<int>(), R, S, (const T*)w.as<T>(), ny, nx, umask.as<uint8_t>());
         {
             const int gcm = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(Pn / 4, BLOCK), n_cu * 4));
@@ -613,7 +613,7 @@ struct Sim {
         // star records of the stars in lst (null: all)
         auto star_rec = [&](const int* lst, const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter,
                             int all_live) {
-            launch(k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, lst, (const Scal*)Sc, th, fm, M,
+            launch_n("k_star_rec", k_star_rec<T, KIND>, dim3(gstar), dim3(BLOCK), 0, st, na, lst, (const Scal*)Sc, th, fm, M,
                    inv_norm_g64, moffat_amp64,
                    (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const int*)mR.as<int>(),
                    (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), frozen_iter, all_live,
@@ -653,10 +653,10 @@ struct Sim {
                    (const double*)part2.as<double>(), nb, it);
         };
 
-        // --- the stars live at the last compaction (fill, H and the star
-        // vectors walk them; count into Sc) and the model-box tile lists
+        // --- the stars live at the last compaction (the fill, the star
+        // records and the freeze marks walk them; count into Sc) and the model-box tile lists
         // restricted to them (cost and freeze renders), rebuilt whenever the
-        // live set has shrunk by a tenth.  The buffers never move, so
+        // live set has shrunk by a fifth.  The buffers never move, so
         // captured trial graphs keep their arguments.
         int live_compacted = -1;
         auto compact_live = [&]() {
@@ -718,7 +718,7 @@ struct Sim {
         const bool use_graphs = !ctx->prof;   // per-launch event timing needs plain launches
         bool D_init = false, have_resid = false;
         for (int iter = 1; iter <= o.max_iter; ++iter) {
-            if (live_compacted < 0 || n_live_now < 0.9 * live_compacted) {
+            if (live_compacted < 0 || n_live_now < 0.8 * live_compacted) {
                 compact_live();
                 live_compacted = n_live_now;
             }
@@ -792,14 +792,14 @@ struct Sim {
                                          (const double*)gprev.as<double>(), (const double*)vprev.as<double>(),
                                          gnew.as<double>(), vnew.as<double>(), xs.as<double>(), wl.as<double>(),
                                          hbar.as<double>(), first, partk.as<double>(), Sc);
-                                launch(k_finish_gk<SV>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc,
+                                launch_n("k_finish_gk", k_finish_gk<SV>, dim3(1), dim3(FINGK_BLOCK), 0, st, Sc,
                                        (const double*)partk.as<double>(), gb_jv, it);
                             };
                             if (solver == CP_SOLVER_LSMR) gk(std::integral_constant<int, CP_SOLVER_LSMR>{});
                             else gk(std::integral_constant<int, CP_SOLVER_LSQR>{});
                         }
                     });
-                    launch(k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
+                    launch_n("k_step", k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
                            (const double*)wl.as<double>(),
                            (const double*)(solver == CP_SOLVER_LSMR ? hbar.as<double>() : nullptr), xs.as<double>(),
                            (const double*)colnorm.as<double>(),
@@ -857,7 +857,7 @@ struct Sim {
             n_run = iter;
             if (f_conv) {
                 // theta <- theta + delta for the accepted step, then freeze everyone
-                launch(k_accept, dim3(gstar), dim3(BLOCK), 0, st, na, p, iter, -1.0, theta.as<double>(),
+                launch_n("k_accept", k_accept, dim3(gstar), dim3(BLOCK), 0, st, na, p, iter, -1.0, theta.as<double>(),
                        (const double*)theta_c.as<double>(), D.as<double>(), (const double*)colnorm.as<double>(),
                        (const double*)dsc.as<double>(), (const uint8_t*)gloc.as<uint8_t>(), live.as<uint8_t>(),
                        frozen_at.as<int>(), conv.as<uint8_t>(), part.as<double>());
@@ -866,7 +866,7 @@ struct Sim {
                 break;
             }
             if (accepted) {
-                launch(k_accept, dim3(gstar), dim3(BLOCK), 0, st, na, p, iter, o.x_tol, theta.as<double>(),
+                launch_n("k_accept", k_accept, dim3(gstar), dim3(BLOCK), 0, st, na, p, iter, o.x_tol, theta.as<double>(),
                        (const double*)theta_c.as<double>(), D.as<double>(), (const double*)colnorm.as<double>(),
                        (const double*)dsc.as<double>(), (const uint8_t*)gloc.as<uint8_t>(), live.as<uint8_t>(),
                        frozen_at.as<int>(), conv.as<uint8_t>(), part.as<double>());
@@ -878,7 +878,7 @@ struct Sim {
                 if (hS->n_frozen_now > 0) {
                     // newly frozen stars are in the live list of the last compaction
                     star_rec(jlist.as<int>(), theta.as<double>(), recs, crs, ers, iter, 0);
-                    launch(k_mark_frozen, dim3(gstar), dim3(BLOCK), 0, st, (const int*)jlist.as<int>(), (const Scal*)Sc,
+                    launch_n("k_mark_frozen", k_mark_frozen, dim3(gstar), dim3(BLOCK), 0, st, (const int*)jlist.as<int>(), (const Scal*)Sc,
                            (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), iter,
                            (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const int*)mR.as<int>(), g,
                            tmark.as<int>());
@@ -900,7 +900,7 @@ struct Sim {
         CP_HIP(hipMemsetAsync(errs.get(), 0, nb8, st));
         if (cov_mode != CP_COV_NONE) {
 #define CP_ERR(PP)                                                                                             \
-    launch(k_errors<T, PP, VT>, dim3(cdiv(na, BLOCK / ERR_GL)), dim3(BLOCK), 0, st, na, R, S, (const int*)ay.as<int>(), \
+    launch_n("k_errors", k_errors<T, PP, VT>, dim3(cdiv(na, BLOCK / ERR_GL)), dim3(BLOCK), 0, st, na, R, S, (const int*)ay.as<int>(), \
            (const int*)ax.as<int>(), (const T*)w.as<T>(), ny, nx, (const VT*)V.as<VT>(),                        \
            (const double*)colnorm.as<double>(), cov_mode, cost / dof, errs.as<double>())
             if (p == 1) CP_ERR(1);
@@ -924,7 +924,7 @@ struct Sim {
         int* d_niter = reinterpret_cast<int*>(cols + OUT_F64 * N);
         uint8_t* d_conv = reinterpret_cast<uint8_t*>(d_niter + N);
         uint8_t* d_valid = d_conv + N;
-        launch(k_results, dim3(cdiv(n, BLOCK)), dim3(BLOCK), 0, st, n, (const int*)slot.as<int>(), d_y0, d_x0, d_f0,
+        launch_n("k_results", k_results, dim3(cdiv(n, BLOCK)), dim3(BLOCK), 0, st, n, (const int*)slot.as<int>(), d_y0, d_x0, d_f0,
                (const double*)theta.as<double>(), (const double*)errs.as<double>(), cov_mode != CP_COV_NONE ? 1 : 0,
                (const uint8_t*)live.as<uint8_t>(), (const uint8_t*)conv.as<uint8_t>(),
                (const int*)frozen_at.as<int>(), n_run, fm, o.fixed_bkg, cols, d_niter, d_conv, d_valid);
@@ -961,7 +961,7 @@ struct Sim {
                 CP_HIP(hipMemcpyAsync(dK.get(), hK.data(), hK.size() * 8, hipMemcpyHostToDevice, st));
             }
             const int dmaxb = (2 * (int)std::ceil(fit_rad) + 2) * (2 * (int)std::ceil(fit_rad) + 2);
-            launch(k_diag<T, KIND>, dim3(cdiv(na, 64 / DIAG_GL)), dim3(64),
+            launch_n("k_diag", k_diag<T, KIND>, dim3(cdiv(na, 64 / DIAG_GL)), dim3(64),
                    (size_t)(64 / DIAG_GL) * dmaxb * sizeof(T), st, na, (const double*)theta.as<double>(), fm,
                    fit_rad, M, inv_norm_g64, moffat_amp64, DP.G, (const T*)resid, (const T*)data.as<T>(),
                    (const T*)w.as<T>(), d_iv ? 1 : 0, ny, nx, (const double*)(dK.get() ? dK.as<double>() : nullptr),
