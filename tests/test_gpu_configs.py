@@ -5,8 +5,9 @@ tiles, through the C ABI against the FP64 oracle on the same inputs.
   512 x 512 crop of its core at the full local density (about one star per
   7 px^2, 3.7e4 stars) against the oracle in both precisions, and the full
   field's run-to-run determinism and mixed-vs-FP64 agreement.
-* Crowded analytic-PSF fields with 7 x 7 stamps (the 16-lane fill and J^T u
-  paths): CircularGaussianPSF and CircularMoffatPSF.
+* Crowded analytic-PSF fields with 7 x 7 stamps (the 16-lane fill without
+  the LDS grid, and the 4-lane H assembly): CircularGaussianPSF and
+  CircularMoffatPSF.
 * Serving with 2048^2 and 4096^2 tiles in the mix: alone == served, and
   every request's device blocks within its admission estimate.
 
