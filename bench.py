@@ -80,8 +80,7 @@ def workload_bytes(kernel, S2, p, pairs_per_star, trace):
     E = N * pairs_per_star row entries (ordered pairs, each star with itself)
     among them and (E + N) / 2 distinct p x p blocks:
       k_gk     reads each distinct block once ((E + N) / 2 p^2 b_d), every row
-               entry's neighbour index, block slot and live flag (E (2 b_i +
-               1)), the row list, offsets and own live flag (N (2 b_i + 1)),
+               entry's word (neighbour, block slot or row length: E 2 b_i),
                the previous step's g and v and the x and w vectors (4 p N b_d)
                and writes g, v, x and w (4 p N b_d);
       k_hbuild reads every live star's stamp rows once (p N S^2 b_f) and the
@@ -95,7 +94,7 @@ def workload_bytes(kernel, S2, p, pairs_per_star, trace):
         E = n_live * pairs_per_star
         blocks = (E + n_live) / 2 * p * p * 8
         if kernel == "k_gk":
-            total += its * (blocks + E * 9 + n_live * 9 + 8 * p * 8 * n_live)
+            total += its * (blocks + E * 8 + 8 * p * 8 * n_live)
         elif kernel == "k_hbuild" and it not in seen:
             seen.add(it)
             total += p * 4 * S2 * n_live + E * 4 + blocks
@@ -739,6 +738,24 @@ def lm_bench(pkg, ctx, n, no_cpu, autotune, steps=3):
                                          f"{tc:.1f} s"}
         log(f"[lm] cpu baseline {done / tc:.1f} fits/s ({done} cutouts, {tc:.1f} s); GPU vs FP64 oracle "
             f"{res['accuracy']['frac_within_1e-4_flux_1e-3_px'] * 100:.3f}% within 1e-4 flux / 1e-3 px")
+        # the same fitter with OpenMP over cutouts on every host thread this
+        # process may use, over a sample about as long again
+        nth = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+        k = min(n, max(chunk, int(done / tc * nth * 5.0)))
+        sto_all = (abi.LMStat * k)()
+        tho_all = np.zeros((k, 6))
+        t = time.perf_counter()
+        lib_o.or_fit_star_batch(desc, c["pix"].ctypes.data_as(abi.f32p), c["offsets"].ctypes.data_as(abi.i64p),
+                                c["ny"].ctypes.data_as(abi.i32p), c["nx"].ctypes.data_as(abi.i32p), None,
+                                c["theta0"].ctypes.data_as(abi.f64p), c["free_mask"], k, C.byref(o64),
+                                tho_all.ctypes.data_as(abi.f64p), None, sto_all, nth)
+        ta = time.perf_counter() - t
+        res["cpu_baseline_all_cores"] = {
+            "value": round(k / ta, 1), "unit": "fits/s", "cores": nth, "kind": "port",
+            "sample": f"FP64 oracle fit_star, OpenMP over cutouts on {nth} threads, the first {k} cutouts of the "
+                      f"batch, {ta:.1f} s",
+            "same_as_one_thread": bool(np.array_equal(tho_all[:min(k, done)], tho[:min(k, done)]))}
+        log(f"[lm] cpu all cores ({nth} threads): {k / ta:.1f} fits/s ({k} cutouts, {ta:.1f} s)")
     return res
 
 
@@ -971,7 +988,8 @@ def finish_oracle(proc, path, r32, timeout=900):
     cpu = {"value": round(n / dt, 2), "unit": "stars/s", "cores": 1, "kind": "port",
            "sample": f"FP64 oracle, 1 thread, the whole 4096x4096 crowded ePSF field ({n} stars, 10 outer "
                      f"iterations; the largest tile of the serving mix), {dt:.1f} s, run beside the GPU sections",
-           "cpu_model": model, "nproc": os.cpu_count()}
+           "cpu_model": model, "nproc": os.cpu_count(),
+           "cpus_usable": len(os.sched_getaffinity(0)), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
     g = (r32["valid"] > 0) & (o["valid"] > 0)
     idx = np.nonzero(g)[0]
     rel = np.abs(r32["flux"][g] - o["flux"][g]) / np.abs(o["flux"][g])

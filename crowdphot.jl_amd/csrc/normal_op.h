@@ -19,9 +19,9 @@
 // never formed (|u_k| = 1).  The scalar recurrences and the x / w (LSMR:
 // h, hbar) updates are those of the pixel-space method, unchanged.
 //
-// One launch per Golub-Kahan step (k_gk): each row forms the v_k of its
-// neighbours where it gathers them, from the g_{k-1} = H v_{k-1} - alpha_{k-1}
-// J^T u_{k-1} and v_{k-1} the previous step stored,
+// One launch per Golub-Kahan step (k_gk, one lane per row entry): each entry
+// forms the v_k of its neighbour where it gathers it, from the g_{k-1} =
+// H v_{k-1} - alpha_{k-1} J^T u_{k-1} and v_{k-1} the previous step stored,
 //
 //   v_k = (g_{k-1} / beta_k - beta_k v_{k-1}) / alpha_k,
 //
@@ -39,15 +39,17 @@
 // Storage: the blocks of one outer iteration live in the slots of the pair
 // rows; the block of a pair (a, b) is built once, in the slot of the row of
 // min(a, b) (its upper entry), and the other row reads it transposed.  Each
-// row entry's (neighbour, slot) word is rewritten with the blocks, so the
-// operator kernels read no live flags of neighbours.  Star vectors are
-// float64 at stride p.
+// row entry's (neighbour, code) word is rewritten with the blocks, so the
+// operator kernels read no live flags.  Star vectors are float64 at stride p.
 //
 // Neighbour lists: stars are binned into cells of side 2R (anchors shifted by
 // R - 1 so every active anchor is inside the grid), so the stars whose stamps
 // overlap a star's are in its cell or the 8 around it; each star's row lists
-// them in cell order, Morton order inside a cell.
+// the star itself first, then the others in cell order, Morton order inside
+// a cell.
 #pragma once
+
+#include <climits>
 
 #include "sim_kernels.h"
 
@@ -79,7 +81,8 @@ static __global__ void k_cell_keys(int na, const int* __restrict__ ay, const int
 }
 
 // Pairs (a, b) of stars whose stamps overlap; count (emit = false) or write
-// the row of star a (emit = true) at rowptr[a].
+// the row of star a (emit = true) at rowptr[a]: a itself, then the others in
+// cell order.
 template <bool EMIT>
 static __global__ void k_pairs(int na, const int* __restrict__ ay, const int* __restrict__ ax, CellGeo g,
                                const int* __restrict__ cstart, const int* __restrict__ cend,
@@ -90,12 +93,15 @@ static __global__ void k_pairs(int na, const int* __restrict__ ay, const int* __
     const int ya = ay[a], xa = ax[a], D = 2 * g.R;
     const int cy = (ya + g.R - 1) / g.CS, cx = (xa + g.R - 1) / g.CS;
     int o = EMIT ? rowptr[a] : 0;
+    // the row's own entry first: it heads the row in the Golub-Kahan step
+    if (EMIT) pb[o] = a;
+    ++o;
     for (int jx = max(cx - 1, 0); jx <= min(cx + 1, g.ncx - 1); ++jx)
         for (int jy = max(cy - 1, 0); jy <= min(cy + 1, g.ncy - 1); ++jy) {
             const int c = jx * g.ncy + jy;
             for (int i = cstart[c]; i < cend[c]; ++i) {
                 const int b = clist[i];
-                if (abs(ay[b] - ya) <= D && abs(ax[b] - xa) <= D) {
+                if (b != a && abs(ay[b] - ya) <= D && abs(ax[b] - xa) <= D) {
                     if (EMIT) pb[o] = b;
                     ++o;
                 }
@@ -131,14 +137,26 @@ static __global__ void k_pair_rev(int na, const int* __restrict__ rowptr, const 
 }
 
 // The record of upper entry u (slot j = ulist[u]): (j, its row, its
-// neighbour, its twin entry).
+// neighbour, its twin entry); a row's own entry (neighbour = row) has no twin
+// and carries the row's length instead.
 static __global__ void k_upper_rec(int n_upper, const int* __restrict__ ulist, const int* __restrict__ rowof,
-                                   const int* __restrict__ pb, const int* __restrict__ rev, int4* __restrict__ urec) {
+                                   const int* __restrict__ pb, const int* __restrict__ rev,
+                                   const int* __restrict__ rowptr, int4* __restrict__ urec) {
     const int u = blockIdx.x * BLOCK + threadIdx.x;
     if (u >= n_upper) return;
-    const int j = ulist[u];
-    urec[u] = make_int4(j, rowof[j], pb[j], rev[j]);
+    const int j = ulist[u], a = rowof[j], b = pb[j];
+    urec[u] = make_int4(j, a, b, b == a ? rowptr[a + 1] - rowptr[a] : rev[j]);
 }
+
+// Entry words (neighbour, code), rewritten by every H assembly: the
+// neighbour is -1 where either star is frozen (the neighbour's vectors and
+// the block are not read; a frozen row's entries are all -1).  The code is
+// the block's slot j (>= 0), -(j + 1) where the row reads the block of the
+// pair's upper entry j transposed, or ENT_HEAD + the row's length for the
+// row's own entry, the first of the row, whose block is in its own slot.
+// Slots stay below 2^30, so the three ranges do not meet.
+constexpr int ENT_HEAD = INT_MIN;
+constexpr int ENT_HEAD_MAX = -(1 << 30) - 1;   // codes <= this are row heads
 
 // H_ab (P x P, row-major: row k is star a's parameter k) of every upper entry
 // (its record: slot j, row a, neighbour b >= a, the twin entry of row b) whose
@@ -146,8 +164,7 @@ static __global__ void k_upper_rec(int n_upper, const int* __restrict__ ulist, c
 // striding over the pixels the two stamps share, the P^2 sums as group
 // reductions in float64, stored in slot j.  Every upper entry also writes the
 // (neighbour, slot) words of both rows' entries for this outer iteration:
-// neighbour -1 where the neighbour is frozen (its vectors and this block are
-// not read), slot -(j + 1) where the row reads the block transposed.
+// the entry words above.
 template <typename T, int GH, int P>
 __global__ __launch_bounds__(BLOCK) void k_hbuild(const int4* __restrict__ urec, int n_upper,
                                                   const uint8_t* __restrict__ live, const int* __restrict__ ay,
@@ -164,8 +181,12 @@ __global__ __launch_bounds__(BLOCK) void k_hbuild(const int4* __restrict__ urec,
         const bool la = live[a], lb = live[b];
         const int dy0 = ay[b] - ay[a], dx0 = ax[b] - ax[a];
         if (sub == 0) {
-            ent[j] = make_int2(lb ? b : -1, j);
-            if (b != a) ent[ur.w] = make_int2(la ? a : -1, -(j + 1));
+            if (b == a) {
+                ent[j] = make_int2(la ? a : -1, ENT_HEAD + ur.w);
+            } else {
+                ent[j] = make_int2(la && lb ? b : -1, j);
+                ent[ur.w] = make_int2(la && lb ? a : -1, -(j + 1));
+            }
         }
         if (!(la && lb)) continue;   // uniform within the group
         double h[PP];
@@ -259,13 +280,14 @@ __device__ __forceinline__ void row_apply(int a, int jb, int j1, const int2* __r
 #pragma unroll
     for (int k = 0; k < P; ++k) y[k] = sd[k] = vd[k] = 0;
     for (int j = jb; j < j1; j += 4) {
-        // one 8-byte load of (neighbour, slot); a frozen neighbour (-1) reads
+        // one 8-byte load of (neighbour, code); a frozen neighbour (-1) reads
         // star 0 and its block slot (in bounds) and contributes nothing
         const unsigned long long e = ew[j];
-        const int bn = (int)(unsigned)(e & 0xffffffffu), sl = (int)(unsigned)(e >> 32);
+        const int bn = (int)(unsigned)(e & 0xffffffffu), cd = (int)(unsigned)(e >> 32);
         const bool on = bn >= 0, dg = bn == a;
         const int b = on ? bn : 0;
-        const bool tr = sl < 0;
+        const bool tr = cd < 0 && cd > ENT_HEAD_MAX;
+        const int sl = cd <= ENT_HEAD_MAX ? j : cd;
         double v[P];
 #pragma unroll
         for (int l = 0; l < P; ++l) {
@@ -303,81 +325,144 @@ __device__ __forceinline__ double pick(const double (&y)[P], int k) {
     return k == 0 ? y0 : (k == 1 ? y1 : y2);
 }
 
-// Lanes per row of the star-space operator kernels (a quad: row_apply's DPP sums).
+// Lanes per row of CG's H p (a quad: row_apply's DPP sums).
 constexpr int GKL = 4;
 
-// One Golub-Kahan step k of LSQR / LSMR, GKL = 4 lanes (a quad) per star row
-// (a quad for every star; rows of frozen stars do nothing; lane k < p owns
-// the row's component k; the first step forms v_1 = -b_scaled / (beta_1
-// alpha_1) and J^T u_1 = -b_scaled / beta_1):
-//   v_k and J^T u_k from (g_{k-1}, v_{k-1}) in gp / vp, y = H v_k,
-//   g_k = y - alpha_k J^T u_k and v_k to gn / vn, and the pending update of
-//   step k - 1 (hbar <- w - del hbar, x <- x + sig hbar, w <- v_k - tau w;
-//   LSQR has del = 0).
+// One Golub-Kahan step it of LSQR / LSMR, one lane per row entry.
+//
+// Rows are consecutive entry ranges headed by the row's own entry (b = a,
+// its code carries the row's length).
+// Block k takes the GKB entries from k * hspan on and sums the rows whose
+// head lies in its first hspan entries; the remaining GKB - hspan lanes only
+// gather for the rows that run past that span (a row longer than the margin
+// and headed near the span's end has its last entries summed by its head
+// from global memory).  Each lane forms its neighbour's v_it where it
+// gathers it (v_it = c1 s1[b] + c2 vp[b]), multiplies by the entry's block
+// (read transposed where the slot is negative) and leaves H_ab v_b in LDS;
+// the head of a live row sums its row, forms g_it = H v_it - alpha J^T u_it
+// and v_it, and applies the pending x / w (LSMR: hbar) update of step it - 1.
+// The first step forms v_1 = -b_scaled / (beta_1 alpha_1) and
+// J^T u_1 = -b_scaled / beta_1.  Frozen rows have every entry word -1, so
+// their lanes gather nothing.
 // Partials: v.y, v.J^T u, |g|^2, |v|^2, |w|^2 and, for LSMR, |x|^2, x.w,
 // x.hbar, w.hbar, |hbar|^2 of the updated vectors.
-// Load order matters: vector loads return in issue order, so the dependent
-// chain (row range, entries, neighbours) goes first and the row's x / w /
-// hbar loads only after the gather.
+// Load order matters: vector loads return in issue order, so the entry word
+// goes first, then the gather and, for a head, its row's x / w / hbar.
+// Two 1024-thread blocks per CU (64 registers).
+constexpr int GKB = 1024;
+
 template <int P, int SOLVER>
-__global__ __launch_bounds__(BLOCK) void k_gk(int na, const uint8_t* __restrict__ live,
-                                              const int* __restrict__ rowptr, const int2* __restrict__ ent,
-                                              const double* __restrict__ H, const double* __restrict__ b_scaled,
-                                              const double* __restrict__ gp, const double* __restrict__ vp,
-                                              double* __restrict__ gn, double* __restrict__ vn,
-                                              double* __restrict__ x, double* __restrict__ wv,
-                                              double* __restrict__ hb, int first, double* __restrict__ part,
-                                              Scal* Sc) {
+__global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_gk(
+    int ne, int hspan, const int2* __restrict__ ent, const double* __restrict__ H,
+    const double* __restrict__ b_scaled, const double* __restrict__ gp, const double* __restrict__ vp,
+    double* __restrict__ gn, double* __restrict__ vn, double* __restrict__ x, double* __restrict__ wv,
+    double* __restrict__ hb, int first, double* __restrict__ part, const Scal* __restrict__ Sc) {
     constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
-    constexpr int NP = LSMR ? 10 : 5, RPB = BLOCK / GKL;
-    __shared__ double sred[NP][NWAVE];
-    const int sub = threadIdx.x % GKL;
-    const int a = xcd_remap(blockIdx.x, gridDim.x) * RPB + threadIdx.x / GKL;
+    constexpr int NP = LSMR ? 10 : 5, PP = P * P;
+    __shared__ double sh[NP][GKB / 64];
+    __shared__ double ty[P][GKB];   // H_ab v_b of every lane's entry
+    const int l = threadIdx.x;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const long long j = (long long)blk * hspan + l;
+    const bool inr = j < ne;
+    const unsigned long long* ewp = reinterpret_cast<const unsigned long long*>(ent);
+    const size_t jc = (size_t)(inr ? j : ne - 1);
+    // the entry word (at a clamped, in-bounds index); a finished solve does
+    // no work
+    const unsigned long long ew = ewp[jc];
+    const int done = Sc->done;
+    const int bn = (int)(unsigned)(ew & 0xffffffffu), cd = (int)(unsigned)(ew >> 32);
+    const bool on = inr && bn >= 0 && !done;
+    const bool hd = cd <= ENT_HEAD_MAX;
+    const int L = (on && hd && l < hspan) ? cd - ENT_HEAD : 0;
+    const bool head = L > 0;   // a live row's own entry (b = a) in this block's span
+    const int b = on ? bn : 0;
+    const bool tr = cd < 0 && !hd;
+    const size_t slot = on ? (size_t)(hd ? jc : (tr ? -cd - 1 : cd)) : 0;
     const double* s1 = first ? b_scaled : gp;
     const int st1 = first ? VP : P;
-    // the row's entry range, live flag and the solver's done flag, loaded
-    // together at clamped (in-bounds) indices and tested once (no early exit,
-    // so no load waits on another); a finished solve does no work
-    const int ac = a < na ? a : na - 1;
-    const size_t ck = (size_t)ac * P + (sub < P ? sub : P - 1);   // this lane's component (sub < P)
-    const int j0 = rowptr[ac], j1 = rowptr[ac + 1];
-    const bool row = (a < na) & (live[ac] != 0) & (j1 > j0) & (Sc->done == 0);
+    double s1v[P], s2v[P], hv[PP];
+#pragma unroll
+    for (int k = 0; k < P; ++k) s1v[k] = s1[(size_t)b * st1 + k];
+#pragma unroll
+    for (int k = 0; k < P; ++k) s2v[k] = first ? 0.0 : vp[(size_t)b * P + k];
+#pragma unroll
+    for (int q = 0; q < PP; ++q) hv[q] = H[slot * PP + q];
+    const size_t rb = (size_t)b * P;
     const double alpha = Sc->alpha;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
     const double beta = first ? 0.0 : Sc->beta_cur;
     const double ib = beta != 0 ? 1.0 / beta : 1.0;
     const double ib1 = first ? 1.0 / Sc->beta1 : 0.0;
-    // v_k = c1 s1 + c2 vp, J^T u_k = d1 s1
+    // v_it = c1 s1 + c2 vp, J^T u_it = d1 s1
     const double c1 = first ? -ib1 * ia : ib * ia, c2 = first ? 0.0 : -beta * ia, d1 = first ? -ib1 : ib;
     const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau, del = first ? 0.0 : Sc->del;
+    double v[P], y[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) v[k] = c1 * s1v[k] + c2 * s2v[k];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < P; ++q) s += (tr ? hv[P * q + k] : hv[P * k + q]) * v[q];
+        y[k] = on ? s : 0.0;
+        ty[k][l] = y[k];
+    }
+    __syncthreads();
     double acc[NP];
 #pragma unroll
     for (int e = 0; e < NP; ++e) acc[e] = 0;
-    if (row) {   // uniform within the quad
-        double y[P], sd[P], vd[P];
-        if (first) row_apply<P, false>(a, j0 + sub, j1, ent, H, s1, st1, c1, vp, c2, y, sd, vd);
-        else row_apply<P, true>(a, j0 + sub, j1, ent, H, s1, st1, c1, vp, c2, y, sd, vd);
-        if (sub < P) {
-            const double xo = first ? 0.0 : x[ck];
-            const double wo = first ? 0.0 : wv[ck];
-            const double ho = (LSMR && !first) ? hb[ck] : 0.0;
-            const double yk = pick<P>(y, sub), vk = pick<P>(vd, sub);
-            const double jk = d1 * pick<P>(sd, sub);
+    if (head) {
+        // the row's x / w / hbar, loaded while the row is summed (loaded with
+        // the gather they would hold registers the gather needs: the step runs
+        // at 64 registers so two blocks share a CU)
+        double xo[P], wo[P], ho[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            xo[k] = !first ? x[rb + k] : 0.0;
+            wo[k] = !first ? wv[rb + k] : 0.0;
+            ho[k] = (LSMR && !first) ? hb[rb + k] : 0.0;
+        }
+        // the row's other entries in order: from LDS within the block, past
+        // it from global memory
+        const int lend = min(l + L, GKB);
+        for (int i = l + 1; i < lend; ++i)
+#pragma unroll
+            for (int k = 0; k < P; ++k) y[k] += ty[k][i];
+        for (long long jj = j + (lend - l); jj < j + L; ++jj) {
+            const unsigned long long e2 = ewp[jj];
+            const int b2n = (int)(unsigned)(e2 & 0xffffffffu), s2l = (int)(unsigned)(e2 >> 32);
+            if (b2n < 0) continue;   // (a row's only head is its first entry)
+            const bool t2 = s2l < 0;
+            const double* h2 = H + (size_t)(t2 ? -s2l - 1 : s2l) * PP;
+            double v2[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                v2[k] = c1 * s1[(size_t)b2n * st1 + k] + (first ? 0.0 : c2 * vp[(size_t)b2n * P + k]);
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+#pragma unroll
+                for (int q = 0; q < P; ++q) y[k] += (t2 ? h2[P * q + k] : h2[P * k + q]) * v2[q];
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const double yk = y[k], vk = v[k];   // the head's own v_a
+            const double jk = d1 * s1v[k];       // and J^T u_a
             const double gk = yk - alpha * jk;
-            gn[ck] = gk;
-            vn[ck] = vk;
+            gn[rb + k] = gk;
+            vn[rb + k] = vk;
             acc[0] += vk * yk;
             acc[1] += vk * jk;
             acc[2] += gk * gk;
             acc[3] += vk * vk;
-            const double hn = wo - del * ho;
-            const double xn = xo + sig * hn;
-            const double wn = vk - tau * wo;
-            x[ck] = xn;
-            wv[ck] = wn;
+            const double hn = wo[k] - del * ho[k];
+            const double xn = xo[k] + sig * hn;
+            const double wn = vk - tau * wo[k];
+            x[rb + k] = xn;
+            wv[rb + k] = wn;
             acc[4] += wn * wn;
             if (LSMR) {
-                hb[ck] = hn;
+                hb[rb + k] = hn;
                 acc[5] += xn * xn;
                 acc[6] += xn * wn;
                 acc[7] += xn * hn;
@@ -386,7 +471,17 @@ __global__ __launch_bounds__(BLOCK) void k_gk(int na, const uint8_t* __restrict_
             }
         }
     }
-    block_store_n<NP>(acc, part, sred);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) acc[e] = wave_sum(acc[e]);
+    if ((l & 63) == 0)
+#pragma unroll
+        for (int e = 0; e < NP; ++e) sh[e][l >> 6] = acc[e];
+    __syncthreads();
+    if (l < NP) {
+        double r = 0;
+        for (int w = 0; w < GKB / 64; ++w) r += sh[l][w];
+        part[(size_t)l * gridDim.x + blk] = r;
+    }
 }
 
 // One-block sum of k_gk's partials (block order) and the step's scalars:
@@ -394,9 +489,9 @@ __global__ __launch_bounds__(BLOCK) void k_gk(int na, const uint8_t* __restrict_
 // recurrences of step it.  The scalar state is staged in LDS while the
 // partials load, so the serial recurrences read and write LDS, not global
 // memory, and it is written back whole at the end.
-// (256 threads: the LSMR recurrences need more than the 128 registers a
+// (512 threads: the LSMR recurrences need more than the 128 registers a
 // 1024-thread block leaves each lane)
-constexpr int FINGK_BLOCK = 256;
+constexpr int FINGK_BLOCK = 512;
 template <int SOLVER>
 static __global__ __launch_bounds__(FINGK_BLOCK) void k_finish_gk(Scal* Sc, const double* __restrict__ part, int nb,
                                                                   int it) {

@@ -121,6 +121,7 @@ struct PairRows {
     DevBuf ent;    // per entry, per outer iteration: (live neighbour or -1, block slot)
     int64_t npairs = 0;
     int n_upper = 0;
+    int max_row = 0;
 };
 
 static int bit_length(int64_t v);
@@ -158,12 +159,21 @@ void build_pairs(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
     cub.ensure(b);
     b = cub.tmp.bytes();
     CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, cnt.as<int>(), P.rowptr.as<int>(), na, st));
-    int hv[2] = {0, 0};
+    // the longest row (the Golub-Kahan step's blocks overlap by its length)
+    DevBuf mrow(ctx, 4);
+    b = 0;
+    CP_HIP(hipcub::DeviceReduce::Max(nullptr, b, cnt.as<int>(), mrow.as<int>(), na, st));
+    cub.ensure(b);
+    b = cub.tmp.bytes();
+    CP_HIP(hipcub::DeviceReduce::Max(cub.tmp.get(), b, cnt.as<int>(), mrow.as<int>(), na, st));
+    int hv[3] = {0, 0, 0};
     CP_HIP(hipMemcpyAsync(&hv[0], P.rowptr.as<int>() + na - 1, 4, hipMemcpyDeviceToHost, st));
     CP_HIP(hipMemcpyAsync(&hv[1], cnt.as<int>() + na - 1, 4, hipMemcpyDeviceToHost, st));
+    CP_HIP(hipMemcpyAsync(&hv[2], mrow.get(), 4, hipMemcpyDeviceToHost, st));
     CP_HIP(hipStreamSynchronize(st));
     const int64_t np64 = (int64_t)hv[0] + hv[1];
-    CP_ARG(np64 < (int64_t)INT_MAX, "too many overlapping star pairs for one fit (>= 2^31)");
+    CP_ARG(np64 < ((int64_t)1 << 30), "too many overlapping star pairs for one fit (>= 2^30)");
+    P.max_row = hv[2];
     P.npairs = np64;
     const int np32 = (int)np64;
     CP_HIP(hipMemcpyAsync(P.rowptr.as<int>() + na, &np32, 4, hipMemcpyHostToDevice, st));
@@ -197,7 +207,8 @@ void build_pairs(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
         throw Error(CP_ESTATE, "star pair rows are not symmetric");
     P.urec.alloc(ctx, (size_t)P.n_upper * 16);
     launch(k_upper_rec, dim3(cdiv(P.n_upper, BLOCK)), dim3(BLOCK), 0, st, P.n_upper, (const int*)ulist.as<int>(),
-           (const int*)rowof.as<int>(), (const int*)P.pb.as<int>(), (const int*)rev.as<int>(), P.urec.as<int4>());
+           (const int*)rowof.as<int>(), (const int*)P.pb.as<int>(), (const int*)rev.as<int>(),
+           (const int*)P.rowptr.as<int>(), P.urec.as<int4>());
     CP_HIP(hipStreamSynchronize(st));   // the local buffers above are released on return
 }
 
@@ -592,7 +603,7 @@ struct Sim {
         // (blocks per CU: the context's tuned launch shapes, else these defaults)
         const cp_tuning& tu = ctx->tuning;
         const int gb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * tuned_bpc(tu.render_blocks_per_cu, 4));
-        // the Golub-Kahan step and CG's H p: GKL lanes per row, a group for
+        // CG's H p: GKL lanes per row, a group for
         // every star; CG's update: one thread per star
         const int gb_jv = (int)std::max<int64_t>(1, cdiv(na, BLOCK / GKL));
         const int gb_jtu = (int)std::max<int64_t>(
@@ -607,8 +618,16 @@ struct Sim {
         const int maxpb = (int)std::max<int64_t>(
             {(int64_t)gb_render, (int64_t)gb_fill, (int64_t)gb_jv, (int64_t)gb_jtu, (int64_t)gstar});
         DevBuf part(ctx, (size_t)2 * maxpb * 8), part2(ctx, (size_t)maxpb * 8);
-        // the Golub-Kahan step's partials (ten per block for LSMR, five for LSQR)
-        DevBuf partk(ctx, (size_t)10 * std::max(gb_jv, gb_jtu) * 8);
+        // CG's partials (two per block)
+        DevBuf partk(ctx, (size_t)2 * std::max(gb_jv, gb_jtu) * 8);
+        // the Golub-Kahan step: one lane per row entry; a block sums the rows
+        // headed in its first gk_span entries, the rest of its lanes cover rows
+        // that run past them (up to the longest row, at most half the block)
+        const int gk_margin = std::min(GKB / 2, std::max(64, 64 * (int)cdiv((int64_t)prs.max_row - 1, 64)));
+        const int gk_span = GKB - gk_margin;
+        const int gb_gk = (int)std::max<int64_t>(1, cdiv(prs.npairs, gk_span));
+        // its partials (ten per block for LSMR, five for LSQR)
+        DevBuf partg(ctx, (size_t)10 * gb_gk * 8);
 
         // star records of the stars in lst (null: all)
         auto star_rec = [&](const int* lst, const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter,
@@ -788,12 +807,13 @@ struct Sim {
                             DevBuf& vnew = (it & 1) ? vv1 : vv0;
                             auto gk = [&](auto sv) {
                                 constexpr int SV = decltype(sv)::value;
-                                launch_n("k_gk", k_gk<PV, SV>, dim3(gb_jv), dim3(BLOCK), 0, st, na, lv, rp, en, Hp, bs,
+                                launch_n("k_gk", k_gk<PV, SV>, dim3(gb_gk), dim3(GKB), 0, st, (int)prs.npairs, gk_span,
+                                         en, Hp, bs,
                                          (const double*)gprev.as<double>(), (const double*)vprev.as<double>(),
                                          gnew.as<double>(), vnew.as<double>(), xs.as<double>(), wl.as<double>(),
-                                         hbar.as<double>(), first, partk.as<double>(), Sc);
+                                         hbar.as<double>(), first, partg.as<double>(), (const Scal*)Sc);
                                 launch_n("k_finish_gk", k_finish_gk<SV>, dim3(1), dim3(FINGK_BLOCK), 0, st, Sc,
-                                       (const double*)partk.as<double>(), gb_jv, it);
+                                         (const double*)partg.as<double>(), gb_gk, it);
                             };
                             if (solver == CP_SOLVER_LSMR) gk(std::integral_constant<int, CP_SOLVER_LSMR>{});
                             else gk(std::integral_constant<int, CP_SOLVER_LSQR>{});
