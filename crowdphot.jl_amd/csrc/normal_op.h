@@ -148,13 +148,15 @@ static __global__ void k_upper_rec(int n_upper, const int* __restrict__ ulist, c
     urec[u] = make_int4(j, a, b, b == a ? rowptr[a + 1] - rowptr[a] : rev[j]);
 }
 
-// Entry words (neighbour, code), rewritten by every H assembly: the
-// neighbour is -1 where either star is frozen (the neighbour's vectors and
-// the block are not read; a frozen row's entries are all -1).  The code is
-// the block's slot j (>= 0), -(j + 1) where the row reads the block of the
-// pair's upper entry j transposed, or ENT_HEAD + the row's length for the
-// row's own entry, the first of the row, whose block is in its own slot.
-// Slots stay below 2^30, so the three ranges do not meet.
+// Entry words (neighbour, code) of the rows live at the last compaction,
+// packed row after row (entry j of such a row at cpos[j], row a from cst[a]),
+// rewritten by every H assembly: the neighbour is -1 where either star is
+// frozen (the neighbour's vectors and the block are not read; a frozen row's
+// entries are all -1).  The code is the slot j (>= 0) of the pair's block,
+// -(j + 1) where the row reads the block of the pair's upper entry j
+// transposed, or ENT_HEAD + the row's length for the row's own entry, the
+// first of the row, whose block is the star's diagonal block Hd[a].  Slots
+// stay below 2^30, so the three ranges do not meet.
 constexpr int ENT_HEAD = INT_MIN;
 constexpr int ENT_HEAD_MAX = -(1 << 30) - 1;   // codes <= this are row heads
 
@@ -170,6 +172,7 @@ __global__ __launch_bounds__(BLOCK) void k_hbuild(const int4* __restrict__ urec,
                                                   const uint8_t* __restrict__ live, const int* __restrict__ ay,
                                                   const int* __restrict__ ax, int R, int S,
                                                   const T* __restrict__ V, double* __restrict__ H,
+                                                  double* __restrict__ Hd, const int* __restrict__ cpos,
                                                   int2* __restrict__ ent) {
     constexpr int GPB = BLOCK / GH, PP = P * P;
     const int sub = threadIdx.x % GH;
@@ -181,11 +184,13 @@ __global__ __launch_bounds__(BLOCK) void k_hbuild(const int4* __restrict__ urec,
         const bool la = live[a], lb = live[b];
         const int dy0 = ay[b] - ay[a], dx0 = ax[b] - ax[a];
         if (sub == 0) {
+            const int cj = cpos[j];
             if (b == a) {
-                ent[j] = make_int2(la ? a : -1, ENT_HEAD + ur.w);
+                if (cj >= 0) ent[cj] = make_int2(la ? a : -1, ENT_HEAD + ur.w);
             } else {
-                ent[j] = make_int2(la && lb ? b : -1, j);
-                ent[ur.w] = make_int2(la && lb ? a : -1, -(j + 1));
+                const int ct = cpos[ur.w];
+                if (cj >= 0) ent[cj] = make_int2(la && lb ? b : -1, j);
+                if (ct >= 0) ent[ct] = make_int2(la && lb ? a : -1, -(j + 1));
             }
         }
         if (!(la && lb)) continue;   // uniform within the group
@@ -219,13 +224,15 @@ __global__ __launch_bounds__(BLOCK) void k_hbuild(const int4* __restrict__ urec,
         }
 #pragma unroll
         for (int e = 0; e < PP; ++e) h[e] = group_sum<GH>(h[e]);
-        // lane e < P^2 writes entry e of the block (GH < P^2: lanes write several)
+        // lane e < P^2 writes entry e of the block (GH < P^2: lanes write
+        // several); a star's diagonal block goes to Hd
+        double* hout = b == a ? Hd + (size_t)a * PP : H + (size_t)j * PP;
         for (int e0 = sub; e0 < PP; e0 += GH) {
             double v = 0;
 #pragma unroll
             for (int e = 0; e < PP; ++e)
                 if (e == e0) v = h[e];
-            H[(size_t)j * PP + e0] = v;
+            hout[e0] = v;
         }
     }
 }
@@ -272,7 +279,8 @@ __device__ __forceinline__ double quad_sum(double v) {
 // separate loads of them; every lane gets all three.
 template <int P, bool TWO>
 __device__ __forceinline__ void row_apply(int a, int jb, int j1, const int2* __restrict__ ent,
-                                          const double* __restrict__ H, const double* __restrict__ s1, int st1,
+                                          const double* __restrict__ H, const double* __restrict__ Hd,
+                                          const double* __restrict__ s1, int st1,
                                           double c1, const double* __restrict__ s2, double c2, double (&y)[P],
                                           double (&sd)[P], double (&vd)[P]) {
     constexpr int PP = P * P;
@@ -287,7 +295,7 @@ __device__ __forceinline__ void row_apply(int a, int jb, int j1, const int2* __r
         const bool on = bn >= 0, dg = bn == a;
         const int b = on ? bn : 0;
         const bool tr = cd < 0 && cd > ENT_HEAD_MAX;
-        const int sl = cd <= ENT_HEAD_MAX ? j : cd;
+        const bool hd = cd <= ENT_HEAD_MAX;
         double v[P];
 #pragma unroll
         for (int l = 0; l < P; ++l) {
@@ -297,7 +305,7 @@ __device__ __forceinline__ void row_apply(int a, int jb, int j1, const int2* __r
             sd[l] = dg ? sv : sd[l];
             vd[l] = dg ? v[l] : vd[l];
         }
-        const double* h = H + (size_t)(tr ? -sl - 1 : sl) * PP;
+        const double* h = hd ? Hd + (size_t)b * PP : H + (size_t)(tr ? -cd - 1 : cd) * PP;
         double hv[PP];
 #pragma unroll
         for (int q = 0; q < PP; ++q) hv[q] = h[q];
@@ -353,7 +361,7 @@ constexpr int GKB = 1024;
 
 template <int P, int SOLVER>
 __global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_gk(
-    int ne, int hspan, const int2* __restrict__ ent, const double* __restrict__ H,
+    int hspan, const int2* __restrict__ ent, const double* __restrict__ H, const double* __restrict__ Hd,
     const double* __restrict__ b_scaled, const double* __restrict__ gp, const double* __restrict__ vp,
     double* __restrict__ gn, double* __restrict__ vn, double* __restrict__ x, double* __restrict__ wv,
     double* __restrict__ hb, int first, double* __restrict__ part, const Scal* __restrict__ Sc) {
@@ -362,23 +370,32 @@ __global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     __shared__ double sh[NP][GKB / 64];
     __shared__ double ty[P][GKB];   // H_ab v_b of every lane's entry
     const int l = threadIdx.x;
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    // the packed entries of the rows live at the last compaction take the
+    // first nbk blocks (dispatched over every XCD, each XCD's blocks renumbered
+    // to one contiguous range); the other blocks and a finished solve do
+    // nothing
+    const int ne = Sc->n_gk_entries, done = Sc->done;
+    if (done) return;   // uniform (the one-block reduction then does nothing)
+    const int nbk = (int)(((long long)ne + hspan - 1) / hspan);
+    if ((int)blockIdx.x >= nbk) {   // uniform
+        if (l < NP) part[(size_t)l * gridDim.x + blockIdx.x] = 0.0;
+        return;
+    }
+    const int blk = xcd_remap(blockIdx.x, nbk);
     const long long j = (long long)blk * hspan + l;
     const bool inr = j < ne;
     const unsigned long long* ewp = reinterpret_cast<const unsigned long long*>(ent);
     const size_t jc = (size_t)(inr ? j : ne - 1);
-    // the entry word (at a clamped, in-bounds index); a finished solve does
-    // no work
+    // the entry word (at a clamped, in-bounds index)
     const unsigned long long ew = ewp[jc];
-    const int done = Sc->done;
     const int bn = (int)(unsigned)(ew & 0xffffffffu), cd = (int)(unsigned)(ew >> 32);
-    const bool on = inr && bn >= 0 && !done;
+    const bool on = inr && bn >= 0;
     const bool hd = cd <= ENT_HEAD_MAX;
     const int L = (on && hd && l < hspan) ? cd - ENT_HEAD : 0;
     const bool head = L > 0;   // a live row's own entry (b = a) in this block's span
     const int b = on ? bn : 0;
     const bool tr = cd < 0 && !hd;
-    const size_t slot = on ? (size_t)(hd ? jc : (tr ? -cd - 1 : cd)) : 0;
+    const double* hsrc = hd ? Hd + (size_t)b * PP : H + (on ? (size_t)(tr ? -cd - 1 : cd) : 0) * PP;
     const double* s1 = first ? b_scaled : gp;
     const int st1 = first ? VP : P;
     double s1v[P], s2v[P], hv[PP];
@@ -387,7 +404,7 @@ __global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 #pragma unroll
     for (int k = 0; k < P; ++k) s2v[k] = first ? 0.0 : vp[(size_t)b * P + k];
 #pragma unroll
-    for (int q = 0; q < PP; ++q) hv[q] = H[slot * PP + q];
+    for (int q = 0; q < PP; ++q) hv[q] = hsrc[q];
     const size_t rb = (size_t)b * P;
     const double alpha = Sc->alpha;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
@@ -539,6 +556,25 @@ static __global__ __launch_bounds__(FINGK_BLOCK) void k_finish_gk(Scal* Sc, cons
     if (threadIdx.x < NW) gw[threadIdx.x] = lw[threadIdx.x];
 }
 
+// Packing the entries of the live rows (at every compaction of the live
+// list): the row lengths of live stars (0 for frozen ones, and for a = na),
+// whose exclusive scan gives each live row's start cst[a]; then every entry
+// j of row a gets cpos[j] = cst[a] + (j - rowptr[a]), or -1 for a frozen
+// row, and Sc->n_gk_entries the total.
+static __global__ void k_live_rowlen(int na, const uint8_t* __restrict__ live, const int* __restrict__ rowptr,
+                                     int* __restrict__ rlen) {
+    const int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a <= na) rlen[a] = (a < na && live[a]) ? rowptr[a + 1] - rowptr[a] : 0;
+}
+static __global__ void k_live_cpos(int na, const uint8_t* __restrict__ live, const int* __restrict__ rowptr,
+                                   const int* __restrict__ cst, int* __restrict__ cpos, Scal* Sc) {
+    const int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a == 0) Sc->n_gk_entries = cst[na];
+    if (a >= na) return;
+    const int j0 = rowptr[a], j1 = rowptr[a + 1], c0 = live[a] ? cst[a] : -1;
+    for (int j = j0; j < j1; ++j) cpos[j] = c0 >= 0 ? c0 + (j - j0) : -1;
+}
+
 // CG on the damped normal equations (J'J + lam^2 I) x = -b_scaled, the
 // direction half: q = H p, a quad of lanes per star row (a quad for every
 // star, lane k < p owning component k) with p = s + beta p_old formed where
@@ -546,8 +582,9 @@ static __global__ __launch_bounds__(FINGK_BLOCK) void k_finish_gk(Scal* Sc, cons
 // writes its own p to pn and q.  Partials p.q and |p|^2.
 template <int P>
 __global__ __launch_bounds__(BLOCK) void k_cg_hv(int na, const uint8_t* __restrict__ live,
-                                                 const int* __restrict__ rowptr, const int2* __restrict__ ent,
-                                                 const double* __restrict__ H, const double* __restrict__ b_scaled,
+                                                 const int* __restrict__ rowptr, const int* __restrict__ cst,
+                                                 const int2* __restrict__ ent, const double* __restrict__ H,
+                                                 const double* __restrict__ Hd, const double* __restrict__ b_scaled,
                                                  const double* __restrict__ s, const double* __restrict__ po,
                                                  double* __restrict__ pn, double* __restrict__ q, int first,
                                                  double* __restrict__ part, Scal* Sc) {
@@ -560,15 +597,16 @@ __global__ __launch_bounds__(BLOCK) void k_cg_hv(int na, const uint8_t* __restri
     // one combined row test over loads issued together, as in k_gk
     const int ac = a < na ? a : na - 1;
     const size_t ck = (size_t)ac * P + (sub < P ? sub : P - 1);
-    const int j0 = rowptr[ac], j1 = rowptr[ac + 1];
+    // a live row is one of the rows packed at the last compaction, from cst[a]
+    const int j0 = cst[ac], j1 = j0 + rowptr[ac + 1] - rowptr[ac];
     const bool row = (a < na) & (live[ac] != 0) & (j1 > j0) & (Sc->done == 0);
     const double cb = first ? 0.0 : Sc->cg_beta;
     const double c1 = first ? -1.0 : 1.0;
     double acc[2] = {0, 0};
     if (row) {
         double y[P], sd[P], pd[P];
-        if (first) row_apply<P, false>(a, j0 + sub, j1, ent, H, s1, st1, c1, po, cb, y, sd, pd);
-        else row_apply<P, true>(a, j0 + sub, j1, ent, H, s1, st1, c1, po, cb, y, sd, pd);
+        if (first) row_apply<P, false>(a, j0 + sub, j1, ent, H, Hd, s1, st1, c1, po, cb, y, sd, pd);
+        else row_apply<P, true>(a, j0 + sub, j1, ent, H, Hd, s1, st1, c1, po, cb, y, sd, pd);
         if (sub < P) {
             const double yk = pick<P>(y, sub), pk = pick<P>(pd, sub);
             pn[ck] = pk;

@@ -55,12 +55,25 @@ constexpr int EPSF_PAD = 2;
 // pixel at offset (dy, dx) adds dy + dx * Qy.  Pixels outside [dylo, dyhi] x
 // [dxlo, dxhi] are outside the grid's closed range (value fill); ok = 0 when
 // no pixel is inside.
+// With os_y = os_x = 4, tap (a, b) at pixel offset (dy, dx) also reads word
+// base + dy + dx * Qy + off(phase, a, b), where phase = (ly mod 4) * 4 +
+// (lx mod 4) of floor(v), floor(u) at the anchor and off depends only on
+// the phase and the plane sides (epsf_tap_off): the renders' fixed-layout
+// path reads the 16 taps at constant offsets.
 template <typename T>
 struct alignas(16) EpsfRec {   // 16-byte aligned: stored and loaded as whole 16-byte words
     T wy[4], wx[4], dwy[4], dwx[4];
     int ra[4], ca[4];
     int dylo, dyhi, dxlo, dxhi, ok;
+    int phase, base;
 };
+
+// Word offset of tap (a, b) from EpsfRec::base + dy + dx * Qy for a star of
+// the given phase, os_y = os_x = 4, polyphase planes of Qy x Qx words.
+__host__ __device__ constexpr int epsf_tap_off(int phase, int a, int b, int Qy, int Qx) {
+    return ((((phase >> 2) + a) & 3) * 4 + (((phase & 3) + b) & 3)) * (Qy * Qx) + (((phase & 3) + b) >> 2) * Qy +
+           (((phase >> 2) + a) >> 2);
+}
 
 // Per-star record used by the gather kernels (rebuilt whenever theta changes).
 template <typename T>
@@ -165,6 +178,10 @@ __device__ __forceinline__ void epsf_rec(const ModelConst<T>& M, double fy, doub
         e.ca[a] = (c % osx) * QQ + (c / osx - (e.ok ? e.dxlo : 0)) * M.Qy;
     }
     if (!e.ok) e.dylo = e.dyhi = e.dxlo = e.dxhi = 0;
+    // padded row ly + a + 4 dy = 4 (Ly + dy) + (ly mod 4) + a, likewise columns
+    const int Ly = floor_div(ly, 4), Lx = floor_div(lx, 4);
+    e.phase = (ly - 4 * Ly) * 4 + (lx - 4 * Lx);
+    e.base = Ly + Lx * M.Qy;
 }
 
 // ePSF value (no flux / bkg) of a star at pixel offset (dy, dx) and its

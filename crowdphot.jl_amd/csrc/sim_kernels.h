@@ -70,8 +70,8 @@ struct Scal {
     double atol, btol;
     int iter, done, itmax, solver;
     // stars live at the last compaction (the lists the fill, H and the star
-    // vectors walk)
-    int n_jtu_stars, pad1;
+    // vectors walk) and the row entries of their rows (the Golub-Kahan step's)
+    int n_jtu_stars, n_gk_entries;
     // CG on the damped normal equations: gamma = |s|^2, pp = |p|^2, the step a
     // and the direction update beta of the current iteration, |s_0|
     double gamma, cg_alpha, cg_beta, pnorm2, qnorm2, s0norm;
@@ -562,8 +562,78 @@ __host__ __device__ __forceinline__ size_t render_lds_bytes(int ngrid) {
     return render_acc_offset<T>(ngrid) + (size_t)RWAVE * RSB * RSB * (sizeof(T) + 1);
 }
 
-// float renders fit 8 waves / SIMD: two 1024-thread blocks per CU
-template <typename T, int KIND, int MODE, bool LDSG>
+// Lane j's value of v (uniform: readlane), for float and double.
+__device__ __forceinline__ float lane_value(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_value(double v, int j) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Planes of the renders' fixed-layout ePSF path: os_y = os_x = 4 grids whose
+// polyphase planes fit RQF x RQF cells are staged with planes of exactly that
+// side, so every tap offset is a constant.
+constexpr int RQF = 28;
+
+// The pixels of one star's clipped box (n of them, h rows, local rows /
+// columns from r_lo / c_lo) for a star whose whole box lies in its in-range
+// window, phase PH and planes of QF x QF cells: the 16 LDS taps at constant
+// offsets from one address per pixel, summed into (FREEZE: subtracted from)
+// the wave's sub-block accumulator.
+// One pixel of render_box_fixed: its model value and accumulator index.
+template <int QF, int PH, typename T>
+__device__ __forceinline__ T render_px_fixed(const float* __restrict__ grid, int base0, int pi, int h, int inv_h,
+                                             int r_lo, int c_lo, const T (&wyv)[4], const T (&wxv)[4], T flux,
+                                             T bkg, int& idx) {
+    const int col = (int)(__umul24((unsigned)pi, (unsigned)inv_h) >> 16);
+    const int row = pi - (int)__umul24((unsigned)col, (unsigned)h);
+    const int lr = r_lo + row, lc = c_lo + col;
+    const float* sp = grid + (base0 + (int)__umul24((unsigned)lc, (unsigned)QF) + lr);
+    float gv[16];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) gv[a * 4 + b] = sp[epsf_tap_off(PH, a, b, QF, QF)];
+    T rowv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        rowv[a] = wxv[0] * (T)gv[a * 4] + wxv[1] * (T)gv[a * 4 + 1] + wxv[2] * (T)gv[a * 4 + 2] + wxv[3] * (T)gv[a * 4 + 3];
+    const T pv = wyv[0] * rowv[0] + wyv[1] * rowv[1] + wyv[2] * rowv[2] + wyv[3] * rowv[3];
+    idx = lc * RSB + lr;
+    return fma(flux, pv, bkg);
+}
+
+// The pixels of one star's clipped box (n of them, h rows, local rows /
+// columns from r_lo / c_lo) for a star whose whole box lies in its in-range
+// window, phase PH and planes of QF x QF cells: the 16 LDS taps at constant
+// offsets from one address per pixel, summed into (FREEZE: subtracted from)
+// the wave's sub-block accumulator.
+template <int QF, int PH, bool FREEZE, typename T>
+__device__ __forceinline__ void render_box_fixed(const float* __restrict__ grid, int base0, int n, int h, int inv_h,
+                                                 int r_lo, int c_lo, const T (&wyv)[4], const T (&wxv)[4], T flux,
+                                                 T bkg, T* sacc, uint8_t* stouch, int lane) {
+    for (int pb = 0; pb < n; pb += 64) {
+        const int pi = pb + lane;
+        if (pi < n) {
+            int idx;
+            const T v = render_px_fixed<QF, PH, T>(grid, base0, pi, h, inv_h, r_lo, c_lo, wyv, wxv, flux, bkg, idx);
+            if (FREEZE) {
+                sacc[idx] -= v;
+                stouch[idx] = 1;
+            } else {
+                sacc[idx] += v;
+            }
+        }
+    }
+}
+
+// float renders fit 8 waves / SIMD: two 1024-thread blocks per CU.  QF > 0:
+// the ePSF grid is staged with RQF x RQF planes at os 4 (render_box_fixed for
+// stars inside their window).
+template <typename T, int KIND, int MODE, bool LDSG, int QF = 0>
 __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
     const CornerRec<T>* __restrict__ crs, const int* __restrict__ tstart, const int* __restrict__ tend,
@@ -617,6 +687,12 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
             // waves of a tile run independently.
             int ai = 0;
             bool touch = false;
+            // fixed-layout ePSF path: the lane's own star's box clipped to the
+            // sub-block (r_lo | c_lo << 8 | h << 16 | wd << 24), p / h
+            // multiplier, first tap address, phase (-1: general path), flux
+            // and stencil weights, taken by the star loop with readlane
+            int fbox = 0, finv = 0, fbase = 0, fph = -1;
+            T fflux = T(0), fwy[4] = {}, fwx[4] = {};
             if (c0 + lane < s1) {
                 ai = list[c0 + lane];
                 const StarRec<T> ri = recs[ai];
@@ -624,11 +700,60 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                     MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (ri.flags & 2) != 0 : (ri.flags & 1) != 0);
                 touch = use && ri.ay - ri.mR <= gy0 + RSB - 1 && ri.ay + ri.mR >= gy0 &&
                         ri.ax - ri.mR <= gx0 + RSB - 1 && ri.ax + ri.mR >= gx0;
+                if constexpr (EPSF_LDS && QF > 0) if (touch) {
+                    const EpsfRec<T>& er = ers[ai];
+                    const int r_lo = max(ri.ay - ri.mR - gy0, 0), r_hi = min(ri.ay + ri.mR - gy0, RSB - 1);
+                    const int c_lo = max(ri.ax - ri.mR - gx0, 0), c_hi = min(ri.ax + ri.mR - gx0, RSB - 1);
+                    const int dy0 = gy0 - ri.ay, dx0 = gx0 - ri.ax;
+                    const int h = r_hi - r_lo + 1;
+                    // the whole clipped box inside the star's in-range window
+                    const bool inw = er.ok && dy0 + r_lo >= er.dylo && dy0 + r_hi <= er.dyhi &&
+                                     dx0 + c_lo >= er.dxlo && dx0 + c_hi <= er.dxhi;
+                    if (inw) {
+                        fbox = r_lo | (c_lo << 8) | (h << 16) | ((c_hi - c_lo + 1) << 24);
+                        finv = RENDER_INV_H[h];
+                        fbase = er.base + dy0 + dx0 * QF;
+                        fph = er.phase;
+                        fflux = ri.flux;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            fwy[t] = er.wy[t];
+                            fwx[t] = er.wx[t];
+                        }
+                    }
+                }
             }
             unsigned long long mine = __ballot(touch);
             while (mine) {
                 const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(mine));
                 mine &= mine - 1;
+                if constexpr (EPSF_LDS && QF > 0) {
+                    const int ph = __builtin_amdgcn_readlane(fph, j);
+                    if (ph >= 0) {
+                        const int bx = __builtin_amdgcn_readlane(fbox, j);
+                        const int r_lo = bx & 255, c_lo = (bx >> 8) & 255, h = (bx >> 16) & 255, wd = bx >> 24;
+                        const int inv_h = __builtin_amdgcn_readlane(finv, j);
+                        const int base0 = __builtin_amdgcn_readlane(fbase, j);
+                        const T flux = lane_value(fflux, j);
+                        T wyv[4], wxv[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            wyv[t] = lane_value(fwy[t], j);
+                            wxv[t] = lane_value(fwx[t], j);
+                        }
+#define CP_RBF(PHV)                                                                                          \
+    case PHV:                                                                                                \
+        render_box_fixed<QF, PHV, MODE == MODE_FREEZE, T>(grid, base0, h * wd, h, inv_h, r_lo, c_lo, wyv, wxv, \
+                                                          flux, M.bkg, sacc, stouch, lane);               \
+        break;
+                        switch (ph) {
+                            CP_RBF(0) CP_RBF(1) CP_RBF(2) CP_RBF(3) CP_RBF(4) CP_RBF(5) CP_RBF(6) CP_RBF(7)
+                            CP_RBF(8) CP_RBF(9) CP_RBF(10) CP_RBF(11) CP_RBF(12) CP_RBF(13) CP_RBF(14) CP_RBF(15)
+                        }
+#undef CP_RBF
+                        continue;
+                    }
+                }
                 const int a = __builtin_amdgcn_readlane(ai, j);
                 const StarRec<T> r = recs[a];
                 // the model box clipped to the sub-block, in local rows / columns;

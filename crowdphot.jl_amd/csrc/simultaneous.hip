@@ -118,7 +118,6 @@ void build_tiles(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
 struct PairRows {
     DevBuf rowptr, pb;
     DevBuf urec;   // per upper entry (b >= a): (slot, row, neighbour, twin entry of the other row)
-    DevBuf ent;    // per entry, per outer iteration: (live neighbour or -1, block slot)
     int64_t npairs = 0;
     int n_upper = 0;
     int max_row = 0;
@@ -184,7 +183,6 @@ void build_pairs(cp_ctx* ctx, Cub& cub, int na, const int* d_ay, const int* d_ax
     // each pair's block is kept once, in its upper entry (b >= a)
     const size_t ne = (size_t)std::max<int64_t>(P.npairs, 1);
     DevBuf rowof(ctx, ne * 4), rev(ctx, ne * 4), ulist(ctx, ne * 4), upper(ctx, ne), nup(ctx, 4);
-    P.ent.alloc(ctx, ne * 8);
     CP_HIP(hipMemsetAsync(cnt.get(), 0, 4, st));
     CP_HIP(hipMemsetAsync(upper.get(), 0, ne, st));
     launch(k_pair_rev, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, (const int*)P.rowptr.as<int>(),
@@ -322,6 +320,7 @@ struct Sim {
     hipStream_t st;
 
     const cp_psf_desc* psfd_;
+    bool fixed_q = false;   // the ePSF grid is staged with RQF x RQF planes
 
     Sim(cp_ctx* c, const HostPsf& p, DevPsf& d, const cp_psf_desc* pd)
         : ctx(c), P(p), DP(d), st(c->stream), psfd_(pd) {}
@@ -349,7 +348,12 @@ struct Sim {
                        int ntiles, const long long* tscan, const uint8_t* umask, T* dw, const T* w, const T* data,
                        T* wres, T* resid, double* part, Scal* Sc, const int* tmark = nullptr, int mark_iter = 0) {
         static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
-        if (ldsg)
+        if (ldsg && fixed_q)
+            launch_n(nm[MODE], k_render<T, KIND, MODE, true, (KIND == KIND_EPSF ? RQF : 0)>, grid, dim3(RBLOCK), sh,
+                     st, M, recs, ers, crs, (const int*)L.start.as<int>(), (const int*)L.end.as<int>(),
+                     (const int*)L.list.as<int>(), g, ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark,
+                     mark_iter);
+        else if (ldsg)
             launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
                      ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
@@ -559,8 +563,14 @@ struct Sim {
         // the normal matrix's p x p blocks and two pairs of star vectors:
         // LSQR / LSMR (g, v) of the previous and the current Golub-Kahan step,
         // CG s, q = H p and the directions of two successive steps
-        DevBuf Hblk(ctx, (size_t)std::max<int64_t>(prs.npairs, 1) * p * p * 8), gv0(ctx, nb8), gv1(ctx, nb8),
-            vv0(ctx, nb8), vv1(ctx, nb8);
+        DevBuf Hblk(ctx, (size_t)std::max<int64_t>(prs.npairs, 1) * p * p * 8), Hdiag(ctx, (size_t)na * p * p * 8),
+            gv0(ctx, nb8), gv1(ctx, nb8), vv0(ctx, nb8), vv1(ctx, nb8);
+        // the entry words of the rows live at the last compaction, packed (cent),
+        // each entry's packed position (cpos, -1 for a frozen row) and each
+        // live row's start (cst), from the live rows' lengths (rlen)
+        const size_t ne_all = (size_t)std::max<int64_t>(prs.npairs, 1);
+        DevBuf cent(ctx, ne_all * 8), cpos(ctx, ne_all * 4), cst(ctx, (size_t)(na + 1) * 4),
+            rlen(ctx, (size_t)(na + 1) * 4);
         for (DevBuf* b : {&gv0, &gv1, &vv0, &vv1, &colnorm, &xs, &wl}) CP_HIP(hipMemsetAsync(b->get(), 0, nb8, st));
         CP_HIP(hipMemsetAsync(bsc.get(), 0, nb32, st));
         DevBuf gloc(ctx, na), recs(ctx, (size_t)na * sizeof(StarRec<T>)), recs_c(ctx, (size_t)na * sizeof(StarRec<T>));
@@ -581,6 +591,10 @@ struct Sim {
         M.poly = nullptr;
         const double inv_norm_g64 = (KIND == KIND_GAUSS) ? 1.0 / (-(M_PI * P.fwhm * P.fwhm / CP_GAUSS_PRE)) : 0.0;
         const double moffat_amp64 = (KIND == KIND_MOFFAT) ? (P.beta - 1) / (M_PI * P.alpha * P.alpha) : 0.0;
+        // os 4 grids whose planes fit RQF x RQF cells are staged with planes of
+        // that side (the renders' fixed-layout path)
+        fixed_q = single_epsf && M.ios_y == 4 && M.ios_x == 4 && M.Qy <= RQF && M.Qx <= RQF;
+        if (fixed_q) M.Qy = M.Qx = RQF;
         const size_t poly_words = single_epsf ? (size_t)M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
         // the LDS grid copy (edge-padded, polyphase) leaves room for two float
         // render / fill blocks per CU (one for float64)
@@ -680,6 +694,17 @@ struct Sim {
         int live_compacted = -1;
         auto compact_live = [&]() {
             {
+                launch(k_live_rowlen, dim3(cdiv(na + 1, BLOCK)), dim3(BLOCK), 0, st, na,
+                       (const uint8_t*)live.as<uint8_t>(), (const int*)prs.rowptr.as<int>(), rlen.as<int>());
+                size_t b = 0;
+                CP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, rlen.as<int>(), cst.as<int>(), na + 1, st));
+                cub.ensure(b);
+                b = cub.tmp.bytes();
+                CP_HIP(hipcub::DeviceScan::ExclusiveSum(cub.tmp.get(), b, rlen.as<int>(), cst.as<int>(), na + 1, st));
+                launch(k_live_cpos, dim3(cdiv(na, BLOCK)), dim3(BLOCK), 0, st, na, (const uint8_t*)live.as<uint8_t>(),
+                       (const int*)prs.rowptr.as<int>(), (const int*)cst.as<int>(), cpos.as<int>(), Sc);
+            }
+            {
                 hipcub::CountingInputIterator<int> cit(0);
                 size_t b = 0;
                 CP_HIP(hipcub::DeviceSelect::Flagged(nullptr, b, cit, (const uint8_t*)live.as<uint8_t>(), jlist.as<int>(),
@@ -718,7 +743,8 @@ struct Sim {
 #define CP_HB(GH)                                                                                                  \
     launch_n("k_hbuild", k_hbuild<VT, GH, PV>, dim3(gb_hb), dim3(BLOCK), 0, st, (const int4*)prs.urec.as<int4>(),    \
              prs.n_upper, (const uint8_t*)live.as<uint8_t>(), (const int*)ay.as<int>(), (const int*)ax.as<int>(), R,   \
-             S, (const VT*)V.as<VT>(), Hblk.as<double>(), prs.ent.as<int2>())
+             S, (const VT*)V.as<VT>(), Hblk.as<double>(), Hdiag.as<double>(), (const int*)cpos.as<int>(),         \
+             cent.as<int2>())
                 if (GHh == 4) CP_HB(4);
                 else if (GHh == 16) CP_HB(16);
                 else CP_HB(64);
@@ -774,7 +800,8 @@ struct Sim {
                     with_p(p, [&](auto pc) {
                         constexpr int PV = decltype(pc)::value;
                         const int* rp = (const int*)prs.rowptr.as<int>();
-                        const int2* en = (const int2*)prs.ent.as<int2>();
+                        const int2* en = (const int2*)cent.as<int2>();
+                        const double* Hdp = (const double*)Hdiag.as<double>();
                         const int* jl = (const int*)jlist.as<int>();
                         const uint8_t* lv = (const uint8_t*)live.as<uint8_t>();
                         const double* Hp = (const double*)Hblk.as<double>();
@@ -786,7 +813,8 @@ struct Sim {
                                 // pcur, the previous one is read from pold
                                 double* pcur = (it & 1) ? vv1.as<double>() : vv0.as<double>();
                                 double* pold = (it & 1) ? vv0.as<double>() : vv1.as<double>();
-                                launch_n("k_cg_hv", k_cg_hv<PV>, dim3(gb_jv), dim3(BLOCK), 0, st, na, lv, rp, en, Hp, bs,
+                                launch_n("k_cg_hv", k_cg_hv<PV>, dim3(gb_jv), dim3(BLOCK), 0, st, na, lv, rp,
+                                         (const int*)cst.as<int>(), en, Hp, Hdp, bs,
                                          (const double*)gv0.as<double>(), (const double*)pold, pcur, gv1.as<double>(),
                                          first, partk.as<double>(), Sc);
                                 launch(k_finish<FIN_CGQ>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc,
@@ -807,8 +835,8 @@ struct Sim {
                             DevBuf& vnew = (it & 1) ? vv1 : vv0;
                             auto gk = [&](auto sv) {
                                 constexpr int SV = decltype(sv)::value;
-                                launch_n("k_gk", k_gk<PV, SV>, dim3(gb_gk), dim3(GKB), 0, st, (int)prs.npairs, gk_span,
-                                         en, Hp, bs,
+                                launch_n("k_gk", k_gk<PV, SV>, dim3(gb_gk), dim3(GKB), 0, st, gk_span, en, Hp, Hdp,
+                                         bs,
                                          (const double*)gprev.as<double>(), (const double*)vprev.as<double>(),
                                          gnew.as<double>(), vnew.as<double>(), xs.as<double>(), wl.as<double>(),
                                          hbar.as<double>(), first, partg.as<double>(), (const Scal*)Sc);
