@@ -43,7 +43,10 @@ __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, LMModel<T> Md, co
         // divergent `continue`): the loop then has one back edge and the
         // whole wave is converged again when the next __shfl reads lane 0.
         if (lane == 0) {
-            for (int k = 0; k < C.npf; ++k) theta_out[(size_t)c * C.npf + k] = R.full[k];
+            // (a constant-bound loop: an index bounded by C.npf would put R in scratch)
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (k < C.npf) theta_out[(size_t)c * C.npf + k] = R.full[k];
             if (R.st.status == 0 && cov_out && C.cov_mode != CP_COV_NONE) {
                 double cv[NP * NP];
                 lm_cov<NP>(C, R, S.npx, cv);

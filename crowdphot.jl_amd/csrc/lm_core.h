@@ -49,7 +49,10 @@ __device__ __forceinline__ void model_fg(const T* prm, T ddy, T ddx, T& f, T G[6
         T sq = (ddx * ddx + ddy * ddy) * if2;
         T norm = -(T(M_PI) * f2 / T(CP_GAUSS_PRE));
         T amp = flux / norm;
-        T g = exp(T(CP_GAUSS_PRE) * sq);
+        // exp(-4 ln 2 sq) = 2^(-4 sq): float32 takes the hardware exp2
+        T g;
+        if constexpr (sizeof(T) == 4) g = __builtin_amdgcn_exp2f(T(-4) * sq);
+        else g = exp(T(CP_GAUSS_PRE) * sq);
         T Ag = amp * g;
         f = fma(amp, g, bkg);
         T gf2 = T(CP_GAUSS_PRE) / f2;
@@ -69,9 +72,19 @@ __device__ __forceinline__ void model_fg(const T* prm, T ddy, T ddx, T& f, T G[6
         T ia = T(1) / a, ia2 = ia * ia;
         T r2 = ddx * ddx + ddy * ddy;
         T u = fma(r2, ia2, T(1));
-        T iu = T(1) / u;
-        T lu = log(u);
-        T prof = exp(-be * lu);
+        // u >= 1: float32 takes the hardware reciprocal, log2 and exp2
+        // (u^-beta = 2^(-beta log2 u)), float64 the library functions
+        T iu, lu, prof;
+        if constexpr (sizeof(T) == 4) {
+            iu = __builtin_amdgcn_rcpf(u);
+            const T l2 = __builtin_amdgcn_logf(u);
+            lu = l2 * T(M_LN2);
+            prof = __builtin_amdgcn_exp2f(-be * l2);
+        } else {
+            iu = T(1) / u;
+            lu = log(u);
+            prof = exp(-be * lu);
+        }
         T inorm = (be - T(1)) / (T(M_PI) * a2);
         T amp = flux * inorm;
         T Ag = amp * prof;

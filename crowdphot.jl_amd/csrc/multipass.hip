@@ -152,7 +152,9 @@ __global__ __launch_bounds__(64) void k_mp_level(MPArgs A, LMModel<T> Md, const 
         ms[m] = f;
     }
     if (lane == 0) {
-        for (int k = 0; k < npf; ++k) A.prm[(size_t)s * npf + k] = R.full[k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (k < npf) A.prm[(size_t)s * npf + k] = R.full[k];
         if (A.C.cov_mode != CP_COV_NONE) {   // _extract_errors! (psf_photometry_single.jl:210-222)
             double cv[NP * NP];
             lm_cov<NP>(A.C, R, nb, cv);
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(64) void k_mp_level(MPArgs A, LMModel<T> Md, const 
     // _star_diagnostics! on this star's residual (neighbours and its own
     // model removed), the input image and its model over the box
     const int rf = npf - 2, rb = npf - 1;
-    const double flux = R.full[rf], bkg = R.full[rb];
+    const double flux = pick6(R.full, rf), bkg = pick6(R.full, rb);
     if (!(flux > 0)) return;
     const double inv_flux = 1.0 / flux;
     const int kw = 2 * A.kh + 1, bx = nb / by;
