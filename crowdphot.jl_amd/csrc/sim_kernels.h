@@ -670,11 +670,24 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
         // initial and final passes
         const int gy = gy0 + lane % 16;
         bool act[4];
+        // the pixel inputs of the final pass (data / dw and w) are loaded here,
+        // with the union mask rather than after it, so their latency overlaps
+        // the star loop instead of following it
+        T pin[4], pw[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int gx = gx0 + 4 * k + lane / 16;
             act[k] = gy <= g.ny && gx <= g.nx;
             const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * g.ny;
+            pin[k] = pw[k] = T(0);
+            if (act[k]) {
+                if (MODE == MODE_RESID || MODE == MODE_COST) {
+                    pin[k] = dw[q];
+                    pw[k] = w[q];
+                } else if (MODE == MODE_FINAL) {
+                    pin[k] = data[q];
+                }
+            }
             if ((MODE == MODE_RESID || MODE == MODE_COST) && act[k]) act[k] = umask[q] != 0;
             const int idx = (4 * k + lane / 16) * RSB + lane % 16;
             sacc[idx] = (MODE == MODE_FREEZE && act[k]) ? dw[q] : T(0);
@@ -830,14 +843,14 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
             const int idx = (4 * k + lane / 16) * RSB + lane % 16;
             const T val = sacc[idx];
             if (MODE == MODE_RESID || MODE == MODE_COST) {
-                T r = val - dw[q];
-                T wr = sqrt(w[q]) * r;
+                T r = val - pin[k];
+                T wr = sqrt(pw[k]) * r;
                 wt_resid[q] = wr;
                 acc += (double)wr * (double)wr;
             } else if (MODE == MODE_FREEZE) {
                 if (stouch[idx]) dw[q] = val;
             } else {
-                resid[q] = data[q] - val;
+                resid[q] = pin[k] - val;
             }
         }
     }
