@@ -210,11 +210,20 @@ __device__ __forceinline__ void accum(const LMConst& C, const LMModel<T>& Md, co
             for (int i = 0; i <= j; ++i) acc.A[tri(i, j)] = fma(wg[i], g[j], acc.A[tri(i, j)]);
         }
     }
+    // the NA + NP + 1 sums in one reduce-scatter (wave_sum's sums exactly)
+    constexpr int NA = NP * (NP + 1) / 2;
+    double s[NA + NP + 1];
 #pragma unroll
-    for (int i = 0; i < NP * (NP + 1) / 2; ++i) out.A[i] = wave_sum(acc.A[i]);
+    for (int i = 0; i < NA; ++i) s[i] = acc.A[i];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) out.b[i] = wave_sum(acc.b[i]);
-    out.cost = wave_sum(acc.cost);
+    for (int i = 0; i < NP; ++i) s[NA + i] = acc.b[i];
+    s[NA + NP] = acc.cost;
+    wave_sum_many<NA + NP + 1>(s);
+#pragma unroll
+    for (int i = 0; i < NA; ++i) out.A[i] = s[i];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) out.b[i] = s[NA + i];
+    out.cost = s[NA + NP];
 }
 
 template <int NP>

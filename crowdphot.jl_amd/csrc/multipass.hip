@@ -211,18 +211,11 @@ __global__ __launch_bounds__(64) void k_mp_level(MPArgs A, LMModel<T> Md, const 
             ngood += 1;
         }
     }
-    qfit = wave_sum(qfit);
-    nclean = wave_sum(nclean);
-    ndirty = wave_sum(ndirty);
-    dcrowd = wave_sum(dcrowd);
-    Sp = wave_sum(Sp);
-    Sq = wave_sum(Sq);
-    Sp0 = wave_sum(Sp0);
-    Sq0 = wave_sum(Sq0);
-    SqG = wave_sum(SqG);
-    ssum = wave_sum(ssum);
-    s2sum = wave_sum(s2sum);
-    ngood = wave_sum(ngood);
+    double sums[12] = {qfit, nclean, ndirty, dcrowd, Sp, Sq, Sp0, Sq0, SqG, ssum, s2sum, ngood};
+    wave_sum_many<12>(sums);
+    qfit = sums[0], nclean = sums[1], ndirty = sums[2], dcrowd = sums[3];
+    Sp = sums[4], Sq = sums[5], Sp0 = sums[6], Sq0 = sums[7], SqG = sums[8];
+    ssum = sums[9], s2sum = sums[10], ngood = sums[11];
     if (lane != 0) return;
     const size_t N = (size_t)A.n;
     double* o = A.diag + s;

@@ -89,6 +89,146 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Cross-lane moves of float64 values without LDS (gfx950): the permlane
+// swaps exchange the upper 32 lanes of one register with the lower 32 of
+// another (rows 1, 3 with rows 0, 2 for the 16-lane form); DPP reads lane
+// l ^ 8 (row rotate by 8), l +- 4 (row shifts) and lanes of the same quad.
+__device__ __forceinline__ unsigned f64_lo(double x) {
+    return (unsigned)(unsigned long long)__double_as_longlong(x);
+}
+__device__ __forceinline__ unsigned f64_hi(double x) {
+    return (unsigned)((unsigned long long)__double_as_longlong(x) >> 32);
+}
+__device__ __forceinline__ double f64_of(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// (a, b) -> a = [a lanes 0-31, b lanes 0-31], b = [a lanes 32-63, b lanes 32-63]
+// (D = 32); rows of 16 for D = 16: a = [a r0, b r0, a r2, b r2], b = [a r1, b r1,
+// a r3, b r3].
+template <int D>
+__device__ __forceinline__ void lane_swap_f64(double& a, double& b) {
+    static_assert(D == 32 || D == 16, "lane_swap_f64: D = 32 or 16");
+    if constexpr (D == 32) {
+        const auto l = __builtin_amdgcn_permlane32_swap(f64_lo(a), f64_lo(b), false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(f64_hi(a), f64_hi(b), false, false);
+        a = f64_of(l[0], h[0]);
+        b = f64_of(l[1], h[1]);
+    } else {
+        const auto l = __builtin_amdgcn_permlane16_swap(f64_lo(a), f64_lo(b), false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(f64_hi(a), f64_hi(b), false, false);
+        a = f64_of(l[0], h[0]);
+        b = f64_of(l[1], h[1]);
+    }
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    return f64_of((unsigned)__builtin_amdgcn_update_dpp(0, (int)f64_lo(x), CTRL, 0xf, 0xf, true),
+                  (unsigned)__builtin_amdgcn_update_dpp(0, (int)f64_hi(x), CTRL, 0xf, 0xf, true));
+}
+
+// The value lane l ^ D holds (D = 8, 4, 2, 1), every lane reading
+// unconditionally (a select between two DPP reads is compiled as a branch,
+// and a lane whose source lane is masked off reads 0).  l ^ 4 is the mirror
+// within 8 lanes (l ^ 7) followed by the mirror within the quad (^ 3).
+template <int D>
+__device__ __forceinline__ double lane_xor_f64(double x) {
+    if constexpr (D == 8) return dpp_f64<0x128>(x);                 // row_ror:8
+    else if constexpr (D == 4) return dpp_f64<0x1B>(dpp_f64<0x141>(x));  // row_half_mirror, quad_perm [3,2,1,0]
+    else if constexpr (D == 2) return dpp_f64<0x4E>(x);             // quad_perm [2,3,0,1]
+    else return dpp_f64<0xB1>(x);                                   // quad_perm [1,0,3,2]
+}
+
+// One reduce-scatter step across lane bit D over the first 2 HALF values:
+// every index is a compile-time constant, so the values stay in registers
+// (a select between array elements at a loop index becomes a dynamically
+// indexed array otherwise).  Across bits 5 and 4 the swap leaves the kept
+// half in one register and the partner's copy of it in the other.
+template <int HALF, int D, int KP, size_t... I>
+__device__ __forceinline__ void wave_rs_step(double (&w)[KP], int lane, std::index_sequence<I...>) {
+    if constexpr (D >= 16) {
+        ((lane_swap_f64<D>(w[I], w[I + HALF]), w[I] = w[I] + w[I + HALF]), ...);
+    } else {
+        const bool hi = (lane & D) != 0;
+        ((w[I] = (hi ? w[I + HALF] : w[I]) + lane_xor_f64<D>(hi ? w[I] : w[I + HALF])), ...);
+    }
+}
+
+template <int HALF, int D, int KP>
+__device__ __forceinline__ void wave_rs(double (&w)[KP], int lane) {
+    if constexpr (HALF >= 1) {
+        wave_rs_step<HALF, D>(w, lane, std::make_index_sequence<HALF>{});
+        wave_rs<HALF / 2, D / 2>(w, lane);
+    }
+}
+
+// t + (the value lane l ^ D holds), for D = 32 .. 1
+template <int D>
+__device__ __forceinline__ double lane_pair_sum(double t, int lane) {
+    if constexpr (D >= 16) {
+        double u = t;
+        lane_swap_f64<D>(t, u);
+        return t + u;
+    } else {
+        return t + lane_xor_f64<D>(t);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ double wave_bfly(double t, int lane) {
+    if constexpr (D >= 1) return wave_bfly<D / 2>(lane_pair_sum<D>(t, lane), lane);
+    else return t;
+}
+
+// All-gather step across lane bit D: each lane holds CNT values and ends with
+// its own and its partner's, the half of the lane with bit D clear first.
+template <int CNT, int D, int KP, size_t... I>
+__device__ __forceinline__ void wave_ag_step(double (&w)[KP], int lane, std::index_sequence<I...>) {
+    if constexpr (D >= 16) {
+        ((w[I + CNT] = w[I], lane_swap_f64<D>(w[I], w[I + CNT])), ...);
+    } else {
+        const bool hi = (lane & D) != 0;
+        double r[CNT];
+        ((r[I] = lane_xor_f64<D>(w[I])), ...);
+        ((w[I + CNT] = hi ? w[I] : r[I], w[I] = hi ? r[I] : w[I]), ...);
+    }
+}
+
+template <int CNT, int D, int KP>
+__device__ __forceinline__ void wave_ag(double (&w)[KP], int lane) {
+    if constexpr (CNT < KP) {
+        wave_ag_step<CNT, D>(w, lane, std::make_index_sequence<CNT>{});
+        wave_ag<CNT * 2, D * 2>(w, lane);
+    }
+}
+
+// Wave sums of K float64 values at once, returned to every lane.  A
+// reduce-scatter over lane bits 5 .. 6-log2(KP) (KP = K rounded up to a power
+// of two): at each step a lane keeps half of its values, sends the other half
+// to the partner lane across that bit and adds the partner's copy of the half
+// it kept; the one value left is summed over the remaining lane bits, lanes
+// (64/KP)i .. (64/KP)(i+1)-1 then hold total i, and an all-gather over the
+// reduce-scatter's bits in reverse hands every lane all K totals.  Each
+// partial sum adds a lane's value to its partner's across the same bits in the
+// same order as wave_sum (32, 16, ..., 1), so every total is wave_sum's bit
+// for bit.  The moves are permlane swaps and DPP, no LDS.  The whole wave
+// must be converged.
+template <int K>
+__device__ __forceinline__ void wave_sum_many(double (&v)[K]) {
+    static_assert(K >= 1 && K <= 32, "wave_sum_many: 1..32 values");
+    constexpr int KP = K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : K <= 16 ? 16 : 32;
+    const int lane = threadIdx.x & 63;
+    double w[KP];
+#pragma unroll
+    for (int i = 0; i < KP; ++i) w[i] = i < K ? v[i] : 0.0;
+    wave_rs<KP / 2, 32>(w, lane);
+    w[0] = wave_bfly<32 / KP>(w[0], lane);
+    wave_ag<1, 64 / KP>(w, lane);
+#pragma unroll
+    for (int i = 0; i < K; ++i) v[i] = w[i];
+}
+
 __device__ __forceinline__ int wave_sum_int(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
