@@ -155,12 +155,17 @@ class LMOpts(C.Structure):
 
 
 class Tuning(C.Structure):
-    """cp_tuning: blocks per compute unit of the grid-stride kernels (0 = default)."""
+    """cp_tuning: blocks per compute unit of the grid-stride kernels and the
+    simultaneous fit's work-tile sizes (render tile side, lanes per
+    normal-matrix block, lanes per star of the fill); 0 = default."""
     _fields_ = [
         ("hbuild_blocks_per_cu", C.c_int32),
         ("cg_blocks_per_cu", C.c_int32),
         ("render_blocks_per_cu", C.c_int32),
         ("lm_blocks_per_cu", C.c_int32),
+        ("render_tile", C.c_int32),
+        ("hbuild_lanes", C.c_int32),
+        ("fill_lanes", C.c_int32),
     ]
 
 

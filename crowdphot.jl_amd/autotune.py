@@ -1,8 +1,10 @@
 """Launch-shape autotuner for the grid-stride kernels.
 
-The tunable shapes are the cp_tuning fields of a context (blocks per compute
+The tunable shapes are the cp_tuning fields of a context: blocks per compute
 unit of the normal-matrix assembly, CG's per-star update, the tile-gather
-renders and the batched per-star LM).  `tune` times complete calls of a workload -- whatever
+renders and the batched per-star LM, and the simultaneous fit's work-tile
+sizes (the render tile side, lanes per normal-matrix block, lanes per star of
+the fill).  `tune` times complete calls of a workload -- whatever
 `run` does, typically one device-resident fit -- over candidate values, one
 field at a time (coordinate descent from the current shape, best of `repeats`
 wall-clock timings per candidate, the call synchronous), leaves the best
@@ -11,10 +13,11 @@ a small JSON file keyed by a workload name, so a later process applies them
 without timing.
 
 Shapes change no result of the batched per-star LM (every cutout's fit is
-independent of the grid) or of the normal-matrix assembly (one group per
-block, whatever the grid).  CG's update shape and the render shape regroup
-per-block partial sums (CG's norms, the costs), a rounding-level difference
-that is fixed for a fixed shape.
+independent of the grid) or of the normal-matrix assembly's grid (one group
+per block, whatever the grid).  CG's update shape and the render shape and
+tile side regroup per-block partial sums (CG's norms, the costs), and the
+lanes per block or per star regroup the sums of H and of J^T r: a
+rounding-level difference that is fixed for a fixed shape.
 """
 import json
 import os
@@ -23,10 +26,15 @@ import time
 DEFAULT_SPACE = {
     "hbuild_blocks_per_cu": (8, 16, 32),
     "render_blocks_per_cu": (2, 4, 8),
+    "render_tile": (32, 64),
+    "hbuild_lanes": (4, 16, 64),
+    "fill_lanes": (16, 64),
 }
 LM_SPACE = {"lm_blocks_per_cu": (2, 4, 8, 16)}
-# the built-in defaults (what 0 stands for), as starting points
-DEFAULTS = {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 8, "render_blocks_per_cu": 4, "lm_blocks_per_cu": 8}
+# the built-in defaults (what 0 stands for), as starting points; 0 for the
+# H-assembly lanes, whose default follows the stamp size
+DEFAULTS = {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 8, "render_blocks_per_cu": 4, "lm_blocks_per_cu": 8,
+            "render_tile": 64, "hbuild_lanes": 0, "fill_lanes": 16}
 
 
 def _time(run, repeats):

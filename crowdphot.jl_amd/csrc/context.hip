@@ -169,6 +169,12 @@ int cp_set_tuning(cp_ctx* c, const cp_tuning* t) {
         CP_ARG(t != nullptr, "NULL argument");
         for (int32_t v : {t->hbuild_blocks_per_cu, t->cg_blocks_per_cu, t->render_blocks_per_cu, t->lm_blocks_per_cu})
             CP_ARG(v >= 0 && v <= 64, "blocks per CU must be in [0, 64] (0 = default), got " + std::to_string(v));
+        CP_ARG(t->render_tile == 0 || t->render_tile == 32 || t->render_tile == 64,
+               "render_tile must be 0 (default), 32 or 64, got " + std::to_string(t->render_tile));
+        CP_ARG(t->hbuild_lanes == 0 || t->hbuild_lanes == 4 || t->hbuild_lanes == 16 || t->hbuild_lanes == 64,
+               "hbuild_lanes must be 0 (default), 4, 16 or 64, got " + std::to_string(t->hbuild_lanes));
+        CP_ARG(t->fill_lanes == 0 || t->fill_lanes == 16 || t->fill_lanes == 64,
+               "fill_lanes must be 0 (default), 16 or 64, got " + std::to_string(t->fill_lanes));
         c->tuning = *t;
     });
 }

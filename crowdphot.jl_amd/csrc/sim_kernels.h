@@ -648,13 +648,16 @@ static __global__ void k_mirror(const Scal* __restrict__ Sc, Scal* __restrict__ 
 // (_render_model! :545-673, _residual_cost! :675-684, _cost! :686-693,
 //  _freeze_star! :739-754, final render :1344-1356)
 // ---------------------------------------------------------------------------
-constexpr int RT = 64;      // render tile side (rows = columns)
-constexpr int RSB = 16;     // sub-block side: one 16 x 16 sub-block per wave, 4 pixels per lane
-static_assert((RT / RSB) * (RT / RSB) == RWAVE, "one sub-block per wave");
+// Render tiles are RT x RT pixels, RT = 64 (the default) or 32, chosen per
+// context (cp_tuning.render_tile); the 16 waves of a block own its 4 x 4
+// sub-blocks of side RT / 4.
+constexpr int RT = 64;      // default render tile side (rows = columns)
+static_assert(RWAVE == 16, "a render tile is 4 x 4 sub-blocks, one per wave");
 
 // tile-gather render.  Each 1024-thread block walks a contiguous range of
-// 64 x 64 tiles; wave w owns the 16 x 16 sub-block (w % 4, w / 4) and each
-// lane the pixels (row lane % 16, columns 4k + lane / 16), k = 0..3.  The
+// RT x RT tiles; wave w owns the sub-block (w % 4, w / 4) of side SB = RT / 4
+// and each lane the pixels (row lane % SB, columns (64 / SB) k + lane / SB),
+// k < SB^2 / 64.  The
 // tile's star list is staged in chunks of 64 by one wave: only the stars this
 // mode renders (live / newly frozen / all), compacted in list order, each with
 // the mask of sub-blocks its model box touches, so a wave skips every other
@@ -692,14 +695,15 @@ __constant__ const int RENDER_INV_H[17] = {0,    65536, 32768, 21846, 16384, 131
                                            7282, 6554,  5958,  5462,  5042,  4682,  4370,  4096};
 
 // Dynamic LDS of the render: the staged ePSF grid (ngrid floats), then per
-// wave a 16 x 16 accumulator of T and a 16 x 16 byte touch map.
+// wave an SB x SB accumulator of T and an SB x SB byte touch map.
 template <typename T>
 __host__ __device__ __forceinline__ size_t render_acc_offset(int ngrid) {
     return ((size_t)ngrid * 4 + 15) & ~(size_t)15;
 }
 template <typename T>
-__host__ __device__ __forceinline__ size_t render_lds_bytes(int ngrid) {
-    return render_acc_offset<T>(ngrid) + (size_t)RWAVE * RSB * RSB * (sizeof(T) + 1);
+__host__ __device__ __forceinline__ size_t render_lds_bytes(int ngrid, int rt) {
+    const size_t sb = (size_t)rt / 4;
+    return render_acc_offset<T>(ngrid) + (size_t)RWAVE * sb * sb * (sizeof(T) + 1);
 }
 
 // Lane j's value of v (uniform: readlane), for float and double.
@@ -724,7 +728,7 @@ constexpr int RQF = 28;
 // offsets from one address per pixel, summed into (FREEZE: subtracted from)
 // the wave's sub-block accumulator.
 // One pixel of render_box_fixed: its model value and accumulator index.
-template <int QF, int PH, typename T>
+template <int QF, int PH, int SB, typename T>
 __device__ __forceinline__ T render_px_fixed(const float* __restrict__ grid, int base0, int pi, int h, int inv_h,
                                              int r_lo, int c_lo, const T (&wyv)[4], const T (&wxv)[4], T flux,
                                              T bkg, int& idx) {
@@ -742,7 +746,7 @@ __device__ __forceinline__ T render_px_fixed(const float* __restrict__ grid, int
     for (int a = 0; a < 4; ++a)
         rowv[a] = wxv[0] * (T)gv[a * 4] + wxv[1] * (T)gv[a * 4 + 1] + wxv[2] * (T)gv[a * 4 + 2] + wxv[3] * (T)gv[a * 4 + 3];
     const T pv = wyv[0] * rowv[0] + wyv[1] * rowv[1] + wyv[2] * rowv[2] + wyv[3] * rowv[3];
-    idx = lc * RSB + lr;
+    idx = lc * SB + lr;
     return fma(flux, pv, bkg);
 }
 
@@ -751,7 +755,7 @@ __device__ __forceinline__ T render_px_fixed(const float* __restrict__ grid, int
 // window, phase PH and planes of QF x QF cells: the 16 LDS taps at constant
 // offsets from one address per pixel, summed into (FREEZE: subtracted from)
 // the wave's sub-block accumulator.
-template <int QF, int PH, bool FREEZE, typename T>
+template <int QF, int PH, int SB, bool FREEZE, typename T>
 __device__ __forceinline__ void render_box_fixed(const float* __restrict__ grid, int base0, int n, int h, int inv_h,
                                                  int r_lo, int c_lo, const T (&wyv)[4], const T (&wxv)[4], T flux,
                                                  T bkg, T* sacc, uint8_t* stouch, int lane) {
@@ -759,7 +763,7 @@ __device__ __forceinline__ void render_box_fixed(const float* __restrict__ grid,
         const int pi = pb + lane;
         if (pi < n) {
             int idx;
-            const T v = render_px_fixed<QF, PH, T>(grid, base0, pi, h, inv_h, r_lo, c_lo, wyv, wxv, flux, bkg, idx);
+            const T v = render_px_fixed<QF, PH, SB, T>(grid, base0, pi, h, inv_h, r_lo, c_lo, wyv, wxv, flux, bkg, idx);
             if (FREEZE) {
                 sacc[idx] -= v;
                 stouch[idx] = 1;
@@ -773,7 +777,7 @@ __device__ __forceinline__ void render_box_fixed(const float* __restrict__ grid,
 // float renders fit 8 waves / SIMD: two 1024-thread blocks per CU.  QF > 0:
 // the ePSF grid is staged with RQF x RQF planes at os 4 (render_box_fixed for
 // stars inside their window).
-template <typename T, int KIND, int MODE, bool LDSG, int QF = 0>
+template <typename T, int KIND, int MODE, bool LDSG, int QF = 0, int RTV = RT>
 __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
     const CornerRec<T>* __restrict__ crs, const int* __restrict__ tstart, const int* __restrict__ tend,
@@ -792,11 +796,13 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     }
     const float* grid = LDSG ? sgrid : M.grids;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int sbr = (wv % (RT / RSB)) * RSB, sbc = (wv / (RT / RSB)) * RSB;
-    // this wave's sub-block accumulator (column-major 16 x 16) and, for the
+    constexpr int SB = RTV / 4, PPL = SB * SB / 64, CPK = 64 / SB;   // pixels per lane, columns per k
+    static_assert(SB * SB % 64 == 0, "whole pixels per lane");
+    const int sbr = (wv % 4) * SB, sbc = (wv / 4) * SB;
+    // this wave's sub-block accumulator (column-major SB x SB) and, for the
     // freeze render, which of its pixels a star touched
-    T* sacc = reinterpret_cast<T*>(smem + render_acc_offset<T>(ngrid)) + wv * (RSB * RSB);
-    uint8_t* stouch = smem + render_acc_offset<T>(ngrid) + RWAVE * RSB * RSB * sizeof(T) + wv * (RSB * RSB);
+    T* sacc = reinterpret_cast<T*>(smem + render_acc_offset<T>(ngrid)) + wv * (SB * SB);
+    uint8_t* stouch = smem + render_acc_offset<T>(ngrid) + RWAVE * SB * SB * sizeof(T) + wv * (SB * SB);
     double acc = 0;
     int t0, t1;
     tile_share(tscan, ntiles, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, t0, t1);
@@ -804,19 +810,19 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
         // the freeze render: only tiles a newly frozen star touches
         if (MODE == MODE_FREEZE && tmark[tile] != mark_iter) continue;
         const int ty = tile % g.nty, tx = tile / g.nty;
-        const int ty0 = ty * RT + 1, tx0 = tx * RT + 1;       // tile origin (1-based)
+        const int ty0 = ty * RTV + 1, tx0 = tx * RTV + 1;     // tile origin (1-based)
         const int gy0 = ty0 + sbr, gx0 = tx0 + sbc;           // this wave's sub-block origin
-        // lane owns pixels (row lane % 16, column 4 k + lane / 16) for the
+        // lane owns pixels (row lane % SB, column CPK k + lane / SB) for the
         // initial and final passes
-        const int gy = gy0 + lane % 16;
-        bool act[4];
+        const int gy = gy0 + lane % SB;
+        bool act[PPL];
         // the pixel inputs of the final pass (data / dw and w) are loaded here,
         // with the union mask rather than after it, so their latency overlaps
         // the star loop instead of following it
-        T pin[4], pw[4];
+        T pin[PPL], pw[PPL];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int gx = gx0 + 4 * k + lane / 16;
+        for (int k = 0; k < PPL; ++k) {
+            const int gx = gx0 + CPK * k + lane / SB;
             act[k] = gy <= g.ny && gx <= g.nx;
             const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * g.ny;
             pin[k] = pw[k] = T(0);
@@ -829,7 +835,7 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                 }
             }
             if ((MODE == MODE_RESID || MODE == MODE_COST) && act[k]) act[k] = umask[q] != 0;
-            const int idx = (4 * k + lane / 16) * RSB + lane % 16;
+            const int idx = (CPK * k + lane / SB) * SB + lane % SB;
             sacc[idx] = (MODE == MODE_FREEZE && act[k]) ? dw[q] : T(0);
             if (MODE == MODE_FREEZE) stouch[idx] = 0;
         }
@@ -851,12 +857,12 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                 const StarRec<T> ri = recs[ai];
                 const bool use =
                     MODE == MODE_FINAL ? true : (MODE == MODE_FREEZE ? (ri.flags & 2) != 0 : (ri.flags & 1) != 0);
-                touch = use && ri.ay - ri.mR <= gy0 + RSB - 1 && ri.ay + ri.mR >= gy0 &&
-                        ri.ax - ri.mR <= gx0 + RSB - 1 && ri.ax + ri.mR >= gx0;
+                touch = use && ri.ay - ri.mR <= gy0 + SB - 1 && ri.ay + ri.mR >= gy0 &&
+                        ri.ax - ri.mR <= gx0 + SB - 1 && ri.ax + ri.mR >= gx0;
                 if constexpr (EPSF_LDS && QF > 0) if (touch) {
                     const EpsfRec<T>& er = ers[ai];
-                    const int r_lo = max(ri.ay - ri.mR - gy0, 0), r_hi = min(ri.ay + ri.mR - gy0, RSB - 1);
-                    const int c_lo = max(ri.ax - ri.mR - gx0, 0), c_hi = min(ri.ax + ri.mR - gx0, RSB - 1);
+                    const int r_lo = max(ri.ay - ri.mR - gy0, 0), r_hi = min(ri.ay + ri.mR - gy0, SB - 1);
+                    const int c_lo = max(ri.ax - ri.mR - gx0, 0), c_hi = min(ri.ax + ri.mR - gx0, SB - 1);
                     const int dy0 = gy0 - ri.ay, dx0 = gx0 - ri.ax;
                     const int h = r_hi - r_lo + 1;
                     // the whole clipped box inside the star's in-range window
@@ -896,7 +902,7 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                         }
 #define CP_RBF(PHV)                                                                                          \
     case PHV:                                                                                                \
-        render_box_fixed<QF, PHV, MODE == MODE_FREEZE, T>(grid, base0, h * wd, h, inv_h, r_lo, c_lo, wyv, wxv, \
+        render_box_fixed<QF, PHV, SB, MODE == MODE_FREEZE, T>(grid, base0, h * wd, h, inv_h, r_lo, c_lo, wyv, wxv, \
                                                           flux, M.bkg, sacc, stouch, lane);               \
         break;
                         switch (ph) {
@@ -912,8 +918,8 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                 // the model box clipped to the sub-block, in local rows / columns;
                 // its h x wd pixels are spread over the lanes (pixel p: column
                 // p / h, row p % h), so no lane idles outside the box
-                const int r_lo = max(r.ay - r.mR - gy0, 0), r_hi = min(r.ay + r.mR - gy0, RSB - 1);
-                const int c_lo = max(r.ax - r.mR - gx0, 0), c_hi = min(r.ax + r.mR - gx0, RSB - 1);
+                const int r_lo = max(r.ay - r.mR - gy0, 0), r_hi = min(r.ay + r.mR - gy0, SB - 1);
+                const int c_lo = max(r.ax - r.mR - gx0, 0), c_hi = min(r.ax + r.mR - gx0, SB - 1);
                 const int h = r_hi - r_lo + 1, n = h * (c_hi - c_lo + 1);
                 const int inv_h = RENDER_INV_H[h];   // p / h == (p inv_h) >> 16 for p < 256
                 // this star's ePSF stencil (epsf_eval with the record in registers)
@@ -964,7 +970,7 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
                         } else {
                             v = star_value<T, KIND>(M, r, T(dy) - r.fy, T(dx) - r.fx, grid, crs + a);
                         }
-                        const int idx = lc * RSB + lr;
+                        const int idx = lc * SB + lr;
                         if (MODE == MODE_FREEZE) {
                             sacc[idx] -= v;
                             stouch[idx] = 1;
@@ -976,11 +982,11 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
             }
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < PPL; ++k) {
             if (!act[k]) continue;
-            const int gx = gx0 + 4 * k + lane / 16;
+            const int gx = gx0 + CPK * k + lane / SB;
             const size_t q = (size_t)(gy - 1) + (size_t)(gx - 1) * g.ny;
-            const int idx = (4 * k + lane / 16) * RSB + lane % 16;
+            const int idx = (CPK * k + lane / SB) * SB + lane % SB;
             const T val = sacc[idx];
             if (MODE == MODE_RESID || MODE == MODE_COST) {
                 T r = val - pin[k];

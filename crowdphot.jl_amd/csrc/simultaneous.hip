@@ -321,6 +321,8 @@ struct Sim {
 
     const cp_psf_desc* psfd_;
     bool fixed_q = false;   // the ePSF grid is staged with RQF x RQF planes
+    int rt_ = RT;           // render tile side (cp_tuning.render_tile)
+    int fill_gl_ = 16;      // lanes per star of the fill for stamps of <= 64 pixels (cp_tuning.fill_lanes)
 
     Sim(cp_ctx* c, const HostPsf& p, DevPsf& d, const cp_psf_desc* pd)
         : ctx(c), P(p), DP(d), st(c->stream), psfd_(pd) {}
@@ -334,7 +336,8 @@ struct Sim {
 #define CP_FILL(E, G)                                                                                        \
     launch_n(nm, k_fill<T, KIND, FINAL, LDSG, E, G, VT>, grid, dim3(FBLOCK), sh, st, M, recs, ers, crs, lst, na, R, S, fm, \
              w, ny, nx, wres, V, cn, b, gloc, gtol, part, Sc)
-        if (S2 <= 64) CP_FILL(4, 16);
+        if (S2 <= 64 && fill_gl_ == 16) CP_FILL(4, 16);
+        else if (S2 <= 64) CP_FILL(1, 64);
         else if (S2 <= 128) CP_FILL(2, 64);
         else if (S2 <= 256) CP_FILL(4, 64);
         else if (S2 <= 512) CP_FILL(8, 64);
@@ -342,25 +345,38 @@ struct Sim {
 #undef CP_FILL
     }
 
+    template <int MODE, int RTV>
+    void render_launch_rt(bool ldsg, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
+                          const EpsfRec<T>* ers, const CornerRec<T>* crs, const TileLists& L, const TileGeo& g,
+                          int ntiles, const long long* tscan, const uint8_t* umask, T* dw, const T* w, const T* data,
+                          T* wres, T* resid, double* part, Scal* Sc, const int* tmark, int mark_iter) {
+        static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
+        if (ldsg && fixed_q)
+            launch_n(nm[MODE], k_render<T, KIND, MODE, true, (KIND == KIND_EPSF ? RQF : 0), RTV>, grid, dim3(RBLOCK),
+                     sh, st, M, recs, ers, crs, (const int*)L.start.as<int>(), (const int*)L.end.as<int>(),
+                     (const int*)L.list.as<int>(), g, ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark,
+                     mark_iter);
+        else if (ldsg)
+            launch_n(nm[MODE], k_render<T, KIND, MODE, true, 0, RTV>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
+                     (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
+                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
+        else
+            launch_n(nm[MODE], k_render<T, KIND, MODE, false, 0, RTV>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
+                     (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
+                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
+    }
+
     template <int MODE>
     void render_launch(bool ldsg, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
                        const EpsfRec<T>* ers, const CornerRec<T>* crs, const TileLists& L, const TileGeo& g,
                        int ntiles, const long long* tscan, const uint8_t* umask, T* dw, const T* w, const T* data,
                        T* wres, T* resid, double* part, Scal* Sc, const int* tmark = nullptr, int mark_iter = 0) {
-        static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
-        if (ldsg && fixed_q)
-            launch_n(nm[MODE], k_render<T, KIND, MODE, true, (KIND == KIND_EPSF ? RQF : 0)>, grid, dim3(RBLOCK), sh,
-                     st, M, recs, ers, crs, (const int*)L.start.as<int>(), (const int*)L.end.as<int>(),
-                     (const int*)L.list.as<int>(), g, ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark,
-                     mark_iter);
-        else if (ldsg)
-            launch_n(nm[MODE], k_render<T, KIND, MODE, true>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
-                     (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
+        if (rt_ == 32)
+            render_launch_rt<MODE, 32>(ldsg, grid, sh, M, recs, ers, crs, L, g, ntiles, tscan, umask, dw, w, data,
+                                       wres, resid, part, Sc, tmark, mark_iter);
         else
-            launch_n(nm[MODE], k_render<T, KIND, MODE, false>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
-                     (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
+            render_launch_rt<MODE, 64>(ldsg, grid, sh, M, recs, ers, crs, L, g, ntiles, tscan, umask, dw, w, data,
+                                       wres, resid, part, Sc, tmark, mark_iter);
     }
 
     void run(const float* d_img, const float* d_iv, int ny, int nx, const double* d_y0, const double* d_x0,
@@ -369,6 +385,9 @@ struct Sim {
         const int64_t Pn = (int64_t)ny * nx;
         const int R = (int)std::ceil(fit_rad), S = 2 * R + 1, S2 = S * S;
         CP_ARG(S2 <= 64 * MAX_EPL, "fit_rad too large for this build (ceil(fit_rad) <= 15)");
+        // work-tile sizes of the context (cp_tuning; 0 = these defaults)
+        rt_ = ctx->tuning.render_tile > 0 ? ctx->tuning.render_tile : RT;
+        fill_gl_ = ctx->tuning.fill_lanes > 0 ? ctx->tuning.fill_lanes : 16;
         // free set (y, x, flux in getproperties order)
         FreeMap fm{};
         int gcs[3] = {0, 0, 0}, np_ = 0;
@@ -522,9 +541,10 @@ struct Sim {
         const double dof = (double)std::max<int64_t>(U - nvar, 1);
         CP_HIP(hipMemcpyAsync(&Sc->dof, &dof, 8, hipMemcpyHostToDevice, st));
 
-        // --- model-box tile lists (render passes, RT x RT tiles) and the rows
+        // --- model-box tile lists (render passes, rt x rt tiles) and the rows
         // of star pairs with overlapping stamps (the normal matrix H = J^T J)
-        TileGeo g{RT, RT, (ny + RT - 1) / RT, (nx + RT - 1) / RT, ny, nx};
+        const int rt = rt_;
+        TileGeo g{rt, rt, (ny + rt - 1) / rt, (nx + rt - 1) / rt, ny, nx};
         const int ntiles = g.nty * g.ntx;
         TileLists Lmod;
         PairRows prs;
@@ -600,7 +620,7 @@ struct Sim {
         // render / fill blocks per CU (one for float64)
         const bool ldsg = single_epsf && poly_words * 4 <= (sizeof(T) == 4 ? 56 : 96) * 1024;
         const size_t grid_sh = ldsg ? ((poly_words * 4 + 15) & ~(size_t)15) : 0;
-        const size_t render_sh = render_lds_bytes<T>(ldsg ? (int)poly_words : 0);
+        const size_t render_sh = render_lds_bytes<T>(ldsg ? (int)poly_words : 0, rt_);
         DevBuf polyg;
         if (ldsg) {
             polyg.alloc(ctx, poly_words * 4);
@@ -622,8 +642,8 @@ struct Sim {
         const int gb_jv = (int)std::max<int64_t>(1, cdiv(na, BLOCK / GKL));
         const int gb_jtu = (int)std::max<int64_t>(
             1, std::min<int64_t>(cdiv(na, BLOCK), (int64_t)n_cu * tuned_bpc(tu.cg_blocks_per_cu, 8)));
-        // the H assembly: GHh lanes per stored block
-        const int GHh = S2 <= 64 ? 4 : (S2 <= 256 ? 16 : 64);
+        // the H assembly: GHh lanes per stored block (by stamp size unless set)
+        const int GHh = tu.hbuild_lanes > 0 ? tu.hbuild_lanes : (S2 <= 64 ? 4 : (S2 <= 256 ? 16 : 64));
         const int gb_hb = (int)std::max<int64_t>(
             1, std::min<int64_t>(cdiv(prs.n_upper, BLOCK / GHh), (int64_t)n_cu * tuned_bpc(tu.hbuild_blocks_per_cu, 32)));
         const int gb_fill = (int)std::min<int64_t>(cdiv(na, FBLOCK / 64), (int64_t)n_cu * 2);

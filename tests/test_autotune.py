@@ -13,7 +13,7 @@ at = pkg.autotune
 class FakeCtx:
     def __init__(self):
         self.t = {k: 0 for k in ("hbuild_blocks_per_cu", "cg_blocks_per_cu", "render_blocks_per_cu",
-                                 "lm_blocks_per_cu")}
+                                 "lm_blocks_per_cu", "render_tile", "hbuild_lanes", "fill_lanes")}
         self.history = []
 
     def get_tuning(self):
@@ -39,6 +39,16 @@ def test_coordinate_descent_finds_the_known_optimum():
         assert shape[k] == v, (k, shape)
     assert ctx.get_tuning()["hbuild_blocks_per_cu"] == 16   # left set on the context
     assert t < 0.01
+
+
+def test_work_tile_sizes_are_searched():
+    ctx = FakeCtx()
+    best = {"render_tile": 32, "hbuild_lanes": 16, "fill_lanes": 64}
+    shape, _ = at.tune(ctx, _run_for(ctx, best), repeats=2)
+    for k, v in best.items():
+        assert shape[k] == v, (k, shape)
+    # 0 (the stamp-size rule) is where the H-assembly lanes start
+    assert ctx.history[0]["hbuild_lanes"] == 0 and ctx.history[0]["render_tile"] == 64
 
 
 def test_defaults_are_the_starting_point_and_kept_without_gain():

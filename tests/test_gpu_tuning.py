@@ -1,6 +1,8 @@
 """Launch shapes (cp_tuning) on the GPU: range checks, the batched per-star
 LM unchanged bit for bit under another grid, the simultaneous fit under
-another shape still on the FP64 oracle, and the autotuner on a real call."""
+other shapes and work-tile sizes (render tile side, lanes per normal-matrix
+block and per star of the fill) still on the FP64 oracle, and the autotuner
+on a real call."""
 import numpy as np
 import pytest
 
@@ -26,6 +28,9 @@ def test_tuning_round_trip_and_range(ctx):
     assert t["hbuild_blocks_per_cu"] == 4 and t["lm_blocks_per_cu"] == 2 and t["cg_blocks_per_cu"] == 0
     with pytest.raises(pkg.ArgumentError, match="blocks per CU"):
         ctx.set_tuning(render_blocks_per_cu=65)
+    for field, bad in (("render_tile", 48), ("hbuild_lanes", 8), ("fill_lanes", 32)):
+        with pytest.raises(pkg.ArgumentError, match=field):
+            ctx.set_tuning(**{field: bad})
     with pytest.raises(pkg.ArgumentError, match="unknown tuning"):
         ctx.set_tuning(tile=3)
     assert ctx.get_tuning() == t
@@ -54,13 +59,18 @@ def test_simultaneous_under_other_shapes_matches_oracle(ctx):
     n = int(round(2e5 * (384 * 384) / 4096 ** 2))
     crowd = sim.config_epsf_crowded(seed=5, shape=(384, 384), n=n, max_iter=10)
     ref = _gpu_run(ctx, crowd, "mixed")
-    for shape in ({"hbuild_blocks_per_cu": 1, "cg_blocks_per_cu": 16, "render_blocks_per_cu": 1},
-                  {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 2, "render_blocks_per_cu": 8}):
+    ref64 = _gpu_run(ctx, crowd, "fp64")
+    for shape in ({"hbuild_blocks_per_cu": 1, "cg_blocks_per_cu": 16, "render_blocks_per_cu": 1,
+                   "render_tile": 32, "hbuild_lanes": 64, "fill_lanes": 64},
+                  {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 2, "render_blocks_per_cu": 8,
+                   "render_tile": 64, "hbuild_lanes": 16, "fill_lanes": 16}):
         ctx.set_tuning(**shape)
         _compare(_gpu_run(ctx, cfg, "fp64"), ro, 1e-7, 1e-7)
         _compare(_gpu_run(ctx, cfg, "mixed"), ro, FLUX_RTOL, POS_ATOL)
-        # crowded field: against the default shape's own run
+        # crowded field: against the default shape's own run (in float64 the
+        # shapes regroup only rounding-level partial sums: every star agrees)
         _compare(_gpu_run(ctx, crowd, "mixed"), ref, FLUX_RTOL, POS_ATOL, frac=0.98)
+        _compare(_gpu_run(ctx, crowd, "fp64"), ref64, 1e-8, 1e-8)
 
 
 def test_autotune_on_a_real_call(ctx):
