@@ -588,6 +588,8 @@ static __global__ void k_star_rec(int na, const int* __restrict__ lst, const Sca
                                   CornerRec<T>* __restrict__ crs, EpsfRec<T>* __restrict__ ers) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= (lst ? Sc->n_jtu_stars : na)) return;
+    // the freeze render's records: nothing to do when no star froze
+    if (frozen_iter > 0 && Sc->n_frozen_now == 0) return;
     const int a = lst ? lst[i] : i;
     double Y, X, F;
     star_yxf(fm, theta + (size_t)a * fm.p, Y, X, F);
@@ -790,6 +792,16 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     __shared__ double sred[RWAVE];
     const int ngrid = LDSG ? M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
+    int t0, t1;
+    tile_share(tscan, ntiles, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, t0, t1);
+    // the freeze render: a block none of whose tiles a newly frozen star
+    // touches stages no grid (the launch follows every accepted step, whether
+    // or not a star froze)
+    if (MODE == MODE_FREEZE) {
+        bool any = false;
+        for (int tile = t0 + (int)threadIdx.x; tile < t1; tile += RBLOCK) any |= tmark[tile] == mark_iter;
+        if (!__syncthreads_or(any)) return;
+    }
     if (LDSG) {
         load_poly(M.poly, sgrid, ngrid, threadIdx.x, RBLOCK);
         __syncthreads();   // the only block barrier: every wave reads the whole grid
@@ -804,8 +816,6 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     T* sacc = reinterpret_cast<T*>(smem + render_acc_offset<T>(ngrid)) + wv * (SB * SB);
     uint8_t* stouch = smem + render_acc_offset<T>(ngrid) + RWAVE * SB * SB * sizeof(T) + wv * (SB * SB);
     double acc = 0;
-    int t0, t1;
-    tile_share(tscan, ntiles, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, t0, t1);
     for (int tile = t0; tile < t1; ++tile) {
         // the freeze render: only tiles a newly frozen star touches
         if (MODE == MODE_FREEZE && tmark[tile] != mark_iter) continue;

@@ -777,8 +777,17 @@ struct Sim {
         // render) is the next iteration's residual: the live-star model minus
         // the frozen-star-subtracted data is the same image, so only the first
         // iteration renders it separately.
+        //
+        // The host reads the device scalars once per damping trial (after its
+        // candidate cost).  An accepted step's freezing is not read back: the
+        // freeze render follows every accepted step and does nothing where no
+        // star froze, and the live count after the step reaches the host with
+        // the next iteration's first trial, so the live lists are recompacted
+        // one iteration after the live set has shrunk by a fifth, and an
+        // iteration that starts with no live star is discarded there.
         double lam = o.lambda_init, cost = 0;
         int n_run = 0, n_live_now = na;
+        bool have_live_count = false;   // Sc->n_live is the count after an accepted step
         TrialGraphs graphs;
         const bool use_graphs = !ctx->prof;   // per-launch event timing needs plain launches
         bool D_init = false, have_resid = false;
@@ -809,7 +818,7 @@ struct Sim {
             // changes none of the state the stop keeps: theta, the residual,
             // the live set).
             double gnorm = 0;
-            bool g_stop = false;
+            bool g_stop = false, none_live = false;
             bool accepted = false, f_conv = false;
             for (int trial = 1; trial <= o.max_trials; ++trial) {
                 const double lam_solve = std::sqrt(lam);
@@ -884,6 +893,13 @@ struct Sim {
                 else trial_body();
                 read_S();
                 if (trial == 1) {
+                    if (have_live_count) {
+                        n_live_now = hS->n_live;
+                        if (n_live_now == 0) {   // every star froze at the last accepted step
+                            none_live = true;
+                            break;
+                        }
+                    }
                     cost = hS->cost;
                     const unsigned long long bits = hS->gmax_bits;
                     std::memcpy(&gnorm, &bits, 8);
@@ -916,6 +932,10 @@ struct Sim {
                 }
                 lam = std::min(lam * o.lambda_up, o.lambda_max);
             }
+            if (none_live) {
+                n_run = iter - 1;
+                break;
+            }
             if (g_stop) {
                 launch(k_freeze_all, dim3(gstar), dim3(BLOCK), 0, st, na, iter - 1, live.as<uint8_t>(),
                        frozen_at.as<int>(), conv.as<uint8_t>());
@@ -941,22 +961,20 @@ struct Sim {
                 finish(k_finish<FIN_ACCEPT>, (int)gstar, 0);
                 launch(k_take_candidate, dim3(1), dim3(64), 0, st, Sc);
                 std::swap(wres_cur, wres_cand);
-                read_S();
-                n_live_now = hS->n_live;
-                if (hS->n_frozen_now > 0) {
-                    // newly frozen stars are in the live list of the last compaction
-                    star_rec(jlist.as<int>(), theta.as<double>(), recs, crs, ers, iter, 0);
-                    launch_n("k_mark_frozen", k_mark_frozen, dim3(gstar), dim3(BLOCK), 0, st, (const int*)jlist.as<int>(), (const Scal*)Sc,
-                           (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), iter,
-                           (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const int*)mR.as<int>(), g,
-                           tmark.as<int>());
-                    render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
-                                               ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Llive, g, ntiles,
-                                               (const long long*)tscan_l.as<long long>(), umask.as<uint8_t>(),
-                                               dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
-                                               part.as<double>(), Sc, (const int*)tmark.as<int>(), iter);
-                }
-                if (hS->n_live == 0) break;
+                have_live_count = true;
+                // the stars frozen at this step (in the live list of the last
+                // compaction) leave the data: records, tile marks and the freeze
+                // render, each doing nothing when none froze
+                star_rec(jlist.as<int>(), theta.as<double>(), recs, crs, ers, iter, 0);
+                launch_n("k_mark_frozen", k_mark_frozen, dim3(gstar), dim3(BLOCK), 0, st, (const int*)jlist.as<int>(), (const Scal*)Sc,
+                       (const uint8_t*)live.as<uint8_t>(), (const int*)frozen_at.as<int>(), iter,
+                       (const int*)ay.as<int>(), (const int*)ax.as<int>(), (const int*)mR.as<int>(), g,
+                       tmark.as<int>());
+                render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
+                                           ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Llive, g, ntiles,
+                                           (const long long*)tscan_l.as<long long>(), umask.as<uint8_t>(),
+                                           dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
+                                           part.as<double>(), Sc, (const int*)tmark.as<int>(), iter);
             }
         }
 
