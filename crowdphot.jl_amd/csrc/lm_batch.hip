@@ -16,7 +16,7 @@
 
 namespace cp {
 
-template <typename T, int KIND, int NP>
+template <typename T, int KIND, int NP, bool UNIT_W>
 __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, LMModel<T> Md, const float* __restrict__ pix,
                                                     const int64_t* __restrict__ offsets,
                                                     const int32_t* __restrict__ cny, const int32_t* __restrict__ cnx,
@@ -36,9 +36,14 @@ __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, LMModel<T> Md, co
 #pragma unroll
         for (int k = 0; k < 6; ++k)
             if (k < C.npf) full0[k] = theta0[(size_t)c * C.npf + k];
-        const CutoutSrc S{pix + off, C.have_iv ? iv + off : nullptr, ny, ny * nx};
         LMOut<NP> R;
-        lm_fit<T, KIND, NP>(C, Md, full0, S, R);
+        if constexpr (UNIT_W) {
+            const CutoutSrcUnit S{pix + off, ny, ny * nx};
+            lm_fit<T, KIND, NP>(C, Md, full0, S, R);
+        } else {
+            const CutoutSrc S{pix + off, C.have_iv ? iv + off : nullptr, ny, ny * nx};
+            lm_fit<T, KIND, NP>(C, Md, full0, S, R);
+        }
         // Lane 0 writes inside an if that falls through to the loop end (no
         // divergent `continue`): the loop then has one back edge and the
         // whole wave is converged again when the next __shfl reads lane 0.
@@ -49,7 +54,7 @@ __global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, LMModel<T> Md, co
                 if (k < C.npf) theta_out[(size_t)c * C.npf + k] = R.full[k];
             if (R.st.status == 0 && cov_out && C.cov_mode != CP_COV_NONE) {
                 double cv[NP * NP];
-                lm_cov<NP>(C, R, S.npx, cv);
+                lm_cov<NP>(C, R, ny * nx, cv);
                 for (int i = 0; i < NP * NP; ++i) cov_out[(size_t)c * NP * NP + i] = cv[i];
             }
             stats[c] = R.st;
@@ -61,7 +66,11 @@ template <typename T, int KIND>
 void launch_lm(int np, dim3 g, hipStream_t s, const LMConst& C, const LMModel<T>& Md, const float* pix,
                const int64_t* off, const int32_t* ny, const int32_t* nx, const float* iv, const double* th0, int n,
                double* th, double* cov, cp_lm_stat* st, int* work) {
-#define CP_LM(NPV) launch_n("k_lm_batch", k_lm_batch<T, KIND, NPV>, g, dim3(BLOCK), 0, s, C, Md, pix, off, ny, nx, iv, th0, n, th, cov, st, work)
+#define CP_LM(NPV)                                                                                          \
+    (C.have_iv ? launch_n("k_lm_batch", k_lm_batch<T, KIND, NPV, false>, g, dim3(BLOCK), 0, s, C, Md, pix, off, ny, \
+                          nx, iv, th0, n, th, cov, st, work)                                                   \
+               : launch_n("k_lm_batch", k_lm_batch<T, KIND, NPV, true>, g, dim3(BLOCK), 0, s, C, Md, pix, off, ny,  \
+                          nx, iv, th0, n, th, cov, st, work))
     switch (np) {
         case 1: CP_LM(1); break;
         case 2: CP_LM(2); break;

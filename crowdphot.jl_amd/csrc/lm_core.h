@@ -169,6 +169,20 @@ struct CutoutSrc {
     }
 };
 
+// A packed cutout without inv_var: every weight is the constant 1, so the
+// weight products of accum! fold away at compile time.
+struct CutoutSrcUnit {
+    const float* pix;
+    int ny, npx;
+    __device__ __forceinline__ void at(int m, double& py, double& px, double& v, double& w) const {
+        const int col = m / ny;
+        py = (double)(m - col * ny + 1);
+        px = (double)(col + 1);
+        v = (double)pix[m];
+        w = 1.0;
+    }
+};
+
 // accum!: cost, b = J'W r, A = J'W J over the fit region (lanes stride over
 // its pixels).  The offsets from the star centre are formed in float64 and
 // then rounded, so absolute image coordinates keep their precision in the
