@@ -636,14 +636,15 @@ static __global__ void k_mark_frozen(const int* __restrict__ lst, const Scal* Sc
         for (int ty = ty0; ty <= ty1; ++ty) tmark[tx * g.nty + ty] = iter;
 }
 
-// The device scalars copied to pinned host memory (vector stores; read by
-// the host after a stream synchronize).
-static __global__ void k_mirror(const Scal* __restrict__ Sc, Scal* __restrict__ hS) {
+// The device scalars copied to pinned host memory by the threads of one block
+// (vector stores; the host reads them after a stream synchronize).
+__device__ __forceinline__ void mirror_scal(const Scal* __restrict__ Sc, Scal* __restrict__ hS) {
     static_assert(sizeof(Scal) % 8 == 0, "Scal copied as 8-byte words");
     const unsigned long long* s = reinterpret_cast<const unsigned long long*>(Sc);
     unsigned long long* d = reinterpret_cast<unsigned long long*>(hS);
     for (int i = threadIdx.x; i < (int)(sizeof(Scal) / 8); i += blockDim.x) d[i] = s[i];
 }
+static __global__ void k_mirror(const Scal* __restrict__ Sc, Scal* __restrict__ hS) { mirror_scal(Sc, hS); }
 
 // ---------------------------------------------------------------------------
 // tile-gather render: residual / candidate cost / freeze subtraction / final
@@ -1163,61 +1164,6 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
 // ---------------------------------------------------------------------------
 // inner solver: LSQR (Paige & Saunders 1982) on the stamp operator
 // ---------------------------------------------------------------------------
-static __global__ void k_lsqr_init(Scal* S, double lam, int itmax, double atol, double btol, int solver) {
-    if (threadIdx.x != 0) return;
-    S->lam = lam;
-    S->itmax = itmax;
-    S->atol = atol;
-    S->btol = btol;
-    S->solver = solver;
-    S->sig = S->tau = S->del = 0;
-    S->iter = 0;
-    S->xnorm2 = 0;
-    for (int i = 0; i < 5; ++i) S->errv[i] = 0;
-    S->xenorm2 = 0;
-    if (solver == CP_SOLVER_CGLS) {
-        // s_0 = J' b - lam^2 x_0 = -b_scaled, p_0 = s_0
-        S->gamma = S->bnorm2;
-        S->pnorm2 = S->bnorm2;
-        S->s0norm = sqrt(S->bnorm2);
-        S->cg_beta = 0;
-        S->cg_alpha = 0;
-        S->done = !(S->bnorm2 > 0);
-        return;
-    }
-    double beta1 = sqrt(S->cost);
-    double alpha = beta1 > 0 ? sqrt(S->bnorm2) / beta1 : 0.0;
-    S->beta1 = beta1;
-    S->alpha = alpha;
-    S->beta_cur = beta1;
-    S->beta_next = 0;
-    S->anorm2 = alpha * alpha;
-    S->done = (beta1 == 0 || alpha == 0);
-    if (solver == CP_SOLVER_LSMR) {
-        S->zetabar = alpha * beta1;
-        S->alphabar = alpha;
-        S->rho = 1;
-        S->rhobar_m = 1;
-        S->cbar = 1;
-        S->sbar = 0;
-        S->betadd = beta1;
-        S->betad = 0;
-        S->rhodold = 1;
-        S->tautildeold = 0;
-        S->thetatilde = 0;
-        S->zeta = 0;
-        S->dsum = 0;
-        S->maxrbar = 0;
-        S->minrbar = 1e100;
-        return;
-    }
-    S->rhobar = alpha;
-    S->phibar = beta1;
-    S->dnorm2 = S->z = S->res2 = 0;
-    S->c2 = -1;
-    S->s2 = 0;
-}
-
 // LSQR scalar recurrences of iteration `it` (one thread), from tt = |J^T u -
 // beta v|^2 and ww = |w|^2 of that iteration (Paige & Saunders 1982, with the
 // Krylov.jl-style stopping tests).
@@ -1370,11 +1316,12 @@ enum : int {
 };
 constexpr int FIN_BLOCK = 1024;
 
+// Block-order sums of the partials of a launch of nb blocks by one
+// FIN_BLOCK-thread block: ta (and tb, a max for the fill's g-cosine) in
+// thread 0.
 template <int OP>
-static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const double* __restrict__ part,
-                                                             const double* __restrict__ part2, int nb, int it) {
-    if ((OP == FIN_CGQ || OP == FIN_CGS) && Sc->done) return;
-    __shared__ double sh[2][FIN_BLOCK / 64];
+__device__ __forceinline__ void fin_reduce(const double* __restrict__ part, const double* __restrict__ part2, int nb,
+                                           double (*sh)[FIN_BLOCK / 64], double& ta, double& tb) {
     double a = 0, b = 0;
     for (int i = threadIdx.x; i < nb; i += FIN_BLOCK) {
         a += part[i];
@@ -1389,47 +1336,131 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const dou
         sh[1][threadIdx.x >> 6] = b;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    double ta = 0, tb = 0;
-    for (int i = 0; i < FIN_BLOCK / 64; ++i) {
-        ta += sh[0][i];
-        tb = OP == FIN_FILL ? fmax(tb, sh[1][i]) : tb + sh[1][i];
-    }
-    if (OP == FIN_COST) Sc->cost = ta;
-    if (OP == FIN_COST_CAND) Sc->cost_cand = ta;
-    if (OP == FIN_FILL) {
-        Sc->bnorm2 = ta;
-        Sc->gmax_bits = (unsigned long long)__double_as_longlong(tb);
-    }
-    if (OP == FIN_CGQ) {
-        // ta = p . H p, tb = |p|^2: the step a = gamma / (p'Hp + lam^2 |p|^2)
-        const double delta = ta + Sc->lam * Sc->lam * tb;
-        if (!(delta > 0)) {
-            Sc->done = 1;
-        } else {
-            Sc->cg_alpha = Sc->gamma / delta;
+    ta = 0;
+    tb = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < FIN_BLOCK / 64; ++i) {
+            ta += sh[0][i];
+            tb = OP == FIN_FILL ? fmax(tb, sh[1][i]) : tb + sh[1][i];
         }
-    }
-    if (OP == FIN_CGS) {
-        // ta = |s_new|^2, tb = s_new . p: the next direction p + = s_new + beta p
-        const double beta = ta / Sc->gamma;
-        Sc->pnorm2 = ta + 2 * beta * tb + beta * beta * Sc->pnorm2;
-        Sc->cg_beta = beta;
-        Sc->gamma = ta;
-        Sc->iter = it;
-        Sc->done = sqrt(ta) <= Sc->atol * Sc->s0norm || it >= Sc->itmax;
-    }
-    if (OP == FIN_ACCEPT) {
-        Sc->n_live = (int)ta;
-        Sc->n_frozen_now = (int)tb;
-    }
-    if (OP == FIN_COUNT) Sc->n_union = (unsigned long long)ta;
 }
 
-// The accepted candidate's cost becomes the current cost (its weighted
-// residual is reused as the next iteration's residual).
-static __global__ void k_take_candidate(Scal* Sc) {
-    if (threadIdx.x == 0) Sc->cost = Sc->cost_cand;
+// mirror (optional): the updated scalars also go to this pinned host record.
+template <int OP>
+static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const double* __restrict__ part,
+                                                             const double* __restrict__ part2, int nb, int it,
+                                                             Scal* mirror) {
+    if ((OP == FIN_CGQ || OP == FIN_CGS) && Sc->done) return;
+    __shared__ double sh[2][FIN_BLOCK / 64];
+    double ta, tb;
+    fin_reduce<OP>(part, part2, nb, sh, ta, tb);
+    if (threadIdx.x == 0) {
+        if (OP == FIN_COST) Sc->cost = ta;
+        if (OP == FIN_COST_CAND) Sc->cost_cand = ta;
+        if (OP == FIN_FILL) {
+            Sc->bnorm2 = ta;
+            Sc->gmax_bits = (unsigned long long)__double_as_longlong(tb);
+        }
+        if (OP == FIN_CGQ) {
+            // ta = p . H p, tb = |p|^2: the step a = gamma / (p'Hp + lam^2 |p|^2)
+            const double delta = ta + Sc->lam * Sc->lam * tb;
+            if (!(delta > 0)) {
+                Sc->done = 1;
+            } else {
+                Sc->cg_alpha = Sc->gamma / delta;
+            }
+        }
+        if (OP == FIN_CGS) {
+            // ta = |s_new|^2, tb = s_new . p: the next direction p + = s_new + beta p
+            const double beta = ta / Sc->gamma;
+            Sc->pnorm2 = ta + 2 * beta * tb + beta * beta * Sc->pnorm2;
+            Sc->cg_beta = beta;
+            Sc->gamma = ta;
+            Sc->iter = it;
+            Sc->done = sqrt(ta) <= Sc->atol * Sc->s0norm || it >= Sc->itmax;
+        }
+        if (OP == FIN_ACCEPT) {
+            // the live and newly frozen counts; the accepted candidate's cost
+            // becomes the current cost (its weighted residual is reused as the
+            // next iteration's residual)
+            Sc->n_live = (int)ta;
+            Sc->n_frozen_now = (int)tb;
+            Sc->cost = Sc->cost_cand;
+        }
+        if (OP == FIN_COUNT) Sc->n_union = (unsigned long long)ta;
+    }
+    if (mirror) {   // uniform
+        __syncthreads();
+        mirror_scal(Sc, mirror);
+    }
+}
+
+// fpart (the first trial of an outer iteration): the fill's partials over fnb
+// blocks are summed first (|b|^2, max g-cosine), as k_finish<FIN_FILL> does.
+static __global__ __launch_bounds__(FIN_BLOCK) void k_lsqr_init(Scal* S, double lam, int itmax, double atol,
+                                                                double btol, int solver,
+                                                                const double* __restrict__ fpart, int fnb) {
+    if (fpart) {   // uniform
+        __shared__ double sh[2][FIN_BLOCK / 64];
+        double ta, tb;
+        fin_reduce<FIN_FILL>(fpart, nullptr, fnb, sh, ta, tb);
+        if (threadIdx.x == 0) {
+            S->bnorm2 = ta;
+            S->gmax_bits = (unsigned long long)__double_as_longlong(tb);
+        }
+    }
+    if (threadIdx.x != 0) return;
+    S->lam = lam;
+    S->itmax = itmax;
+    S->atol = atol;
+    S->btol = btol;
+    S->solver = solver;
+    S->sig = S->tau = S->del = 0;
+    S->iter = 0;
+    S->xnorm2 = 0;
+    for (int i = 0; i < 5; ++i) S->errv[i] = 0;
+    S->xenorm2 = 0;
+    if (solver == CP_SOLVER_CGLS) {
+        // s_0 = J' b - lam^2 x_0 = -b_scaled, p_0 = s_0
+        S->gamma = S->bnorm2;
+        S->pnorm2 = S->bnorm2;
+        S->s0norm = sqrt(S->bnorm2);
+        S->cg_beta = 0;
+        S->cg_alpha = 0;
+        S->done = !(S->bnorm2 > 0);
+        return;
+    }
+    double beta1 = sqrt(S->cost);
+    double alpha = beta1 > 0 ? sqrt(S->bnorm2) / beta1 : 0.0;
+    S->beta1 = beta1;
+    S->alpha = alpha;
+    S->beta_cur = beta1;
+    S->beta_next = 0;
+    S->anorm2 = alpha * alpha;
+    S->done = (beta1 == 0 || alpha == 0);
+    if (solver == CP_SOLVER_LSMR) {
+        S->zetabar = alpha * beta1;
+        S->alphabar = alpha;
+        S->rho = 1;
+        S->rhobar_m = 1;
+        S->cbar = 1;
+        S->sbar = 0;
+        S->betadd = beta1;
+        S->betad = 0;
+        S->rhodold = 1;
+        S->tautildeold = 0;
+        S->thetatilde = 0;
+        S->zeta = 0;
+        S->dsum = 0;
+        S->maxrbar = 0;
+        S->minrbar = 1e100;
+        return;
+    }
+    S->rhobar = alpha;
+    S->phibar = beta1;
+    S->dnorm2 = S->z = S->res2 = 0;
+    S->c2 = -1;
+    S->s2 = 0;
 }
 
 // Final x update, step capping and candidate theta (:1224-1230, _cap_position_step! :700-716).

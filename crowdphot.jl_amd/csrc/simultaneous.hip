@@ -533,7 +533,7 @@ struct Sim {
             launch(k_count_mask, dim3(gcm), dim3(BLOCK), 0, st, (const uint8_t*)umask.as<uint8_t>(), Pn,
                    cm_part.as<double>());
             launch(k_finish<FIN_COUNT>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc, (const double*)cm_part.as<double>(),
-                   (const double*)nullptr, gcm, 0);
+                   (const double*)nullptr, gcm, 0, (Scal*)nullptr);
             read_S();
         }
         const int64_t U = (int64_t)hS->n_union;
@@ -701,9 +701,9 @@ struct Sim {
                     fill_launch<false, false>(S2, dim3(gb_fill), 0, M, rp, ep, cp, (const int*)jlist.as<int>(), na, R, S, fm, w.as<T>(), ny, nx, (const T*)wres_cur, V.as<VT>(), colnorm.as<double>(), bsc.as<double>(), gloc.as<uint8_t>(), o.g_tol, part.as<double>(), Sc);
             }
         };
-        auto finish = [&](auto kern, int nb, int it) {
+        auto finish = [&](auto kern, int nb, int it, Scal* mirror = nullptr) {
             launch(kern, dim3(1), dim3(FIN_BLOCK), 0, st, Sc, (const double*)part.as<double>(),
-                   (const double*)part2.as<double>(), nb, it);
+                   (const double*)part2.as<double>(), nb, it, mirror);
         };
 
         // --- the stars live at the last compaction (the fill, the star
@@ -790,13 +790,19 @@ struct Sim {
         bool have_live_count = false;   // Sc->n_live is the count after an accepted step
         TrialGraphs graphs;
         const bool use_graphs = !ctx->prof;   // per-launch event timing needs plain launches
-        bool D_init = false, have_resid = false;
+        bool D_init = false, have_resid = false, recs_valid = false;
         for (int iter = 1; iter <= o.max_iter; ++iter) {
             if (live_compacted < 0 || n_live_now < 0.8 * live_compacted) {
                 compact_live();
                 live_compacted = n_live_now;
             }
-            star_rec(jlist.as<int>(), theta.as<double>(), recs, crs, ers, 0, 0);
+            // the star records of theta: built once; afterwards an accepted
+            // step hands over its candidate's (theta_c is the new theta),
+            // rebuilt where stars froze, and a rejected step leaves them
+            if (!recs_valid) {
+                star_rec(jlist.as<int>(), theta.as<double>(), recs, crs, ers, 0, 0);
+                recs_valid = true;
+            }
             if (!have_resid) {
                 render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
                                           ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, (const long long*)tscan.as<long long>(),
@@ -805,8 +811,7 @@ struct Sim {
                 finish(k_finish<FIN_COST>, gb_render, 0);
                 have_resid = true;
             }
-            fill(false);
-            finish(k_finish<FIN_FILL>, gb_fill, 0);
+            fill(false);   // its partials are summed by the first trial's k_lsqr_init
             build_H();
             if (!D_init) {
                 CP_HIP(hipMemcpyAsync(D.get(), colnorm.get(), nb8, hipMemcpyDeviceToDevice, st));
@@ -822,8 +827,9 @@ struct Sim {
             bool accepted = false, f_conv = false;
             for (int trial = 1; trial <= o.max_trials; ++trial) {
                 const double lam_solve = std::sqrt(lam);
-                launch(k_lsqr_init, dim3(1), dim3(64), 0, st, Sc, lam_solve, o.inner_iterations, o.linear_tol,
-                       o.linear_tol, solver);
+                launch(k_lsqr_init, dim3(1), dim3(FIN_BLOCK), 0, st, Sc, lam_solve, o.inner_iterations, o.linear_tol,
+                       o.linear_tol, solver, trial == 1 ? (const double*)part.as<double>() : (const double*)nullptr,
+                       gb_fill);
                 // inner solve, capped step and candidate cost: one graph launch per trial
                 auto trial_body = [&]() {
                     with_p(p, [&](auto pc) {
@@ -848,13 +854,13 @@ struct Sim {
                                          first, partk.as<double>(), Sc);
                                 launch(k_finish<FIN_CGQ>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc,
                                        (const double*)partk.as<double>(), (const double*)partk.as<double>() + gb_jv,
-                                       gb_jv, it);
+                                       gb_jv, it, (Scal*)nullptr);
                                 launch_n("k_cg_update", k_cg_update<PV>, dim3(gb_jtu), dim3(BLOCK), 0, st, jl, lv, bs,
                                          (const double*)pcur, (const double*)gv1.as<double>(), gv0.as<double>(),
                                          xs.as<double>(), first, partk.as<double>(), Sc);
                                 launch(k_finish<FIN_CGS>, dim3(1), dim3(FIN_BLOCK), 0, st, Sc,
                                        (const double*)partk.as<double>(), (const double*)partk.as<double>() + gb_jtu,
-                                       gb_jtu, it);
+                                       gb_jtu, it, (Scal*)nullptr);
                                 continue;
                             }
                             // (g, v) of step it - 1 in buffer set (it - 1) & 1, step it writes set it & 1
@@ -887,11 +893,11 @@ struct Sim {
                                              ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Llive, g, ntiles, (const long long*)tscan_l.as<long long>(),
                                              umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cand,
                                              nullptr, part.as<double>(), Sc);
-                    finish(k_finish<FIN_COST_CAND>, gb_render, 0);
+                    finish(k_finish<FIN_COST_CAND>, gb_render, 0, hS);   // and the scalars to the host
                 };
                 if (use_graphs) graphs.run(st, wres_cur, trial_body);
                 else trial_body();
-                read_S();
+                CP_HIP(hipStreamSynchronize(st));
                 if (trial == 1) {
                     if (have_live_count) {
                         n_live_now = hS->n_live;
@@ -959,8 +965,12 @@ struct Sim {
                        (const double*)dsc.as<double>(), (const uint8_t*)gloc.as<uint8_t>(), live.as<uint8_t>(),
                        frozen_at.as<int>(), conv.as<uint8_t>(), part.as<double>());
                 finish(k_finish<FIN_ACCEPT>, (int)gstar, 0);
-                launch(k_take_candidate, dim3(1), dim3(64), 0, st, Sc);
+                // the candidate's residual and records are the new point's (the
+                // captured trial graphs follow, keyed by the residual buffer)
                 std::swap(wres_cur, wres_cand);
+                std::swap(recs, recs_c);
+                std::swap(crs, crs_c);
+                std::swap(ers, ers_c);
                 have_live_count = true;
                 // the stars frozen at this step (in the live list of the last
                 // compaction) leave the data: records, tile marks and the freeze

@@ -100,7 +100,9 @@ def test_admitted_requests_stay_within_their_estimate():
 def test_destroy_completes_queued_requests_and_wakes_waiters():
     import ctypes as C
     import threading
-    reqs = _requests() * 2   # a queue long enough that some requests are still waiting at destroy
+    # a queue long enough (about 0.2 s of fits on one slot) that some requests
+    # are still waiting when destroy comes 50 ms after the submits
+    reqs = _requests() * 8
     srv = pkg.serving.Server(0, 0, slots=1)
     tickets = [srv.submit(t["image"], t["psf"], t["sources"], t["fit_rad"], **_kw(t)) for t in reqs]
     rcs = {}
