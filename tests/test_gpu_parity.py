@@ -122,6 +122,12 @@ def test_isolated_noiseless_recovery_both_precisions(ctx):
     np.testing.assert_allclose(r64.flux, ro.flux, rtol=1e-10)
     np.testing.assert_allclose(r64.y, ro.y, atol=1e-10)
     np.testing.assert_allclose(r64.flux, fs, rtol=1e-6)
+    # every star freezes before max_iter: the loop ends on the iteration that
+    # starts with no live star (:1282-1283), with the oracle's count and each
+    # star's freeze iteration
+    assert r64.n_passes == ro.n_passes < 40, (r64.n_passes, ro.n_passes)
+    np.testing.assert_array_equal(r64.n_iter, ro.n_iter)
+    assert r64.converged.all() and ro.converged.all()
     r32 = pkg.fit_all_stars_simultaneous(img, psf, cat, 5, precision="mixed", ctx=ctx, **kw)
     assert r32.valid.all()
     np.testing.assert_allclose(r32.flux, fs, rtol=1e-5)
@@ -141,6 +147,8 @@ def test_isolated_noiseless_recovery_epsf(ctx):
     r64 = pkg.fit_all_stars_simultaneous(img, gpsf, cat, 5, precision="fp64", ctx=ctx, **kw)
     np.testing.assert_allclose(r64.flux, ro.flux, rtol=1e-10)
     np.testing.assert_allclose(r64.flux, fs, rtol=1e-6)
+    assert r64.n_passes == ro.n_passes, (r64.n_passes, ro.n_passes)
+    np.testing.assert_array_equal(r64.n_iter, ro.n_iter)
     r32 = pkg.fit_all_stars_simultaneous(img, gpsf, cat, 5, precision="mixed", ctx=ctx, **kw)
     np.testing.assert_allclose(r32.flux, fs, rtol=1e-5)
 
@@ -168,6 +176,8 @@ def test_gaussian_field_parity(ctx):
     r64 = _gpu_run(ctx, cfg, "fp64")
     _compare(r64, ro, 1e-7, 1e-7)
     assert r64.n_passes == ro.n_passes
+    np.testing.assert_array_equal(r64.n_iter, ro.n_iter)   # every star's freeze iteration
+    np.testing.assert_array_equal(r64.converged, ro.converged)
     r32 = _gpu_run(ctx, cfg, "mixed")
     rel, dpos = _compare(r32, ro, FLUX_RTOL, POS_ATOL)
     # errors and chi^2 of the epilogue
