@@ -16,8 +16,18 @@
 
 namespace cp {
 
+// Waves per SIMD of a batched-LM instance: three for the float32 fits of an
+// analytic model with up to four free parameters (168 registers; the spills
+// this costs sit in the per-fit setup, the damped step and the result write,
+// not in the per-pixel passes: C3 18.2 -> 15.4 ms), the compiler's choice
+// otherwise (the ePSF fits would spill inside their bicubic passes).
+template <typename T, int KIND, int NP>
+constexpr int lm_waves() {
+    return sizeof(T) == 4 && KIND != KIND_EPSF && NP <= 4 ? 3 : 1;
+}
+
 template <typename T, int KIND, int NP, bool UNIT_W>
-__global__ __launch_bounds__(BLOCK) void k_lm_batch(LMConst C, LMModel<T> Md, const float* __restrict__ pix,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(lm_waves<T, KIND, NP>()))) void k_lm_batch(LMConst C, LMModel<T> Md, const float* __restrict__ pix,
                                                     const int64_t* __restrict__ offsets,
                                                     const int32_t* __restrict__ cny, const int32_t* __restrict__ cnx,
                                                     const float* __restrict__ iv, const double* __restrict__ theta0,
