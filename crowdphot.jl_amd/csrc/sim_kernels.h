@@ -660,13 +660,15 @@ static_assert(RWAVE == 16, "a render tile is 4 x 4 sub-blocks, one per wave");
 // tile-gather render.  Each 1024-thread block walks a contiguous range of
 // RT x RT tiles; wave w owns the sub-block (w % 4, w / 4) of side SB = RT / 4
 // and each lane the pixels (row lane % SB, columns (64 / SB) k + lane / SB),
-// k < SB^2 / 64.  The
-// tile's star list is staged in chunks of 64 by one wave: only the stars this
-// mode renders (live / newly frozen / all), compacted in list order, each with
-// the mask of sub-blocks its model box touches, so a wave skips every other
-// star with one LDS read.  Star records are then read with wave-uniform
-// addresses (scalar loads, kept in SGPRs).  Per-pixel sums run over stars in
-// list (Morton) order: deterministic and the reference's summation order.
+// k < SB^2 / 64.  Every wave walks the tile's star list in chunks of 64, one
+// entry per lane: the lane reads its star's record, tests whether this mode
+// renders the star (live / newly frozen / all) and whether its model box
+// touches the wave's sub-block, and prepares the star's clipped box and
+// stencil constants; the wave then takes the touching stars in list order
+// (ballot) and reads each one's constants from its lane (readlane).  No block
+// barrier: the 16 waves of a tile run independently.  Per-pixel sums run over
+// stars in list (Morton) order: deterministic and the reference's summation
+// order.
 // Render cost of a tile: its star-list length plus a fixed per-tile term.
 constexpr int TILE_COST0 = 16;
 static __global__ void k_tile_cost(const int* __restrict__ tstart, const int* __restrict__ tend, int ntiles,
