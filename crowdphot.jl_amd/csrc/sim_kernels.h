@@ -1398,7 +1398,7 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_finish(Scal* Sc, const dou
 }
 
 // fpart (the first trial of an outer iteration): the fill's partials over fnb
-// blocks are summed first (|b|^2, max g-cosine), as k_finish<FIN_FILL> does.
+// blocks are summed first (|b|^2, max g-cosine: the FIN_FILL reduction).
 static __global__ __launch_bounds__(FIN_BLOCK) void k_lsqr_init(Scal* S, double lam, int itmax, double atol,
                                                                 double btol, int solver,
                                                                 const double* __restrict__ fpart, int fnb) {
