@@ -695,6 +695,18 @@ __device__ __forceinline__ void tile_share(const long long* __restrict__ scan, i
     t1 = upper(total * (b + 1) / nb);
 }
 
+// Every render block's share of the tiles, computed once per cost scan (the
+// scan changes only when the tile lists are rebuilt), so a render block
+// starts with one load instead of two binary searches.
+static __global__ void k_tile_shares(const long long* __restrict__ scan, int ntiles, int nb,
+                                     int2* __restrict__ shares) {
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    if (b >= nb) return;
+    int t0, t1;
+    tile_share(scan, ntiles, b, nb, t0, t1);
+    shares[b] = make_int2(t0, t1);
+}
+
 // ceil(65536 / h) for h = 1..16: (p * RENDER_INV_H[h]) >> 16 == p / h for 0 <= p < 256
 __constant__ const int RENDER_INV_H[17] = {0,    65536, 32768, 21846, 16384, 13108, 10923, 9363, 8192,
                                            7282, 6554,  5958,  5462,  5042,  4682,  4370,  4096};
@@ -786,7 +798,7 @@ template <typename T, int KIND, int MODE, bool LDSG, int QF = 0, int RTV = RT>
 __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     ModelConst<T> M, const StarRec<T>* __restrict__ recs, const EpsfRec<T>* __restrict__ ers,
     const CornerRec<T>* __restrict__ crs, const int* __restrict__ tstart, const int* __restrict__ tend,
-    const int* __restrict__ list, TileGeo g, int ntiles, const long long* __restrict__ tscan,
+    const int* __restrict__ list, TileGeo g, int ntiles, const int2* __restrict__ shares,
     const uint8_t* __restrict__ umask, T* __restrict__ dw, const T* __restrict__ w, const T* __restrict__ data,
     T* __restrict__ wt_resid, T* __restrict__ resid, double* __restrict__ partials, Scal* S,
     const int* __restrict__ tmark, int mark_iter) {
@@ -795,8 +807,9 @@ __global__ __launch_bounds__(RBLOCK, sizeof(T) == 4 ? 8 : 4) void k_render(
     __shared__ double sred[RWAVE];
     const int ngrid = LDSG ? M.ios_y * M.ios_x * M.Qy * M.Qx : 0;
     float* sgrid = reinterpret_cast<float*>(smem);
-    int t0, t1;
-    tile_share(tscan, ntiles, xcd_remap(blockIdx.x, gridDim.x), gridDim.x, t0, t1);
+    // this block's tiles (its share of the tile-cost scan)
+    const int2 share = shares[xcd_remap(blockIdx.x, gridDim.x)];
+    const int t0 = share.x, t1 = share.y;
     // the freeze render: a block none of whose tiles a newly frozen star
     // touches stages no grid (the launch follows every accepted step, whether
     // or not a star froze)

@@ -348,34 +348,34 @@ struct Sim {
     template <int MODE, int RTV>
     void render_launch_rt(bool ldsg, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
                           const EpsfRec<T>* ers, const CornerRec<T>* crs, const TileLists& L, const TileGeo& g,
-                          int ntiles, const long long* tscan, const uint8_t* umask, T* dw, const T* w, const T* data,
+                          int ntiles, const int2* shares, const uint8_t* umask, T* dw, const T* w, const T* data,
                           T* wres, T* resid, double* part, Scal* Sc, const int* tmark, int mark_iter) {
         static const char* nm[4] = {"k_render_resid", "k_render_cost", "k_render_freeze", "k_render_final"};
         if (ldsg && fixed_q)
             launch_n(nm[MODE], k_render<T, KIND, MODE, true, (KIND == KIND_EPSF ? RQF : 0), RTV>, grid, dim3(RBLOCK),
                      sh, st, M, recs, ers, crs, (const int*)L.start.as<int>(), (const int*)L.end.as<int>(),
-                     (const int*)L.list.as<int>(), g, ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark,
+                     (const int*)L.list.as<int>(), g, ntiles, shares, umask, dw, w, data, wres, resid, part, Sc, tmark,
                      mark_iter);
         else if (ldsg)
             launch_n(nm[MODE], k_render<T, KIND, MODE, true, 0, RTV>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
+                     ntiles, shares, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
         else
             launch_n(nm[MODE], k_render<T, KIND, MODE, false, 0, RTV>, grid, dim3(RBLOCK), sh, st, M, recs, ers, crs,
                      (const int*)L.start.as<int>(), (const int*)L.end.as<int>(), (const int*)L.list.as<int>(), g,
-                     ntiles, tscan, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
+                     ntiles, shares, umask, dw, w, data, wres, resid, part, Sc, tmark, mark_iter);
     }
 
     template <int MODE>
     void render_launch(bool ldsg, dim3 grid, size_t sh, const ModelConst<T>& M, const StarRec<T>* recs,
                        const EpsfRec<T>* ers, const CornerRec<T>* crs, const TileLists& L, const TileGeo& g,
-                       int ntiles, const long long* tscan, const uint8_t* umask, T* dw, const T* w, const T* data,
+                       int ntiles, const int2* shares, const uint8_t* umask, T* dw, const T* w, const T* data,
                        T* wres, T* resid, double* part, Scal* Sc, const int* tmark = nullptr, int mark_iter = 0) {
         if (rt_ == 32)
-            render_launch_rt<MODE, 32>(ldsg, grid, sh, M, recs, ers, crs, L, g, ntiles, tscan, umask, dw, w, data,
+            render_launch_rt<MODE, 32>(ldsg, grid, sh, M, recs, ers, crs, L, g, ntiles, shares, umask, dw, w, data,
                                        wres, resid, part, Sc, tmark, mark_iter);
         else
-            render_launch_rt<MODE, 64>(ldsg, grid, sh, M, recs, ers, crs, L, g, ntiles, tscan, umask, dw, w, data,
+            render_launch_rt<MODE, 64>(ldsg, grid, sh, M, recs, ers, crs, L, g, ntiles, shares, umask, dw, w, data,
                                        wres, resid, part, Sc, tmark, mark_iter);
     }
 
@@ -637,6 +637,11 @@ struct Sim {
         // (blocks per CU: the context's tuned launch shapes, else these defaults)
         const cp_tuning& tu = ctx->tuning;
         const int gb_render = (int)std::min<int64_t>(ntiles, (int64_t)n_cu * tuned_bpc(tu.render_blocks_per_cu, 4));
+        // each render block's tiles over all stars' lists (Lmod) and the live
+        // lists (rebuilt with them)
+        DevBuf shares_m(ctx, (size_t)gb_render * sizeof(int2)), shares_l(ctx, (size_t)gb_render * sizeof(int2));
+        launch(k_tile_shares, dim3(cdiv(gb_render, BLOCK)), dim3(BLOCK), 0, st, (const long long*)tscan.as<long long>(),
+               ntiles, gb_render, shares_m.as<int2>());
         // CG's H p: GKL lanes per row, a group for
         // every star; CG's update: one thread per star
         const int gb_jv = (int)std::max<int64_t>(1, cdiv(na, BLOCK / GKL));
@@ -756,6 +761,8 @@ struct Sim {
             b2 = cub.tmp.bytes();
             CP_HIP(hipcub::DeviceScan::InclusiveSum(cub.tmp.get(), b2, tcost_l.as<long long>(),
                                                     tscan_l.as<long long>(), ntiles, st));
+            launch(k_tile_shares, dim3(cdiv(gb_render, BLOCK)), dim3(BLOCK), 0, st,
+                   (const long long*)tscan_l.as<long long>(), ntiles, gb_render, shares_l.as<int2>());
         };
         auto build_H = [&]() {
             with_p(p, [&](auto pc) {
@@ -805,7 +812,7 @@ struct Sim {
             }
             if (!have_resid) {
                 render_launch<MODE_RESID>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
-                                          ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, (const long long*)tscan.as<long long>(),
+                                          ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Lmod, g, ntiles, (const int2*)shares_m.as<int2>(),
                                           umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
                                           part.as<double>(), Sc);
                 finish(k_finish<FIN_COST>, gb_render, 0);
@@ -890,7 +897,7 @@ struct Sim {
                            (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
                     star_rec(jlist.as<int>(), theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
                     render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
-                                             ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Llive, g, ntiles, (const long long*)tscan_l.as<long long>(),
+                                             ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Llive, g, ntiles, (const int2*)shares_l.as<int2>(),
                                              umask.as<uint8_t>(), dw.as<T>(), w.as<T>(), data.as<T>(), wres_cand,
                                              nullptr, part.as<double>(), Sc);
                     finish(k_finish<FIN_COST_CAND>, gb_render, 0, hS);   // and the scalars to the host
@@ -982,7 +989,7 @@ struct Sim {
                        tmark.as<int>());
                 render_launch<MODE_FREEZE>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(),
                                            ers.as<EpsfRec<T>>(), crs.as<CornerRec<T>>(), Llive, g, ntiles,
-                                           (const long long*)tscan_l.as<long long>(), umask.as<uint8_t>(),
+                                           (const int2*)shares_l.as<int2>(), umask.as<uint8_t>(),
                                            dw.as<T>(), w.as<T>(), data.as<T>(), wres_cur, nullptr,
                                            part.as<double>(), Sc, (const int*)tmark.as<int>(), iter);
             }
@@ -1007,7 +1014,7 @@ struct Sim {
         T* resid = ubuf.as<T>();
         render_launch<MODE_FINAL>(ldsg, dim3(gb_render), render_sh, M, recs.as<StarRec<T>>(), ers.as<EpsfRec<T>>(),
                                   crs.as<CornerRec<T>>(),
-                                  Lmod, g, ntiles, (const long long*)tscan.as<long long>(), umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
+                                  Lmod, g, ntiles, (const int2*)shares_m.as<int2>(), umask.as<uint8_t>(), dw.as<T>(), w.as<T>(),
                                   data.as<T>(), wres_cur, resid, part.as<double>(), Sc);
         // --- per-catalogue-star results, assembled on the device (:1289-1300,
         // :1331-1337, :1414-1430)
