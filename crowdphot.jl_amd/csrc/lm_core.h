@@ -148,17 +148,19 @@ __device__ __forceinline__ T pick6t(const T* v, int k) {
     return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : k == 3 ? v[3] : k == 4 ? v[4] : v[5];
 }
 
-// Pixel sources of accum!: pixel m of the fit region gives its 1-based
-// image coordinates, its data value and its weight (0 for a masked pixel).
+// Pixel sources of accum!: pixel m of the fit region (column-major, rows()
+// rows; row and col its position, which the caller steps along with m) gives
+// its 1-based image coordinates, its data value and its weight (0 for a
+// masked pixel).
 // A packed column-major cutout fitted in its own coordinates
 // (inds = axes(cutout)).
 struct CutoutSrc {
     const float* pix;
     const float* iv;   // NULL: unit weights
     int ny, npx;
-    __device__ __forceinline__ void at(int m, double& py, double& px, double& v, double& w) const {
-        const int col = m / ny;
-        py = (double)(m - col * ny + 1);
+    __device__ __forceinline__ int rows() const { return ny; }
+    __device__ __forceinline__ void at(int m, int row, int col, double& py, double& px, double& v, double& w) const {
+        py = (double)(row + 1);
         px = (double)(col + 1);
         v = (double)pix[m];
         w = 1.0;
@@ -174,12 +176,32 @@ struct CutoutSrc {
 struct CutoutSrcUnit {
     const float* pix;
     int ny, npx;
-    __device__ __forceinline__ void at(int m, double& py, double& px, double& v, double& w) const {
-        const int col = m / ny;
-        py = (double)(m - col * ny + 1);
+    __device__ __forceinline__ int rows() const { return ny; }
+    __device__ __forceinline__ void at(int m, int row, int col, double& py, double& px, double& v, double& w) const {
+        py = (double)(row + 1);
         px = (double)(col + 1);
         v = (double)pix[m];
         w = 1.0;
+    }
+};
+
+// Row and column of pixel m = lane + 64 k of a column-major region of ny
+// rows, stepped with k: one division per pass instead of one per pixel.
+struct PixCursor {
+    int row, col, dr, dc, ny;
+    __device__ __forceinline__ PixCursor(int lane, int rows) : ny(rows > 0 ? rows : 1) {
+        col = lane / ny;
+        row = lane - col * ny;
+        dc = 64 / ny;
+        dr = 64 - dc * ny;
+    }
+    __device__ __forceinline__ void next() {
+        row += dr;
+        col += dc;
+        if (row >= ny) {
+            row -= ny;
+            ++col;
+        }
     }
 };
 
@@ -202,9 +224,10 @@ __device__ __forceinline__ void accum(const LMConst& C, const LMModel<T>& Md, co
 #pragma unroll
     for (int i = 0; i < NP; ++i) acc.b[i] = 0;
     acc.cost = 0;
-    for (int m = lane; m < S.npx; m += 64) {
+    PixCursor pc(lane, S.rows());
+    for (int m = lane; m < S.npx; m += 64, pc.next()) {
         double py, px, v, w;
-        S.at(m, py, px, v, w);
+        S.at(m, pc.row, pc.col, py, px, v, w);
         T f, G[6];
         model_eval<T, KIND>(Md, cr, prm, (T)(py - full[0]), (T)(px - full[1]), f, G);
         const T r = f - (T)v;
@@ -391,9 +414,10 @@ __device__ void lm_fit(const LMConst& C, const LMModel<T>& Md, const double* ful
     bool bad = npx - NP <= 0;
     if (C.have_iv && !bad) {
         int nvalid = 0;
-        for (int m = lane; m < npx; m += 64) {
+        PixCursor pc(lane, S.rows());
+        for (int m = lane; m < npx; m += 64, pc.next()) {
             double py, px, v, w;
-            S.at(m, py, px, v, w);
+            S.at(m, pc.row, pc.col, py, px, v, w);
             nvalid += w > 0 ? 1 : 0;
         }
         bad = !(wave_sum_int(nvalid) > NP);

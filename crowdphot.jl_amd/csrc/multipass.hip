@@ -65,9 +65,9 @@ struct BoxSrc {
     const double* cut;
     const float* ivs;
     int by, ylo, xlo, npx;
-    __device__ __forceinline__ void at(int m, double& py, double& px, double& v, double& w) const {
-        const int col = m / by;
-        py = (double)(ylo + m - col * by);
+    __device__ __forceinline__ int rows() const { return by; }
+    __device__ __forceinline__ void at(int m, int row, int col, double& py, double& px, double& v, double& w) const {
+        py = (double)(ylo + row);
         px = (double)(xlo + col);
         v = cut[m];
         w = 1.0;
