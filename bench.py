@@ -671,7 +671,7 @@ def lm_bench(pkg, ctx, n, no_cpu, autotune, steps=3):
             fit()
             torch.cuda.synchronize()
         shape, _ = pkg.autotune.tune(ctx, run, space=pkg.autotune.LM_SPACE, repeats=2)
-        log(f"[lm] autotuned lm_blocks_per_cu = {shape['lm_blocks_per_cu']}")
+        log(f"[lm] autotuned lm_blocks_per_cu = {shape['lm_blocks_per_cu']}, lm_work_tile = {shape['lm_work_tile']}")
     t = time.perf_counter()
     for _ in range(steps):
         fit()
@@ -691,6 +691,7 @@ def lm_bench(pkg, ctx, n, no_cpu, autotune, steps=3):
                        f"FWHM U(1.8, 3.0) beta 2.5, free (y, x, flux, bkg), Poisson, unweighted",
            "fits_per_s": round(n / dt, 1), "ms_per_batch": round(dt * 1e3, 3),
            "lm_blocks_per_cu": shape["lm_blocks_per_cu"] if shape else "built-in default (8)",
+           "lm_work_tile": shape["lm_work_tile"] if shape else "built-in default (1)",
            "pixels_per_s": round(float(s2.sum()) / dt, 1),
            "mean_iterations": round(float(st["iterations"].mean()), 3),
            "converged_frac": round(float(st["converged"].mean()), 5),

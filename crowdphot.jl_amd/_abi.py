@@ -157,7 +157,8 @@ class LMOpts(C.Structure):
 class Tuning(C.Structure):
     """cp_tuning: blocks per compute unit of the grid-stride kernels and the
     simultaneous fit's work-tile sizes (render tile side, lanes per
-    normal-matrix block, lanes per star of the fill); 0 = default."""
+    normal-matrix block, lanes per star of the fill) and the batched LM's work
+    tile (cutouts per work-counter increment); 0 = default."""
     _fields_ = [
         ("hbuild_blocks_per_cu", C.c_int32),
         ("cg_blocks_per_cu", C.c_int32),
@@ -166,6 +167,7 @@ class Tuning(C.Structure):
         ("render_tile", C.c_int32),
         ("hbuild_lanes", C.c_int32),
         ("fill_lanes", C.c_int32),
+        ("lm_work_tile", C.c_int32),
     ]
 
 

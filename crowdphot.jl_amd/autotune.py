@@ -4,7 +4,7 @@ The tunable shapes are the cp_tuning fields of a context: blocks per compute
 unit of the normal-matrix assembly, CG's per-star update, the tile-gather
 renders and the batched per-star LM, and the simultaneous fit's work-tile
 sizes (the render tile side, lanes per normal-matrix block, lanes per star of
-the fill).  `tune` times complete calls of a workload -- whatever
+the fill) and the batched LM's work tile (cutouts per work-counter increment).  `tune` times complete calls of a workload -- whatever
 `run` does, typically one device-resident fit -- over candidate values, one
 field at a time (coordinate descent from the current shape, best of `repeats`
 wall-clock timings per candidate, the call synchronous), leaves the best
@@ -30,11 +30,11 @@ DEFAULT_SPACE = {
     "hbuild_lanes": (4, 16, 64),
     "fill_lanes": (16, 64),
 }
-LM_SPACE = {"lm_blocks_per_cu": (2, 4, 8, 16)}
+LM_SPACE = {"lm_blocks_per_cu": (2, 4, 8, 16), "lm_work_tile": (1, 2, 4, 8)}
 # the built-in defaults (what 0 stands for), as starting points; 0 for the
 # H-assembly lanes, whose default follows the stamp size
 DEFAULTS = {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 8, "render_blocks_per_cu": 4, "lm_blocks_per_cu": 8,
-            "render_tile": 64, "hbuild_lanes": 0, "fill_lanes": 16}
+            "render_tile": 64, "hbuild_lanes": 0, "fill_lanes": 16, "lm_work_tile": 1}
 
 
 def _time(run, repeats):

@@ -175,6 +175,8 @@ int cp_set_tuning(cp_ctx* c, const cp_tuning* t) {
                "hbuild_lanes must be 0 (default), 4, 16 or 64, got " + std::to_string(t->hbuild_lanes));
         CP_ARG(t->fill_lanes == 0 || t->fill_lanes == 16 || t->fill_lanes == 64,
                "fill_lanes must be 0 (default), 16 or 64, got " + std::to_string(t->fill_lanes));
+        CP_ARG(t->lm_work_tile >= 0 && t->lm_work_tile <= 64,
+               "lm_work_tile must be in [0, 64] (0 = default), got " + std::to_string(t->lm_work_tile));
         c->tuning = *t;
     });
 }

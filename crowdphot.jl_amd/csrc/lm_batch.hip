@@ -33,41 +33,45 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(lm_waves<
                                                     const float* __restrict__ iv, const double* __restrict__ theta0,
                                                     int n, double* __restrict__ theta_out,
                                                     double* __restrict__ cov_out, cp_lm_stat* __restrict__ stats,
-                                                    int* __restrict__ work) {
+                                                    int* __restrict__ work, int wtile) {
     const int lane = threadIdx.x & 63;
     for (;;) {
-        int c = 0;
-        if (lane == 0) c = atomicAdd(work, 1);
-        c = __shfl(c, 0, 64);
-        if (c >= n) return;
-        const int ny = cny[c], nx = cnx[c];
-        const int64_t off = offsets[c];
-        double full0[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (k < C.npf) full0[k] = theta0[(size_t)c * C.npf + k];
-        LMOut<NP> R;
-        if constexpr (UNIT_W) {
-            const CutoutSrcUnit S{pix + off, ny, ny * nx};
-            lm_fit<T, KIND, NP>(C, Md, full0, S, R);
-        } else {
-            const CutoutSrc S{pix + off, C.have_iv ? iv + off : nullptr, ny, ny * nx};
-            lm_fit<T, KIND, NP>(C, Md, full0, S, R);
-        }
-        // Lane 0 writes inside an if that falls through to the loop end (no
-        // divergent `continue`): the loop then has one back edge and the
-        // whole wave is converged again when the next __shfl reads lane 0.
-        if (lane == 0) {
-            // (a constant-bound loop: an index bounded by C.npf would put R in scratch)
+        // a work tile of wtile consecutive cutouts per counter increment
+        int c0 = 0;
+        if (lane == 0) c0 = atomicAdd(work, wtile);
+        c0 = __shfl(c0, 0, 64);
+        if (c0 >= n) return;
+        const int c1 = min(c0 + wtile, n);
+        for (int c = c0; c < c1; ++c) {
+            const int ny = cny[c], nx = cnx[c];
+            const int64_t off = offsets[c];
+            double full0[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int k = 0; k < 6; ++k)
-                if (k < C.npf) theta_out[(size_t)c * C.npf + k] = R.full[k];
-            if (R.st.status == 0 && cov_out && C.cov_mode != CP_COV_NONE) {
-                double cv[NP * NP];
-                lm_cov<NP>(C, R, ny * nx, cv);
-                for (int i = 0; i < NP * NP; ++i) cov_out[(size_t)c * NP * NP + i] = cv[i];
+                if (k < C.npf) full0[k] = theta0[(size_t)c * C.npf + k];
+            LMOut<NP> R;
+            if constexpr (UNIT_W) {
+                const CutoutSrcUnit S{pix + off, ny, ny * nx};
+                lm_fit<T, KIND, NP>(C, Md, full0, S, R);
+            } else {
+                const CutoutSrc S{pix + off, C.have_iv ? iv + off : nullptr, ny, ny * nx};
+                lm_fit<T, KIND, NP>(C, Md, full0, S, R);
             }
-            stats[c] = R.st;
+            // Lane 0 writes inside an if that falls through to the loop end (no
+            // divergent `continue`): the loops then have one back edge each and
+            // the whole wave is converged again when the next __shfl reads lane 0.
+            if (lane == 0) {
+                // (a constant-bound loop: an index bounded by C.npf would put R in scratch)
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if (k < C.npf) theta_out[(size_t)c * C.npf + k] = R.full[k];
+                if (R.st.status == 0 && cov_out && C.cov_mode != CP_COV_NONE) {
+                    double cv[NP * NP];
+                    lm_cov<NP>(C, R, ny * nx, cv);
+                    for (int i = 0; i < NP * NP; ++i) cov_out[(size_t)c * NP * NP + i] = cv[i];
+                }
+                stats[c] = R.st;
+            }
         }
     }
 }
@@ -75,12 +79,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(lm_waves<
 template <typename T, int KIND>
 void launch_lm(int np, dim3 g, hipStream_t s, const LMConst& C, const LMModel<T>& Md, const float* pix,
                const int64_t* off, const int32_t* ny, const int32_t* nx, const float* iv, const double* th0, int n,
-               double* th, double* cov, cp_lm_stat* st, int* work) {
+               double* th, double* cov, cp_lm_stat* st, int* work, int wtile) {
 #define CP_LM(NPV)                                                                                          \
     (C.have_iv ? launch_n("k_lm_batch", k_lm_batch<T, KIND, NPV, false>, g, dim3(BLOCK), 0, s, C, Md, pix, off, ny, \
-                          nx, iv, th0, n, th, cov, st, work)                                                   \
+                          nx, iv, th0, n, th, cov, st, work, wtile)                                            \
                : launch_n("k_lm_batch", k_lm_batch<T, KIND, NPV, true>, g, dim3(BLOCK), 0, s, C, Md, pix, off, ny,  \
-                          nx, iv, th0, n, th, cov, st, work))
+                          nx, iv, th0, n, th, cov, st, work, wtile))
     switch (np) {
         case 1: CP_LM(1); break;
         case 2: CP_LM(2); break;
@@ -105,7 +109,8 @@ void lm_batch_launch(cp_ctx* ctx, const cp_psf_desc* psf, const HostPsf& P, cons
         using T = decltype(tag);
         constexpr int K = decltype(kind)::value;
         const LMModel<T> Md{model_const<T>(P, grids, 0.0), G};
-        launch_lm<T, K>(np, g, st, C, Md, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work);
+        launch_lm<T, K>(np, g, st, C, Md, d_pix, d_off, d_ny, d_nx, d_iv, d_th0, n, d_th, d_cov, d_st, work,
+                        ctx->tuning.lm_work_tile > 0 ? ctx->tuning.lm_work_tile : 1);
     };
     using KG = std::integral_constant<int, KIND_GAUSS>;
     using KM = std::integral_constant<int, KIND_MOFFAT>;

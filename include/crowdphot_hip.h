@@ -189,6 +189,8 @@ typedef struct cp_tuning {
     int32_t render_tile;           /* render tile side in pixels: 32 or 64 (default 64) */
     int32_t hbuild_lanes;          /* lanes per normal-matrix block: 4, 16 or 64 (default by stamp size) */
     int32_t fill_lanes;            /* lanes per star of the fill for stamps of <= 64 pixels: 16 or 64 (default 16) */
+    /* work tile of the batched per-star LM */
+    int32_t lm_work_tile;          /* cutouts a wavefront takes per work-counter increment: 1..64 (default 1) */
 } cp_tuning;
 int cp_get_tuning(cp_ctx* ctx, cp_tuning* out);
 int cp_set_tuning(cp_ctx* ctx, const cp_tuning* t);

@@ -28,7 +28,7 @@ def test_tuning_round_trip_and_range(ctx):
     assert t["hbuild_blocks_per_cu"] == 4 and t["lm_blocks_per_cu"] == 2 and t["cg_blocks_per_cu"] == 0
     with pytest.raises(pkg.ArgumentError, match="blocks per CU"):
         ctx.set_tuning(render_blocks_per_cu=65)
-    for field, bad in (("render_tile", 48), ("hbuild_lanes", 8), ("fill_lanes", 32)):
+    for field, bad in (("render_tile", 48), ("hbuild_lanes", 8), ("fill_lanes", 32), ("lm_work_tile", 65)):
         with pytest.raises(pkg.ArgumentError, match=field):
             ctx.set_tuning(**{field: bad})
     with pytest.raises(pkg.ArgumentError, match="unknown tuning"):
@@ -41,8 +41,8 @@ def test_batched_lm_identical_under_other_grids(ctx):
     model = pkg.CircularMoffatPSF(alpha=1.0, beta=c["beta"])
     cut = [c["pix"][o:o + s * s].reshape(s, s).T for o, s in zip(c["offsets"], c["ny"])]
     ref = pkg.fit_star_batch(model, cut, c["theta0"], fixed=("alpha", "beta"), ctx=ctx)
-    for bpc in (1, 2, 16):
-        ctx.set_tuning(lm_blocks_per_cu=bpc)
+    for bpc, tile in ((1, 1), (2, 4), (16, 2), (8, 64)):
+        ctx.set_tuning(lm_blocks_per_cu=bpc, lm_work_tile=tile)
         got = pkg.fit_star_batch(model, cut, c["theta0"], fixed=("alpha", "beta"), ctx=ctx)
         np.testing.assert_array_equal(got[0], ref[0])
         np.testing.assert_array_equal(got[1], ref[1])
