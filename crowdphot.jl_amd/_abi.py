@@ -158,7 +158,8 @@ class Tuning(C.Structure):
     """cp_tuning: blocks per compute unit of the grid-stride kernels and the
     simultaneous fit's work-tile sizes (render tile side, lanes per
     normal-matrix block, lanes per star of the fill) and the batched LM's work
-    tile (cutouts per work-counter increment); 0 = default."""
+    tile (cutouts per work-counter increment), and the largest Golub-Kahan grid
+    whose step reductions fold into the next step; 0 = default."""
     _fields_ = [
         ("hbuild_blocks_per_cu", C.c_int32),
         ("cg_blocks_per_cu", C.c_int32),
@@ -168,6 +169,7 @@ class Tuning(C.Structure):
         ("hbuild_lanes", C.c_int32),
         ("fill_lanes", C.c_int32),
         ("lm_work_tile", C.c_int32),
+        ("gk_fold_blocks", C.c_int32),
     ]
 
 

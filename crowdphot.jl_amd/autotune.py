@@ -34,7 +34,7 @@ LM_SPACE = {"lm_blocks_per_cu": (2, 4, 8, 16), "lm_work_tile": (1, 2, 4, 8)}
 # the built-in defaults (what 0 stands for), as starting points; 0 for the
 # H-assembly lanes, whose default follows the stamp size
 DEFAULTS = {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 8, "render_blocks_per_cu": 4, "lm_blocks_per_cu": 8,
-            "render_tile": 64, "hbuild_lanes": 0, "fill_lanes": 16, "lm_work_tile": 1}
+            "render_tile": 64, "hbuild_lanes": 0, "fill_lanes": 16, "lm_work_tile": 1, "gk_fold_blocks": 0}
 
 
 def _time(run, repeats):

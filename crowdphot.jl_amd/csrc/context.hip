@@ -177,6 +177,8 @@ int cp_set_tuning(cp_ctx* c, const cp_tuning* t) {
                "fill_lanes must be 0 (default), 16 or 64, got " + std::to_string(t->fill_lanes));
         CP_ARG(t->lm_work_tile >= 0 && t->lm_work_tile <= 64,
                "lm_work_tile must be in [0, 64] (0 = default), got " + std::to_string(t->lm_work_tile));
+        CP_ARG(t->gk_fold_blocks >= 0 && t->gk_fold_blocks <= (1 << 20),
+               "gk_fold_blocks must be in [0, 1048576] (0 = default), got " + std::to_string(t->gk_fold_blocks));
         c->tuning = *t;
     });
 }

@@ -336,6 +336,76 @@ __device__ __forceinline__ double pick(const double (&y)[P], int k) {
 // Lanes per row of CG's H p (a quad: row_apply's DPP sums).
 constexpr int GKL = 4;
 
+// The scalar state of a fit staged in LDS (as 8-byte words, threads < NW)
+// and written back.
+constexpr int SCAL_WORDS = sizeof(Scal) / 8;
+static_assert(sizeof(Scal) % 8 == 0, "Scal staged as 8-byte words");
+__device__ __forceinline__ void scal_stage(Scal& Ls, const Scal* Sc, int tid) {
+    if (tid < SCAL_WORDS)
+        reinterpret_cast<unsigned long long*>(&Ls)[tid] = reinterpret_cast<const unsigned long long*>(Sc)[tid];
+}
+__device__ __forceinline__ void scal_store(Scal* Sc, const Scal& Ls, int tid) {
+    if (tid < SCAL_WORDS)
+        reinterpret_cast<unsigned long long*>(Sc)[tid] = reinterpret_cast<const unsigned long long*>(&Ls)[tid];
+}
+
+// The scalars of Golub-Kahan step it from its partials (nb blocks, NP each):
+// their block-order sum, beta_{k+1} and alpha_{k+1} by the expansion above,
+// then the LSQR or LSMR recurrences of step it, applied to the staged state
+// Ls (left unchanged once the solve is done).  Threads t < FINGK_BLOCK sum
+// the blocks t, t + FINGK_BLOCK, ..., so every caller sums in one order;
+// threads past them only pass the barriers.  Ends with a barrier.
+constexpr int FINGK_BLOCK = 512;
+
+// Thread 0's part of gk_scalars (sh: the NP x 8 wave sums).  No fused
+// multiply-adds here or in the recurrences it calls, so the reduction kernel
+// and a folded step round alike wherever the code is inlined.
+template <int SOLVER>
+__device__ __forceinline__ void gk_advance(Scal* Ls, const double* sh, int it) {
+#pragma clang fp contract(off)
+    constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
+    constexpr int NP = LSMR ? 10 : 5;
+    double t[NP];
+    for (int e = 0; e < NP; ++e) {
+        t[e] = 0;
+        for (int i = 0; i < FINGK_BLOCK / 64; ++i) t[e] += sh[e * (FINGK_BLOCK / 64) + i];
+    }
+    const double al = Ls->alpha;
+    const double beta = sqrt(fmax(0.0, t[0] - 2 * al * t[1] + al * al));
+    Ls->beta_next = beta;
+    const double gv = t[0] - al * t[1];
+    const double tt = beta > 0 ? fmax(0.0, t[2] / (beta * beta) - 2 * gv + beta * beta * t[3]) : 0.0;
+    if (LSMR) {
+        // |x|^2, x.h, x.hbar, |h|^2, h.hbar, |hbar|^2
+        const double q[6] = {t[5], t[6], t[7], t[4], t[8], t[9]};
+        lsmr_advance(Ls, tt, q, it);
+    } else {
+        lsqr_advance(Ls, tt, t[4], it);
+    }
+}
+
+template <int SOLVER>
+__device__ __forceinline__ void gk_scalars(Scal& Ls, const double* __restrict__ part, int nb, int it, int tid) {
+    constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
+    constexpr int NP = LSMR ? 10 : 5;
+    __shared__ double sh[NP][FINGK_BLOCK / 64];
+    double a[NP];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) a[e] = 0;
+    if (tid < FINGK_BLOCK)
+        for (int i = tid; i < nb; i += FINGK_BLOCK)
+#pragma unroll
+            for (int e = 0; e < NP; ++e) a[e] += part[(size_t)e * nb + i];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) a[e] = wave_sum(a[e]);
+    if ((tid & 63) == 0 && tid < FINGK_BLOCK)
+#pragma unroll
+        for (int e = 0; e < NP; ++e) sh[e][tid >> 6] = a[e];
+    __syncthreads();
+    if (tid == 0 && !Ls.done) gk_advance<SOLVER>(&Ls, &sh[0][0], it);
+    __syncthreads();
+}
+
 // One Golub-Kahan step it of LSQR / LSMR, one lane per row entry.
 //
 // Rows are consecutive entry ranges headed by the row's own entry (b = a,
@@ -357,35 +427,52 @@ constexpr int GKL = 4;
 // Load order matters: vector loads return in issue order, so the entry word
 // goes first, then the gather and, for a head, its row's x / w / hbar.
 // Two 1024-thread blocks per CU (64 registers).
+//
+// FOLD (grids of at most one block per CU, one 1024-thread block per CU at
+// 128 registers): the step also does the previous step's reduction, so a
+// solve is one launch per step plus one reduction at its end.  Every block
+// reduces the previous step's partials (part_prev, in the one order of
+// gk_scalars) from the state Sc before that reduction into LDS and runs on the
+// result; block 0 leaves the result in Sc_next for the next step.  The state
+// and the partials alternate between two buffers, so no block reads what
+// another block of the same launch writes.  The first step reduces nothing
+// and passes Sc on.
 constexpr int GKB = 1024;
 
-template <int P, int SOLVER>
-__global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_gk(
+template <int P, int SOLVER, bool FOLD>
+__global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(FOLD ? 4 : 8, FOLD ? 4 : 8))) void k_gk(
     int hspan, const int2* __restrict__ ent, const double* __restrict__ H, const double* __restrict__ Hd,
     const double* __restrict__ b_scaled, const double* __restrict__ gp, const double* __restrict__ vp,
     double* __restrict__ gn, double* __restrict__ vn, double* __restrict__ x, double* __restrict__ wv,
-    double* __restrict__ hb, int first, double* __restrict__ part, const Scal* __restrict__ Sc) {
+    double* __restrict__ hb, int first, double* __restrict__ part, const Scal* __restrict__ Sc,
+    const double* __restrict__ part_prev, Scal* __restrict__ Sc_next, int it) {
     constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
     constexpr int NP = LSMR ? 10 : 5, PP = P * P;
     __shared__ double sh[NP][GKB / 64];
     __shared__ double ty[P][GKB];   // H_ab v_b of every lane's entry
     const int l = threadIdx.x;
-    // the packed entries of the rows live at the last compaction take the
-    // first nbk blocks (dispatched over every XCD, each XCD's blocks renumbered
-    // to one contiguous range); the other blocks and a finished solve do
-    // nothing
-    const int ne = Sc->n_gk_entries, done = Sc->done;
-    if (done) return;   // uniform (the one-block reduction then does nothing)
+    const Scal* S = Sc;
+    // the packed entries of the rows live at the last compaction (ne, fixed
+    // within a solve) take the first nbk blocks (dispatched over every XCD,
+    // each XCD's blocks renumbered to one contiguous range); the other blocks
+    // and a finished solve do nothing.  FOLD: they find out after the fold,
+    // whose reduction runs while the block's gathers are in flight.
+    if constexpr (!FOLD)
+        if (Sc->done) return;   // uniform (the one-block reduction then does nothing)
+    const int ne = Sc->n_gk_entries;
     const int nbk = (int)(((long long)ne + hspan - 1) / hspan);
-    if ((int)blockIdx.x >= nbk) {   // uniform
-        if (l < NP) part[(size_t)l * gridDim.x + blockIdx.x] = 0.0;
-        return;
+    const bool lblk = (int)blockIdx.x < nbk;   // uniform
+    if constexpr (!FOLD) {
+        if (!lblk) {
+            if (l < NP) part[(size_t)l * gridDim.x + blockIdx.x] = 0.0;
+            return;
+        }
     }
-    const int blk = xcd_remap(blockIdx.x, nbk);
+    const int blk = lblk ? xcd_remap(blockIdx.x, nbk) : 0;
     const long long j = (long long)blk * hspan + l;
-    const bool inr = j < ne;
+    const bool inr = lblk && j < ne;
     const unsigned long long* ewp = reinterpret_cast<const unsigned long long*>(ent);
-    const size_t jc = (size_t)(inr ? j : ne - 1);
+    const size_t jc = (size_t)(inr ? j : (ne > 0 ? ne - 1 : 0));
     // the entry word (at a clamped, in-bounds index)
     const unsigned long long ew = ewp[jc];
     const int bn = (int)(unsigned)(ew & 0xffffffffu), cd = (int)(unsigned)(ew >> 32);
@@ -405,15 +492,28 @@ __global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     for (int k = 0; k < P; ++k) s2v[k] = first ? 0.0 : vp[(size_t)b * P + k];
 #pragma unroll
     for (int q = 0; q < PP; ++q) hv[q] = hsrc[q];
+    if constexpr (FOLD) {
+        __shared__ Scal Ls;
+        scal_stage(Ls, Sc, l);
+        if (first) __syncthreads();
+        else gk_scalars<SOLVER>(Ls, part_prev, gridDim.x, it - 1, l);
+        if (blockIdx.x == 0) scal_store(Sc_next, Ls, l);
+        S = &Ls;
+        if (Ls.done) return;   // uniform
+        if (!lblk) {
+            if (l < NP) part[(size_t)l * gridDim.x + blockIdx.x] = 0.0;
+            return;
+        }
+    }
     const size_t rb = (size_t)b * P;
-    const double alpha = Sc->alpha;
+    const double alpha = S->alpha;
     const double ia = alpha != 0 ? 1.0 / alpha : 1.0;
-    const double beta = first ? 0.0 : Sc->beta_cur;
+    const double beta = first ? 0.0 : S->beta_cur;
     const double ib = beta != 0 ? 1.0 / beta : 1.0;
-    const double ib1 = first ? 1.0 / Sc->beta1 : 0.0;
+    const double ib1 = first ? 1.0 / S->beta1 : 0.0;
     // v_it = c1 s1 + c2 vp, J^T u_it = d1 s1
     const double c1 = first ? -ib1 * ia : ib * ia, c2 = first ? 0.0 : -beta * ia, d1 = first ? -ib1 : ib;
-    const double sig = first ? 0.0 : Sc->sig, tau = first ? 0.0 : Sc->tau, del = first ? 0.0 : Sc->del;
+    const double sig = first ? 0.0 : S->sig, tau = first ? 0.0 : S->tau, del = first ? 0.0 : S->del;
     double v[P], y[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) v[k] = c1 * s1v[k] + c2 * s2v[k];
@@ -430,6 +530,10 @@ __global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 #pragma unroll
     for (int e = 0; e < NP; ++e) acc[e] = 0;
     if (head) {
+        // (no fused multiply-adds here: the unfolded and the folded kernel then
+        // round the row's updates alike, whatever their register budgets make
+        // of the code)
+#pragma clang fp contract(off)
         // the row's x / w / hbar, loaded while the row is summed (loaded with
         // the gather they would hold registers the gather needs: the step runs
         // at 64 registers so two blocks share a CU)
@@ -501,59 +605,19 @@ __global__ __launch_bounds__(GKB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     }
 }
 
-// One-block sum of k_gk's partials (block order) and the step's scalars:
-// beta_{k+1}, alpha_{k+1} by the expansion above, then the LSQR or LSMR
-// recurrences of step it.  The scalar state is staged in LDS while the
-// partials load, so the serial recurrences read and write LDS, not global
-// memory, and it is written back whole at the end.
+// One-block reduction of Golub-Kahan step it (after k_gk, or after the last
+// folded step): the state in Sc_in advanced by the step's partials, written to
+// Sc_out.  The scalar state is staged in LDS while the partials load, so the
+// serial recurrences read and write LDS, not global memory.
 // (512 threads: the LSMR recurrences need more than the 128 registers a
 // 1024-thread block leaves each lane)
-constexpr int FINGK_BLOCK = 512;
 template <int SOLVER>
-static __global__ __launch_bounds__(FINGK_BLOCK) void k_finish_gk(Scal* Sc, const double* __restrict__ part, int nb,
-                                                                  int it) {
-    constexpr bool LSMR = SOLVER == CP_SOLVER_LSMR;
-    constexpr int NP = LSMR ? 10 : 5, NW = sizeof(Scal) / 8;
-    static_assert(sizeof(Scal) % 8 == 0 && NW <= FINGK_BLOCK, "Scal staged as 8-byte words");
+static __global__ __launch_bounds__(FINGK_BLOCK) void k_finish_gk(const Scal* Sc_in, Scal* Sc_out,
+                                                                  const double* __restrict__ part, int nb, int it) {
     __shared__ Scal Ls;
-    __shared__ double sh[NP][FINGK_BLOCK / 64];
-    unsigned long long* lw = reinterpret_cast<unsigned long long*>(&Ls);
-    unsigned long long* gw = reinterpret_cast<unsigned long long*>(Sc);
-    if (threadIdx.x < NW) lw[threadIdx.x] = gw[threadIdx.x];
-    double a[NP];
-#pragma unroll
-    for (int e = 0; e < NP; ++e) a[e] = 0;
-    for (int i = threadIdx.x; i < nb; i += FINGK_BLOCK)
-#pragma unroll
-        for (int e = 0; e < NP; ++e) a[e] += part[(size_t)e * nb + i];
-#pragma unroll
-    for (int e = 0; e < NP; ++e) a[e] = wave_sum(a[e]);
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int e = 0; e < NP; ++e) sh[e][threadIdx.x >> 6] = a[e];
-    __syncthreads();
-    if (Ls.done) return;   // uniform
-    if (threadIdx.x == 0) {
-        double t[NP];
-        for (int e = 0; e < NP; ++e) {
-            t[e] = 0;
-            for (int i = 0; i < FINGK_BLOCK / 64; ++i) t[e] += sh[e][i];
-        }
-        const double al = Ls.alpha;
-        const double beta = sqrt(fmax(0.0, t[0] - 2 * al * t[1] + al * al));
-        Ls.beta_next = beta;
-        const double gv = t[0] - al * t[1];
-        const double tt = beta > 0 ? fmax(0.0, t[2] / (beta * beta) - 2 * gv + beta * beta * t[3]) : 0.0;
-        if (LSMR) {
-            // |x|^2, x.h, x.hbar, |h|^2, h.hbar, |hbar|^2
-            const double q[6] = {t[5], t[6], t[7], t[4], t[8], t[9]};
-            lsmr_advance(&Ls, tt, q, it);
-        } else {
-            lsqr_advance(&Ls, tt, t[4], it);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < NW) gw[threadIdx.x] = lw[threadIdx.x];
+    scal_stage(Ls, Sc_in, threadIdx.x);
+    gk_scalars<SOLVER>(Ls, part, nb, it, threadIdx.x);
+    scal_store(Sc_out, Ls, threadIdx.x);
 }
 
 // Packing the entries of the live rows (at every compaction of the live

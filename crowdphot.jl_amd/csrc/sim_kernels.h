@@ -1183,6 +1183,7 @@ __global__ __launch_bounds__(FBLOCK, sizeof(T) == 4 && EPL <= 2 ? 8 : 4) void k_
 // beta v|^2 and ww = |w|^2 of that iteration (Paige & Saunders 1982, with the
 // Krylov.jl-style stopping tests).
 __device__ void lsqr_advance(Scal* Sc, double tt, double ww, int it) {
+#pragma clang fp contract(off)   // rounded alike wherever it is inlined
     const double alpha = Sc->alpha, beta = Sc->beta_next;
     const double lam = Sc->lam;
     double alpha_new = alpha;
@@ -1247,6 +1248,7 @@ __device__ void lsqr_advance(Scal* Sc, double tt, double ww, int it) {
 // |hbar|^2 of the vectors after the previous iteration's update, which give
 // |x_it| once this iteration's del and sigma are known.
 __device__ void lsmr_advance(Scal* Sc, double tt, const double* q, int it) {
+#pragma clang fp contract(off)   // rounded alike wherever it is inlined
     const double beta = Sc->beta_next, lam = Sc->lam;
     double alpha = Sc->alpha;
     if (beta != 0) alpha = sqrt(tt);

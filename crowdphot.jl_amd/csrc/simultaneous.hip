@@ -665,8 +665,12 @@ struct Sim {
         const int gk_margin = std::min(GKB / 2, std::max(64, 64 * (int)cdiv((int64_t)prs.max_row - 1, 64)));
         const int gk_span = GKB - gk_margin;
         const int gb_gk = (int)std::max<int64_t>(1, cdiv(prs.npairs, gk_span));
+        // grids of at most one block per CU fold each step's reduction into
+        // the next step (k_gk<..., true>): its partials and scalar state
+        // alternate between two buffers
+        const bool gk_fold = gb_gk <= (tu.gk_fold_blocks > 0 ? tu.gk_fold_blocks : n_cu);
         // its partials (ten per block for LSMR, five for LSQR)
-        DevBuf partg(ctx, (size_t)10 * gb_gk * 8);
+        DevBuf partg(ctx, (size_t)(gk_fold ? 20 : 10) * gb_gk * 8), scal_gk(ctx, 2 * sizeof(Scal));
 
         // star records of the stars in lst (null: all)
         auto star_rec = [&](const int* lst, const double* th, DevBuf& r, DevBuf& c, DevBuf& e, int frozen_iter,
@@ -877,13 +881,32 @@ struct Sim {
                             DevBuf& vnew = (it & 1) ? vv1 : vv0;
                             auto gk = [&](auto sv) {
                                 constexpr int SV = decltype(sv)::value;
-                                launch_n("k_gk", k_gk<PV, SV>, dim3(gb_gk), dim3(GKB), 0, st, gk_span, en, Hp, Hdp,
-                                         bs,
-                                         (const double*)gprev.as<double>(), (const double*)vprev.as<double>(),
+                                if (gk_fold) {
+                                    // step it: partials to set it & 1, the state before step it - 1's
+                                    // reduction from scal_gk[it & 1] (the fit's own for the first step),
+                                    // after it to scal_gk[(it + 1) & 1]
+                                    double* pc = partg.as<double>() + (size_t)(it & 1) * 10 * gb_gk;
+                                    const double* pp = partg.as<double>() + (size_t)((it - 1) & 1) * 10 * gb_gk;
+                                    Scal* sg = scal_gk.as<Scal>();
+                                    const Scal* sin = first ? (const Scal*)Sc : sg + (it & 1);
+                                    Scal* snext = sg + ((it + 1) & 1);
+                                    launch_n("k_gk", k_gk<PV, SV, true>, dim3(gb_gk), dim3(GKB), 0, st, gk_span, en,
+                                             Hp, Hdp, bs, (const double*)gprev.as<double>(),
+                                             (const double*)vprev.as<double>(), gnew.as<double>(),
+                                             vnew.as<double>(), xs.as<double>(), wl.as<double>(), hbar.as<double>(),
+                                             first, pc, sin, pp, snext, it);
+                                    if (it == o.inner_iterations)
+                                        launch_n("k_finish_gk", k_finish_gk<SV>, dim3(1), dim3(FINGK_BLOCK), 0, st,
+                                                 (const Scal*)snext, Sc, (const double*)pc, gb_gk, it);
+                                    return;
+                                }
+                                launch_n("k_gk", k_gk<PV, SV, false>, dim3(gb_gk), dim3(GKB), 0, st, gk_span, en, Hp,
+                                         Hdp, bs, (const double*)gprev.as<double>(), (const double*)vprev.as<double>(),
                                          gnew.as<double>(), vnew.as<double>(), xs.as<double>(), wl.as<double>(),
-                                         hbar.as<double>(), first, partg.as<double>(), (const Scal*)Sc);
-                                launch_n("k_finish_gk", k_finish_gk<SV>, dim3(1), dim3(FINGK_BLOCK), 0, st, Sc,
-                                         (const double*)partg.as<double>(), gb_gk, it);
+                                         hbar.as<double>(), first, partg.as<double>(), (const Scal*)Sc,
+                                         (const double*)nullptr, (Scal*)nullptr, it);
+                                launch_n("k_finish_gk", k_finish_gk<SV>, dim3(1), dim3(FINGK_BLOCK), 0, st,
+                                         (const Scal*)Sc, Sc, (const double*)partg.as<double>(), gb_gk, it);
                             };
                             if (solver == CP_SOLVER_LSMR) gk(std::integral_constant<int, CP_SOLVER_LSMR>{});
                             else gk(std::integral_constant<int, CP_SOLVER_LSQR>{});

@@ -142,9 +142,9 @@ end
 # fit's work-tile sizes, 0 = built-in default).
 mutable struct CpTuning
     hbuild_blocks_per_cu::Int32; cg_blocks_per_cu::Int32; render_blocks_per_cu::Int32; lm_blocks_per_cu::Int32
-    render_tile::Int32; hbuild_lanes::Int32; fill_lanes::Int32; lm_work_tile::Int32
+    render_tile::Int32; hbuild_lanes::Int32; fill_lanes::Int32; lm_work_tile::Int32; gk_fold_blocks::Int32
 end
-CpTuning() = CpTuning(0, 0, 0, 0, 0, 0, 0, 0)
+CpTuning() = CpTuning(0, 0, 0, 0, 0, 0, 0, 0, 0)
 
 function get_tuning(ctx::Context)
     t = CpTuning()

@@ -191,6 +191,10 @@ typedef struct cp_tuning {
     int32_t fill_lanes;            /* lanes per star of the fill for stamps of <= 64 pixels: 16 or 64 (default 16) */
     /* work tile of the batched per-star LM */
     int32_t lm_work_tile;          /* cutouts a wavefront takes per work-counter increment: 1..64 (default 1) */
+    /* Golub-Kahan steps of the simultaneous fit whose grid has at most this many blocks fold each step's
+       reduction into the next step's launch (results identical either way): 1..1048576 (default: one
+       block per CU) */
+    int32_t gk_fold_blocks;
 } cp_tuning;
 int cp_get_tuning(cp_ctx* ctx, cp_tuning* out);
 int cp_set_tuning(cp_ctx* ctx, const cp_tuning* t);

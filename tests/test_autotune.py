@@ -14,7 +14,7 @@ class FakeCtx:
     def __init__(self):
         self.t = {k: 0 for k in ("hbuild_blocks_per_cu", "cg_blocks_per_cu", "render_blocks_per_cu",
                                  "lm_blocks_per_cu", "render_tile", "hbuild_lanes", "fill_lanes",
-                                 "lm_work_tile")}
+                                 "lm_work_tile", "gk_fold_blocks")}
         self.history = []
 
     def get_tuning(self):

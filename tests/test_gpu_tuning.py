@@ -1,8 +1,8 @@
 """Launch shapes (cp_tuning) on the GPU: range checks, the batched per-star
 LM unchanged bit for bit under another grid, the simultaneous fit under
 other shapes and work-tile sizes (render tile side, lanes per normal-matrix
-block and per star of the fill) still on the FP64 oracle, and the autotuner
-on a real call."""
+block and per star of the fill) still on the FP64 oracle, folded Golub-Kahan
+steps bit for bit equal to unfolded ones, and the autotuner on a real call."""
 import numpy as np
 import pytest
 
@@ -28,7 +28,8 @@ def test_tuning_round_trip_and_range(ctx):
     assert t["hbuild_blocks_per_cu"] == 4 and t["lm_blocks_per_cu"] == 2 and t["cg_blocks_per_cu"] == 0
     with pytest.raises(pkg.ArgumentError, match="blocks per CU"):
         ctx.set_tuning(render_blocks_per_cu=65)
-    for field, bad in (("render_tile", 48), ("hbuild_lanes", 8), ("fill_lanes", 32), ("lm_work_tile", 65)):
+    for field, bad in (("render_tile", 48), ("hbuild_lanes", 8), ("fill_lanes", 32), ("lm_work_tile", 65),
+                       ("gk_fold_blocks", -1)):
         with pytest.raises(pkg.ArgumentError, match=field):
             ctx.set_tuning(**{field: bad})
     with pytest.raises(pkg.ArgumentError, match="unknown tuning"):
@@ -71,6 +72,34 @@ def test_simultaneous_under_other_shapes_matches_oracle(ctx):
         # shapes regroup only rounding-level partial sums: every star agrees)
         _compare(_gpu_run(ctx, crowd, "mixed"), ref, FLUX_RTOL, POS_ATOL, frac=0.98)
         _compare(_gpu_run(ctx, crowd, "fp64"), ref64, 1e-8, 1e-8)
+
+
+@pytest.mark.parametrize("solver", ["lsqr", "lsmr"])
+def test_folded_golub_kahan_steps_identical(ctx, solver):
+    """Golub-Kahan steps with each step's reduction folded into the next
+    launch give the results of one reduction launch per step, bit for bit:
+    a 512x512 tile (a grid of several blocks, folded by default) with
+    folding off, and the 4096x4096-density crop with folding forced on."""
+    t = sim.serving_tile(3, 512, "gauss")
+    kw = dict(fixed=t["fixed"], inv_var=t["inv_var"], model_rad=t["model_rad"], max_iter=t["max_iter"],
+              solver=solver, ctx=ctx)
+    n = int(round(2e5 * (384 * 384) / 4096 ** 2))
+    crowd = sim.config_epsf_crowded(seed=5, shape=(384, 384), n=n, max_iter=10)
+    ref = pkg.fit_all_stars_simultaneous(t["image"], t["psf"], t["sources"], t["fit_rad"], **kw)
+    refc = _gpu_run_solver(ctx, crowd, solver)
+    for fold in (1, 1 << 20):
+        ctx.set_tuning(gk_fold_blocks=fold)
+        got = pkg.fit_all_stars_simultaneous(t["image"], t["psf"], t["sources"], t["fit_rad"], **kw)
+        gotc = _gpu_run_solver(ctx, crowd, solver)
+        for a, b in ((got, ref), (gotc, refc)):
+            for f in ("y", "x", "flux", "flux_err", "chisq", "n_iter", "valid", "converged"):
+                np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f"{f} fold {fold}")
+            assert a.n_passes == b.n_passes
+
+
+def _gpu_run_solver(ctx, cfg, solver):
+    from test_gpu_parity import _gpu_run
+    return _gpu_run(ctx, cfg, "mixed", solver=solver)
 
 
 def test_autotune_on_a_real_call(ctx):
