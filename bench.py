@@ -115,6 +115,39 @@ def measured_traffic(kernel):
     return None
 
 
+def profile_tile(pkg, side, model, fits):
+    """`fits` plain fits of one tile of the serving mix (inputs resident in
+    HBM, row-major), for kernel traces of the small-tile path."""
+    import torch
+    abi = pkg._abi
+    dev = torch.device("cuda:0")
+    t = pkg.simulation.serving_tile(7, side, model)
+    psf_model, mask = pkg.photometry._sim_psf_and_mask(t["psf"], t["fixed"])
+    o = abi.sim_opts_default()
+    o.model_rad = t["model_rad"]
+    o.max_iter = t["max_iter"]
+    o.free_mask = mask
+    o.fixed_bkg = 0.0
+    o.layout = abi.CP_LAYOUT_ROWMAJOR
+    desc = psf_model.desc()
+    src = t["sources"]
+    dv = [torch.from_numpy(np.ascontiguousarray(a, np.float32).ravel()).to(dev) for a in (t["image"], t["inv_var"])]
+    dv += [torch.from_numpy(np.ascontiguousarray(src[k])).to(dev) for k in ("y", "x", "flux")]
+    n = t["n"]
+    cols = {k: np.zeros(n) for k in ("y", "x", "flux", "flux_err")}
+    r = abi.SimResult()
+    for k, v in cols.items():
+        setattr(r, k, v.ctypes.data_as(abi.f64p))
+    ctx = pkg.Context(0)
+    torch.cuda.synchronize()
+    for i in range(fits):
+        a = time.perf_counter()
+        ctx.check(ctx.lib.cp_fit_simultaneous_device(
+            ctx.handle, C.byref(desc), *[C.c_void_p(d.data_ptr()) for d in dv[:2]], side, side,
+            *[C.c_void_p(d.data_ptr()) for d in dv[2:]], n, float(t["fit_rad"]), C.byref(o), C.byref(r)))
+        log(f"[bench] profile tile {side}^2 {model}, {n} stars: fit {i} {(time.perf_counter() - a) * 1e3:.3f} ms")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +179,9 @@ def main():
                     help="one sequential 3-pass fit_all_stars, for rocprofv3 runs")
     ap.add_argument("--profile-lm", action="store_true",
                     help="one batched per-star LM call over the --lm-cutouts batch, for rocprofv3 runs")
+    ap.add_argument("--profile-tile", type=int, default=0,
+                    help="--profile-fits fits of one serving-mix tile of this side, for rocprofv3 runs")
+    ap.add_argument("--profile-tile-model", default="gauss", choices=("gauss", "epsf"))
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -166,6 +202,9 @@ def main():
     pkg = load_pkg()
     abi = pkg._abi
     sim = pkg.simulation
+    if args.profile_tile:
+        profile_tile(pkg, args.profile_tile, args.profile_tile_model, max(args.profile_fits, 1))
+        return
     side, nstar = args.side, args.stars
     t0 = time.time()
     cfg = sim.config_epsf_crowded(seed=2, shape=(side, side), n=nstar, max_iter=10)
