@@ -95,6 +95,17 @@ def test_folded_golub_kahan_steps_identical(ctx, solver):
             for f in ("y", "x", "flux", "flux_err", "chisq", "n_iter", "valid", "converged"):
                 np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f"{f} fold {fold}")
             assert a.n_passes == b.n_passes
+    # the shortest solves: one step (the closing reduction reads the first
+    # step's state copy) and two
+    for inner in (1, 2):
+        ctx.set_tuning(gk_fold_blocks=0)
+        ref = pkg.fit_all_stars_simultaneous(t["image"], t["psf"], t["sources"], t["fit_rad"],
+                                             inner_iterations=inner, **kw)
+        ctx.set_tuning(gk_fold_blocks=1)
+        got = pkg.fit_all_stars_simultaneous(t["image"], t["psf"], t["sources"], t["fit_rad"],
+                                             inner_iterations=inner, **kw)
+        for f in ("y", "x", "flux", "flux_err", "n_iter"):
+            np.testing.assert_array_equal(getattr(got, f), getattr(ref, f), err_msg=f"{f} inner {inner}")
 
 
 def _gpu_run_solver(ctx, cfg, solver):
