@@ -356,6 +356,7 @@ __device__ __forceinline__ void scal_store(Scal* Sc, const Scal& Ls, int tid) {
 // the blocks t, t + FINGK_BLOCK, ..., so every caller sums in one order;
 // threads past them only pass the barriers.  Ends with a barrier.
 constexpr int FINGK_BLOCK = 512;
+static_assert(SCAL_WORDS <= FINGK_BLOCK, "the scalar state is staged by one word per thread");
 
 // Thread 0's part of gk_scalars (sh: the NP x 8 wave sums).  No fused
 // multiply-adds here or in the recurrences it calls, so the reduction kernel
