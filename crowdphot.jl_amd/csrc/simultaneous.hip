@@ -385,8 +385,12 @@ struct Sim {
         const int64_t Pn = (int64_t)ny * nx;
         const int R = (int)std::ceil(fit_rad), S = 2 * R + 1, S2 = S * S;
         CP_ARG(S2 <= 64 * MAX_EPL, "fit_rad too large for this build (ceil(fit_rad) <= 15)");
-        // work-tile sizes of the context (cp_tuning; 0 = these defaults)
-        rt_ = ctx->tuning.render_tile > 0 ? ctx->tuning.render_tile : RT;
+        // work-tile sizes of the context (cp_tuning; 0 = these defaults).  An
+        // image of fewer 64-pixel tiles than CUs renders in 32-pixel tiles,
+        // four times the blocks (a 512 x 512 tile: 3-7 % faster per fit; from
+        // 1024 x 1024 on the 64-pixel tiles are as fast or faster)
+        const int64_t nt64 = (int64_t)((ny + RT - 1) / RT) * ((nx + RT - 1) / RT);
+        rt_ = ctx->tuning.render_tile > 0 ? ctx->tuning.render_tile : (nt64 < ctx->n_cu ? 32 : RT);
         fill_gl_ = ctx->tuning.fill_lanes > 0 ? ctx->tuning.fill_lanes : 16;
         // free set (y, x, flux in getproperties order)
         FreeMap fm{};

@@ -186,7 +186,8 @@ typedef struct cp_tuning {
     int32_t render_blocks_per_cu;  /* tile-gather renders, 1024-thread blocks (default 4) */
     int32_t lm_blocks_per_cu;      /* batched per-star LM, 256-thread blocks (default 8) */
     /* work-tile sizes of the simultaneous fit */
-    int32_t render_tile;           /* render tile side in pixels: 32 or 64 (default 64) */
+    int32_t render_tile;           /* render tile side in pixels: 32 or 64 (default: 32 for images of fewer
+                                      64-pixel tiles than CUs, else 64) */
     int32_t hbuild_lanes;          /* lanes per normal-matrix block: 4, 16 or 64 (default by stamp size) */
     int32_t fill_lanes;            /* lanes per star of the fill for stamps of <= 64 pixels: 16 or 64 (default 16) */
     /* work tile of the batched per-star LM */

@@ -48,8 +48,8 @@ def test_work_tile_sizes_are_searched():
     shape, _ = at.tune(ctx, _run_for(ctx, best), repeats=2)
     for k, v in best.items():
         assert shape[k] == v, (k, shape)
-    # 0 (the stamp-size rule) is where the H-assembly lanes start
-    assert ctx.history[0]["hbuild_lanes"] == 0 and ctx.history[0]["render_tile"] == 64
+    # 0 (the stamp-size and image-size rules) is where the H-assembly lanes and the render tile start
+    assert ctx.history[0]["hbuild_lanes"] == 0 and ctx.history[0]["render_tile"] == 0
 
 
 def test_defaults_are_the_starting_point_and_kept_without_gain():
