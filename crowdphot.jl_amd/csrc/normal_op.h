@@ -621,6 +621,25 @@ static __global__ __launch_bounds__(FINGK_BLOCK) void k_finish_gk(const Scal* Sc
     scal_store(Sc_out, Ls, threadIdx.x);
 }
 
+// The end of a folded solve: every block reduces the last step's partials
+// (nb blocks, step it) from the state Sc_in before that reduction, block 0
+// writes the result to Sc_out, and each thread takes one star's final x
+// update, capped step and candidate theta (step_star, as k_step after
+// k_finish_gk).
+template <int SOLVER>
+static __global__ __launch_bounds__(FINGK_BLOCK) void k_step_fold(
+    int na, int p, int k_y, int k_x, double max_step, const double* __restrict__ wv, const double* __restrict__ hb,
+    double* __restrict__ x, const double* __restrict__ colnorm, const double* __restrict__ theta,
+    double* __restrict__ dsc, double* __restrict__ theta_c, const uint8_t* __restrict__ live,
+    const Scal* __restrict__ Sc_in, Scal* __restrict__ Sc_out, const double* __restrict__ part, int nb, int it) {
+    __shared__ Scal Ls;
+    scal_stage(Ls, Sc_in, threadIdx.x);
+    gk_scalars<SOLVER>(Ls, part, nb, it, threadIdx.x);
+    if (blockIdx.x == 0) scal_store(Sc_out, Ls, threadIdx.x);
+    const int a = blockIdx.x * FINGK_BLOCK + threadIdx.x;
+    if (a < na) step_star(a, p, k_y, k_x, max_step, wv, hb, x, colnorm, theta, dsc, theta_c, live, &Ls);
+}
+
 // Packing the entries of the live rows (at every compaction of the live
 // list): the row lengths of live stars (0 for frozen ones, and for a = na),
 // whose exclusive scan gives each live row's start cst[a]; then every entry

@@ -1483,15 +1483,19 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_lsqr_init(Scal* S, double 
 // Final x update, step capping and candidate theta (:1224-1230, _cap_position_step! :700-716).
 // The last iteration's pending update is x + sig (w - del hbar) (LSQR: del = 0;
 // CG: sig = 0, x is already final).
-static __global__ void k_step(int na, int p, int k_y, int k_x, double max_step, const double* __restrict__ wv,
-                       const double* __restrict__ hb, double* __restrict__ x, const double* __restrict__ colnorm,
-                       const double* __restrict__ theta, double* __restrict__ dsc, double* __restrict__ theta_c,
-                       const uint8_t* __restrict__ live, const Scal* Sc) {
-    int a = blockIdx.x * BLOCK + threadIdx.x;
-    if (a >= na) return;
+// One star's final x update, capped step and candidate theta, from the
+// solve's final scalars S.  No fused multiply-adds, so every kernel that
+// inlines it rounds alike.
+__device__ __forceinline__ void step_star(int a, int p, int k_y, int k_x, double max_step,
+                                          const double* __restrict__ wv, const double* __restrict__ hb,
+                                          double* __restrict__ x, const double* __restrict__ colnorm,
+                                          const double* __restrict__ theta, double* __restrict__ dsc,
+                                          double* __restrict__ theta_c, const uint8_t* __restrict__ live,
+                                          const Scal* S) {
+#pragma clang fp contract(off)
     const size_t base = (size_t)a * p;
-    const bool any_iter = Sc->iter > 0;
-    const double sig = Sc->sig, del = Sc->del;
+    const bool any_iter = S->iter > 0;
+    const double sig = S->sig, del = S->del;
     double d[3];
     for (int k = 0; k < p; ++k) {
         const double h = hb ? wv[base + k] - del * hb[base + k] : wv[base + k];
@@ -1512,6 +1516,15 @@ static __global__ void k_step(int na, int p, int k_y, int k_x, double max_step, 
         dsc[base + k] = colnorm[base + k] * d[k];
         theta_c[base + k] = theta[base + k] + d[k];
     }
+}
+
+static __global__ void k_step(int na, int p, int k_y, int k_x, double max_step, const double* __restrict__ wv,
+                       const double* __restrict__ hb, double* __restrict__ x, const double* __restrict__ colnorm,
+                       const double* __restrict__ theta, double* __restrict__ dsc, double* __restrict__ theta_c,
+                       const uint8_t* __restrict__ live, const Scal* Sc) {
+    const int a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= na) return;
+    step_star(a, p, k_y, k_x, max_step, wv, hb, x, colnorm, theta, dsc, theta_c, live, Sc);
 }
 
 // Accepted step: theta, D, per-star freeze tests (:1246-1283, _x_tol_local :810-823).

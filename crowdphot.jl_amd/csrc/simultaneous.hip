@@ -899,9 +899,7 @@ struct Sim {
                                              (const double*)vprev.as<double>(), gnew.as<double>(),
                                              vnew.as<double>(), xs.as<double>(), wl.as<double>(), hbar.as<double>(),
                                              first, pc, sin, pp, snext, it);
-                                    if (it == o.inner_iterations)
-                                        launch_n("k_finish_gk", k_finish_gk<SV>, dim3(1), dim3(FINGK_BLOCK), 0, st,
-                                                 (const Scal*)snext, Sc, (const double*)pc, gb_gk, it);
+                                    // (the last step's reduction closes the solve in k_step_fold)
                                     return;
                                 }
                                 launch_n("k_gk", k_gk<PV, SV, false>, dim3(gb_gk), dim3(GKB), 0, st, gk_span, en, Hp,
@@ -916,12 +914,31 @@ struct Sim {
                             else gk(std::integral_constant<int, CP_SOLVER_LSQR>{});
                         }
                     });
-                    launch_n("k_step", k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
-                           (const double*)wl.as<double>(),
-                           (const double*)(solver == CP_SOLVER_LSMR ? hbar.as<double>() : nullptr), xs.as<double>(),
-                           (const double*)colnorm.as<double>(),
-                           (const double*)theta.as<double>(), dsc.as<double>(), theta_c.as<double>(),
-                           (const uint8_t*)live.as<uint8_t>(), (const Scal*)Sc);
+                    const double* hbp = solver == CP_SOLVER_LSMR ? hbar.as<double>() : nullptr;
+                    if (gk_fold && solver != CP_SOLVER_CGLS) {
+                        // the last Golub-Kahan step's reduction (its partials in set
+                        // inner & 1, the state before it in scal_gk[(inner + 1) & 1])
+                        // with the step itself
+                        const int itm = o.inner_iterations;
+                        const double* pc = partg.as<double>() + (size_t)(itm & 1) * 10 * gb_gk;
+                        const Scal* sin = scal_gk.as<Scal>() + ((itm + 1) & 1);
+                        auto sf = [&](auto sv) {
+                            constexpr int SV = decltype(sv)::value;
+                            launch_n("k_step", k_step_fold<SV>, dim3(cdiv(na, FINGK_BLOCK)), dim3(FINGK_BLOCK), 0, st,
+                                     na, p, k_y, k_x, o.max_step, (const double*)wl.as<double>(), hbp,
+                                     xs.as<double>(), (const double*)colnorm.as<double>(),
+                                     (const double*)theta.as<double>(), dsc.as<double>(), theta_c.as<double>(),
+                                     (const uint8_t*)live.as<uint8_t>(), sin, Sc, pc, gb_gk, itm);
+                        };
+                        if (solver == CP_SOLVER_LSMR) sf(std::integral_constant<int, CP_SOLVER_LSMR>{});
+                        else sf(std::integral_constant<int, CP_SOLVER_LSQR>{});
+                    } else {
+                        launch_n("k_step", k_step, dim3(gstar), dim3(BLOCK), 0, st, na, p, k_y, k_x, o.max_step,
+                                 (const double*)wl.as<double>(), hbp, xs.as<double>(),
+                                 (const double*)colnorm.as<double>(), (const double*)theta.as<double>(),
+                                 dsc.as<double>(), theta_c.as<double>(), (const uint8_t*)live.as<uint8_t>(),
+                                 (const Scal*)Sc);
+                    }
                     star_rec(jlist.as<int>(), theta_c.as<double>(), recs_c, crs_c, ers_c, 0, 0);
                     render_launch<MODE_COST>(ldsg, dim3(gb_render), render_sh, M, recs_c.as<StarRec<T>>(),
                                              ers_c.as<EpsfRec<T>>(), crs_c.as<CornerRec<T>>(), Llive, g, ntiles, (const int2*)shares_l.as<int2>(),
