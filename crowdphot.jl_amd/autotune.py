@@ -34,7 +34,7 @@ LM_SPACE = {"lm_blocks_per_cu": (2, 4, 8, 16), "lm_work_tile": (1, 2, 4, 8)}
 # the built-in defaults (what 0 stands for), as starting points; 0 for the
 # H-assembly lanes, whose default follows the stamp size, and for the render
 # tile, whose default follows the image size
-DEFAULTS = {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 8, "render_blocks_per_cu": 4, "lm_blocks_per_cu": 8,
+DEFAULTS = {"hbuild_blocks_per_cu": 32, "cg_blocks_per_cu": 8, "render_blocks_per_cu": 2, "lm_blocks_per_cu": 8,
             "render_tile": 0, "hbuild_lanes": 0, "fill_lanes": 16, "lm_work_tile": 1, "gk_fold_blocks": 0}
 
 

@@ -183,7 +183,7 @@ int cp_profile_report(cp_ctx* ctx, char* buf, int32_t cap);
 typedef struct cp_tuning {
     int32_t hbuild_blocks_per_cu;  /* normal-matrix assembly, 256-thread blocks (default 32) */
     int32_t cg_blocks_per_cu;      /* CG step update per star, 256-thread blocks (default 8) */
-    int32_t render_blocks_per_cu;  /* tile-gather renders, 1024-thread blocks (default 4) */
+    int32_t render_blocks_per_cu;  /* tile-gather renders, 1024-thread blocks (default 2) */
     int32_t lm_blocks_per_cu;      /* batched per-star LM, 256-thread blocks (default 8) */
     /* work-tile sizes of the simultaneous fit */
     int32_t render_tile;           /* render tile side in pixels: 32 or 64 (default: 32 for images of fewer
