@@ -1480,12 +1480,11 @@ static __global__ __launch_bounds__(FIN_BLOCK) void k_lsqr_init(Scal* S, double 
     S->s2 = 0;
 }
 
-// Final x update, step capping and candidate theta (:1224-1230, _cap_position_step! :700-716).
-// The last iteration's pending update is x + sig (w - del hbar) (LSQR: del = 0;
-// CG: sig = 0, x is already final).
-// One star's final x update, capped step and candidate theta, from the
-// solve's final scalars S.  No fused multiply-adds, so every kernel that
-// inlines it rounds alike.
+// One star's final x update, capped step and candidate theta
+// (psf_photometry_simultaneous.jl:1224-1230, _cap_position_step! :700-716), from
+// the solve's final scalars S.  The last iteration's pending update is
+// x + sig (w - del hbar) (LSQR: del = 0; CG: sig = 0, x is already final).  No
+// fused multiply-adds, so every kernel that inlines it rounds alike.
 __device__ __forceinline__ void step_star(int a, int p, int k_y, int k_x, double max_step,
                                           const double* __restrict__ wv, const double* __restrict__ hb,
                                           double* __restrict__ x, const double* __restrict__ colnorm,
